@@ -1,0 +1,995 @@
+// Oracle core: TEST INFRASTRUCTURE ONLY.
+//
+// A scalar, clean-room C++ restatement of the math ProtoRay's stream path tracer runs per
+// path (sampler, camera, warps, distributions, microfacet LUTs, BSDFs, lights, hit
+// geometry).  It is the checker the HIP path is compared against and the timed CPU
+// baseline; it is never linked into the product library.  Each function cites the
+// reference file:line it follows.  Arithmetic is IEEE FP32 (the reference's SIMD rcp /
+// rsqrt are approximation + one Newton step, math/simd/vfloat8_avx2.h:227-232,327-342,
+// so float results agree with the reference to a few ulp, not bit for bit; the sampler is
+// integer work and is bit-exact).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cfloat>
+#include <algorithm>
+
+namespace orc {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kPosMax = FLT_MAX;
+constexpr float kEps = FLT_EPSILON;  // sys/constants.h:71 (numeric_limits<float>::epsilon)
+
+struct V3
+{
+  float x, y, z;
+  V3() : x(0), y(0), z(0) {}
+  V3(float a) : x(a), y(a), z(a) {}
+  V3(float a, float b, float c) : x(a), y(b), z(c) {}
+  float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
+inline V3 operator*(V3 a, V3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline V3 operator*(V3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator*(float s, V3 a) { return V3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator/(V3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+inline V3& operator+=(V3& a, V3 b) { a = a + b; return a; }
+inline V3& operator*=(V3& a, V3 b) { a = a * b; return a; }
+inline V3& operator*=(V3& a, float s) { a = a * s; return a; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline float lengthSqr(V3 a) { return dot(a, a); }
+// math/vec3.h:399-402 (a * rsqrt(dot(a,a)))
+inline V3 normalize(V3 a) { return a * (1.0f / std::sqrt(dot(a, a))); }
+inline float reduceMax(V3 a) { return std::max(std::max(a.x, a.y), a.z); }
+inline V3 vmax(V3 a, V3 b) { return V3(std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)); }
+inline V3 vmin(V3 a, V3 b) { return V3(std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z)); }
+inline float sqr(float a) { return a * a; }
+inline float clampf(float x, float lo, float hi) { return std::min(std::max(x, lo), hi); }
+inline float lerpf(float a, float b, float s) { return a + s * (b - a); }  // math/math.h:89-92
+inline float rcpSafe(float x) { return x == 0.f ? kPosMax : 1.0f / x; }  // math/simd.h:107-111
+// image/color.h:17-22
+inline float luminance(V3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }
+inline float cos2sin(float x) { return std::sqrt(std::max(1.f - x * x, 0.f)); }  // math/math.h:57-66
+
+struct Basis { V3 U, V, N; };
+inline V3 toGlobal(const Basis& b, V3 a) { return a.x * b.U + a.y * b.V + a.z * b.N; }      // math/basis3.h:72-75,139-142
+inline V3 toLocal(const Basis& b, V3 a) { return V3(dot(a, b.U), dot(a, b.V), dot(a, b.N)); }  // math/basis3.h:77-80
+
+// math/basis3.h:173-180
+inline void makeFrame(V3& U, V3& V, V3 N)
+{
+  V3 U0(0.f, N.z, -N.y);
+  V3 U1(-N.z, 0.f, N.x);
+  U = normalize(lengthSqr(U0) > lengthSqr(U1) ? U0 : U1);
+  V = normalize(cross(N, U));
+}
+inline Basis makeBasis(V3 N) { Basis b; makeFrame(b.U, b.V, N); b.N = N; return b; }
+
+// ---------------------------------------------------------------------------------------
+// Sobol-Burley sampler (bit-exact integer part)
+// ---------------------------------------------------------------------------------------
+
+// sampling/sobol_burley.cpp:9-51 holds the reversed-bit direction numbers of the first four
+// Sobol dimensions.  Dimensions 0 and 1 have closed forms (sobol_burley.h:110-128) and only
+// dimension 2 is used by a 3D sample, so this restatement generates the dimension-2 column
+// from its primitive polynomial (x^2 + x + 1, m = {1, 3}; Joe-Kuo) instead of keeping a table.
+inline const uint32_t* sobolDim2Reversed()
+{
+  static uint32_t t[32];
+  static bool init = false;
+  if (!init)
+  {
+    // standard (MSB-first) direction numbers v_k = m_k << (32-k), then bit-reverse
+    uint32_t v[33];
+    uint32_t m1 = 1, m2 = 3;
+    v[1] = m1 << 31;
+    v[2] = m2 << 30;
+    for (int k = 3; k <= 32; ++k)
+      v[k] = v[k - 2] ^ (v[k - 2] >> 2) ^ v[k - 1];  // s = 2, a = 1: m_k = 2 m_{k-1} ^ 4 m_{k-2} ^ m_{k-2}
+    for (int k = 1; k <= 32; ++k)
+    {
+      uint32_t x = v[k], r = 0;
+      for (int b = 0; b < 32; ++b) r |= ((x >> b) & 1u) << (31 - b);
+      t[k - 1] = r;
+    }
+    init = true;
+  }
+  return t;
+}
+
+inline uint32_t reverseBits(uint32_t x)  // math/simd.h:140-147
+{
+  x = ((x & 0xaaaaaaaau) >> 1) | ((x & 0x55555555u) << 1);
+  x = ((x & 0xccccccccu) >> 2) | ((x & 0x33333333u) << 2);
+  x = ((x & 0xf0f0f0f0u) >> 4) | ((x & 0x0f0f0f0fu) << 4);
+  x = ((x & 0xff00ff00u) >> 8) | ((x & 0x00ff00ffu) << 8);
+  return (x >> 16) | (x << 16);
+}
+inline uint32_t hashHP(uint32_t i)  // sampling/sobol_burley.h:31-44
+{
+  i ^= i >> 16; i *= 0x21f0aaadu; i ^= i >> 15; i *= 0xd35a2d97u; i ^= i >> 15;
+  return i ^ 0xe6fe3bebu;
+}
+inline uint32_t reversedBitOwen(uint32_t n, uint32_t seed)  // sampling/sobol_burley.h:52-62
+{
+  n ^= n * 0x3d20adeau; n += seed; n *= (seed >> 16) | 1u; n ^= n * 0x05526c56u; n ^= n * 0x53a22864u;
+  return n;
+}
+// math/simd.h:76-84 with the uint32 -> float conversion of the default AVX2 host build
+// (math/simd/vfloat8_avx2.h:171-178): the low 31 bits are converted, then 2^31 is added as a
+// float, i.e. two roundings.  (The AVX-512 and NEON builds convert in one rounding and can
+// differ by one ulp.)
+inline float uintToFloatAvx2(uint32_t x)
+{
+  float lo = float(int32_t(x & 0x7fffffffu));
+  float hi = (x & 0x80000000u) ? 2147483648.0f : 0.0f;
+  return lo + hi;
+}
+inline float toFloatUnormExcl(uint32_t x) { return uintToFloatAvx2(x) * (1.0f / 4294967808.0f); }
+
+// sampling/sobol_burley.h:150-190 (SIMD form: bit-serial table walk for dims >= 2)
+inline float sobolBurley(uint32_t revBitIndex, uint32_t dim, uint32_t scramble)
+{
+  uint32_t r = 0;
+  if (dim == 0)
+    r = reverseBits(revBitIndex);
+  else if (dim == 1)
+  {
+    uint32_t v = revBitIndex;
+    v ^= v << 16;
+    v ^= (v & 0x00FF00FFu) << 8;
+    v ^= (v & 0x0F0F0F0Fu) << 4;
+    v ^= (v & 0x33333333u) << 2;
+    v ^= (v & 0x55555555u) << 1;
+    r = reverseBits(v);
+  }
+  else
+  {
+    const uint32_t* tab = sobolDim2Reversed();  // only dimension 2 is reached (3D samples)
+    for (int i = 0; revBitIndex != 0; ++i, revBitIndex <<= 1)
+      if (revBitIndex & 0x80000000u) r ^= tab[i];
+  }
+  return toFloatUnormExcl(reverseBits(reversedBitOwen(r, scramble)));
+}
+
+// math/hash.h:12-42
+inline uint32_t murmurMix(uint32_t h, uint32_t k)
+{
+  k *= 0xcc9e2d51u; k = (k << 15) | (k >> 17); k *= 0x1b873593u;
+  h ^= k; h = ((h << 13) | (h >> 19)) * 5u + 0xe6546b64u;
+  return h;
+}
+inline uint32_t murmurFinalize(uint32_t h)
+{
+  h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+  return h;
+}
+
+// SobolBurleySamplerSimd<false> (sampling/sobol_burley_sampler_simd.h:36-85)
+struct Sampler
+{
+  uint32_t index = 0, scramble = 0;
+  void set(uint32_t seed, uint32_t sampleIndex, uint32_t pixelId)
+  {
+    index = sampleIndex;
+    scramble = murmurFinalize(murmurMix(seed, pixelId));
+  }
+  float get1D(uint32_t dim) const  // sobol_burley.h:222-238
+  {
+    uint32_t s = scramble ^ hashHP(dim);
+    uint32_t i = reversedBitOwen(reverseBits(index), s ^ 0xbff95bfeu);
+    return sobolBurley(i, 0, s ^ 0x635c77bdu);
+  }
+  void get2D(uint32_t set, float& a, float& b) const  // sobol_burley.h:250-267
+  {
+    uint32_t s = scramble ^ hashHP(set);
+    uint32_t i = reversedBitOwen(reverseBits(index), s ^ 0xf8ade99au);
+    a = sobolBurley(i, 0, s ^ 0xe0aaaf76u);
+    b = sobolBurley(i, 1, s ^ 0x94964d4eu);
+  }
+  void get3D(uint32_t set, float& a, float& b, float& c) const  // sobol_burley.h:279-297
+  {
+    uint32_t s = scramble ^ hashHP(set);
+    uint32_t i = reversedBitOwen(reverseBits(index), s ^ 0xcaa726acu);
+    a = sobolBurley(i, 0, s ^ 0x9e78e391u);
+    b = sobolBurley(i, 1, s ^ 0x67c33241u);
+    c = sobolBurley(i, 2, s ^ 0x78c395c5u);
+  }
+};
+
+// image/morton.h:14-21: pixel offset of task i inside a tile of width tileSizeX (8x8 Morton blocks)
+inline void morton8x8(int i, int tileSizeX, int& x, int& y)
+{
+  int logTile = 0;
+  while ((1 << logTile) < tileSizeX) ++logTile;
+  int m = i & 63;
+  // de-interleave the 6-bit Morton code (x in even bits, y in odd bits)
+  int mx = (m & 1) | ((m >> 1) & 2) | ((m >> 2) & 4);
+  int my = ((m >> 1) & 1) | ((m >> 2) & 2) | ((m >> 3) & 4);
+  x = mx + ((i >> 3) & (tileSizeX - 8));
+  y = my + ((i >> (logTile + 3)) << 3);
+}
+
+// ---------------------------------------------------------------------------------------
+// Shape warps (sampling/shape_sampler.h)
+// ---------------------------------------------------------------------------------------
+inline V3 cosineSampleHemisphere(float& pdf, float sx, float sy)  // :69-86
+{
+  float cosTheta = std::sqrt(sx);
+  float sinTheta = std::sqrt(1.f - sx);
+  float phi = 2.f * kPi * sy;
+  pdf = cosTheta * (1.f / kPi);
+  return V3(std::cos(phi) * sinTheta, std::sin(phi) * sinTheta, cosTheta);
+}
+inline float cosineSampleHemispherePdf(float c) { return c * (1.f / kPi); }  // :94-98
+inline V3 uniformSampleSphere(float& pdf, float sx, float sy)  // :23-40
+{
+  float cosTheta = 1.f - 2.f * sx;
+  float sinTheta = 2.f * std::sqrt(sx * (1.f - sx));
+  float phi = 2.f * kPi * sy;
+  pdf = 1.f / (4.f * kPi);
+  return V3(std::cos(phi) * sinTheta, std::sin(phi) * sinTheta, cosTheta);
+}
+inline V3 uniformSampleCone(float& pdf, float sx, float sy, float angle)  // :100-117
+{
+  float cosTheta = 1.f - sx * (1.f - std::cos(angle));
+  float sinTheta = cos2sin(cosTheta);
+  float phi = 2.f * kPi * sy;
+  pdf = 1.f / (4.f * kPi * sqr(std::sin(0.5f * angle)));
+  return V3(std::cos(phi) * sinTheta, std::sin(phi) * sinTheta, cosTheta);
+}
+inline float uniformSampleConePdf(float cosTheta, float angle)  // :119-123
+{
+  return cosTheta < std::cos(angle) ? 0.f : 1.f / (4.f * kPi * sqr(std::sin(0.5f * angle)));
+}
+inline void uniformSampleDisk(float sx, float sy, float& x, float& y)  // :169-179
+{
+  float r = std::sqrt(sx);
+  float theta = 2.f * kPi * sy;
+  x = r * std::cos(theta);
+  y = r * std::sin(theta);
+}
+
+// ---------------------------------------------------------------------------------------
+// Discrete distributions (sampling/distribution1d.{h,cpp}, distribution2d.{h,cpp})
+// ---------------------------------------------------------------------------------------
+// distribution1d.cpp:18-32: cdf has size+1 entries
+inline void dist1dInit(float* cdf, const float* f, int size)
+{
+  cdf[0] = 0.f;
+  for (int i = 1; i < size + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1];
+  float norm = (cdf[size] == 0.f) ? 0.f : 1.0f / cdf[size];
+  for (int i = 1; i < size + 1; ++i) cdf[i] *= norm;
+  cdf[size] = 1.f;
+}
+// distribution1d.cpp:35-55 (upper bound over cdf[0..size-1], minus one, clamped at 0)
+inline int dist1dLookup(const float* cdf, int size, float s)
+{
+  int first = 0, len = size;
+  while (len > 0)
+  {
+    int half = len >> 1, mid = first + half;
+    if (s < cdf[mid]) len = half;
+    else { first = mid + 1; len = len - half - 1; }
+  }
+  return std::max(first - 1, 0);
+}
+// distribution1d.h:76-83
+inline int dist1dSampleReuse(const float* cdf, int size, float& pdf, float& s)
+{
+  int i = dist1dLookup(cdf, size, s);
+  pdf = cdf[i + 1] - cdf[i];
+  s = (s - cdf[i]) * (1.0f / pdf);
+  return i;
+}
+inline float dist1dPdf(const float* cdf, int size, int i)  // distribution1d.h:96-101
+{
+  if (i < 0 || i >= size) return 0.f;
+  return cdf[i + 1] - cdf[i];
+}
+// distribution2d.cpp:16-34: xDists[H][W+1], yDist[H+1]
+inline void dist2dInit(float* xDists, float* yDist, const float* f, int W, int H)
+{
+  float* fy = new float[H];
+  for (int y = 0; y < H; ++y)
+  {
+    fy[y] = 0.f;
+    for (int x = 0; x < W; ++x) fy[y] += f[y * W + x];
+    dist1dInit(xDists + y * (W + 1), f + y * W, W);
+  }
+  dist1dInit(yDist, fy, H);
+  delete[] fy;
+}
+// distribution2d.h:53-64
+inline void dist2dSampleReuse(const float* xDists, const float* yDist, int W, int H, float& pdf, float& sx, float& sy, int& ix, int& iy)
+{
+  float yPdf, xPdf;
+  iy = dist1dSampleReuse(yDist, H, yPdf, sy);
+  ix = dist1dSampleReuse(xDists + iy * (W + 1), W, xPdf, sx);
+  pdf = xPdf * yPdf;
+}
+inline float dist2dPdf(const float* xDists, const float* yDist, int W, int H, int ix, int iy)  // distribution2d.h:79-86
+{
+  if (iy < 0 || iy >= H) return 0.f;
+  return dist1dPdf(xDists + iy * (W + 1), W, ix) * dist1dPdf(yDist, H, iy);
+}
+
+// ---------------------------------------------------------------------------------------
+// GGX microfacet distribution (bsdf/ggx_distribution.h) and roughness mapping
+// ---------------------------------------------------------------------------------------
+inline float roughnessToAlpha(float r) { return std::max(sqr(r), 0.001f); }  // bsdf/microfacet.h:10-24 (isotropic)
+
+struct GGX
+{
+  float ax, ay;
+  explicit GGX(float a) : ax(a), ay(a) {}
+  float eval(V3 wh) const  // :24-31
+  {
+    float c2 = sqr(wh.z);
+    float e = (sqr(wh.x / ax) + sqr(wh.y / ay)) / c2;
+    return 1.0f / (kPi * ax * ay * sqr(c2 * (1.f + e)));
+  }
+  float lambda(V3 w) const  // :84-90
+  {
+    float c2 = sqr(w.z);
+    float invA2 = (sqr(w.x * ax) + sqr(w.y * ay)) / c2;
+    return 0.5f * (-1.f + std::sqrt(1.f + invA2));
+  }
+  float G1(V3 w) const { return 1.0f / (1.f + lambda(w)); }                  // :92-95
+  float G2(V3 wo, V3 wi) const { return 1.0f / (1.f + lambda(wo) + lambda(wi)); }  // :99-102
+  float evalVisible(V3 wh, V3 wo, float cosThetaOH, float& pdf) const  // :42-48
+  {
+    float D = eval(wh);
+    pdf = G1(wo) * std::fabs(cosThetaOH) * D / std::fabs(wo.z);
+    return D;
+  }
+  // Heitz 2017 VNDF sampling (:52-78)
+  V3 sampleVisible(V3 wo, float& pdf, float sx, float sy) const
+  {
+    V3 V = normalize(V3(ax * wo.x, ay * wo.y, wo.z));
+    V3 T1 = (V.z < 0.9999f) ? normalize(cross(V, V3(0, 0, 1))) : V3(1, 0, 0);
+    V3 T2 = cross(T1, V);
+    float a = 1.f / (1.f + V.z);
+    float r = std::sqrt(sx);
+    float phi = (sy < a) ? sy / a * kPi : kPi + (sy - a) / (1.f - a) * kPi;
+    float P1 = r * std::cos(phi);
+    float P2 = r * std::sin(phi) * ((sy < a) ? 1.f : V.z);
+    V3 wh = P1 * T1 + P2 * T2 + std::sqrt(std::max(0.f, 1.f - P1 * P1 - P2 * P2)) * V;
+    wh = normalize(V3(ax * wh.x, ay * wh.y, std::max(0.f, wh.z)));
+    pdf = G1(wo) * std::fabs(dot(wo, wh)) * eval(wh) / std::fabs(wo.z);
+    return wh;
+  }
+};
+
+// bsdf/optics.h
+inline V3 reflect(V3 I, V3 N) { return 2.f * dot(I, N) * N - I; }                // :12-16
+inline V3 reflect(V3 I, V3 N, float cosI) { return 2.f * cosI * N - I; }        // :20-24
+inline V3 refract(V3 I, V3 N, float cosI, float cosT, float eta) { return eta * (cosI * N - I) - cosT * N; }  // :30-34
+inline float fresnelDielectric(float cosI, float cosT, float eta)  // :61-67
+{
+  float Rper = (eta * cosI - cosT) / (eta * cosI + cosT);
+  float Rpar = (cosI - eta * cosT) / (cosI + eta * cosT);
+  return 0.5f * (Rpar * Rpar + Rper * Rper);
+}
+inline float fresnelDielectric(float cosI, float eta)  // :84-99 (SIMD form)
+{
+  cosI = std::max(cosI, 0.f);
+  float k = 1.f - eta * eta * (1.f - cosI * cosI);
+  if (!(k >= 0.f)) return 1.f;
+  return fresnelDielectric(cosI, std::sqrt(k), eta);
+}
+inline float fresnelDielectricEx(float cosI, float& cosT, float eta)  // :122-139 (SIMD form)
+{
+  cosT = 0.f;
+  cosI = std::fabs(cosI);
+  float k = 1.f - eta * eta * (1.f - cosI * cosI);
+  if (!(k >= 0.f)) return 1.f;
+  cosT = std::sqrt(k);
+  return fresnelDielectric(cosI, cosT, eta);
+}
+inline float fresnelDielectricAvg(float eta)  // :144-175
+{
+  float re = 1.0f / eta;
+  if (re >= 1.f) return (re - 1.f) / (4.08567f + 1.00071f * re);
+  float re2 = sqr(re), re3 = re2 * re;
+  return 0.997118f + 0.1014f * re - 0.965241f * re2 - 0.130607f * re3;
+}
+inline float fresnelConductor(float cosI, float eta, float k)  // :179-190
+{
+  cosI = std::fabs(cosI);
+  float tmp = sqr(eta) + sqr(k);
+  float Rpar = (tmp * sqr(cosI) - 2.f * eta * cosI + 1.f) / (tmp * sqr(cosI) + 2.f * eta * cosI + 1.f);
+  float Rper = (tmp - 2.f * eta * cosI + sqr(cosI)) / (tmp + 2.f * eta * cosI + sqr(cosI));
+  return 0.5f * (Rpar + Rper);
+}
+// :214-226 (Gulbrandsen 2014)
+inline void artisticToIor(float r, float g, float& eta, float& k)
+{
+  r = std::min(r, 0.99f);
+  float nMin = (1.f - r) / (1.f + r);
+  float nMax = (1.f + std::sqrt(r)) / (1.f - std::sqrt(r));
+  float n = g * nMin + (1.f - g) * nMax;
+  float k2 = (sqr(n + 1.f) * r - sqr(n - 1.f)) / (1.f - r);
+  eta = n;
+  k = std::sqrt(std::max(k2, 0.f));
+}
+inline float artisticAvg(float r, float g)  // :228-240 (Kulla-Conty fit)
+{
+  float r2 = r * r, r3 = r2 * r, g2 = g * g, g3 = g2 * g;
+  return 0.087237f + 0.0230685f * g - 0.0864902f * g2 + 0.0774594f * g3
+       + 0.782654f * r - 0.136432f * r2 + 0.278708f * r3
+       + 0.19744f * g * r + 0.0360605f * g2 * r - 0.2586f * g * r2;
+}
+
+// ---------------------------------------------------------------------------------------
+// Energy-compensation LUTs (bsdf/albedo_tables.{h,cpp}, bsdf/albedo.h) and lookups
+// (math/interpolation.h)
+// ---------------------------------------------------------------------------------------
+inline float interp1D(float x, const float* f, int size)  // interpolation.h:26-36
+{
+  float xc = clampf(x, 0.f, float(size - 1));
+  float s = xc - std::floor(xc);
+  int x0 = std::min(int(xc), size - 1), x1 = std::min(x0 + 1, size - 1);
+  return lerpf(f[x0], f[x1], s);
+}
+inline float interp2D(float px, float py, const float* f, int sx, int sy)  // :77-98
+{
+  float xc = clampf(px, 0.f, float(sx - 1)), yc = clampf(py, 0.f, float(sy - 1));
+  float fx = xc - std::floor(xc), fy = yc - std::floor(yc);
+  int x0 = std::min(int(xc), sx - 1), x1 = std::min(x0 + 1, sx - 1);
+  int y0 = std::min(int(yc), sy - 1), y1 = std::min(y0 + 1, sy - 1);
+  float f0 = lerpf(f[x0 + y0 * sx], f[x1 + y0 * sx], fx);
+  float f1 = lerpf(f[x0 + y1 * sx], f[x1 + y1 * sx], fx);
+  return lerpf(f0, f1, fy);
+}
+inline float interp3D(float px, float py, float pz, const float* f, int n)  // :197-227 (cube n^3)
+{
+  float xc = clampf(px, 0.f, float(n - 1)), yc = clampf(py, 0.f, float(n - 1)), zc = clampf(pz, 0.f, float(n - 1));
+  float sx = xc - std::floor(xc), sy = yc - std::floor(yc), sz = zc - std::floor(zc);
+  int x0 = std::min(int(xc), n - 1), x1 = std::min(x0 + 1, n - 1);
+  int y0 = std::min(int(yc), n - 1), y1 = std::min(y0 + 1, n - 1);
+  int z0 = std::min(int(zc), n - 1), z1 = std::min(z0 + 1, n - 1);
+  int ny = n, nz = n * n;
+  float f00 = lerpf(f[x0 + y0 * ny + z0 * nz], f[x1 + y0 * ny + z0 * nz], sx);
+  float f01 = lerpf(f[x0 + y1 * ny + z0 * nz], f[x1 + y1 * ny + z0 * nz], sx);
+  float f10 = lerpf(f[x0 + y0 * ny + z1 * nz], f[x1 + y0 * ny + z1 * nz], sx);
+  float f11 = lerpf(f[x0 + y1 * ny + z1 * nz], f[x1 + y1 * ny + z1 * nz], sx);
+  return lerpf(lerpf(f00, f01, sy), lerpf(f10, f11, sy), sz);
+}
+
+struct Luts
+{
+  float ggxDir[32 * 32];            // MicrofacetAlbedoTable::dir  [roughness][cosThetaO]
+  float ggxAvg[32];                 // MicrofacetAlbedoTable::avg  [roughness]
+  float dielEtaDir[16 * 16 * 16];   // MicrofacetDielectricAlbedoTable::etaDir [rough][eta][cos], eta in [1/3,1]
+  float dielEtaAvg[16 * 16];
+  float dielRcpEtaDir[16 * 16 * 16];  // eta in [1,3]
+  float dielRcpEtaAvg[16 * 16];
+};
+
+// albedo_tables.h:20-42
+inline float ggxAlbedo(const Luts& L, float cosThetaO, float roughness)
+{
+  return interp2D(cosThetaO * 31.f, roughness * 31.f, L.ggxDir, 32, 32);
+}
+inline float ggxAlbedoAvg(const Luts& L, float roughness) { return interp1D(roughness * 31.f, L.ggxAvg, 32); }
+// albedo_tables.h:82-154 (minIor 1, maxIor 3; rcp(maxIor) evaluated as 1/3)
+inline float dielAlbedo(const Luts& L, float cosThetaO, float eta, float roughness)
+{
+  if (eta <= 1.f)
+  {
+    const float minEta = 1.0f / 3.0f, maxEta = 1.0f;
+    float e = (eta - minEta) / (maxEta - minEta);
+    return interp3D(cosThetaO * 15.f, e * 15.f, roughness * 15.f, L.dielEtaDir, 16);
+  }
+  float e = (eta - 1.f) / (3.f - 1.f);
+  return interp3D(cosThetaO * 15.f, e * 15.f, roughness * 15.f, L.dielRcpEtaDir, 16);
+}
+inline float dielAlbedoAvg(const Luts& L, float eta, float roughness)
+{
+  if (eta <= 1.f)
+  {
+    const float minEta = 1.0f / 3.0f, maxEta = 1.0f;
+    float e = (eta - minEta) / (maxEta - minEta);
+    return interp2D(e * 15.f, roughness * 15.f, L.dielEtaAvg, 16, 16);
+  }
+  float e = (eta - 1.f) / (3.f - 1.f);
+  return interp2D(e * 15.f, roughness * 15.f, L.dielRcpEtaAvg, 16, 16);
+}
+
+void precomputeLuts(Luts& L);  // orc_luts.cpp
+
+// ---------------------------------------------------------------------------------------
+// Materials and BSDFs
+// ---------------------------------------------------------------------------------------
+namespace BsdfType {
+enum : int {
+  DiffuseReflection = 1 << 0, GlossyReflection = 1 << 1, SpecularReflection = 1 << 2,
+  DiffuseTransmission = 1 << 3, GlossyTransmission = 1 << 4, SpecularTransmission = 1 << 5,
+  UnbentTransmission = 1 << 6, Transparency = 1 << 7,
+  Diffuse = DiffuseReflection | DiffuseTransmission, Glossy = GlossyReflection | GlossyTransmission,
+  Specular = SpecularReflection | SpecularTransmission, Continuation = UnbentTransmission | Transparency,
+  Smooth = Diffuse | Glossy, Delta = Specular | Continuation,
+};
+}  // bsdf/bsdf.h:15-44
+
+// Material kinds supported on the stream path (uniform StandardMaterial parameters,
+// material/standard.cpp:409-573, each producing a MultiBsdf with one child of weight 1):
+//   0 diffuse    : baseColor, specularWeight = 0            -> DiffuseBrdf
+//   1 conductor  : metalness = 1, baseColor = r, specularColor = g, specularRoughness
+//                                                           -> RoughConductorBrdf<FresnelConductorArtistic>
+//   2 dielectric : transmissionWeight = 1, specularIOR, specularRoughness
+//                                                           -> RoughDielectricBsdf + media interface
+enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2 };
+
+struct Material  // 32-byte record shared with the HIP path's material table
+{
+  int kind;
+  float r, g, b;     // albedo (diffuse) / reflectivity (conductor)
+  float er, eg, eb;  // edge tint (conductor)
+  float roughness;   // specularRoughness (alpha = max(roughness^2, 0.001))
+  float ior;         // specularIOR (dielectric), relative to an ambient IOR of 1
+  int pad[2];
+};
+
+struct Ctx  // core/shading_context.h:113-124 subset
+{
+  V3 p;
+  Basis f;  // geometric shading frame from postIntersect (+ orientation flip)
+  V3 Ng;
+  bool backfacing;
+  float eps;
+};
+
+// core/shading_context.h:144-191: shading normal adaption + Gram-Schmidt frame
+inline Basis makeShadingFrame(const Ctx& ctx, V3 wo)
+{
+  Basis fr;
+  V3 N = ctx.f.N;
+  V3 wi = reflect(wo, N);
+  if (dot(wi, ctx.Ng) < 0.f)
+  {
+    float a = -dot(wi, ctx.Ng);
+    float b = dot(N, ctx.Ng);
+    if (b < 0.f) { b = -b; wi = -wi; }
+    V3 wi1 = normalize(wi + a * rcpSafe(b) * N + 1e-5f * ctx.Ng);
+    N = normalize(wo + wi1);
+  }
+  fr.N = N;
+  fr.U = ctx.f.U - dot(ctx.f.U, N) * N;
+  float Ulen2 = lengthSqr(fr.U);
+  fr.U = fr.U * (1.0f / std::sqrt(Ulen2));
+  fr.V = ctx.f.V - dot(ctx.f.V, N) * N;
+  fr.V = fr.V - dot(fr.V, fr.U) * fr.U;
+  float Vlen2 = lengthSqr(fr.V);
+  fr.V = fr.V * (1.0f / std::sqrt(Vlen2));
+  if (Ulen2 == 0.f || Vlen2 == 0.f) makeFrame(fr.U, fr.V, fr.N);
+  return fr;
+}
+
+// Per-hit BSDF instance (MultiBsdf{child, weight 1}).
+struct Bsdf
+{
+  int kind;
+  int flags;
+  Basis fr;
+  V3 color;            // diffuse
+  // conductor
+  V3 eta, k, rr, gg;
+  float Eavg;
+  V3 fmsScale;
+  // dielectric
+  float deta, EavgEta, EavgRcpEta, fmsRatio;
+  float roughness;
+  GGX ggx{0.001f};
+  float importance;
+};
+
+inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, int mediumId, int mediumIntId)
+{
+  Bsdf b;
+  b.kind = m.kind;
+  b.fr = makeShadingFrame(ctx, wo);
+  float roughness = clampf(m.roughness, 0.f, 1.f);
+  b.roughness = roughness;
+  b.ggx = GGX(roughnessToAlpha(roughness));
+  if (m.kind == kDiffuse)
+  {
+    b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
+    b.flags = BsdfType::DiffuseReflection;
+    b.importance = reduceMax(b.color);  // bsdf/diffuse.h:72-75
+  }
+  else if (m.kind == kConductor)
+  {
+    // rough_conductor.h:121-130 + optics.h:243-261
+    b.rr = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
+    b.gg = V3(clampf(m.er, 0.f, 1.f), clampf(m.eg, 0.f, 1.f), clampf(m.eb, 0.f, 1.f));
+    artisticToIor(b.rr.x, b.gg.x, b.eta.x, b.k.x);
+    artisticToIor(b.rr.y, b.gg.y, b.eta.y, b.k.y);
+    artisticToIor(b.rr.z, b.gg.z, b.eta.z, b.k.z);
+    b.Eavg = ggxAlbedoAvg(L, roughness);
+    V3 Favg(artisticAvg(b.rr.x, b.gg.x), artisticAvg(b.rr.y, b.gg.y), artisticAvg(b.rr.z, b.gg.z));
+    V3 one(1.f);
+    V3 den = one - Favg * (one - V3(b.Eavg));
+    b.fmsScale = V3(sqr(Favg.x) * b.Eavg / den.x, sqr(Favg.y) * b.Eavg / den.y, sqr(Favg.z) * b.Eavg / den.z);
+    b.flags = BsdfType::GlossyReflection;
+    b.importance = 1.f;
+  }
+  else
+  {
+    // material/standard.cpp:436-437 (fromOutside = !backfacing; the media interface sets
+    // backfacing = (mediumId == mediumIntId), material/material.h:66-76)
+    bool backfacing = (mediumId == mediumIntId);
+    float ior = m.ior < 1.f ? 1.0f / m.ior : m.ior;
+    float eta = !backfacing ? 1.0f / ior : ior;
+    eta = clampf(eta, 1.f / 3.f, 3.f);
+    b.deta = eta;
+    // rough_dielectric.h:279-291
+    b.EavgEta = dielAlbedoAvg(L, eta, roughness);
+    b.EavgRcpEta = dielAlbedoAvg(L, 1.0f / eta, roughness);
+    float FavgEta = fresnelDielectricAvg(eta);
+    float FavgRcpEta = fresnelDielectricAvg(1.0f / eta);
+    float a = (1.f - FavgEta) / (1.f - b.EavgRcpEta);
+    float bb = (1.f - FavgRcpEta) / (1.f - b.EavgEta);
+    float x = bb / (a + bb);
+    b.fmsRatio = 1.f - x * (1.f - FavgEta);
+    b.flags = BsdfType::GlossyReflection | BsdfType::GlossyTransmission;
+    b.importance = 1.f;
+  }
+  return b;
+}
+
+// --- diffuse (bsdf/diffuse.h:55-68)
+inline V3 diffuseEval(const Bsdf& b, V3 wi, float& pdf)
+{
+  float c = std::max(dot(wi, b.fr.N), 0.f);
+  pdf = cosineSampleHemispherePdf(c);
+  return b.color * (1.f / kPi) * c;
+}
+inline V3 diffuseSample(const Bsdf& b, const Ctx& ctx, V3& wi, float& pdf, int& type, float sx, float sy)
+{
+  wi = toGlobal(b.fr, cosineSampleHemisphere(pdf, sx, sy));
+  type = BsdfType::DiffuseReflection;
+  return dot(wi, ctx.Ng) > 0.f ? b.color : V3(0.f);
+}
+
+// --- rough conductor (bsdf/rough_conductor.h:132-202)
+inline V3 fresnelArtistic(const Bsdf& b, float c)
+{
+  return V3(fresnelConductor(c, b.eta.x, b.k.x), fresnelConductor(c, b.eta.y, b.k.y), fresnelConductor(c, b.eta.z, b.k.z));
+}
+inline V3 conductorEval(const Luts& L, const Bsdf& b, V3 wo, V3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N), cosI = dot(wi, b.fr.N);
+  if (!(cosO > 0.f && cosI > 0.f)) return V3(0.f);
+  V3 wh = normalize(wi + wo);
+  float cosOH = dot(wo, wh);
+  V3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  V3 F = fresnelArtistic(b, cosOH);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  float Eo = ggxAlbedo(L, cosO, b.roughness), Ei = ggxAlbedo(L, cosI, b.roughness);
+  V3 fms = b.fmsScale * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.Eavg))) * cosI);
+  pdf = whPdf * (1.0f / (4.f * std::fabs(cosOH)));
+  return F * (D * G * (1.0f / (4.f * cosO))) + fms;
+}
+inline V3 conductorSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy)
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  V3 wo0 = toLocal(b.fr, wo);
+  float whPdf;
+  V3 wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+  wi = reflect(wo, wh);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI > 0.f && dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+  float cosOH = dot(wo, wh);
+  V3 wi0 = toLocal(b.fr, wi);
+  V3 F = fresnelArtistic(b, cosOH);
+  float G = b.ggx.G2(wo0, wi0);
+  float Eo = ggxAlbedo(L, cosO, b.roughness), Ei = ggxAlbedo(L, cosI, b.roughness);
+  V3 fms = b.fmsScale * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.Eavg))) * cosI);
+  type = BsdfType::GlossyReflection;
+  pdf = whPdf * (1.0f / (4.f * std::fabs(cosOH)));
+  return F * (G * rcpSafe(b.ggx.G1(wo0))) + fms * (1.0f / pdf);
+}
+
+// --- rough dielectric (bsdf/rough_dielectric.h:296-526)
+inline float dielEvalSingle(const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return 0.f;
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return 0.f;
+  bool refl = cosI > 0.f;
+  V3 wh = refl ? wi + wo : b.deta * wo + wi;
+  wh = normalize(wh);
+  if (dot(wh, b.fr.N) < 0.f) wh = -wh;
+  float cosOH = dot(wo, wh), cosIH = dot(wi, wh);
+  float F = fresnelDielectric(cosOH, b.deta);
+  float value;
+  if (refl)
+  {
+    pdf = F * (1.0f / (4.f * std::fabs(cosOH)));
+    value = F * (1.0f / (4.f * cosO));
+  }
+  else
+  {
+    pdf = (1.f - F) * (1.0f / sqr(b.deta)) * std::fabs(cosIH) * (1.0f / sqr(cosOH + (1.0f / b.deta) * cosIH));
+    value = pdf * std::fabs(cosOH) * (1.0f / cosO);
+  }
+  V3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  pdf = pdf * whPdf;
+  return value * D * G;
+}
+inline void dielFms(const Luts& L, const Bsdf& b, float Eo, float cosI, float& fmsPdf, float& fmsValue)
+{
+  if (cosI > 0.f)
+  {
+    float Ei = dielAlbedo(L, cosI, b.deta, b.roughness);
+    fmsPdf = b.fmsRatio * cosineSampleHemispherePdf(cosI);
+    fmsValue = b.fmsRatio * (1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.EavgEta))) * cosI;
+  }
+  else
+  {
+    float Ei = dielAlbedo(L, std::fabs(cosI), 1.0f / b.deta, b.roughness);
+    fmsPdf = (1.f - b.fmsRatio) * cosineSampleHemispherePdf(std::fabs(cosI));
+    fmsValue = (1.f - b.fmsRatio) * (1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.EavgRcpEta))) * std::fabs(cosI);
+  }
+}
+inline V3 dielEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+  float Eo = dielAlbedo(L, cosO, b.deta, b.roughness);
+  float fmsPdf, fmsValue;
+  dielFms(L, b, Eo, cosI, fmsPdf, fmsValue);
+  float singlePdf;
+  float singleValue = dielEvalSingle(b, ctx, wo, wi, singlePdf);
+  pdf = Eo * singlePdf + (1.f - Eo) * fmsPdf;
+  return V3(singleValue + fmsValue);
+}
+inline V3 dielSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float Eo = dielAlbedo(L, cosO, b.deta, b.roughness);
+  bool ok;
+  if (sc <= Eo)
+  {
+    V3 wo0 = toLocal(b.fr, wo);
+    float whPdf;
+    V3 wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+    float cosOH = dot(wo, wh);
+    float cosTH;
+    float F = fresnelDielectricEx(cosOH, cosTH, b.deta);
+    float sc1 = sc * (1.0f / Eo);
+    if (sc1 <= F)
+    {
+      wi = reflect(wo, wh, cosOH);
+      ok = dot(wi, b.fr.N) > 0.f;
+    }
+    else
+    {
+      wi = refract(wo, wh, cosOH, cosTH, b.deta);
+      ok = dot(wi, b.fr.N) < 0.f;
+    }
+  }
+  else
+  {
+    float sc1 = (sc - Eo) * (1.0f / (1.f - Eo));
+    float cpdf;
+    V3 dir = toGlobal(b.fr, cosineSampleHemisphere(cpdf, sx, sy));
+    wi = (sc1 <= b.fmsRatio) ? dir : -dir;
+    ok = true;
+  }
+  if (!ok) return V3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+  type = cosI > 0.f ? BsdfType::GlossyReflection : BsdfType::GlossyTransmission;
+  float fmsPdf, fmsValue;
+  dielFms(L, b, Eo, cosI, fmsPdf, fmsValue);
+  float singlePdf;
+  float singleValue = dielEvalSingle(b, ctx, wo, wi, singlePdf);
+  pdf = Eo * singlePdf + (1.f - Eo) * fmsPdf;
+  return V3((singleValue + fmsValue) * (1.0f / pdf));
+}
+
+// MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210): eval pdf is scaled by
+// importance / importanceSum; sample returns the child's weight times the child scale (1);
+// a zero importance sum disables sampling.
+inline V3 bsdfEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  V3 v(0.f);
+  pdf = 0.f;
+  if (b.importance > 0.f)
+  {
+    float cp = 0.f;
+    if (b.kind == kDiffuse) v = diffuseEval(b, wi, cp);
+    else if (b.kind == kConductor) v = conductorEval(L, b, wo, wi, cp);
+    else v = dielEval(L, b, ctx, wo, wi, cp);
+    pdf = cp * b.importance;
+  }
+  pdf *= 1.0f / b.importance;
+  return v;
+}
+inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  if (!(b.importance > 0.f)) return V3(0.f);
+  if (b.kind == kDiffuse) return diffuseSample(b, ctx, wi, pdf, type, sx, sy);
+  if (b.kind == kConductor) return conductorSample(L, b, ctx, wo, wi, pdf, type, sx, sy);
+  return dielSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
+}
+inline V3 bsdfAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
+{
+  if (b.kind == kDiffuse) return b.color;
+  if (b.kind == kConductor) return b.rr;
+  return V3(1.f);
+}
+
+// ---------------------------------------------------------------------------------------
+// Lights (light/ambient_light.h, distant_light.h, image_light.h; render/scene.h:137-250)
+// ---------------------------------------------------------------------------------------
+enum LightKind : int { kAmbient = 0, kDistant = 1, kImage = 2 };
+
+struct Light  // 32-byte record shared with the HIP path
+{
+  int kind;
+  float r, g, b;       // Ke (ambient / distant) or scale (image)
+  float dx, dy, dz;    // direction (distant)
+  float angle;         // half angle (distant), radians
+};
+
+struct EnvImage  // lat-long environment in texture row order (row 0 = nadir)
+{
+  int W = 0, H = 0;          // texel size
+  const float* rgb = nullptr;  // [H][W][3]
+  int LW = 0, LH = 0;        // importance LUT size (after 2x bucketing)
+  float* xDists = nullptr;   // [LH][LW+1]
+  float* yDist = nullptr;    // [LH+1]
+};
+
+// image/image_texture.h:252-283 bilinear fetch, addressU = Repeat, addressV = Clamp
+inline V3 envTexel(const EnvImage& e, int x, int y)
+{
+  const float* p = e.rgb + (size_t(y) * e.W + x) * 3;
+  return V3(p[0], p[1], p[2]);
+}
+inline V3 envLookup(const EnvImage& e, float u, float v)
+{
+  float x = u - 0.5f / float(e.W);
+  float y = v - 0.5f / float(e.H);
+  float xr = (x - std::floor(x)) * float(e.W);
+  float yr = clampf(y * float(e.H), 0.f, float(e.H - 1));
+  float sx = xr - std::floor(xr), sy = yr - std::floor(yr);
+  int x0 = std::min(std::max(int(xr), 0), e.W - 1);
+  int y0 = std::min(std::max(int(yr), 0), e.H - 1);
+  int x1 = (x0 + 1 == e.W) ? 0 : x0 + 1;
+  int y1 = std::min(y0 + 1, e.H - 1);
+  V3 c00 = envTexel(e, x0, y0), c11 = envTexel(e, x1, y1), c01 = envTexel(e, x0, y1), c10 = envTexel(e, x1, y0);
+  V3 c0 = c00 + sx * (c10 - c00);
+  V3 c1 = c01 + sx * (c11 - c01);
+  return c0 + sy * (c1 - c0);
+}
+
+struct LightSet
+{
+  int count = 0;
+  const Light* lights = nullptr;
+  float cdf[9];          // light power distribution (Distribution1D over <= 8 lights)
+  const EnvImage* env = nullptr;
+  float sceneRadius = 0.f;
+};
+
+// ambient_light.h:53-57, distant_light.h:73-77, image_light.h:137-148
+inline float lightPower(const Light& l, const EnvImage* env, float sceneRadius)
+{
+  V3 c(l.r, l.g, l.b);
+  if (l.kind == kAmbient) return 4.f * sqr(kPi) * sqr(sceneRadius) * luminance(c);
+  if (l.kind == kDistant) return 4.f * sqr(kPi) * sqr(std::sin(0.5f * l.angle)) * sqr(sceneRadius) * luminance(c);
+  float sumL = 0.f;
+  for (int y = 0; y < env->H; ++y)
+    for (int x = 0; x < env->W; ++x) sumL += luminance(envTexel(*env, x, y));
+  return 4.f * sqr(kPi) * sqr(sceneRadius) * (sumL / float(env->W * env->H));
+}
+
+// Evaluate one environment light toward direction wi.
+inline V3 envLightEval(const Light& l, const EnvImage* env, V3 wi, float& pdf)
+{
+  pdf = 0.f;
+  if (l.kind == kAmbient)  // ambient_light.h:33-37
+  {
+    pdf = 1.f / (4.f * kPi);
+    return V3(l.r, l.g, l.b);
+  }
+  if (l.kind == kDistant)  // distant_light.h:48-57
+  {
+    V3 dir(l.dx, l.dy, l.dz);
+    float c = dot(wi, dir);
+    if (!(c >= std::cos(l.angle))) return V3(0.f);
+    pdf = uniformSampleConePdf(c, l.angle);
+    return V3(l.r, l.g, l.b);
+  }
+  // image_light.h:79-91
+  float theta = std::acos(clampf(wi.y, -1.f, 1.f));
+  float phi = std::atan2(wi.x, -wi.z);
+  float u = 0.5f + phi * (1.0f / (2.0f * kPi));
+  float v = 1.0f - theta * (1.0f / kPi);
+  int ix = std::min(std::max(int(u * float(env->LW)), 0), env->LW - 1);
+  int iy = std::min(std::max(int(v * float(env->LH)), 0), env->LH - 1);
+  pdf = dist2dPdf(env->xDists, env->yDist, env->LW, env->LH, ix, iy) * float(env->LW) * float(env->LH) *
+        (1.0f / (2.0f * kPi * kPi * std::sin(theta)));
+  return envLookup(*env, u, v) * V3(l.r, l.g, l.b);
+}
+
+// Sample one light from a 2D sample; returns Le / pdf (times the image scale).
+inline V3 envLightSample(const Light& l, const EnvImage* env, V3& wi, float& pdf, float& dist, float sx, float sy)
+{
+  dist = kPosMax;
+  if (l.kind == kAmbient)  // ambient_light.h:46-51
+  {
+    wi = uniformSampleSphere(pdf, sx, sy);
+    return V3(l.r, l.g, l.b) * (1.0f / pdf);
+  }
+  if (l.kind == kDistant)  // distant_light.h:66-71
+  {
+    Basis fr = makeBasis(V3(l.dx, l.dy, l.dz));
+    wi = toGlobal(fr, uniformSampleCone(pdf, sx, sy, l.angle));
+    return V3(l.r, l.g, l.b) * (1.0f / pdf);
+  }
+  // image_light.h:112-134
+  int ix, iy;
+  dist2dSampleReuse(env->xDists, env->yDist, env->LW, env->LH, pdf, sx, sy, ix, iy);
+  float u = (float(ix) + sx) * (1.0f / float(env->LW));
+  float v = (float(iy) + sy) * (1.0f / float(env->LH));
+  float theta = (1.0f - v) * kPi;
+  float phi = (u - 0.5f) * (2.0f * kPi);
+  float st = std::sin(theta);
+  wi = V3(std::sin(phi) * st, std::cos(theta), -std::cos(phi) * st);
+  pdf *= float(env->LW) * float(env->LH) * (1.0f / (2.0f * kPi * kPi * st));
+  return envLookup(*env, u, v) * V3(l.r, l.g, l.b) * (1.0f / pdf);
+}
+
+// render/scene.h:137-157: sum of all environment lights, pdf weighted by pick probability
+inline V3 evalEnv(const LightSet& ls, V3 wi, float& pdf)
+{
+  V3 value(0.f);
+  pdf = 0.f;
+  for (int i = 0; i < ls.count; ++i)
+  {
+    float cp;
+    value += envLightEval(ls.lights[i], ls.env, wi, cp);
+    pdf += cp * dist1dPdf(ls.cdf, ls.count, i);
+  }
+  return value;
+}
+// render/scene.h:215-250
+inline V3 sampleLight(const LightSet& ls, V3& wi, float& pdf, float& dist, float sx, float sy, float sc)
+{
+  if (ls.count == 1) return envLightSample(ls.lights[0], ls.env, wi, pdf, dist, sx, sy);
+  float pick;
+  float c = sc;
+  int id = dist1dSampleReuse(ls.cdf, ls.count, pick, c);
+  V3 w = envLightSample(ls.lights[id], ls.env, wi, pdf, dist, sx, sy);
+  pdf *= pick;
+  return w * (1.0f / pick);
+}
+
+// Power-heuristic MIS (render/path_renderer_stream.h:1077-1080)
+inline float misWeight(float a, float b) { return sqr(a) * rcpSafe(sqr(a) + sqr(b)); }
+
+}  // namespace orc

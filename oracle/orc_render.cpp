@@ -1,0 +1,962 @@
+// Oracle: scene, CPU BVH, per-path integrator and C API.  TEST INFRASTRUCTURE ONLY.
+//
+// This is the clean-room CPU restatement of ProtoRay's stream path tracer
+// (render/path_renderer_stream.h:195-1073) used as the parity checker for the HIP path and
+// as the timed CPU baseline.  The reference processes 1024-path SoA streams sorted by
+// material; paths never interact, so this restatement runs each path to completion with the
+// same per-path state machine, which yields the same per-path results.  Triangle
+// intersection is a Moller-Trumbore test evaluated with FP contraction off (see Makefile),
+// so hit distances and barycentrics are bit-identical to the HIP kernels, which evaluate
+// the same expression order with contraction disabled.
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+#include "orc_core.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------------------
+struct Tri { V3 v0, e1, e2; int prim; };
+struct Node { float bmin[3], bmax[3]; int first; int count; };  // count == 0: inner, first = left child
+
+struct Scene
+{
+  std::vector<V3> pos, nrm;
+  std::vector<int> idx, triMat;
+  std::vector<Material> mats;
+  std::vector<Light> lights;
+  std::vector<float> envRgb, envX, envY;
+  EnvImage env;
+  LightSet ls;
+  float sceneRadius = 0.f;
+  V3 bmin, bmax;
+  std::vector<Node> nodes;
+  std::vector<Tri> tris;
+  Luts luts;
+  // Interior medium ID per material.  Media are deduplicated by (color, IOR) with ID 0 the
+  // clear ambient medium (material/material_factory.cpp:20-21,159-175); transmission color is
+  // 1 for every supported dielectric, so distinct IORs get IDs 1, 2, ... in material order.
+  std::vector<int> matMedium;
+};
+
+static void assignMedia(Scene& s)
+{
+  std::vector<float> iors;
+  s.matMedium.assign(s.mats.size(), 0);
+  for (size_t m = 0; m < s.mats.size(); ++m)
+  {
+    if (s.mats[m].kind != kDielectric) continue;
+    float ior = s.mats[m].ior < 1.f ? 1.0f / s.mats[m].ior : s.mats[m].ior;
+    int id = -1;
+    for (size_t k = 0; k < iors.size(); ++k) if (iors[k] == ior) id = int(k) + 1;
+    if (id < 0) { iors.push_back(ior); id = int(iors.size()); }
+    s.matMedium[m] = id;
+  }
+}
+
+// geometry/triangle_mesh.cpp / render/scene.cpp: bounds over all vertices referenced
+static void computeBounds(Scene& s)
+{
+  s.bmin = V3(kPosMax);
+  s.bmax = V3(-kPosMax);
+  for (size_t t = 0; t < s.idx.size(); ++t)
+  {
+    s.bmin = vmin(s.bmin, s.pos[s.idx[t]]);
+    s.bmax = vmax(s.bmax, s.pos[s.idx[t]]);
+  }
+  s.sceneRadius = length(s.bmax - s.bmin) * 0.5f;  // render/scene.cpp:367
+}
+
+// ---------------------------------------------------------------------------------------
+// Moller-Trumbore, two-sided, hit accepted for near <= t <= far.  Same expression order as
+// the HIP kernels; compiled with -ffp-contract=off.
+// ---------------------------------------------------------------------------------------
+static inline bool intersectTri(const Tri& tr, V3 o, V3 d, float tnear, float tfar, float& t, float& u, float& v)
+{
+  V3 pvec = cross(d, tr.e2);
+  float det = dot(tr.e1, pvec);
+  if (det == 0.f) return false;
+  float inv = 1.0f / det;
+  V3 tvec = o - tr.v0;
+  u = dot(tvec, pvec) * inv;
+  if (!(u >= 0.f && u <= 1.f)) return false;
+  V3 qvec = cross(tvec, tr.e1);
+  v = dot(d, qvec) * inv;
+  if (!(v >= 0.f && u + v <= 1.f)) return false;
+  t = dot(tr.e2, qvec) * inv;
+  return t >= tnear && t <= tfar;
+}
+
+// ---------------------------------------------------------------------------------------
+// Binned-SAH BVH2 (CPU acceleration for the oracle; structure is irrelevant to results)
+// ---------------------------------------------------------------------------------------
+struct BuildRef { float bmin[3], bmax[3], c[3]; int tri; };
+
+static void buildRecursive(std::vector<Node>& nodes, std::vector<BuildRef>& refs, int nodeIdx, int begin, int end, int depth)
+{
+  Node& n = nodes[nodeIdx];
+  float bmin[3] = {kPosMax, kPosMax, kPosMax}, bmax[3] = {-kPosMax, -kPosMax, -kPosMax};
+  float cmin[3] = {kPosMax, kPosMax, kPosMax}, cmax[3] = {-kPosMax, -kPosMax, -kPosMax};
+  for (int i = begin; i < end; ++i)
+    for (int a = 0; a < 3; ++a)
+    {
+      bmin[a] = std::min(bmin[a], refs[i].bmin[a]); bmax[a] = std::max(bmax[a], refs[i].bmax[a]);
+      cmin[a] = std::min(cmin[a], refs[i].c[a]); cmax[a] = std::max(cmax[a], refs[i].c[a]);
+    }
+  for (int a = 0; a < 3; ++a) { nodes[nodeIdx].bmin[a] = bmin[a]; nodes[nodeIdx].bmax[a] = bmax[a]; }
+  int count = end - begin;
+  if (count <= 4 || depth > 60)
+  {
+    nodes[nodeIdx].first = begin;
+    nodes[nodeIdx].count = count;
+    return;
+  }
+  const int NB = 16;
+  int bestAxis = -1, bestSplit = -1;
+  float bestCost = kPosMax;
+  for (int a = 0; a < 3; ++a)
+  {
+    float ext = cmax[a] - cmin[a];
+    if (!(ext > 0.f)) continue;
+    int cnt[NB] = {0};
+    float lo[NB][3], hi[NB][3];
+    for (int b = 0; b < NB; ++b) for (int k = 0; k < 3; ++k) { lo[b][k] = kPosMax; hi[b][k] = -kPosMax; }
+    float scale = NB / ext;
+    for (int i = begin; i < end; ++i)
+    {
+      int b = std::min(NB - 1, int((refs[i].c[a] - cmin[a]) * scale));
+      cnt[b]++;
+      for (int k = 0; k < 3; ++k) { lo[b][k] = std::min(lo[b][k], refs[i].bmin[k]); hi[b][k] = std::max(hi[b][k], refs[i].bmax[k]); }
+    }
+    float rArea[NB];
+    int rCnt[NB];
+    float l[3] = {kPosMax, kPosMax, kPosMax}, h[3] = {-kPosMax, -kPosMax, -kPosMax};
+    int c = 0;
+    for (int b = NB - 1; b > 0; --b)
+    {
+      c += cnt[b];
+      for (int k = 0; k < 3; ++k) { l[k] = std::min(l[k], lo[b][k]); h[k] = std::max(h[k], hi[b][k]); }
+      float dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+      rArea[b] = c ? (dx * dy + dy * dz + dz * dx) : 0.f;
+      rCnt[b] = c;
+    }
+    for (int k = 0; k < 3; ++k) { l[k] = kPosMax; h[k] = -kPosMax; }
+    c = 0;
+    for (int b = 0; b < NB - 1; ++b)
+    {
+      c += cnt[b];
+      for (int k = 0; k < 3; ++k) { l[k] = std::min(l[k], lo[b][k]); h[k] = std::max(h[k], hi[b][k]); }
+      float dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+      float lArea = c ? (dx * dy + dy * dz + dz * dx) : 0.f;
+      if (c == 0 || rCnt[b + 1] == 0) continue;
+      float cost = lArea * c + rArea[b + 1] * rCnt[b + 1];
+      if (cost < bestCost) { bestCost = cost; bestAxis = a; bestSplit = b; }
+    }
+  }
+  int mid;
+  if (bestAxis < 0)
+  {
+    mid = begin + count / 2;  // all centroids equal: split in the middle
+  }
+  else
+  {
+    float scale = NB / (cmax[bestAxis] - cmin[bestAxis]);
+    auto it = std::partition(refs.begin() + begin, refs.begin() + end, [&](const BuildRef& r) {
+      return std::min(NB - 1, int((r.c[bestAxis] - cmin[bestAxis]) * scale)) <= bestSplit;
+    });
+    mid = int(it - refs.begin());
+    if (mid == begin || mid == end) mid = begin + count / 2;
+  }
+  int left = (int)nodes.size();
+  nodes.push_back(Node());
+  nodes.push_back(Node());
+  nodes[nodeIdx].first = left;
+  nodes[nodeIdx].count = 0;
+  (void)n;
+  buildRecursive(nodes, refs, left, begin, mid, depth + 1);
+  buildRecursive(nodes, refs, left + 1, mid, end, depth + 1);
+}
+
+static void buildBvh(Scene& s)
+{
+  int nt = (int)s.triMat.size();
+  std::vector<BuildRef> refs(nt);
+  for (int t = 0; t < nt; ++t)
+  {
+    V3 p0 = s.pos[s.idx[3 * t]], p1 = s.pos[s.idx[3 * t + 1]], p2 = s.pos[s.idx[3 * t + 2]];
+    V3 lo = vmin(p0, vmin(p1, p2)), hi = vmax(p0, vmax(p1, p2));
+    BuildRef& r = refs[t];
+    for (int a = 0; a < 3; ++a)
+    {
+      // pad by a relative 2^-20 so a box never culls a hit the triangle test reports
+      float pad = std::max(std::fabs(lo[a]), std::fabs(hi[a])) * 9.5367431640625e-7f + 1e-30f;
+      r.bmin[a] = lo[a] - pad; r.bmax[a] = hi[a] + pad;
+      r.c[a] = 0.5f * (lo[a] + hi[a]);
+    }
+    r.tri = t;
+  }
+  s.nodes.clear();
+  s.nodes.reserve(size_t(nt) * 2 / 2 + 16);
+  s.nodes.push_back(Node());
+  if (nt > 0) buildRecursive(s.nodes, refs, 0, 0, nt, 0);
+  else { s.nodes[0].count = 0; s.nodes[0].first = 0; for (int a = 0; a < 3; ++a) { s.nodes[0].bmin[a] = 1; s.nodes[0].bmax[a] = -1; } }
+  s.tris.resize(nt);
+  for (int i = 0; i < nt; ++i)
+  {
+    int t = refs[i].tri;
+    V3 p0 = s.pos[s.idx[3 * t]], p1 = s.pos[s.idx[3 * t + 1]], p2 = s.pos[s.idx[3 * t + 2]];
+    s.tris[i].v0 = p0; s.tris[i].e1 = p1 - p0; s.tris[i].e2 = p2 - p0; s.tris[i].prim = t;
+  }
+}
+
+struct RayHit { V3 o, d; float tnear, tfar; int prim; float u, v; };
+
+static inline bool boxHit(const Node& n, V3 o, V3 inv, float tnear, float tfar, float& tEntry)
+{
+  float t0 = tnear, t1 = tfar;
+  for (int a = 0; a < 3; ++a)
+  {
+    float oa = a == 0 ? o.x : (a == 1 ? o.y : o.z);
+    float ia = a == 0 ? inv.x : (a == 1 ? inv.y : inv.z);
+    float ta = (n.bmin[a] - oa) * ia, tb = (n.bmax[a] - oa) * ia;
+    if (ta > tb) std::swap(ta, tb);
+    tb *= 1.0000004f;  // 1 + 2*gamma(3): conservative far bound
+    t0 = ta > t0 ? ta : t0;
+    t1 = tb < t1 ? tb : t1;
+    if (!(t0 <= t1)) return false;
+  }
+  tEntry = t0;
+  return true;
+}
+
+static void traverse(const Scene& s, RayHit& r, bool anyHit, long long* nodeVisits = nullptr)
+{
+  V3 inv(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  r.prim = -1;
+  float tfar = r.tfar;
+  long long visits = 0;
+  while (sp)
+  {
+    const Node& n = s.nodes[stack[--sp]];
+    float te;
+    ++visits;
+    if (!boxHit(n, r.o, inv, r.tnear, tfar, te)) continue;
+    if (n.count > 0)
+    {
+      for (int i = 0; i < n.count; ++i)
+      {
+        const Tri& tr = s.tris[n.first + i];
+        float t, u, v;
+        if (intersectTri(tr, r.o, r.d, r.tnear, tfar, t, u, v))
+        {
+          if (anyHit) { r.prim = tr.prim; r.tfar = t; if (nodeVisits) *nodeVisits += visits; return; }
+          if (t < tfar || r.prim < 0 || (t == tfar && tr.prim < r.prim))
+          {
+            tfar = t; r.prim = tr.prim; r.u = u; r.v = v;
+          }
+        }
+      }
+    }
+    else
+    {
+      // push the far child first so the near child is popped first
+      const Node& a = s.nodes[n.first];
+      const Node& b = s.nodes[n.first + 1];
+      float ta, tb;
+      bool ha = boxHit(a, r.o, inv, r.tnear, tfar, ta), hb = boxHit(b, r.o, inv, r.tnear, tfar, tb);
+      if (ha && hb)
+      {
+        if (ta <= tb) { stack[sp++] = n.first + 1; stack[sp++] = n.first; }
+        else { stack[sp++] = n.first; stack[sp++] = n.first + 1; }
+      }
+      else if (ha) stack[sp++] = n.first;
+      else if (hb) stack[sp++] = n.first + 1;
+    }
+  }
+  if (r.prim >= 0) r.tfar = tfar;
+  if (nodeVisits) *nodeVisits += visits;
+}
+
+static void bruteForce(const Scene& s, RayHit& r, bool anyHit)
+{
+  r.prim = -1;
+  float tfar = r.tfar;
+  for (size_t i = 0; i < s.tris.size(); ++i)
+  {
+    float t, u, v;
+    if (intersectTri(s.tris[i], r.o, r.d, r.tnear, tfar, t, u, v))
+    {
+      if (anyHit) { r.prim = s.tris[i].prim; r.tfar = t; return; }
+      if (r.prim < 0 || t < tfar || (t == tfar && s.tris[i].prim < r.prim)) { tfar = t; r.prim = s.tris[i].prim; r.u = u; r.v = v; }
+    }
+  }
+  if (r.prim >= 0) r.tfar = tfar;
+}
+
+// ---------------------------------------------------------------------------------------
+// Hit geometry: geometry/triangle_mesh.cpp:152-213 + render/scene.h:61-72
+// ---------------------------------------------------------------------------------------
+static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u, float v)
+{
+  Ctx c;
+  c.p = o + t * d;  // core/ray_simd.h:58-68
+  c.eps = std::max(t, reduceMax(V3(std::fabs(c.p.x), std::fabs(c.p.y), std::fabs(c.p.z)))) * 0x1.fp-18f;
+  int i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+  V3 p0 = s.pos[i0], p1 = s.pos[i1], p2 = s.pos[i2];
+  V3 Ng = normalize(cross(p1 - p0, p2 - p0));
+  V3 N = Ng;
+  if (!s.nrm.empty())
+  {
+    float b0 = 1.f - u - v;
+    N = normalize(b0 * s.nrm[i0] + u * s.nrm[i1] + v * s.nrm[i2]);
+    if (dot(N, Ng) < 0.f) N = -N;
+  }
+  c.Ng = Ng;
+  c.f.N = N;
+  makeFrame(c.f.U, c.f.V, N);  // no texcoords: triangle_mesh.cpp:235-242
+  c.backfacing = dot(Ng, d) > 0.f;
+  if (c.backfacing) { c.Ng = -c.Ng; c.f.N = -c.f.N; }
+  return c;
+}
+
+// ---------------------------------------------------------------------------------------
+// Integrator
+// ---------------------------------------------------------------------------------------
+struct Camera
+{
+  float pos[3], U[3], V[3], N[3];
+  float fov, nearClip, lensRadius, focalDistance;
+};
+
+struct RenderParams
+{
+  int W, H;
+  uint32_t seed;
+  int maxDepth, rrDepth;
+  float minWeight, maxRadiance;
+};
+
+struct Accum  // sums are FP32, added in sample order (image/accum_buffer.h:439-486)
+{
+  float* color;   // [3][W*H] planar
+  float* weight;  // [W*H]
+  float* albedo; float* albedoW;
+  float* albedo1; float* albedo1W;
+  float* normal; float* normalW;
+  float* normal1; float* normal1W;
+  float* depth; float* depthW;
+  // per-sample debug outputs of the last pass (nullable)
+  float* sampleL;      // [3][W*H]
+  uint32_t* pathHash;  // [W*H]
+  int* firstPrim;      // [W*H]
+};
+
+static inline void add3(float* buf, float* w, int npix, int pix, V3 c)
+{
+  if (!buf) return;
+  buf[pix] += c.x; buf[npix + pix] += c.y; buf[2 * npix + pix] += c.z;
+  w[pix] += 1.f;
+}
+
+// integrator.h:27-46
+static inline V3 clampL(V3 L, float maxRadiance)
+{
+  V3 c(0.f < L.x ? L.x : 0.f, 0.f < L.y ? L.y : 0.f, 0.f < L.z ? L.z : 0.f);
+  float y = luminance(c);
+  if (y > maxRadiance) c *= maxRadiance * (1.0f / y);
+  return c;
+}
+
+static inline uint32_t hashStep(uint32_t h, uint32_t x) { return murmurMix(h, x); }
+
+struct Stats { long long rays = 0; long long nodes = 0; };
+
+// One path of one sample (path_renderer_stream.h:209-268 init, :276-1071 bounce loop)
+static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, int x, int y, uint32_t sampleIndex,
+                      Accum& A, Stats& st, bool recordSample)
+{
+  const int npix = P.W * P.H;
+  const int pix = y * P.W + x;
+  const uint32_t pixelId = uint32_t(x) | (uint32_t(y) << 16);
+  Sampler smp;
+  smp.set(P.seed, sampleIndex, pixelId);
+  float psx, psy, lsx, lsy;
+  smp.get2D(0, psx, psy);  // sampleDimPixel
+  float imx = (float(x) + psx) * (1.0f / float(P.W));
+  float imy = (float(y) + psy) * (1.0f / float(P.H));
+  smp.get2D(2, lsx, lsy);  // sampleDimLens
+
+  // camera/pinhole_camera.h:72-133, thin_lens_camera.h:155-182
+  V3 U(cam.U[0], cam.U[1], cam.U[2]), Vb(cam.V[0], cam.V[1], cam.V[2]), Nb(cam.N[0], cam.N[1], cam.N[2]);
+  float aspect = float(P.W) / float(P.H);
+  float hh = std::tan(0.5f * cam.fov), hw = hh * aspect;
+  V3 imageO = toGlobal(Basis{U, Vb, Nb}, V3(-hw, hh, -1.f));
+  V3 imageDx = toGlobal(Basis{U, Vb, Nb}, V3(2.f * hw, 0.f, 0.f));
+  V3 imageDy = toGlobal(Basis{U, Vb, Nb}, V3(0.f, -2.f * hh, 0.f));
+  V3 org(cam.pos[0], cam.pos[1], cam.pos[2]);
+  V3 dir;
+  float tnear;
+  if (cam.lensRadius == 0.f)  // render/device_cpu.cpp:117-128 picks the pinhole camera
+  {
+    dir = imageO + imx * imageDx + imy * imageDy;
+    float len = length(dir);
+    dir = dir * (1.0f / len);
+    tnear = cam.nearClip * len;
+  }
+  else
+  {
+    float lx, ly;
+    uniformSampleDisk(lsx, lsy, lx, ly);
+    lx *= cam.lensRadius; ly *= cam.lensRadius;
+    V3 lensOffset = toGlobal(Basis{U, Vb, Nb}, V3(lx, ly, 0.f));
+    dir = cam.focalDistance * (imageO + imx * imageDx + imy * imageDy) - lensOffset;
+    float len = length(dir);
+    org = org + lensOffset;
+    dir = dir * (1.0f / len);
+    tnear = (cam.nearClip / cam.focalDistance) * len;
+  }
+
+  V3 T(1.f), L(0.f);
+  int mediumId = 0;
+  float lastPdf = 1.f;
+  int lastType = BsdfType::SpecularReflection;
+  bool aov1Done = false, aov2Done = false;
+  uint32_t hash = 0x9e3779b9u;
+  int firstPrim = -2;
+  V3 camN(Nb);
+
+  for (int depth = 0;; ++depth)
+  {
+    const int sampleDim = 4 + 7 * depth;
+    RayHit r;
+    r.o = org; r.d = dir; r.tnear = tnear; r.tfar = kPosMax;
+    traverse(s, r, false, nullptr);
+    st.rays++;
+    if (depth == 0) firstPrim = r.prim;
+    hash = hashStep(hash, uint32_t(r.prim + 1));
+    int matId = r.prim >= 0 ? s.triMat[r.prim] : 0;
+    if (matId == 0)
+    {
+      // miss shading (:285-415)
+      float lightPdf;
+      V3 Le = evalEnv(s.ls, dir, lightPdf);
+      float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
+      L += T * w * Le;
+      V3 c = clampL(L, P.maxRadiance);
+      add3(A.color, A.weight, npix, pix, c);
+      if (!aov1Done)
+      {
+        add3(A.albedo1, A.albedo1W, npix, pix, V3(0.f));
+        add3(A.normal1, A.normal1W, npix, pix, V3(0.f));
+        if (A.depth) { A.depth[pix] += kPosMax * 2.f; A.depthW[pix] += 1.f; }  // +inf
+      }
+      if (!aov2Done)
+      {
+        add3(A.albedo, A.albedoW, npix, pix, V3(0.f));
+        add3(A.normal, A.normalW, npix, pix, V3(0.f));
+      }
+      if (recordSample)
+      {
+        if (A.sampleL) { A.sampleL[pix] = c.x; A.sampleL[npix + pix] = c.y; A.sampleL[2 * npix + pix] = c.z; }
+        if (A.pathHash) A.pathHash[pix] = hash;
+        if (A.firstPrim) A.firstPrim[pix] = firstPrim;
+      }
+      return;
+    }
+
+    bool ext = false;
+    bool dirOk = false;
+    V3 dirL(0.f);
+    V3 shadowO, shadowD;
+    float shadowNear = 0.f, shadowFar = 0.f;
+    V3 nextO, nextD;
+    float nextNear = 0.f;
+    if (depth < P.maxDepth)
+    {
+      const Material& m = s.mats[matId];
+      Ctx ctx = postIntersect(s, org, dir, r.tfar, r.prim, r.u, r.v);
+      V3 wo = -dir;
+      const int mediumIntId = s.matMedium[matId];
+      Bsdf b = makeBsdf(s.luts, m, ctx, wo, mediumId, mediumIntId);
+      // sampler.resetSample with the same index and pixel (:641-642) leaves the state unchanged
+      bool mAov1 = !aov1Done, mAov2 = !aov2Done;
+      bool mRR = !(mAov1 || mAov2);
+      // media of the configs are clear (transmission color 1): pow(1, far) == 1 (:653-657)
+
+      // direct light (:659-718)
+      if (b.flags & BsdfType::Smooth)
+      {
+        float a0, a1, a2;
+        smp.get3D(sampleDim + 3, a0, a1, a2);
+        V3 wiL;
+        float lightPdf, dist;
+        V3 lw = sampleLight(s.ls, wiL, lightPdf, dist, a0, a1, a2);
+        if (reduceMax(lw) > 0.f && lightPdf > 0.f)
+        {
+          float bsdfPdf;
+          V3 f = bsdfEval(s.luts, b, ctx, wo, wiL, bsdfPdf);
+          if (reduceMax(f) > 0.f)
+          {
+            shadowO = ctx.p; shadowD = wiL; shadowNear = ctx.eps; shadowFar = dist - ctx.eps;
+            V3 dirT = T * f;
+            if (depth >= P.rrDepth)
+            {
+              float contProb = mRR ? std::min(luminance(dirT * (1.0f / bsdfPdf)), 0.95f) : 1.f;
+              bsdfPdf *= contProb;
+            }
+            float w = misWeight(lightPdf, bsdfPdf);
+            dirL = dirT * w * lw;
+            dirOk = true;
+          }
+        }
+      }
+
+      // BSDF sample (:720-730)
+      float b0, b1, b2;
+      smp.get3D(sampleDim + 0, b0, b1, b2);
+      V3 wi(0.f);
+      float pdf = 0.f;
+      int type = 0;
+      V3 bw = bsdfSample(s.luts, b, ctx, wo, wi, pdf, type, b0, b1, b2);
+      ext = reduceMax(bw) > 0.f && pdf > 0.f;
+      int extType = ext ? type : 0;
+      hash = hashStep(hash, uint32_t(extType));
+
+      if (mAov1)  // first-hit AOVs (:739-819)
+      {
+        add3(A.albedo1, A.albedo1W, npix, pix, bsdfAlbedo(b));
+        add3(A.normal1, A.normal1W, npix, pix, b.fr.N);
+        if (A.depth) { A.depth[pix] += dot(ctx.p - V3(cam.pos[0], cam.pos[1], cam.pos[2]), -camN); A.depthW[pix] += 1.f; }
+        aov1Done = true;
+      }
+      mAov2 = mAov2 && (extType & BsdfType::Delta) == 0;  // (:828-832)
+      if (mAov2)
+      {
+        add3(A.albedo, A.albedoW, npix, pix, bsdfAlbedo(b));
+        add3(A.normal, A.normalW, npix, pix, b.fr.N);
+        aov2Done = true;
+      }
+
+      T *= bw;
+      ext = ext && reduceMax(T) >= P.minWeight;
+      if (depth >= P.rrDepth)  // (:880-887)
+      {
+        float contProb = mRR ? std::min(luminance(T), 0.95f) : 1.f;
+        ext = ext && smp.get1D(sampleDim + 6) <= contProb;
+        T *= 1.0f / contProb;
+        pdf *= contProb;
+      }
+      if (ext)
+      {
+        nextO = ctx.p; nextD = wi; nextNear = ctx.eps;
+        if ((type & BsdfType::Continuation) == 0) { lastPdf = pdf; lastType = type; }
+        // material.h:84-95: crossing a media interface swaps interior/exterior
+        if (m.kind == kDielectric && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
+          mediumId = (mediumId == mediumIntId) ? 0 : mediumIntId;
+      }
+    }
+
+    // shadow ray + final pass (:951-1065)
+    if (dirOk)
+    {
+      RayHit sr;
+      sr.o = shadowO; sr.d = shadowD; sr.tnear = shadowNear; sr.tfar = shadowFar;
+      traverse(s, sr, true, nullptr);
+      st.rays++;
+      if (sr.prim < 0) L += dirL;
+    }
+    if (!ext)
+    {
+      V3 c = clampL(L, P.maxRadiance);
+      add3(A.color, A.weight, npix, pix, c);
+      if (recordSample)
+      {
+        if (A.sampleL) { A.sampleL[pix] = c.x; A.sampleL[npix + pix] = c.y; A.sampleL[2 * npix + pix] = c.z; }
+        if (A.pathHash) A.pathHash[pix] = hash;
+        if (A.firstPrim) A.firstPrim[pix] = firstPrim;
+      }
+      return;
+    }
+    org = nextO; dir = nextD; tnear = nextNear;
+  }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ---------------------------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+void* orc_scene_create(int nverts, const float* pos, const float* nrm, int ntris, const int* idx, const int* triMat,
+                       int nmats, const void* mats)
+{
+  Scene* s = new Scene();
+  s->pos.resize(nverts);
+  for (int i = 0; i < nverts; ++i) s->pos[i] = V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  if (nrm)
+  {
+    s->nrm.resize(nverts);
+    for (int i = 0; i < nverts; ++i) s->nrm[i] = V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+  }
+  s->idx.assign(idx, idx + 3 * size_t(ntris));
+  s->triMat.assign(triMat, triMat + ntris);
+  s->mats.assign((const Material*)mats, (const Material*)mats + nmats);
+  computeBounds(*s);
+  assignMedia(*s);
+  precomputeLuts(s->luts);
+  buildBvh(*s);
+  return s;
+}
+
+void orc_scene_destroy(void* p) { delete (Scene*)p; }
+
+float orc_scene_radius(void* p) { return ((Scene*)p)->sceneRadius; }
+
+// lights: records of orc::Light; env image in texture row order (row 0 = nadir), lutMax as
+// the reference's lutSize prop (light/image_light.h:25-40, 2x bucketing while too large)
+int orc_scene_set_lights(void* p, int nlights, const void* lights, int envW, int envH, const float* envRgb, int lutMax)
+{
+  Scene& s = *(Scene*)p;
+  if (nlights < 1 || nlights > 8) return -1;
+  s.lights.assign((const Light*)lights, (const Light*)lights + nlights);
+  if (envRgb && envW > 0 && envH > 0)
+  {
+    s.envRgb.assign(envRgb, envRgb + size_t(envW) * envH * 3);
+    s.env.W = envW; s.env.H = envH; s.env.rgb = s.envRgb.data();
+    int lw = envW, lh = envH, bucket = 1;
+    if (lutMax > 0)
+      while (std::max(lw, lh) > lutMax && lw % 2 == 0 && lh % 2 == 0) { lw /= 2; lh /= 2; bucket *= 2; }
+    std::vector<float> w(size_t(lw) * lh);
+    for (int y = 0; y < lh; ++y)
+      for (int x = 0; x < lw; ++x)
+      {
+        float sum = 0.f;
+        for (int y2 = 0; y2 < bucket; ++y2)
+          for (int x2 = 0; x2 < bucket; ++x2) sum += luminance(envTexel(s.env, x * bucket + x2, y * bucket + y2));
+        w[size_t(y) * lw + x] = sum * std::sin((float(y) + 0.5f) / float(lh) * kPi);
+      }
+    s.envX.resize(size_t(lh) * (lw + 1));
+    s.envY.resize(lh + 1);
+    dist2dInit(s.envX.data(), s.envY.data(), w.data(), lw, lh);
+    s.env.LW = lw; s.env.LH = lh; s.env.xDists = s.envX.data(); s.env.yDist = s.envY.data();
+  }
+  s.ls.count = nlights;
+  s.ls.lights = s.lights.data();
+  s.ls.env = s.env.rgb ? &s.env : nullptr;
+  s.ls.sceneRadius = s.sceneRadius;
+  float pw[8];
+  for (int i = 0; i < nlights; ++i) pw[i] = lightPower(s.lights[i], s.ls.env, s.sceneRadius);
+  dist1dInit(s.ls.cdf, pw, nlights);
+  return 0;
+}
+
+void orc_get_light_cdf(void* p, float* out)
+{
+  Scene& s = *(Scene*)p;
+  for (int i = 0; i <= s.ls.count; ++i) out[i] = s.ls.cdf[i];
+}
+
+void orc_get_luts(void* p, float* out)  // ggxDir, ggxAvg, dielEtaDir, dielEtaAvg, dielRcpEtaDir, dielRcpEtaAvg
+{
+  std::memcpy(out, &((Scene*)p)->luts, sizeof(Luts));
+}
+
+void orc_compute_luts(float* out)
+{
+  Luts* L = new Luts();
+  precomputeLuts(*L);
+  std::memcpy(out, L, sizeof(Luts));
+  delete L;
+}
+
+// rays: n x [ox oy oz dx dy dz near far]; mode 0 = BVH closest, 1 = brute-force closest,
+// 2 = BVH any-hit, 3 = brute-force any-hit
+void orc_intersect(void* p, int n, const float* rays, int mode, int* prim, float* t, float* u, float* v)
+{
+  Scene& s = *(Scene*)p;
+  for (int i = 0; i < n; ++i)
+  {
+    const float* q = rays + 8 * i;
+    RayHit r;
+    r.o = V3(q[0], q[1], q[2]); r.d = V3(q[3], q[4], q[5]); r.tnear = q[6]; r.tfar = q[7];
+    r.u = r.v = 0.f;
+    bool any = mode >= 2;
+    if (mode == 0 || mode == 2) traverse(s, r, any);
+    else bruteForce(s, r, any);
+    prim[i] = r.prim;
+    if (t) t[i] = r.prim >= 0 ? r.tfar : q[7];
+    if (u) u[i] = r.u;
+    if (v) v[i] = r.v;
+  }
+}
+
+// One sample pass per sample index in [sppStart, sppStart+sppCount), pixels split over
+// `threads` std::threads by 32x32 tiles (the reference's Tasking::run over tiles,
+// sys/tasking.h:22-41).  rowBegin/rowEnd bound the rendered rows (the CPU baseline renders a
+// bounded band of the frame).  Returns the number of rays traced.
+long long orc_render(void* p, const void* camp, int W, int H, int rowBegin, int rowEnd, int sppStart, int sppCount,
+                     uint32_t seed, int maxDepth, int rrDepth, float minWeight, float maxRadiance, int threads,
+                     float** buffers /* color, weight, albedo, albedoW, albedo1, albedo1W, normal, normalW, normal1, normal1W, depth, depthW, sampleL */,
+                     uint32_t* pathHash, int* firstPrim)
+{
+  Scene& s = *(Scene*)p;
+  const Camera& cam = *(const Camera*)camp;
+  RenderParams P{W, H, seed, maxDepth, rrDepth, minWeight, maxRadiance};
+  Accum A;
+  A.color = buffers[0]; A.weight = buffers[1];
+  A.albedo = buffers[2]; A.albedoW = buffers[3];
+  A.albedo1 = buffers[4]; A.albedo1W = buffers[5];
+  A.normal = buffers[6]; A.normalW = buffers[7];
+  A.normal1 = buffers[8]; A.normal1W = buffers[9];
+  A.depth = buffers[10]; A.depthW = buffers[11];
+  A.sampleL = buffers[12];
+  A.pathHash = pathHash; A.firstPrim = firstPrim;
+  if (threads < 1) threads = 1;
+  rowBegin = std::max(rowBegin, 0);
+  rowEnd = std::min(rowEnd, H);
+  const int tilesX = (W + 31) / 32, tilesY = (rowEnd - rowBegin + 31) / 32;
+  std::atomic<long long> rays{0};
+  for (int sIdx = sppStart; sIdx < sppStart + sppCount; ++sIdx)
+  {
+    std::atomic<int> next{0};
+    const bool last = sIdx == sppStart + sppCount - 1;
+    auto worker = [&]() {
+      Stats st;
+      for (;;)
+      {
+        int tile = next.fetch_add(1);
+        if (tile >= tilesX * tilesY) break;
+        int tx = tile % tilesX, ty = tile / tilesX;
+        for (int y = rowBegin + ty * 32; y < std::min(rowBegin + ty * 32 + 32, rowEnd); ++y)
+          for (int x = tx * 32; x < std::min(tx * 32 + 32, W); ++x)
+            tracePath(s, cam, P, x, y, uint32_t(sIdx), A, st, last);
+      }
+      rays += st.rays;
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < threads; ++i) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+  }
+  return rays.load();
+}
+
+// ---- unit-level entry points used by the golden-vector tests --------------------------
+
+void orc_sampler(int n, const uint32_t* seedPidIdx, int ndim, float* out1, float* out2, float* out3)
+{
+  for (int i = 0; i < n; ++i)
+  {
+    Sampler s;
+    s.set(seedPidIdx[3 * i], seedPidIdx[3 * i + 2], seedPidIdx[3 * i + 1]);
+    for (int d = 0; d < ndim; ++d)
+    {
+      out1[i * ndim + d] = s.get1D(d);
+      s.get2D(d, out2[(i * ndim + d) * 2], out2[(i * ndim + d) * 2 + 1]);
+      s.get3D(d, out3[(i * ndim + d) * 3], out3[(i * ndim + d) * 3 + 1], out3[(i * ndim + d) * 3 + 2]);
+    }
+  }
+}
+
+void orc_morton(int n, int tileSizeX, int* xy)
+{
+  for (int i = 0; i < n; ++i) morton8x8(i, tileSizeX, xy[2 * i], xy[2 * i + 1]);
+}
+
+// cfg: [w h fov nearClip lensRadius focal px py pz]; s: n x 4; out: n x 2 x 7 (pinhole, thin lens)
+void orc_camera_rays(const float* cfg, int n, const float* s, float* out)
+{
+  V3 U(-1.f, 0.f, 0.f), Vb(0.f, 1.f, 0.f), Nb(0.f, 0.f, -1.f);
+  Basis B{U, Vb, Nb};
+  float W = cfg[0], H = cfg[1], fov = cfg[2], nearClip = cfg[3], lensRadius = cfg[4], focal = cfg[5];
+  V3 pos(cfg[6], cfg[7], cfg[8]);
+  float hh = std::tan(0.5f * fov), hw = hh * (W / H);
+  V3 imageO = toGlobal(B, V3(-hw, hh, -1.f)), imageDx = toGlobal(B, V3(2.f * hw, 0.f, 0.f)), imageDy = toGlobal(B, V3(0.f, -2.f * hh, 0.f));
+  for (int i = 0; i < n; ++i)
+  {
+    const float* q = s + 4 * i;
+    float* o = out + 14 * i;
+    V3 d = imageO + q[0] * imageDx + q[1] * imageDy;
+    float len = length(d);
+    V3 dn = d * (1.0f / len);
+    o[0] = pos.x; o[1] = pos.y; o[2] = pos.z; o[3] = dn.x; o[4] = dn.y; o[5] = dn.z; o[6] = nearClip * len;
+    float lx, ly;
+    uniformSampleDisk(q[2], q[3], lx, ly);
+    lx *= lensRadius; ly *= lensRadius;
+    V3 off = toGlobal(B, V3(lx, ly, 0.f));
+    V3 d2 = focal * (imageO + q[0] * imageDx + q[1] * imageDy) - off;
+    float len2 = length(d2);
+    V3 o2 = pos + off;
+    V3 dn2 = d2 * (1.0f / len2);
+    o[7] = o2.x; o[8] = o2.y; o[9] = o2.z; o[10] = dn2.x; o[11] = dn2.y; o[12] = dn2.z; o[13] = (nearClip / focal) * len2;
+  }
+}
+
+void orc_warps(int n, const float* s, float* cosOut, float* sphOut, float* diskOut, float* coneOut /* n x 2 x 4 */)
+{
+  const float angles[2] = {0.1f, 0.00445f};
+  for (int i = 0; i < n; ++i)
+  {
+    float pdf;
+    V3 c = cosineSampleHemisphere(pdf, s[2 * i], s[2 * i + 1]);
+    cosOut[4 * i] = c.x; cosOut[4 * i + 1] = c.y; cosOut[4 * i + 2] = c.z; cosOut[4 * i + 3] = pdf;
+    V3 sp = uniformSampleSphere(pdf, s[2 * i], s[2 * i + 1]);
+    sphOut[4 * i] = sp.x; sphOut[4 * i + 1] = sp.y; sphOut[4 * i + 2] = sp.z; sphOut[4 * i + 3] = pdf;
+    uniformSampleDisk(s[2 * i], s[2 * i + 1], diskOut[2 * i], diskOut[2 * i + 1]);
+    for (int a = 0; a < 2; ++a)
+    {
+      V3 cn = uniformSampleCone(pdf, s[2 * i], s[2 * i + 1], angles[a]);
+      float* o = coneOut + (i * 2 + a) * 4;
+      o[0] = cn.x; o[1] = cn.y; o[2] = cn.z; o[3] = pdf;
+    }
+  }
+}
+
+void orc_dist1d(const float* w, int size, int n, const float* s, int* idx, float* pdfSr, float* pdfI /* size+2 */)
+{
+  std::vector<float> cdf(size + 1);
+  dist1dInit(cdf.data(), w, size);
+  for (int i = 0; i < n; ++i)
+  {
+    float pdf, sr = s[i];
+    idx[i] = dist1dSampleReuse(cdf.data(), size, pdf, sr);
+    pdfSr[2 * i] = pdf; pdfSr[2 * i + 1] = sr;
+  }
+  for (int i = -1; i <= size; ++i) pdfI[i + 1] = dist1dPdf(cdf.data(), size, i);
+}
+
+void orc_dist2d(const float* w, int W, int H, int n, const float* s, int* idx, float* pdfSr, float* pdfCells)
+{
+  std::vector<float> xd(size_t(H) * (W + 1)), yd(H + 1);
+  dist2dInit(xd.data(), yd.data(), w, W, H);
+  for (int i = 0; i < n; ++i)
+  {
+    float pdf, sx = s[2 * i], sy = s[2 * i + 1];
+    int ix, iy;
+    dist2dSampleReuse(xd.data(), yd.data(), W, H, pdf, sx, sy, ix, iy);
+    idx[2 * i] = ix; idx[2 * i + 1] = iy;
+    pdfSr[3 * i] = pdf; pdfSr[3 * i + 1] = sx; pdfSr[3 * i + 2] = sy;
+  }
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) pdfCells[y * W + x] = dist2dPdf(xd.data(), yd.data(), W, H, x, y);
+}
+
+// params: n x [kind r g b er eg eb roughness eta]; geo: n x [U V N Ng wo wi] (18); s: n x 3
+// outSample: n x [weight3 wi3 pdf]; outType: n; outEval: n x [value3 pdf]
+// Uses precomputed LUTs `luts` (layout of orc::Luts).
+void orc_bsdf(int n, const float* luts, const float* params, const float* geo, const float* s, float* outSample, int* outType, float* outEval)
+{
+  const Luts& L = *(const Luts*)luts;
+  for (int i = 0; i < n; ++i)
+  {
+    const float* pr = params + 9 * i;
+    const float* g = geo + 18 * i;
+    Ctx ctx;
+    ctx.f.U = V3(g[0], g[1], g[2]); ctx.f.V = V3(g[3], g[4], g[5]); ctx.f.N = V3(g[6], g[7], g[8]);
+    ctx.Ng = V3(g[9], g[10], g[11]);
+    V3 wo(g[12], g[13], g[14]), wi(g[15], g[16], g[17]);
+    Bsdf b;
+    b.kind = int(pr[0]);
+    b.fr = ctx.f;  // golden vectors construct the BSDF on the given frame directly
+    b.roughness = pr[7];
+    b.ggx = GGX(roughnessToAlpha(pr[7]));
+    Material m{};
+    m.kind = b.kind; m.r = pr[1]; m.g = pr[2]; m.b = pr[3]; m.er = pr[4]; m.eg = pr[5]; m.eb = pr[6]; m.roughness = pr[7];
+    if (b.kind == kDielectric) m.ior = 1.0f / pr[8];  // eta = 1/ior seen from outside
+    // build parameters exactly as makeBsdf does, then keep the supplied frame
+    Ctx c2 = ctx;
+    Bsdf built = makeBsdf(L, m, c2, wo, 0, b.kind == kDielectric ? (pr[8] > 1.f ? 0 : 1) : -1);
+    built.fr = ctx.f;
+    if (b.kind == kDielectric)
+    {
+      // supplied eta is used as is (the reference constructs RoughDielectricBsdf(eta) directly)
+      float eta = pr[8];
+      built.deta = eta;
+      built.EavgEta = dielAlbedoAvg(L, eta, built.roughness);
+      built.EavgRcpEta = dielAlbedoAvg(L, 1.0f / eta, built.roughness);
+      float FavgEta = fresnelDielectricAvg(eta), FavgRcpEta = fresnelDielectricAvg(1.0f / eta);
+      float a = (1.f - FavgEta) / (1.f - built.EavgRcpEta), bb = (1.f - FavgRcpEta) / (1.f - built.EavgEta);
+      float x = bb / (a + bb);
+      built.fmsRatio = 1.f - x * (1.f - FavgEta);
+    }
+    V3 swi(0.f);
+    float spdf = 0.f;
+    int stype = 0;
+    V3 w = bsdfSample(L, built, ctx, wo, swi, spdf, stype, s[3 * i], s[3 * i + 1], s[3 * i + 2]);
+    float* o = outSample + 7 * i;
+    o[0] = w.x; o[1] = w.y; o[2] = w.z; o[3] = swi.x; o[4] = swi.y; o[5] = swi.z; o[6] = spdf;
+    outType[i] = stype;
+    float epdf;
+    V3 ev = bsdfEval(L, built, ctx, wo, wi, epdf);
+    float* e = outEval + 4 * i;
+    e[0] = ev.x; e[1] = ev.y; e[2] = ev.z; e[3] = epdf;
+  }
+}
+
+// rays: n x [o d t u v]; prim: n; out ctx: n x 16 [p eps Ng U V N] before the orientation flip,
+// frame: n x 9 after flip + makeFrame(wo = -d)
+void orc_post_intersect(int nverts, const float* pos, const float* nrm, const int* idx, int n, const float* rays,
+                        const int* prim, float* outCtx, float* outFrame)
+{
+  Scene s;
+  s.pos.resize(nverts);
+  for (int i = 0; i < nverts; ++i) s.pos[i] = V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  if (nrm)
+  {
+    s.nrm.resize(nverts);
+    for (int i = 0; i < nverts; ++i) s.nrm[i] = V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+  }
+  int maxIdx = 0;
+  for (int i = 0; i < n; ++i) maxIdx = std::max(maxIdx, prim[i]);
+  s.idx.assign(idx, idx + 3 * size_t(maxIdx + 1));
+  for (int i = 0; i < n; ++i)
+  {
+    const float* q = rays + 9 * i;
+    V3 o(q[0], q[1], q[2]), d(q[3], q[4], q[5]);
+    Ctx c = postIntersect(s, o, d, q[6], prim[i], q[7], q[8]);
+    // undo the flip for the pre-flip output
+    V3 Ng = c.backfacing ? -c.Ng : c.Ng, N = c.backfacing ? -c.f.N : c.f.N;
+    float* oc = outCtx + 16 * i;
+    float vals[16] = {c.p.x, c.p.y, c.p.z, c.eps, Ng.x, Ng.y, Ng.z, c.f.U.x, c.f.U.y, c.f.U.z, c.f.V.x, c.f.V.y, c.f.V.z, N.x, N.y, N.z};
+    std::memcpy(oc, vals, sizeof(vals));
+    Basis fr = makeShadingFrame(c, -d);
+    float* of = outFrame + 9 * i;
+    float fv[9] = {fr.U.x, fr.U.y, fr.U.z, fr.V.x, fr.V.y, fr.V.z, fr.N.x, fr.N.y, fr.N.z};
+    std::memcpy(of, fv, sizeof(fv));
+  }
+}
+
+void orc_env_eval(void* p, int n, const float* dirs, float* out /* n x 4: value3, pdf */)
+{
+  Scene& s = *(Scene*)p;
+  for (int i = 0; i < n; ++i)
+  {
+    float pdf;
+    V3 v = evalEnv(s.ls, V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), pdf);
+    out[4 * i] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = pdf;
+  }
+}
+
+void orc_light_sample(void* p, int n, const float* s3, float* out /* n x 8: wi3 weight3 pdf dist */)
+{
+  Scene& s = *(Scene*)p;
+  for (int i = 0; i < n; ++i)
+  {
+    V3 wi;
+    float pdf, dist;
+    V3 w = sampleLight(s.ls, wi, pdf, dist, s3[3 * i], s3[3 * i + 1], s3[3 * i + 2]);
+    float* o = out + 8 * i;
+    o[0] = wi.x; o[1] = wi.y; o[2] = wi.z; o[3] = w.x; o[4] = w.y; o[5] = w.z; o[6] = pdf; o[7] = dist;
+  }
+}
+
+}  // extern "C"

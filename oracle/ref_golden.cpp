@@ -1,0 +1,466 @@
+// Golden-vector harness: TEST INFRASTRUCTURE ONLY.
+//
+// Compiled in the build container against the reference's own sources (include path
+// /root/reference, see oracle/Makefile.ref) and run once by tests/golden/make_golden.py to
+// write the fixtures under tests/golden/.  Nothing in the product links or loads it.
+//
+// Every block calls the reference's SIMD (stream-path) code on inputs generated here and
+// dumps inputs + outputs as raw little-endian arrays plus a manifest.  The reference code
+// exercised is, per block:
+//   sampler   sampling/sobol_burley_sampler_simd.h:36-85, sampling/sobol_burley.h:150-297
+//   morton    image/morton.h:32-39 (+ image/morton.cpp tables)
+//   camera    camera/pinhole_camera.h:72-133, camera/thin_lens_camera.h:155-182
+//   warps     sampling/shape_sampler.h (cosine hemisphere, sphere, cone, disk)
+//   dist      sampling/distribution1d.{h,cpp}, sampling/distribution2d.{h,cpp}
+//   luts      bsdf/albedo_tables.cpp:173-186
+//   bsdf      bsdf/diffuse.h:45-89, bsdf/rough_conductor.h:106-228,
+//             bsdf/rough_dielectric.h:265-548, bsdf/multi_bsdf.h:151-260, bsdf/optics.h
+//   postisect geometry/triangle_mesh.cpp:152-257, core/ray_simd.h:63-68,
+//             core/shading_context.h:144-191 (makeFrame)
+
+#include <cstdio>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sampling/sobol_burley_sampler_simd.h"
+#include "image/morton.h"
+#include "camera/thin_lens_camera.h"
+#include "sampling/shape_sampler.h"
+#include "sampling/distribution1d.h"
+#include "sampling/distribution2d.h"
+#include "bsdf/albedo_tables.h"
+#include "bsdf/diffuse.h"
+#include "bsdf/rough_conductor.h"
+#include "bsdf/rough_dielectric.h"
+#include "bsdf/multi_bsdf.h"
+#include "geometry/triangle_mesh.h"
+
+using namespace prt;
+
+static std::string gOutDir;
+static FILE* gManifest = nullptr;
+static bool gFirst = true;
+
+template <class T>
+static void dump(const char* name, const char* dtype, const std::vector<T>& v, std::vector<int> shape)
+{
+  std::string path = gOutDir + "/" + name + ".bin";
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) { perror(path.c_str()); exit(1); }
+  fwrite(v.data(), sizeof(T), v.size(), f);
+  fclose(f);
+  fprintf(gManifest, "%s\n  \"%s\": {\"dtype\": \"%s\", \"shape\": [", gFirst ? "" : ",", name, dtype);
+  for (size_t i = 0; i < shape.size(); ++i) fprintf(gManifest, "%s%d", i ? ", " : "", shape[i]);
+  fprintf(gManifest, "]}");
+  gFirst = false;
+}
+
+// Harness-local input generator (not reference code): 32-bit xorshift, floats in [0,1).
+struct InputRng
+{
+  uint32_t s;
+  explicit InputRng(uint32_t seed) : s(seed ? seed : 1u) {}
+  uint32_t u() { s ^= s << 13; s ^= s >> 17; s ^= s << 5; return s; }
+  float f() { return float(u() >> 8) * (1.0f / 16777216.0f); }
+  Vec3f dir()
+  {
+    for (;;)
+    {
+      Vec3f d(2.f*f()-1.f, 2.f*f()-1.f, 2.f*f()-1.f);
+      float l2 = dot(d, d);
+      if (l2 > 1e-4f && l2 <= 1.f) return d * (1.0f / std::sqrt(l2));
+    }
+  }
+};
+
+static void goldenSampler()
+{
+  const uint32_t seeds[] = {0u, 1u, 0x9e3779b9u};
+  const uint32_t pixelIds[] = {0u, 1u, 511u | (511u << 16), 1919u | (1079u << 16),
+                               37u | (200u << 16), 4095u | (4095u << 16), 12u | (3u << 16), 800u | (9u << 16)};
+  const uint32_t indices[] = {0u, 1u, 2u, 3u, 7u, 15u, 63u, 64u, 1000u, 65535u, 1u << 20, 0x7fffffffu};
+  const int nDim = 64;
+  std::vector<uint32_t> in;   // seed, pixelId, index
+  std::vector<float> out1, out2, out3;
+  for (uint32_t seed : seeds)
+    for (uint32_t pid : pixelIds)
+      for (uint32_t idx : indices)
+      {
+        in.push_back(seed); in.push_back(pid); in.push_back(idx);
+        SobolBurleySamplerSimd<false> sampler;
+        sampler.init(4 + 7 * 100, 0, seed);
+        SobolBurleySamplerSimd<false>::State st;
+        sampler.setSample(vbool(one), st, vuint(idx), vuint(pid));
+        for (int d = 0; d < nDim; ++d)
+        {
+          vfloat a = sampler.get1D(vbool(one), st, d);
+          Vec2vf b = sampler.get2D(vbool(one), st, d);
+          Vec3vf c = sampler.get3D(vbool(one), st, d);
+          out1.push_back(a[0]);
+          out2.push_back(b.x[0]); out2.push_back(b.y[0]);
+          out3.push_back(c.x[0]); out3.push_back(c.y[0]); out3.push_back(c.z[0]);
+        }
+      }
+  int n = (int)(in.size() / 3);
+  dump("sampler_in", "u4", in, {n, 3});
+  dump("sampler_1d", "f4", out1, {n, nDim});
+  dump("sampler_2d", "f4", out2, {n, nDim, 2});
+  dump("sampler_3d", "f4", out3, {n, nDim, 3});
+}
+
+static void goldenMorton()
+{
+  std::vector<int32_t> out;
+  for (int taskId = 0; taskId < 1024; taskId += simdSize)
+  {
+    Vec2vi p = getMorton8x8Step<32>(taskId);
+    for (int k = 0; k < simdSize; ++k) { out.push_back(p.x[k]); out.push_back(p.y[k]); }
+  }
+  dump("morton32", "i4", out, {1024, 2});
+}
+
+static void goldenCamera()
+{
+  // in: [width, height, fov, nearClip, lensRadius, focalDistance, posx, posy, posz]
+  struct Cfg { int w, h; float fov, nearClip, lensRadius, focal; Vec3f pos; };
+  const Cfg cfgs[] = {
+    {512, 512, 45.f * float(pi) / 180.f, 0.01f, 0.f, 1.f, Vec3f(0.f)},
+    {1920, 1080, 45.f * float(pi) / 180.f, 0.01f, 0.05f, 15.f, Vec3f(0.f)},
+    {640, 360, 30.f * float(pi) / 180.f, 0.05f, 0.2f, 4.f, Vec3f(1.f, 2.f, -3.f)},
+  };
+  const Basis3f basis(Vec3f(-1.f, 0.f, 0.f), Vec3f(0.f, 1.f, 0.f), Vec3f(0.f, 0.f, -1.f));
+  InputRng rng(1234);
+  const int nRay = 256;
+  std::vector<float> cfgOut, sIn, rayOut;
+  for (const Cfg& c : cfgs)
+  {
+    cfgOut.insert(cfgOut.end(), {float(c.w), float(c.h), c.fov, c.nearClip, c.lensRadius, c.focal, c.pos.x, c.pos.y, c.pos.z});
+    Props props;
+    props.set("position", c.pos);
+    props.set("basis", basis);
+    props.set("nearClip", c.nearClip);
+    props.set("fov", c.fov);
+    props.set("width", c.w);
+    props.set("height", c.h);
+    props.set("lensRadius", c.lensRadius);
+    props.set("focalDistance", c.focal);
+    PinholeCamera pin(props);
+    ThinLensCamera thin(props);
+    for (int i = 0; i < nRay; i += simdSize)
+    {
+      CameraSampleSimd cs;
+      for (int k = 0; k < simdSize; ++k)
+      {
+        cs.image.x[k] = rng.f(); cs.image.y[k] = rng.f();
+        cs.lens.x[k] = rng.f(); cs.lens.y[k] = rng.f();
+        sIn.insert(sIn.end(), {cs.image.x[k], cs.image.y[k], cs.lens.x[k], cs.lens.y[k]});
+      }
+      RaySimd r0, r1;
+      pin.getRay(r0, cs);
+      thin.getRay(r1, cs);
+      for (int k = 0; k < simdSize; ++k)
+      {
+        rayOut.insert(rayOut.end(), {r0.org.x[k], r0.org.y[k], r0.org.z[k], r0.dir.x[k], r0.dir.y[k], r0.dir.z[k], r0.near[k],
+                                     r1.org.x[k], r1.org.y[k], r1.org.z[k], r1.dir.x[k], r1.dir.y[k], r1.dir.z[k], r1.near[k]});
+      }
+    }
+  }
+  int nc = 3;
+  dump("camera_cfg", "f4", cfgOut, {nc, 9});
+  dump("camera_s", "f4", sIn, {nc, nRay, 4});
+  dump("camera_rays", "f4", rayOut, {nc, nRay, 2, 7});
+}
+
+static void goldenWarps()
+{
+  InputRng rng(777);
+  const int n = 512;
+  std::vector<float> sIn, outCos, outSph, outCone, outDisk;
+  const float angles[2] = {0.1f, 0.00445f};
+  for (int i = 0; i < n; i += simdSize)
+  {
+    Vec2vf s;
+    for (int k = 0; k < simdSize; ++k) { s.x[k] = rng.f(); s.y[k] = rng.f(); sIn.push_back(s.x[k]); sIn.push_back(s.y[k]); }
+    vfloat pc, ps;
+    Vec3vf c = cosineSampleHemisphere(pc, s);
+    Vec3vf sp = uniformSampleSphere(ps, s);
+    Vec2vf d = uniformSampleDisk(s);
+    for (int k = 0; k < simdSize; ++k)
+    {
+      outCos.insert(outCos.end(), {c.x[k], c.y[k], c.z[k], pc[k]});
+      outSph.insert(outSph.end(), {sp.x[k], sp.y[k], sp.z[k], ps[k]});
+      outDisk.insert(outDisk.end(), {d.x[k], d.y[k]});
+    }
+    for (float a : angles)
+    {
+      vfloat pa;
+      Vec3vf cn = uniformSampleCone(pa, s, vfloat(a));
+      for (int k = 0; k < simdSize; ++k) outCone.insert(outCone.end(), {cn.x[k], cn.y[k], cn.z[k], pa[k]});
+    }
+  }
+  dump("warp_s", "f4", sIn, {n, 2});
+  dump("warp_cosine", "f4", outCos, {n, 4});
+  dump("warp_sphere", "f4", outSph, {n, 4});
+  dump("warp_disk", "f4", outDisk, {n, 2});
+  dump("warp_cone", "f4", outCone, {n / simdSize, 2, simdSize, 4});
+}
+
+static void goldenDistributions()
+{
+  // 1D: light-power style weights, including a zero bucket
+  const float w1[5] = {0.2f, 5.0f, 0.0f, 1.3f, 0.01f};
+  Distribution1D d1(w1, 5);
+  InputRng rng(99);
+  const int n = 256;
+  std::vector<float> s1In, s1Out;
+  std::vector<int32_t> i1Out;
+  for (int i = 0; i < n; i += simdSize)
+  {
+    vfloat s;
+    for (int k = 0; k < simdSize; ++k) { s[k] = rng.f(); s1In.push_back(s[k]); }
+    vfloat pdf;
+    vfloat sr = s;
+    vint idx = d1.sampleReuse(vbool(one), pdf, sr);
+    for (int k = 0; k < simdSize; ++k) { i1Out.push_back(idx[k]); s1Out.push_back(pdf[k]); s1Out.push_back(sr[k]); }
+  }
+  std::vector<float> p1;
+  for (int i = -1; i <= 5; ++i) p1.push_back(d1.pdf(i));
+  dump("dist1_w", "f4", std::vector<float>(w1, w1 + 5), {5});
+  dump("dist1_s", "f4", s1In, {n});
+  dump("dist1_idx", "i4", i1Out, {n});
+  dump("dist1_pdf_sr", "f4", s1Out, {n, 2});
+  dump("dist1_pdf_i", "f4", p1, {7});
+
+  // 2D: 16x8 grid with a zero row and zero cells
+  const int W = 16, H = 8;
+  std::vector<float> w2(W * H);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x)
+      w2[y * W + x] = (y == 3) ? 0.f : ((x == 5) ? 0.f : (0.1f + rng.f() * 4.f * float(1 + (x * y) % 3)));
+  Distribution2D d2(w2.data(), Vec2i(W, H));
+  std::vector<float> s2In, s2Out;
+  std::vector<int32_t> i2Out;
+  for (int i = 0; i < n; i += simdSize)
+  {
+    Vec2vf s;
+    for (int k = 0; k < simdSize; ++k) { s.x[k] = rng.f(); s.y[k] = rng.f(); s2In.push_back(s.x[k]); s2In.push_back(s.y[k]); }
+    vfloat pdf;
+    Vec2vf sr = s;
+    Vec2vi idx = d2.sampleReuse(vbool(one), pdf, sr);
+    for (int k = 0; k < simdSize; ++k)
+    {
+      i2Out.push_back(idx.x[k]); i2Out.push_back(idx.y[k]);
+      s2Out.insert(s2Out.end(), {pdf[k], sr.x[k], sr.y[k]});
+    }
+  }
+  std::vector<float> p2;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x)
+      p2.push_back(d2.pdf(Vec2i(x, y)));
+  dump("dist2_w", "f4", w2, {H, W});
+  dump("dist2_s", "f4", s2In, {n, 2});
+  dump("dist2_idx", "i4", i2Out, {n, 2});
+  dump("dist2_pdf_sr", "f4", s2Out, {n, 3});
+  dump("dist2_pdf_cells", "f4", p2, {H, W});
+}
+
+static void goldenLuts()
+{
+  auto v = [](const float* p, int n) { return std::vector<float>(p, p + n); };
+  dump("lut_ggx_dir", "f4", v(MicrofacetAlbedoTable::dir, 32 * 32), {32, 32});
+  dump("lut_ggx_avg", "f4", v(MicrofacetAlbedoTable::avg, 32), {32});
+  dump("lut_diel_eta_dir", "f4", v(MicrofacetDielectricAlbedoTable::etaDir, 16 * 16 * 16), {16, 16, 16});
+  dump("lut_diel_eta_avg", "f4", v(MicrofacetDielectricAlbedoTable::etaAvg, 16 * 16), {16, 16});
+  dump("lut_diel_rcpeta_dir", "f4", v(MicrofacetDielectricAlbedoTable::rcpEtaDir, 16 * 16 * 16), {16, 16, 16});
+  dump("lut_diel_rcpeta_avg", "f4", v(MicrofacetDielectricAlbedoTable::rcpEtaAvg, 16 * 16), {16, 16});
+}
+
+// BSDF kinds: 0 diffuse, 1 rough conductor (artistic Fresnel), 2 rough dielectric.  Each is
+// wrapped in a MultiBsdfSimd with weight 1, as StandardMaterial builds it (material/standard.cpp:428-573).
+static void goldenBsdf()
+{
+  InputRng rng(4242);
+  const int nPerKind = 768;
+  // params: kind, r,g,b (albedo / reflectivity), er,eg,eb (edge tint), roughness, eta
+  std::vector<float> params, geo, samp, outSample, outEval;
+  std::vector<int32_t> outType;
+  int n = 0;
+  for (int kind = 0; kind < 3; ++kind)
+  {
+    for (int i = 0; i < nPerKind; i += simdSize)
+    {
+      // per-batch material (uniform across lanes, as in a material bucket)
+      Vec3f col(0.05f + 0.9f * rng.f(), 0.05f + 0.9f * rng.f(), 0.05f + 0.9f * rng.f());
+      Vec3f edge(rng.f(), rng.f(), rng.f());
+      float roughness = (i % 64 == 0) ? 0.0f : std::sqrt(0.01f + 0.5f * rng.f());
+      float eta = (kind == 2 && (i / simdSize) % 2) ? 1.5f : 1.f / 1.5f;
+
+      Basis3vf frame;
+      ShadingContextSimd* ctx = new ShadingContextSimd();
+      Vec3vf wo, wi;
+      Vec3vf s3;
+      for (int k = 0; k < simdSize; ++k)
+      {
+        Vec3f N = rng.dir();
+        Vec3f U, V;
+        makeFrame(U, V, N);
+        Vec3f Ng = N;
+        if (rng.f() < 0.3f) { Ng = normalize(N + 0.2f * rng.dir()); if (dot(Ng, N) < 0.2f) Ng = N; }
+        Vec3f o = rng.dir();
+        if (rng.f() < 0.85f && dot(o, Ng) < 0.f) o = -o;
+        Vec3f w = rng.dir();
+        if (kind != 2 && rng.f() < 0.7f && dot(w, N) < 0.f) w = -w;
+        frame.U.x[k] = U.x; frame.U.y[k] = U.y; frame.U.z[k] = U.z;
+        frame.V.x[k] = V.x; frame.V.y[k] = V.y; frame.V.z[k] = V.z;
+        frame.N.x[k] = N.x; frame.N.y[k] = N.y; frame.N.z[k] = N.z;
+        ctx->Ng.x[k] = Ng.x; ctx->Ng.y[k] = Ng.y; ctx->Ng.z[k] = Ng.z;
+        wo.x[k] = o.x; wo.y[k] = o.y; wo.z[k] = o.z;
+        wi.x[k] = w.x; wi.y[k] = w.y; wi.z[k] = w.z;
+        s3.x[k] = rng.f(); s3.y[k] = rng.f(); s3.z[k] = rng.f();
+        params.insert(params.end(), {float(kind), col.x, col.y, col.z, edge.x, edge.y, edge.z, roughness, eta});
+        geo.insert(geo.end(), {U.x, U.y, U.z, V.x, V.y, V.z, N.x, N.y, N.z, Ng.x, Ng.y, Ng.z, o.x, o.y, o.z, w.x, w.y, w.z});
+        samp.insert(samp.end(), {s3.x[k], s3.y[k], s3.z[k]});
+      }
+      vbool m = one;
+      BsdfSimd* child = nullptr;
+      DiffuseBrdfSimd* diffuse = nullptr;
+      RoughConductorBrdfSimd<FresnelConductorArtistic<vfloat>>* cond = nullptr;
+      RoughDielectricBsdfSimd* diel = nullptr;
+      if (kind == 0)
+        child = diffuse = new DiffuseBrdfSimd(m, &frame, Vec3vf(col));
+      else if (kind == 1)
+        child = cond = new RoughConductorBrdfSimd<FresnelConductorArtistic<vfloat>>(m, &frame, FresnelConductorArtistic<vfloat>(Vec3vf(col), Vec3vf(edge)), vfloat(roughness), vfloat(0.f));
+      else
+        child = diel = new RoughDielectricBsdfSimd(m, &frame, vfloat(eta), vfloat(roughness), vfloat(0.f));
+      MultiBsdfSimd multi;
+      multi.add(m, child, vfloat(1.f));
+
+      Vec3vf swi = zero;
+      vfloat spdf = zero;
+      vint stype = zero;
+      Vec3vf sw = multi.sample(m, *ctx, wo, swi, spdf, stype, BsdfSampleSimd(s3));
+      vfloat epdf;
+      Vec3vf ev = multi.eval(m, *ctx, wo, wi, epdf);
+      for (int k = 0; k < simdSize; ++k)
+      {
+        outSample.insert(outSample.end(), {sw.x[k], sw.y[k], sw.z[k], swi.x[k], swi.y[k], swi.z[k], spdf[k]});
+        outType.push_back(stype[k]);
+        outEval.insert(outEval.end(), {ev.x[k], ev.y[k], ev.z[k], epdf[k]});
+      }
+      delete child;
+      delete ctx;
+      n += simdSize;
+    }
+  }
+  dump("bsdf_params", "f4", params, {n, 9});
+  dump("bsdf_geo", "f4", geo, {n, 18});
+  dump("bsdf_s", "f4", samp, {n, 3});
+  dump("bsdf_sample", "f4", outSample, {n, 7});
+  dump("bsdf_sample_type", "i4", outType, {n});
+  dump("bsdf_eval", "f4", outEval, {n, 4});
+}
+
+static void goldenPostIntersect()
+{
+  InputRng rng(31337);
+  const int nTri = 64;
+  const int nRay = 512;
+  for (int withNormals = 0; withNormals < 2; ++withNormals)
+  {
+    TriangleMesh mesh;
+    mesh.alloc(nTri, nTri * 3, withNormals != 0, 0, false);
+    std::vector<float> verts, norms;
+    std::vector<int32_t> idx;
+    for (int t = 0; t < nTri; ++t)
+    {
+      Vec3f c(20.f * rng.f() - 10.f, 8.f * rng.f(), 5.f + 25.f * rng.f());
+      for (int j = 0; j < 3; ++j)
+      {
+        Vec3f p = c + 1.5f * rng.dir();
+        mesh.setPosition(t * 3 + j, p);
+        verts.insert(verts.end(), {p.x, p.y, p.z});
+        if (withNormals)
+        {
+          Vec3f nn = rng.dir();
+          mesh.setNormal(t * 3 + j, nn);
+          norms.insert(norms.end(), {nn.x, nn.y, nn.z});
+        }
+      }
+      mesh.indices[t] = Vec3i(t * 3, t * 3 + 1, t * 3 + 2);
+      idx.insert(idx.end(), {t * 3, t * 3 + 1, t * 3 + 2});
+    }
+    mesh.materialIds[0] = 1;
+
+    std::vector<float> rayIn, ctxOut, frameOut;
+    std::vector<int32_t> primIn;
+    for (int i = 0; i < nRay; i += simdSize)
+    {
+      RaySimd ray;
+      HitSimd hit;
+      for (int k = 0; k < simdSize; ++k)
+      {
+        int prim = int(rng.u() % nTri);
+        float u = rng.f(), v = rng.f();
+        if (u + v > 1.f) { u = 1.f - u; v = 1.f - v; }
+        Vec3f o(20.f * rng.f() - 10.f, 8.f * rng.f() - 4.f, 30.f * rng.f() - 5.f);
+        Vec3f d = rng.dir();
+        float t = 0.5f + 40.f * rng.f();
+        ray.org.x[k] = o.x; ray.org.y[k] = o.y; ray.org.z[k] = o.z;
+        ray.dir.x[k] = d.x; ray.dir.y[k] = d.y; ray.dir.z[k] = d.z;
+        ray.near[k] = 0.f; ray.far[k] = t;
+        hit.primId[k] = prim; hit.geomId[k] = 0; hit.instId[k] = -1; hit.u[k] = u; hit.v[k] = v;
+        rayIn.insert(rayIn.end(), {o.x, o.y, o.z, d.x, d.y, d.z, t, u, v});
+        primIn.push_back(prim);
+      }
+      ShadingContextSimd* ctx = new ShadingContextSimd();
+      vbool m = one;
+      mesh.postIntersect(m, ray, hit, *ctx, 0);
+      for (int k = 0; k < simdSize; ++k)
+        ctxOut.insert(ctxOut.end(), {ctx->p.x[k], ctx->p.y[k], ctx->p.z[k], ctx->eps[k],
+                                     ctx->Ng.x[k], ctx->Ng.y[k], ctx->Ng.z[k],
+                                     ctx->f.U.x[k], ctx->f.U.y[k], ctx->f.U.z[k],
+                                     ctx->f.V.x[k], ctx->f.V.y[k], ctx->f.V.z[k],
+                                     ctx->f.N.x[k], ctx->f.N.y[k], ctx->f.N.z[k]});
+      // Orientation flip toward the viewer, as render/scene.h:61-72 applies after postIntersect
+      vbool backfacing = m & (dot(ctx->Ng, ray.dir) > 0.f);
+      set(backfacing, ctx->Ng, -ctx->Ng);
+      set(backfacing, ctx->f.N, -ctx->f.N);
+      ctx->begin();
+      Vec3vf wo = -ray.dir;
+      const Basis3vf* fr = ctx->makeFrame(m, wo);
+      for (int k = 0; k < simdSize; ++k)
+        frameOut.insert(frameOut.end(), {fr->U.x[k], fr->U.y[k], fr->U.z[k], fr->V.x[k], fr->V.y[k], fr->V.z[k], fr->N.x[k], fr->N.y[k], fr->N.z[k]});
+      delete ctx;
+    }
+    std::string sfx = withNormals ? "_n" : "";
+    dump(("pi_verts" + sfx).c_str(), "f4", verts, {nTri * 3, 3});
+    if (withNormals) dump("pi_normals_n", "f4", norms, {nTri * 3, 3});
+    dump(("pi_indices" + sfx).c_str(), "i4", idx, {nTri, 3});
+    dump(("pi_rays" + sfx).c_str(), "f4", rayIn, {nRay, 9});
+    dump(("pi_prim" + sfx).c_str(), "i4", primIn, {nRay});
+    dump(("pi_ctx" + sfx).c_str(), "f4", ctxOut, {nRay, 16});
+    dump(("pi_frame" + sfx).c_str(), "f4", frameOut, {nRay, 9});
+  }
+}
+
+int main(int argc, char** argv)
+{
+  if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
+  gOutDir = argv[1];
+  gManifest = fopen((gOutDir + "/manifest.json").c_str(), "w");
+  fprintf(gManifest, "{");
+  precomputeAlbedoTables();
+  goldenSampler();
+  goldenMorton();
+  goldenCamera();
+  goldenWarps();
+  goldenDistributions();
+  goldenLuts();
+  goldenBsdf();
+  goldenPostIntersect();
+  fprintf(gManifest, "\n}\n");
+  fclose(gManifest);
+  return 0;
+}

@@ -1,0 +1,130 @@
+"""The oracle (CPU restatement) against golden vectors produced by the reference's own code.
+
+Fixtures: tests/golden/ref_vectors.npz, written by tests/golden/make_golden.py from the
+harness oracle/ref_golden.cpp compiled against /root/reference.  Integer work is compared
+bit for bit; float work to tolerances that cover the reference's rcp/rsqrt approximations
+(math/simd/vfloat8_avx2.h:227-232,327-342) and ISPC transcendentals (math/simd.h:150-657).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def test_sampler_bit_exact(golden):
+    # sampling/sobol_burley_sampler_simd.h:36-85 via sobol_burley.h:222-297
+    o1, o2, o3 = O.sampler(golden["sampler_in"], 64)
+    assert np.array_equal(o1.view(np.uint32), golden["sampler_1d"].view(np.uint32))
+    assert np.array_equal(o2.view(np.uint32), golden["sampler_2d"].view(np.uint32))
+    assert np.array_equal(o3.view(np.uint32), golden["sampler_3d"].view(np.uint32))
+
+
+def test_sampler_closed_forms():
+    # Dimension 0 of a 1D sample is Owen-scrambled van der Corput: over 2^k consecutive
+    # indices the samples are stratified, one per interval of width 2^-k.
+    seeds = np.array([[7, 3 | (5 << 16), i] for i in range(256)], np.uint32)
+    o1, o2, _ = O.sampler(seeds, 8)
+    for d in range(8):
+        assert sorted(np.floor(o1[:, d] * 256).astype(int)) == list(range(256))
+    # A 2D set is a (0,8,2)-net: every 16x16 elementary box holds exactly one point
+    for d in range(8):
+        cells = np.floor(o2[:, d, 0] * 16).astype(int) * 16 + np.floor(o2[:, d, 1] * 16).astype(int)
+        assert len(np.unique(cells)) == 256
+        cells = np.floor(o2[:, d, 0] * 256).astype(int) * 1 + np.floor(o2[:, d, 1] * 1).astype(int)
+        assert len(np.unique(cells)) == 256
+    assert (o1 >= 0).all() and (o1 < 1).all()
+
+
+def test_morton_order(golden):
+    # image/morton.h:32-39 for 32-pixel-wide tiles
+    assert np.array_equal(O.morton(1024, 32), golden["morton32"])
+
+
+def test_camera_rays(golden):
+    # camera/pinhole_camera.h:128-133, camera/thin_lens_camera.h:175-182
+    for c in range(golden["camera_cfg"].shape[0]):
+        out = O.camera_rays(golden["camera_cfg"][c], golden["camera_s"][c])
+        ref = golden["camera_rays"][c]
+        np.testing.assert_allclose(out[..., :6], ref[..., :6], atol=1e-6, rtol=0)
+        np.testing.assert_allclose(out[..., 6], ref[..., 6], rtol=2e-5)
+
+
+def test_shape_warps(golden):
+    c, sp, d, cn = O.warps(golden["warp_s"])
+    np.testing.assert_allclose(c, golden["warp_cosine"], atol=1e-6)
+    np.testing.assert_allclose(sp, golden["warp_sphere"], atol=1e-6)
+    np.testing.assert_allclose(d, golden["warp_disk"], atol=1e-6)
+    ref_cone = golden["warp_cone"].transpose(0, 2, 1, 3).reshape(-1, 2, 4)
+    np.testing.assert_allclose(cn[..., :3], ref_cone[..., :3], atol=5e-6)
+    np.testing.assert_allclose(cn[..., 3], ref_cone[..., 3], rtol=1e-5)
+
+
+def test_distributions(golden):
+    idx, pdf_sr, pdf_i = O.dist1d(golden["dist1_w"], golden["dist1_s"])
+    assert np.array_equal(idx, golden["dist1_idx"])
+    np.testing.assert_allclose(pdf_sr, golden["dist1_pdf_sr"], atol=1e-6)
+    np.testing.assert_allclose(pdf_i, golden["dist1_pdf_i"], atol=1e-7)
+    idx, pdf_sr, cells = O.dist2d(golden["dist2_w"], golden["dist2_s"])
+    assert np.array_equal(idx, golden["dist2_idx"])
+    np.testing.assert_allclose(pdf_sr[:, 0], golden["dist2_pdf_sr"][:, 0], rtol=1e-5)
+    np.testing.assert_allclose(pdf_sr[:, 1:], golden["dist2_pdf_sr"][:, 1:], atol=2e-5)
+    np.testing.assert_allclose(cells, golden["dist2_pdf_cells"], atol=1e-7)
+
+
+def test_energy_compensation_luts(golden):
+    # bsdf/albedo_tables.cpp:173-186.  The column cosThetaO = 0 at eta = 1 is a singular limit
+    # (grazing VNDF sampling at alpha = 0.001) where both sides amplify rounding differences; no
+    # supported material reads it (IOR is clamped to [1/3, 3] and eta = 1 means no interface).
+    _, L = O.compute_luts()
+    np.testing.assert_allclose(L["ggx_dir"], golden["lut_ggx_dir"].ravel(), atol=5e-6)
+    np.testing.assert_allclose(L["ggx_avg"], golden["lut_ggx_avg"].ravel(), atol=5e-6)
+    eta_dir = L["diel_eta_dir"].reshape(16, 16, 16).copy()
+    ref_eta_dir = golden["lut_diel_eta_dir"].copy()
+    eta_dir[:, 15, 0] = ref_eta_dir[:, 15, 0] = 0
+    np.testing.assert_allclose(eta_dir, ref_eta_dir, atol=2e-5)
+    rcp_dir = L["diel_rcpeta_dir"].reshape(16, 16, 16).copy()
+    ref_rcp_dir = golden["lut_diel_rcpeta_dir"].copy()
+    np.testing.assert_allclose(rcp_dir[:, 0, 0], ref_rcp_dir[:, 0, 0], atol=2e-3)
+    rcp_dir[:, 0, 0] = ref_rcp_dir[:, 0, 0] = 0
+    np.testing.assert_allclose(rcp_dir, ref_rcp_dir, atol=2e-5)
+    np.testing.assert_allclose(L["diel_rcpeta_avg"], golden["lut_diel_rcpeta_avg"].ravel(), atol=2e-5)
+    # the eta average integrates the singular column with weight ~0: still within 1e-3
+    np.testing.assert_allclose(L["diel_eta_avg"], golden["lut_diel_eta_avg"].ravel(), atol=1e-3)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_bsdf_sample_eval(golden, kind):
+    # bsdf/diffuse.h:55-68, rough_conductor.h:132-202, rough_dielectric.h:296-526 wrapped in
+    # MultiBsdfSimd (multi_bsdf.h:166-210), evaluated on the same frames, directions and samples
+    flat, _ = O.compute_luts()
+    p, geo, s = golden["bsdf_params"], golden["bsdf_geo"], golden["bsdf_s"]
+    so, st, ev = O.bsdf(flat, p, geo, s)
+    m = p[:, 0] == kind
+    ref_s, ref_t, ref_e = golden["bsdf_sample"][m], golden["bsdf_sample_type"][m], golden["bsdf_eval"][m]
+    so, st, ev = so[m], st[m], ev[m]
+    valid_ref = np.abs(ref_s[:, :3]).max(1) > 0
+    valid = np.abs(so[:, :3]).max(1) > 0
+    assert np.array_equal(valid, valid_ref)
+    assert np.array_equal(st[valid], ref_t[valid])
+    np.testing.assert_allclose(so[valid, :3], ref_s[valid, :3], rtol=2e-4, atol=1e-6)
+    np.testing.assert_allclose(so[valid, 3:6], ref_s[valid, 3:6], atol=2e-5)
+    rough0 = p[m][valid, 7] == 0
+    np.testing.assert_allclose(so[valid][~rough0, 6], ref_s[valid][~rough0, 6], rtol=1e-3)
+    np.testing.assert_allclose(so[valid][rough0, 6], ref_s[valid][rough0, 6], rtol=1e-2)
+    np.testing.assert_allclose(ev[:, :3], ref_e[:, :3], rtol=5e-4, atol=1e-6)
+    erough0 = p[m][:, 7] == 0
+    np.testing.assert_allclose(ev[~erough0, 3], ref_e[~erough0, 3], rtol=1e-3, atol=1e-6)
+    np.testing.assert_allclose(ev[erough0, 3], ref_e[erough0, 3], rtol=1e-2, atol=1e-6)
+
+
+@pytest.mark.parametrize("suffix", ["", "_n"])
+def test_post_intersect_and_frame(golden, suffix):
+    # geometry/triangle_mesh.cpp:152-257, core/ray_simd.h:63-68, core/shading_context.h:144-191
+    normals = golden["pi_normals_n"] if suffix else None
+    ctx, fr = O.post_intersect(golden["pi_verts" + suffix], normals, golden["pi_indices" + suffix],
+                               golden["pi_rays" + suffix], golden["pi_prim" + suffix])
+    ref = golden["pi_ctx" + suffix]
+    np.testing.assert_allclose(ctx[:, :3], ref[:, :3], atol=1e-5)
+    np.testing.assert_allclose(ctx[:, 3], ref[:, 3], rtol=1e-6)
+    np.testing.assert_allclose(ctx[:, 4:], ref[:, 4:], atol=1e-5)
+    np.testing.assert_allclose(fr, golden["pi_frame" + suffix], atol=1e-5)
