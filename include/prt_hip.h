@@ -1,0 +1,188 @@
+/*
+ * prt_hip.h -- C ABI of the MI355X (gfx950) wavefront path tracer that stands in for
+ * ProtoRay's stream renderer (RendererFactoryStream -> PathRendererStream behind the
+ * Device / Renderer interfaces).
+ *
+ * Conventions: every call returns PRT_OK (0) or a negative PRT_ERR_* code; on error
+ * prt_last_error() returns a message (thread-local).  No C++ exception crosses this ABI (the
+ * reference throws std::invalid_argument / std::logic_error / std::runtime_error, e.g.
+ * render/path_renderer_stream.h:109-112,153-154; those conditions map to PRT_ERR_INVALID).
+ * All pointers are host pointers unless the name says "device".  A context owns one HIP
+ * device and one stream; calls on one context must not overlap.
+ */
+#ifndef PRT_HIP_H
+#define PRT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRT_OK 0
+#define PRT_ERR_INVALID -1   /* bad argument / state (reference: std::invalid_argument, std::logic_error) */
+#define PRT_ERR_HIP -2       /* HIP runtime failure */
+#define PRT_ERR_NOMEM -3     /* device allocation failed */
+#define PRT_ERR_STATE -4     /* call order violated (e.g. render before a scene) */
+
+typedef struct prt_context prt_context;
+
+/* Material record (32 bytes).  Uniform StandardMaterial parameter sets of the stream path
+ * (material/standard.cpp:21-183 ctor, :409-573 SIMD lobes), each a MultiBsdf with one lobe:
+ *   kind 0 diffuse    : baseColor = (r,g,b), specularWeight 0          -> DiffuseBrdf
+ *   kind 1 conductor  : metalness 1, baseColor = (r,g,b), specularColor = (er,eg,eb),
+ *                       specularRoughness = roughness                  -> RoughConductorBrdf
+ *                                                                         <FresnelConductorArtistic>
+ *   kind 2 dielectric : transmissionWeight 1, specularIOR = ior,
+ *                       specularRoughness = roughness                  -> RoughDielectricBsdf
+ * Material ID 0 is reserved for the environment (render/scene.cpp:204-207). */
+typedef struct prt_material {
+  int32_t kind;
+  float r, g, b;
+  float er, eg, eb;
+  float roughness;
+  float ior;
+  int32_t pad[2];
+} prt_material;
+
+/* Environment light record (32 bytes): light/ambient_light.h (kind 0, Ke = rgb),
+ * light/distant_light.h (kind 1, Ke = rgb, direction, half angle in radians),
+ * light/image_light.h (kind 2, the lat-long image given to prt_scene_set_lights, scaled by rgb).
+ * A sun/sky light (light/sun_sky_light.cpp) is passed as its baked pair: a distant sun plus
+ * an image sky, in that order. */
+typedef struct prt_light {
+  int32_t kind;
+  float r, g, b;
+  float dx, dy, dz;
+  float angle;
+} prt_light;
+
+/* Camera (camera/pinhole_camera.h:72-117, camera/thin_lens_camera.h:155-161): position,
+ * basis U, V, N (the view looks along -N), vertical fov in radians, near clip, lens radius
+ * (0 selects the pinhole camera, render/device_cpu.cpp:117-128) and focal distance. */
+typedef struct prt_camera {
+  float pos[3];
+  float U[3], V[3], N[3];
+  float fov, nearClip, lensRadius, focalDistance;
+} prt_camera;
+
+/* Renderer properties (render/path_renderer_stream.h:114-116, render/integrator.h:58-62). */
+typedef struct prt_renderer_params {
+  int32_t maxDepth;   /* default 100 */
+  int32_t rrDepth;    /* default 8 */
+  float minWeight;    /* default 1e-4 */
+  float maxRadiance;  /* default FLT_MAX (no clamping) */
+} prt_renderer_params;
+
+/* Per-call statistics (render/path_renderer_stream.h:167-178): rays traced (primary,
+ * extension and shadow), render time, Mray/s, samples per pixel rendered. */
+typedef struct prt_stats {
+  double rays;
+  double ms;
+  double mray;
+  int32_t spp;
+} prt_stats;
+
+/* AOV selection bits for prt_frame_init (render/device_cpu.cpp:178-232 aov* props). */
+#define PRT_AOV_ALBEDO   (1u << 0)  /* aovAlbedo  : albedo at the first non-delta hit */
+#define PRT_AOV_ALBEDO1  (1u << 1)  /* aovAlbedo1 : first-hit albedo */
+#define PRT_AOV_NORMAL   (1u << 2)  /* aovNormal  */
+#define PRT_AOV_NORMAL1  (1u << 3)  /* aovNormal1 */
+#define PRT_AOV_DEPTH    (1u << 4)  /* aovDepth   : linear depth of the first hit */
+#define PRT_AOV_VAR      (1u << 5)  /* aovHdrVar  : color sample variance (sum of squares) */
+
+/* Buffer selectors for prt_blit. */
+#define PRT_BUF_COLOR   0
+#define PRT_BUF_ALBEDO  1
+#define PRT_BUF_ALBEDO1 2
+#define PRT_BUF_NORMAL  3
+#define PRT_BUF_NORMAL1 4
+#define PRT_BUF_DEPTH   5
+
+/* --- context ------------------------------------------------------------------------- */
+/* Replaces DeviceCpu construction (render/device_cpu.cpp:20-23, incl. the albedo LUT
+ * precompute, bsdf/albedo_tables.cpp:173-186). */
+int prt_context_create(int device_ordinal, prt_context** out);
+int prt_context_destroy(prt_context* ctx);
+const char* prt_last_error(void);
+/* Waits for all work on the context's device (hipDeviceSynchronize); brackets timed regions. */
+int prt_device_synchronize(prt_context* ctx);
+/* Device name, CU count and HBM size (Device::getInfo, render/device.h:60). */
+int prt_get_info(prt_context* ctx, char* buf, int len);
+
+/* --- scene (Device::initScene / Scene ctor, render/scene.cpp:65-134) ------------------ */
+/* positions: nverts x 3; normals: nverts x 3 or NULL; indices: ntris x 3;
+ * tri_material: ntris material IDs (>= 1).  Uploads to HBM. */
+int prt_scene_set_geometry(prt_context* ctx, int32_t nverts, const float* positions, const float* normals,
+                           int32_t ntris, const int32_t* indices, const int32_t* tri_material);
+int prt_scene_set_materials(prt_context* ctx, int32_t n, const prt_material* materials);
+/* Device::initLights (render/scene.cpp:365-450): light list in reference order, optional
+ * lat-long environment image env_rgb (env_h x env_w x 3, texture row order: row 0 = nadir,
+ * image/image_texture.h:396), lut_max as the image light's lutSize prop. */
+int prt_scene_set_lights(prt_context* ctx, int32_t n, const prt_light* lights, int32_t env_w, int32_t env_h,
+                         const float* env_rgb, int32_t lut_max);
+/* BVH build on the device (replaces the Embree build, accel/embree_intersector.cpp:11-90).
+ * build_ms receives the build time (Embree's buildMs stat). */
+int prt_bvh_build(prt_context* ctx, double* build_ms);
+
+/* --- renderer (Device::initRenderer -> RendererFactoryStream::make "pathStream",
+ *     render/renderer_factory_stream.cpp:17-76) --------------------------------------- */
+int prt_renderer_init(prt_context* ctx, const prt_renderer_params* params);
+/* Renderer::initSampler (render/path_renderer_stream.h:144-148): seed, sampleIndex = 0. */
+int prt_sampler_init(prt_context* ctx, uint32_t seed);
+/* Device::initCamera (render/device_cpu.cpp:130-134). */
+int prt_set_camera(prt_context* ctx, const prt_camera* camera);
+/* Device::initFrame / clearFrame (render/device_cpu.cpp:178-232, FrameBuffer::clear).
+ * Width must be a multiple of 32 and height of 8, each <= 4096
+ * (render/path_renderer_stream.h:109-112). */
+int prt_frame_init(prt_context* ctx, int32_t width, int32_t height, uint32_t aov_mask);
+int prt_frame_clear(prt_context* ctx);
+/* Renderer::render (render/path_renderer_stream.h:150-187) called n_spp times: each pass
+ * renders one sample per pixel with the current sampleIndex, accumulates it in sample
+ * order, and increments sampleIndex.  Synchronous. */
+int prt_render(prt_context* ctx, int32_t n_spp, prt_stats* stats);
+/* Device::blitFrame (render/device_cpu.cpp:316-376, AccumBuffer3f::blit accum_buffer.h:607-649):
+ * mean = sum * rcpSafe(weight), interleaved RGB (depth: one float).  var may be NULL. */
+int prt_blit(prt_context* ctx, int32_t buffer, float* dst, float* var);
+/* Raw FP32 accumulator sums (planar, 3 x W*H) and weights (W*H) of the color buffer. */
+int prt_get_accum(prt_context* ctx, float* sum, float* weight);
+/* Device::getFrameError (AccumBuffer3f::getError, accum_buffer.h:752-805) against the
+ * checkpoint copy taken at the last power-of-two sample count. */
+int prt_frame_error(prt_context* ctx, float* error);
+
+/* --- queries (Renderer::queryRay / queryOcclusionRay, render/renderer.h:47-128) ------- */
+/* rays: n x [ox oy oz dx dy dz near far]; closest hit per ray: prim (-1 = miss), t, u, v. */
+int prt_query_rays(prt_context* ctx, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v);
+/* any hit in [near, far]: occluded[i] = 1 / 0. */
+int prt_query_occlusion(prt_context* ctx, int32_t n, const float* rays, int32_t* occluded);
+/* Same two queries on device buffers already in HBM, on the context stream (asynchronous). */
+int prt_query_rays_device(prt_context* ctx, int32_t n, const float* rays_device, int32_t* prim_device,
+                          float* t_device, float* u_device, float* v_device);
+
+/* --- per-sample debug outputs of the last pass (parity checks) ------------------------- */
+/* When enabled, each pass also stores per pixel: the clamped sample radiance (planar
+ * 3 x W*H), a hash of the path's hit / lobe sequence and the first-hit primitive ID. */
+int prt_debug_enable(prt_context* ctx, int32_t enable);
+int prt_debug_get(prt_context* ctx, float* sample_l, uint32_t* path_hash, int32_t* first_prim);
+/* Sobol-Burley sampler on the device: seed_pid_idx n x 3, outputs n x ndim (1D), n x ndim x 2,
+ * n x ndim x 3 (sampling/sobol_burley_sampler_simd.h:36-85). */
+int prt_debug_sampler(prt_context* ctx, int32_t n, const uint32_t* seed_pid_idx, int32_t ndim, float* out1,
+                      float* out2, float* out3);
+/* The energy-compensation LUTs computed at context creation (PRT_LUT_FLOATS = 9760 floats:
+ * GGX dir 32x32, GGX avg 32, dielectric eta dir 16^3, eta avg 16^2, rcp-eta dir 16^3,
+ * rcp-eta avg 16^2; bsdf/albedo_tables.h:14-157). */
+int prt_debug_luts(prt_context* ctx, float* out);
+/* BVH statistics: inner node count, triangle count, max depth, scene triangle count. */
+int prt_debug_bvh_stats(prt_context* ctx, int64_t* stats4);
+/* Device-side kernel timing of the last prt_render call, per kernel class
+ * (generate, intersect, shade, occluded) in ms, from HIP events on the context stream, and the
+ * kernels' own counters (the analogue of RayStats::nodeCount / primCount, core/ray_stats.h:10-45):
+ * counters6 = closest-hit rays, inner-node visits, triangle tests, then the same three for the
+ * shadow (any-hit) rays; launches2 = number of intersect and occluded launches. */
+int prt_debug_kernel_ms(prt_context* ctx, double* ms4, double* counters6, double* launches2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRT_HIP_H */

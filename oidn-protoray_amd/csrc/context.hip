@@ -1,0 +1,1010 @@
+// Host driver of the MI355X stream path tracer: the C ABI declared in include/prt_hip.h.
+//
+// It plays the roles of DeviceCpu (render/device_cpu.cpp), Scene (render/scene.cpp) and
+// PathRendererStream::render (render/path_renderer_stream.h:150-187) for one HIP device:
+// scene arrays are uploaded to HBM once, the BVH is built on the device, and each render pass
+// launches the wavefront kernels of render.hip on the context's stream.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/prt_hip.h"
+#include "internal.h"
+
+using namespace prt;
+
+namespace {
+
+thread_local std::string gLastError;
+
+int fail(int code, const std::string& msg)
+{
+  gLastError = msg;
+  return code;
+}
+
+#define HIP_TRY(x)                                                                            \
+  do {                                                                                        \
+    hipError_t e_ = (x);                                                                      \
+    if (e_ != hipSuccess) return fail(e_ == hipErrorOutOfMemory ? PRT_ERR_NOMEM : PRT_ERR_HIP, \
+                                      std::string(#x ": ") + hipGetErrorString(e_));          \
+  } while (0)
+
+template <class T>
+struct DevBuf
+{
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count)
+  {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, sizeof(T) * count);
+  }
+  void release()
+  {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+// ----------------------------------------------------------------------------------------
+// Host math shared by the LUT precompute and the per-material constants (IEEE FP32)
+// ----------------------------------------------------------------------------------------
+struct H3 { float x, y, z; };
+inline H3 hmk(float a, float b, float c) { return H3{a, b, c}; }
+inline float hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline H3 hcross(H3 a, H3 b) { return hmk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline H3 hscale(H3 a, float s) { return hmk(a.x * s, a.y * s, a.z * s); }
+inline H3 hadd(H3 a, H3 b) { return hmk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline H3 hsub(H3 a, H3 b) { return hmk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline H3 hnorm(H3 a) { return hscale(a, 1.0f / std::sqrt(hdot(a, a))); }
+inline float hsqr(float a) { return a * a; }
+inline float hclamp(float x, float lo, float hi) { return std::min(std::max(x, lo), hi); }
+inline float hlerp(float a, float b, float s) { return a + s * (b - a); }
+
+// GGX with isotropic alpha (bsdf/ggx_distribution.h), host copy for the LUT quadrature
+struct HostGGX
+{
+  float a;
+  float D(H3 w) const
+  {
+    float c2 = hsqr(w.z);
+    float e = (hsqr(w.x / a) + hsqr(w.y / a)) / c2;
+    return 1.0f / (3.14159265358979323846f * a * a * hsqr(c2 * (1.f + e)));
+  }
+  float lambda(H3 w) const
+  {
+    float c2 = hsqr(w.z);
+    return 0.5f * (-1.f + std::sqrt(1.f + (hsqr(w.x * a) + hsqr(w.y * a)) / c2));
+  }
+  float G1(H3 w) const { return 1.0f / (1.f + lambda(w)); }
+  float G2(H3 wo, H3 wi) const { return 1.0f / (1.f + lambda(wo) + lambda(wi)); }
+  H3 sampleVisible(H3 wo, float sx, float sy) const
+  {
+    const float pi = 3.14159265358979323846f;
+    H3 V = hnorm(hmk(a * wo.x, a * wo.y, wo.z));
+    H3 T1 = V.z < 0.9999f ? hnorm(hcross(V, hmk(0, 0, 1))) : hmk(1, 0, 0);
+    H3 T2 = hcross(T1, V);
+    float aa = 1.f / (1.f + V.z);
+    float r = std::sqrt(sx);
+    float phi = (sy < aa) ? sy / aa * pi : pi + (sy - aa) / (1.f - aa) * pi;
+    float P1 = r * std::cos(phi), P2 = r * std::sin(phi) * ((sy < aa) ? 1.f : V.z);
+    H3 wh = hadd(hadd(hscale(T1, P1), hscale(T2, P2)), hscale(V, std::sqrt(std::max(0.f, 1.f - P1 * P1 - P2 * P2))));
+    return hnorm(hmk(a * wh.x, a * wh.y, std::max(0.f, wh.z)));
+  }
+};
+
+inline float hostFresnelEx(float cosI, float& cosT, float eta)  // bsdf/optics.h:122-139
+{
+  cosT = 0.f;
+  cosI = std::fabs(cosI);
+  float k = 1.f - eta * eta * (1.f - cosI * cosI);
+  if (!(k >= 0.f)) return 1.f;
+  cosT = std::sqrt(k);
+  float Rper = (eta * cosI - cosT) / (eta * cosI + cosT), Rpar = (cosI - eta * cosT) / (cosI + eta * cosT);
+  return 0.5f * (Rpar * Rpar + Rper * Rper);
+}
+
+// bsdf/albedo.h:16-120 single-scattering albedo estimators and albedo_tables.cpp:23-107 layout
+float lutIntegrate(float cosO, const HostGGX& m, float eta, bool dielectric)
+{
+  const int n = 32;
+  float sum = 0.f;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+    {
+      float sx = std::min((float(i) + 0.5f) / float(n), 1.f - 1e-6f), sy = std::min((float(j) + 0.5f) / float(n), 1.f - 1e-6f);
+      float c = std::max(cosO, 1e-6f);
+      H3 wo = hmk(std::sqrt(std::max(1.f - c * c, 0.f)), 0.f, c);
+      H3 wh = m.sampleVisible(wo, sx, sy);
+      float cosOH = hdot(wo, wh);
+      H3 wi = hsub(hscale(wh, 2.f * cosOH), wo);
+      if (!dielectric)
+      {
+        if (wi.z > 0.f) sum += m.G2(wo, wi) * (1.0f / m.G1(wo));
+        continue;
+      }
+      float cosTH;
+      float F = hostFresnelEx(cosOH, cosTH, eta);
+      float w = 0.f;
+      if (wi.z > 0.f) w += F * (m.G2(wo, wi) * (1.0f / m.G1(wo)));
+      H3 wt = hsub(hscale(hsub(hscale(wh, cosOH), wo), eta), hscale(wh, cosTH));
+      if (wt.z < 0.f) w += (1.f - F) * (m.G2(wo, wt) * (1.0f / m.G1(wo)));
+      sum += w;
+    }
+  return std::min(sum / float(n * n), 1.f);
+}
+float lutInterp1(float x, const float* f, int size)
+{
+  float xc = hclamp(x, 0.f, float(size - 1));
+  float s = xc - std::floor(xc);
+  int x0 = std::min(int(xc), size - 1), x1 = std::min(x0 + 1, size - 1);
+  return hlerp(f[x0], f[x1], s);
+}
+float lutAvg(const float* f, int size)
+{
+  const int n = 1025;
+  float sum = 0.f;
+  for (int i = 0; i < n; ++i)
+  {
+    float c = float(i) / float(n - 1);
+    sum += lutInterp1(c * float(size - 1), f, size) * c * ((i == 0 || i == n - 1) ? 0.5f : 1.f);
+  }
+  return std::min(2.f * (sum / float(n - 1)), 1.f);
+}
+void computeLuts(std::vector<float>& L)
+{
+  L.assign(PRT_LUT_FLOATS, 0.f);
+  for (int j = 0; j < 32; ++j)
+  {
+    HostGGX m{std::max(hsqr(float(j) / 31.f), 0.001f)};
+    float* d = L.data() + PRT_LUT_GGX_DIR + j * 32;
+    for (int i = 0; i < 32; ++i) d[i] = lutIntegrate(float(i) / 31.f, m, 1.f, false);
+    L[PRT_LUT_GGX_AVG + j] = lutAvg(d, 32);
+  }
+  for (int table = 0; table < 2; ++table)
+  {
+    float minEta = table == 0 ? 1.0f / 3.0f : 1.f, maxEta = table == 0 ? 1.f : 3.f;
+    float* dir = L.data() + (table == 0 ? PRT_LUT_DIEL_ETA_DIR : PRT_LUT_DIEL_RCPETA_DIR);
+    float* avg = L.data() + (table == 0 ? PRT_LUT_DIEL_ETA_AVG : PRT_LUT_DIEL_RCPETA_AVG);
+    for (int k = 0; k < 16; ++k)
+    {
+      HostGGX m{std::max(hsqr(float(k) / 15.f), 0.001f)};
+      for (int j = 0; j < 16; ++j)
+      {
+        float eta = hlerp(minEta, maxEta, float(j) / 15.f);
+        float* d = dir + (k * 16 + j) * 16;
+        for (int i = 0; i < 16; ++i) d[i] = lutIntegrate(float(i) / 15.f, m, eta, true);
+        avg[k * 16 + j] = lutAvg(d, 16);
+      }
+    }
+  }
+}
+float lutInterp2(float px, float py, const float* f, int n)
+{
+  float xc = hclamp(px, 0.f, float(n - 1)), yc = hclamp(py, 0.f, float(n - 1));
+  float sx = xc - std::floor(xc), sy = yc - std::floor(yc);
+  int x0 = std::min(int(xc), n - 1), x1 = std::min(x0 + 1, n - 1), y0 = std::min(int(yc), n - 1), y1 = std::min(y0 + 1, n - 1);
+  return hlerp(hlerp(f[x0 + y0 * n], f[x1 + y0 * n], sx), hlerp(f[x0 + y1 * n], f[x1 + y1 * n], sx), sy);
+}
+float dielAvg(const std::vector<float>& L, float eta, float roughness)  // albedo_tables.h:129-154
+{
+  if (eta <= 1.f)
+  {
+    float e = (eta - 1.0f / 3.0f) / (1.0f - 1.0f / 3.0f);
+    return lutInterp2(e * 15.f, roughness * 15.f, L.data() + PRT_LUT_DIEL_ETA_AVG, 16);
+  }
+  float e = (eta - 1.f) / 2.f;
+  return lutInterp2(e * 15.f, roughness * 15.f, L.data() + PRT_LUT_DIEL_RCPETA_AVG, 16);
+}
+float fresnelAvg(float eta)  // bsdf/optics.h:144-175
+{
+  float re = 1.0f / eta;
+  if (re >= 1.f) return (re - 1.f) / (4.08567f + 1.00071f * re);
+  float re2 = re * re, re3 = re2 * re;
+  return 0.997118f + 0.1014f * re - 0.965241f * re2 - 0.130607f * re3;
+}
+
+void hostMakeFrame(H3 N, H3& U, H3& V)  // math/basis3.h:173-180
+{
+  H3 U0 = hmk(0.f, N.z, -N.y), U1 = hmk(-N.z, 0.f, N.x);
+  U = hnorm(hdot(U0, U0) > hdot(U1, U1) ? U0 : U1);
+  V = hnorm(hcross(N, U));
+}
+
+bool isPowerOfTwo(int x) { return (x & (x - 1)) == 0; }
+
+}  // namespace
+
+struct prt_context
+{
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t props{};
+  int cus = 0;
+  // scene
+  int nverts = 0, ntris = 0, maxMaterialId = 0;
+  DevBuf<float> pos, nrm;
+  DevBuf<int> idx, triMat;
+  std::vector<float> hostPos;
+  std::vector<int> hostIdx;
+  std::vector<prt_material> materials;
+  DevBuf<prt_material_constants> mats;
+  std::vector<float> luts;
+  DevBuf<float> lut;
+  std::vector<prt_light_dev> lights;
+  std::vector<float> lightCdf;
+  DevBuf<float4> envTex;
+  DevBuf<float> envX, envY;
+  int envW = 0, envH = 0, envLW = 0, envLH = 0;
+  float sceneRadius = 0.f;
+  bool lightsReady = false;
+  BvhData bvh;
+  bool bvhReady = false;
+  // renderer
+  prt_renderer_params params{100, 8, 1e-4f, 3.402823466e38f};
+  bool rendererReady = false;
+  uint32_t seed = 0, sampleIndex = 0;
+  prt_camera cam{};
+  bool camReady = false;
+  // frame
+  int W = 0, H = 0, npix = 0;
+  uint32_t aovMask = 0;
+  int count = 0;
+  DevBuf<float> color, weight, color2, colorC, weightC;
+  DevBuf<float> albedo, albedoW, albedo1, albedo1W, normal, normalW, normal1, normal1W, depth, depthW;
+  DevBuf<float> dbgL;
+  DevBuf<uint32_t> dbgHash;
+  DevBuf<int> dbgFirst;
+  bool debug = false;
+  // path state
+  DevBuf<float4> rayO, rayD, hit, TP, Lr, shO, shD, dirL;
+  DevBuf<int> flags, q0, q1, shadowQ, counters, spill;
+  DevBuf<uint32_t> hash;
+  DevBuf<unsigned long long> stats;
+  LaunchCfg launch{};
+  int spillStride = 0;
+  // timing
+  hipEvent_t ev[8] = {};
+  double kernelMs[4] = {0, 0, 0, 0};
+  double lastStats[6] = {0, 0, 0, 0, 0, 0};
+  double launches[2] = {0, 0};
+};
+
+#define CTX_CHECK(c)                                                 \
+  do {                                                               \
+    if (!(c)) return fail(PRT_ERR_INVALID, "null context");          \
+    hipError_t e_ = hipSetDevice((c)->device);                       \
+    if (e_ != hipSuccess) return fail(PRT_ERR_HIP, hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" {
+
+const char* prt_last_error(void) { return gLastError.c_str(); }
+
+int prt_context_create(int device_ordinal, prt_context** out)
+{
+  if (!out) return fail(PRT_ERR_INVALID, "out is null");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device_ordinal < 0 || device_ordinal >= n) return fail(PRT_ERR_INVALID, "invalid device ordinal");
+  HIP_TRY(hipSetDevice(device_ordinal));
+  prt_context* c = new prt_context();
+  c->device = device_ordinal;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(PRT_ERR_HIP, hipGetErrorString(e)); }
+  hipGetDeviceProperties(&c->props, device_ordinal);
+  c->cus = c->props.multiProcessorCount;
+  c->launch.gridTrace = c->cus * 4;
+  c->launch.gridShade = c->cus * 8;
+  c->spillStride = c->launch.gridTrace * kTraceBlock;
+  for (auto& ev : c->ev) hipEventCreate(&ev);
+  // albedo LUT precompute, as DeviceCpu's constructor does (render/device_cpu.cpp:20-23)
+  computeLuts(c->luts);
+  if (c->lut.alloc(PRT_LUT_FLOATS) != hipSuccess ||
+      hipMemcpy(c->lut.p, c->luts.data(), sizeof(float) * PRT_LUT_FLOATS, hipMemcpyHostToDevice) != hipSuccess ||
+      c->spill.alloc(size_t(PRT_MAX_STACK - PRT_LDS_STACK) * c->spillStride) != hipSuccess ||
+      c->stats.alloc(8) != hipSuccess)
+  {
+    prt_context_destroy(c);
+    return fail(PRT_ERR_NOMEM, "device allocation failed");
+  }
+  *out = c;
+  return PRT_OK;
+}
+
+int prt_context_destroy(prt_context* c)
+{
+  if (!c) return PRT_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  freeBvh(c->bvh);
+  for (auto& ev : c->ev) if (ev) hipEventDestroy(ev);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PRT_OK;
+}
+
+int prt_device_synchronize(prt_context* c)
+{
+  CTX_CHECK(c);
+  HIP_TRY(hipDeviceSynchronize());
+  return PRT_OK;
+}
+
+int prt_get_info(prt_context* c, char* buf, int len)
+{
+  CTX_CHECK(c);
+  std::string s = std::string(c->props.name) + " (" + c->props.gcnArchName + "), " + std::to_string(c->cus) + " CUs, " +
+                  std::to_string(c->props.totalGlobalMem >> 20) + " MiB";
+  if (buf && len > 0)
+  {
+    std::strncpy(buf, s.c_str(), size_t(len) - 1);
+    buf[len - 1] = 0;
+  }
+  return PRT_OK;
+}
+
+int prt_scene_set_geometry(prt_context* c, int32_t nverts, const float* positions, const float* normals, int32_t ntris,
+                           const int32_t* indices, const int32_t* tri_material)
+{
+  CTX_CHECK(c);
+  if (nverts < 0 || ntris < 0 || (nverts > 0 && !positions) || (ntris > 0 && (!indices || !tri_material)))
+    return fail(PRT_ERR_INVALID, "invalid geometry arrays");
+  if (ntris >= (1 << 28)) return fail(PRT_ERR_INVALID, "too many triangles (limit 2^28)");
+  for (int64_t i = 0; i < int64_t(ntris) * 3; ++i)
+    if (indices[i] < 0 || indices[i] >= nverts) return fail(PRT_ERR_INVALID, "triangle index out of range");
+  int maxMat = 0;
+  for (int i = 0; i < ntris; ++i)
+  {
+    if (tri_material[i] < 1) return fail(PRT_ERR_INVALID, "material ID 0 is reserved for the environment");
+    maxMat = std::max(maxMat, tri_material[i]);
+  }
+  c->maxMaterialId = maxMat;
+  c->nverts = nverts;
+  c->ntris = ntris;
+  HIP_TRY(c->pos.alloc(size_t(nverts) * 3));
+  HIP_TRY(c->idx.alloc(size_t(ntris) * 3));
+  HIP_TRY(c->triMat.alloc(size_t(ntris)));
+  if (nverts) HIP_TRY(hipMemcpy(c->pos.p, positions, sizeof(float) * 3 * nverts, hipMemcpyHostToDevice));
+  if (ntris)
+  {
+    HIP_TRY(hipMemcpy(c->idx.p, indices, sizeof(int) * 3 * size_t(ntris), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->triMat.p, tri_material, sizeof(int) * size_t(ntris), hipMemcpyHostToDevice));
+  }
+  if (normals)
+  {
+    HIP_TRY(c->nrm.alloc(size_t(nverts) * 3));
+    HIP_TRY(hipMemcpy(c->nrm.p, normals, sizeof(float) * 3 * nverts, hipMemcpyHostToDevice));
+  }
+  else c->nrm.release();
+  // scene bounds over the referenced vertices (render/scene.cpp:81, :367)
+  float lo[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f}, hi[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+  for (int64_t i = 0; i < int64_t(ntris) * 3; ++i)
+    for (int k = 0; k < 3; ++k)
+    {
+      float v = positions[3 * int64_t(indices[i]) + k];
+      lo[k] = std::min(lo[k], v);
+      hi[k] = std::max(hi[k], v);
+    }
+  H3 ext = ntris ? hmk(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) : hmk(0, 0, 0);
+  c->sceneRadius = std::sqrt(hdot(ext, ext)) * 0.5f;
+  c->bvhReady = false;
+  c->lightsReady = false;
+  return PRT_OK;
+}
+
+int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
+{
+  CTX_CHECK(c);
+  if (n < 1 || !m) return fail(PRT_ERR_INVALID, "need at least the reserved material 0");
+  c->materials.assign(m, m + n);
+  std::vector<prt_material_constants> k(n);
+  std::vector<float> iors;
+  for (int i = 0; i < n; ++i)
+  {
+    prt_material_constants& q = k[i];
+    std::memset(&q, 0, sizeof(q));
+    const prt_material& s = m[i];
+    if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_DIELECTRIC) return fail(PRT_ERR_INVALID, "unknown material kind");
+    q.kind = s.kind;
+    q.roughness = hclamp(s.roughness, 0.f, 1.f);
+    q.alpha = std::max(hsqr(q.roughness), 0.001f);  // bsdf/microfacet.h:10-24
+    q.importance = 1.f;
+    float rgb[3] = {hclamp(s.r, 0.f, 1.f), hclamp(s.g, 0.f, 1.f), hclamp(s.b, 0.f, 1.f)};
+    if (s.kind == PRT_KIND_DIFFUSE)
+    {
+      for (int j = 0; j < 3; ++j) q.c0[j] = rgb[j];
+      q.importance = std::max(std::max(rgb[0], rgb[1]), rgb[2]);  // bsdf/diffuse.h:72-75
+    }
+    else if (s.kind == PRT_KIND_CONDUCTOR)
+    {
+      float edge[3] = {hclamp(s.er, 0.f, 1.f), hclamp(s.eg, 0.f, 1.f), hclamp(s.eb, 0.f, 1.f)};
+      q.Eavg = lutInterp1(q.roughness * 31.f, c->luts.data() + PRT_LUT_GGX_AVG, 32);
+      for (int j = 0; j < 3; ++j)
+      {
+        // bsdf/optics.h:214-240 (Gulbrandsen + Kulla-Conty average)
+        float r = std::min(rgb[j], 0.99f), g = edge[j];
+        float nMin = (1.f - r) / (1.f + r), nMax = (1.f + std::sqrt(r)) / (1.f - std::sqrt(r));
+        float nn = g * nMin + (1.f - g) * nMax;
+        float k2 = (hsqr(nn + 1.f) * r - hsqr(nn - 1.f)) / (1.f - r);
+        q.c0[j] = nn;
+        q.c1[j] = std::sqrt(std::max(k2, 0.f));
+        float rr = rgb[j], r2 = rr * rr, r3 = r2 * rr, g2 = g * g, g3 = g2 * g;
+        float Favg = 0.087237f + 0.0230685f * g - 0.0864902f * g2 + 0.0774594f * g3 + 0.782654f * rr - 0.136432f * r2 +
+                     0.278708f * r3 + 0.19744f * g * rr + 0.0360605f * g2 * rr - 0.2586f * g * r2;
+        q.c2[j] = hsqr(Favg) * q.Eavg / (1.f - Favg * (1.f - q.Eavg));  // rough_conductor.h:127-129
+        q.rr[j] = rgb[j];
+      }
+    }
+    else
+    {
+      float ior = s.ior < 1.f ? 1.0f / s.ior : s.ior;
+      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "dielectric IOR must differ from the ambient IOR 1");
+      for (int o = 0; o < 2; ++o)
+      {
+        // material/standard.cpp:436-437: outside eta = 1/ior, inside eta = ior, clamped to [1/3, 3]
+        float eta = hclamp(o == 0 ? 1.0f / ior : ior, 1.f / 3.f, 3.f);
+        float Ee = dielAvg(c->luts, eta, q.roughness), Er = dielAvg(c->luts, 1.0f / eta, q.roughness);
+        float Fe = fresnelAvg(eta), Fr = fresnelAvg(1.0f / eta);
+        float a = (1.f - Fe) / (1.f - Er), b = (1.f - Fr) / (1.f - Ee);
+        float x = b / (a + b);
+        q.eta[o] = eta;
+        q.EavgEta[o] = Ee;
+        q.EavgRcpEta[o] = Er;
+        q.fmsRatio[o] = 1.f - x * (1.f - Fe);  // rough_dielectric.h:283-291
+      }
+      // media deduplicated by (color, IOR), ID 0 = ambient (material/material_factory.cpp:20-21,159-175)
+      int id = -1;
+      for (size_t t = 0; t < iors.size(); ++t) if (iors[t] == ior) id = int(t) + 1;
+      if (id < 0) { iors.push_back(ior); id = int(iors.size()); }
+      q.mediumInt = id;
+    }
+  }
+  HIP_TRY(c->mats.alloc(n));
+  HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * n, hipMemcpyHostToDevice));
+  return PRT_OK;
+}
+
+int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int32_t env_w, int32_t env_h, const float* env_rgb,
+                         int32_t lut_max)
+{
+  CTX_CHECK(c);
+  if (n < 1 || n > PRT_MAX_LIGHTS || !lights) return fail(PRT_ERR_INVALID, "1..8 lights required");
+  bool needImage = false;
+  for (int i = 0; i < n; ++i) needImage |= lights[i].kind == PRT_LIGHT_IMAGE;
+  if (needImage && (!env_rgb || env_w <= 0 || env_h <= 0)) return fail(PRT_ERR_INVALID, "image light without an image");
+  const float pi = 3.14159265358979323846f;
+  std::vector<float> sumL;
+  double envMean = 0.0;
+  if (env_rgb && env_w > 0 && env_h > 0)
+  {
+    // light/image_light.h:20-54: importance LUT with 2x bucketing down to lut_max
+    std::vector<float4> tex(size_t(env_w) * env_h);
+    for (size_t i = 0; i < tex.size(); ++i) tex[i] = make_float4(env_rgb[3 * i], env_rgb[3 * i + 1], env_rgb[3 * i + 2], 0.f);
+    int lw = env_w, lh = env_h, bucket = 1;
+    if (lut_max > 0)
+      while (std::max(lw, lh) > lut_max && lw % 2 == 0 && lh % 2 == 0) { lw /= 2; lh /= 2; bucket *= 2; }
+    auto lum = [&](int x, int y) {
+      const float* p = env_rgb + (size_t(y) * env_w + x) * 3;
+      return 0.212671f * p[0] + 0.715160f * p[1] + 0.072169f * p[2];
+    };
+    std::vector<float> w(size_t(lw) * lh);
+    for (int y = 0; y < lh; ++y)
+      for (int x = 0; x < lw; ++x)
+      {
+        float sum = 0.f;
+        for (int y2 = 0; y2 < bucket; ++y2)
+          for (int x2 = 0; x2 < bucket; ++x2) sum += lum(x * bucket + x2, y * bucket + y2);
+        w[size_t(y) * lw + x] = sum * std::sin((float(y) + 0.5f) / float(lh) * pi);
+      }
+    // sampling/distribution2d.cpp:16-34 and distribution1d.cpp:18-32
+    auto init1d = [](float* cdf, const float* f, int size) {
+      cdf[0] = 0.f;
+      for (int i = 1; i < size + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1];
+      float norm = (cdf[size] == 0.f) ? 0.f : 1.0f / cdf[size];
+      for (int i = 1; i < size + 1; ++i) cdf[i] *= norm;
+      cdf[size] = 1.f;
+    };
+    std::vector<float> xd(size_t(lh) * (lw + 1)), yd(lh + 1), fy(lh);
+    for (int y = 0; y < lh; ++y)
+    {
+      fy[y] = 0.f;
+      for (int x = 0; x < lw; ++x) fy[y] += w[size_t(y) * lw + x];
+      init1d(xd.data() + size_t(y) * (lw + 1), w.data() + size_t(y) * lw, lw);
+    }
+    init1d(yd.data(), fy.data(), lh);
+    float s = 0.f;  // light/image_light.h:137-148
+    for (int y = 0; y < env_h; ++y)
+      for (int x = 0; x < env_w; ++x) s += lum(x, y);
+    envMean = s / float(env_w * env_h);
+    HIP_TRY(c->envTex.alloc(tex.size()));
+    HIP_TRY(c->envX.alloc(xd.size()));
+    HIP_TRY(c->envY.alloc(yd.size()));
+    HIP_TRY(hipMemcpy(c->envTex.p, tex.data(), sizeof(float4) * tex.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->envX.p, xd.data(), sizeof(float) * xd.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->envY.p, yd.data(), sizeof(float) * yd.size(), hipMemcpyHostToDevice));
+    c->envW = env_w; c->envH = env_h; c->envLW = lw; c->envLH = lh;
+  }
+  c->lights.assign(n, prt_light_dev{});
+  std::vector<float> power(n);
+  const float R = c->sceneRadius;
+  for (int i = 0; i < n; ++i)
+  {
+    const prt_light& l = lights[i];
+    prt_light_dev& d = c->lights[i];
+    std::memset(&d, 0, sizeof(d));
+    d.kind = l.kind;
+    d.r = l.r; d.g = l.g; d.b = l.b;
+    float lumC = 0.212671f * l.r + 0.715160f * l.g + 0.072169f * l.b;
+    if (l.kind == PRT_LIGHT_AMBIENT)
+      power[i] = 4.f * hsqr(pi) * hsqr(R) * lumC;  // ambient_light.h:53-57
+    else if (l.kind == PRT_LIGHT_DISTANT)
+    {
+      H3 dir = hnorm(hmk(l.dx, l.dy, l.dz));  // distant_light.h:31-36
+      d.dx = dir.x; d.dy = dir.y; d.dz = dir.z;
+      d.cosAngle = std::cos(l.angle);
+      d.conePdf = 1.0f / (4.f * pi * hsqr(std::sin(0.5f * l.angle)));
+      H3 U, V;
+      hostMakeFrame(dir, U, V);
+      d.fu[0] = U.x; d.fu[1] = U.y; d.fu[2] = U.z;
+      d.fv[0] = V.x; d.fv[1] = V.y; d.fv[2] = V.z;
+      power[i] = 4.f * hsqr(pi) * hsqr(std::sin(0.5f * l.angle)) * hsqr(R) * lumC;  // distant_light.h:73-77
+    }
+    else if (l.kind == PRT_LIGHT_IMAGE)
+      power[i] = 4.f * hsqr(pi) * hsqr(R) * float(envMean) * lumC;
+    else
+      return fail(PRT_ERR_INVALID, "unknown light kind");
+  }
+  c->lightCdf.assign(n + 1, 0.f);
+  c->lightCdf[0] = 0.f;
+  for (int i = 1; i <= n; ++i) c->lightCdf[i] = c->lightCdf[i - 1] + power[i - 1];
+  float norm = c->lightCdf[n] == 0.f ? 0.f : 1.0f / c->lightCdf[n];
+  for (int i = 1; i <= n; ++i) c->lightCdf[i] *= norm;
+  c->lightCdf[n] = 1.f;
+  c->lightsReady = true;
+  return PRT_OK;
+}
+
+int prt_bvh_build(prt_context* c, double* build_ms)
+{
+  CTX_CHECK(c);
+  if (!c->pos.p && c->ntris > 0) return fail(PRT_ERR_STATE, "no geometry");
+  std::string err;
+  hipError_t e = buildBvh(c->pos.p, c->idx.p, c->ntris, c->bvh, c->stream, err);
+  if (e != hipSuccess) return fail(PRT_ERR_HIP, err);
+  c->bvhReady = true;
+  if (build_ms) *build_ms = c->bvh.buildMs;
+  return PRT_OK;
+}
+
+int prt_renderer_init(prt_context* c, const prt_renderer_params* p)
+{
+  CTX_CHECK(c);
+  if (p)
+  {
+    if (p->maxDepth < 0 || p->maxDepth > 1000 || p->rrDepth < 0) return fail(PRT_ERR_INVALID, "invalid depth parameters");
+    c->params = *p;
+  }
+  HIP_TRY(c->counters.alloc(size_t(2) * (c->params.maxDepth + 3)));
+  c->rendererReady = true;
+  c->sampleIndex = 0;  // PathRendererStream ctor calls initSampler(0) (:118-119)
+  c->seed = 0;
+  return PRT_OK;
+}
+
+int prt_sampler_init(prt_context* c, uint32_t seed)
+{
+  CTX_CHECK(c);
+  c->seed = seed;
+  c->sampleIndex = 0;
+  return PRT_OK;
+}
+
+int prt_set_camera(prt_context* c, const prt_camera* cam)
+{
+  CTX_CHECK(c);
+  if (!cam) return fail(PRT_ERR_INVALID, "camera is null");
+  if (!(cam->fov > 0.f && cam->fov < 3.14159265f)) return fail(PRT_ERR_INVALID, "invalid fov");
+  c->cam = *cam;
+  c->camReady = true;
+  return PRT_OK;
+}
+
+static hipError_t allocFrame(prt_context* c)
+{
+  const size_t n = size_t(c->npix);
+  hipError_t e;
+  auto A = [&](DevBuf<float>& b, size_t cnt, bool on) -> hipError_t {
+    if (!on) { b.release(); return hipSuccess; }
+    return b.alloc(cnt);
+  };
+  if ((e = A(c->color, 3 * n, true)) || (e = A(c->weight, n, true)) || (e = A(c->colorC, 3 * n, true)) ||
+      (e = A(c->weightC, n, true)) || (e = A(c->color2, 3 * n, c->aovMask & PRT_AOV_VAR)) ||
+      (e = A(c->albedo, 3 * n, c->aovMask & PRT_AOV_ALBEDO)) || (e = A(c->albedoW, n, c->aovMask & PRT_AOV_ALBEDO)) ||
+      (e = A(c->albedo1, 3 * n, c->aovMask & PRT_AOV_ALBEDO1)) || (e = A(c->albedo1W, n, c->aovMask & PRT_AOV_ALBEDO1)) ||
+      (e = A(c->normal, 3 * n, c->aovMask & PRT_AOV_NORMAL)) || (e = A(c->normalW, n, c->aovMask & PRT_AOV_NORMAL)) ||
+      (e = A(c->normal1, 3 * n, c->aovMask & PRT_AOV_NORMAL1)) || (e = A(c->normal1W, n, c->aovMask & PRT_AOV_NORMAL1)) ||
+      (e = A(c->depth, n, c->aovMask & PRT_AOV_DEPTH)) || (e = A(c->depthW, n, c->aovMask & PRT_AOV_DEPTH)))
+    return e;
+  if ((e = c->rayO.alloc(n)) || (e = c->rayD.alloc(n)) || (e = c->hit.alloc(n)) || (e = c->TP.alloc(n)) || (e = c->Lr.alloc(n)) ||
+      (e = c->shO.alloc(n)) || (e = c->shD.alloc(n)) || (e = c->dirL.alloc(n)) || (e = c->flags.alloc(n)) ||
+      (e = c->q0.alloc(n)) || (e = c->q1.alloc(n)) || (e = c->shadowQ.alloc(n)))
+    return e;
+  return hipSuccess;
+}
+
+int prt_frame_clear(prt_context* c)
+{
+  CTX_CHECK(c);
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  const size_t n = size_t(c->npix);
+  DevBuf<float>* bufs[] = {&c->color, &c->weight, &c->color2, &c->colorC, &c->weightC, &c->albedo, &c->albedoW, &c->albedo1,
+                           &c->albedo1W, &c->normal, &c->normalW, &c->normal1, &c->normal1W, &c->depth, &c->depthW};
+  for (auto* b : bufs)
+    if (b->p) HIP_TRY(hipMemsetAsync(b->p, 0, sizeof(float) * b->n, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->count = 0;
+  (void)n;
+  return PRT_OK;
+}
+
+int prt_frame_init(prt_context* c, int32_t width, int32_t height, uint32_t aov_mask)
+{
+  CTX_CHECK(c);
+  // render/path_renderer_stream.h:109-112
+  if (width <= 0 || height <= 0 || width % 32 != 0 || height % 8 != 0)
+    return fail(PRT_ERR_INVALID, "image size is not divisible by the tile size");
+  if (width > 4096 || height > 4096) return fail(PRT_ERR_INVALID, "image size is too large");
+  c->W = width;
+  c->H = height;
+  c->npix = width * height;
+  c->aovMask = aov_mask;
+  hipError_t e = allocFrame(c);
+  if (e != hipSuccess) return fail(PRT_ERR_NOMEM, hipGetErrorString(e));
+  if (c->debug)
+  {
+    HIP_TRY(c->dbgL.alloc(size_t(3) * c->npix));
+    HIP_TRY(c->dbgHash.alloc(c->npix));
+    HIP_TRY(c->dbgFirst.alloc(c->npix));
+    HIP_TRY(c->hash.alloc(c->npix));
+  }
+  return prt_frame_clear(c);
+}
+
+int prt_debug_enable(prt_context* c, int32_t enable)
+{
+  CTX_CHECK(c);
+  c->debug = enable != 0;
+  if (c->debug && c->npix)
+  {
+    HIP_TRY(c->dbgL.alloc(size_t(3) * c->npix));
+    HIP_TRY(c->dbgHash.alloc(c->npix));
+    HIP_TRY(c->dbgFirst.alloc(c->npix));
+    HIP_TRY(c->hash.alloc(c->npix));
+  }
+  if (!c->debug) { c->dbgL.release(); c->dbgHash.release(); c->dbgFirst.release(); c->hash.release(); }
+  return PRT_OK;
+}
+
+static RenderArgs makeArgs(prt_context* c)
+{
+  RenderArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.lut = c->lut.p;
+  a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
+  a.nLights = int(c->lights.size());
+  for (int i = 0; i < a.nLights; ++i) a.lights[i] = c->lights[i];
+  for (int i = 0; i <= a.nLights; ++i) a.lightCdf[i] = c->lightCdf[i];
+  a.env.texels = c->envTex.p; a.env.xDists = c->envX.p; a.env.yDist = c->envY.p;
+  a.env.W = c->envW; a.env.H = c->envH; a.env.LW = c->envLW; a.env.LH = c->envLH;
+  // camera/pinhole_camera.h:72-90 and thin_lens_camera.h:155-161
+  const prt_camera& k = c->cam;
+  const float aspect = float(c->W) / float(c->H);
+  const float hh = std::tan(0.5f * k.fov), hw = hh * aspect;
+  H3 U = hmk(k.U[0], k.U[1], k.U[2]), V = hmk(k.V[0], k.V[1], k.V[2]), N = hmk(k.N[0], k.N[1], k.N[2]);
+  auto basisMul = [&](float x, float y, float z) { return hadd(hadd(hscale(U, x), hscale(V, y)), hscale(N, z)); };
+  H3 iO = basisMul(-hw, hh, -1.f), iDx = basisMul(2.f * hw, 0.f, 0.f), iDy = basisMul(0.f, -2.f * hh, 0.f);
+  float* dst[3] = {a.imageO, a.imageDx, a.imageDy};
+  H3 src[3] = {iO, iDx, iDy};
+  for (int j = 0; j < 3; ++j) { dst[j][0] = src[j].x; dst[j][1] = src[j].y; dst[j][2] = src[j].z; }
+  for (int j = 0; j < 3; ++j) { a.camPos[j] = k.pos[j]; a.camU[j] = k.U[j]; a.camV[j] = k.V[j]; a.camN[j] = k.N[j]; }
+  a.lensRadius = k.lensRadius;
+  a.focal = k.focalDistance;
+  a.nearClip = k.nearClip;
+  a.nearClipScale = k.nearClip / k.focalDistance;
+  a.pinhole = k.lensRadius == 0.f;  // render/device_cpu.cpp:117-128
+  a.W = c->W; a.H = c->H; a.npix = c->npix;
+  a.invW = 1.0f / float(c->W);
+  a.invH = 1.0f / float(c->H);
+  prt_frame_dev& f = a.fb;
+  f.color = c->color.p; f.weight = c->weight.p; f.color2 = c->color2.p;
+  f.albedo = c->albedo.p; f.albedoW = c->albedoW.p; f.albedo1 = c->albedo1.p; f.albedo1W = c->albedo1W.p;
+  f.normal = c->normal.p; f.normalW = c->normalW.p; f.normal1 = c->normal1.p; f.normal1W = c->normal1W.p;
+  f.depth = c->depth.p; f.depthW = c->depthW.p;
+  f.dbgL = c->debug ? c->dbgL.p : nullptr;
+  f.dbgHash = c->debug ? c->dbgHash.p : nullptr;
+  f.dbgFirst = c->debug ? c->dbgFirst.p : nullptr;
+  a.seed = c->seed;
+  a.maxDepth = c->params.maxDepth;
+  a.rrDepth = c->params.rrDepth;
+  a.minWeight = c->params.minWeight;
+  a.maxRadiance = c->params.maxRadiance;
+  a.rayO = c->rayO.p; a.rayD = c->rayD.p; a.hit = c->hit.p; a.TP = c->TP.p; a.Lr = c->Lr.p; a.flags = c->flags.p;
+  a.hash = c->debug ? c->hash.p : nullptr;
+  a.shO = c->shO.p; a.shD = c->shD.p; a.dirL = c->dirL.p;
+  a.queue[0] = c->q0.p; a.queue[1] = c->q1.p; a.shadowQ = c->shadowQ.p; a.counters = c->counters.p;
+  a.spill = c->spill.p; a.spillStride = c->spillStride;
+  a.stats = c->stats.p;
+  return a;
+}
+
+int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
+{
+  CTX_CHECK(c);
+  if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
+  if (!c->lightsReady) return fail(PRT_ERR_STATE, "lights not set");
+  if (!c->mats.p) return fail(PRT_ERR_STATE, "materials not set");
+  if (c->maxMaterialId >= int(c->materials.size())) return fail(PRT_ERR_INVALID, "triangle material ID beyond the material table");
+  if (!c->rendererReady) return fail(PRT_ERR_STATE, "renderer not initialized");
+  if (!c->camReady) return fail(PRT_ERR_STATE, "camera not set");
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  if (n_spp < 0) return fail(PRT_ERR_INVALID, "negative sample count");
+  RenderArgs a = makeArgs(c);
+  HIP_TRY(hipMemsetAsync(c->stats.p, 0, sizeof(unsigned long long) * 8, c->stream));
+  for (int k = 0; k < 4; ++k) c->kernelMs[k] = 0.0;
+  c->launches[0] = c->launches[1] = 0;
+  auto t0 = std::chrono::high_resolution_clock::now();
+  for (int s = 0; s < n_spp; ++s)
+  {
+    // FrameBuffer::begin: checkpoint copy at power-of-two sample counts (accum_buffer.h:252-260)
+    if (isPowerOfTwo(c->count))
+    {
+      HIP_TRY(hipMemcpyAsync(c->colorC.p, c->color.p, sizeof(float) * 3 * c->npix, hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->weightC.p, c->weight.p, sizeof(float) * c->npix, hipMemcpyDeviceToDevice, c->stream));
+    }
+    a.sampleIndex = c->sampleIndex;
+    hipEventRecord(c->ev[0], c->stream);
+    launchGenerate(a, c->stream);
+    hipEventRecord(c->ev[1], c->stream);
+    float msGen = 0.f, msI = 0.f, msS = 0.f, msO = 0.f;
+    for (int depth = 0; depth <= a.maxDepth; ++depth)
+    {
+      hipEventRecord(c->ev[2], c->stream);
+      launchIntersect(a, depth, c->launch, c->stream);
+      hipEventRecord(c->ev[3], c->stream);
+      launchShade(a, depth, c->launch, c->stream);
+      hipEventRecord(c->ev[4], c->stream);
+      launchOccluded(a, depth, c->launch, c->stream);
+      hipEventRecord(c->ev[5], c->stream);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventSynchronize(c->ev[5]));
+      float m1 = 0, m2 = 0, m3 = 0;
+      hipEventElapsedTime(&m1, c->ev[2], c->ev[3]);
+      hipEventElapsedTime(&m2, c->ev[3], c->ev[4]);
+      hipEventElapsedTime(&m3, c->ev[4], c->ev[5]);
+      msI += m1; msS += m2; msO += m3;
+      c->launches[0] += 1; c->launches[1] += 1;
+      // stop early once no path survives (checked on the host after each depth)
+      int alive = 0;
+      HIP_TRY(hipMemcpyAsync(&alive, c->counters.p + 2 * (depth + 1), sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if (alive == 0) break;
+    }
+    hipEventElapsedTime(&msGen, c->ev[0], c->ev[1]);
+    c->kernelMs[0] += msGen; c->kernelMs[1] += msI; c->kernelMs[2] += msS; c->kernelMs[3] += msO;
+    c->sampleIndex++;  // (:185-186)
+    c->count++;        // FrameBuffer::finish (accum_buffer.h:263-267)
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  auto t1 = std::chrono::high_resolution_clock::now();
+  unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIP_TRY(hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 6; ++k) c->lastStats[k] = double(st[k]);
+  if (stats)
+  {
+    double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    double rays = double(st[0]) + double(st[3]);
+    stats->rays = rays;
+    stats->ms = ms;
+    stats->mray = ms > 0 ? rays / 1000.0 / ms : 0.0;
+    stats->spp = n_spp;
+  }
+  return PRT_OK;
+}
+
+int prt_blit(prt_context* c, int32_t buffer, float* dst, float* var)
+{
+  CTX_CHECK(c);
+  if (!dst) return fail(PRT_ERR_INVALID, "dst is null");
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  const size_t n = size_t(c->npix);
+  DevBuf<float>* s = nullptr;
+  DevBuf<float>* w = nullptr;
+  int ch = 3;
+  switch (buffer)
+  {
+    case PRT_BUF_COLOR: s = &c->color; w = &c->weight; break;
+    case PRT_BUF_ALBEDO: s = &c->albedo; w = &c->albedoW; break;
+    case PRT_BUF_ALBEDO1: s = &c->albedo1; w = &c->albedo1W; break;
+    case PRT_BUF_NORMAL: s = &c->normal; w = &c->normalW; break;
+    case PRT_BUF_NORMAL1: s = &c->normal1; w = &c->normal1W; break;
+    case PRT_BUF_DEPTH: s = &c->depth; w = &c->depthW; ch = 1; break;
+    default: return fail(PRT_ERR_INVALID, "unknown buffer");
+  }
+  if (!s->p) return fail(PRT_ERR_INVALID, "buffer not enabled in prt_frame_init");
+  if (var && (buffer != PRT_BUF_COLOR || !c->color2.p)) return fail(PRT_ERR_INVALID, "sample variance not enabled for the buffer");
+  std::vector<float> hs(ch * n), hw(n), h2;
+  HIP_TRY(hipMemcpy(hs.data(), s->p, sizeof(float) * ch * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hw.data(), w->p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (var)
+  {
+    h2.resize(3 * n);
+    HIP_TRY(hipMemcpy(h2.data(), c->color2.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+  }
+  for (size_t i = 0; i < n; ++i)
+  {
+    float r = hw[i] == 0.f ? 3.402823466e38f : 1.0f / hw[i];  // rcpSafe
+    for (int k = 0; k < ch; ++k) dst[i * ch + k] = hs[k * n + i] * r;
+    if (var)  // accum_buffer.h:629-631
+    {
+      float r1 = std::max(hw[i] - 1.f, 1.f);
+      for (int k = 0; k < 3; ++k)
+        var[i * 3 + k] = std::max(h2[k * n + i] - hsqr(hs[k * n + i]) * r, 0.f) * (1.0f / r1);
+    }
+  }
+  return PRT_OK;
+}
+
+int prt_get_accum(prt_context* c, float* sum, float* weight)
+{
+  CTX_CHECK(c);
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  if (sum) HIP_TRY(hipMemcpy(sum, c->color.p, sizeof(float) * 3 * c->npix, hipMemcpyDeviceToHost));
+  if (weight) HIP_TRY(hipMemcpy(weight, c->weight.p, sizeof(float) * c->npix, hipMemcpyDeviceToHost));
+  return PRT_OK;
+}
+
+int prt_frame_error(prt_context* c, float* error)
+{
+  CTX_CHECK(c);
+  if (!error) return fail(PRT_ERR_INVALID, "error is null");
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  const size_t n = size_t(c->npix);
+  std::vector<float> s(3 * n), w(n), sc(3 * n), wc(n);
+  HIP_TRY(hipMemcpy(s.data(), c->color.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(w.data(), c->weight.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(sc.data(), c->colorC.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(wc.data(), c->weightC.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  double err = 0.0;  // accum_buffer.h:752-805
+  for (size_t i = 0; i < n; ++i)
+  {
+    float r = w[i] == 0.f ? 3.402823466e38f : 1.0f / w[i], rc = wc[i] == 0.f ? 3.402823466e38f : 1.0f / wc[i];
+    float I[3], B[3];
+    for (int k = 0; k < 3; ++k) { I[k] = s[k * n + i] * r; B[k] = sc[k * n + i] * rc; }
+    float norm = I[0] + I[1] + I[2];
+    if (norm < 1.f) norm = std::sqrt(norm);
+    err += (std::fabs(I[0] - B[0]) + std::fabs(I[1] - B[1]) + std::fabs(I[2] - B[2])) / (norm + 1e-4f);
+  }
+  *error = float(err / double(n));
+  return PRT_OK;
+}
+
+int prt_query_rays_device(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v)
+{
+  CTX_CHECK(c);
+  if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
+  if (n < 0 || (n > 0 && (!rays || !prim))) return fail(PRT_ERR_INVALID, "invalid query arrays");
+  if (n == 0) return PRT_OK;
+  BvhView b{c->bvh.nodes, c->bvh.tris, c->bvh.root};
+  launchQuery(b, n, rays, prim, t, u, v, 0, c->spill.p, c->spillStride, c->launch.gridTrace, c->stream);
+  HIP_TRY(hipGetLastError());
+  return PRT_OK;
+}
+
+static int queryHost(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v, int anyHit)
+{
+  CTX_CHECK(c);
+  if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
+  if (n < 0 || (n > 0 && (!rays || !prim))) return fail(PRT_ERR_INVALID, "invalid query arrays");
+  if (n == 0) return PRT_OK;
+  DevBuf<float> dr, dt, du, dv;
+  DevBuf<int> dp;
+  HIP_TRY(dr.alloc(size_t(8) * n));
+  HIP_TRY(dp.alloc(n));
+  HIP_TRY(dt.alloc(n));
+  HIP_TRY(du.alloc(n));
+  HIP_TRY(dv.alloc(n));
+  HIP_TRY(hipMemcpy(dr.p, rays, sizeof(float) * 8 * n, hipMemcpyHostToDevice));
+  BvhView b{c->bvh.nodes, c->bvh.tris, c->bvh.root};
+  launchQuery(b, n, dr.p, dp.p, dt.p, du.p, dv.p, anyHit, c->spill.p, c->spillStride, c->launch.gridTrace, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(prim, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (t) HIP_TRY(hipMemcpy(t, dt.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (u) HIP_TRY(hipMemcpy(u, du.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (v) HIP_TRY(hipMemcpy(v, dv.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return PRT_OK;
+}
+
+int prt_query_rays(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v)
+{
+  return queryHost(c, n, rays, prim, t, u, v, 0);
+}
+
+int prt_query_occlusion(prt_context* c, int32_t n, const float* rays, int32_t* occluded)
+{
+  int rc = queryHost(c, n, rays, occluded, nullptr, nullptr, nullptr, 1);
+  if (rc != PRT_OK) return rc;
+  for (int i = 0; i < n; ++i) occluded[i] = occluded[i] >= 0 ? 1 : 0;
+  return PRT_OK;
+}
+
+int prt_debug_get(prt_context* c, float* sample_l, uint32_t* path_hash, int32_t* first_prim)
+{
+  CTX_CHECK(c);
+  if (!c->debug || !c->dbgL.p) return fail(PRT_ERR_STATE, "debug outputs not enabled");
+  if (sample_l) HIP_TRY(hipMemcpy(sample_l, c->dbgL.p, sizeof(float) * 3 * c->npix, hipMemcpyDeviceToHost));
+  if (path_hash) HIP_TRY(hipMemcpy(path_hash, c->dbgHash.p, sizeof(uint32_t) * c->npix, hipMemcpyDeviceToHost));
+  if (first_prim) HIP_TRY(hipMemcpy(first_prim, c->dbgFirst.p, sizeof(int) * c->npix, hipMemcpyDeviceToHost));
+  return PRT_OK;
+}
+
+int prt_debug_sampler(prt_context* c, int32_t n, const uint32_t* in, int32_t ndim, float* o1, float* o2, float* o3)
+{
+  CTX_CHECK(c);
+  if (n <= 0 || ndim <= 0 || !in || !o1 || !o2 || !o3) return fail(PRT_ERR_INVALID, "invalid sampler arrays");
+  DevBuf<uint32_t> di;
+  DevBuf<float> d1, d2, d3;
+  HIP_TRY(di.alloc(size_t(3) * n));
+  HIP_TRY(d1.alloc(size_t(n) * ndim));
+  HIP_TRY(d2.alloc(size_t(n) * ndim * 2));
+  HIP_TRY(d3.alloc(size_t(n) * ndim * 3));
+  HIP_TRY(hipMemcpy(di.p, in, sizeof(uint32_t) * 3 * n, hipMemcpyHostToDevice));
+  launchDebugSampler(n, di.p, ndim, d1.p, d2.p, d3.p, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(o1, d1.p, sizeof(float) * n * ndim, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(o2, d2.p, sizeof(float) * n * ndim * 2, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(o3, d3.p, sizeof(float) * n * ndim * 3, hipMemcpyDeviceToHost));
+  return PRT_OK;
+}
+
+int prt_debug_luts(prt_context* c, float* out)
+{
+  CTX_CHECK(c);
+  if (!out) return fail(PRT_ERR_INVALID, "out is null");
+  std::memcpy(out, c->luts.data(), sizeof(float) * PRT_LUT_FLOATS);
+  return PRT_OK;
+}
+
+int prt_debug_bvh_stats(prt_context* c, int64_t* s)
+{
+  CTX_CHECK(c);
+  if (!s) return fail(PRT_ERR_INVALID, "stats is null");
+  s[0] = c->bvh.nodeCount;
+  s[1] = c->bvh.ntris;
+  s[2] = c->bvh.maxDepth;
+  s[3] = c->ntris;
+  return PRT_OK;
+}
+
+int prt_debug_kernel_ms(prt_context* c, double* ms4, double* counters6, double* launches2)
+{
+  CTX_CHECK(c);
+  if (ms4) for (int k = 0; k < 4; ++k) ms4[k] = c->kernelMs[k];
+  if (counters6) for (int k = 0; k < 6; ++k) counters6[k] = c->lastStats[k];
+  if (launches2) { launches2[0] = c->launches[0]; launches2[1] = c->launches[1]; }
+  return PRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,691 @@
+// Device-side math of the stream path tracer for gfx950: vectors, the Sobol-Burley sampler,
+// sample warps, discrete distributions, GGX / Fresnel, the energy-compensation LUT lookups,
+// the three BSDFs of the stream path and the environment lights.  Every function follows
+// the reference file:line it cites; one GPU lane evaluates what one SIMD lane of the
+// reference evaluates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "prt_types.h"
+
+#define PRT_DEV __device__ __forceinline__
+
+namespace prt {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kPosMax = 3.402823466e38f;
+
+struct f3
+{
+  float x, y, z;
+  PRT_DEV f3() {}
+  PRT_DEV f3(float a) : x(a), y(a), z(a) {}
+  PRT_DEV f3(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+PRT_DEV f3 operator+(f3 a, f3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+PRT_DEV f3 operator-(f3 a, f3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+PRT_DEV f3 operator-(f3 a) { return f3(-a.x, -a.y, -a.z); }
+PRT_DEV f3 operator*(f3 a, f3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+PRT_DEV f3 operator*(f3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+PRT_DEV f3 operator*(float s, f3 a) { return f3(a.x * s, a.y * s, a.z * s); }
+PRT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PRT_DEV f3 cross(f3 a, f3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+PRT_DEV float lengthSqr(f3 a) { return dot(a, a); }
+PRT_DEV float length(f3 a) { return sqrtf(dot(a, a)); }
+PRT_DEV f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }  // math/vec3.h:399-402
+PRT_DEV float reduceMax(f3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
+PRT_DEV float sqr(float a) { return a * a; }
+PRT_DEV float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+PRT_DEV float lerpf(float a, float b, float s) { return a + s * (b - a); }          // math/math.h:89-92
+PRT_DEV float rcpSafe(float x) { return x == 0.f ? kPosMax : 1.0f / x; }            // math/simd.h:107-111
+PRT_DEV float luminance(f3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }  // image/color.h:17-22
+PRT_DEV float cos2sin(float x) { return sqrtf(fmaxf(1.f - x * x, 0.f)); }           // math/math.h:57-66
+
+struct Frame { f3 U, V, N; };
+PRT_DEV f3 toGlobal(const Frame& b, f3 a) { return a.x * b.U + a.y * b.V + a.z * b.N; }           // math/basis3.h:72-75
+PRT_DEV f3 toLocal(const Frame& b, f3 a) { return f3(dot(a, b.U), dot(a, b.V), dot(a, b.N)); }    // math/basis3.h:77-80
+
+// math/basis3.h:173-180
+PRT_DEV void makeFrame(f3& U, f3& V, f3 N)
+{
+  f3 U0(0.f, N.z, -N.y), U1(-N.z, 0.f, N.x);
+  U = normalize(lengthSqr(U0) > lengthSqr(U1) ? U0 : U1);
+  V = normalize(cross(N, U));
+}
+
+// ---------------------------------------------------------------------------------------
+// Sobol-Burley sampler (sampling/sobol_burley.h, sobol_burley_sampler_simd.h) -- bit-exact
+// ---------------------------------------------------------------------------------------
+PRT_DEV uint32_t reverseBits(uint32_t x) { return __builtin_bitreverse32(x); }  // math/simd.h:140-147
+PRT_DEV uint32_t hashHP(uint32_t i)  // sobol_burley.h:31-44
+{
+  i ^= i >> 16; i *= 0x21f0aaadu; i ^= i >> 15; i *= 0xd35a2d97u; i ^= i >> 15;
+  return i ^ 0xe6fe3bebu;
+}
+PRT_DEV uint32_t reversedBitOwen(uint32_t n, uint32_t seed)  // sobol_burley.h:52-62
+{
+  n ^= n * 0x3d20adeau; n += seed; n *= (seed >> 16) | 1u; n ^= n * 0x05526c56u; n ^= n * 0x53a22864u;
+  return n;
+}
+// math/simd.h:76-84, with the default AVX2 host build's two-step uint32 -> float conversion
+// (math/simd/vfloat8_avx2.h:171-178): low 31 bits converted, then 2^31 added.
+PRT_DEV float toFloatUnormExcl(uint32_t x)
+{
+  float lo = (float)(int32_t)(x & 0x7fffffffu);
+  float hi = (x & 0x80000000u) ? 2147483648.0f : 0.0f;
+  return (lo + hi) * (1.0f / 4294967808.0f);
+}
+// Sobol dimension 2 direction numbers with reversed bits (the third column of
+// sampling/sobol_burley.cpp:9-51), generated at compile time from the primitive polynomial
+// x^2 + x + 1 with initial m = {1, 3}: v_k = v_{k-1} ^ v_{k-2} ^ (v_{k-2} >> 2), MSB-first,
+// then bit-reversed.
+struct SobolColumn { uint32_t v[32]; };
+constexpr SobolColumn makeSobolDim2()
+{
+  SobolColumn c{};
+  uint32_t v[33] = {};
+  v[1] = 1u << 31;
+  v[2] = 3u << 30;
+  for (int k = 3; k <= 32; ++k) v[k] = v[k - 1] ^ v[k - 2] ^ (v[k - 2] >> 2);
+  for (int k = 1; k <= 32; ++k)
+  {
+    uint32_t r = 0;
+    for (int b = 0; b < 32; ++b) r |= ((v[k] >> b) & 1u) << (31 - b);
+    c.v[k - 1] = r;
+  }
+  return c;
+}
+__constant__ static const SobolColumn kSobolDim2 = makeSobolDim2();
+
+PRT_DEV uint32_t sobolDim1(uint32_t v)  // sobol_burley.h:117-128 (Ahmed 2024)
+{
+  v ^= v << 16;
+  v ^= (v & 0x00FF00FFu) << 8;
+  v ^= (v & 0x0F0F0F0Fu) << 4;
+  v ^= (v & 0x33333333u) << 2;
+  v ^= (v & 0x55555555u) << 1;
+  return reverseBits(v);
+}
+PRT_DEV uint32_t sobolDim2(uint32_t rev)  // sobol_burley.h:129-143: XOR of columns for set bits
+{
+  uint32_t r = 0;
+  // bit k of the reversed index (counting from the MSB) selects column k
+  while (rev)
+  {
+    int j = __builtin_clz(rev);
+    r ^= kSobolDim2.v[j];
+    rev &= ~(0x80000000u >> j);
+  }
+  return r;
+}
+PRT_DEV float owenFinish(uint32_t r, uint32_t scramble) { return toFloatUnormExcl(reverseBits(reversedBitOwen(r, scramble))); }
+
+struct Sampler
+{
+  uint32_t index, scramble;
+  // setSample (sobol_burley_sampler_simd.h:36-62, math/hash.h:12-42)
+  PRT_DEV void set(uint32_t seed, uint32_t sampleIndex, uint32_t pixelId)
+  {
+    index = sampleIndex;
+    uint32_t k = pixelId * 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    uint32_t h = seed ^ k;
+    h = ((h << 13) | (h >> 19)) * 5u + 0xe6546b64u;
+    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+    scramble = h;
+  }
+  PRT_DEV float get1D(uint32_t dim) const  // sobol_burley.h:222-238
+  {
+    uint32_t s = scramble ^ hashHP(dim);
+    uint32_t i = reversedBitOwen(reverseBits(index), s ^ 0xbff95bfeu);
+    return owenFinish(reverseBits(i), s ^ 0x635c77bdu);
+  }
+  PRT_DEV void get2D(uint32_t set, float& a, float& b) const  // sobol_burley.h:250-267
+  {
+    uint32_t s = scramble ^ hashHP(set);
+    uint32_t i = reversedBitOwen(reverseBits(index), s ^ 0xf8ade99au);
+    a = owenFinish(reverseBits(i), s ^ 0xe0aaaf76u);
+    b = owenFinish(sobolDim1(i), s ^ 0x94964d4eu);
+  }
+  PRT_DEV void get3D(uint32_t set, float& a, float& b, float& c) const  // sobol_burley.h:279-297
+  {
+    uint32_t s = scramble ^ hashHP(set);
+    uint32_t i = reversedBitOwen(reverseBits(index), s ^ 0xcaa726acu);
+    a = owenFinish(reverseBits(i), s ^ 0x9e78e391u);
+    b = owenFinish(sobolDim1(i), s ^ 0x67c33241u);
+    c = owenFinish(sobolDim2(i), s ^ 0x78c395c5u);
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// Warps (sampling/shape_sampler.h)
+// ---------------------------------------------------------------------------------------
+PRT_DEV f3 cosineSampleHemisphere(float& pdf, float sx, float sy)  // :69-86
+{
+  float ct = sqrtf(sx), st = sqrtf(1.f - sx);
+  float sp, cp;
+  sincosf(2.f * kPi * sy, &sp, &cp);
+  pdf = ct * (1.f / kPi);
+  return f3(cp * st, sp * st, ct);
+}
+PRT_DEV f3 uniformSampleSphere(float& pdf, float sx, float sy)  // :23-40
+{
+  float ct = 1.f - 2.f * sx, st = 2.f * sqrtf(sx * (1.f - sx));
+  float sp, cp;
+  sincosf(2.f * kPi * sy, &sp, &cp);
+  pdf = 1.f / (4.f * kPi);
+  return f3(cp * st, sp * st, ct);
+}
+PRT_DEV f3 uniformSampleCone(float& pdf, float sx, float sy, float cosAngle, float conePdf)  // :100-117
+{
+  float ct = 1.f - sx * (1.f - cosAngle);
+  float st = cos2sin(ct);
+  float sp, cp;
+  sincosf(2.f * kPi * sy, &sp, &cp);
+  pdf = conePdf;
+  return f3(cp * st, sp * st, ct);
+}
+
+// ---------------------------------------------------------------------------------------
+// Discrete distributions (sampling/distribution1d.h:76-101, distribution1d.cpp:35-55)
+// ---------------------------------------------------------------------------------------
+PRT_DEV int dist1dLookup(const float* cdf, int size, float s)
+{
+  int first = 0, len = size;
+  while (len > 0)
+  {
+    int half = len >> 1, mid = first + half;
+    if (s < cdf[mid]) len = half;
+    else { first = mid + 1; len = len - half - 1; }
+  }
+  return first - 1 > 0 ? first - 1 : 0;
+}
+PRT_DEV int dist1dSampleReuse(const float* cdf, int size, float& pdf, float& s)
+{
+  int i = dist1dLookup(cdf, size, s);
+  float c0 = cdf[i], c1 = cdf[i + 1];
+  pdf = c1 - c0;
+  s = (s - c0) * (1.0f / pdf);
+  return i;
+}
+PRT_DEV float dist1dPdf(const float* cdf, int size, int i) { return (i < 0 || i >= size) ? 0.f : cdf[i + 1] - cdf[i]; }
+
+// ---------------------------------------------------------------------------------------
+// GGX (bsdf/ggx_distribution.h) -- isotropic alpha = max(roughness^2, 0.001)
+// ---------------------------------------------------------------------------------------
+struct GGX
+{
+  float a;
+  PRT_DEV float D(f3 wh) const  // :24-31
+  {
+    float c2 = sqr(wh.z);
+    float e = (sqr(wh.x / a) + sqr(wh.y / a)) / c2;
+    return 1.0f / (kPi * a * a * sqr(c2 * (1.f + e)));
+  }
+  PRT_DEV float lambda(f3 w) const  // :84-90
+  {
+    float c2 = sqr(w.z);
+    float invA2 = (sqr(w.x * a) + sqr(w.y * a)) / c2;
+    return 0.5f * (-1.f + sqrtf(1.f + invA2));
+  }
+  PRT_DEV float G1(f3 w) const { return 1.0f / (1.f + lambda(w)); }
+  PRT_DEV float G2(f3 wo, f3 wi) const { return 1.0f / (1.f + lambda(wo) + lambda(wi)); }
+  PRT_DEV float evalVisible(f3 wh, f3 wo, float cosOH, float& pdf) const  // :42-48
+  {
+    float d = D(wh);
+    pdf = G1(wo) * fabsf(cosOH) * d / fabsf(wo.z);
+    return d;
+  }
+  PRT_DEV f3 sampleVisible(f3 wo, float& pdf, float sx, float sy) const  // :52-78 (Heitz 2017)
+  {
+    f3 V = normalize(f3(a * wo.x, a * wo.y, wo.z));
+    f3 T1 = (V.z < 0.9999f) ? normalize(cross(V, f3(0.f, 0.f, 1.f))) : f3(1.f, 0.f, 0.f);
+    f3 T2 = cross(T1, V);
+    float aa = 1.f / (1.f + V.z);
+    float r = sqrtf(sx);
+    float phi = (sy < aa) ? sy / aa * kPi : kPi + (sy - aa) / (1.f - aa) * kPi;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    float P1 = r * cp;
+    float P2 = r * sp * ((sy < aa) ? 1.f : V.z);
+    f3 wh = P1 * T1 + P2 * T2 + sqrtf(fmaxf(0.f, 1.f - P1 * P1 - P2 * P2)) * V;
+    wh = normalize(f3(a * wh.x, a * wh.y, fmaxf(0.f, wh.z)));
+    pdf = G1(wo) * fabsf(dot(wo, wh)) * D(wh) / fabsf(wo.z);
+    return wh;
+  }
+};
+
+// bsdf/optics.h
+PRT_DEV f3 reflect(f3 I, f3 N) { return 2.f * dot(I, N) * N - I; }
+PRT_DEV f3 reflect(f3 I, f3 N, float cosI) { return 2.f * cosI * N - I; }
+PRT_DEV f3 refract(f3 I, f3 N, float cosI, float cosT, float eta) { return eta * (cosI * N - I) - cosT * N; }
+PRT_DEV float fresnelDielectric(float cosI, float cosT, float eta)  // :61-67
+{
+  float Rper = (eta * cosI - cosT) / (eta * cosI + cosT);
+  float Rpar = (cosI - eta * cosT) / (cosI + eta * cosT);
+  return 0.5f * (Rpar * Rpar + Rper * Rper);
+}
+PRT_DEV float fresnelDielectric(float cosI, float eta)  // :84-99
+{
+  cosI = fmaxf(cosI, 0.f);
+  float k = 1.f - eta * eta * (1.f - cosI * cosI);
+  return (k >= 0.f) ? fresnelDielectric(cosI, sqrtf(k), eta) : 1.f;
+}
+PRT_DEV float fresnelDielectricEx(float cosI, float& cosT, float eta)  // :122-139
+{
+  cosI = fabsf(cosI);
+  float k = 1.f - eta * eta * (1.f - cosI * cosI);
+  if (!(k >= 0.f)) { cosT = 0.f; return 1.f; }
+  cosT = sqrtf(k);
+  return fresnelDielectric(cosI, cosT, eta);
+}
+PRT_DEV float fresnelConductor(float cosI, float eta, float k)  // :179-190
+{
+  cosI = fabsf(cosI);
+  float c2 = sqr(cosI), tmp = sqr(eta) + sqr(k), e2 = 2.f * eta * cosI;
+  float Rpar = (tmp * c2 - e2 + 1.f) / (tmp * c2 + e2 + 1.f);
+  float Rper = (tmp - e2 + c2) / (tmp + e2 + c2);
+  return 0.5f * (Rpar + Rper);
+}
+
+// ---------------------------------------------------------------------------------------
+// LUT lookups (math/interpolation.h:26-227, bsdf/albedo_tables.h:20-154)
+// ---------------------------------------------------------------------------------------
+PRT_DEV float interp1D(float x, const float* f, int size)
+{
+  float xc = clampf(x, 0.f, float(size - 1));
+  float s = xc - floorf(xc);
+  int x0 = min(int(xc), size - 1), x1 = min(x0 + 1, size - 1);
+  return lerpf(f[x0], f[x1], s);
+}
+PRT_DEV float interp2D(float px, float py, const float* f, int n)
+{
+  float xc = clampf(px, 0.f, float(n - 1)), yc = clampf(py, 0.f, float(n - 1));
+  float sx = xc - floorf(xc), sy = yc - floorf(yc);
+  int x0 = min(int(xc), n - 1), x1 = min(x0 + 1, n - 1);
+  int y0 = min(int(yc), n - 1), y1 = min(y0 + 1, n - 1);
+  float f0 = lerpf(f[x0 + y0 * n], f[x1 + y0 * n], sx);
+  float f1 = lerpf(f[x0 + y1 * n], f[x1 + y1 * n], sx);
+  return lerpf(f0, f1, sy);
+}
+PRT_DEV float interp3D(float px, float py, float pz, const float* f, int n)
+{
+  float xc = clampf(px, 0.f, float(n - 1)), yc = clampf(py, 0.f, float(n - 1)), zc = clampf(pz, 0.f, float(n - 1));
+  float sx = xc - floorf(xc), sy = yc - floorf(yc), sz = zc - floorf(zc);
+  int x0 = min(int(xc), n - 1), x1 = min(x0 + 1, n - 1);
+  int y0 = min(int(yc), n - 1), y1 = min(y0 + 1, n - 1);
+  int z0 = min(int(zc), n - 1), z1 = min(z0 + 1, n - 1);
+  int ny = n, nz = n * n;
+  float f00 = lerpf(f[x0 + y0 * ny + z0 * nz], f[x1 + y0 * ny + z0 * nz], sx);
+  float f01 = lerpf(f[x0 + y1 * ny + z0 * nz], f[x1 + y1 * ny + z0 * nz], sx);
+  float f10 = lerpf(f[x0 + y0 * ny + z1 * nz], f[x1 + y0 * ny + z1 * nz], sx);
+  float f11 = lerpf(f[x0 + y1 * ny + z1 * nz], f[x1 + y1 * ny + z1 * nz], sx);
+  return lerpf(lerpf(f00, f01, sy), lerpf(f10, f11, sy), sz);
+}
+PRT_DEV float ggxAlbedo(const float* lut, float cosO, float roughness)  // MicrofacetAlbedoTable::eval
+{
+  return interp2D(cosO * 31.f, roughness * 31.f, lut + PRT_LUT_GGX_DIR, 32);
+}
+// MicrofacetDielectricAlbedoTable::eval (eta <= 1: etaDir over [1/3, 1], else rcpEtaDir over [1, 3])
+PRT_DEV float dielAlbedo(const float* lut, float cosO, float eta, float roughness)
+{
+  if (eta <= 1.f)
+  {
+    float e = (eta - (1.0f / 3.0f)) / (1.0f - (1.0f / 3.0f));
+    return interp3D(cosO * 15.f, e * 15.f, roughness * 15.f, lut + PRT_LUT_DIEL_ETA_DIR, 16);
+  }
+  float e = (eta - 1.f) / (3.f - 1.f);
+  return interp3D(cosO * 15.f, e * 15.f, roughness * 15.f, lut + PRT_LUT_DIEL_RCPETA_DIR, 16);
+}
+
+// ---------------------------------------------------------------------------------------
+// Shading context and BSDFs
+// ---------------------------------------------------------------------------------------
+namespace BsdfType {
+enum : int {
+  DiffuseReflection = 1 << 0, GlossyReflection = 1 << 1, SpecularReflection = 1 << 2,
+  DiffuseTransmission = 1 << 3, GlossyTransmission = 1 << 4, SpecularTransmission = 1 << 5,
+  UnbentTransmission = 1 << 6, Transparency = 1 << 7,
+  Smooth = DiffuseReflection | DiffuseTransmission | GlossyReflection | GlossyTransmission,
+  Delta = SpecularReflection | SpecularTransmission | UnbentTransmission | Transparency,
+  Continuation = UnbentTransmission | Transparency,
+};
+}  // bsdf/bsdf.h:15-44
+
+struct Ctx { f3 p, Ng; Frame f; float eps; };
+
+// core/shading_context.h:144-191 (normal adaption + Gram-Schmidt)
+PRT_DEV Frame shadingFrame(const Ctx& c, f3 wo)
+{
+  f3 N = c.f.N;
+  f3 wi = reflect(wo, N);
+  if (dot(wi, c.Ng) < 0.f)
+  {
+    float a = -dot(wi, c.Ng), b = dot(N, c.Ng);
+    if (b < 0.f) { b = -b; wi = -wi; }
+    f3 wi1 = normalize(wi + a * rcpSafe(b) * N + 1e-5f * c.Ng);
+    N = normalize(wo + wi1);
+  }
+  Frame fr;
+  fr.N = N;
+  fr.U = c.f.U - dot(c.f.U, N) * N;
+  float ul = lengthSqr(fr.U);
+  fr.U = fr.U * (1.0f / sqrtf(ul));
+  fr.V = c.f.V - dot(c.f.V, N) * N;
+  fr.V = fr.V - dot(fr.V, fr.U) * fr.U;
+  float vl = lengthSqr(fr.V);
+  fr.V = fr.V * (1.0f / sqrtf(vl));
+  if (ul == 0.f || vl == 0.f) makeFrame(fr.U, fr.V, fr.N);
+  return fr;
+}
+
+// One BSDF instance for a hit: the material record plus per-material constants precomputed
+// on the host at upload (prt_material_constants); the per-lane part is the frame.
+struct Bsdf
+{
+  Frame fr;
+  int kind;
+  float roughness;
+  GGX ggx;
+  // diffuse: color; conductor: eta/k per channel + fms terms; dielectric: eta + fms terms
+  f3 c0, c1, c2;  // diffuse: color | conductor: eta, k, fmsScale
+  f3 rr;          // conductor reflectivity (albedo AOV)
+  float Eavg;     // conductor: average albedo
+  float eta, EavgEta, EavgRcpEta, fmsRatio;  // dielectric (oriented)
+  float importance;
+};
+
+PRT_DEV f3 bsdfAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
+{
+  return b.kind == PRT_KIND_DIFFUSE ? b.c0 : (b.kind == PRT_KIND_CONDUCTOR ? b.rr : f3(1.f));
+}
+
+// bsdf/diffuse.h:55-68
+PRT_DEV f3 diffuseEval(const Bsdf& b, f3 wi, float& pdf)
+{
+  float c = fmaxf(dot(wi, b.fr.N), 0.f);
+  pdf = c * (1.f / kPi);
+  return b.c0 * (1.f / kPi) * c;
+}
+// bsdf/rough_conductor.h:132-202
+PRT_DEV f3 conductorF(const Bsdf& b, float c)
+{
+  return f3(fresnelConductor(c, b.c0.x, b.c1.x), fresnelConductor(c, b.c0.y, b.c1.y), fresnelConductor(c, b.c0.z, b.c1.z));
+}
+PRT_DEV f3 conductorEval(const float* lut, const Bsdf& b, f3 wo, f3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N), cosI = dot(wi, b.fr.N);
+  if (!(cosO > 0.f && cosI > 0.f)) return f3(0.f);
+  f3 wh = normalize(wi + wo);
+  float cosOH = dot(wo, wh);
+  f3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  f3 F = conductorF(b, cosOH);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  float Eo = ggxAlbedo(lut, cosO, b.roughness), Ei = ggxAlbedo(lut, cosI, b.roughness);
+  f3 fms = b.c2 * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.Eavg))) * cosI);
+  pdf = whPdf * (1.0f / (4.f * fabsf(cosOH)));
+  return F * (D * G * (1.0f / (4.f * cosO))) + fms;
+}
+PRT_DEV f3 conductorSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy)
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  f3 wo0 = toLocal(b.fr, wo);
+  float whPdf;
+  f3 wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+  wi = reflect(wo, wh);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI > 0.f && dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  float cosOH = dot(wo, wh);
+  f3 wi0 = toLocal(b.fr, wi);
+  f3 F = conductorF(b, cosOH);
+  float G = b.ggx.G2(wo0, wi0);
+  float Eo = ggxAlbedo(lut, cosO, b.roughness), Ei = ggxAlbedo(lut, cosI, b.roughness);
+  f3 fms = b.c2 * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.Eavg))) * cosI);
+  type = BsdfType::GlossyReflection;
+  pdf = whPdf * (1.0f / (4.f * fabsf(cosOH)));
+  return F * (G * rcpSafe(b.ggx.G1(wo0))) + fms * (1.0f / pdf);
+}
+// bsdf/rough_dielectric.h:296-526
+PRT_DEV float dielSingle(const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return 0.f;
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f)) return 0.f;
+  bool refl = cosI > 0.f;
+  f3 wh = normalize(refl ? wi + wo : b.eta * wo + wi);
+  if (dot(wh, b.fr.N) < 0.f) wh = -wh;
+  float cosOH = dot(wo, wh), cosIH = dot(wi, wh);
+  float F = fresnelDielectric(cosOH, b.eta);
+  float value;
+  if (refl)
+  {
+    pdf = F * (1.0f / (4.f * fabsf(cosOH)));
+    value = F * (1.0f / (4.f * cosO));
+  }
+  else
+  {
+    pdf = (1.f - F) * (1.0f / sqr(b.eta)) * fabsf(cosIH) * (1.0f / sqr(cosOH + (1.0f / b.eta) * cosIH));
+    value = pdf * fabsf(cosOH) * (1.0f / cosO);
+  }
+  f3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  pdf *= whPdf;
+  return value * D * G;
+}
+PRT_DEV void dielFms(const float* lut, const Bsdf& b, float Eo, float cosI, float& fmsPdf, float& fmsValue)
+{
+  if (cosI > 0.f)
+  {
+    float Ei = dielAlbedo(lut, cosI, b.eta, b.roughness);
+    fmsPdf = b.fmsRatio * cosI * (1.f / kPi);
+    fmsValue = b.fmsRatio * (1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.EavgEta))) * cosI;
+  }
+  else
+  {
+    float ac = fabsf(cosI);
+    float Ei = dielAlbedo(lut, ac, 1.0f / b.eta, b.roughness);
+    fmsPdf = (1.f - b.fmsRatio) * ac * (1.f / kPi);
+    fmsValue = (1.f - b.fmsRatio) * (1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.EavgRcpEta))) * ac;
+  }
+}
+PRT_DEV f3 dielEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  float Eo = dielAlbedo(lut, cosO, b.eta, b.roughness);
+  float fmsPdf, fmsValue, sPdf;
+  dielFms(lut, b, Eo, cosI, fmsPdf, fmsValue);
+  float sValue = dielSingle(b, c, wo, wi, sPdf);
+  pdf = Eo * sPdf + (1.f - Eo) * fmsPdf;
+  return f3(sValue + fmsValue);
+}
+PRT_DEV f3 dielSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float Eo = dielAlbedo(lut, cosO, b.eta, b.roughness);
+  bool ok;
+  if (sc <= Eo)
+  {
+    f3 wo0 = toLocal(b.fr, wo);
+    float whPdf;
+    f3 wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+    float cosOH = dot(wo, wh), cosTH;
+    float F = fresnelDielectricEx(cosOH, cosTH, b.eta);
+    if (sc * (1.0f / Eo) <= F) { wi = reflect(wo, wh, cosOH); ok = dot(wi, b.fr.N) > 0.f; }
+    else { wi = refract(wo, wh, cosOH, cosTH, b.eta); ok = dot(wi, b.fr.N) < 0.f; }
+  }
+  else
+  {
+    float sc1 = (sc - Eo) * (1.0f / (1.f - Eo));
+    float cp;
+    f3 dir = toGlobal(b.fr, cosineSampleHemisphere(cp, sx, sy));
+    wi = (sc1 <= b.fmsRatio) ? dir : -dir;
+    ok = true;
+  }
+  if (!ok) return f3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  type = cosI > 0.f ? BsdfType::GlossyReflection : BsdfType::GlossyTransmission;
+  float fmsPdf, fmsValue, sPdf;
+  dielFms(lut, b, Eo, cosI, fmsPdf, fmsValue);
+  float sValue = dielSingle(b, c, wo, wi, sPdf);
+  pdf = Eo * sPdf + (1.f - Eo) * fmsPdf;
+  return f3((sValue + fmsValue) * (1.0f / pdf));
+}
+
+// MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210)
+PRT_DEV f3 bsdfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  f3 v(0.f);
+  pdf = 0.f;
+  if (b.importance > 0.f)
+  {
+    float cp;
+    if (b.kind == PRT_KIND_DIFFUSE) v = diffuseEval(b, wi, cp);
+    else if (b.kind == PRT_KIND_CONDUCTOR) v = conductorEval(lut, b, wo, wi, cp);
+    else v = dielEval(lut, b, c, wo, wi, cp);
+    pdf = cp * b.importance;
+  }
+  pdf *= 1.0f / b.importance;
+  return v;
+}
+PRT_DEV f3 bsdfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  if (!(b.importance > 0.f)) return f3(0.f);
+  if (b.kind == PRT_KIND_DIFFUSE)  // bsdf/diffuse.h:61-67
+  {
+    wi = toGlobal(b.fr, cosineSampleHemisphere(pdf, sx, sy));
+    type = BsdfType::DiffuseReflection;
+    return dot(wi, c.Ng) > 0.f ? b.c0 : f3(0.f);
+  }
+  if (b.kind == PRT_KIND_CONDUCTOR) return conductorSample(lut, b, c, wo, wi, pdf, type, sx, sy);
+  return dielSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+}
+
+// Builds the BSDF for a hit from the material constants (prt_material_constants, computed on
+// the host from the formulas of rough_conductor.h:121-130 and rough_dielectric.h:279-291).
+PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool inside)
+{
+  Bsdf b;
+  b.fr = shadingFrame(c, wo);
+  b.kind = m.kind;
+  b.roughness = m.roughness;
+  b.ggx.a = m.alpha;
+  b.importance = m.importance;
+  b.c0 = f3(m.c0[0], m.c0[1], m.c0[2]);
+  b.c1 = f3(m.c1[0], m.c1[1], m.c1[2]);
+  b.c2 = f3(m.c2[0], m.c2[1], m.c2[2]);
+  b.rr = f3(m.rr[0], m.rr[1], m.rr[2]);
+  b.Eavg = m.Eavg;
+  int o = inside ? 1 : 0;  // material/material.h:66-76: inside the medium = backfacing
+  b.eta = m.eta[o];
+  b.EavgEta = m.EavgEta[o];
+  b.EavgRcpEta = m.EavgRcpEta[o];
+  b.fmsRatio = m.fmsRatio[o];
+  return b;
+}
+
+// ---------------------------------------------------------------------------------------
+// Environment lights (light/*.h, render/scene.h:137-250)
+// ---------------------------------------------------------------------------------------
+struct EnvView
+{
+  const float4* texels;  // [H][W] rgb + pad, texture row order
+  const float* xDists;   // [LH][LW+1]
+  const float* yDist;    // [LH+1]
+  int W, H, LW, LH;
+};
+
+// image/image_texture.h:252-283: bilinear, u repeat, v clamp
+PRT_DEV f3 envLookup(const EnvView& e, float u, float v)
+{
+  float x = u - 0.5f / float(e.W), y = v - 0.5f / float(e.H);
+  float xr = (x - floorf(x)) * float(e.W);
+  float yr = clampf(y * float(e.H), 0.f, float(e.H - 1));
+  float sx = xr - floorf(xr), sy = yr - floorf(yr);
+  int x0 = min(max(int(xr), 0), e.W - 1), y0 = min(max(int(yr), 0), e.H - 1);
+  int x1 = (x0 + 1 == e.W) ? 0 : x0 + 1, y1 = min(y0 + 1, e.H - 1);
+  float4 c00 = e.texels[y0 * e.W + x0], c10 = e.texels[y0 * e.W + x1];
+  float4 c01 = e.texels[y1 * e.W + x0], c11 = e.texels[y1 * e.W + x1];
+  f3 a(c00.x, c00.y, c00.z), b(c10.x, c10.y, c10.z), c(c01.x, c01.y, c01.z), d(c11.x, c11.y, c11.z);
+  f3 c0 = a + sx * (b - a), c1 = c + sx * (d - c);
+  return c0 + sy * (c1 - c0);
+}
+
+PRT_DEV f3 envEvalOne(const prt_light_dev& l, const EnvView& e, f3 wi, float& pdf)
+{
+  pdf = 0.f;
+  if (l.kind == PRT_LIGHT_AMBIENT)  // ambient_light.h:33-37
+  {
+    pdf = 1.f / (4.f * kPi);
+    return f3(l.r, l.g, l.b);
+  }
+  if (l.kind == PRT_LIGHT_DISTANT)  // distant_light.h:48-57
+  {
+    float c = dot(wi, f3(l.dx, l.dy, l.dz));
+    if (!(c >= l.cosAngle)) return f3(0.f);
+    pdf = l.conePdf;
+    return f3(l.r, l.g, l.b);
+  }
+  // image_light.h:79-91
+  float theta = acosf(clampf(wi.y, -1.f, 1.f));
+  float phi = atan2f(wi.x, -wi.z);
+  float u = 0.5f + phi * (1.0f / (2.0f * kPi));
+  float v = 1.0f - theta * (1.0f / kPi);
+  int ix = min(max(int(u * float(e.LW)), 0), e.LW - 1);
+  int iy = min(max(int(v * float(e.LH)), 0), e.LH - 1);
+  float p = dist1dPdf(e.xDists + iy * (e.LW + 1), e.LW, ix) * dist1dPdf(e.yDist, e.LH, iy);
+  pdf = p * float(e.LW) * float(e.LH) * (1.0f / (2.0f * kPi * kPi * sinf(theta)));
+  return envLookup(e, u, v) * f3(l.r, l.g, l.b);
+}
+
+PRT_DEV f3 envSampleOne(const prt_light_dev& l, const EnvView& e, f3& wi, float& pdf, float sx, float sy)
+{
+  if (l.kind == PRT_LIGHT_AMBIENT)  // ambient_light.h:46-51
+  {
+    wi = uniformSampleSphere(pdf, sx, sy);
+    return f3(l.r, l.g, l.b) * (1.0f / pdf);
+  }
+  if (l.kind == PRT_LIGHT_DISTANT)  // distant_light.h:66-71
+  {
+    Frame fr;
+    fr.U = f3(l.fu[0], l.fu[1], l.fu[2]);
+    fr.V = f3(l.fv[0], l.fv[1], l.fv[2]);
+    fr.N = f3(l.dx, l.dy, l.dz);
+    wi = toGlobal(fr, uniformSampleCone(pdf, sx, sy, l.cosAngle, l.conePdf));
+    return f3(l.r, l.g, l.b) * (1.0f / pdf);
+  }
+  // image_light.h:112-134 (Distribution2D::sampleReuse, distribution2d.h:53-64)
+  float yPdf, xPdf;
+  int iy = dist1dSampleReuse(e.yDist, e.LH, yPdf, sy);
+  int ix = dist1dSampleReuse(e.xDists + iy * (e.LW + 1), e.LW, xPdf, sx);
+  pdf = xPdf * yPdf;
+  float u = (float(ix) + sx) * (1.0f / float(e.LW));
+  float v = (float(iy) + sy) * (1.0f / float(e.LH));
+  float theta = (1.0f - v) * kPi, phi = (u - 0.5f) * (2.0f * kPi);
+  float sp, cp, st, ct;
+  sincosf(phi, &sp, &cp);
+  sincosf(theta, &st, &ct);
+  wi = f3(sp * st, ct, -cp * st);
+  pdf *= float(e.LW) * float(e.LH) * (1.0f / (2.0f * kPi * kPi * st));
+  return envLookup(e, u, v) * f3(l.r, l.g, l.b) * (1.0f / pdf);
+}
+
+// MIS power heuristic (render/path_renderer_stream.h:1077-1080)
+PRT_DEV float misWeight(float a, float b) { return sqr(a) * rcpSafe(sqr(a) + sqr(b)); }
+
+}  // namespace prt

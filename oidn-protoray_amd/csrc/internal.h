@@ -1,0 +1,99 @@
+// Internal interfaces between the host driver (context.hip), the BVH builder (bvh.hip) and
+// the wavefront kernels (render.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <string>
+#include "prt_types.h"
+#include "dev_math.h"
+#include "trace.h"
+
+namespace prt {
+
+struct BvhData
+{
+  prt_bvh_node* nodes = nullptr;
+  float4* tris = nullptr;
+  int root = 0;
+  int ntris = 0;
+  long long nodeCount = 0;
+  int maxDepth = 0;
+  double buildMs = 0.0;
+};
+
+hipError_t buildBvh(const float* dPos, const int* dIdx, int ntris, BvhData& out, hipStream_t stream, std::string& err);
+void freeBvh(BvhData& b);
+
+// Everything a sample pass reads, passed by value to each kernel.
+struct RenderArgs
+{
+  // scene
+  const float* pos;
+  const float* nrm;  // may be null
+  const int* idx;
+  const int* triMat;
+  const prt_material_constants* mats;
+  const float* lut;
+  BvhView bvh;
+  prt_light_dev lights[PRT_MAX_LIGHTS];
+  float lightCdf[PRT_MAX_LIGHTS + 1];
+  int nLights;
+  EnvView env;
+  // camera (precomputed as PinholeCamera / ThinLensCamera constructors do)
+  float camPos[3], imageO[3], imageDx[3], imageDy[3], camU[3], camV[3], camN[3];
+  float lensRadius, focal, nearClip, nearClipScale;
+  int pinhole;
+  // frame
+  int W, H, npix;
+  float invW, invH;
+  prt_frame_dev fb;
+  // renderer
+  uint32_t seed, sampleIndex;
+  int maxDepth, rrDepth;
+  float minWeight, maxRadiance;
+  // path state, indexed by path slot (8x8 pixel tiles, 64 consecutive slots per tile)
+  float4* rayO;   // origin, tnear
+  float4* rayD;   // direction, unused
+  float4* hit;    // t, u, v, prim bits
+  float4* TP;     // throughput, lastBsdfPdf
+  float4* Lr;     // radiance, unused
+  int* flags;     // lastBsdfType | AOV flags | direct/terminate flags | medium << 16
+  uint32_t* hash; // per-path event hash (debug), may be null
+  float4* shO;    // shadow ray origin, tnear
+  float4* shD;    // shadow ray direction, tfar
+  float4* dirL;   // unshadowed direct light
+  int* queue[2];
+  int* shadowQ;
+  int* counters;  // [2*d] paths queued for depth d (d >= 1), [2*d+1] shadow rays of depth d
+  int* spill;     // traversal stack overflow, (PRT_MAX_STACK - PRT_LDS_STACK) x spillStride
+  int spillStride;
+  unsigned long long* stats;  // closest hit: [0] rays, [1] inner-node visits, [2] triangle tests; shadow: [3..5]
+};
+
+// Kernel launchers (render.hip)
+struct LaunchCfg
+{
+  int gridTrace;  // blocks of the persistent traversal kernels
+  int gridShade;  // blocks of the persistent shading kernel
+};
+void launchGenerate(const RenderArgs& a, hipStream_t s);
+void launchIntersect(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s);
+void launchShade(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s);
+void launchOccluded(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s);
+void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
+                 int* spill, int spillStride, int grid, hipStream_t s);
+void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s);
+
+constexpr int kTraceBlock = 256;
+constexpr int kShadeBlock = 256;
+
+// Slot <-> pixel mapping: slot = (tile * 64) + (ly * 8 + lx), tiles row-major over W/8 columns.
+PRT_DEV void slotToPixel(int slot, int W, int& x, int& y)
+{
+  int tile = slot >> 6, within = slot & 63;
+  int tilesX = W >> 3;
+  x = (tile % tilesX) * 8 + (within & 7);
+  y = (tile / tilesX) * 8 + (within >> 3);
+}
+
+}  // namespace prt

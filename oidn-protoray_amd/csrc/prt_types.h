@@ -1,0 +1,99 @@
+// Plain-old-data records shared by the host driver and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+#define PRT_KIND_DIFFUSE 0
+#define PRT_KIND_CONDUCTOR 1
+#define PRT_KIND_DIELECTRIC 2
+
+#define PRT_LIGHT_AMBIENT 0
+#define PRT_LIGHT_DISTANT 1
+#define PRT_LIGHT_IMAGE 2
+#define PRT_MAX_LIGHTS 8
+
+// Energy-compensation LUTs, one flat float array (bsdf/albedo_tables.h:14-157):
+//   MicrofacetAlbedoTable dir [32 roughness][32 cos], avg [32]
+//   MicrofacetDielectricAlbedoTable etaDir [16 rough][16 eta][16 cos] (eta in [1/3,1]), etaAvg
+//   [16][16], rcpEtaDir (eta in [1,3]), rcpEtaAvg
+#define PRT_LUT_GGX_DIR 0
+#define PRT_LUT_GGX_AVG 1024
+#define PRT_LUT_DIEL_ETA_DIR 1056
+#define PRT_LUT_DIEL_ETA_AVG 5152
+#define PRT_LUT_DIEL_RCPETA_DIR 5408
+#define PRT_LUT_DIEL_RCPETA_AVG 9504
+#define PRT_LUT_FLOATS 9760
+
+// Per-material constants derived on the host at upload (the per-hit constructor work of
+// RoughConductorBrdfSimd, rough_conductor.h:121-130, and RoughDielectricBsdfSimd,
+// rough_dielectric.h:279-291, depends on the material only).  128 bytes.
+struct prt_material_constants
+{
+  int32_t kind;
+  float roughness;   // clamped specularRoughness
+  float alpha;       // max(roughness^2, 0.001)
+  float importance;  // MultiBsdf importance of the single lobe (diffuse: max(color))
+  float c0[3];       // diffuse color | conductor eta
+  float c1[3];       // conductor k
+  float c2[3];       // conductor fmsScale
+  float rr[3];       // conductor reflectivity (albedo)
+  float Eavg;        // conductor average albedo
+  float eta[2];      // dielectric eta seen from outside [0] / inside [1]
+  float EavgEta[2];
+  float EavgRcpEta[2];
+  float fmsRatio[2];
+  int32_t mediumInt;  // interior medium ID (0 for non-dielectrics)
+  int32_t pad[6];
+};
+
+// Environment light record as the kernels read it.  64 bytes.
+struct prt_light_dev
+{
+  int32_t kind;
+  float r, g, b;
+  float dx, dy, dz;
+  float cosAngle;  // distant: cos(half angle)
+  float conePdf;   // distant: 1 / (4 pi sin^2(angle/2)) (sampling/shape_sampler.h:100-123)
+  float fu[3];     // distant: sampling frame U (math/basis3.h:173-180)
+  float fv[3];     // distant: sampling frame V
+  float pad;
+};
+
+// BVH node (64 bytes): the bounding boxes of both children (so one node fetch tests two
+// boxes) and the two child references.  child >= 0: inner node index; child < 0: leaf,
+// (child & 0x7fffffff) = first_triangle << 3 | (count - 1).
+struct prt_bvh_node
+{
+  float lox[2], hix[2];  // [child0, child1] ... packed as x: lo0, hi0, lo1, hi1
+  float loy[2], hiy[2];
+  float loz[2], hiz[2];
+  int32_t child[2];
+  int32_t pad[2];
+};
+
+#define PRT_LEAF_MAX 4
+#define PRT_LDS_STACK 32
+#define PRT_MAX_STACK 128
+
+// Path flags (per pixel, one int).
+#define PRT_F_LASTTYPE_MASK 0xff
+#define PRT_F_AOV1 (1 << 8)       // first-hit AOVs written  (path_renderer_stream.h:75)
+#define PRT_F_AOV2 (1 << 9)       // other AOVs written       (path_renderer_stream.h:76)
+#define PRT_F_DIRECT (1 << 10)    // shadow ray pending       (path_renderer_stream.h:74)
+#define PRT_F_TERMINATE (1 << 11) // path ends after the shadow test
+#define PRT_F_MEDIUM_SHIFT 16
+
+struct prt_frame_dev  // device pointers of the frame accumulators (FP32 sums, planar)
+{
+  float* color;   // 3 x npix
+  float* weight;  // npix
+  float* color2;  // 3 x npix (sum of squares) or null
+  float* albedo; float* albedoW;
+  float* albedo1; float* albedo1W;
+  float* normal; float* normalW;
+  float* normal1; float* normal1W;
+  float* depth; float* depthW;
+  float* dbgL;          // 3 x npix or null
+  uint32_t* dbgHash;    // npix or null
+  int32_t* dbgFirst;    // npix or null
+};

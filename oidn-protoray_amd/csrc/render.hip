@@ -1,0 +1,517 @@
+// Wavefront kernels of the stream path tracer on gfx950.
+//
+// One sample pass (render/path_renderer_stream.h:150-187 with renderTile :195-1073) is:
+//   kGenerate                  camera rays for every pixel            (:209-268)
+//   for depth = 0 .. maxDepth:
+//     kIntersect               closest hit of the queued paths         (:280)
+//     kShade                   miss / hit shading, NEE shadow ray,     (:285-415, :586-949)
+//                              BSDF sample, RR, compaction of survivors
+//     kOccluded                any hit of the shadow rays, unoccluded  (:951-1065)
+//                              direct light added, terminated paths accumulated
+// The reference works on 1024-path streams per 32x32 tile with a counting sort by material;
+// here all paths of the frame form one queue, compacted with wave-aggregated atomics, and
+// per-path state lives in HBM at a fixed slot per pixel (8x8 tiles).  Each pixel gets exactly
+// one sample per pass, so the FP32 accumulators are updated without atomics and in sample
+// order, as AccumBuffer3f::addSerial does (image/accum_buffer.h:439-486).
+#include "internal.h"
+
+namespace prt {
+
+namespace {
+
+PRT_DEV int laneId() { return __lane_id(); }
+
+// Reserves one queue entry per lane with pred set; returns the entry index (or -1).
+PRT_DEV int waveAppend(int* counter, bool pred)
+{
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return -1;
+  int leader = __ffsll((unsigned long long)mask) - 1;
+  int base = 0;
+  if (laneId() == leader) base = atomicAdd(counter, __popcll(mask));
+  base = __shfl(base, leader);
+  uint64_t below = mask & ((1ull << laneId()) - 1ull);
+  return pred ? base + __popcll(below) : -1;
+}
+
+PRT_DEV unsigned long long waveSum(unsigned long long v)
+{
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  return v;
+}
+
+PRT_DEV uint32_t hashMix(uint32_t h, uint32_t k)  // math/hash.h:12-30 (murmurHash3Mix)
+{
+  k *= 0xcc9e2d51u; k = (k << 15) | (k >> 17); k *= 0x1b873593u;
+  h ^= k; h = ((h << 13) | (h >> 19)) * 5u + 0xe6546b64u;
+  return h;
+}
+
+PRT_DEV f3 ld3(const float* p) { return f3(p[0], p[1], p[2]); }
+
+// Coarse direction cell (1/32 per axis) folded into the debug path hash.
+PRT_DEV uint32_t dirCell(f3 d)
+{
+  uint32_t qx = uint32_t(int(floorf(d.x * 32.f)) + 64) & 0xffu;
+  uint32_t qy = uint32_t(int(floorf(d.y * 32.f)) + 64) & 0xffu;
+  uint32_t qz = uint32_t(int(floorf(d.z * 32.f)) + 64) & 0xffu;
+  return qx | (qy << 8) | (qz << 16);
+}
+
+PRT_DEV void addAov3(float* buf, float* w, int npix, int pix, f3 c)
+{
+  if (!buf) return;
+  buf[pix] += c.x; buf[npix + pix] += c.y; buf[2 * npix + pix] += c.z;
+  w[pix] += 1.f;
+}
+
+// integrator.h:27-46
+PRT_DEV f3 clampL(f3 L, float maxRadiance)
+{
+  f3 c(0.f < L.x ? L.x : 0.f, 0.f < L.y ? L.y : 0.f, 0.f < L.z ? L.z : 0.f);
+  float y = luminance(c);
+  if (y > maxRadiance) c = c * (maxRadiance * (1.0f / y));
+  return c;
+}
+
+// Adds the path's radiance as this pass's sample of its pixel (path_renderer_stream.h:323,1026).
+PRT_DEV void finalizePath(const RenderArgs& a, int slot, f3 L, uint32_t hash)
+{
+  int x, y;
+  slotToPixel(slot, a.W, x, y);
+  int pix = y * a.W + x;
+  f3 c = clampL(L, a.maxRadiance);
+  const prt_frame_dev& fb = a.fb;
+  fb.color[pix] += c.x;
+  fb.color[a.npix + pix] += c.y;
+  fb.color[2 * a.npix + pix] += c.z;
+  fb.weight[pix] += 1.f;
+  if (fb.color2)
+  {
+    fb.color2[pix] += c.x * c.x;
+    fb.color2[a.npix + pix] += c.y * c.y;
+    fb.color2[2 * a.npix + pix] += c.z * c.z;
+  }
+  if (fb.dbgL)
+  {
+    fb.dbgL[pix] = c.x; fb.dbgL[a.npix + pix] = c.y; fb.dbgL[2 * a.npix + pix] = c.z;
+    fb.dbgHash[pix] = hash;
+  }
+}
+
+// render/scene.h:137-157
+PRT_DEV f3 evalEnv(const RenderArgs& a, f3 wi, float& pdf, uint32_t& hitMask)
+{
+  f3 v(0.f);
+  pdf = 0.f;
+  hitMask = 0;
+  for (int i = 0; i < a.nLights; ++i)
+  {
+    float cp;
+    v = v + envEvalOne(a.lights[i], a.env, wi, cp);
+    if (cp > 0.f) hitMask |= 1u << i;
+    pdf += cp * (a.lightCdf[i + 1] - a.lightCdf[i]);
+  }
+  return v;
+}
+
+// render/scene.h:215-250 (Distribution1D::sampleReuse for the pick, s.c remapped)
+PRT_DEV f3 sampleLight(const RenderArgs& a, f3& wi, float& pdf, float sx, float sy, float sc, int& picked)
+{
+  picked = 0;
+  if (a.nLights == 1) return envSampleOne(a.lights[0], a.env, wi, pdf, sx, sy);
+  float pick;
+  int id = dist1dSampleReuse(a.lightCdf, a.nLights, pick, sc);
+  picked = id;
+  f3 w = envSampleOne(a.lights[id], a.env, wi, pdf, sx, sy);
+  pdf *= pick;
+  return w * (1.0f / pick);
+}
+
+// geometry/triangle_mesh.cpp:152-213 + render/scene.h:61-72
+PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, float u, float v)
+{
+  Ctx c;
+  c.p = o + t * d;
+  c.eps = fmaxf(t, fmaxf(fmaxf(fabsf(c.p.x), fabsf(c.p.y)), fabsf(c.p.z))) * 0x1.fp-18f;
+  int i0 = a.idx[3 * prim], i1 = a.idx[3 * prim + 1], i2 = a.idx[3 * prim + 2];
+  f3 p0 = ld3(a.pos + 3 * i0), p1 = ld3(a.pos + 3 * i1), p2 = ld3(a.pos + 3 * i2);
+  f3 Ng = normalize(cross(p1 - p0, p2 - p0));
+  f3 N = Ng;
+  if (a.nrm)
+  {
+    float b0 = 1.f - u - v;
+    N = normalize(b0 * ld3(a.nrm + 3 * i0) + u * ld3(a.nrm + 3 * i1) + v * ld3(a.nrm + 3 * i2));
+    if (dot(N, Ng) < 0.f) N = -N;
+  }
+  makeFrame(c.f.U, c.f.V, N);
+  if (dot(Ng, d) > 0.f) { Ng = -Ng; N = -N; }
+  c.Ng = Ng;
+  c.f.N = N;
+  return c;
+}
+
+__global__ void __launch_bounds__(256) kGenerate(RenderArgs a)
+{
+  int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k < 2 * (a.maxDepth + 2); k += blockDim.x) a.counters[k] = 0;
+  if (slot >= a.npix) return;
+  int x, y;
+  slotToPixel(slot, a.W, x, y);
+  uint32_t pixelId = uint32_t(x) | (uint32_t(y) << 16);  // path_renderer_stream.h:219-220
+  Sampler smp;
+  smp.set(a.seed, a.sampleIndex, pixelId);
+  float psx, psy, lsx, lsy;
+  smp.get2D(0, psx, psy);  // sampleDimPixel (integrator.h:52)
+  smp.get2D(2, lsx, lsy);  // sampleDimLens (integrator.h:53)
+  float imx = (float(x) + psx) * a.invW, imy = (float(y) + psy) * a.invH;
+  f3 imageO = ld3(a.imageO), imageDx = ld3(a.imageDx), imageDy = ld3(a.imageDy);
+  f3 org = ld3(a.camPos);
+  f3 dir;
+  float tnear;
+  if (a.pinhole)  // camera/pinhole_camera.h:128-133
+  {
+    dir = imageO + imx * imageDx + imy * imageDy;
+    float len = length(dir);
+    dir = dir * (1.0f / len);
+    tnear = a.nearClip * len;
+  }
+  else  // camera/thin_lens_camera.h:175-182, sampling/shape_sampler.h:169-179
+  {
+    float r = sqrtf(lsx), sp, cp;
+    sincosf(2.f * kPi * lsy, &sp, &cp);
+    float lx = r * cp * a.lensRadius, ly = r * sp * a.lensRadius;
+    f3 off = lx * ld3(a.camU) + ly * ld3(a.camV) + 0.f * ld3(a.camN);
+    dir = a.focal * (imageO + imx * imageDx + imy * imageDy) - off;
+    float len = length(dir);
+    org = org + off;
+    dir = dir * (1.0f / len);
+    tnear = a.nearClipScale * len;
+  }
+  a.rayO[slot] = make_float4(org.x, org.y, org.z, tnear);
+  a.rayD[slot] = make_float4(dir.x, dir.y, dir.z, 0.f);
+  a.TP[slot] = make_float4(1.f, 1.f, 1.f, 1.f);  // throughput 1, lastBsdfPdf 1
+  a.Lr[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+  a.flags[slot] = BsdfType::SpecularReflection;  // lastBsdfType, medium 0, no AOVs yet
+  if (a.hash) a.hash[slot] = 0x9e3779b9u;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) kIntersect(RenderArgs a, int depth)
+{
+  __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
+  const int wave = threadIdx.x >> 6;
+  TraceStack st;
+  st.lds = lds + wave * (PRT_LDS_STACK * 64) + laneId();
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = a.spill + gtid;
+  st.stride = a.spillStride;
+  const int count = depth == 0 ? a.npix : a.counters[2 * depth];
+  const int* q = a.queue[depth & 1];
+  uint32_t nodes = 0, tris = 0, rays = 0;
+  for (int i = gtid; i < count; i += gridDim.x * blockDim.x)
+  {
+    int slot = depth == 0 ? i : q[i];
+    float4 ro = a.rayO[slot], rd = a.rayD[slot];
+    float t, u = 0.f, v = 0.f;
+    int prim = traceRay<false>(a.bvh, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, st, nodes, tris);
+    a.hit[slot] = make_float4(t, u, v, __int_as_float(prim));
+    ++rays;
+  }
+  unsigned long long r = waveSum(rays), nv = waveSum(nodes), tt = waveSum(tris);
+  if (laneId() == 0 && r)
+  {
+    atomicAdd(&a.stats[0], r);
+    atomicAdd(&a.stats[1], nv);
+    atomicAdd(&a.stats[2], tt);
+  }
+}
+
+__global__ void __launch_bounds__(kShadeBlock) kShade(RenderArgs a, int depth)
+{
+  const int count = depth == 0 ? a.npix : a.counters[2 * depth];
+  const int* qIn = a.queue[depth & 1];
+  int* qOut = a.queue[(depth + 1) & 1];
+  int* cntOut = a.counters + 2 * (depth + 1);
+  int* cntShadow = a.counters + 2 * depth + 1;
+  const int sampleDim = 4 + 7 * depth;  // sampleDimBaseSize + sampleDimBounceSize * depth
+  const int stride = gridDim.x * blockDim.x;
+  // all lanes of a wave run the same number of iterations so the appends stay converged
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int waveBase = gtid - laneId();
+  for (int base = waveBase; base < count; base += stride)
+  {
+    const int i = base + laneId();
+    const bool active = i < count;
+    bool wantExt = false, wantShadow = false;
+    int slot = 0;
+    if (active)
+    {
+      slot = depth == 0 ? i : qIn[i];
+      int x, y;
+      slotToPixel(slot, a.W, x, y);
+      const int pix = y * a.W + x;
+      const uint32_t pixelId = uint32_t(x) | (uint32_t(y) << 16);
+      float4 h = a.hit[slot];
+      int prim = __float_as_int(h.w);
+      float4 ro = a.rayO[slot], rd = a.rayD[slot], tp = a.TP[slot], lr = a.Lr[slot];
+      f3 o(ro.x, ro.y, ro.z), d(rd.x, rd.y, rd.z), T(tp.x, tp.y, tp.z), L(lr.x, lr.y, lr.z);
+      float lastPdf = tp.w;
+      int fl = a.flags[slot];
+      int lastType = fl & PRT_F_LASTTYPE_MASK;
+      uint32_t hsh = a.hash ? hashMix(a.hash[slot], uint32_t(prim + 1)) : 0u;
+      if (depth == 0 && a.fb.dbgFirst) a.fb.dbgFirst[pix] = prim;
+      int matId = prim >= 0 ? a.triMat[prim] : 0;
+      fl &= ~(PRT_F_DIRECT | PRT_F_TERMINATE);
+      if (matId == 0)
+      {
+        // miss (path_renderer_stream.h:285-415): environment light with MIS
+        float lightPdf;
+        uint32_t envMask;
+        f3 Le = evalEnv(a, d, lightPdf, envMask);
+        if (a.hash) hsh = hashMix(hsh, 0x100u | envMask);
+        float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
+        L = L + T * w * Le;
+        if (!(fl & PRT_F_AOV1))
+        {
+          addAov3(a.fb.albedo1, a.fb.albedo1W, a.npix, pix, f3(0.f));
+          addAov3(a.fb.normal1, a.fb.normal1W, a.npix, pix, f3(0.f));
+          if (a.fb.depth) { a.fb.depth[pix] += __int_as_float(0x7f800000); a.fb.depthW[pix] += 1.f; }
+        }
+        if (!(fl & PRT_F_AOV2))
+        {
+          addAov3(a.fb.albedo, a.fb.albedoW, a.npix, pix, f3(0.f));
+          addAov3(a.fb.normal, a.fb.normalW, a.npix, pix, f3(0.f));
+        }
+        finalizePath(a, slot, L, hsh);
+      }
+      else if (depth >= a.maxDepth)
+      {
+        finalizePath(a, slot, L, hsh);  // no shading beyond maxDepth (:615)
+      }
+      else
+      {
+        const prt_material_constants& m = a.mats[matId];
+        Ctx ctx = postIntersect(a, o, d, h.x, prim, h.y, h.z);
+        f3 wo = -d;
+        int mediumId = fl >> PRT_F_MEDIUM_SHIFT;
+        bool inside = (m.kind == PRT_KIND_DIELECTRIC) && (mediumId == m.mediumInt);
+        Bsdf b = makeBsdf(m, ctx, wo, inside);
+        Sampler smp;
+        smp.set(a.seed, a.sampleIndex, pixelId);  // resetSample (:641-642)
+        bool mAov1 = !(fl & PRT_F_AOV1), mAov2 = !(fl & PRT_F_AOV2);
+        bool mRR = !(mAov1 || mAov2);
+        // direct light (:659-718); every supported lobe is Smooth
+        f3 dirL(0.f);
+        {
+          float s0, s1, s2;
+          smp.get3D(sampleDim + 3, s0, s1, s2);
+          f3 wiL;
+          float lightPdf;
+          int picked;
+          f3 lw = sampleLight(a, wiL, lightPdf, s0, s1, s2, picked);
+          if (a.hash) hsh = hashMix(hashMix(hsh, 0x200u | uint32_t(picked)), dirCell(wiL));
+          if (reduceMax(lw) > 0.f && lightPdf > 0.f)
+          {
+            float bsdfPdf;
+            f3 f = bsdfEval(a.lut, b, ctx, wo, wiL, bsdfPdf);
+            if (reduceMax(f) > 0.f)
+            {
+              f3 dirT = T * f;
+              if (depth >= a.rrDepth)
+              {
+                float contProb = mRR ? fminf(luminance(dirT * (1.0f / bsdfPdf)), 0.95f) : 1.f;
+                bsdfPdf *= contProb;
+              }
+              float w = misWeight(lightPdf, bsdfPdf);
+              dirL = dirT * w * lw;
+              a.shO[slot] = make_float4(ctx.p.x, ctx.p.y, ctx.p.z, ctx.eps);
+              a.shD[slot] = make_float4(wiL.x, wiL.y, wiL.z, kPosMax - ctx.eps);
+              a.dirL[slot] = make_float4(dirL.x, dirL.y, dirL.z, 0.f);
+              wantShadow = true;
+              fl |= PRT_F_DIRECT;
+            }
+          }
+        }
+        // BSDF sample (:720-730)
+        float s0, s1, s2;
+        smp.get3D(sampleDim + 0, s0, s1, s2);
+        f3 wi(0.f);
+        float pdf = 0.f;
+        int type = 0;
+        f3 bw = bsdfSample(a.lut, b, ctx, wo, wi, pdf, type, s0, s1, s2);
+        bool ext = reduceMax(bw) > 0.f && pdf > 0.f;
+        int extType = ext ? type : 0;
+        if (a.hash)
+        {
+          hsh = hashMix(hsh, uint32_t(extType));
+          if (ext) hsh = hashMix(hsh, dirCell(wi));
+        }
+        if (mAov1)  // first-hit AOVs (:739-819)
+        {
+          addAov3(a.fb.albedo1, a.fb.albedo1W, a.npix, pix, bsdfAlbedo(b));
+          addAov3(a.fb.normal1, a.fb.normal1W, a.npix, pix, b.fr.N);
+          if (a.fb.depth)
+          {
+            a.fb.depth[pix] += dot(ctx.p - ld3(a.camPos), -ld3(a.camN));
+            a.fb.depthW[pix] += 1.f;
+          }
+          fl |= PRT_F_AOV1;
+        }
+        mAov2 = mAov2 && (extType & BsdfType::Delta) == 0;  // (:828-832)
+        if (mAov2)
+        {
+          addAov3(a.fb.albedo, a.fb.albedoW, a.npix, pix, bsdfAlbedo(b));
+          addAov3(a.fb.normal, a.fb.normalW, a.npix, pix, b.fr.N);
+          fl |= PRT_F_AOV2;
+        }
+        T = T * bw;
+        ext = ext && reduceMax(T) >= a.minWeight;
+        if (depth >= a.rrDepth)  // Russian roulette (:880-887)
+        {
+          float contProb = mRR ? fminf(luminance(T), 0.95f) : 1.f;
+          ext = ext && smp.get1D(sampleDim + 6) <= contProb;
+          T = T * (1.0f / contProb);
+          pdf *= contProb;
+        }
+        if (ext)
+        {
+          // next ray (:892-912)
+          a.rayO[slot] = make_float4(ctx.p.x, ctx.p.y, ctx.p.z, ctx.eps);
+          a.rayD[slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
+          if ((type & BsdfType::Continuation) == 0) { lastPdf = pdf; lastType = type; }
+          if (m.kind == PRT_KIND_DIELECTRIC && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
+            mediumId = (mediumId == m.mediumInt) ? 0 : m.mediumInt;
+          a.TP[slot] = make_float4(T.x, T.y, T.z, lastPdf);
+          wantExt = true;
+        }
+        fl = (fl & ~(PRT_F_LASTTYPE_MASK | (0xffff << PRT_F_MEDIUM_SHIFT))) | (lastType & PRT_F_LASTTYPE_MASK) |
+             (mediumId << PRT_F_MEDIUM_SHIFT);
+        if (!ext)
+        {
+          if (wantShadow) fl |= PRT_F_TERMINATE;  // accumulate after the shadow test
+          else finalizePath(a, slot, L, hsh);
+        }
+      }
+      a.flags[slot] = fl;
+      if (a.hash) a.hash[slot] = hsh;
+    }
+    int e = waveAppend(cntOut, wantExt);
+    if (wantExt) qOut[e] = slot;
+    int s = waveAppend(cntShadow, wantShadow);
+    if (wantShadow) a.shadowQ[s] = slot;
+  }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) kOccluded(RenderArgs a, int depth)
+{
+  __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
+  const int wave = threadIdx.x >> 6;
+  TraceStack st;
+  st.lds = lds + wave * (PRT_LDS_STACK * 64) + laneId();
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = a.spill + gtid;
+  st.stride = a.spillStride;
+  const int count = a.counters[2 * depth + 1];
+  uint32_t nodes = 0, tris = 0, rays = 0;
+  for (int i = gtid; i < count; i += gridDim.x * blockDim.x)
+  {
+    int slot = a.shadowQ[i];
+    float4 so = a.shO[slot], sd = a.shD[slot];
+    float t, u, v;
+    int prim = traceRay<true>(a.bvh, f3(so.x, so.y, so.z), f3(sd.x, sd.y, sd.z), so.w, sd.w, t, u, v, st, nodes, tris);
+    ++rays;
+    float4 lr = a.Lr[slot];
+    f3 L(lr.x, lr.y, lr.z);
+    if (prim < 0)  // unoccluded: add the direct light (:991-1002)
+    {
+      float4 dl = a.dirL[slot];
+      L = L + f3(dl.x, dl.y, dl.z);
+      a.Lr[slot] = make_float4(L.x, L.y, L.z, 0.f);
+    }
+    uint32_t hsh = 0u;
+    if (a.hash)  // the visibility event is part of the path's identity
+    {
+      hsh = hashMix(a.hash[slot], prim < 0 ? 1u : 2u);
+      a.hash[slot] = hsh;
+    }
+    if (a.flags[slot] & PRT_F_TERMINATE) finalizePath(a, slot, L, hsh);
+  }
+  unsigned long long r = waveSum(rays), nv = waveSum(nodes), tt = waveSum(tris);
+  if (laneId() == 0 && r)
+  {
+    atomicAdd(&a.stats[3], r);
+    atomicAdd(&a.stats[4], nv);
+    atomicAdd(&a.stats[5], tt);
+  }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) kQuery(BvhView bvh, int n, const float* __restrict__ rays, int* __restrict__ prim,
+                                                      float* __restrict__ tOut, float* __restrict__ uOut, float* __restrict__ vOut,
+                                                      int anyHit, int* spill, int spillStride)
+{
+  __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
+  const int wave = threadIdx.x >> 6;
+  TraceStack st;
+  st.lds = lds + wave * (PRT_LDS_STACK * 64) + laneId();
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = spill + gtid;
+  st.stride = spillStride;
+  uint32_t nodes = 0, tris = 0;
+  for (int i = gtid; i < n; i += gridDim.x * blockDim.x)
+  {
+    const float* r = rays + 8 * i;
+    f3 o(r[0], r[1], r[2]), d(r[3], r[4], r[5]);
+    float t, u = 0.f, v = 0.f;
+    int p = anyHit ? traceRay<true>(bvh, o, d, r[6], r[7], t, u, v, st, nodes, tris)
+                   : traceRay<false>(bvh, o, d, r[6], r[7], t, u, v, st, nodes, tris);
+    prim[i] = p;
+    if (tOut) tOut[i] = p >= 0 ? t : r[7];
+    if (uOut) uOut[i] = u;
+    if (vOut) vOut[i] = v;
+  }
+}
+
+__global__ void kDebugSampler(int n, const uint32_t* __restrict__ in, int ndim, float* o1, float* o2, float* o3)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Sampler s;
+  s.set(in[3 * i], in[3 * i + 2], in[3 * i + 1]);
+  for (int d = 0; d < ndim; ++d)
+  {
+    o1[i * ndim + d] = s.get1D(d);
+    s.get2D(d, o2[(i * ndim + d) * 2], o2[(i * ndim + d) * 2 + 1]);
+    s.get3D(d, o3[(i * ndim + d) * 3], o3[(i * ndim + d) * 3 + 1], o3[(i * ndim + d) * 3 + 2]);
+  }
+}
+
+}  // namespace
+
+void launchGenerate(const RenderArgs& a, hipStream_t s)
+{
+  kGenerate<<<(a.npix + 255) / 256, 256, 0, s>>>(a);
+}
+void launchIntersect(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s)
+{
+  kIntersect<<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+}
+void launchShade(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s)
+{
+  kShade<<<c.gridShade, kShadeBlock, 0, s>>>(a, depth);
+}
+void launchOccluded(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s)
+{
+  kOccluded<<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+}
+void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit, int* spill,
+                 int spillStride, int grid, hipStream_t s)
+{
+  kQuery<<<grid, kTraceBlock, 0, s>>>(bvh, n, rays, prim, t, u, v, anyHit, spill, spillStride);
+}
+void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s)
+{
+  kDebugSampler<<<(n + 63) / 64, 64, 0, s>>>(n, in, ndim, o1, o2, o3);
+}
+
+}  // namespace prt

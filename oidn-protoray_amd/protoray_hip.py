@@ -1,0 +1,268 @@
+"""Python binding of the MI355X stream path tracer (libprt_hip.so, C ABI in include/prt_hip.h).
+
+``DeviceHip`` mirrors the reference's ``Device`` interface (render/device.h:56-114) for the
+stream path renderer: initScene / initLights / initRenderer / initSampler / initCamera /
+initFrame / clearFrame / render / blitFrame / getFrameError / queryRay.  Arguments keep the
+reference's meaning (Props names in the docstrings); errors raise like the reference's C++
+exceptions (ValueError for std::invalid_argument / std::logic_error, RuntimeError otherwise).
+
+The library is loaded from this directory; there is no fallback: without the built HIP
+library (``make -C oidn-protoray_amd``) every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libprt_hip.so")
+
+PRT_OK, PRT_ERR_INVALID, PRT_ERR_HIP, PRT_ERR_NOMEM, PRT_ERR_STATE = 0, -1, -2, -3, -4
+AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR = 1, 2, 4, 8, 16, 32
+BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
+LUT_FLOATS = 9760
+
+MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
+                           ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
+                           ("ior", "<f4"), ("pad", "<i4", 2)])
+LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
+                        ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
+CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
+                         ("fov", "<f4"), ("nearClip", "<f4"), ("lensRadius", "<f4"),
+                         ("focalDistance", "<f4")])
+
+
+class PrtStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_double), ("ms", ctypes.c_double), ("mray", ctypes.c_double),
+                ("spp", ctypes.c_int32)]
+
+
+class RendererParams(ctypes.Structure):
+    _fields_ = [("maxDepth", ctypes.c_int32), ("rrDepth", ctypes.c_int32), ("minWeight", ctypes.c_float),
+                ("maxRadiance", ctypes.c_float)]
+
+
+_lib = None
+
+# (name, argtypes) of every entry point in include/prt_hip.h
+_vp, _i, _u, _f = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_float
+SIGNATURES = {
+    "prt_context_create": [_i, ctypes.POINTER(ctypes.c_void_p)],
+    "prt_context_destroy": [_vp],
+    "prt_last_error": [],
+    "prt_device_synchronize": [_vp],
+    "prt_get_info": [_vp, ctypes.c_char_p, _i],
+    "prt_scene_set_geometry": [_vp, _i, _vp, _vp, _i, _vp, _vp],
+    "prt_scene_set_materials": [_vp, _i, _vp],
+    "prt_scene_set_lights": [_vp, _i, _vp, _i, _i, _vp, _i],
+    "prt_bvh_build": [_vp, ctypes.POINTER(ctypes.c_double)],
+    "prt_renderer_init": [_vp, _vp],
+    "prt_sampler_init": [_vp, _u],
+    "prt_set_camera": [_vp, _vp],
+    "prt_frame_init": [_vp, _i, _i, _u],
+    "prt_frame_clear": [_vp],
+    "prt_render": [_vp, _i, ctypes.POINTER(PrtStats)],
+    "prt_blit": [_vp, _i, _vp, _vp],
+    "prt_get_accum": [_vp, _vp, _vp],
+    "prt_frame_error": [_vp, ctypes.POINTER(ctypes.c_float)],
+    "prt_query_rays": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "prt_query_occlusion": [_vp, _i, _vp, _vp],
+    "prt_query_rays_device": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "prt_debug_enable": [_vp, _i],
+    "prt_debug_get": [_vp, _vp, _vp, _vp],
+    "prt_debug_sampler": [_vp, _i, _vp, _i, _vp, _vp, _vp],
+    "prt_debug_luts": [_vp, _vp],
+    "prt_debug_bvh_stats": [_vp, _vp],
+    "prt_debug_kernel_ms": [_vp, _vp, _vp, _vp],
+}
+
+
+def load_library(path=LIB_PATH):
+    """Loads libprt_hip.so and declares every entry point; raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError("libprt_hip.so is not built (run `make -C oidn-protoray_amd`); "
+                               "there is no CPU fallback")
+        L = ctypes.CDLL(path)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "prt_last_error" else ctypes.c_int32
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != PRT_OK:
+        msg = _lib.prt_last_error().decode()
+        if rc in (PRT_ERR_INVALID, PRT_ERR_STATE):
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class DeviceHip:
+    """Device for the stream path renderer on one HIP device (cf. DeviceCpu)."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.prt_context_create(device, ctypes.byref(h)))
+        self.h = h
+        self.size = None
+        self.aov_mask = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.prt_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def getInfo(self):
+        buf = ctypes.create_string_buffer(256)
+        _check(_lib.prt_get_info(self.h, buf, 256))
+        return buf.value.decode()
+
+    # --- scene (Device::initScene, Scene ctor render/scene.cpp:65-134) -------------------
+    def initScene(self, positions, indices, tri_material, materials, normals=None):
+        self._pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        self._idx = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+        self._mat = np.ascontiguousarray(tri_material, np.int32)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        _check(_lib.prt_scene_set_geometry(self.h, self._pos.shape[0], _ptr(self._pos), _ptr(nrm),
+                                           self._idx.shape[0], _ptr(self._idx), _ptr(self._mat)))
+        m = np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        _check(_lib.prt_scene_set_materials(self.h, m.shape[0], _ptr(m)))
+        ms = ctypes.c_double(0)
+        _check(_lib.prt_bvh_build(self.h, ctypes.byref(ms)))
+        self.build_ms = ms.value
+        return self.build_ms
+
+    def initLights(self, lights, env_rgb=None, lut_size=128):
+        """lights: LIGHT_DTYPE records in reference order; env_rgb: H x W x 3, row 0 = nadir."""
+        l = np.ascontiguousarray(lights, LIGHT_DTYPE)
+        env = None if env_rgb is None else np.ascontiguousarray(env_rgb, np.float32)
+        H, W = (0, 0) if env is None else env.shape[:2]
+        _check(_lib.prt_scene_set_lights(self.h, l.shape[0], _ptr(l), W, H, _ptr(env), lut_size))
+
+    # --- renderer (RendererFactoryStream::make "pathStream") ------------------------------
+    def initRenderer(self, maxDepth=100, rrDepth=8, minWeight=1e-4, maxRadiance=3.4028234663852886e38):
+        p = RendererParams(maxDepth, rrDepth, minWeight, maxRadiance)
+        _check(_lib.prt_renderer_init(self.h, ctypes.byref(p)))
+
+    def initSampler(self, seed):
+        _check(_lib.prt_sampler_init(self.h, seed))
+
+    def initCamera(self, camera):
+        c = np.ascontiguousarray(camera, CAMERA_DTYPE)
+        _check(_lib.prt_set_camera(self.h, _ptr(c)))
+
+    def initFrame(self, width, height, aov_mask=0):
+        _check(_lib.prt_frame_init(self.h, width, height, aov_mask))
+        self.size = (width, height)
+        self.aov_mask = aov_mask
+
+    def clearFrame(self):
+        _check(_lib.prt_frame_clear(self.h))
+
+    def render(self, spp=1):
+        """Renders `spp` sample passes; returns the stats dict (rays, ms, mray, spp)."""
+        st = PrtStats()
+        _check(_lib.prt_render(self.h, spp, ctypes.byref(st)))
+        return {"rays": st.rays, "ms": st.ms, "mray": st.mray, "spp": st.spp}
+
+    def blitFrame(self, buffer=BUF_COLOR, variance=False):
+        W, H = self.size
+        ch = 1 if buffer == BUF_DEPTH else 3
+        out = np.zeros((H, W, ch) if ch == 3 else (H, W), np.float32)
+        var = np.zeros((H, W, 3), np.float32) if variance else None
+        _check(_lib.prt_blit(self.h, buffer, _ptr(out), _ptr(var)))
+        return (out, var) if variance else out
+
+    def getAccum(self):
+        W, H = self.size
+        s = np.zeros((3, W * H), np.float32)
+        w = np.zeros(W * H, np.float32)
+        _check(_lib.prt_get_accum(self.h, _ptr(s), _ptr(w)))
+        return s, w
+
+    def getFrameError(self):
+        e = ctypes.c_float(0)
+        _check(_lib.prt_frame_error(self.h, ctypes.byref(e)))
+        return e.value
+
+    # --- queries (Renderer::queryRay / queryOcclusionRay) ---------------------------------
+    def queryRays(self, rays):
+        """rays: n x 8 [o d near far] -> (prim, t, u, v)."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        prim = np.zeros(n, np.int32)
+        t, u, v = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        _check(_lib.prt_query_rays(self.h, n, _ptr(r), _ptr(prim), _ptr(t), _ptr(u), _ptr(v)))
+        return prim, t, u, v
+
+    def queryOcclusion(self, rays):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        occ = np.zeros(r.shape[0], np.int32)
+        _check(_lib.prt_query_occlusion(self.h, r.shape[0], _ptr(r), _ptr(occ)))
+        return occ
+
+    # --- debug / parity hooks ---------------------------------------------------------------
+    def enableDebug(self, enable=True):
+        _check(_lib.prt_debug_enable(self.h, 1 if enable else 0))
+
+    def debugOutputs(self):
+        W, H = self.size
+        L = np.zeros((3, W * H), np.float32)
+        hsh = np.zeros(W * H, np.uint32)
+        first = np.zeros(W * H, np.int32)
+        _check(_lib.prt_debug_get(self.h, _ptr(L), _ptr(hsh), _ptr(first)))
+        return L, hsh, first
+
+    def sampler(self, seed_pid_idx, ndim):
+        q = np.ascontiguousarray(seed_pid_idx, np.uint32)
+        n = q.shape[0]
+        o1, o2, o3 = (np.zeros((n, ndim), np.float32), np.zeros((n, ndim, 2), np.float32),
+                      np.zeros((n, ndim, 3), np.float32))
+        _check(_lib.prt_debug_sampler(self.h, n, _ptr(q), ndim, _ptr(o1), _ptr(o2), _ptr(o3)))
+        return o1, o2, o3
+
+    def luts(self):
+        out = np.zeros(LUT_FLOATS, np.float32)
+        _check(_lib.prt_debug_luts(self.h, _ptr(out)))
+        return out
+
+    def bvhStats(self):
+        s = np.zeros(4, np.int64)
+        _check(_lib.prt_debug_bvh_stats(self.h, _ptr(s)))
+        return {"inner_nodes": int(s[0]), "triangles": int(s[1]), "max_depth": int(s[2])}
+
+    def synchronize(self):
+        _check(_lib.prt_device_synchronize(self.h))
+
+    def kernelTimes(self):
+        """Per-kernel-class device time of the last render call (HIP events) and the traversal
+        kernels' counters (rays, inner-node visits, triangle tests)."""
+        ms = np.zeros(4, np.float64)
+        cn = np.zeros(6, np.float64)
+        ln = np.zeros(2, np.float64)
+        _check(_lib.prt_debug_kernel_ms(self.h, _ptr(ms), _ptr(cn), _ptr(ln)))
+        return {"generate_ms": ms[0], "intersect_ms": ms[1], "shade_ms": ms[2], "occluded_ms": ms[3],
+                "rays": cn[0], "node_visits": cn[1], "tri_tests": cn[2],
+                "shadow_rays": cn[3], "shadow_node_visits": cn[4], "shadow_tri_tests": cn[5],
+                "intersect_launches": ln[0], "occluded_launches": ln[1]}
+
+
+def load_scene(dev, scene):
+    """Uploads a scene dict from scenes/scenegen.py and configures renderer, camera and frame."""
+    dev.initScene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"])
+    dev.initLights(scene["lights"], scene.get("env"))
+    dev.initRenderer(maxDepth=scene["max_depth"])
+    dev.initSampler(scene.get("seed", 0))
+    dev.initCamera(scene["camera"])

@@ -1,0 +1,223 @@
+"""Seeded synthetic scenes for the stream path tracer's workloads.
+
+Two generators, both deterministic for a seed (PCG32, the generator of math/random.h:17-60):
+
+* ``spheres_ambient`` -- a 40x40 ground quad at y = -1 plus N icospheres with centres uniform in
+  [-10,10]x[0,8]x[5,30] and radii U[0.2,1.0], all diffuse with albedo U[0.1,0.9]^3, lit by a
+  constant ambient light of radiance 1.  512x512, 16 spp, max depth 5, pinhole camera
+  (vertical fov 45 deg) at the origin looking +z.  Defaults: 256 spheres at subdivision 3.
+* ``spheres_sunsky`` -- the same generator with 4096 icospheres at subdivision 4
+  (20,971,522 triangles), materials drawn per sphere from a seeded palette (40 % diffuse,
+  30 % rough GGX conductor with gold-like eta/k, 30 % rough GGX dielectric of IOR 1.5), lit by a
+  sun/sky (elevation 35 deg, turbidity 3) plus one distant light; 1920x1080, 64 spp, max
+  depth 8, thin-lens camera (lens radius 0.05, focus distance 15).
+
+Choices the workload description leaves open, fixed here:
+  * the ground quad spans x in [-20,20], z in [-2.5,37.5] (centred under the spheres), faces +y,
+    diffuse albedo 0.5;
+  * "GGX alpha a" maps to roughness sqrt(a) because the reference uses alpha = roughness^2
+    (bsdf/microfacet.h:10-15);
+  * gold eta/k (RGB) are drawn within +-10 % of (0.143, 0.374, 1.442) / (3.983, 2.385, 1.603) and
+    converted to the artistic reflectivity / edge tint the reference's conductor takes by
+    inverting Gulbrandsen's mapping (bsdf/optics.h:214-226);
+  * the material palette has 1024 entries (the reference's stream sort holds at most 2048
+    material IDs, render/renderer_stream.h:110);
+  * the distant light has radiance (300, 280, 250), direction normalize(-0.6, 0.7, -0.4) and an
+    angular diameter of 2 deg; the sun azimuth is 30 deg (reference convention atan2(x, -z));
+  * lens radius = 0.05 (the "aperture").
+Material ID 0 is reserved for the environment (render/scene.cpp:204-207); triangles use IDs >= 1.
+"""
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
+                           ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
+                           ("ior", "<f4"), ("pad", "<i4", 2)])
+LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
+                        ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
+CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
+                         ("fov", "<f4"), ("nearClip", "<f4"), ("lensRadius", "<f4"),
+                         ("focalDistance", "<f4")])
+KIND_DIFFUSE, KIND_CONDUCTOR, KIND_DIELECTRIC = 0, 1, 2
+LIGHT_AMBIENT, LIGHT_DISTANT, LIGHT_IMAGE = 0, 1, 2
+
+
+class Pcg32:
+    """PCG-XSH-RR 64/32 (O'Neill 2014), seeded like math/random.h:29-36."""
+
+    MULT = 6364136223846793005
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed=0xcafef00dd15ea5e5, seq=1442695040888963407 >> 1):
+        self.inc = ((seq << 1) | 1) & self.MASK
+        self.state = (seed + self.inc) & self.MASK
+        self.state = (self.state * self.MULT + self.inc) & self.MASK
+
+    def u32(self):
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & self.MASK
+        xs = (((old >> 18) ^ old) >> 27) & 0xffffffff
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xffffffff
+
+    def f(self, a=0.0, b=1.0):
+        return a + (self.u32() * 2.0 ** -32) * (b - a)
+
+
+def icosphere(subdiv):
+    """Unit icosphere (float64): 10*4^s + 2 vertices, 20*4^s outward-wound triangles."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = f
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts)
+    F = np.array(faces, np.int64)
+    # make every triangle wind counter-clockwise seen from outside (cross product points out)
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[F].mean(axis=1)) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F
+
+
+def gold_artistic(rng):
+    """Gold-like eta/k (RGB, +-10 %) -> artistic reflectivity r and edge tint g."""
+    eta0, k0 = (0.143, 0.374, 1.442), (3.983, 2.385, 1.603)
+    r, g = [], []
+    for c in range(3):
+        n = eta0[c] * rng.f(0.9, 1.1)
+        k = k0[c] * rng.f(0.9, 1.1)
+        rc = ((n - 1) ** 2 + k ** 2) / ((n + 1) ** 2 + k ** 2)
+        rc = min(rc, 0.99)
+        nmin = (1 - rc) / (1 + rc)
+        nmax = (1 + math.sqrt(rc)) / (1 - math.sqrt(rc))
+        gc = (nmax - n) / (nmax - nmin)
+        r.append(rc)
+        g.append(min(max(gc, 0.0), 1.0))
+    return r, g
+
+
+def _material(kind, rgb=(0, 0, 0), edge=(0, 0, 0), roughness=0.0, ior=1.5):
+    m = np.zeros((), MATERIAL_DTYPE)
+    m["kind"] = kind
+    m["r"], m["g"], m["b"] = rgb
+    m["er"], m["eg"], m["eb"] = edge
+    m["roughness"] = roughness
+    m["ior"] = ior
+    return m
+
+
+def _camera(fov_deg=45.0, lens_radius=0.0, focal=1.0, near=0.01):
+    c = np.zeros((), CAMERA_DTYPE)
+    c["pos"] = (0, 0, 0)
+    # looking along +z: the view direction is -N (camera/pinhole_camera.h:86-89)
+    c["U"], c["V"], c["N"] = (-1, 0, 0), (0, 1, 0), (0, 0, -1)
+    c["fov"] = math.radians(fov_deg)
+    c["nearClip"] = near
+    c["lensRadius"] = lens_radius
+    c["focalDistance"] = focal
+    return c
+
+
+def _assemble(rng, n_spheres, subdiv, sphere_material):
+    Vs, Fs = icosphere(subdiv)
+    # ground quad y = -1, x in [-20, 20], z in [-2.5, 37.5], normal +y
+    gv = np.array([[-20, -1, -2.5], [20, -1, -2.5], [20, -1, 37.5], [-20, -1, 37.5]], np.float64)
+    gf = np.array([[0, 2, 1], [0, 3, 2]], np.int64)
+    centers = np.zeros((n_spheres, 3))
+    radii = np.zeros(n_spheres)
+    mat_ids = np.zeros(n_spheres, np.int64)
+    for i in range(n_spheres):
+        centers[i] = (rng.f(-10, 10), rng.f(0, 8), rng.f(5, 30))
+        radii[i] = rng.f(0.2, 1.0)
+        mat_ids[i] = sphere_material(i)
+    pos = np.concatenate([gv, (Vs[None] * radii[:, None, None] + centers[:, None, :]).reshape(-1, 3)])
+    idx = np.concatenate([gf, (Fs[None] + 4 + Vs.shape[0] * np.arange(n_spheres)[:, None, None]).reshape(-1, 3)])
+    tri_mat = np.concatenate([np.ones(2, np.int64), np.repeat(mat_ids, Fs.shape[0])])
+    return pos.astype(np.float32), idx.astype(np.int32), tri_mat.astype(np.int32)
+
+
+def spheres_ambient(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=16, max_depth=5):
+    rng = Pcg32(seed)
+    mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
+    sphere_mats = []
+
+    def pick(i):
+        sphere_mats.append(_material(KIND_DIFFUSE, (rng.f(0.1, 0.9), rng.f(0.1, 0.9), rng.f(0.1, 0.9))))
+        return 2 + i
+
+    pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, pick)
+    lights = np.zeros(1, LIGHT_DTYPE)
+    lights[0] = (LIGHT_AMBIENT, 1.0, 1.0, 1.0, 0, 0, 0, 0)
+    return dict(name="spheres_ambient", positions=pos, indices=idx, tri_material=tri_mat,
+                materials=np.array(mats + sphere_mats, MATERIAL_DTYPE), lights=lights, env=None,
+                camera=_camera(), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)
+
+
+def load_sunsky(path=os.path.join(HERE, "sunsky_e35_t3_az30.npz")):
+    z = np.load(path)
+    # the bake is stored zenith-first; the image light reads texture rows bottom-up
+    # (image/image_texture.h:396), so flip to nadir-first
+    return dict(sky=np.ascontiguousarray(z["sky"][::-1]), sun_radiance=z["sun_radiance"],
+                sun_radius=float(z["sun_radius"][0]), sun_dir=z["sun_dir"])
+
+
+def spheres_sunsky(seed=0, n_spheres=4096, subdiv=4, width=1920, height=1080, spp=64, max_depth=8, palette=1024):
+    rng = Pcg32(seed)
+    mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
+    for _ in range(palette):
+        u = rng.f()
+        if u < 0.4:
+            mats.append(_material(KIND_DIFFUSE, (rng.f(0.1, 0.9), rng.f(0.1, 0.9), rng.f(0.1, 0.9))))
+        elif u < 0.7:
+            alpha = rng.f(0.05, 0.5)
+            r, g = gold_artistic(rng)
+            mats.append(_material(KIND_CONDUCTOR, r, g, math.sqrt(alpha)))
+        else:
+            alpha = rng.f(0.01, 0.3)
+            mats.append(_material(KIND_DIELECTRIC, roughness=math.sqrt(alpha), ior=1.5))
+    pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, lambda i: 2 + rng.u32() % palette)
+    ss = load_sunsky()
+    d = np.array([-0.6, 0.7, -0.4])
+    d /= np.linalg.norm(d)
+    lights = np.zeros(3, LIGHT_DTYPE)
+    # render/scene.cpp:375-431 order: distant light, then the sun/sky pair (sun, sky)
+    lights[0] = (LIGHT_DISTANT, 300.0, 280.0, 250.0, d[0], d[1], d[2], 0.5 * math.radians(2.0))
+    sr = ss["sun_radiance"]
+    sd = ss["sun_dir"]
+    lights[1] = (LIGHT_DISTANT, sr[0], sr[1], sr[2], sd[0], sd[1], sd[2], ss["sun_radius"])
+    lights[2] = (LIGHT_IMAGE, 1.0, 1.0, 1.0, 0, 0, 0, 0)
+    return dict(name="spheres_sunsky", positions=pos, indices=idx, tri_material=tri_mat,
+                materials=np.array(mats, MATERIAL_DTYPE), lights=lights, env=ss["sky"],
+                camera=_camera(lens_radius=0.05, focal=15.0), width=width, height=height, spp=spp,
+                max_depth=max_depth, seed=0)
+
+
+def relmse(x, ref):
+    """mean over pixels and channels of (x - y)^2 / (y^2 + 1e-2)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(ref, np.float64)
+    return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))

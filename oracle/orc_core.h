@@ -965,25 +965,31 @@ inline V3 envLightSample(const Light& l, const EnvImage* env, V3& wi, float& pdf
 }
 
 // render/scene.h:137-157: sum of all environment lights, pdf weighted by pick probability
-inline V3 evalEnv(const LightSet& ls, V3 wi, float& pdf)
+inline V3 evalEnv(const LightSet& ls, V3 wi, float& pdf, uint32_t* hitMask = nullptr)
 {
   V3 value(0.f);
   pdf = 0.f;
+  uint32_t mask = 0;
   for (int i = 0; i < ls.count; ++i)
   {
     float cp;
-    value += envLightEval(ls.lights[i], ls.env, wi, cp);
+    V3 v = envLightEval(ls.lights[i], ls.env, wi, cp);
+    if (cp > 0.f) mask |= 1u << i;
+    value += v;
     pdf += cp * dist1dPdf(ls.cdf, ls.count, i);
   }
+  if (hitMask) *hitMask = mask;
   return value;
 }
 // render/scene.h:215-250
-inline V3 sampleLight(const LightSet& ls, V3& wi, float& pdf, float& dist, float sx, float sy, float sc)
+inline V3 sampleLight(const LightSet& ls, V3& wi, float& pdf, float& dist, float sx, float sy, float sc, int* picked = nullptr)
 {
+  if (picked) *picked = 0;
   if (ls.count == 1) return envLightSample(ls.lights[0], ls.env, wi, pdf, dist, sx, sy);
   float pick;
   float c = sc;
   int id = dist1dSampleReuse(ls.cdf, ls.count, pick, c);
+  if (picked) *picked = id;
   V3 w = envLightSample(ls.lights[id], ls.env, wi, pdf, dist, sx, sy);
   pdf *= pick;
   return w * (1.0f / pick);

@@ -374,6 +374,13 @@ static inline V3 clampL(V3 L, float maxRadiance)
 }
 
 static inline uint32_t hashStep(uint32_t h, uint32_t x) { return murmurMix(h, x); }
+// Coarse direction cell (1/32 per axis): two paths whose sampled directions fall in different
+// cells are not the same path even when they hit the same primitive.
+static inline uint32_t dirCell(V3 d)
+{
+  auto q = [](float x) { return uint32_t(int(std::floor(x * 32.f)) + 64) & 0xffu; };
+  return q(d.x) | (q(d.y) << 8) | (q(d.z) << 16);
+}
 
 struct Stats { long long rays = 0; long long nodes = 0; };
 
@@ -445,7 +452,9 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
     {
       // miss shading (:285-415)
       float lightPdf;
-      V3 Le = evalEnv(s.ls, dir, lightPdf);
+      uint32_t envMask;
+      V3 Le = evalEnv(s.ls, dir, lightPdf, &envMask);
+      hash = hashStep(hash, 0x100u | envMask);  // which environment lights the ray sees
       float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
       L += T * w * Le;
       V3 c = clampL(L, P.maxRadiance);
@@ -496,7 +505,10 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
         smp.get3D(sampleDim + 3, a0, a1, a2);
         V3 wiL;
         float lightPdf, dist;
-        V3 lw = sampleLight(s.ls, wiL, lightPdf, dist, a0, a1, a2);
+        int picked;
+        V3 lw = sampleLight(s.ls, wiL, lightPdf, dist, a0, a1, a2, &picked);
+        hash = hashStep(hash, 0x200u | uint32_t(picked));  // which light NEE sampled
+        hash = hashStep(hash, dirCell(wiL));
         if (reduceMax(lw) > 0.f && lightPdf > 0.f)
         {
           float bsdfPdf;
@@ -527,6 +539,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       ext = reduceMax(bw) > 0.f && pdf > 0.f;
       int extType = ext ? type : 0;
       hash = hashStep(hash, uint32_t(extType));
+      if (ext) hash = hashStep(hash, dirCell(wi));
 
       if (mAov1)  // first-hit AOVs (:739-819)
       {
@@ -569,6 +582,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       sr.o = shadowO; sr.d = shadowD; sr.tnear = shadowNear; sr.tfar = shadowFar;
       traverse(s, sr, true, nullptr);
       st.rays++;
+      hash = hashStep(hash, sr.prim < 0 ? 1u : 2u);  // the visibility event is part of the path
       if (sr.prim < 0) L += dirL;
     }
     if (!ext)
@@ -628,6 +642,12 @@ int orc_scene_set_lights(void* p, int nlights, const void* lights, int envW, int
   Scene& s = *(Scene*)p;
   if (nlights < 1 || nlights > 8) return -1;
   s.lights.assign((const Light*)lights, (const Light*)lights + nlights);
+  for (Light& l : s.lights)
+    if (l.kind == kDistant)  // light/distant_light.h:34 normalizes the direction
+    {
+      V3 d = normalize(V3(l.dx, l.dy, l.dz));
+      l.dx = d.x; l.dy = d.y; l.dz = d.z;
+    }
   if (envRgb && envW > 0 && envH > 0)
   {
     s.envRgb.assign(envRgb, envRgb + size_t(envW) * envH * 3);

@@ -1,0 +1,170 @@
+"""HIP path (through the C ABI) against the oracle and the reference's golden vectors.
+
+Bars: integer work (sampler values, closest-hit primitive IDs and barycentrics) bit-exact;
+per-pixel FP32 radiance within 1e-3 relative (1e-5 absolute floor) on identical paths, i.e.
+samples whose hit / lobe event hash matches the oracle's.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import protoray_hip as P
+import scenegen as G
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-3
+ATOL = 1e-5
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = P.DeviceHip(0)
+    yield d
+    d.close()
+
+
+def test_info(dev):
+    info = dev.getInfo()
+    print(info)
+    assert "gfx950" in info
+
+
+def test_sampler_bit_exact(dev, golden):
+    o1, o2, o3 = dev.sampler(golden["sampler_in"], 64)
+    assert np.array_equal(o1.view(np.uint32), golden["sampler_1d"].view(np.uint32))
+    assert np.array_equal(o2.view(np.uint32), golden["sampler_2d"].view(np.uint32))
+    assert np.array_equal(o3.view(np.uint32), golden["sampler_3d"].view(np.uint32))
+
+
+def test_luts(dev, golden):
+    lut = dev.luts()
+    _, orc = O.compute_luts()
+    off = 0
+    for name, n in O.LUT_SIZES:
+        np.testing.assert_allclose(lut[off:off + n], orc[name], atol=1e-6, err_msg=name)
+        off += n
+    np.testing.assert_allclose(lut[:1024], golden["lut_ggx_dir"].ravel(), atol=5e-6)
+
+
+def _random_scene(seed, ntri=3000):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform([-10, -2, 0], [10, 8, 30], size=(ntri, 3))
+    v = (c[:, None, :] + rng.normal(size=(ntri, 3, 3)) * 0.8).astype(np.float32).reshape(-1, 3)
+    idx = np.arange(ntri * 3, dtype=np.int32).reshape(-1, 3)
+    # a shared-edge fan so exact-t ties occur on shared vertices
+    fan_c = np.array([0, 3, 15], np.float32)
+    fan = [fan_c]
+    for k in range(8):
+        a = 2 * np.pi * k / 8
+        fan.append(fan_c + np.array([np.cos(a), np.sin(a), 0.3], np.float32))
+    base = v.shape[0]
+    v = np.concatenate([v, np.array(fan, np.float32)])
+    fidx = np.array([[base, base + 1 + k, base + 1 + (k + 1) % 8] for k in range(8)], np.int32)
+    idx = np.concatenate([idx, fidx])
+    return v, idx
+
+
+def _random_rays(seed, n, toward=None):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform([-12, -3, -5], [12, 9, 5], size=(n, 3))
+    if toward is None:
+        d = rng.normal(size=(n, 3))
+    else:
+        d = toward[rng.integers(0, len(toward), n)] - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    far = np.full(n, 3.4028234663852886e38)
+    far[: n // 4] = rng.uniform(1, 30, n // 4)  # bounded segments
+    return np.concatenate([o, d, np.zeros((n, 1)), far[:, None]], axis=1).astype(np.float32)
+
+
+def test_closest_hit_matches_brute_force(dev):
+    v, idx = _random_scene(1)
+    tri_mat = np.ones(idx.shape[0], np.int32)
+    mats = np.zeros(2, P.MATERIAL_DTYPE)
+    dev.initScene(v, idx, tri_mat, mats)
+    stats = dev.bvhStats()
+    print(stats)
+    assert stats["triangles"] == idx.shape[0]
+    scene = O.Scene(v, idx, tri_mat, np.zeros(2, O.MATERIAL_DTYPE))
+    centroids = v[idx].mean(axis=1)
+    rays = _random_rays(2, 20000, toward=centroids)
+    # rays through the fan's shared centre vertex hit several triangles at the same t
+    fan_rays = _random_rays(3, 64)
+    fan_rays[:, 3:6] = np.array([0, 3, 15], np.float32) - fan_rays[:, :3]
+    fan_rays[:, 3:6] /= np.linalg.norm(fan_rays[:, 3:6], axis=1, keepdims=True)
+    rays = np.concatenate([rays, fan_rays])
+    prim, t, u, w = dev.queryRays(rays)
+    bprim, bt, bu, bw = scene.intersect(rays, mode=1)
+    assert (prim >= 0).sum() > 10000
+    assert np.array_equal(prim, bprim)
+    hit = prim >= 0
+    assert np.array_equal(t[hit].view(np.uint32), bt[hit].view(np.uint32))
+    assert np.array_equal(u[hit].view(np.uint32), bu[hit].view(np.uint32))
+    assert np.array_equal(w[hit].view(np.uint32), bw[hit].view(np.uint32))
+    occ = dev.queryOcclusion(rays)
+    bocc, _, _, _ = scene.intersect(rays, mode=3)
+    assert np.array_equal(occ, (bocc >= 0).astype(np.int32))
+
+
+def _parity(dev, scene, W, H, spp, min_match):
+    dev.enableDebug(True)
+    P.load_scene(dev, scene)
+    dev.initFrame(W, H, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+    osc = O.Scene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"])
+    osc.set_lights(scene["lights"], scene.get("env"))
+    acc = None
+    for s in range(spp):
+        dev.render(1)
+        L, hsh, first = dev.debugOutputs()
+        acc, _ = osc.render(scene["camera"], W, H, spp_start=s, spp_count=1, seed=0, max_depth=scene["max_depth"],
+                            aovs=("albedo1", "normal1"), accum=acc, record=True)
+        ref_first = acc["_first"]
+        # first hits are pure intersection work: bit-exact
+        assert np.array_equal(first, ref_first), f"spp {s}: {np.sum(first != ref_first)} first-hit mismatches"
+        same = hsh == acc["_hash"]
+        frac = same.mean()
+        print(f"spp {s}: identical paths {frac:.4f}")
+        assert frac >= min_match
+        ref = acc["_sampleL"]
+        bad = same & ~np.isclose(L, ref, rtol=RTOL, atol=ATOL).all(axis=0)
+        if bad.any():
+            os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+            np.savez(os.path.join(ROOT, "gpurun_out", f"parity_fail_{scene['name']}_{s}.npz"), L=L, hsh=hsh,
+                     first=first, refL=ref, refHash=acc["_hash"], refFirst=ref_first, bad=np.flatnonzero(bad))
+            print("mismatched pixels", np.flatnonzero(bad), L[:, bad].T, ref[:, bad].T)
+        np.testing.assert_allclose(L[:, same], ref[:, same], rtol=RTOL, atol=ATOL)
+    gs, gw = dev.getAccum()
+    assert np.array_equal(gw, acc["weight"])
+    img = dev.blitFrame()
+    assert np.isfinite(img).all()
+    alb = dev.blitFrame(P.BUF_ALBEDO1).reshape(-1, 3).T
+    ref_alb = acc["albedo1"] / np.maximum(acc["albedo1W"], 1)
+    close = np.isclose(alb, ref_alb, rtol=1e-4, atol=1e-5).all(axis=0)
+    assert close.mean() > 0.995
+    dev.enableDebug(False)
+
+
+def test_render_parity_ambient_diffuse(dev):
+    scene = G.spheres_ambient(n_spheres=24, subdiv=2)
+    _parity(dev, scene, 64, 64, 4, 0.99)
+
+
+def test_render_parity_sunsky_mixed(dev):
+    scene = G.spheres_sunsky(n_spheres=48, subdiv=2, palette=24)
+    _parity(dev, scene, 96, 64, 3, 0.97)
+
+
+def test_render_stats_and_error(dev):
+    scene = G.spheres_ambient(n_spheres=16, subdiv=1)
+    P.load_scene(dev, scene)
+    dev.initFrame(64, 32)
+    st = dev.render(4)
+    assert st["spp"] == 4 and st["rays"] > 64 * 32 * 4
+    err = dev.getFrameError()
+    assert 0.0 <= err < 1.0
+    with pytest.raises(ValueError):
+        dev.initFrame(60, 32)  # width not a multiple of 32 (path_renderer_stream.h:109-110)
