@@ -1,0 +1,172 @@
+"""Benchmark: path-traced Msamples/s of the stream path tracer on one MI355X.
+
+Workload (configuration 1 of the stream renderer's benchmark set): 1920x1080, 64 spp, max
+depth 8, thin-lens camera, 4096 icospheres at subdivision 4 (20,971,522 triangles) with
+diffuse / rough-conductor / rough-dielectric materials, lit by a sun/sky plus a distant light;
+Sobol-Burley seed 0.  One step = one sample pass (one sample for every pixel, one
+Renderer::render call, render/path_renderer_stream.h:150-187); --steps 64 renders the full
+frame.  Scene generation, upload and the BVH build happen before the timed region (the BVH
+build time is reported separately, as Embree's buildMs is).
+
+Output: one JSON line with the metric, the roofline of the dominant kernel (algorithmic bytes
+from the kernels' own counters over the HIP-event time of that kernel) and a CPU baseline: the
+oracle (the project's C++ restatement of the reference path) timed on the host cores on a
+bounded sample of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("oidn-protoray_amd", os.path.join("oidn-protoray_amd", "scenes")):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per unit of work (fixed by the queue records the kernels read and write).
+NODE_BYTES = 64        # one BVH node: both children's boxes + child references
+TRI_BYTES = 48         # one triangle record: v0, e1, e2 (+ primitive ID)
+RAY_BYTES_CLOSEST = 4 + 32 + 16   # queue entry, ray (origin+tnear, direction), hit (t, u, v, prim)
+RAY_BYTES_SHADOW = 4 + 32 + 4 + 16 + 16  # queue entry, shadow ray, flags, radiance read + direct light
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=64)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
+    p.add_argument("--small", action="store_true", help="tiny scene, for profiling smoke runs only")
+    return p.parse_args()
+
+
+def cpu_baseline(scene, target_s):
+    """Times the oracle on a bounded band of rows of the same frame, on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    W, H = scene["width"], scene["height"]
+    t0 = time.time()
+    osc = O.Scene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"])
+    osc.set_lights(scene["lights"], scene.get("env"))
+    build_s = time.time() - t0
+    rows = (H // 2 - 16, H // 2 + 16)
+    # calibrate with one pass, then size the sample to about target_s seconds
+    t0 = time.time()
+    osc.render(scene["camera"], W, H, spp_start=0, spp_count=1, max_depth=scene["max_depth"], threads=threads, rows=rows)
+    t1 = time.time() - t0
+    spp = max(1, min(scene["spp"], int(target_s / max(t1, 1e-3))))
+    t0 = time.time()
+    _, rays = osc.render(scene["camera"], W, H, spp_start=0, spp_count=spp, max_depth=scene["max_depth"],
+                         threads=threads, rows=rows)
+    dt = time.time() - t0
+    samples = (rows[1] - rows[0]) * W * spp
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {rows[0]}-{rows[1]} of the {W}x{H} frame x {spp} spp ({samples} samples, "
+                      f"{dt:.1f} s, {rays / dt / 1e6:.2f} Mray/s); CPU BVH build {build_s:.1f} s not timed"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by "
+              f"image tile across GPUs by design, but no multi-GPU run is part of this build)", file=sys.stderr)
+        return 2
+    import protoray_hip as P
+    import scenegen as G
+
+    if args.small:
+        scene = G.spheres_sunsky(n_spheres=64, subdiv=2, width=256, height=128)
+    else:
+        scene = G.spheres_sunsky()
+    W, H = scene["width"], scene["height"]
+    ntris = int(scene["indices"].shape[0])
+    dev = P.DeviceHip(0)
+    info = dev.getInfo()
+    build_ms = dev.initScene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"])
+    dev.initLights(scene["lights"], scene["env"])
+    dev.initRenderer(maxDepth=scene["max_depth"])
+    dev.initCamera(scene["camera"])
+    dev.initFrame(W, H)
+    bvh = dev.bvhStats()
+
+    dev.initSampler(0)
+    for _ in range(args.warmup):
+        dev.render(1)
+    # timed region: sample indices 0 .. steps-1 of seed 0 into a cleared frame
+    dev.initSampler(0)
+    dev.clearFrame()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    st = dev.render(args.steps)
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = dev.kernelTimes()
+    img = dev.blitFrame()
+
+    samples = args.steps * W * H
+    value = samples / elapsed / 1e6
+    # roofline of the dominant kernel (bytes per launch / mean launch time == total / total)
+    kernels = {
+        "intersect": (kt["intersect_ms"], kt["rays"] * RAY_BYTES_CLOSEST + kt["node_visits"] * NODE_BYTES
+                      + kt["tri_tests"] * TRI_BYTES, kt["intersect_launches"]),
+        "occluded": (kt["occluded_ms"], kt["shadow_rays"] * RAY_BYTES_SHADOW + kt["shadow_node_visits"] * NODE_BYTES
+                     + kt["shadow_tri_tests"] * TRI_BYTES, kt["occluded_launches"]),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    ms, nbytes, launches = kernels[dom]
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    result = {
+        "metric": "path-traced Msamples/s (1080p, depth 8)",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"spheres_sunsky {W}x{H}, {args.steps} spp of 64, max depth {scene['max_depth']}, "
+                        f"{ntris} triangles, thin lens r=0.05 f=15, sun/sky (35 deg, T=3) + distant light, seed 0",
+            "device": info,
+            "bvh_build_ms": round(build_ms, 1),
+            "bvh_max_depth": bvh["max_depth"],
+            "mray_per_s": round(st["rays"] / elapsed / 1e6, 1),
+            "rays": int(st["rays"]),
+            "kernel_ms": {k: round(kt[k + "_ms"], 2) for k in ("generate", "intersect", "shade", "occluded")},
+            "nodes_per_ray": round(kt["node_visits"] / max(kt["rays"], 1), 2),
+            "tris_per_ray": round(kt["tri_tests"] / max(kt["rays"], 1), 2),
+            "image_mean": [round(float(x), 5) for x in img.reshape(-1, 3).mean(axis=0)],
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": round(nbytes / max(launches, 1)),
+            "launches": int(launches),
+            "mean_launch_ms": round(ms / max(launches, 1), 4),
+        },
+    }
+    dev.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
+        result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
