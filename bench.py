@@ -42,6 +42,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--small", action="store_true", help="tiny scene, for profiling smoke runs only")
+    p.add_argument("--copy-calibration-mib", type=int, default=0,
+                   help="before the timed region, run one device-to-device copy of this many MiB (a copy "
+                        "kernel of known traffic that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -95,6 +98,10 @@ def main():
     dev.initFrame(W, H)
     bvh = dev.bvhStats()
 
+    if args.copy_calibration_mib > 0:
+        copy_ms = dev.copyCalibration(args.copy_calibration_mib << 20)
+        print(f"copy calibration: {args.copy_calibration_mib} MiB in {copy_ms:.3f} ms "
+              f"({2 * (args.copy_calibration_mib << 20) / copy_ms / 1e6:.0f} GB/s read+write)", file=sys.stderr)
     dev.initSampler(0)
     for _ in range(args.warmup):
         dev.render(1)

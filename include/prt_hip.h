@@ -105,6 +105,9 @@ typedef struct prt_stats {
 int prt_context_create(int device_ordinal, prt_context** out);
 int prt_context_destroy(prt_context* ctx);
 const char* prt_last_error(void);
+/* Copies `bytes` bytes device-to-device with hipMemcpyAsync (a plain copy kernel of known
+ * size, used to calibrate the HBM FETCH_SIZE / WRITE_SIZE counters); ms receives its time. */
+int prt_debug_copy(prt_context* ctx, int64_t bytes, double* ms);
 /* Waits for all work on the context's device (hipDeviceSynchronize); brackets timed regions. */
 int prt_device_synchronize(prt_context* ctx);
 /* Device name, CU count and HBM size (Device::getInfo, render/device.h:60). */

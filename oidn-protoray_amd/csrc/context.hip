@@ -270,6 +270,8 @@ struct prt_context
   DevBuf<int> flags, q0, q1, shadowQ, counters, spill;
   DevBuf<uint32_t> hash;
   DevBuf<unsigned long long> stats;
+  DevBuf<RenderArgs> argsDev;
+  RenderArgs* argsHost = nullptr;  // pinned staging copy
   LaunchCfg launch{};
   int spillStride = 0;
   // timing
@@ -313,7 +315,8 @@ int prt_context_create(int device_ordinal, prt_context** out)
   if (c->lut.alloc(PRT_LUT_FLOATS) != hipSuccess ||
       hipMemcpy(c->lut.p, c->luts.data(), sizeof(float) * PRT_LUT_FLOATS, hipMemcpyHostToDevice) != hipSuccess ||
       c->spill.alloc(size_t(PRT_MAX_STACK - PRT_LDS_STACK) * c->spillStride) != hipSuccess ||
-      c->stats.alloc(8) != hipSuccess)
+      c->stats.alloc(8) != hipSuccess || c->argsDev.alloc(1) != hipSuccess ||
+      hipHostMalloc((void**)&c->argsHost, sizeof(RenderArgs), hipHostMallocDefault) != hipSuccess)
   {
     prt_context_destroy(c);
     return fail(PRT_ERR_NOMEM, "device allocation failed");
@@ -328,9 +331,28 @@ int prt_context_destroy(prt_context* c)
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   freeBvh(c->bvh);
+  if (c->argsHost) hipHostFree(c->argsHost);
   for (auto& ev : c->ev) if (ev) hipEventDestroy(ev);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
+  return PRT_OK;
+}
+
+int prt_debug_copy(prt_context* c, int64_t bytes, double* ms)
+{
+  CTX_CHECK(c);
+  if (bytes <= 0 || bytes > (int64_t(1) << 36)) return fail(PRT_ERR_INVALID, "invalid copy size");
+  DevBuf<char> a, b;
+  HIP_TRY(a.alloc(size_t(bytes)));
+  HIP_TRY(b.alloc(size_t(bytes)));
+  HIP_TRY(hipMemsetAsync(a.p, 1, size_t(bytes), c->stream));
+  HIP_TRY(hipEventRecord(c->ev[6], c->stream));
+  HIP_TRY(hipMemcpyAsync(b.p, a.p, size_t(bytes), hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipEventRecord(c->ev[7], c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev[7]));
+  float m = 0.f;
+  HIP_TRY(hipEventElapsedTime(&m, c->ev[6], c->ev[7]));
+  if (ms) *ms = m;
   return PRT_OK;
 }
 
@@ -760,6 +782,9 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
   if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
   if (n_spp < 0) return fail(PRT_ERR_INVALID, "negative sample count");
   RenderArgs a = makeArgs(c);
+  *c->argsHost = a;
+  HIP_TRY(hipMemcpyAsync(c->argsDev.p, c->argsHost, sizeof(RenderArgs), hipMemcpyHostToDevice, c->stream));
+  const RenderArgs* ad = c->argsDev.p;
   HIP_TRY(hipMemsetAsync(c->stats.p, 0, sizeof(unsigned long long) * 8, c->stream));
   for (int k = 0; k < 4; ++k) c->kernelMs[k] = 0.0;
   c->launches[0] = c->launches[1] = 0;
@@ -772,19 +797,18 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
       HIP_TRY(hipMemcpyAsync(c->colorC.p, c->color.p, sizeof(float) * 3 * c->npix, hipMemcpyDeviceToDevice, c->stream));
       HIP_TRY(hipMemcpyAsync(c->weightC.p, c->weight.p, sizeof(float) * c->npix, hipMemcpyDeviceToDevice, c->stream));
     }
-    a.sampleIndex = c->sampleIndex;
     hipEventRecord(c->ev[0], c->stream);
-    launchGenerate(a, c->stream);
+    launchGenerate(ad, a.npix, c->sampleIndex, c->stream);
     hipEventRecord(c->ev[1], c->stream);
     float msGen = 0.f, msI = 0.f, msS = 0.f, msO = 0.f;
     for (int depth = 0; depth <= a.maxDepth; ++depth)
     {
       hipEventRecord(c->ev[2], c->stream);
-      launchIntersect(a, depth, c->launch, c->stream);
+      launchIntersect(ad, depth, c->launch, c->stream);
       hipEventRecord(c->ev[3], c->stream);
-      launchShade(a, depth, c->launch, c->stream);
+      launchShade(ad, depth, c->sampleIndex, c->launch, c->stream);
       hipEventRecord(c->ev[4], c->stream);
-      launchOccluded(a, depth, c->launch, c->stream);
+      launchOccluded(ad, depth, c->launch, c->stream);
       hipEventRecord(c->ev[5], c->stream);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipEventSynchronize(c->ev[5]));

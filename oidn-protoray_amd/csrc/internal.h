@@ -48,7 +48,7 @@ struct RenderArgs
   float invW, invH;
   prt_frame_dev fb;
   // renderer
-  uint32_t seed, sampleIndex;
+  uint32_t seed;
   int maxDepth, rrDepth;
   float minWeight, maxRadiance;
   // path state, indexed by path slot (8x8 pixel tiles, 64 consecutive slots per tile)
@@ -76,10 +76,12 @@ struct LaunchCfg
   int gridTrace;  // blocks of the persistent traversal kernels
   int gridShade;  // blocks of the persistent shading kernel
 };
-void launchGenerate(const RenderArgs& a, hipStream_t s);
-void launchIntersect(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s);
-void launchShade(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s);
-void launchOccluded(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s);
+// The kernels read RenderArgs from device memory (one copy per render call); passing the
+// ~1 KB record by value would make every lane copy it to scratch.
+void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStream_t s);
+void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s);
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, hipStream_t s);
+void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
                  int* spill, int spillStride, int grid, hipStream_t s);
 void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s);

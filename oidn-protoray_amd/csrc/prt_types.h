@@ -72,7 +72,9 @@ struct prt_bvh_node
 };
 
 #define PRT_LEAF_MAX 4
+#ifndef PRT_LDS_STACK
 #define PRT_LDS_STACK 32
+#endif
 #define PRT_MAX_STACK 128
 
 // Path flags (per pixel, one int).

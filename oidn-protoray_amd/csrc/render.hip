@@ -151,8 +151,9 @@ PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, fl
   return c;
 }
 
-__global__ void __launch_bounds__(256) kGenerate(RenderArgs a)
+__global__ void __launch_bounds__(256) kGenerate(const RenderArgs* __restrict__ ap, uint32_t sampleIndex)
 {
+  const RenderArgs& a = *ap;
   int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0)
     for (int k = threadIdx.x; k < 2 * (a.maxDepth + 2); k += blockDim.x) a.counters[k] = 0;
@@ -161,7 +162,7 @@ __global__ void __launch_bounds__(256) kGenerate(RenderArgs a)
   slotToPixel(slot, a.W, x, y);
   uint32_t pixelId = uint32_t(x) | (uint32_t(y) << 16);  // path_renderer_stream.h:219-220
   Sampler smp;
-  smp.set(a.seed, a.sampleIndex, pixelId);
+  smp.set(a.seed, sampleIndex, pixelId);
   float psx, psy, lsx, lsy;
   smp.get2D(0, psx, psy);  // sampleDimPixel (integrator.h:52)
   smp.get2D(2, lsx, lsy);  // sampleDimLens (integrator.h:53)
@@ -197,8 +198,9 @@ __global__ void __launch_bounds__(256) kGenerate(RenderArgs a)
   if (a.hash) a.hash[slot] = 0x9e3779b9u;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) kIntersect(RenderArgs a, int depth)
+__global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __restrict__ ap, int depth)
 {
+  const RenderArgs& a = *ap;
   __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
   const int wave = threadIdx.x >> 6;
   TraceStack st;
@@ -227,8 +229,9 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(RenderArgs a, int dept
   }
 }
 
-__global__ void __launch_bounds__(kShadeBlock) kShade(RenderArgs a, int depth)
+__global__ void __launch_bounds__(kShadeBlock) kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
 {
+  const RenderArgs& a = *ap;
   const int count = depth == 0 ? a.npix : a.counters[2 * depth];
   const int* qIn = a.queue[depth & 1];
   int* qOut = a.queue[(depth + 1) & 1];
@@ -298,7 +301,7 @@ __global__ void __launch_bounds__(kShadeBlock) kShade(RenderArgs a, int depth)
         bool inside = (m.kind == PRT_KIND_DIELECTRIC) && (mediumId == m.mediumInt);
         Bsdf b = makeBsdf(m, ctx, wo, inside);
         Sampler smp;
-        smp.set(a.seed, a.sampleIndex, pixelId);  // resetSample (:641-642)
+        smp.set(a.seed, sampleIndex, pixelId);  // resetSample (:641-642)
         bool mAov1 = !(fl & PRT_F_AOV1), mAov2 = !(fl & PRT_F_AOV2);
         bool mRR = !(mAov1 || mAov2);
         // direct light (:659-718); every supported lobe is Smooth
@@ -403,8 +406,9 @@ __global__ void __launch_bounds__(kShadeBlock) kShade(RenderArgs a, int depth)
   }
 }
 
-__global__ void __launch_bounds__(kTraceBlock) kOccluded(RenderArgs a, int depth)
+__global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth)
 {
+  const RenderArgs& a = *ap;
   __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
   const int wave = threadIdx.x >> 6;
   TraceStack st;
@@ -488,19 +492,19 @@ __global__ void kDebugSampler(int n, const uint32_t* __restrict__ in, int ndim, 
 
 }  // namespace
 
-void launchGenerate(const RenderArgs& a, hipStream_t s)
+void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStream_t s)
 {
-  kGenerate<<<(a.npix + 255) / 256, 256, 0, s>>>(a);
+  kGenerate<<<(npix + 255) / 256, 256, 0, s>>>(a, sampleIndex);
 }
-void launchIntersect(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s)
+void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s)
 {
   kIntersect<<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
 }
-void launchShade(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s)
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, hipStream_t s)
 {
-  kShade<<<c.gridShade, kShadeBlock, 0, s>>>(a, depth);
+  kShade<<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex);
 }
-void launchOccluded(const RenderArgs& a, int depth, const LaunchCfg& c, hipStream_t s)
+void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s)
 {
   kOccluded<<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
 }

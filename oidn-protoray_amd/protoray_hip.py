@@ -50,6 +50,7 @@ SIGNATURES = {
     "prt_context_create": [_i, ctypes.POINTER(ctypes.c_void_p)],
     "prt_context_destroy": [_vp],
     "prt_last_error": [],
+    "prt_debug_copy": [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)],
     "prt_device_synchronize": [_vp],
     "prt_get_info": [_vp, ctypes.c_char_p, _i],
     "prt_scene_set_geometry": [_vp, _i, _vp, _vp, _i, _vp, _vp],
@@ -242,6 +243,12 @@ class DeviceHip:
         s = np.zeros(4, np.int64)
         _check(_lib.prt_debug_bvh_stats(self.h, _ptr(s)))
         return {"inner_nodes": int(s[0]), "triangles": int(s[1]), "max_depth": int(s[2])}
+
+    def copyCalibration(self, nbytes):
+        """Runs one device-to-device copy of nbytes (a copy kernel of known traffic); returns ms."""
+        ms = ctypes.c_double(0)
+        _check(_lib.prt_debug_copy(self.h, int(nbytes), ctypes.byref(ms)))
+        return ms.value
 
     def synchronize(self):
         _check(_lib.prt_device_synchronize(self.h))
