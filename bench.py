@@ -28,7 +28,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Algorithmic bytes per unit of work (fixed by the queue records the kernels read and write).
-NODE_BYTES = 64        # one BVH node: both children's boxes + child references
+NODE_BYTES = 128       # one 4-wide BVH node: four child boxes + child references
 TRI_BYTES = 48         # one triangle record: v0, e1, e2 (+ primitive ID)
 RAY_BYTES_CLOSEST = 4 + 32 + 16   # queue entry, ray (origin+tnear, direction), hit (t, u, v, prim)
 RAY_BYTES_SHADOW = 4 + 32 + 4 + 16 + 16  # queue entry, shadow ray, flags, radiance read + direct light

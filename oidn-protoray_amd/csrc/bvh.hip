@@ -2,8 +2,8 @@
 //
 // Linear BVH (Karras 2012): 63-bit Morton codes of triangle centroids, a radix sort
 // (hipcub/rocPRIM onesweep), one thread per inner node to find its key range and split,
-// bottom-up bounding boxes with one atomic arrival counter per node, then emission of
-// 64-byte nodes that hold both children's boxes.  Subtrees covering <= PRT_LEAF_MAX
+// bottom-up bounding boxes with one atomic arrival counter per node, then a top-down collapse
+// into 4-wide 128-byte nodes (greedy surface-area opening).  Subtrees covering <= PRT_LEAF_MAX
 // consecutive (Morton-ordered) triangles become leaves, and the triangles are stored in that
 // order as (v0, e1, e2) records for the traversal kernels.
 #include <hip/hip_runtime.h>
@@ -199,41 +199,80 @@ __device__ __forceinline__ void padBox(Box& b)
   }
 }
 
-__global__ void __launch_bounds__(kBlock) kEmit(int n, const int* __restrict__ childL, const int* __restrict__ childR,
-                                                const int2* __restrict__ range, const float* __restrict__ leafBox,
-                                                const float* __restrict__ innerBox, prt_bvh_node* __restrict__ nodes)
+__device__ __forceinline__ float halfArea(const float* b)
 {
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n - 1) return;
-  prt_bvh_node nd;
-  int cref[2] = {childL[i], childR[i]};
-  Box bb[2];
-  for (int s = 0; s < 2; ++s)
+  float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Top-down collapse of the binary tree into 4-wide nodes, one level of the wide tree per
+// launch.  Each wide node starts from the two children of its binary node and repeatedly
+// opens the child with the largest surface area until it has four children or none can be
+// opened; children covering <= PRT_LEAF_MAX triangles are leaves.
+__global__ void __launch_bounds__(kBlock) kCollapse(const int2* __restrict__ frontier, int count, int2* __restrict__ next,
+                                                    int* __restrict__ nextCount, int* __restrict__ nodeCounter,
+                                                    const int* __restrict__ childL, const int* __restrict__ childR,
+                                                    const int2* __restrict__ range, const float* __restrict__ leafBox,
+                                                    const float* __restrict__ innerBox, prt_bvh4_node* __restrict__ nodes)
+{
+  int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= count) return;
+  int2 e = frontier[f];
+  int b = e.x, k = e.y;
+  int codes[4];
+  int n = 2;
+  codes[0] = childL[b];
+  codes[1] = childR[b];
+  auto openable = [&](int c) { return c >= 0 && (range[c].y - range[c].x + 1) > PRT_LEAF_MAX; };
+  while (n < 4)
   {
-    int c = cref[s];
-    const float* src;
-    if (c < 0)
+    int best = -1;
+    float bestA = -1.f;
+    for (int q = 0; q < n; ++q)
+      if (openable(codes[q]))
+      {
+        float a = halfArea(innerBox + 6 * codes[q]);
+        if (a > bestA) { bestA = a; best = q; }
+      }
+    if (best < 0) break;
+    int c = codes[best];
+    codes[best] = childL[c];
+    codes[n++] = childR[c];
+  }
+  prt_bvh4_node nd;
+  for (int q = 0; q < 4; ++q)
+  {
+    nd.pad[q] = 0;
+    if (q >= n)
     {
-      src = leafBox + 6 * (~c);
-      cref[s] = leafCode(~c, 1);
+      nd.lox[q] = nd.loy[q] = nd.loz[q] = 3.4e38f;
+      nd.hix[q] = nd.hiy[q] = nd.hiz[q] = -3.4e38f;
+      nd.child[q] = -1;  // empty slot: never visited (traversal masks child == -1)
+      continue;
+    }
+    int c = codes[q];
+    Box bb;
+    const float* src = c < 0 ? leafBox + 6 * (~c) : innerBox + 6 * c;
+    for (int a = 0; a < 3; ++a) { bb.lo[a] = src[a]; bb.hi[a] = src[3 + a]; }
+    padBox(bb);
+    nd.lox[q] = bb.lo[0]; nd.hix[q] = bb.hi[0];
+    nd.loy[q] = bb.lo[1]; nd.hiy[q] = bb.hi[1];
+    nd.loz[q] = bb.lo[2]; nd.hiz[q] = bb.hi[2];
+    if (c < 0)
+      nd.child[q] = leafCode(~c, 1);
+    else if (!openable(c))
+    {
+      int2 r = range[c];
+      nd.child[q] = leafCode(r.x, r.y - r.x + 1);
     }
     else
     {
-      src = innerBox + 6 * c;
-      int2 r = range[c];
-      int cnt = r.y - r.x + 1;
-      if (cnt <= PRT_LEAF_MAX) cref[s] = leafCode(r.x, cnt);
+      int j = atomicAdd(nodeCounter, 1);
+      next[atomicAdd(nextCount, 1)] = make_int2(c, j);
+      nd.child[q] = j;
     }
-    for (int k = 0; k < 3; ++k) { bb[s].lo[k] = src[k]; bb[s].hi[k] = src[3 + k]; }
-    padBox(bb[s]);
   }
-  nd.lox[0] = bb[0].lo[0]; nd.lox[1] = bb[1].lo[0]; nd.hix[0] = bb[0].hi[0]; nd.hix[1] = bb[1].hi[0];
-  nd.loy[0] = bb[0].lo[1]; nd.loy[1] = bb[1].lo[1]; nd.hiy[0] = bb[0].hi[1]; nd.hiy[1] = bb[1].hi[1];
-  nd.loz[0] = bb[0].lo[2]; nd.loz[1] = bb[1].lo[2]; nd.hiz[0] = bb[0].hi[2]; nd.hiz[1] = bb[1].hi[2];
-  nd.child[0] = cref[0];
-  nd.child[1] = cref[1];
-  nd.pad[0] = nd.pad[1] = 0;
-  nodes[i] = nd;
+  nodes[k] = nd;
 }
 
 // Triangle records in leaf order: (v0, prim), (e1 = v1 - v0), (e2 = v2 - v0).
@@ -247,28 +286,6 @@ __global__ void __launch_bounds__(kBlock) kTris(const float* __restrict__ pos, c
   tris[3 * k + 0] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
   tris[3 * k + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.f);
   tris[3 * k + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.f);
-}
-
-// Depth of every emitted leaf (walking parents), reduced to the maximum; bounds the stack.
-__global__ void __launch_bounds__(kBlock) kDepth(int n, const int* __restrict__ parentLeaf, const int* __restrict__ parentInner,
-                                                 const int2* __restrict__ range, int* __restrict__ maxDepth, int* __restrict__ leafCount)
-{
-  int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n || n < 2) return;
-  // the emitted leaf containing k is the highest ancestor with range <= PRT_LEAF_MAX (or k itself)
-  int p = parentLeaf[k];
-  int depth = 1;
-  bool collapsed = false;
-  while (p >= 0)
-  {
-    int2 r = range[p];
-    if (r.y - r.x + 1 > PRT_LEAF_MAX || p == 0) ++depth;
-    else collapsed = true;
-    p = parentInner[p];
-  }
-  atomicMax(maxDepth, depth);
-  (void)collapsed;
-  (void)leafCount;
 }
 
 inline unsigned blocks(long long n) { return unsigned((n + kBlock - 1) / kBlock); }
@@ -291,7 +308,7 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
     // empty scene: one degenerate triangle (det == 0, never hit) under a single leaf
     BVH_CHECK(hipMalloc(&out.tris, 3 * sizeof(float4)));
     BVH_CHECK(hipMemsetAsync(out.tris, 0, 3 * sizeof(float4), stream));
-    BVH_CHECK(hipMalloc(&out.nodes, sizeof(prt_bvh_node)));
+    BVH_CHECK(hipMalloc(&out.nodes, sizeof(prt_bvh4_node)));
     out.root = int(0x80000000u);
     out.nodeCount = 0;
     out.maxDepth = 1;
@@ -302,8 +319,8 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
   float* cbox = nullptr;
   uint64_t *keys = nullptr, *keys2 = nullptr;
   int *vals = nullptr, *vals2 = nullptr, *childL = nullptr, *childR = nullptr, *parentInner = nullptr, *parentLeaf = nullptr;
-  int *arrive = nullptr, *dmax = nullptr;
-  int2* range = nullptr;
+  int *arrive = nullptr, *counts = nullptr;
+  int2 *range = nullptr, *front0 = nullptr, *front1 = nullptr;
   float *leafBox = nullptr, *innerBox = nullptr;
   void* temp = nullptr;
   size_t tempBytes = 0;
@@ -320,16 +337,17 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
   BVH_CHECK(hipMalloc(&parentLeaf, sizeof(int) * n));
   BVH_CHECK(hipMalloc(&arrive, sizeof(int) * inner));
   BVH_CHECK(hipMalloc(&range, sizeof(int2) * inner));
+  BVH_CHECK(hipMalloc(&front0, sizeof(int2) * inner));
+  BVH_CHECK(hipMalloc(&front1, sizeof(int2) * inner));
+  BVH_CHECK(hipMalloc(&counts, sizeof(int) * 2));
   BVH_CHECK(hipMalloc(&leafBox, sizeof(float) * 6 * size_t(n)));
   BVH_CHECK(hipMalloc(&innerBox, sizeof(float) * 6 * size_t(inner)));
-  BVH_CHECK(hipMalloc(&dmax, sizeof(int) * 2));
-  BVH_CHECK(hipMalloc(&out.nodes, sizeof(prt_bvh_node) * inner));
+  BVH_CHECK(hipMalloc(&out.nodes, sizeof(prt_bvh4_node) * inner));
   BVH_CHECK(hipMalloc(&out.tris, sizeof(float4) * 3 * size_t(n)));
 
   const float initBox[8] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f, 0.f, 0.f};
   BVH_CHECK(hipMemcpyAsync(cbox, initBox, sizeof(initBox), hipMemcpyHostToDevice, stream));
   BVH_CHECK(hipMemsetAsync(arrive, 0, sizeof(int) * inner, stream));
-  BVH_CHECK(hipMemsetAsync(dmax, 0, sizeof(int) * 2, stream));
   kCentroids<<<blocks(n), kBlock, 0, stream>>>(dPos, dIdx, n, cent, cbox);
   kMorton<<<blocks(n), kBlock, 0, stream>>>(cent, n, cbox, keys, vals);
   BVH_CHECK(hipGetLastError());
@@ -337,29 +355,45 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
   BVH_CHECK(hipMalloc(&temp, tempBytes));
   BVH_CHECK(hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keys, keys2, vals, vals2, n, 0, 63, stream));
   if (n > 1)
-  {
     kHierarchy<<<blocks(n - 1), kBlock, 0, stream>>>(keys2, n, childL, childR, parentInner, parentLeaf, range);
-  }
   kBoxes<<<blocks(n), kBlock, 0, stream>>>(dPos, dIdx, vals2, n, childL, childR, parentInner, parentLeaf, arrive, leafBox, innerBox);
   kTris<<<blocks(n), kBlock, 0, stream>>>(dPos, dIdx, vals2, n, out.tris);
+  BVH_CHECK(hipGetLastError());
+  int levels = 1, wideNodes = 0;
   if (n > 1)
   {
-    kEmit<<<blocks(n - 1), kBlock, 0, stream>>>(n, childL, childR, range, leafBox, innerBox, out.nodes);
-    kDepth<<<blocks(n), kBlock, 0, stream>>>(n, parentLeaf, parentInner, range, dmax, dmax + 1);
+    // level-by-level collapse into 4-wide nodes; wide node 0 = binary root 0
+    int2 root = make_int2(0, 0);
+    int one = 1;
+    BVH_CHECK(hipMemcpyAsync(front0, &root, sizeof(int2), hipMemcpyHostToDevice, stream));
+    BVH_CHECK(hipMemcpyAsync(counts, &one, sizeof(int), hipMemcpyHostToDevice, stream));
+    int count = 1;
+    levels = 0;
+    while (count > 0)
+    {
+      BVH_CHECK(hipMemsetAsync(counts + 1, 0, sizeof(int), stream));
+      kCollapse<<<blocks(count), kBlock, 0, stream>>>(front0, count, front1, counts + 1, counts, childL, childR, range, leafBox,
+                                                      innerBox, out.nodes);
+      BVH_CHECK(hipGetLastError());
+      int h[2] = {0, 0};
+      BVH_CHECK(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, stream));
+      BVH_CHECK(hipStreamSynchronize(stream));
+      wideNodes = h[0];
+      count = h[1];
+      std::swap(front0, front1);
+      ++levels;
+    }
   }
-  BVH_CHECK(hipGetLastError());
-  int depth[2] = {1, 0};
-  BVH_CHECK(hipMemcpyAsync(depth, dmax, sizeof(depth), hipMemcpyDeviceToHost, stream));
   BVH_CHECK(hipStreamSynchronize(stream));
   out.root = n > 1 ? 0 : int(0x80000000u);
-  out.nodeCount = n > 1 ? n - 1 : 0;
-  out.maxDepth = n > 1 ? depth[0] : 1;
+  out.nodeCount = n > 1 ? wideNodes : 0;
+  out.maxDepth = levels;
   hipFree(cent); hipFree(cbox); hipFree(keys); hipFree(keys2); hipFree(vals); hipFree(vals2);
   hipFree(childL); hipFree(childR); hipFree(parentInner); hipFree(parentLeaf); hipFree(arrive); hipFree(range);
-  hipFree(leafBox); hipFree(innerBox); hipFree(dmax); hipFree(temp);
+  hipFree(front0); hipFree(front1); hipFree(counts); hipFree(leafBox); hipFree(innerBox); hipFree(temp);
   auto t1 = std::chrono::high_resolution_clock::now();
   out.buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  if (out.maxDepth >= PRT_MAX_STACK)
+  if (3 * out.maxDepth >= PRT_MAX_STACK)  // a wide node pushes at most three children
   {
     err = "BVH deeper than the traversal stack (" + std::to_string(out.maxDepth) + " levels)";
     return hipErrorInvalidValue;

@@ -12,12 +12,12 @@ namespace prt {
 
 struct BvhData
 {
-  prt_bvh_node* nodes = nullptr;
+  prt_bvh4_node* nodes = nullptr;
   float4* tris = nullptr;
   int root = 0;
   int ntris = 0;
-  long long nodeCount = 0;
-  int maxDepth = 0;
+  long long nodeCount = 0;  // 4-wide nodes
+  int maxDepth = 0;         // levels of the 4-wide tree
   double buildMs = 0.0;
 };
 

@@ -71,6 +71,18 @@ struct prt_bvh_node
   int32_t pad[2];
 };
 
+// 4-wide BVH node (128 bytes, one L2 line): the boxes of up to four children as SoA (lo/hi per
+// axis for children 0..3) and four child references with the same encoding as above.  Unused
+// slots hold an empty box (lo = +inf, hi = -inf) that no ray hits.
+struct prt_bvh4_node
+{
+  float lox[4], hix[4];
+  float loy[4], hiy[4];
+  float loz[4], hiz[4];
+  int32_t child[4];
+  int32_t pad[4];
+};
+
 #define PRT_LEAF_MAX 4
 #ifndef PRT_LDS_STACK
 #define PRT_LDS_STACK 32

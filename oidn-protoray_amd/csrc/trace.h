@@ -10,7 +10,7 @@ namespace prt {
 
 struct BvhView
 {
-  const prt_bvh_node* nodes;
+  const prt_bvh4_node* nodes;
   const float4* tris;  // 3 float4 per triangle in leaf order: (v0, prim bits), (e1, 0), (e2, 0)
   int root;            // root reference (inner node 0, or a leaf when the scene is tiny)
 };
@@ -35,32 +35,34 @@ PRT_DEV bool intersectTriangle(float4 a, float4 b, float4 c, f3 o, f3 d, float t
   return t >= tnear && t <= tfar;
 }
 
-// Slab test of both children of a node.  The far distance is widened by 1 + 2*gamma(3) and
-// the boxes are padded at build time, so rounding never culls a hit the triangle test reports.
-PRT_DEV void boxPair(const prt_bvh_node* nodes, int n, f3 o, f3 id, float tnear, float tfar, bool& h0, bool& h1, float& t0, float& t1,
-                     int& c0, int& c1)
+// Slab tests of the four children of a wide node.  The far distance is widened by
+// 1 + 2*gamma(3) and the boxes are padded at build time, so rounding never culls a hit the
+// triangle test reports.  Returns the entry distance per child (+inf where missed or empty).
+PRT_DEV void boxQuad(const prt_bvh4_node* nodes, int n, f3 o, f3 id, float tnear, float tfar, float t[4], int c[4])
 {
   const float4* p = reinterpret_cast<const float4*>(nodes + n);
-  float4 bx = p[0], by = p[1], bz = p[2];
-  int4 ch = reinterpret_cast<const int4*>(p)[3];
-  // child 0: (bx.x, bx.z) (by.x, by.z) (bz.x, bz.z); child 1: (bx.y, bx.w) ...
-  float ax0 = (bx.x - o.x) * id.x, ax1 = (bx.z - o.x) * id.x;
-  float ay0 = (by.x - o.y) * id.y, ay1 = (by.z - o.y) * id.y;
-  float az0 = (bz.x - o.z) * id.z, az1 = (bz.z - o.z) * id.z;
-  float bx0 = (bx.y - o.x) * id.x, bx1 = (bx.w - o.x) * id.x;
-  float by0 = (by.y - o.y) * id.y, by1 = (by.w - o.y) * id.y;
-  float bz0 = (bz.y - o.z) * id.z, bz1 = (bz.w - o.z) * id.z;
+  float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+  int4 ch = reinterpret_cast<const int4*>(p)[6];
   const float g = 1.0000004f;
-  float an = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tnear));
-  float af = fminf(fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fmaxf(az0, az1)) * g, tfar);
-  float bn = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tnear));
-  float bf = fminf(fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fmaxf(bz0, bz1)) * g, tfar);
-  h0 = an <= af;
-  h1 = bn <= bf;
-  t0 = an;
-  t1 = bn;
-  c0 = ch.x;
-  c1 = ch.y;
+  const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+  const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+  const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+  c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+  {
+    float x0 = (lxs[q] - o.x) * id.x, x1 = (hxs[q] - o.x) * id.x;
+    float y0 = (lys[q] - o.y) * id.y, y1 = (hys[q] - o.y) * id.y;
+    float z0 = (lzs[q] - o.z) * id.z, z1 = (hzs[q] - o.z) * id.z;
+    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
+    float tf = fminf(fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * g, tfar);
+    t[q] = (tn <= tf && c[q] != -1) ? tn : __int_as_float(0x7f800000);
+  }
+}
+
+PRT_DEV void cswap(float& ta, int& ca, float& tb, int& cb)
+{
+  if (tb < ta) { float t = ta; ta = tb; tb = t; int c = ca; ca = cb; cb = c; }
 }
 
 struct TraceStack
@@ -85,8 +87,9 @@ struct TraceStack
   }
 };
 
-// Closest hit.  Ties in t go to the smaller primitive ID, so the result does not depend on
-// the traversal order.  Returns the primitive ID or -1; nodeVisits counts inner-node fetches.
+// Closest hit (kAnyHit = false) or any hit.  Ties in t go to the smaller primitive ID, so the
+// result does not depend on the traversal order.  Returns the primitive ID or -1; nodeVisits
+// counts wide-node fetches.
 template <bool kAnyHit>
 PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, float& tHit, float& uHit, float& vHit,
                      TraceStack& st, uint32_t& nodeVisits, uint32_t& triTests)
@@ -95,24 +98,29 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
   int best = -1;
   st.sp = 0;
   int node = bvh.root;
+  const float inf = __int_as_float(0x7f800000);
   for (;;)
   {
     if (node >= 0)
     {
-      bool h0, h1;
-      float t0, t1;
-      int c0, c1;
+      float t[4];
+      int c[4];
       ++nodeVisits;
-      boxPair(bvh.nodes, node, o, id, tnear, tfar, h0, h1, t0, t1, c0, c1);
-      if (h0 && h1)
+      boxQuad(bvh.nodes, node, o, id, tnear, tfar, t, c);
+      // closest first: sorting network on (t, child), misses sort to the end as +inf
+      cswap(t[0], c[0], t[1], c[1]);
+      cswap(t[2], c[2], t[3], c[3]);
+      cswap(t[0], c[0], t[2], c[2]);
+      cswap(t[1], c[1], t[3], c[3]);
+      cswap(t[1], c[1], t[2], c[2]);
+      if (t[0] != inf)
       {
-        if (t1 < t0) { int tmp = c0; c0 = c1; c1 = tmp; }
-        st.push(c1);
-        node = c0;
+        if (t[3] != inf) st.push(c[3]);
+        if (t[2] != inf) st.push(c[2]);
+        if (t[1] != inf) st.push(c[1]);
+        node = c[0];
         continue;
       }
-      if (h0) { node = c0; continue; }
-      if (h1) { node = c1; continue; }
     }
     else
     {
@@ -121,16 +129,16 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
       for (int k = 0; k < count; ++k)
       {
         const float4* tp = bvh.tris + 3 * (first + k);
-        float4 a = tp[0], b = tp[1], c = tp[2];
-        float t, u, v;
+        float4 a = tp[0], b = tp[1], cc = tp[2];
+        float th, u, v;
         ++triTests;
-        if (intersectTriangle(a, b, c, o, d, tnear, tfar, t, u, v))
+        if (intersectTriangle(a, b, cc, o, d, tnear, tfar, th, u, v))
         {
           int prim = __float_as_int(a.w);
-          if (kAnyHit) { tHit = t; return prim; }
-          if (t < tfar || best < 0 || prim < best)
+          if (kAnyHit) { tHit = th; return prim; }
+          if (th < tfar || best < 0 || prim < best)
           {
-            tfar = t; best = prim; uHit = u; vHit = v;
+            tfar = th; best = prim; uHit = u; vHit = v;
           }
         }
       }
