@@ -34,6 +34,19 @@ RAY_BYTES_CLOSEST = 4 + 32 + 16   # queue entry, ray (origin+tnear, direction), 
 RAY_BYTES_SHADOW = 4 + 32 + 4 + 16 + 16  # queue entry, shadow ray, flags, radiance read + direct light
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE, calibrated against a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_spheres_sunsky_1080p.txt")
+    if not os.path.exists(path):
+        return None
+    key = "traffic_bytes_per_launch.k" + kernel[0].upper() + kernel[1:] + " ="
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.split("=")[1])
+    return None
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -161,7 +174,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dom),
             "algorithmic_bytes_per_launch": round(nbytes / max(launches, 1)),
             "launches": int(launches),
             "mean_launch_ms": round(ms / max(launches, 1), 4),
