@@ -27,14 +27,22 @@ extern "C" {
 
 typedef struct prt_context prt_context;
 
-/* Material record (32 bytes).  Uniform StandardMaterial parameter sets of the stream path
- * (material/standard.cpp:21-183 ctor, :409-573 SIMD lobes), each a MultiBsdf with one lobe:
+/* Material record (44 bytes).  Uniform StandardMaterial parameter sets of the stream path
+ * (material/standard.cpp:21-183 ctor, :409-597 SIMD lobes), each a MultiBsdf with one lobe:
  *   kind 0 diffuse    : baseColor = (r,g,b), specularWeight 0          -> DiffuseBrdf
  *   kind 1 conductor  : metalness 1, baseColor = (r,g,b), specularColor = (er,eg,eb),
  *                       specularRoughness = roughness                  -> RoughConductorBrdf
  *                                                                         <FresnelConductorArtistic>
  *   kind 2 dielectric : transmissionWeight 1, specularIOR = ior,
  *                       specularRoughness = roughness                  -> RoughDielectricBsdf
+ *   kind 3 coated     : a rough dielectric layer over a diffuse base   -> RoughCoatingBsdf
+ *                       (rough_coating.h:270-593) over DiffuseBrdf.  The StandardMaterial
+ *                       default (metalness 0, specularWeight w > 0, specularRoughness > 0) is
+ *                       baseColor = (r,g,b), coat color (er,eg,eb) = 1, thickness = 1,
+ *                       weight = w, ior = specularIOR (standard.cpp:512-518); a coat layer
+ *                       over a diffuse base (specularWeight 0, coatWeight w) takes coatColor,
+ *                       coatThickness, coatRoughness and coatIOR (standard.cpp:580-597).
+ * thickness and weight are read by kind 3 only.
  * Material ID 0 is reserved for the environment (render/scene.cpp:204-207). */
 typedef struct prt_material {
   int32_t kind;
@@ -42,7 +50,8 @@ typedef struct prt_material {
   float er, eg, eb;
   float roughness;
   float ior;
-  int32_t pad[2];
+  float thickness;
+  float weight;
 } prt_material;
 
 /* Environment light record (32 bytes): light/ambient_light.h (kind 0, Ke = rgb),
@@ -171,10 +180,12 @@ int prt_debug_get(prt_context* ctx, float* sample_l, uint32_t* path_hash, int32_
  * n x ndim x 3 (sampling/sobol_burley_sampler_simd.h:36-85). */
 int prt_debug_sampler(prt_context* ctx, int32_t n, const uint32_t* seed_pid_idx, int32_t ndim, float* out1,
                       float* out2, float* out3);
-/* The energy-compensation LUTs computed at context creation (PRT_LUT_FLOATS = 9760 floats:
- * GGX dir 32x32, GGX avg 32, dielectric eta dir 16^3, eta avg 16^2, rcp-eta dir 16^3,
- * rcp-eta avg 16^2; bsdf/albedo_tables.h:14-157). */
-int prt_debug_luts(prt_context* ctx, float* out);
+/* The energy-compensation LUTs computed at context creation (14112 floats: GGX dir 32x32,
+ * GGX avg 32, dielectric eta dir 16^3, eta avg 16^2, rcp-eta dir 16^3, rcp-eta avg 16^2,
+ * dielectric reflection dir 16^3, avg 16^2; bsdf/albedo_tables.h:14-205).  *count receives the
+ * number of floats; out (capacity floats) may be NULL to query it.  PRT_ERR_INVALID if
+ * capacity is too small. */
+int prt_debug_luts(prt_context* ctx, float* out, int32_t capacity, int32_t* count);
 /* BVH statistics: inner node count, triangle count, max depth, scene triangle count. */
 int prt_debug_bvh_stats(prt_context* ctx, int64_t* stats4);
 /* Device-side kernel timing of the last prt_render call, per kernel class

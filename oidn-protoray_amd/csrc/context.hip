@@ -114,7 +114,9 @@ inline float hostFresnelEx(float cosI, float& cosT, float eta)  // bsdf/optics.h
 }
 
 // bsdf/albedo.h:16-120 single-scattering albedo estimators and albedo_tables.cpp:23-107 layout
-float lutIntegrate(float cosO, const HostGGX& m, float eta, bool dielectric)
+// mode 0: GGX reflection (albedo.h:18-40); 1: dielectric reflection + refraction (:63-103);
+// 2: dielectric reflection only (:124-151)
+float lutIntegrate(float cosO, const HostGGX& m, float eta, int mode)
 {
   const int n = 32;
   float sum = 0.f;
@@ -127,9 +129,16 @@ float lutIntegrate(float cosO, const HostGGX& m, float eta, bool dielectric)
       H3 wh = m.sampleVisible(wo, sx, sy);
       float cosOH = hdot(wo, wh);
       H3 wi = hsub(hscale(wh, 2.f * cosOH), wo);
-      if (!dielectric)
+      if (mode == 0)
       {
         if (wi.z > 0.f) sum += m.G2(wo, wi) * (1.0f / m.G1(wo));
+        continue;
+      }
+      if (mode == 2)
+      {
+        float cosTH;
+        float F = hostFresnelEx(std::max(cosOH, 0.f), cosTH, eta);  // optics.h:84-99 clamps cosI at 0
+        if (wi.z > 0.f) sum += F * (m.G2(wo, wi) * (1.0f / m.G1(wo)));
         continue;
       }
       float cosTH;
@@ -167,14 +176,18 @@ void computeLuts(std::vector<float>& L)
   {
     HostGGX m{std::max(hsqr(float(j) / 31.f), 0.001f)};
     float* d = L.data() + PRT_LUT_GGX_DIR + j * 32;
-    for (int i = 0; i < 32; ++i) d[i] = lutIntegrate(float(i) / 31.f, m, 1.f, false);
+    for (int i = 0; i < 32; ++i) d[i] = lutIntegrate(float(i) / 31.f, m, 1.f, 0);
     L[PRT_LUT_GGX_AVG + j] = lutAvg(d, 32);
   }
-  for (int table = 0; table < 2; ++table)
+  // MicrofacetDielectricAlbedoTable (eta in [1/3, 1] and [1, 3]) and
+  // MicrofacetDielectricReflectionAlbedoTable (eta in [1/3, 1]), albedo_tables.cpp:94-116
+  const int dirOff[3] = {PRT_LUT_DIEL_ETA_DIR, PRT_LUT_DIEL_RCPETA_DIR, PRT_LUT_DIEL_REFL_DIR};
+  const int avgOff[3] = {PRT_LUT_DIEL_ETA_AVG, PRT_LUT_DIEL_RCPETA_AVG, PRT_LUT_DIEL_REFL_AVG};
+  for (int table = 0; table < 3; ++table)
   {
-    float minEta = table == 0 ? 1.0f / 3.0f : 1.f, maxEta = table == 0 ? 1.f : 3.f;
-    float* dir = L.data() + (table == 0 ? PRT_LUT_DIEL_ETA_DIR : PRT_LUT_DIEL_RCPETA_DIR);
-    float* avg = L.data() + (table == 0 ? PRT_LUT_DIEL_ETA_AVG : PRT_LUT_DIEL_RCPETA_AVG);
+    float minEta = table == 1 ? 1.f : 1.0f / 3.0f, maxEta = table == 1 ? 3.f : 1.f;
+    float* dir = L.data() + dirOff[table];
+    float* avg = L.data() + avgOff[table];
     for (int k = 0; k < 16; ++k)
     {
       HostGGX m{std::max(hsqr(float(k) / 15.f), 0.001f)};
@@ -182,7 +195,7 @@ void computeLuts(std::vector<float>& L)
       {
         float eta = hlerp(minEta, maxEta, float(j) / 15.f);
         float* d = dir + (k * 16 + j) * 16;
-        for (int i = 0; i < 16; ++i) d[i] = lutIntegrate(float(i) / 15.f, m, eta, true);
+        for (int i = 0; i < 16; ++i) d[i] = lutIntegrate(float(i) / 15.f, m, eta, table == 2 ? 2 : 1);
         avg[k * 16 + j] = lutAvg(d, 16);
       }
     }
@@ -204,6 +217,11 @@ float dielAvg(const std::vector<float>& L, float eta, float roughness)  // albed
   }
   float e = (eta - 1.f) / 2.f;
   return lutInterp2(e * 15.f, roughness * 15.f, L.data() + PRT_LUT_DIEL_RCPETA_AVG, 16);
+}
+float dielReflAvg(const std::vector<float>& L, float eta, float roughness)  // albedo_tables.h:187-194
+{
+  float e = (eta - 1.0f / 3.0f) / (1.0f - 1.0f / 3.0f);
+  return lutInterp2(e * 15.f, roughness * 15.f, L.data() + PRT_LUT_DIEL_REFL_AVG, 16);
 }
 float fresnelAvg(float eta)  // bsdf/optics.h:144-175
 {
@@ -437,7 +455,7 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
     prt_material_constants& q = k[i];
     std::memset(&q, 0, sizeof(q));
     const prt_material& s = m[i];
-    if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_DIELECTRIC) return fail(PRT_ERR_INVALID, "unknown material kind");
+    if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_COATED) return fail(PRT_ERR_INVALID, "unknown material kind");
     q.kind = s.kind;
     q.roughness = hclamp(s.roughness, 0.f, 1.f);
     q.alpha = std::max(hsqr(q.roughness), 0.001f);  // bsdf/microfacet.h:10-24
@@ -467,6 +485,30 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
         q.c2[j] = hsqr(Favg) * q.Eavg / (1.f - Favg * (1.f - q.Eavg));  // rough_conductor.h:127-129
         q.rr[j] = rgb[j];
       }
+    }
+    else if (s.kind == PRT_KIND_COATED)
+    {
+      // RoughCoatingBsdf over a diffuse base (material/standard.cpp:512-518, 580-597,
+      // bsdf/rough_coating.h:287-295).  Opaque, so never a media interface: eta = 1 / ior.
+      float ior = s.ior < 1.f ? 1.0f / s.ior : s.ior;
+      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "coat IOR must differ from the ambient IOR 1");
+      float eta = hclamp(1.0f / ior, 1.f / 3.f, 3.f);
+      float thickness = std::max(s.thickness, 0.f), scale = hclamp(s.weight, 0.f, 1.f);
+      float coat[3] = {hclamp(s.er, 0.f, 1.f), hclamp(s.eg, 0.f, 1.f), hclamp(s.eb, 0.f, 1.f)};
+      float baseImportance = std::max(std::max(rgb[0], rgb[1]), rgb[2]);  // bsdf/diffuse.h:70-73
+      for (int j = 0; j < 3; ++j)
+      {
+        q.c0[j] = rgb[j];
+        q.c1[j] = coat[j];
+        // getAlbedo (rough_coating.h:567-571): lerp(base, base * coat^thickness, scale)
+        q.rr[j] = rgb[j] + scale * (rgb[j] * std::pow(coat[j], thickness) - rgb[j]);
+      }
+      q.c2[0] = thickness;
+      q.c2[1] = scale;
+      q.c2[2] = baseImportance;
+      q.eta[0] = q.eta[1] = eta <= 1.f ? eta : 1.0f / eta;
+      q.Eavg = dielReflAvg(c->luts, eta, q.roughness) * scale;  // Ravg
+      q.importance = std::max(1.f, baseImportance);              // rough_coating.h:562-565
     }
     else
     {
@@ -1003,10 +1045,12 @@ int prt_debug_sampler(prt_context* c, int32_t n, const uint32_t* in, int32_t ndi
   return PRT_OK;
 }
 
-int prt_debug_luts(prt_context* c, float* out)
+int prt_debug_luts(prt_context* c, float* out, int32_t capacity, int32_t* count)
 {
   CTX_CHECK(c);
-  if (!out) return fail(PRT_ERR_INVALID, "out is null");
+  if (count) *count = PRT_LUT_FLOATS;
+  if (!out) return PRT_OK;
+  if (capacity < PRT_LUT_FLOATS) return fail(PRT_ERR_INVALID, "LUT buffer too small");
   std::memcpy(out, c->luts.data(), sizeof(float) * PRT_LUT_FLOATS);
   return PRT_OK;
 }

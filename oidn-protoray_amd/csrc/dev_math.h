@@ -340,6 +340,13 @@ PRT_DEV float dielAlbedo(const float* lut, float cosO, float eta, float roughnes
   return interp3D(cosO * 15.f, e * 15.f, roughness * 15.f, lut + PRT_LUT_DIEL_RCPETA_DIR, 16);
 }
 
+// MicrofacetDielectricReflectionAlbedoTable::eval (eta in [1/3, 1])
+PRT_DEV float dielReflAlbedo(const float* lut, float cosO, float eta, float roughness)
+{
+  float e = (eta - (1.0f / 3.0f)) / (1.0f - (1.0f / 3.0f));
+  return interp3D(cosO * 15.f, e * 15.f, roughness * 15.f, lut + PRT_LUT_DIEL_REFL_DIR, 16);
+}
+
 // ---------------------------------------------------------------------------------------
 // Shading context and BSDFs
 // ---------------------------------------------------------------------------------------
@@ -389,8 +396,9 @@ struct Bsdf
   int kind;
   float roughness;
   GGX ggx;
-  // diffuse: color; conductor: eta/k per channel + fms terms; dielectric: eta + fms terms
-  f3 c0, c1, c2;  // diffuse: color | conductor: eta, k, fmsScale
+  // diffuse: color; conductor: eta/k per channel + fms terms; dielectric: eta + fms terms;
+  // coated: diffuse color, coat color, (thickness, scale, diffuse importance), Ravg in Eavg
+  f3 c0, c1, c2;  // diffuse: color | conductor: eta, k, fmsScale | coated: see above
   f3 rr;          // conductor reflectivity (albedo AOV)
   float Eavg;     // conductor: average albedo
   float eta, EavgEta, EavgRcpEta, fmsRatio;  // dielectric (oriented)
@@ -399,7 +407,8 @@ struct Bsdf
 
 PRT_DEV f3 bsdfAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
 {
-  return b.kind == PRT_KIND_DIFFUSE ? b.c0 : (b.kind == PRT_KIND_CONDUCTOR ? b.rr : f3(1.f));
+  if (b.kind == PRT_KIND_DIFFUSE) return b.c0;
+  return (b.kind == PRT_KIND_DIELECTRIC) ? f3(1.f) : b.rr;  // conductor reflectivity | coated albedo
 }
 
 // bsdf/diffuse.h:55-68
@@ -547,6 +556,105 @@ PRT_DEV f3 dielSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
   return f3((sValue + fmsValue) * (1.0f / pdf));
 }
 
+// Coated diffuse: RoughCoatingBsdfSimd over DiffuseBrdfSimd (bsdf/rough_coating.h:297-593).
+// c0 = diffuse color, c1 = coat color, c2 = (thickness, scale, diffuse importance),
+// Eavg = Ravg, eta <= 1.
+PRT_DEV float refractCos(float cosI, float eta)  // optics.h:50-56
+{
+  return sqrtf(fmaxf(1.f - eta * eta * (1.f - cosI * cosI), 0.f));
+}
+// absorption through the coat along the refracted path and energy conservation of a diffuse
+// base [Kelemen and Szirmay-Kalos 2001] (:331-350)
+PRT_DEV f3 coatBase(const float* lut, const Bsdf& b, f3 baseValue, float cosO, float cosI, float Ro)
+{
+  float length = 1.0f / refractCos(cosO, b.eta) + 1.0f / refractCos(fabsf(cosI), b.eta);
+  if (cosI <= 0.f) length *= 0.5f;
+  const float t = b.c2.x * length;
+  f3 absorbed(baseValue.x * powf(b.c1.x, t), baseValue.y * powf(b.c1.y, t), baseValue.z * powf(b.c1.z, t));
+  baseValue = baseValue + b.c2.y * (absorbed - baseValue);
+  float Ri = dielReflAlbedo(lut, fabsf(cosI), b.eta, b.roughness) * b.c2.y;
+  return baseValue * ((1.f - Ro) * (1.f - Ri) * (1.0f / (1.f - b.Eavg)));
+}
+PRT_DEV float coatPickProb(const Bsdf& b, float Ro)
+{
+  float baseImportance = (1.f - Ro) * b.c2.z;
+  return Ro * (1.0f / (Ro + baseImportance));
+}
+// specular reflection of the coat around the half vector wh (:365-378)
+PRT_DEV float coatReflection(const Bsdf& b, f3 wo, f3 wi, f3 wh, float cosO, float& coatPdf)
+{
+  float cosOH = dot(wo, wh);
+  float F = fresnelDielectric(cosOH, b.eta) * b.c2.y;
+  f3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  coatPdf = whPdf * (1.0f / (4.f * fabsf(cosOH)));
+  return F * D * G * (1.0f / (4.f * cosO));
+}
+PRT_DEV f3 coatEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  float basePdf;
+  f3 baseValue = diffuseEval(b, wi, basePdf);
+  float Ro = dielReflAlbedo(lut, cosO, b.eta, b.roughness) * b.c2.y;
+  baseValue = coatBase(lut, b, baseValue, cosO, cosI, Ro);
+  float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
+  if (cosI > 0.f)
+  {
+    float coatPdf;
+    float coatValue = coatReflection(b, wo, wi, normalize(wo + wi), cosO, coatPdf);
+    pdf = coatPick * coatPdf + basePick * basePdf;
+    return f3(coatValue) + baseValue;
+  }
+  pdf = basePdf * basePick;
+  return baseValue;
+}
+PRT_DEV f3 coatSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  f3 wo0 = toLocal(b.fr, wo);
+  float Ro = dielReflAlbedo(lut, cosO, b.eta, b.roughness) * b.c2.y;
+  f3 wh, baseValue;
+  float basePdf;
+  float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
+  if (sc <= coatPick)
+  {
+    float whPdf;
+    wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+    wi = reflect(wo, wh, dot(wo, wh));
+    if (!(dot(wi, b.fr.N) > 0.f)) return f3(0.f);
+    type = BsdfType::GlossyReflection;
+    baseValue = diffuseEval(b, wi, basePdf);
+  }
+  else
+  {
+    // the diffuse base reads only the 2D part of the remapped sample (:424-427)
+    wi = toGlobal(b.fr, cosineSampleHemisphere(basePdf, sx, sy));
+    if (!(dot(wi, c.Ng) > 0.f && fmaxf(b.c0.x, fmaxf(b.c0.y, b.c0.z)) > 0.f)) return f3(0.f);
+    type = BsdfType::DiffuseReflection;
+    baseValue = b.c0 * basePdf;
+    wh = normalize(wo + wi);
+  }
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  baseValue = coatBase(lut, b, baseValue, cosO, cosI, Ro);
+  if (cosI <= 0.f)
+  {
+    pdf = basePick * basePdf;
+    return baseValue * (1.0f / pdf);
+  }
+  float coatPdf;
+  float coatValue = coatReflection(b, wo, wi, wh, cosO, coatPdf);
+  pdf = coatPick * coatPdf + basePick * basePdf;
+  return (f3(coatValue) + baseValue) * (1.0f / pdf);
+}
+
 // MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210)
 PRT_DEV f3 bsdfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
 {
@@ -557,6 +665,7 @@ PRT_DEV f3 bsdfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi,
     float cp;
     if (b.kind == PRT_KIND_DIFFUSE) v = diffuseEval(b, wi, cp);
     else if (b.kind == PRT_KIND_CONDUCTOR) v = conductorEval(lut, b, wo, wi, cp);
+    else if (b.kind == PRT_KIND_COATED) v = coatEval(lut, b, c, wo, wi, cp);
     else v = dielEval(lut, b, c, wo, wi, cp);
     pdf = cp * b.importance;
   }
@@ -573,6 +682,7 @@ PRT_DEV f3 bsdfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
     return dot(wi, c.Ng) > 0.f ? b.c0 : f3(0.f);
   }
   if (b.kind == PRT_KIND_CONDUCTOR) return conductorSample(lut, b, c, wo, wi, pdf, type, sx, sy);
+  if (b.kind == PRT_KIND_COATED) return coatSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
   return dielSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
 }
 

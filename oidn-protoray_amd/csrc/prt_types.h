@@ -6,6 +6,7 @@
 #define PRT_KIND_DIFFUSE 0
 #define PRT_KIND_CONDUCTOR 1
 #define PRT_KIND_DIELECTRIC 2
+#define PRT_KIND_COATED 3
 
 #define PRT_LIGHT_AMBIENT 0
 #define PRT_LIGHT_DISTANT 1
@@ -16,13 +17,17 @@
 //   MicrofacetAlbedoTable dir [32 roughness][32 cos], avg [32]
 //   MicrofacetDielectricAlbedoTable etaDir [16 rough][16 eta][16 cos] (eta in [1/3,1]), etaAvg
 //   [16][16], rcpEtaDir (eta in [1,3]), rcpEtaAvg
+//   MicrofacetDielectricReflectionAlbedoTable etaDir [16 rough][16 eta][16 cos] (eta in
+//   [1/3,1]), etaAvg [16][16] (bsdf/albedo_tables.h:160-205, read by the coating)
 #define PRT_LUT_GGX_DIR 0
 #define PRT_LUT_GGX_AVG 1024
 #define PRT_LUT_DIEL_ETA_DIR 1056
 #define PRT_LUT_DIEL_ETA_AVG 5152
 #define PRT_LUT_DIEL_RCPETA_DIR 5408
 #define PRT_LUT_DIEL_RCPETA_AVG 9504
-#define PRT_LUT_FLOATS 9760
+#define PRT_LUT_DIEL_REFL_DIR 9760
+#define PRT_LUT_DIEL_REFL_AVG 13856
+#define PRT_LUT_FLOATS 14112
 
 // Per-material constants derived on the host at upload (the per-hit constructor work of
 // RoughConductorBrdfSimd, rough_conductor.h:121-130, and RoughDielectricBsdfSimd,
@@ -33,12 +38,12 @@ struct prt_material_constants
   float roughness;   // clamped specularRoughness
   float alpha;       // max(roughness^2, 0.001)
   float importance;  // MultiBsdf importance of the single lobe (diffuse: max(color))
-  float c0[3];       // diffuse color | conductor eta
-  float c1[3];       // conductor k
-  float c2[3];       // conductor fmsScale
-  float rr[3];       // conductor reflectivity (albedo)
-  float Eavg;        // conductor average albedo
-  float eta[2];      // dielectric eta seen from outside [0] / inside [1]
+  float c0[3];       // diffuse color | conductor eta | coated: diffuse color
+  float c1[3];       // conductor k | coated: coat color (transmittance)
+  float c2[3];       // conductor fmsScale | coated: thickness, weight (scale), diffuse importance
+  float rr[3];       // conductor reflectivity | coated: albedo (the albedo AOV value)
+  float Eavg;        // conductor average albedo | coated: Ravg (average coat reflectance x scale)
+  float eta[2];      // dielectric eta seen from outside [0] / inside [1] | coated: eta (<= 1)
   float EavgEta[2];
   float EavgRcpEta[2];
   float fmsRatio[2];

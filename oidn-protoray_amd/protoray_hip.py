@@ -20,11 +20,9 @@ LIB_PATH = os.path.join(HERE, "libprt_hip.so")
 PRT_OK, PRT_ERR_INVALID, PRT_ERR_HIP, PRT_ERR_NOMEM, PRT_ERR_STATE = 0, -1, -2, -3, -4
 AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR = 1, 2, 4, 8, 16, 32
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
-LUT_FLOATS = 9760
-
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
-                           ("ior", "<f4"), ("pad", "<i4", 2)])
+                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
@@ -72,7 +70,7 @@ SIGNATURES = {
     "prt_debug_enable": [_vp, _i],
     "prt_debug_get": [_vp, _vp, _vp, _vp],
     "prt_debug_sampler": [_vp, _i, _vp, _i, _vp, _vp, _vp],
-    "prt_debug_luts": [_vp, _vp],
+    "prt_debug_luts": [_vp, _vp, ctypes.c_int32, _vp],
     "prt_debug_bvh_stats": [_vp, _vp],
     "prt_debug_kernel_ms": [_vp, _vp, _vp, _vp],
 }
@@ -235,8 +233,10 @@ class DeviceHip:
         return o1, o2, o3
 
     def luts(self):
-        out = np.zeros(LUT_FLOATS, np.float32)
-        _check(_lib.prt_debug_luts(self.h, _ptr(out)))
+        n = ctypes.c_int32(0)
+        _check(_lib.prt_debug_luts(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        _check(_lib.prt_debug_luts(self.h, _ptr(out), out.size, ctypes.byref(n)))
         return out
 
     def bvhStats(self):

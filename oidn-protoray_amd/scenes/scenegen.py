@@ -36,13 +36,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
-                           ("ior", "<f4"), ("pad", "<i4", 2)])
+                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
                          ("fov", "<f4"), ("nearClip", "<f4"), ("lensRadius", "<f4"),
                          ("focalDistance", "<f4")])
-KIND_DIFFUSE, KIND_CONDUCTOR, KIND_DIELECTRIC = 0, 1, 2
+KIND_DIFFUSE, KIND_CONDUCTOR, KIND_DIELECTRIC, KIND_COATED = 0, 1, 2, 3
 LIGHT_AMBIENT, LIGHT_DISTANT, LIGHT_IMAGE = 0, 1, 2
 
 
@@ -120,13 +120,15 @@ def gold_artistic(rng):
     return r, g
 
 
-def _material(kind, rgb=(0, 0, 0), edge=(0, 0, 0), roughness=0.0, ior=1.5):
+def _material(kind, rgb=(0, 0, 0), edge=(0, 0, 0), roughness=0.0, ior=1.5, thickness=0.0, weight=0.0):
     m = np.zeros((), MATERIAL_DTYPE)
     m["kind"] = kind
     m["r"], m["g"], m["b"] = rgb
     m["er"], m["eg"], m["eb"] = edge
     m["roughness"] = roughness
     m["ior"] = ior
+    m["thickness"] = thickness
+    m["weight"] = weight
     return m
 
 
@@ -185,13 +187,28 @@ def load_sunsky(path=os.path.join(HERE, "sunsky_e35_t3_az30.npz")):
                 sun_radius=float(z["sun_radius"][0]), sun_dir=z["sun_dir"])
 
 
-def spheres_sunsky(seed=0, n_spheres=4096, subdiv=4, width=1920, height=1080, spp=64, max_depth=8, palette=1024):
+def spheres_sunsky(seed=0, n_spheres=4096, subdiv=4, width=1920, height=1080, spp=64, max_depth=8, palette=1024,
+                   coated=0.0):
+    """coated: fraction of the diffuse palette entries given a rough coat (kind 3: IOR U[1.3,1.8],
+    roughness U[0.1,0.7], tinted coat color and thickness for half of them, the StandardMaterial
+    specular layer - white, thickness 1, weight 1 - for the other half).  0 keeps the random
+    stream, and so the scene, of the plain generator."""
     rng = Pcg32(seed)
     mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
     for _ in range(palette):
         u = rng.f()
         if u < 0.4:
-            mats.append(_material(KIND_DIFFUSE, (rng.f(0.1, 0.9), rng.f(0.1, 0.9), rng.f(0.1, 0.9))))
+            rgb = (rng.f(0.1, 0.9), rng.f(0.1, 0.9), rng.f(0.1, 0.9))
+            if coated > 0 and rng.f() < coated:
+                ior, rough = rng.f(1.3, 1.8), rng.f(0.1, 0.7)
+                if rng.f() < 0.5:
+                    mats.append(_material(KIND_COATED, rgb, (1, 1, 1), rough, ior, thickness=1.0, weight=1.0))
+                else:
+                    tint = (rng.f(0.3, 1.0), rng.f(0.3, 1.0), rng.f(0.3, 1.0))
+                    mats.append(_material(KIND_COATED, rgb, tint, rough, ior, thickness=rng.f(0.2, 2.0),
+                                          weight=rng.f(0.3, 1.0)))
+            else:
+                mats.append(_material(KIND_DIFFUSE, rgb))
         elif u < 0.7:
             alpha = rng.f(0.05, 0.5)
             r, g = gold_artistic(rng)

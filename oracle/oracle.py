@@ -31,14 +31,15 @@ _lib = None
 
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
-                           ("ior", "<f4"), ("pad", "<i4", 2)])
+                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
                          ("fov", "<f4"), ("nearClip", "<f4"), ("lensRadius", "<f4"),
                          ("focalDistance", "<f4")])
 LUT_SIZES = [("ggx_dir", 32 * 32), ("ggx_avg", 32), ("diel_eta_dir", 16 ** 3), ("diel_eta_avg", 16 ** 2),
-             ("diel_rcpeta_dir", 16 ** 3), ("diel_rcpeta_avg", 16 ** 2)]
+             ("diel_rcpeta_dir", 16 ** 3), ("diel_rcpeta_avg", 16 ** 2), ("diel_refl_dir", 16 ** 3),
+             ("diel_refl_avg", 16 ** 2)]
 LUT_FLOATS = sum(n for _, n in LUT_SIZES)
 
 
@@ -153,11 +154,14 @@ def compute_luts():
 
 
 def bsdf(luts_flat, params, geo, s):
+    """params: n x [kind r g b er eg eb roughness eta] (+ [thickness weight] for coated rows)."""
     params, geo, s = c32(params), c32(geo), c32(s)
+    if params.shape[1] == 9:
+        params = np.ascontiguousarray(np.concatenate([params, np.zeros((params.shape[0], 2), np.float32)], 1))
     n = params.shape[0]
     so = np.zeros((n, 7), np.float32)
     st = np.zeros(n, np.int32)
-    ev = np.zeros((n, 4), np.float32)
+    ev = np.zeros((n, 7), np.float32)  # value rgb, pdf, albedo rgb
     lib().orc_bsdf(n, c32(luts_flat).ctypes.data, params.ctypes.data, geo.ctypes.data, s.ctypes.data,
                    so.ctypes.data, st.ctypes.data, ev.ctypes.data)
     return so, st, ev

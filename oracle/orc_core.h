@@ -470,6 +470,8 @@ struct Luts
   float dielEtaAvg[16 * 16];
   float dielRcpEtaDir[16 * 16 * 16];  // eta in [1,3]
   float dielRcpEtaAvg[16 * 16];
+  float dielReflDir[16 * 16 * 16];  // MicrofacetDielectricReflectionAlbedoTable::etaDir, eta in [1/3,1]
+  float dielReflAvg[16 * 16];
 };
 
 // albedo_tables.h:20-42
@@ -502,6 +504,20 @@ inline float dielAlbedoAvg(const Luts& L, float eta, float roughness)
   return interp2D(e * 15.f, roughness * 15.f, L.dielRcpEtaAvg, 16, 16);
 }
 
+// MicrofacetDielectricReflectionAlbedoTable::eval / evalAvg (albedo_tables.h:160-205)
+inline float dielReflAlbedo(const Luts& L, float cosThetaO, float eta, float roughness)
+{
+  const float minEta = 1.0f / 3.0f, maxEta = 1.0f;
+  float e = (eta - minEta) / (maxEta - minEta);
+  return interp3D(cosThetaO * 15.f, e * 15.f, roughness * 15.f, L.dielReflDir, 16);
+}
+inline float dielReflAlbedoAvg(const Luts& L, float eta, float roughness)
+{
+  const float minEta = 1.0f / 3.0f, maxEta = 1.0f;
+  float e = (eta - minEta) / (maxEta - minEta);
+  return interp2D(e * 15.f, roughness * 15.f, L.dielReflAvg, 16, 16);
+}
+
 void precomputeLuts(Luts& L);  // orc_luts.cpp
 
 // ---------------------------------------------------------------------------------------
@@ -525,7 +541,7 @@ enum : int {
 //                                                           -> RoughConductorBrdf<FresnelConductorArtistic>
 //   2 dielectric : transmissionWeight = 1, specularIOR, specularRoughness
 //                                                           -> RoughDielectricBsdf + media interface
-enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2 };
+enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2, kCoated = 3 };
 
 struct Material  // 32-byte record shared with the HIP path's material table
 {
@@ -533,8 +549,9 @@ struct Material  // 32-byte record shared with the HIP path's material table
   float r, g, b;     // albedo (diffuse) / reflectivity (conductor)
   float er, eg, eb;  // edge tint (conductor)
   float roughness;   // specularRoughness (alpha = max(roughness^2, 0.001))
-  float ior;         // specularIOR (dielectric), relative to an ambient IOR of 1
-  int pad[2];
+  float ior;         // specularIOR (dielectric) / coat IOR (coated), relative to an ambient IOR of 1
+  float thickness;   // coated: coat thickness (absorption path scale)
+  float weight;      // coated: coat weight (specularWeight / coatWeight)
 };
 
 struct Ctx  // core/shading_context.h:113-124 subset
@@ -585,6 +602,10 @@ struct Bsdf
   V3 fmsScale;
   // dielectric
   float deta, EavgEta, EavgRcpEta, fmsRatio;
+  // coated diffuse: base = color (diffuse), coat transmittance color, thickness, weight (scale),
+  // eta <= 1, Ravg = average coat reflectance * scale, importance of the diffuse base
+  V3 coatColor;
+  float thickness, scale, ceta, Ravg, baseImportance;
   float roughness;
   GGX ggx{0.001f};
   float importance;
@@ -620,7 +641,7 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
     b.flags = BsdfType::GlossyReflection;
     b.importance = 1.f;
   }
-  else
+  else if (m.kind == kDielectric)
   {
     // material/standard.cpp:436-437 (fromOutside = !backfacing; the media interface sets
     // backfacing = (mediumId == mediumIntId), material/material.h:66-76)
@@ -640,6 +661,24 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
     b.fmsRatio = 1.f - x * (1.f - FavgEta);
     b.flags = BsdfType::GlossyReflection | BsdfType::GlossyTransmission;
     b.importance = 1.f;
+  }
+  else
+  {
+    // coated diffuse (material/standard.cpp:512-518, 580-597): RoughCoatingBsdf over a diffuse
+    // base; opaque materials are never media interfaces, so fromOutside holds and
+    // eta = ambientIor / coatIor (material/material.h:54-61)
+    b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
+    b.coatColor = V3(clampf(m.er, 0.f, 1.f), clampf(m.eg, 0.f, 1.f), clampf(m.eb, 0.f, 1.f));
+    b.thickness = std::max(m.thickness, 0.f);
+    b.scale = clampf(m.weight, 0.f, 1.f);
+    float ior = m.ior < 1.f ? 1.0f / m.ior : m.ior;
+    float eta = clampf(1.0f / ior, 1.f / 3.f, 3.f);
+    // rough_coating.h:287-295: the member eta is folded to <= 1, Ravg uses the argument as given
+    b.Ravg = dielReflAlbedoAvg(L, eta, roughness) * b.scale;
+    b.ceta = eta <= 1.f ? eta : 1.0f / eta;
+    b.baseImportance = reduceMax(b.color);  // bsdf/diffuse.h:70-73
+    b.flags = BsdfType::DiffuseReflection | BsdfType::GlossyReflection;
+    b.importance = std::max(1.f, b.baseImportance);  // rough_coating.h:562-565
   }
   return b;
 }
@@ -809,6 +848,114 @@ inline V3 dielSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
   return V3((singleValue + fmsValue) * (1.0f / pdf));
 }
 
+// --- coated diffuse (bsdf/rough_coating.h:297-593, RoughCoatingBsdfSimd over DiffuseBrdfSimd)
+inline float refractCos(float cosI, float eta)  // optics.h:50-56
+{
+  float k = 1.f - eta * eta * (1.f - cosI * cosI);
+  return std::sqrt(std::max(k, 0.f));
+}
+// absorption through the coat and energy conservation for a diffuse base (:331-350)
+inline V3 coatBase(const Luts& L, const Bsdf& b, V3 baseValue, float cosO, float cosI, float Ro)
+{
+  float length = 1.0f / refractCos(cosO, b.ceta) + 1.0f / refractCos(std::fabs(cosI), b.ceta);
+  if (cosI <= 0.f) length *= 0.5f;
+  const float t = b.thickness * length;
+  V3 absorbed(baseValue.x * std::pow(b.coatColor.x, t), baseValue.y * std::pow(b.coatColor.y, t),
+               baseValue.z * std::pow(b.coatColor.z, t));
+  baseValue = baseValue + b.scale * (absorbed - baseValue);
+  float Ri = dielReflAlbedo(L, std::fabs(cosI), b.ceta, b.roughness) * b.scale;
+  float T = (1.f - Ro) * (1.f - Ri) * (1.0f / (1.f - b.Ravg));
+  return baseValue * T;
+}
+inline float coatPickProb(const Bsdf& b, float Ro)
+{
+  float coatImportance = Ro;
+  float baseImportance = (1.f - Ro) * b.baseImportance;
+  return coatImportance * (1.0f / (coatImportance + baseImportance));
+}
+// the coat's specular reflection at (wo, wi) around the half vector wh (:365-378)
+inline float coatReflection(const Bsdf& b, V3 wo, V3 wi, V3 wh, float cosO, float& coatPdf)
+{
+  float cosOH = dot(wo, wh);
+  float F = fresnelDielectric(cosOH, b.ceta) * b.scale;
+  V3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  coatPdf = whPdf * (1.0f / (4.f * std::fabs(cosOH)));
+  return F * D * G * (1.0f / (4.f * cosO));
+}
+inline V3 coatEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+  float basePdf;
+  V3 baseValue = diffuseEval(b, wi, basePdf);
+  float Ro = dielReflAlbedo(L, cosO, b.ceta, b.roughness) * b.scale;
+  baseValue = coatBase(L, b, baseValue, cosO, cosI, Ro);
+  float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
+  if (cosI > 0.f)
+  {
+    float coatPdf;
+    float coatValue = coatReflection(b, wo, wi, normalize(wo + wi), cosO, coatPdf);
+    pdf = coatPick * coatPdf + basePick * basePdf;
+    return V3(coatValue) + baseValue;
+  }
+  pdf = basePdf * basePick;
+  return baseValue;
+}
+inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  V3 wo0 = toLocal(b.fr, wo);
+  float Ro = dielReflAlbedo(L, cosO, b.ceta, b.roughness) * b.scale;
+  wi = V3(0.f);
+  pdf = 0.f;
+  type = 0;
+  V3 wh(0.f), baseValue(0.f);
+  float basePdf = 0.f;
+  float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
+  if (sc <= coatPick)
+  {
+    float whPdf;
+    wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+    wi = reflect(wo, wh, dot(wo, wh));
+    if (!(dot(wi, b.fr.N) > 0.f)) return V3(0.f);
+    type = BsdfType::GlossyReflection;
+    baseValue = diffuseEval(b, wi, basePdf);
+  }
+  else
+  {
+    // the diffuse base uses only the 2D part of the remapped sample (:424-427)
+    V3 baseWi;
+    float basePdf2;
+    int baseType;
+    V3 baseWeight = diffuseSample(b, ctx, baseWi, basePdf2, baseType, sx, sy);
+    if (!(reduceMax(baseWeight) > 0.f)) return V3(0.f);
+    wi = baseWi;
+    basePdf = basePdf2;
+    type = baseType;
+    baseValue = baseWeight * basePdf;
+    wh = normalize(wo + wi);
+  }
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+  baseValue = coatBase(L, b, baseValue, cosO, cosI, Ro);
+  if (cosI <= 0.f)
+  {
+    pdf = basePick * basePdf;
+    return baseValue * (1.0f / pdf);
+  }
+  float coatPdf;
+  float coatValue = coatReflection(b, wo, wi, wh, cosO, coatPdf);
+  pdf = coatPick * coatPdf + basePick * basePdf;
+  return (V3(coatValue) + baseValue) * (1.0f / pdf);
+}
+
 // MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210): eval pdf is scaled by
 // importance / importanceSum; sample returns the child's weight times the child scale (1);
 // a zero importance sum disables sampling.
@@ -821,6 +968,7 @@ inline V3 bsdfEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, f
     float cp = 0.f;
     if (b.kind == kDiffuse) v = diffuseEval(b, wi, cp);
     else if (b.kind == kConductor) v = conductorEval(L, b, wo, wi, cp);
+    else if (b.kind == kCoated) v = coatEval(L, b, ctx, wo, wi, cp);
     else v = dielEval(L, b, ctx, wo, wi, cp);
     pdf = cp * b.importance;
   }
@@ -832,12 +980,19 @@ inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
   if (!(b.importance > 0.f)) return V3(0.f);
   if (b.kind == kDiffuse) return diffuseSample(b, ctx, wi, pdf, type, sx, sy);
   if (b.kind == kConductor) return conductorSample(L, b, ctx, wo, wi, pdf, type, sx, sy);
+  if (b.kind == kCoated) return coatSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
   return dielSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
 }
 inline V3 bsdfAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
 {
   if (b.kind == kDiffuse) return b.color;
   if (b.kind == kConductor) return b.rr;
+  if (b.kind == kCoated)  // rough_coating.h:567-571
+  {
+    V3 absorbed(b.color.x * std::pow(b.coatColor.x, b.thickness), b.color.y * std::pow(b.coatColor.y, b.thickness),
+                 b.color.z * std::pow(b.coatColor.z, b.thickness));
+    return b.color + b.scale * (absorbed - b.color);
+  }
   return V3(1.f);
 }
 

@@ -1,7 +1,8 @@
 // Oracle: energy-compensation LUT precompute.  TEST INFRASTRUCTURE ONLY.
 // Restates bsdf/albedo.h:16-120 (stratified 32x32 midpoint quadrature of the GGX and
 // dielectric single-scattering albedo) and bsdf/albedo_tables.cpp:23-107 (table layout and
-// trapezoid average).  Only the two tables the stream path's BSDFs read are built.
+// trapezoid average).  Only the tables the stream path's BSDFs read are built: GGX, dielectric
+// (both eta ranges) and dielectric reflection (the coating's).
 #include "orc_core.h"
 
 namespace orc {
@@ -37,6 +38,20 @@ static float dielAlbedoSample(float cosThetaO, float eta, const GGX& m, float sx
   return w;
 }
 
+// albedo.h:124-151 (reflection only)
+static float dielReflAlbedoSample(float cosThetaO, float eta, const GGX& m, float sx, float sy)
+{
+  cosThetaO = std::max(cosThetaO, 1e-6f);
+  V3 wo(cos2sin(cosThetaO), 0.f, cosThetaO);
+  float whPdf;
+  V3 wh = m.sampleVisible(wo, whPdf, sx, sy);
+  float cosOH = dot(wo, wh);
+  float F = fresnelDielectric(cosOH, eta);
+  V3 wi = reflect(wo, wh, cosOH);
+  if (wi.z <= 0.f) return 0.f;
+  return F * (m.G2(wo, wi) * rcpSafe(m.G1(wo)));
+}
+
 template <class F>
 static float integrateStratified(F&& f)  // albedo.h:42-58 (n = sqrt(1024) = 32)
 {
@@ -64,6 +79,7 @@ static float integrateAvg(const float* f, int size)  // albedo_tables.cpp:23-36
   return std::min(2.f * (sum / float(n - 1)), 1.f);
 }
 
+template <bool kReflOnly>
 static void dielTable(float minEta, float maxEta, float* dir, float* avg)  // albedo_tables.cpp:61-92
 {
   const int size = 16;
@@ -78,7 +94,9 @@ static void dielTable(float minEta, float maxEta, float* dir, float* avg)  // al
       for (int i = 0; i < size; ++i)
       {
         const float c = float(i) / float(size - 1);
-        d[i] = integrateStratified([&](float sx, float sy) { return dielAlbedoSample(c, eta, m, sx, sy); });
+        d[i] = integrateStratified([&](float sx, float sy) {
+          return kReflOnly ? dielReflAlbedoSample(c, eta, m, sx, sy) : dielAlbedoSample(c, eta, m, sx, sy);
+        });
       }
       avg[k * size + j] = integrateAvg(d, size);
     }
@@ -101,8 +119,10 @@ void precomputeLuts(Luts& L)
     L.ggxAvg[j] = integrateAvg(d, 32);
   }
   // albedo_tables.cpp:94-107 (rcp(maxIor) = 1/3, rcp(minIor) = 1)
-  dielTable(1.0f / 3.0f, 1.0f, L.dielEtaDir, L.dielEtaAvg);
-  dielTable(1.0f, 3.0f, L.dielRcpEtaDir, L.dielRcpEtaAvg);
+  dielTable<false>(1.0f / 3.0f, 1.0f, L.dielEtaDir, L.dielEtaAvg);
+  dielTable<false>(1.0f, 3.0f, L.dielRcpEtaDir, L.dielRcpEtaAvg);
+  // albedo_tables.cpp:109-116
+  dielTable<true>(1.0f / 3.0f, 1.0f, L.dielReflDir, L.dielReflAvg);
 }
 
 }  // namespace orc

@@ -870,7 +870,8 @@ void orc_dist2d(const float* w, int W, int H, int n, const float* s, int* idx, f
     for (int x = 0; x < W; ++x) pdfCells[y * W + x] = dist2dPdf(xd.data(), yd.data(), W, H, x, y);
 }
 
-// params: n x [kind r g b er eg eb roughness eta]; geo: n x [U V N Ng wo wi] (18); s: n x 3
+// params: n x [kind r g b er eg eb roughness eta thickness weight]; geo: n x [U V N Ng wo wi] (18);
+// s: n x 3; out: sample n x [weight wi pdf], type n, eval n x [value pdf albedo].  Coated rows: r g b = diffuse color, er eg eb = coat color, eta = ambient/coat IOR.
 // outSample: n x [weight3 wi3 pdf]; outType: n; outEval: n x [value3 pdf]
 // Uses precomputed LUTs `luts` (layout of orc::Luts).
 void orc_bsdf(int n, const float* luts, const float* params, const float* geo, const float* s, float* outSample, int* outType, float* outEval)
@@ -878,7 +879,7 @@ void orc_bsdf(int n, const float* luts, const float* params, const float* geo, c
   const Luts& L = *(const Luts*)luts;
   for (int i = 0; i < n; ++i)
   {
-    const float* pr = params + 9 * i;
+    const float* pr = params + 11 * i;
     const float* g = geo + 18 * i;
     Ctx ctx;
     ctx.f.U = V3(g[0], g[1], g[2]); ctx.f.V = V3(g[3], g[4], g[5]); ctx.f.N = V3(g[6], g[7], g[8]);
@@ -891,7 +892,8 @@ void orc_bsdf(int n, const float* luts, const float* params, const float* geo, c
     b.ggx = GGX(roughnessToAlpha(pr[7]));
     Material m{};
     m.kind = b.kind; m.r = pr[1]; m.g = pr[2]; m.b = pr[3]; m.er = pr[4]; m.eg = pr[5]; m.eb = pr[6]; m.roughness = pr[7];
-    if (b.kind == kDielectric) m.ior = 1.0f / pr[8];  // eta = 1/ior seen from outside
+    if (b.kind == kDielectric || b.kind == kCoated) m.ior = 1.0f / pr[8];  // eta = 1/ior seen from outside
+    m.thickness = pr[9]; m.weight = pr[10];
     // build parameters exactly as makeBsdf does, then keep the supplied frame
     Ctx c2 = ctx;
     Bsdf built = makeBsdf(L, m, c2, wo, 0, b.kind == kDielectric ? (pr[8] > 1.f ? 0 : 1) : -1);
@@ -917,8 +919,9 @@ void orc_bsdf(int n, const float* luts, const float* params, const float* geo, c
     outType[i] = stype;
     float epdf;
     V3 ev = bsdfEval(L, built, ctx, wo, wi, epdf);
-    float* e = outEval + 4 * i;
-    e[0] = ev.x; e[1] = ev.y; e[2] = ev.z; e[3] = epdf;
+    V3 alb = bsdfAlbedo(built);
+    float* e = outEval + 7 * i;
+    e[0] = ev.x; e[1] = ev.y; e[2] = ev.z; e[3] = epdf; e[4] = alb.x; e[5] = alb.y; e[6] = alb.z;
   }
 }
 

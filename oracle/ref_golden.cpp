@@ -15,6 +15,7 @@
 //   luts      bsdf/albedo_tables.cpp:173-186
 //   bsdf      bsdf/diffuse.h:45-89, bsdf/rough_conductor.h:106-228,
 //             bsdf/rough_dielectric.h:265-548, bsdf/multi_bsdf.h:151-260, bsdf/optics.h
+//   coating   bsdf/rough_coating.h:270-593 over bsdf/diffuse.h:45-89, in a MultiBsdfSimd
 //   postisect geometry/triangle_mesh.cpp:152-257, core/ray_simd.h:63-68,
 //             core/shading_context.h:144-191 (makeFrame)
 
@@ -35,6 +36,7 @@
 #include "bsdf/rough_conductor.h"
 #include "bsdf/rough_dielectric.h"
 #include "bsdf/multi_bsdf.h"
+#include "bsdf/rough_coating.h"
 #include "geometry/triangle_mesh.h"
 
 using namespace prt;
@@ -275,6 +277,8 @@ static void goldenLuts()
   dump("lut_diel_eta_avg", "f4", v(MicrofacetDielectricAlbedoTable::etaAvg, 16 * 16), {16, 16});
   dump("lut_diel_rcpeta_dir", "f4", v(MicrofacetDielectricAlbedoTable::rcpEtaDir, 16 * 16 * 16), {16, 16, 16});
   dump("lut_diel_rcpeta_avg", "f4", v(MicrofacetDielectricAlbedoTable::rcpEtaAvg, 16 * 16), {16, 16});
+  dump("lut_diel_refl_dir", "f4", v(MicrofacetDielectricReflectionAlbedoTable::etaDir, 16 * 16 * 16), {16, 16, 16});
+  dump("lut_diel_refl_avg", "f4", v(MicrofacetDielectricReflectionAlbedoTable::etaAvg, 16 * 16), {16, 16});
 }
 
 // BSDF kinds: 0 diffuse, 1 rough conductor (artistic Fresnel), 2 rough dielectric.  Each is
@@ -360,6 +364,87 @@ static void goldenBsdf()
   dump("bsdf_sample", "f4", outSample, {n, 7});
   dump("bsdf_sample_type", "i4", outType, {n});
   dump("bsdf_eval", "f4", outEval, {n, 4});
+}
+
+// Coated diffuse: RoughCoatingBsdfSimd over a DiffuseBrdfSimd, wrapped in a MultiBsdfSimd with
+// weight 1, as StandardMaterial builds its specular layer over the opaque base
+// (material/standard.cpp:512-518) and its coat layer (:580-597).  The eta handed to the coating
+// is ambientIor / coatIor (fromOutside), clamped to [1/3, 3]; color, thickness and weight vary.
+// params: r,g,b (diffuse), cr,cg,cb (coat color), roughness, eta, thickness, weight
+static void goldenCoating()
+{
+  InputRng rng(5151);
+  const int nTotal = 1024;
+  std::vector<float> params, geo, samp, outSample, outEval;
+  std::vector<int32_t> outType;
+  int n = 0;
+  for (int i = 0; i < nTotal; i += simdSize)
+  {
+    const int batch = i / simdSize;
+    Vec3f col(0.02f + 0.96f * rng.f(), 0.02f + 0.96f * rng.f(), 0.02f + 0.96f * rng.f());
+    Vec3f coatCol = (batch % 3 == 0) ? Vec3f(1.f) : Vec3f(0.2f + 0.8f * rng.f(), 0.2f + 0.8f * rng.f(), 0.2f + 0.8f * rng.f());
+    float thickness = (batch % 3 == 0) ? 1.f : 2.f * rng.f();
+    float weight = (batch % 4 == 0) ? 1.f : 0.2f + 0.8f * rng.f();
+    float roughness = std::sqrt(0.005f + 0.6f * rng.f());
+    const float iors[4] = {1.5f, 1.3f, 2.0f, 2.8f};
+    float eta = 1.f / iors[batch % 4];
+
+    Basis3vf frame;
+    ShadingContextSimd* ctx = new ShadingContextSimd();
+    Vec3vf wo, wi, s3;
+    for (int k = 0; k < simdSize; ++k)
+    {
+      Vec3f N = rng.dir();
+      Vec3f U, V;
+      makeFrame(U, V, N);
+      Vec3f Ng = N;
+      if (rng.f() < 0.3f) { Ng = normalize(N + 0.2f * rng.dir()); if (dot(Ng, N) < 0.2f) Ng = N; }
+      Vec3f o = rng.dir();
+      if (rng.f() < 0.9f && dot(o, N) < 0.f) o = -o;
+      Vec3f w = rng.dir();
+      if (rng.f() < 0.8f && dot(w, N) < 0.f) w = -w;
+      frame.U.x[k] = U.x; frame.U.y[k] = U.y; frame.U.z[k] = U.z;
+      frame.V.x[k] = V.x; frame.V.y[k] = V.y; frame.V.z[k] = V.z;
+      frame.N.x[k] = N.x; frame.N.y[k] = N.y; frame.N.z[k] = N.z;
+      ctx->Ng.x[k] = Ng.x; ctx->Ng.y[k] = Ng.y; ctx->Ng.z[k] = Ng.z;
+      wo.x[k] = o.x; wo.y[k] = o.y; wo.z[k] = o.z;
+      wi.x[k] = w.x; wi.y[k] = w.y; wi.z[k] = w.z;
+      s3.x[k] = rng.f(); s3.y[k] = rng.f(); s3.z[k] = rng.f();
+      params.insert(params.end(), {col.x, col.y, col.z, coatCol.x, coatCol.y, coatCol.z, roughness, eta, thickness, weight});
+      geo.insert(geo.end(), {U.x, U.y, U.z, V.x, V.y, V.z, N.x, N.y, N.z, Ng.x, Ng.y, Ng.z, o.x, o.y, o.z, w.x, w.y, w.z});
+      samp.insert(samp.end(), {s3.x[k], s3.y[k], s3.z[k]});
+    }
+    vbool m = one;
+    DiffuseBrdfSimd* diffuse = new DiffuseBrdfSimd(m, &frame, Vec3vf(col));
+    RoughCoatingBsdfSimd* coat = new RoughCoatingBsdfSimd(m, &frame, diffuse, vfloat(eta), Vec3vf(coatCol), vfloat(thickness),
+                                                         vfloat(roughness), vfloat(0.f), vfloat(weight));
+    MultiBsdfSimd multi;
+    multi.add(m, coat, vfloat(1.f));
+
+    Vec3vf swi = zero;
+    vfloat spdf = zero;
+    vint stype = zero;
+    Vec3vf sw = multi.sample(m, *ctx, wo, swi, spdf, stype, BsdfSampleSimd(s3));
+    vfloat epdf;
+    Vec3vf ev = multi.eval(m, *ctx, wo, wi, epdf);
+    Vec3vf alb = multi.getAlbedo(m);
+    for (int k = 0; k < simdSize; ++k)
+    {
+      outSample.insert(outSample.end(), {sw.x[k], sw.y[k], sw.z[k], swi.x[k], swi.y[k], swi.z[k], spdf[k]});
+      outType.push_back(stype[k]);
+      outEval.insert(outEval.end(), {ev.x[k], ev.y[k], ev.z[k], epdf[k], alb.x[k], alb.y[k], alb.z[k]});
+    }
+    delete coat;
+    delete diffuse;
+    delete ctx;
+    n += simdSize;
+  }
+  dump("coat_params", "f4", params, {n, 10});
+  dump("coat_geo", "f4", geo, {n, 18});
+  dump("coat_s", "f4", samp, {n, 3});
+  dump("coat_sample", "f4", outSample, {n, 7});
+  dump("coat_sample_type", "i4", outType, {n});
+  dump("coat_eval", "f4", outEval, {n, 7});
 }
 
 static void goldenPostIntersect()
@@ -459,6 +544,7 @@ int main(int argc, char** argv)
   goldenDistributions();
   goldenLuts();
   goldenBsdf();
+  goldenCoating();
   goldenPostIntersect();
   fprintf(gManifest, "\n}\n");
   fclose(gManifest);

@@ -158,6 +158,13 @@ def test_render_parity_sunsky_mixed(dev):
     _parity(dev, scene, 96, 64, 3, 0.97)
 
 
+def test_render_parity_sunsky_coated(dev):
+    # every diffuse palette entry coated (kind 3, RoughCoatingBsdf over a diffuse base)
+    scene = G.spheres_sunsky(n_spheres=48, subdiv=2, palette=24, coated=1.0)
+    assert (scene["materials"]["kind"] == G.KIND_COATED).sum() >= 5
+    _parity(dev, scene, 96, 64, 3, 0.97)
+
+
 def test_render_stats_and_error(dev):
     scene = G.spheres_ambient(n_spheres=16, subdiv=1)
     P.load_scene(dev, scene)

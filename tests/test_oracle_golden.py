@@ -90,6 +90,12 @@ def test_energy_compensation_luts(golden):
     np.testing.assert_allclose(L["diel_rcpeta_avg"], golden["lut_diel_rcpeta_avg"].ravel(), atol=2e-5)
     # the eta average integrates the singular column with weight ~0: still within 1e-3
     np.testing.assert_allclose(L["diel_eta_avg"], golden["lut_diel_eta_avg"].ravel(), atol=1e-3)
+    # MicrofacetDielectricReflectionAlbedoTable (albedo_tables.cpp:109-116), read by the coating
+    refl_dir = L["diel_refl_dir"].reshape(16, 16, 16).copy()
+    ref_refl_dir = golden["lut_diel_refl_dir"].copy()
+    refl_dir[:, 15, 0] = ref_refl_dir[:, 15, 0] = 0
+    np.testing.assert_allclose(refl_dir, ref_refl_dir, atol=2e-5)
+    np.testing.assert_allclose(L["diel_refl_avg"], golden["lut_diel_refl_avg"].ravel(), atol=1e-3)
 
 
 @pytest.mark.parametrize("kind", [0, 1, 2])
@@ -115,6 +121,29 @@ def test_bsdf_sample_eval(golden, kind):
     erough0 = p[m][:, 7] == 0
     np.testing.assert_allclose(ev[~erough0, 3], ref_e[~erough0, 3], rtol=1e-3, atol=1e-6)
     np.testing.assert_allclose(ev[erough0, 3], ref_e[erough0, 3], rtol=1e-2, atol=1e-6)
+
+
+def test_coated_diffuse_sample_eval(golden):
+    # bsdf/rough_coating.h:297-593 over bsdf/diffuse.h:55-68 in MultiBsdfSimd: absorption through
+    # the coat, Kelemen energy conservation, coat/base pick by importance; albedo :567-571
+    flat, _ = O.compute_luts()
+    cp = golden["coat_params"]
+    p = np.concatenate([np.full((cp.shape[0], 1), 3, np.float32), cp], 1)
+    so, st, ev = O.bsdf(flat, p, golden["coat_geo"], golden["coat_s"])
+    ref_s, ref_t, ref_e = golden["coat_sample"], golden["coat_sample_type"], golden["coat_eval"]
+    valid_ref = np.abs(ref_s[:, :3]).max(1) > 0
+    valid = np.abs(so[:, :3]).max(1) > 0
+    assert valid.sum() > 0.8 * len(valid)
+    assert np.array_equal(valid, valid_ref)
+    assert np.array_equal(st[valid], ref_t[valid])
+    assert set(np.unique(st[valid])) == {1, 2}  # both the base and the coat get sampled
+    np.testing.assert_allclose(so[valid, :3], ref_s[valid, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(so[valid, 3:6], ref_s[valid, 3:6], atol=2e-5)
+    np.testing.assert_allclose(so[valid, 6], ref_s[valid, 6], rtol=1e-3)
+    assert (np.abs(ev[:, :3]).max(1) > 0).mean() > 0.6
+    np.testing.assert_allclose(ev[:, :3], ref_e[:, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(ev[:, 3], ref_e[:, 3], rtol=1e-3, atol=1e-6)
+    np.testing.assert_allclose(ev[:, 4:7], ref_e[:, 4:7], rtol=1e-5)
 
 
 @pytest.mark.parametrize("suffix", ["", "_n"])
