@@ -27,7 +27,7 @@ extern "C" {
 
 typedef struct prt_context prt_context;
 
-/* Material record (44 bytes).  Uniform StandardMaterial parameter sets of the stream path
+/* Material record (56 bytes).  Uniform StandardMaterial parameter sets of the stream path
  * (material/standard.cpp:21-183 ctor, :409-597 SIMD lobes), each a MultiBsdf with one lobe:
  *   kind 0 diffuse    : baseColor = (r,g,b), specularWeight 0          -> DiffuseBrdf
  *   kind 1 conductor  : metalness 1, baseColor = (r,g,b), specularColor = (er,eg,eb),
@@ -43,6 +43,9 @@ typedef struct prt_context prt_context;
  *                       over a diffuse base (specularWeight 0, coatWeight w) takes coatColor,
  *                       coatThickness, coatRoughness and coatIOR (standard.cpp:580-597).
  * thickness and weight are read by kind 3 only.
+ * emission = emissionColor * emissionLuminance (standard.cpp:71-74, :663-666), any kind: a
+ * non-zero emission makes the material emissive, its triangles form the scene's area light
+ * (light/triangle_light.h, render/scene.cpp:83-116) and a path that hits one ends there.
  * Material ID 0 is reserved for the environment (render/scene.cpp:204-207). */
 typedef struct prt_material {
   int32_t kind;
@@ -52,6 +55,7 @@ typedef struct prt_material {
   float ior;
   float thickness;
   float weight;
+  float emission[3];
 } prt_material;
 
 /* Environment light record (32 bytes): light/ambient_light.h (kind 0, Ke = rgb),
@@ -128,9 +132,11 @@ int prt_get_info(prt_context* ctx, char* buf, int len);
 int prt_scene_set_geometry(prt_context* ctx, int32_t nverts, const float* positions, const float* normals,
                            int32_t ntris, const int32_t* indices, const int32_t* tri_material);
 int prt_scene_set_materials(prt_context* ctx, int32_t n, const prt_material* materials);
-/* Device::initLights (render/scene.cpp:365-450): light list in reference order, optional
- * lat-long environment image env_rgb (env_h x env_w x 3, texture row order: row 0 = nadir,
- * image/image_texture.h:396), lut_max as the image light's lutSize prop. */
+/* Device::initLights (render/scene.cpp:365-450): n (0..8) environment lights in reference
+ * order, optional lat-long environment image env_rgb (env_h x env_w x 3, texture row order:
+ * row 0 = nadir, image/image_texture.h:396), lut_max as the image light's lutSize prop.  Call
+ * after the geometry and materials: the area light (emissive triangles) heads the light list
+ * and is built here; with no lights at all a default ambient light (Ke = 1) is added. */
 int prt_scene_set_lights(prt_context* ctx, int32_t n, const prt_light* lights, int32_t env_w, int32_t env_h,
                          const float* env_rgb, int32_t lut_max);
 /* BVH build on the device (replaces the Embree build, accel/embree_intersector.cpp:11-90).

@@ -252,14 +252,17 @@ struct prt_context
   int nverts = 0, ntris = 0, maxMaterialId = 0;
   DevBuf<float> pos, nrm;
   DevBuf<int> idx, triMat;
-  std::vector<float> hostPos;
-  std::vector<int> hostIdx;
   std::vector<prt_material> materials;
   DevBuf<prt_material_constants> mats;
   std::vector<float> luts;
   DevBuf<float> lut;
   std::vector<prt_light_dev> lights;
   std::vector<float> lightCdf;
+  DevBuf<float4> areaTris;
+  DevBuf<float> areaCdf;
+  DevBuf<int> areaMap;
+  int areaCount = 0;
+  float areaPower = 0.f;
   DevBuf<float4> envTex;
   DevBuf<float> envX, envY;
   int envW = 0, envH = 0, envLW = 0, envLH = 0;
@@ -460,6 +463,12 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
     q.roughness = hclamp(s.roughness, 0.f, 1.f);
     q.alpha = std::max(hsqr(q.roughness), 0.001f);  // bsdf/microfacet.h:10-24
     q.importance = 1.f;
+    for (int j = 0; j < 3; ++j)
+    {
+      if (!(s.emission[j] >= 0.f)) return fail(PRT_ERR_INVALID, "emission must be non-negative");
+      q.emission[j] = s.emission[j];
+    }
+    q.emissive = (s.emission[0] > 0.f || s.emission[1] > 0.f || s.emission[2] > 0.f) ? 1 : 0;
     float rgb[3] = {hclamp(s.r, 0.f, 1.f), hclamp(s.g, 0.f, 1.f), hclamp(s.b, 0.f, 1.f)};
     if (s.kind == PRT_KIND_DIFFUSE)
     {
@@ -536,6 +545,67 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
   }
   HIP_TRY(c->mats.alloc(n));
   HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * n, hipMemcpyHostToDevice));
+  c->lightsReady = false;  // the area light depends on the materials: set the lights again
+  return PRT_OK;
+}
+
+// light/triangle_light.h:36-64 + render/scene.cpp:83-116: one TriangleLight over the emissive
+// triangles of the mesh (the uploaded geometry is one TriangleMesh) in primitive order, picked
+// by power |cross(e1, e2)| * luminance(emission).  The geometry is read back from the device
+// only when some material emits.
+static int buildAreaLight(prt_context* c)
+{
+  c->areaCount = 0;
+  c->areaPower = 0.f;
+  c->areaTris.release(); c->areaCdf.release(); c->areaMap.release();
+  bool anyEmissive = false;
+  for (const prt_material& m : c->materials)
+    anyEmissive |= (m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f);
+  if (!anyEmissive || c->ntris == 0) return PRT_OK;
+  std::vector<int> triMat(c->ntris), idx(size_t(c->ntris) * 3);
+  std::vector<float> pos(size_t(c->nverts) * 3);
+  HIP_TRY(hipMemcpy(triMat.data(), c->triMat.p, sizeof(int) * triMat.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(idx.data(), c->idx.p, sizeof(int) * idx.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(pos.data(), c->pos.p, sizeof(float) * pos.size(), hipMemcpyDeviceToHost));
+  std::vector<int> map(c->ntris, -1);
+  std::vector<float4> tris;
+  std::vector<float> w;
+  for (int t = 0; t < c->ntris; ++t)
+  {
+    if (triMat[t] >= int(c->materials.size())) return fail(PRT_ERR_INVALID, "triangle material ID beyond the material table");
+    const prt_material& m = c->materials[triMat[t]];
+    if (!(m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f)) continue;
+    const float* p0 = &pos[3 * size_t(idx[3 * t])];
+    const float* p1 = &pos[3 * size_t(idx[3 * t + 1])];
+    const float* p2 = &pos[3 * size_t(idx[3 * t + 2])];
+    H3 v0 = hmk(p0[0], p0[1], p0[2]), v1 = hmk(p1[0], p1[1], p1[2]), v2 = hmk(p2[0], p2[1], p2[2]);
+    H3 nrm = hcross(hsub(v1, v0), hsub(v2, v0));
+    map[t] = int(tris.size() / 4);
+    tris.push_back(make_float4(v0.x, v0.y, v0.z, m.emission[0]));
+    tris.push_back(make_float4(v1.x, v1.y, v1.z, m.emission[1]));
+    tris.push_back(make_float4(v2.x, v2.y, v2.z, m.emission[2]));
+    tris.push_back(make_float4(nrm.x, nrm.y, nrm.z, 0.f));
+    float lum = 0.212671f * m.emission[0] + 0.715160f * m.emission[1] + 0.072169f * m.emission[2];
+    w.push_back(std::sqrt(hdot(nrm, nrm)) * lum);
+  }
+  const int n = int(w.size());
+  if (n == 0) return PRT_OK;
+  std::vector<float> cdf(n + 1);
+  cdf[0] = 0.f;  // sampling/distribution1d.cpp:18-32
+  for (int i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + w[i - 1];
+  float norm = cdf[n] == 0.f ? 0.f : 1.0f / cdf[n];
+  for (int i = 1; i <= n; ++i) cdf[i] *= norm;
+  cdf[n] = 1.f;
+  float sumL = 0.f;  // getPower (triangle_light.h:152-162)
+  for (int i = 0; i < n; ++i) sumL += w[i];
+  c->areaPower = 3.14159265358979323846f * sumL;
+  HIP_TRY(c->areaTris.alloc(tris.size()));
+  HIP_TRY(c->areaCdf.alloc(cdf.size()));
+  HIP_TRY(c->areaMap.alloc(map.size()));
+  HIP_TRY(hipMemcpy(c->areaTris.p, tris.data(), sizeof(float4) * tris.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->areaCdf.p, cdf.data(), sizeof(float) * cdf.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->areaMap.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+  c->areaCount = n;
   return PRT_OK;
 }
 
@@ -543,7 +613,21 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
                          int32_t lut_max)
 {
   CTX_CHECK(c);
-  if (n < 1 || n > PRT_MAX_LIGHTS || !lights) return fail(PRT_ERR_INVALID, "1..8 lights required");
+  if (n < 0 || n > PRT_MAX_LIGHTS || (n > 0 && !lights)) return fail(PRT_ERR_INVALID, "0..8 environment lights");
+  if (c->materials.empty()) return fail(PRT_ERR_STATE, "materials must be set before the lights");
+  {
+    int rc = buildAreaLight(c);
+    if (rc != PRT_OK) return rc;
+  }
+  prt_light defaultAmbient{};
+  if (n == 0 && c->areaCount == 0)
+  {
+    // render/scene.cpp:432-438: no lights at all -> a default ambient light (Ke = 1)
+    defaultAmbient.kind = PRT_LIGHT_AMBIENT;
+    defaultAmbient.r = defaultAmbient.g = defaultAmbient.b = 1.f;
+    lights = &defaultAmbient;
+    n = 1;
+  }
   bool needImage = false;
   for (int i = 0; i < n; ++i) needImage |= lights[i].kind == PRT_LIGHT_IMAGE;
   if (needImage && (!env_rgb || env_w <= 0 || env_h <= 0)) return fail(PRT_ERR_INVALID, "image light without an image");
@@ -629,12 +713,15 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
     else
       return fail(PRT_ERR_INVALID, "unknown light kind");
   }
-  c->lightCdf.assign(n + 1, 0.f);
-  c->lightCdf[0] = 0.f;
-  for (int i = 1; i <= n; ++i) c->lightCdf[i] = c->lightCdf[i - 1] + power[i - 1];
-  float norm = c->lightCdf[n] == 0.f ? 0.f : 1.0f / c->lightCdf[n];
-  for (int i = 1; i <= n; ++i) c->lightCdf[i] *= norm;
-  c->lightCdf[n] = 1.f;
+  // render/scene.cpp:440-445: the light list is the area light (if any), then the
+  // environment lights; one power distribution over all of them
+  if (c->areaCount > 0) power.insert(power.begin(), c->areaPower);
+  const int total = int(power.size());
+  c->lightCdf.assign(total + 1, 0.f);
+  for (int i = 1; i <= total; ++i) c->lightCdf[i] = c->lightCdf[i - 1] + power[i - 1];
+  float norm = c->lightCdf[total] == 0.f ? 0.f : 1.0f / c->lightCdf[total];
+  for (int i = 1; i <= total; ++i) c->lightCdf[i] *= norm;
+  c->lightCdf[total] = 1.f;
   c->lightsReady = true;
   return PRT_OK;
 }
@@ -767,8 +854,10 @@ static RenderArgs makeArgs(prt_context* c)
   a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.lut = c->lut.p;
   a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
   a.nLights = int(c->lights.size());
+  a.lightOffset = c->areaCount > 0 ? 1 : 0;
   for (int i = 0; i < a.nLights; ++i) a.lights[i] = c->lights[i];
-  for (int i = 0; i <= a.nLights; ++i) a.lightCdf[i] = c->lightCdf[i];
+  for (int i = 0; i <= a.nLights + a.lightOffset; ++i) a.lightCdf[i] = c->lightCdf[i];
+  a.areaTris = c->areaTris.p; a.areaCdf = c->areaCdf.p; a.areaMap = c->areaMap.p; a.areaCount = c->areaCount;
   a.env.texels = c->envTex.p; a.env.xDists = c->envX.p; a.env.yDist = c->envY.p;
   a.env.W = c->envW; a.env.H = c->envH; a.env.LW = c->envLW; a.env.LH = c->envLH;
   // camera/pinhole_camera.h:72-90 and thin_lens_camera.h:155-161

@@ -35,9 +35,16 @@ struct RenderArgs
   const prt_material_constants* mats;
   const float* lut;
   BvhView bvh;
-  prt_light_dev lights[PRT_MAX_LIGHTS];
-  float lightCdf[PRT_MAX_LIGHTS + 1];
-  int nLights;
+  prt_light_dev lights[PRT_MAX_LIGHTS];  // environment lights
+  float lightCdf[PRT_MAX_LIGHTS + 2];    // power distribution over [area light,] environment lights
+  int nLights;                           // environment lights
+  int lightOffset;                       // 1 if the area light heads the light list
+  // area light (light/triangle_light.h): emissive triangles as 4 float4 (v0 | r, v1 | g, v2 | b,
+  // n = cross(v1 - v0, v2 - v0)), their power CDF, and primitive -> emissive index
+  const float4* areaTris;
+  const float* areaCdf;
+  const int* areaMap;
+  int areaCount;
   EnvView env;
   // camera (precomputed as PinholeCamera / ThinLensCamera constructors do)
   float camPos[3], imageO[3], imageDx[3], imageDy[3], camU[3], camV[3], camN[3];

@@ -48,7 +48,9 @@ struct prt_material_constants
   float EavgRcpEta[2];
   float fmsRatio[2];
   int32_t mediumInt;  // interior medium ID (0 for non-dielectrics)
-  int32_t pad[6];
+  float emission[3];  // emissionColor * emissionLuminance
+  int32_t emissive;   // hits end the path as area-light hits (render/path_renderer_stream.h:419-582)
+  int32_t pad[2];
 };
 
 // Environment light record as the kernels read it.  64 bytes.

@@ -110,20 +110,63 @@ PRT_DEV f3 evalEnv(const RenderArgs& a, f3 wi, float& pdf, uint32_t& hitMask)
     float cp;
     v = v + envEvalOne(a.lights[i], a.env, wi, cp);
     if (cp > 0.f) hitMask |= 1u << i;
-    pdf += cp * (a.lightCdf[i + 1] - a.lightCdf[i]);
+    const int k = a.lightOffset + i;
+    pdf += cp * (a.lightCdf[k + 1] - a.lightCdf[k]);
   }
   return v;
 }
 
-// render/scene.h:215-250 (Distribution1D::sampleReuse for the pick, s.c remapped)
-PRT_DEV f3 sampleLight(const RenderArgs& a, f3& wi, float& pdf, float sx, float sy, float sc, int& picked)
+// light/triangle_light.h:84-100 + render/scene.h:159-195: the area light seen along wi at
+// distance dist on primitive prim
+PRT_DEV f3 evalArea(const RenderArgs& a, f3 wi, float dist, int prim, float& pdf)
+{
+  pdf = 0.f;
+  int tri = a.areaMap[prim];
+  float pickProb = dist1dPdf(a.areaCdf, a.areaCount, tri);
+  float4 nn = a.areaTris[4 * tri + 3];
+  float dirDotN = dot(wi, f3(nn.x, nn.y, nn.z));
+  if (dirDotN >= 0.f) return f3(0.f);
+  pdf = 2.0f * dist * dist * rcpSafe(fabsf(dirDotN)) * pickProb;
+  pdf *= a.lightCdf[1] - a.lightCdf[0];
+  return f3(a.areaTris[4 * tri].w, a.areaTris[4 * tri + 1].w, a.areaTris[4 * tri + 2].w);
+}
+
+// light/triangle_light.h:123-150: triangle by power, point uniform on it (shape_sampler.h:
+// 135-152, Heitz 2019 map); wi is left unchanged when the sample faces away
+PRT_DEV f3 sampleArea(const RenderArgs& a, f3 p, f3& wi, float& pdf, float& dist, float sx, float sy, float sc)
+{
+  pdf = 0.f;
+  int tri = dist1dLookup(a.areaCdf, a.areaCount, sc);
+  float pickProb = a.areaCdf[tri + 1] - a.areaCdf[tri];
+  float4 A = a.areaTris[4 * tri], B = a.areaTris[4 * tri + 1], C = a.areaTris[4 * tri + 2], Nn = a.areaTris[4 * tri + 3];
+  float u = sx, v = sy;
+  if (v > u) { u *= 0.5f; v -= u; }
+  else { v *= 0.5f; u -= v; }
+  f3 v0(A.x, A.y, A.z), v1(B.x, B.y, B.z), v2(C.x, C.y, C.z);
+  f3 dir = (v2 + u * (v0 - v2) + v * (v1 - v2)) - p;
+  dist = length(dir);
+  float dirDotN = dot(dir, f3(Nn.x, Nn.y, Nn.z));
+  if (!(dirDotN < 0.f)) return f3(0.f);
+  wi = dir * (1.0f / dist);
+  pdf = 2.0f * dist * dist * dist * rcpSafe(fabsf(dirDotN)) * pickProb;
+  return f3(A.w, B.w, C.w) * (1.0f / pdf);
+}
+
+// render/scene.h:215-250 (Distribution1D::sampleReuse for the pick, s.c remapped); the light
+// list is [area light,] environment lights.  dist: distance to the sampled point (area light)
+// or kPosMax.
+PRT_DEV f3 sampleLight(const RenderArgs& a, f3 p, f3& wi, float& pdf, float& dist, float sx, float sy, float sc, int& picked)
 {
   picked = 0;
-  if (a.nLights == 1) return envSampleOne(a.lights[0], a.env, wi, pdf, sx, sy);
+  dist = kPosMax;
+  const int total = a.lightOffset + a.nLights;
+  if (total == 1)
+    return a.lightOffset ? sampleArea(a, p, wi, pdf, dist, sx, sy, sc) : envSampleOne(a.lights[0], a.env, wi, pdf, sx, sy);
   float pick;
-  int id = dist1dSampleReuse(a.lightCdf, a.nLights, pick, sc);
+  int id = dist1dSampleReuse(a.lightCdf, total, pick, sc);
   picked = id;
-  f3 w = envSampleOne(a.lights[id], a.env, wi, pdf, sx, sy);
+  f3 w = (a.lightOffset && id == 0) ? sampleArea(a, p, wi, pdf, dist, sx, sy, sc)
+                                     : envSampleOne(a.lights[id - a.lightOffset], a.env, wi, pdf, sx, sy);
   pdf *= pick;
   return w * (1.0f / pick);
 }
@@ -288,6 +331,31 @@ __global__ void __launch_bounds__(kShadeBlock) kShade(const RenderArgs* __restri
         }
         finalizePath(a, slot, L, hsh);
       }
+      else if (a.mats[matId].emissive)
+      {
+        // area-light hit (:419-582), at any depth: emission with MIS, then the path ends
+        float lightPdf;
+        f3 Le = evalArea(a, d, h.x, prim, lightPdf);
+        float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
+        L = L + T * w * Le;
+        f3 Nv = normalize(-d);
+        if (!(fl & PRT_F_AOV1))
+        {
+          addAov3(a.fb.albedo1, a.fb.albedo1W, a.npix, pix, f3(1.f));
+          addAov3(a.fb.normal1, a.fb.normal1W, a.npix, pix, Nv);
+          if (a.fb.depth)
+          {
+            a.fb.depth[pix] += dot((o + d * h.x) - ld3(a.camPos), -ld3(a.camN));
+            a.fb.depthW[pix] += 1.f;
+          }
+        }
+        if (!(fl & PRT_F_AOV2))
+        {
+          addAov3(a.fb.albedo, a.fb.albedoW, a.npix, pix, f3(1.f));
+          addAov3(a.fb.normal, a.fb.normalW, a.npix, pix, Nv);
+        }
+        finalizePath(a, slot, L, hsh);
+      }
       else if (depth >= a.maxDepth)
       {
         finalizePath(a, slot, L, hsh);  // no shading beyond maxDepth (:615)
@@ -309,10 +377,10 @@ __global__ void __launch_bounds__(kShadeBlock) kShade(const RenderArgs* __restri
         {
           float s0, s1, s2;
           smp.get3D(sampleDim + 3, s0, s1, s2);
-          f3 wiL;
-          float lightPdf;
+          f3 wiL(0.f);
+          float lightPdf, lightDist;
           int picked;
-          f3 lw = sampleLight(a, wiL, lightPdf, s0, s1, s2, picked);
+          f3 lw = sampleLight(a, ctx.p, wiL, lightPdf, lightDist, s0, s1, s2, picked);
           if (a.hash) hsh = hashMix(hashMix(hsh, 0x200u | uint32_t(picked)), dirCell(wiL));
           if (reduceMax(lw) > 0.f && lightPdf > 0.f)
           {
@@ -329,7 +397,7 @@ __global__ void __launch_bounds__(kShadeBlock) kShade(const RenderArgs* __restri
               float w = misWeight(lightPdf, bsdfPdf);
               dirL = dirT * w * lw;
               a.shO[slot] = make_float4(ctx.p.x, ctx.p.y, ctx.p.z, ctx.eps);
-              a.shD[slot] = make_float4(wiL.x, wiL.y, wiL.z, kPosMax - ctx.eps);
+              a.shD[slot] = make_float4(wiL.x, wiL.y, wiL.z, lightDist - ctx.eps);  // [eps, dist - eps] (:686)
               a.dirL[slot] = make_float4(dirL.x, dirL.y, dirL.z, 0.f);
               wantShadow = true;
               fl |= PRT_F_DIRECT;

@@ -22,7 +22,8 @@ AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR = 1, 2, 4, 
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
-                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4")])
+                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
+                           ("emission", "<f4", 3)])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),

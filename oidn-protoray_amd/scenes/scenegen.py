@@ -36,7 +36,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
-                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4")])
+                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
+                           ("emission", "<f4", 3)])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
@@ -162,7 +163,11 @@ def _assemble(rng, n_spheres, subdiv, sphere_material):
     return pos.astype(np.float32), idx.astype(np.int32), tri_mat.astype(np.int32)
 
 
-def spheres_ambient(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=16, max_depth=5):
+def spheres_ambient(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=16, max_depth=5, emissive=0,
+                    emission=4.0, ambient=1.0):
+    """emissive: the first `emissive` spheres also emit emission x a random color (an area light
+    over their triangles, drawn from a second PCG32 stream so the geometry is unchanged);
+    ambient: radiance of the ambient light, None for no environment light at all."""
     rng = Pcg32(seed)
     mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
     sphere_mats = []
@@ -172,8 +177,12 @@ def spheres_ambient(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=
         return 2 + i
 
     pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, pick)
-    lights = np.zeros(1, LIGHT_DTYPE)
-    lights[0] = (LIGHT_AMBIENT, 1.0, 1.0, 1.0, 0, 0, 0, 0)
+    erng = Pcg32(seed + 7919)
+    for i in range(min(emissive, n_spheres)):
+        sphere_mats[i]["emission"] = (emission * erng.f(0.3, 1.0), emission * erng.f(0.3, 1.0), emission * erng.f(0.3, 1.0))
+    lights = np.zeros(0 if ambient is None else 1, LIGHT_DTYPE)
+    if ambient is not None:
+        lights[0] = (LIGHT_AMBIENT, ambient, ambient, ambient, 0, 0, 0, 0)
     return dict(name="spheres_ambient", positions=pos, indices=idx, tri_material=tri_mat,
                 materials=np.array(mats + sphere_mats, MATERIAL_DTYPE), lights=lights, env=None,
                 camera=_camera(), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)

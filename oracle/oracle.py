@@ -31,7 +31,8 @@ _lib = None
 
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
-                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4")])
+                           ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
+                           ("emission", "<f4", 3)])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
@@ -77,7 +78,8 @@ def lib():
         L.orc_bsdf.argtypes = [i, vp, vp, vp, vp, vp, vp, vp]
         L.orc_post_intersect.argtypes = [i, vp, vp, vp, i, vp, vp, vp, vp]
         L.orc_env_eval.argtypes = [vp, i, vp, vp]
-        L.orc_light_sample.argtypes = [vp, i, vp, vp]
+        L.orc_light_sample.argtypes = [vp, i, vp, vp, vp]
+        L.orc_light_count.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -207,8 +209,12 @@ class Scene:
     def radius(self):
         return lib().orc_scene_radius(self.h)
 
+    def light_count(self):
+        """Lights in the sampling list: the area light (emissive triangles) first, if any."""
+        return lib().orc_light_count(self.h)
+
     def light_cdf(self):
-        out = np.zeros(self.lights.shape[0] + 1, np.float32)
+        out = np.zeros(self.light_count() + 1, np.float32)
         lib().orc_get_light_cdf(self.h, out.ctypes.data)
         return out
 
@@ -226,10 +232,12 @@ class Scene:
         lib().orc_env_eval(self.h, dirs.shape[0], dirs.ctypes.data, out.ctypes.data)
         return out
 
-    def light_sample(self, s3):
+    def light_sample(self, s3, org=(0.0, 0.0, 0.0)):
+        """n x [wi3 weight3 pdf dist] for light samples taken from the point org."""
         s3 = c32(s3)
+        o = c32(org)
         out = np.zeros((s3.shape[0], 8), np.float32)
-        lib().orc_light_sample(self.h, s3.shape[0], s3.ctypes.data, out.ctypes.data)
+        lib().orc_light_sample(self.h, s3.shape[0], s3.ctypes.data, o.ctypes.data, out.ctypes.data)
         return out
 
     def render(self, camera, W, H, spp_start=0, spp_count=1, seed=0, max_depth=5, rr_depth=8, min_weight=1e-4,

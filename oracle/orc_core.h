@@ -552,7 +552,10 @@ struct Material  // 32-byte record shared with the HIP path's material table
   float ior;         // specularIOR (dielectric) / coat IOR (coated), relative to an ambient IOR of 1
   float thickness;   // coated: coat thickness (absorption path scale)
   float weight;      // coated: coat weight (specularWeight / coatWeight)
+  float emission[3]; // emissionColor * emissionLuminance; non-zero makes the material an area light
 };
+
+inline bool isEmissive(const Material& m) { return m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f; }
 
 struct Ctx  // core/shading_context.h:113-124 subset
 {
@@ -1041,14 +1044,62 @@ inline V3 envLookup(const EnvImage& e, float u, float v)
   return c0 + sy * (c1 - c0);
 }
 
+// light/triangle_light.h: the emissive triangles of the mesh as one area light, triangles
+// picked by power (|cross(e1, e2)| * luminance), uniform on the picked triangle.
+struct AreaLight
+{
+  std::vector<V3> v0, v1, v2, n, color;  // n = cross(v1 - v0, v2 - v0), not normalized
+  std::vector<float> cdf;                // Distribution1D over the emissive triangles
+  std::vector<int> triMap;               // primitive ID -> emissive triangle index or -1
+  float power = 0.f;                     // getPower (:152-162)
+};
+
+// light list of render/scene.cpp:365-450: the area light first (if any), then the
+// environment lights; lightDistribution over their powers
 struct LightSet
 {
-  int count = 0;
+  int count = 0;                 // environment lights
+  int offset = 0;                // 1 if an area light precedes them
   const Light* lights = nullptr;
-  float cdf[9];          // light power distribution (Distribution1D over <= 8 lights)
+  const AreaLight* area = nullptr;
+  float cdf[10];                 // light power distribution (Distribution1D over <= 9 lights)
   const EnvImage* env = nullptr;
   float sceneRadius = 0.f;
 };
+
+// sampling/shape_sampler.h:135-152 [Heitz 2019, low-distortion map triangle <- square]
+inline V3 uniformSampleTriangle(float sx, float sy, V3 v0, V3 v1, V3 v2)
+{
+  float u = sx, v = sy;
+  if (v > u) { u *= 0.5f; v -= u; }
+  else { v *= 0.5f; u -= v; }
+  return v2 + u * (v0 - v2) + v * (v1 - v2);
+}
+// triangle_light.h:84-100 (eval on a hit of an emissive triangle at distance dist)
+inline V3 areaLightEval(const AreaLight& a, V3 wi, float dist, int prim, float& pdf)
+{
+  int tri = a.triMap[prim];
+  float pickProb = dist1dPdf(a.cdf.data(), int(a.v0.size()), tri);
+  float dirDotN = dot(wi, a.n[tri]);
+  if (dirDotN >= 0.f) { pdf = 0.f; return V3(0.f); }
+  pdf = 2.0f * dist * dist * rcpSafe(std::fabs(dirDotN)) * pickProb;
+  return a.color[tri];
+}
+// triangle_light.h:123-150 (wi is left unchanged when the sample faces away)
+inline V3 areaLightSample(const AreaLight& a, V3 p, V3& wi, float& pdf, float& dist, float sx, float sy, float sc)
+{
+  pdf = 0.f;
+  float pickProb;
+  int tri = dist1dLookup(a.cdf.data(), int(a.v0.size()), sc);
+  pickProb = a.cdf[tri + 1] - a.cdf[tri];
+  V3 dir = uniformSampleTriangle(sx, sy, a.v0[tri], a.v1[tri], a.v2[tri]) - p;
+  dist = length(dir);
+  float dirDotN = dot(dir, a.n[tri]);
+  if (!(dirDotN < 0.f)) return V3(0.f);
+  wi = dir * (1.0f / dist);
+  pdf = 2.0f * dist * dist * dist * rcpSafe(std::fabs(dirDotN)) * pickProb;
+  return a.color[tri] * (1.0f / pdf);
+}
 
 // ambient_light.h:53-57, distant_light.h:73-77, image_light.h:137-148
 inline float lightPower(const Light& l, const EnvImage* env, float sceneRadius)
@@ -1131,21 +1182,31 @@ inline V3 evalEnv(const LightSet& ls, V3 wi, float& pdf, uint32_t* hitMask = nul
     V3 v = envLightEval(ls.lights[i], ls.env, wi, cp);
     if (cp > 0.f) mask |= 1u << i;
     value += v;
-    pdf += cp * dist1dPdf(ls.cdf, ls.count, i);
+    pdf += cp * dist1dPdf(ls.cdf, ls.offset + ls.count, ls.offset + i);
   }
   if (hitMask) *hitMask = mask;
   return value;
 }
+// render/scene.h:159-195: the area light seen along wi at distance dist on primitive prim
+inline V3 evalArea(const LightSet& ls, V3 wi, float dist, int prim, float& pdf)
+{
+  V3 v = areaLightEval(*ls.area, wi, dist, prim, pdf);
+  pdf *= dist1dPdf(ls.cdf, ls.offset + ls.count, 0);
+  return v;
+}
 // render/scene.h:215-250
-inline V3 sampleLight(const LightSet& ls, V3& wi, float& pdf, float& dist, float sx, float sy, float sc, int* picked = nullptr)
+inline V3 sampleLight(const LightSet& ls, V3 p, V3& wi, float& pdf, float& dist, float sx, float sy, float sc, int* picked = nullptr)
 {
   if (picked) *picked = 0;
-  if (ls.count == 1) return envLightSample(ls.lights[0], ls.env, wi, pdf, dist, sx, sy);
+  const int total = ls.offset + ls.count;
+  if (total == 1)
+    return ls.area ? areaLightSample(*ls.area, p, wi, pdf, dist, sx, sy, sc) : envLightSample(ls.lights[0], ls.env, wi, pdf, dist, sx, sy);
   float pick;
   float c = sc;
-  int id = dist1dSampleReuse(ls.cdf, ls.count, pick, c);
+  int id = dist1dSampleReuse(ls.cdf, total, pick, c);
   if (picked) *picked = id;
-  V3 w = envLightSample(ls.lights[id], ls.env, wi, pdf, dist, sx, sy);
+  V3 w = (ls.area && id == 0) ? areaLightSample(*ls.area, p, wi, pdf, dist, sx, sy, c)
+                              : envLightSample(ls.lights[id - ls.offset], ls.env, wi, pdf, dist, sx, sy);
   pdf *= pick;
   return w * (1.0f / pick);
 }

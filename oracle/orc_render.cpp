@@ -31,6 +31,7 @@ struct Scene
   std::vector<float> envRgb, envX, envY;
   EnvImage env;
   LightSet ls;
+  AreaLight area;
   float sceneRadius = 0.f;
   V3 bmin, bmax;
   std::vector<Node> nodes;
@@ -55,6 +56,33 @@ static void assignMedia(Scene& s)
     if (id < 0) { iors.push_back(ior); id = int(iors.size()); }
     s.matMedium[m] = id;
   }
+}
+
+// light/triangle_light.h:36-64 + render/scene.cpp:83-116: the emissive triangles of the mesh
+// (one TriangleMesh: the flattened geometry) in primitive order, weights |n| * luminance
+static void buildAreaLight(Scene& s)
+{
+  AreaLight& a = s.area;
+  a = AreaLight();
+  a.triMap.assign(s.triMat.size(), -1);
+  std::vector<float> w;
+  for (size_t t = 0; t < s.triMat.size(); ++t)
+  {
+    const Material& m = s.mats[s.triMat[t]];
+    if (s.triMat[t] == 0 || !isEmissive(m)) continue;
+    V3 p0 = s.pos[s.idx[3 * t]], p1 = s.pos[s.idx[3 * t + 1]], p2 = s.pos[s.idx[3 * t + 2]];
+    V3 n = cross(p1 - p0, p2 - p0);
+    V3 c(m.emission[0], m.emission[1], m.emission[2]);
+    a.triMap[t] = int(a.v0.size());
+    a.v0.push_back(p0); a.v1.push_back(p1); a.v2.push_back(p2); a.n.push_back(n); a.color.push_back(c);
+    w.push_back(length(n) * luminance(c));
+  }
+  if (a.v0.empty()) return;
+  a.cdf.resize(a.v0.size() + 1);
+  dist1dInit(a.cdf.data(), w.data(), int(w.size()));
+  float sumL = 0.f;
+  for (size_t i = 0; i < a.v0.size(); ++i) sumL += length(a.n[i]) * luminance(a.color[i]);
+  a.power = kPi * sumL;
 }
 
 // geometry/triangle_mesh.cpp / render/scene.cpp: bounds over all vertices referenced
@@ -478,6 +506,36 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       }
       return;
     }
+    if (isEmissive(s.mats[matId]))
+    {
+      // area-light hit (:419-582), at any depth: emission with MIS, then the path ends
+      float lightPdf;
+      V3 Le = evalArea(s.ls, dir, r.tfar, r.prim, lightPdf);
+      float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
+      L += T * w * Le;
+      V3 c = clampL(L, P.maxRadiance);
+      add3(A.color, A.weight, npix, pix, c);
+      V3 Nv = normalize(-dir);
+      if (!aov1Done)
+      {
+        add3(A.albedo1, A.albedo1W, npix, pix, V3(1.f));
+        add3(A.normal1, A.normal1W, npix, pix, Nv);
+        V3 ph = org + dir * r.tfar;
+        if (A.depth) { A.depth[pix] += dot(ph - V3(cam.pos[0], cam.pos[1], cam.pos[2]), -camN); A.depthW[pix] += 1.f; }
+      }
+      if (!aov2Done)
+      {
+        add3(A.albedo, A.albedoW, npix, pix, V3(1.f));
+        add3(A.normal, A.normalW, npix, pix, Nv);
+      }
+      if (recordSample)
+      {
+        if (A.sampleL) { A.sampleL[pix] = c.x; A.sampleL[npix + pix] = c.y; A.sampleL[2 * npix + pix] = c.z; }
+        if (A.pathHash) A.pathHash[pix] = hash;
+        if (A.firstPrim) A.firstPrim[pix] = firstPrim;
+      }
+      return;
+    }
 
     bool ext = false;
     bool dirOk = false;
@@ -503,10 +561,10 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       {
         float a0, a1, a2;
         smp.get3D(sampleDim + 3, a0, a1, a2);
-        V3 wiL;
+        V3 wiL(0.f);
         float lightPdf, dist;
         int picked;
-        V3 lw = sampleLight(s.ls, wiL, lightPdf, dist, a0, a1, a2, &picked);
+        V3 lw = sampleLight(s.ls, ctx.p, wiL, lightPdf, dist, a0, a1, a2, &picked);
         hash = hashStep(hash, 0x200u | uint32_t(picked));  // which light NEE sampled
         hash = hashStep(hash, dirCell(wiL));
         if (reduceMax(lw) > 0.f && lightPdf > 0.f)
@@ -640,8 +698,17 @@ float orc_scene_radius(void* p) { return ((Scene*)p)->sceneRadius; }
 int orc_scene_set_lights(void* p, int nlights, const void* lights, int envW, int envH, const float* envRgb, int lutMax)
 {
   Scene& s = *(Scene*)p;
-  if (nlights < 1 || nlights > 8) return -1;
+  if (nlights < 0 || nlights > 8) return -1;
+  buildAreaLight(s);
   s.lights.assign((const Light*)lights, (const Light*)lights + nlights);
+  if (nlights == 0 && s.area.v0.empty())
+  {
+    // render/scene.cpp:432-438: no lights at all -> a default ambient light (Ke = 1)
+    Light amb{};
+    amb.kind = kAmbient; amb.r = amb.g = amb.b = 1.f;
+    s.lights.push_back(amb);
+    nlights = 1;
+  }
   for (Light& l : s.lights)
     if (l.kind == kDistant)  // light/distant_light.h:34 normalizes the direction
     {
@@ -671,18 +738,27 @@ int orc_scene_set_lights(void* p, int nlights, const void* lights, int envW, int
   }
   s.ls.count = nlights;
   s.ls.lights = s.lights.data();
+  s.ls.area = s.area.v0.empty() ? nullptr : &s.area;
+  s.ls.offset = s.ls.area ? 1 : 0;
   s.ls.env = s.env.rgb ? &s.env : nullptr;
   s.ls.sceneRadius = s.sceneRadius;
-  float pw[8];
-  for (int i = 0; i < nlights; ++i) pw[i] = lightPower(s.lights[i], s.ls.env, s.sceneRadius);
-  dist1dInit(s.ls.cdf, pw, nlights);
+  float pw[9];
+  if (s.ls.area) pw[0] = s.area.power;
+  for (int i = 0; i < nlights; ++i) pw[s.ls.offset + i] = lightPower(s.lights[i], s.ls.env, s.sceneRadius);
+  dist1dInit(s.ls.cdf, pw, s.ls.offset + nlights);
   return 0;
+}
+
+int orc_light_count(void* p)
+{
+  Scene& s = *(Scene*)p;
+  return s.ls.offset + s.ls.count;
 }
 
 void orc_get_light_cdf(void* p, float* out)
 {
   Scene& s = *(Scene*)p;
-  for (int i = 0; i <= s.ls.count; ++i) out[i] = s.ls.cdf[i];
+  for (int i = 0; i <= s.ls.offset + s.ls.count; ++i) out[i] = s.ls.cdf[i];
 }
 
 void orc_get_luts(void* p, float* out)  // ggxDir, ggxAvg, dielEtaDir, dielEtaAvg, dielRcpEtaDir, dielRcpEtaAvg
@@ -969,14 +1045,15 @@ void orc_env_eval(void* p, int n, const float* dirs, float* out /* n x 4: value3
   }
 }
 
-void orc_light_sample(void* p, int n, const float* s3, float* out /* n x 8: wi3 weight3 pdf dist */)
+// samples the light list from the point org (the area light's samples depend on it)
+void orc_light_sample(void* p, int n, const float* s3, const float* org, float* out /* n x 8: wi3 weight3 pdf dist */)
 {
   Scene& s = *(Scene*)p;
   for (int i = 0; i < n; ++i)
   {
-    V3 wi;
+    V3 wi(0.f);
     float pdf, dist;
-    V3 w = sampleLight(s.ls, wi, pdf, dist, s3[3 * i], s3[3 * i + 1], s3[3 * i + 2]);
+    V3 w = sampleLight(s.ls, V3(org[0], org[1], org[2]), wi, pdf, dist, s3[3 * i], s3[3 * i + 1], s3[3 * i + 2]);
     float* o = out + 8 * i;
     o[0] = wi.x; o[1] = wi.y; o[2] = wi.z; o[3] = w.x; o[4] = w.y; o[5] = w.z; o[6] = pdf; o[7] = dist;
   }

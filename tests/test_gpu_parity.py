@@ -165,6 +165,13 @@ def test_render_parity_sunsky_coated(dev):
     _parity(dev, scene, 96, 64, 3, 0.97)
 
 
+@pytest.mark.parametrize("ambient", [0.5, None])
+def test_render_parity_area_lights(dev, ambient):
+    # emissive spheres (one TriangleLight over their triangles) with and without an ambient light
+    scene = G.spheres_ambient(n_spheres=24, subdiv=2, emissive=6, ambient=ambient)
+    _parity(dev, scene, 64, 64, 4, 0.98)
+
+
 def test_render_stats_and_error(dev):
     scene = G.spheres_ambient(n_spheres=16, subdiv=1)
     P.load_scene(dev, scene)

@@ -13,6 +13,10 @@ headers cannot be compiled here: they need TBB headers).
   diffuse base; a white base under a clear coat conserves energy (Kelemen-style compensation
   makes the directional albedo Ro + (1 - Ro) = 1 up to LUT interpolation), so the furnace
   returns ~1; a tinted coat absorbs more along longer paths, so thicker coats are darker.
+* Area light (light/triangle_light.h): a camera ray that hits an emitter returns its emission
+  exactly (no MIS at depth 0); a diffuse floor under an emitting ceiling that fills its
+  hemisphere reflects rho * Le (light sampling and BSDF sampling combined by MIS stay unbiased
+  only if both pdfs are right); the light pick is proportional to the lights' powers.
 """
 import numpy as np
 import pytest
@@ -26,7 +30,7 @@ def plane_scene(kind, rgb=(0.6, 0.6, 0.6), edge=(1, 1, 1), roughness=0.5, ior=1.
     v = np.array([[-1000, 0, -1000], [1000, 0, -1000], [1000, 0, 1000], [-1000, 0, 1000]], np.float32)
     idx = np.array([[0, 2, 1], [0, 3, 2]], np.int32)
     mats = np.zeros(2, O.MATERIAL_DTYPE)
-    mats[1] = (kind, *rgb, *edge, roughness, ior, thickness, weight)
+    mats[1] = (kind, *rgb, *edge, roughness, ior, thickness, weight, (0, 0, 0))
     s = O.Scene(v, idx, np.ones(2, np.int32), mats)
     lights = np.zeros(1, O.LIGHT_DTYPE)
     lights[0] = (0, 1.0, 1.0, 1.0, 0, 0, 0, 0)
@@ -183,3 +187,68 @@ def test_tinted_coat_absorbs_with_thickness():
     assert r[0] > r[1] > r[2]
     # the green/blue channels (coat color 0.9) lose less than red (0.5)
     assert means[2][1] / means[0][1] > means[2][0] / means[0][0]
+
+
+def _slab(rho, Le, ceiling_y=1.0, half=1000.0):
+    """A diffuse floor (y = 0, facing up) under an emitting ceiling (y = ceiling_y, emitting down)."""
+    h = half
+    v = np.array([[-h, 0, -h], [h, 0, -h], [h, 0, h], [-h, 0, h],
+                  [-h, ceiling_y, -h], [h, ceiling_y, -h], [h, ceiling_y, h], [-h, ceiling_y, h]], np.float32)
+    # floor normal +y: cross(v2 - v0, v1 - v0) ordering as in plane_scene; ceiling normal -y
+    idx = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7]], np.int32)
+    mats = np.zeros(3, O.MATERIAL_DTYPE)
+    mats[1]["kind"] = 0
+    mats[1]["r"] = mats[1]["g"] = mats[1]["b"] = rho
+    mats[2]["kind"] = 0
+    mats[2]["emission"] = Le
+    s = O.Scene(v, idx, np.array([1, 1, 2, 2], np.int32), mats)
+    s.set_lights(np.zeros(0, O.LIGHT_DTYPE))
+    return s
+
+
+def test_area_light_geometry_orientation():
+    v = np.array([[-1, 1, -1], [1, 1, -1], [1, 1, 1]], np.float32)
+    n = np.cross(v[1] - v[0], v[2] - v[0])
+    assert n[1] < 0  # the ceiling winding emits downward
+
+
+def test_direct_view_of_an_emitter_is_exact():
+    Le = (2.0, 3.0, 0.5)
+    s = _slab(0.5, Le, ceiling_y=1.0)
+    cam = np.zeros((), O.CAMERA_DTYPE)
+    cam["pos"] = (0, 0.5, 0)
+    cam["U"], cam["V"], cam["N"] = (1, 0, 0), (0, 0, 1), (0, -1, 0)  # looking up (+y)
+    cam["fov"] = np.radians(30)
+    cam["nearClip"] = 0.01
+    acc, _ = s.render(cam, 32, 32, 0, 1, max_depth=3, record=True)
+    assert s.light_count() == 1
+    np.testing.assert_array_equal(acc["_sampleL"], np.array(Le, np.float32)[:, None].repeat(32 * 32, 1))
+
+
+@pytest.mark.parametrize("rho", [0.3, 0.8])
+def test_floor_under_emitting_ceiling_reflects_rho_le(rho):
+    Le = 2.0
+    s = _slab(rho, (Le, Le, Le), ceiling_y=1.0)
+    cam = np.zeros((), O.CAMERA_DTYPE)
+    cam["pos"] = (0, 0.5, 0)
+    cam["U"], cam["V"], cam["N"] = (1, 0, 0), (0, 0, 1), (0, 1, 0)  # looking down
+    cam["fov"] = np.radians(20)
+    cam["nearClip"] = 0.01
+    acc, _ = s.render(cam, 32, 32, 0, 128, max_depth=1)
+    img = acc["color"] / acc["weight"]
+    # the 2 km ceiling at height 1 covers the floor's hemisphere up to ~1e-6 of its cosine mass
+    assert abs(img.mean() / (rho * Le) - 1.0) < 0.01
+
+
+def test_light_pick_proportional_to_power():
+    Le = 2.0
+    s = _slab(0.5, (Le, Le, Le), half=10.0)
+    amb = np.zeros(1, O.LIGHT_DTYPE)
+    amb[0] = (0, 0.7, 0.7, 0.7, 0, 0, 0, 0)
+    s.set_lights(amb)
+    cdf = s.light_cdf()
+    assert s.light_count() == 2 and cdf.shape == (3,)
+    # TriangleLight::getPower = pi * sum |cross(e1, e2)| * luminance = pi * 2 * area * Le
+    area_power = np.pi * 2 * (20.0 * 20.0) * Le
+    amb_power = 4 * np.pi ** 2 * s.radius ** 2 * 0.7
+    np.testing.assert_allclose(cdf[1], area_power / (area_power + amb_power), rtol=1e-5)
