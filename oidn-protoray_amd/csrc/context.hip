@@ -327,7 +327,7 @@ int prt_context_create(int device_ordinal, prt_context** out)
   if (e != hipSuccess) { delete c; return fail(PRT_ERR_HIP, hipGetErrorString(e)); }
   hipGetDeviceProperties(&c->props, device_ordinal);
   c->cus = c->props.multiProcessorCount;
-  c->launch.gridTrace = c->cus * 4;
+  c->launch.gridTrace = c->cus * PRT_TRACE_BLOCKS_PER_CU;
   c->launch.gridShade = c->cus * 8;
   c->spillStride = c->launch.gridTrace * kTraceBlock;
   for (auto& ev : c->ev) hipEventCreate(&ev);
@@ -335,7 +335,7 @@ int prt_context_create(int device_ordinal, prt_context** out)
   computeLuts(c->luts);
   if (c->lut.alloc(PRT_LUT_FLOATS) != hipSuccess ||
       hipMemcpy(c->lut.p, c->luts.data(), sizeof(float) * PRT_LUT_FLOATS, hipMemcpyHostToDevice) != hipSuccess ||
-      c->spill.alloc(size_t(PRT_MAX_STACK - PRT_LDS_STACK) * c->spillStride) != hipSuccess ||
+      c->spill.alloc(2 * size_t(PRT_MAX_STACK - PRT_LDS_STACK) * c->spillStride) != hipSuccess ||
       c->stats.alloc(8) != hipSuccess || c->argsDev.alloc(1) != hipSuccess ||
       hipHostMalloc((void**)&c->argsHost, sizeof(RenderArgs), hipHostMallocDefault) != hipSuccess)
   {

@@ -72,7 +72,7 @@ struct RenderArgs
   int* queue[2];
   int* shadowQ;
   int* counters;  // [2*d] paths queued for depth d (d >= 1), [2*d+1] shadow rays of depth d
-  int* spill;     // traversal stack overflow, (PRT_MAX_STACK - PRT_LDS_STACK) x spillStride
+  int* spill;     // traversal stack overflow, 2 x (PRT_MAX_STACK - PRT_LDS_STACK) x spillStride (nodes, distances)
   int spillStride;
   unsigned long long* stats;  // closest hit: [0] rays, [1] inner-node visits, [2] triangle tests; shadow: [3..5]
 };

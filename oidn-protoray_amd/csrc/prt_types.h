@@ -91,10 +91,17 @@ struct prt_bvh4_node
 };
 
 #define PRT_LEAF_MAX 4
+// Traversal stack entries per lane kept in LDS (8 B each: node + entry distance) and resident
+// blocks of 256 lanes per CU for the persistent traversal kernels.  10 x 8 B x 256 = 20 KB per
+// block, 7 blocks per CU (140 KB of the 160 KB LDS, 7 waves per SIMD): measured best on the
+// 1080p workload against (16, 4), (12, 5), (12, 6), (9, 8) and (8, 8).
 #ifndef PRT_LDS_STACK
-#define PRT_LDS_STACK 32
+#define PRT_LDS_STACK 10
 #endif
 #define PRT_MAX_STACK 128
+#ifndef PRT_TRACE_BLOCKS_PER_CU
+#define PRT_TRACE_BLOCKS_PER_CU 7
+#endif
 
 // Path flags (per pixel, one int).
 #define PRT_F_LASTTYPE_MASK 0xff

@@ -244,13 +244,9 @@ __global__ void __launch_bounds__(256) kGenerate(const RenderArgs* __restrict__ 
 __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __restrict__ ap, int depth)
 {
   const RenderArgs& a = *ap;
-  __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
-  const int wave = threadIdx.x >> 6;
-  TraceStack st;
-  st.lds = lds + wave * (PRT_LDS_STACK * 64) + laneId();
+  __shared__ TraceLds<kTraceBlock> lds;
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  st.spill = a.spill + gtid;
-  st.stride = a.spillStride;
+  TraceStack st = makeStack(lds, a.spill, a.spillStride, gtid);
   const int count = depth == 0 ? a.npix : a.counters[2 * depth];
   const int* q = a.queue[depth & 1];
   uint32_t nodes = 0, tris = 0, rays = 0;
@@ -477,13 +473,9 @@ __global__ void __launch_bounds__(kShadeBlock) kShade(const RenderArgs* __restri
 __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth)
 {
   const RenderArgs& a = *ap;
-  __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
-  const int wave = threadIdx.x >> 6;
-  TraceStack st;
-  st.lds = lds + wave * (PRT_LDS_STACK * 64) + laneId();
+  __shared__ TraceLds<kTraceBlock> lds;
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  st.spill = a.spill + gtid;
-  st.stride = a.spillStride;
+  TraceStack st = makeStack(lds, a.spill, a.spillStride, gtid);
   const int count = a.counters[2 * depth + 1];
   uint32_t nodes = 0, tris = 0, rays = 0;
   for (int i = gtid; i < count; i += gridDim.x * blockDim.x)
@@ -522,13 +514,9 @@ __global__ void __launch_bounds__(kTraceBlock) kQuery(BvhView bvh, int n, const 
                                                       float* __restrict__ tOut, float* __restrict__ uOut, float* __restrict__ vOut,
                                                       int anyHit, int* spill, int spillStride)
 {
-  __shared__ int lds[PRT_LDS_STACK * kTraceBlock];
-  const int wave = threadIdx.x >> 6;
-  TraceStack st;
-  st.lds = lds + wave * (PRT_LDS_STACK * 64) + laneId();
+  __shared__ TraceLds<kTraceBlock> lds;
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  st.spill = spill + gtid;
-  st.stride = spillStride;
+  TraceStack st = makeStack(lds, spill, spillStride, gtid);
   uint32_t nodes = 0, tris = 0;
   for (int i = gtid; i < n; i += gridDim.x * blockDim.x)
   {

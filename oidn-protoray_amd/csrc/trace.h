@@ -1,7 +1,8 @@
-// BVH traversal on gfx950: closest hit and any hit, one ray per lane, with the traversal
-// stack in LDS (PRT_LDS_STACK entries per lane, lane-interleaved so a wave's pushes hit 64
-// distinct banks) and a per-lane overflow area in HBM for deeper trees.  Replaces Embree's
-// rtcIntersect8/16 and rtcOccluded8/16 (accel/embree_intersector_stream.h:66-131).
+// BVH traversal on gfx950: closest hit and any hit, one ray per lane, with a traversal stack
+// of (node, entry distance) in LDS (PRT_LDS_STACK entries per lane, lane-interleaved so a
+// wave's pushes hit 64 distinct banks) and a per-lane overflow area in HBM for deeper trees;
+// closest-hit traversal drops popped entries that start beyond the current hit.  Replaces
+// Embree's rtcIntersect8/16 and rtcOccluded8/16 (accel/embree_intersector_stream.h:66-131).
 #pragma once
 
 #include "dev_math.h"
@@ -65,27 +66,59 @@ PRT_DEV void cswap(float& ta, int& ca, float& tb, int& cb)
   if (tb < ta) { float t = ta; ta = tb; tb = t; int c = ca; ca = cb; cb = c; }
 }
 
+// Per-lane traversal stack of (node, entry distance): the first PRT_LDS_STACK entries in LDS
+// (two lane-interleaved columns, so a wave's accesses hit 64 distinct banks), deeper entries in
+// the lane's overflow column in HBM (nodes in rows [0, PRT_MAX_STACK - PRT_LDS_STACK), the
+// distances in the rows after them).
 struct TraceStack
 {
-  int* lds;       // this lane's column: lds[k * 64]
+  int* lds;       // this lane's node column: lds[k * 64]
+  float* ldsT;    // this lane's distance column: ldsT[k * 64]
   int* spill;     // this lane's overflow column: spill[k * stride]
   int stride;
   int sp;
   // The build rejects trees deeper than PRT_MAX_STACK, so the last check never drops an entry;
   // it only keeps every store inside the allocated overflow area.
-  PRT_DEV void push(int v)
+  PRT_DEV void push(int v, float t)
   {
-    if (sp < PRT_LDS_STACK) lds[sp * 64] = v;
-    else if (sp < PRT_MAX_STACK) spill[(sp - PRT_LDS_STACK) * stride] = v;
+    if (sp < PRT_LDS_STACK) { lds[sp * 64] = v; ldsT[sp * 64] = t; }
+    else if (sp < PRT_MAX_STACK)
+    {
+      spill[(sp - PRT_LDS_STACK) * stride] = v;
+      spill[(sp - PRT_LDS_STACK + (PRT_MAX_STACK - PRT_LDS_STACK)) * stride] = __float_as_int(t);
+    }
     else return;
     ++sp;
   }
-  PRT_DEV int pop()
+  PRT_DEV int pop(float& t)
   {
     --sp;
-    return sp < PRT_LDS_STACK ? lds[sp * 64] : spill[(sp - PRT_LDS_STACK) * stride];
+    if (sp < PRT_LDS_STACK) { t = ldsT[sp * 64]; return lds[sp * 64]; }
+    t = __int_as_float(spill[(sp - PRT_LDS_STACK + (PRT_MAX_STACK - PRT_LDS_STACK)) * stride]);
+    return spill[(sp - PRT_LDS_STACK) * stride];
   }
 };
+
+// LDS a block of kThreads lanes needs for its stacks (nodes then distances).
+template <int kThreads>
+struct TraceLds
+{
+  int node[PRT_LDS_STACK * kThreads];
+  float dist[PRT_LDS_STACK * kThreads];
+};
+// This lane's stack over the block's LDS and the overflow area (gtid-th column of spill).
+template <int kThreads>
+PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int gtid)
+{
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  TraceStack st;
+  st.lds = l.node + wave * (PRT_LDS_STACK * 64) + lane;
+  st.ldsT = l.dist + wave * (PRT_LDS_STACK * 64) + lane;
+  st.spill = spill + gtid;
+  st.stride = stride;
+  st.sp = 0;
+  return st;
+}
 
 // Closest hit (kAnyHit = false) or any hit.  Ties in t go to the smaller primitive ID, so the
 // result does not depend on the traversal order.  Returns the primitive ID or -1; nodeVisits
@@ -115,9 +148,9 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
       cswap(t[1], c[1], t[2], c[2]);
       if (t[0] != inf)
       {
-        if (t[3] != inf) st.push(c[3]);
-        if (t[2] != inf) st.push(c[2]);
-        if (t[1] != inf) st.push(c[1]);
+        if (t[3] != inf) st.push(c[3], t[3]);
+        if (t[2] != inf) st.push(c[2], t[2]);
+        if (t[1] != inf) st.push(c[1], t[1]);
         node = c[0];
         continue;
       }
@@ -143,11 +176,15 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
         }
       }
     }
-    if (st.sp == 0) break;
-    node = st.pop();
+    // next node: skip entries that start beyond the closest hit found so far (an entry at
+    // exactly tfar may still hold a tie with a smaller primitive ID)
+    float tEntry;
+    do
+    {
+      if (st.sp == 0) { tHit = tfar; return best; }
+      node = st.pop(tEntry);
+    } while (!kAnyHit && tEntry > tfar);
   }
-  tHit = tfar;
-  return best;
 }
 
 }  // namespace prt
