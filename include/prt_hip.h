@@ -177,6 +177,22 @@ int prt_query_occlusion(prt_context* ctx, int32_t n, const float* rays, int32_t*
 int prt_query_rays_device(prt_context* ctx, int32_t n, const float* rays_device, int32_t* prim_device,
                           float* t_device, float* u_device, float* v_device);
 
+/* --- differentiable rendering ----------------------------------------------------------
+ * Path-replay adjoint.  No counterpart in the reference (it renders forward only); added for
+ * inverse rendering of material parameters.  Clears the frame and renders spp sample passes
+ * (indices 0 .. spp-1 of the sampler seed); forms the loss mean over pixels and channels of
+ * (I - target)^2 with target an H x W x 3 image in prt_blit layout; returns *loss and
+ * dLoss / d(albedo r, g, b, GGX alpha) per material in grad[4 * material + k] (grad_capacity
+ * floats, >= 4 x the material count).  Differentiated: the diffuse colour of kinds 0 and 3 and
+ * the coat alpha (= roughness^2) of kind 3; other entries are 0.  The gradient is that of the
+ * primal estimator with its samples fixed (detached sampling); paths are replayed from the
+ * seed, not stored.  Requires maxDepth <= 8.  *ms: device time of the replay kernels. */
+int prt_adjoint(prt_context* ctx, const float* target, int32_t spp, double* loss, float* grad, int32_t grad_capacity,
+                double* ms);
+/* Replays sample sample_index of every pixel as the adjoint does and returns its radiance
+ * (planar [3][H*W]); equal to the primal sample's radiance (the replay follows the same path). */
+int prt_debug_adjoint_replay(prt_context* ctx, uint32_t sample_index, float* radiance);
+
 /* --- per-sample debug outputs of the last pass (parity checks) ------------------------- */
 /* When enabled, each pass also stores per pixel: the clamped sample radiance (planar
  * 3 x W*H), a hash of the path's hit / lobe sequence and the first-hit primitive ID. */

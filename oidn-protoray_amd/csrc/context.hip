@@ -1020,6 +1020,84 @@ int prt_blit(prt_context* c, int32_t buffer, float* dst, float* var)
   return PRT_OK;
 }
 
+// Path-replay adjoint: see include/prt_hip.h.  The primal image is the frame after spp sample
+// passes (indices 0 .. spp-1 of the current seed); the loss and its per-pixel adjoint are formed
+// on the host in double precision; the replay kernels run on the context stream.
+int prt_adjoint(prt_context* c, const float* target, int32_t spp, double* loss, float* grad, int32_t grad_capacity, double* ms)
+{
+  CTX_CHECK(c);
+  if (!target || !grad) return fail(PRT_ERR_INVALID, "target and grad are required");
+  if (spp < 1) return fail(PRT_ERR_INVALID, "spp must be >= 1");
+  if (c->params.maxDepth > PRT_ADJ_MAX_VERTICES)
+    return fail(PRT_ERR_INVALID, "the adjoint supports maxDepth <= " + std::to_string(PRT_ADJ_MAX_VERTICES));
+  const int nMat = int(c->materials.size());
+  const int nParams = 4 * nMat;
+  if (grad_capacity < nParams) return fail(PRT_ERR_INVALID, "grad needs 4 floats per material");
+  int rc = prt_frame_clear(c);
+  if (rc != PRT_OK) return rc;
+  c->sampleIndex = 0;
+  rc = prt_render(c, spp, nullptr);
+  if (rc != PRT_OK) return rc;
+  const size_t n = size_t(c->npix);
+  std::vector<float> sum(3 * n), w(n), delta(3 * n);
+  HIP_TRY(hipMemcpy(sum.data(), c->color.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(w.data(), c->weight.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  double L = 0.0;
+  const double norm = 1.0 / (3.0 * double(n));
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k)
+    {
+      double I = double(sum[k * n + i]) / double(w[i]);
+      double r = I - double(target[3 * i + k]);
+      L += r * r;
+      delta[k * n + i] = float(2.0 * r * norm);
+    }
+  if (loss) *loss = L * norm;
+  const int blocks = adjointBlocks(c->npix, c->spillStride);
+  DevBuf<float> dDelta, partial, dGrad;
+  HIP_TRY(dDelta.alloc(3 * n));
+  HIP_TRY(partial.alloc(size_t(blocks) * nParams));
+  HIP_TRY(dGrad.alloc(nParams));
+  HIP_TRY(hipMemcpy(dDelta.p, delta.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemsetAsync(partial.p, 0, sizeof(float) * partial.n, c->stream));
+  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  for (int s = 0; s < spp; ++s)
+    launchAdjoint(c->argsDev.p, blocks, dDelta.p, 1.0f / float(spp), nParams, partial.p, uint32_t(s), nullptr, c->stream);
+  launchAdjointReduce(partial.p, blocks, nParams, dGrad.p, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  HIP_TRY(hipMemcpyAsync(grad, dGrad.p, sizeof(float) * nParams, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+  if (ms) *ms = t;
+  return PRT_OK;
+}
+
+int prt_debug_adjoint_replay(prt_context* c, uint32_t sample_index, float* radiance)
+{
+  CTX_CHECK(c);
+  if (!radiance) return fail(PRT_ERR_INVALID, "radiance is null");
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  if (c->params.maxDepth > PRT_ADJ_MAX_VERTICES)
+    return fail(PRT_ERR_INVALID, "the adjoint supports maxDepth <= " + std::to_string(PRT_ADJ_MAX_VERTICES));
+  // the kernel arguments of the last prt_render call are on the device
+  if (c->count == 0) return fail(PRT_ERR_STATE, "render at least one pass first");
+  const size_t n = size_t(c->npix);
+  const int blocks = adjointBlocks(c->npix, c->spillStride);
+  DevBuf<float> dDelta, partial, dL;
+  HIP_TRY(dDelta.alloc(3 * n));
+  HIP_TRY(partial.alloc(size_t(blocks) * 4));
+  HIP_TRY(dL.alloc(3 * n));
+  HIP_TRY(hipMemsetAsync(dDelta.p, 0, sizeof(float) * 3 * n, c->stream));
+  HIP_TRY(hipMemsetAsync(partial.p, 0, sizeof(float) * partial.n, c->stream));
+  launchAdjoint(c->argsDev.p, blocks, dDelta.p, 1.f, 0, partial.p, sample_index, dL.p, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(radiance, dL.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return PRT_OK;
+}
+
 int prt_get_accum(prt_context* c, float* sum, float* weight)
 {
   CTX_CHECK(c);

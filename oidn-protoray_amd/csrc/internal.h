@@ -91,6 +91,12 @@ void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const Lau
 void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
                  int* spill, int spillStride, int grid, hipStream_t s);
+// Path-replay adjoint (adjoint.hip): partial is [blocks][nParams] (zeroed by the caller before the
+// first sample pass), grad [nParams]; delta is the per-pixel dLoss/dI (planar rgb).
+int adjointBlocks(int npix, int spillStride);
+void launchAdjoint(const RenderArgs* a, int blocks, const float* delta, float invSpp, int nParams, float* partial,
+                   uint32_t sampleIndex, float* replayL, hipStream_t s);
+void launchAdjointReduce(const float* partial, int nBlocks, int nParams, float* grad, hipStream_t s);
 void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s);
 
 constexpr int kTraceBlock = 256;

@@ -1,0 +1,158 @@
+// Per-path building blocks shared by the wavefront kernels (render.hip) and the path-replay
+// adjoint (adjoint.hip): camera rays, post-intersection, light evaluation and sampling, the
+// radiance clamp.  Both replay identical paths because they call the same code.
+#pragma once
+
+#include "internal.h"
+
+namespace prt {
+
+PRT_DEV uint32_t hashMix(uint32_t h, uint32_t k)  // math/hash.h:12-30 (murmurHash3Mix)
+{
+  k *= 0xcc9e2d51u; k = (k << 15) | (k >> 17); k *= 0x1b873593u;
+  h ^= k; h = ((h << 13) | (h >> 19)) * 5u + 0xe6546b64u;
+  return h;
+}
+
+PRT_DEV f3 ld3(const float* p) { return f3(p[0], p[1], p[2]); }
+
+// integrator.h:27-46
+PRT_DEV f3 clampL(f3 L, float maxRadiance)
+{
+  f3 c(0.f < L.x ? L.x : 0.f, 0.f < L.y ? L.y : 0.f, 0.f < L.z ? L.z : 0.f);
+  float y = luminance(c);
+  if (y > maxRadiance) c = c * (maxRadiance * (1.0f / y));
+  return c;
+}
+
+// render/scene.h:137-157
+PRT_DEV f3 evalEnv(const RenderArgs& a, f3 wi, float& pdf, uint32_t& hitMask)
+{
+  f3 v(0.f);
+  pdf = 0.f;
+  hitMask = 0;
+  for (int i = 0; i < a.nLights; ++i)
+  {
+    float cp;
+    v = v + envEvalOne(a.lights[i], a.env, wi, cp);
+    if (cp > 0.f) hitMask |= 1u << i;
+    const int k = a.lightOffset + i;
+    pdf += cp * (a.lightCdf[k + 1] - a.lightCdf[k]);
+  }
+  return v;
+}
+
+// light/triangle_light.h:84-100 + render/scene.h:159-195: the area light seen along wi at
+// distance dist on primitive prim
+PRT_DEV f3 evalArea(const RenderArgs& a, f3 wi, float dist, int prim, float& pdf)
+{
+  pdf = 0.f;
+  int tri = a.areaMap[prim];
+  float pickProb = dist1dPdf(a.areaCdf, a.areaCount, tri);
+  float4 nn = a.areaTris[4 * tri + 3];
+  float dirDotN = dot(wi, f3(nn.x, nn.y, nn.z));
+  if (dirDotN >= 0.f) return f3(0.f);
+  pdf = 2.0f * dist * dist * rcpSafe(fabsf(dirDotN)) * pickProb;
+  pdf *= a.lightCdf[1] - a.lightCdf[0];
+  return f3(a.areaTris[4 * tri].w, a.areaTris[4 * tri + 1].w, a.areaTris[4 * tri + 2].w);
+}
+
+// light/triangle_light.h:123-150: triangle by power, point uniform on it (shape_sampler.h:
+// 135-152, Heitz 2019 map); wi is left unchanged when the sample faces away
+PRT_DEV f3 sampleArea(const RenderArgs& a, f3 p, f3& wi, float& pdf, float& dist, float sx, float sy, float sc)
+{
+  pdf = 0.f;
+  int tri = dist1dLookup(a.areaCdf, a.areaCount, sc);
+  float pickProb = a.areaCdf[tri + 1] - a.areaCdf[tri];
+  float4 A = a.areaTris[4 * tri], B = a.areaTris[4 * tri + 1], C = a.areaTris[4 * tri + 2], Nn = a.areaTris[4 * tri + 3];
+  float u = sx, v = sy;
+  if (v > u) { u *= 0.5f; v -= u; }
+  else { v *= 0.5f; u -= v; }
+  f3 v0(A.x, A.y, A.z), v1(B.x, B.y, B.z), v2(C.x, C.y, C.z);
+  f3 dir = (v2 + u * (v0 - v2) + v * (v1 - v2)) - p;
+  dist = length(dir);
+  float dirDotN = dot(dir, f3(Nn.x, Nn.y, Nn.z));
+  if (!(dirDotN < 0.f)) return f3(0.f);
+  wi = dir * (1.0f / dist);
+  pdf = 2.0f * dist * dist * dist * rcpSafe(fabsf(dirDotN)) * pickProb;
+  return f3(A.w, B.w, C.w) * (1.0f / pdf);
+}
+
+// render/scene.h:215-250 (Distribution1D::sampleReuse for the pick, s.c remapped); the light
+// list is [area light,] environment lights.  dist: distance to the sampled point (area light)
+// or kPosMax.
+PRT_DEV f3 sampleLight(const RenderArgs& a, f3 p, f3& wi, float& pdf, float& dist, float sx, float sy, float sc, int& picked)
+{
+  picked = 0;
+  dist = kPosMax;
+  const int total = a.lightOffset + a.nLights;
+  if (total == 1)
+    return a.lightOffset ? sampleArea(a, p, wi, pdf, dist, sx, sy, sc) : envSampleOne(a.lights[0], a.env, wi, pdf, sx, sy);
+  float pick;
+  int id = dist1dSampleReuse(a.lightCdf, total, pick, sc);
+  picked = id;
+  f3 w = (a.lightOffset && id == 0) ? sampleArea(a, p, wi, pdf, dist, sx, sy, sc)
+                                     : envSampleOne(a.lights[id - a.lightOffset], a.env, wi, pdf, sx, sy);
+  pdf *= pick;
+  return w * (1.0f / pick);
+}
+
+// geometry/triangle_mesh.cpp:152-213 + render/scene.h:61-72
+PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, float u, float v)
+{
+  Ctx c;
+  c.p = o + t * d;
+  c.eps = fmaxf(t, fmaxf(fmaxf(fabsf(c.p.x), fabsf(c.p.y)), fabsf(c.p.z))) * 0x1.fp-18f;
+  int i0 = a.idx[3 * prim], i1 = a.idx[3 * prim + 1], i2 = a.idx[3 * prim + 2];
+  f3 p0 = ld3(a.pos + 3 * i0), p1 = ld3(a.pos + 3 * i1), p2 = ld3(a.pos + 3 * i2);
+  f3 Ng = normalize(cross(p1 - p0, p2 - p0));
+  f3 N = Ng;
+  if (a.nrm)
+  {
+    float b0 = 1.f - u - v;
+    N = normalize(b0 * ld3(a.nrm + 3 * i0) + u * ld3(a.nrm + 3 * i1) + v * ld3(a.nrm + 3 * i2));
+    if (dot(N, Ng) < 0.f) N = -N;
+  }
+  makeFrame(c.f.U, c.f.V, N);
+  if (dot(Ng, d) > 0.f) { Ng = -Ng; N = -N; }
+  c.Ng = Ng;
+  c.f.N = N;
+  return c;
+}
+
+
+// Camera ray of pixel (x, y) for sample sampleIndex (path_renderer_stream.h:209-268): pixel
+// jitter from dimension set 0, lens sample from set 2.
+PRT_DEV void cameraRay(const RenderArgs& a, int x, int y, uint32_t sampleIndex, f3& org, f3& dir, float& tnear)
+{
+  uint32_t pixelId = uint32_t(x) | (uint32_t(y) << 16);  // path_renderer_stream.h:219-220
+  Sampler smp;
+  smp.set(a.seed, sampleIndex, pixelId);
+  float psx, psy, lsx, lsy;
+  smp.get2D(0, psx, psy);  // sampleDimPixel (integrator.h:52)
+  smp.get2D(2, lsx, lsy);  // sampleDimLens (integrator.h:53)
+  float imx = (float(x) + psx) * a.invW, imy = (float(y) + psy) * a.invH;
+  f3 imageO = ld3(a.imageO), imageDx = ld3(a.imageDx), imageDy = ld3(a.imageDy);
+  org = ld3(a.camPos);
+  if (a.pinhole)  // camera/pinhole_camera.h:128-133
+  {
+    dir = imageO + imx * imageDx + imy * imageDy;
+    float len = length(dir);
+    dir = dir * (1.0f / len);
+    tnear = a.nearClip * len;
+  }
+  else  // camera/thin_lens_camera.h:175-182, sampling/shape_sampler.h:169-179
+  {
+    float r = sqrtf(lsx), sp, cp;
+    sincosf(2.f * kPi * lsy, &sp, &cp);
+    float lx = r * cp * a.lensRadius, ly = r * sp * a.lensRadius;
+    f3 off = lx * ld3(a.camU) + ly * ld3(a.camV) + 0.f * ld3(a.camN);
+    dir = a.focal * (imageO + imx * imageDx + imy * imageDy) - off;
+    float len = length(dir);
+    org = org + off;
+    dir = dir * (1.0f / len);
+    tnear = a.nearClipScale * len;
+  }
+}
+
+}  // namespace prt

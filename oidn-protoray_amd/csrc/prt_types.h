@@ -102,6 +102,8 @@ struct prt_bvh4_node
 #define PRT_LDS_STACK 10
 #endif
 #define PRT_MAX_STACK 128
+// Path vertices the adjoint records per lane (the adjoint requires maxDepth <= this).
+#define PRT_ADJ_MAX_VERTICES 8
 #ifndef PRT_TRACE_BLOCKS_PER_CU
 #define PRT_TRACE_BLOCKS_PER_CU 7
 #endif

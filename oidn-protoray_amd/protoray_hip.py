@@ -72,6 +72,8 @@ SIGNATURES = {
     "prt_debug_get": [_vp, _vp, _vp, _vp],
     "prt_debug_sampler": [_vp, _i, _vp, _i, _vp, _vp, _vp],
     "prt_debug_luts": [_vp, _vp, ctypes.c_int32, _vp],
+    "prt_adjoint": [_vp, _vp, _i, _vp, _vp, _i, _vp],
+    "prt_debug_adjoint_replay": [_vp, _u, _vp],
     "prt_debug_bvh_stats": [_vp, _vp],
     "prt_debug_kernel_ms": [_vp, _vp, _vp, _vp],
 }
@@ -232,6 +234,21 @@ class DeviceHip:
                       np.zeros((n, ndim, 3), np.float32))
         _check(_lib.prt_debug_sampler(self.h, n, _ptr(q), ndim, _ptr(o1), _ptr(o2), _ptr(o3)))
         return o1, o2, o3
+
+    # --- differentiable rendering ----------------------------------------------------------
+    def adjoint(self, target, spp, n_materials):
+        """Path-replay adjoint: (loss, grad [n_materials x 4] = dLoss/d(r, g, b, alpha), device ms)."""
+        t = np.ascontiguousarray(target, np.float32)
+        grad = np.zeros((n_materials, 4), np.float32)
+        loss, ms = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(_lib.prt_adjoint(self.h, _ptr(t), int(spp), ctypes.byref(loss), _ptr(grad), grad.size, ctypes.byref(ms)))
+        return loss.value, grad, ms.value
+
+    def adjointReplay(self, sample_index):
+        W, H = self.size
+        out = np.zeros((3, W * H), np.float32)
+        _check(_lib.prt_debug_adjoint_replay(self.h, int(sample_index), _ptr(out)))
+        return out
 
     def luts(self):
         n = ctypes.c_int32(0)

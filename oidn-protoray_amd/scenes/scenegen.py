@@ -188,6 +188,43 @@ def spheres_ambient(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=
                 camera=_camera(), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)
 
 
+def spheres_adjoint(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=32, max_depth=4, n_materials=64):
+    """Differentiable-rendering scene (config-0 style): the ground (diffuse 0.5) plus n_spheres
+    icospheres sharing n_materials coated-diffuse materials (material IDs 2 .. n_materials+1;
+    albedo U[0.2,0.8]^3, coat IOR 1.5, coat alpha U[0.05,0.4], i.e. roughness sqrt(alpha)),
+    under the ambient light and a distant light."""
+    rng = Pcg32(seed)
+    mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
+    for _ in range(n_materials):
+        rgb = (rng.f(0.2, 0.8), rng.f(0.2, 0.8), rng.f(0.2, 0.8))
+        alpha = rng.f(0.05, 0.4)
+        mats.append(_material(KIND_COATED, rgb, (1, 1, 1), math.sqrt(alpha), 1.5, thickness=1.0, weight=1.0))
+    pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, lambda i: 2 + i % n_materials)
+    lights = np.zeros(2, LIGHT_DTYPE)
+    d = np.array([-0.5, 0.8, -0.3])
+    d /= np.linalg.norm(d)
+    lights[0] = (LIGHT_AMBIENT, 0.6, 0.6, 0.6, 0, 0, 0, 0)
+    lights[1] = (LIGHT_DISTANT, 40.0, 38.0, 35.0, d[0], d[1], d[2], math.radians(3.0))
+    return dict(name="spheres_adjoint", positions=pos, indices=idx, tri_material=tri_mat,
+                materials=np.array(mats, MATERIAL_DTYPE), lights=lights, env=None,
+                camera=_camera(), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)
+
+
+def perturbed_materials(materials, seed=1, d_albedo=0.1, d_alpha=0.05):
+    """Target parameters of the differentiable-rendering configuration: every coated material's
+    albedo moved by +-d_albedo per channel and its alpha by +-d_alpha (signs from PCG32 seed)."""
+    rng = Pcg32(seed)
+    m = materials.copy()
+    for i in range(m.shape[0]):
+        if m[i]["kind"] != KIND_COATED:
+            continue
+        for ch in ("r", "g", "b"):
+            m[i][ch] = min(max(m[i][ch] + (d_albedo if rng.f() < 0.5 else -d_albedo), 0.02), 0.98)
+        alpha = float(m[i]["roughness"]) ** 2 + (d_alpha if rng.f() < 0.5 else -d_alpha)
+        m[i]["roughness"] = math.sqrt(min(max(alpha, 0.01), 0.9))
+    return m
+
+
 def load_sunsky(path=os.path.join(HERE, "sunsky_e35_t3_az30.npz")):
     z = np.load(path)
     # the bake is stored zenith-first; the image light reads texture rows bottom-up

@@ -80,6 +80,9 @@ def lib():
         L.orc_env_eval.argtypes = [vp, i, vp, vp]
         L.orc_light_sample.argtypes = [vp, i, vp, vp, vp]
         L.orc_light_count.argtypes = [vp]
+        L.orc_adjoint.restype = ctypes.c_double
+        L.orc_adjoint.argtypes = [vp, vp, i, i, i, u, i, i, f, vp, i, vp]
+        L.orc_bsdf_grad.argtypes = [i, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -169,6 +172,17 @@ def bsdf(luts_flat, params, geo, s):
     return so, st, ev
 
 
+def bsdf_grad(luts_flat, params, geo):
+    """FP64 value and parameter derivatives of the BSDF: n x [value rgb, d/d albedo rgb, d/d alpha rgb].
+    params rows as bsdf() (11 columns; 9-column rows are padded)."""
+    params, geo = c32(params), c32(geo)
+    if params.shape[1] == 9:
+        params = np.ascontiguousarray(np.concatenate([params, np.zeros((params.shape[0], 2), np.float32)], 1))
+    out = np.zeros((params.shape[0], 9), np.float64)
+    lib().orc_bsdf_grad(params.shape[0], c32(luts_flat).ctypes.data, params.ctypes.data, geo.ctypes.data, out.ctypes.data)
+    return out
+
+
 def post_intersect(verts, normals, indices, rays, prim):
     verts, rays = c32(verts), c32(rays)
     indices, prim = i32(indices), i32(prim)
@@ -239,6 +253,17 @@ class Scene:
         out = np.zeros((s3.shape[0], 8), np.float32)
         lib().orc_light_sample(self.h, s3.shape[0], s3.ctypes.data, o.ctypes.data, out.ctypes.data)
         return out
+
+    def adjoint(self, camera, W, H, target, spp, seed=0, max_depth=4, rr_depth=8, min_weight=1e-4, threads=None):
+        """FP64 gradient of mean((I - target)^2) w.r.t. (albedo r, g, b, alpha) per material: (loss, grad[nMat, 4])."""
+        cam = np.ascontiguousarray(camera, CAMERA_DTYPE)
+        t = c32(target)
+        nmat = self.materials.shape[0]
+        grad = np.zeros((nmat, 4), np.float64)
+        threads = threads or min(os.cpu_count() or 1, 16)
+        loss = lib().orc_adjoint(self.h, cam.ctypes.data, W, H, spp, seed, max_depth, rr_depth, min_weight, t.ctypes.data,
+                                 threads, grad.ctypes.data)
+        return loss, grad
 
     def render(self, camera, W, H, spp_start=0, spp_count=1, seed=0, max_depth=5, rr_depth=8, min_weight=1e-4,
                max_radiance=3.4028234663852886e38, threads=None, rows=None, aovs=(), accum=None, record=False):

@@ -413,8 +413,120 @@ static inline uint32_t dirCell(V3 d)
 struct Stats { long long rays = 0; long long nodes = 0; };
 
 // One path of one sample (path_renderer_stream.h:209-268 init, :276-1071 bounce loop)
+// ---- FP64 parameter derivatives of the BSDF values (the adjoint's checker) --------------
+// value = f * cos as bsdfEval returns it; dA[c] = d value_c / d albedo_c; dAl = d value / d alpha
+// (alpha = roughness^2).  Diffuse: rho / pi cos.  Coated diffuse (rough_coating.h:297-378):
+// F s D G2 / (4 cos_o) [cos_i > 0] + rho / pi cos_i absorb (1 - Ro)(1 - Ri) / (1 - Ravg), with
+// the reflection-albedo tables interpolated linearly (their slope along the roughness axis is
+// the derivative).  Every operation after the float inputs is in double.
+struct GradD { double v[3] = {0, 0, 0}, dA[3] = {0, 0, 0}, dAl[3] = {0, 0, 0}; };
+
+static double lerpd(double a, double b, double t) { return a + t * (b - a); }
+static double clampd(double x, double lo, double hi) { return std::min(std::max(x, lo), hi); }
+// interp3D (and its slope along z) in double over a float table
+static double interp3Dd(double px, double py, double pz, const float* f, int n, bool slopeZ)
+{
+  if (slopeZ && !(pz > 0.0 && pz < double(n - 1))) return 0.0;
+  double xc = clampd(px, 0, n - 1), yc = clampd(py, 0, n - 1), zc = clampd(pz, 0, n - 1);
+  double sx = xc - std::floor(xc), sy = yc - std::floor(yc), sz = zc - std::floor(zc);
+  int x0 = std::min(int(xc), n - 1), x1 = std::min(x0 + 1, n - 1);
+  int y0 = std::min(int(yc), n - 1), y1 = std::min(y0 + 1, n - 1);
+  int z0 = std::min(int(zc), n - 1), z1 = std::min(z0 + 1, n - 1);
+  auto at = [&](int x, int y, int z) { return double(f[x + y * n + z * n * n]); };
+  double f0 = lerpd(lerpd(at(x0, y0, z0), at(x1, y0, z0), sx), lerpd(at(x0, y1, z0), at(x1, y1, z0), sx), sy);
+  double f1 = lerpd(lerpd(at(x0, y0, z1), at(x1, y0, z1), sx), lerpd(at(x0, y1, z1), at(x1, y1, z1), sx), sy);
+  return slopeZ ? f1 - f0 : lerpd(f0, f1, sz);
+}
+static double interp2Dd(double px, double py, const float* f, int n, bool slopeY)
+{
+  if (slopeY && !(py > 0.0 && py < double(n - 1))) return 0.0;
+  double xc = clampd(px, 0, n - 1), yc = clampd(py, 0, n - 1);
+  double sx = xc - std::floor(xc), sy = yc - std::floor(yc);
+  int x0 = std::min(int(xc), n - 1), x1 = std::min(x0 + 1, n - 1);
+  int y0 = std::min(int(yc), n - 1), y1 = std::min(y0 + 1, n - 1);
+  double f0 = lerpd(f[x0 + y0 * n], f[x1 + y0 * n], sx), f1 = lerpd(f[x0 + y1 * n], f[x1 + y1 * n], sx);
+  return slopeY ? f1 - f0 : lerpd(f0, f1, sy);
+}
+struct D3 { double x, y, z; };
+static D3 d3(V3 a) { return D3{a.x, a.y, a.z}; }
+static double dotd(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static D3 normd(D3 a) { double l = std::sqrt(dotd(a, a)); return D3{a.x / l, a.y / l, a.z / l}; }
+
+static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, const float* coatParams = nullptr)
+{
+  GradD g;
+  if (!(b.importance > 0.f)) return g;
+  const D3 N = d3(b.fr.N), U = d3(b.fr.U), Vv = d3(b.fr.V), o = d3(wo), i = d3(wi);
+  if (b.kind == kDiffuse)
+  {
+    double c = std::max(dotd(i, N), 0.0);
+    double rho[3] = {b.color.x, b.color.y, b.color.z};
+    for (int k = 0; k < 3; ++k) { g.v[k] = rho[k] / M_PI * c; g.dA[k] = c / M_PI; }
+    return g;
+  }
+  if (b.kind != kCoated) return g;
+  const double cosO = dotd(o, N);
+  if (!(cosO > 0.0)) return g;
+  const double cosI = dotd(i, N);
+  if (!(cosI * dotd(i, d3(ctx.Ng)) > 0.0)) return g;
+  const double eta = b.ceta, s = b.scale, r = b.roughness, a = std::max(double(r) * r, 0.001);
+  auto refr = [&](double c) { return std::sqrt(std::max(1.0 - eta * eta * (1.0 - c * c), 0.0)); };
+  double length = 1.0 / refr(cosO) + 1.0 / refr(std::fabs(cosI));
+  if (cosI <= 0.0) length *= 0.5;
+  const double t = double(b.thickness) * length;
+  const double coat[3] = {b.coatColor.x, b.coatColor.y, b.coatColor.z}, rho[3] = {b.color.x, b.color.y, b.color.z};
+  const double e = (eta - 1.0 / 3.0) / (1.0 - 1.0 / 3.0);
+  const double Ro = s * interp3Dd(cosO * 15, e * 15, r * 15, L.dielReflDir, 16, false);
+  const double Ri = s * interp3Dd(std::fabs(cosI) * 15, e * 15, r * 15, L.dielReflDir, 16, false);
+  const double Ravg = s * interp2Dd(e * 15, r * 15, L.dielReflAvg, 16, false);
+  const double dRo = 15 * s * interp3Dd(cosO * 15, e * 15, r * 15, L.dielReflDir, 16, true);
+  const double dRi = 15 * s * interp3Dd(std::fabs(cosI) * 15, e * 15, r * 15, L.dielReflDir, 16, true);
+  const double dRavg = 15 * s * interp2Dd(e * 15, r * 15, L.dielReflAvg, 16, true);
+  const double inv = 1.0 / (1.0 - Ravg);
+  const double T = (1 - Ro) * (1 - Ri) * inv;
+  const double dTdr = (-dRo * (1 - Ri) - (1 - Ro) * dRi) * inv + (1 - Ro) * (1 - Ri) * dRavg * inv * inv;
+  const double drda = (a > 0.001 && r > 0) ? 0.5 / r : 0.0;
+  for (int k = 0; k < 3; ++k)
+  {
+    double unit = (1.0 + s * (std::pow(coat[k], t) - 1.0)) * std::max(cosI, 0.0) / M_PI;
+    g.v[k] = rho[k] * unit * T;
+    g.dA[k] = unit * T;
+    g.dAl[k] = rho[k] * unit * dTdr * drda;
+  }
+  if (cosI > 0.0)
+  {
+    D3 h = normd(D3{o.x + i.x, o.y + i.y, o.z + i.z});
+    double cosOH = dotd(o, h);
+    double ci = std::max(cosOH, 0.0), kk = 1.0 - eta * eta * (1.0 - ci * ci), F = 1.0;
+    if (kk >= 0.0)
+    {
+      double ct = std::sqrt(kk);
+      double Rper = (eta * ci - ct) / (eta * ci + ct), Rpar = (ci - eta * ct) / (ci + eta * ct);
+      F = 0.5 * (Rpar * Rpar + Rper * Rper);
+    }
+    F *= s;
+    D3 o0{dotd(o, U), dotd(o, Vv), dotd(o, N)}, i0{dotd(i, U), dotd(i, Vv), dotd(i, N)}, h0{dotd(h, U), dotd(h, Vv), dotd(h, N)};
+    double sh = h0.x * h0.x + h0.y * h0.y, z2 = h0.z * h0.z, q = a * a * z2 + sh;
+    double D = a * a / (M_PI * q * q);
+    double dD = a > 0.001 ? (2 * a / (M_PI * q * q)) * (1 - 2 * a * a * z2 / q) : 0.0;
+    auto lam = [&](D3 w) { double t2 = (w.x * w.x + w.y * w.y) / (w.z * w.z); return 0.5 * (-1 + std::sqrt(1 + a * a * t2)); };
+    auto dlam = [&](D3 w) { double t2 = (w.x * w.x + w.y * w.y) / (w.z * w.z); return 0.5 * a * t2 / std::sqrt(1 + a * a * t2); };
+    double G = 1.0 / (1.0 + lam(o0) + lam(i0));
+    double dG = a > 0.001 ? -G * G * (dlam(o0) + dlam(i0)) : 0.0;
+    double kf = F / (4 * cosO);
+    for (int k = 0; k < 3; ++k) { g.v[k] += kf * D * G; g.dAl[k] += kf * (dD * G + D * dG); }
+  }
+  return g;
+}
+
+// One path vertex as the adjoint sees it: the vertex's material, its unoccluded next-event
+// contribution with the BSDF derivatives at the light direction, and the radiance gathered up
+// to and including that estimate with the derivatives at the sampled continuation.
+struct AdjVertex { int mat = 0; bool nee = false, ext = false; V3 dirL, Lrun; GradD gNee, gExt; };
+struct AdjPath { std::vector<AdjVertex> v; V3 L; };
+
 static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, int x, int y, uint32_t sampleIndex,
-                      Accum& A, Stats& st, bool recordSample)
+                      Accum& A, Stats& st, bool recordSample, AdjPath* adj = nullptr)
 {
   const int npix = P.W * P.H;
   const int pix = y * P.W + x;
@@ -504,6 +616,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
         if (A.pathHash) A.pathHash[pix] = hash;
         if (A.firstPrim) A.firstPrim[pix] = firstPrim;
       }
+      if (adj) adj->L = L;
       return;
     }
     if (isEmissive(s.mats[matId]))
@@ -534,11 +647,14 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
         if (A.pathHash) A.pathHash[pix] = hash;
         if (A.firstPrim) A.firstPrim[pix] = firstPrim;
       }
+      if (adj) adj->L = L;
       return;
     }
 
     bool ext = false;
     bool dirOk = false;
+    AdjVertex av;
+    bool haveAv = false;
     V3 dirL(0.f);
     V3 shadowO, shadowD;
     float shadowNear = 0.f, shadowFar = 0.f;
@@ -583,6 +699,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
             float w = misWeight(lightPdf, bsdfPdf);
             dirL = dirT * w * lw;
             dirOk = true;
+            if (adj) av.gNee = bsdfEvalGradD(s.luts, b, ctx, wo, wiL);
           }
         }
       }
@@ -631,6 +748,14 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
         if (m.kind == kDielectric && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
           mediumId = (mediumId == mediumIntId) ? 0 : mediumIntId;
       }
+      if (adj && (m.kind == kDiffuse || m.kind == kCoated))
+      {
+        haveAv = true;
+        av.mat = matId;
+        av.ext = ext;
+        av.dirL = dirL;
+        if (ext) av.gExt = bsdfEvalGradD(s.luts, b, ctx, wo, wi);
+      }
     }
 
     // shadow ray + final pass (:951-1065)
@@ -642,6 +767,13 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       st.rays++;
       hash = hashStep(hash, sr.prim < 0 ? 1u : 2u);  // the visibility event is part of the path
       if (sr.prim < 0) L += dirL;
+      av.nee = sr.prim < 0;
+    }
+    if (haveAv)
+    {
+      av.nee = av.nee && dirOk;
+      av.Lrun = L;
+      adj->v.push_back(av);
     }
     if (!ext)
     {
@@ -653,6 +785,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
         if (A.pathHash) A.pathHash[pix] = hash;
         if (A.firstPrim) A.firstPrim[pix] = firstPrim;
       }
+      if (adj) adj->L = L;
       return;
     }
     org = nextO; dir = nextD; tnear = nextNear;
@@ -844,6 +977,114 @@ long long orc_render(void* p, const void* camp, int W, int H, int rowBegin, int 
     for (auto& t : pool) t.join();
   }
   return rays.load();
+}
+
+// FP64 gradient of the loss mean((I - target)^2) over pixels and channels with respect to
+// (albedo r, g, b, alpha) of every material, for the image of sample passes 0 .. spp-1; the
+// estimator of the HIP adjoint (detached sampling, suffix radiance times df/f at each vertex),
+// evaluated along the oracle's own paths with per-vertex records.  Returns the loss.
+double orc_adjoint(void* p, const void* camp, int W, int H, int spp, uint32_t seed, int maxDepth, int rrDepth,
+                   float minWeight, const float* target, int threads, double* grad /* nMat x 4 */)
+{
+  Scene& s = *(Scene*)p;
+  const Camera& cam = *(const Camera*)camp;
+  const int npix = W * H;
+  std::vector<float> color(3 * size_t(npix), 0.f), weight(npix, 0.f);
+  float* bufs[13] = {color.data(), weight.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr};
+  orc_render(p, camp, W, H, 0, H, 0, spp, seed, maxDepth, rrDepth, minWeight, 3.402823466e38f, threads, bufs, nullptr, nullptr);
+  const double norm = 1.0 / (3.0 * npix);
+  std::vector<double> delta(3 * size_t(npix));
+  double loss = 0.0;
+  for (int i = 0; i < npix; ++i)
+    for (int k = 0; k < 3; ++k)
+    {
+      double I = double(color[size_t(k) * npix + i]) / double(weight[i]);
+      double r = I - double(target[3 * size_t(i) + k]);
+      loss += r * r;
+      delta[size_t(k) * npix + i] = 2.0 * r * norm;
+    }
+  const int nMat = int(s.mats.size());
+  RenderParams P{W, H, seed, maxDepth, rrDepth, minWeight, 3.402823466e38f};
+  if (threads < 1) threads = 1;
+  std::vector<std::vector<double>> part(threads, std::vector<double>(4 * size_t(nMat), 0.0));
+  std::atomic<int> next{0};
+  auto worker = [&](int tid) {
+    Accum A{};
+    Stats st;
+    std::vector<double>& g = part[tid];
+    for (;;)
+    {
+      int row = next.fetch_add(1);
+      if (row >= H) break;
+      for (int x = 0; x < W; ++x)
+      {
+        const int pix = row * W + x;
+        const double dl[3] = {delta[pix] / spp, delta[size_t(npix) + pix] / spp, delta[2 * size_t(npix) + pix] / spp};
+        for (int smp = 0; smp < spp; ++smp)
+        {
+          AdjPath ap;
+          tracePath(s, cam, P, x, row, uint32_t(smp), A, st, false, &ap);
+          const double Lt[3] = {ap.L.x, ap.L.y, ap.L.z};
+          for (const AdjVertex& v : ap.v)
+          {
+            double* gm = &g[4 * size_t(v.mat)];
+            const double dL[3] = {v.dirL.x, v.dirL.y, v.dirL.z}, Lr[3] = {v.Lrun.x, v.Lrun.y, v.Lrun.z};
+            for (int c = 0; c < 3; ++c)
+            {
+              if (v.nee && v.gNee.v[c] > 0.0)
+              {
+                double w = dl[c] * dL[c] / v.gNee.v[c];
+                gm[c] += w * v.gNee.dA[c];
+                gm[3] += w * v.gNee.dAl[c];
+              }
+              if (v.ext && v.gExt.v[c] > 0.0)
+              {
+                double w = dl[c] * (Lt[c] - Lr[c]) / v.gExt.v[c];
+                gm[c] += w * v.gExt.dA[c];
+                gm[3] += w * v.gExt.dAl[c];
+              }
+            }
+          }
+        }
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int i = 1; i < threads; ++i) pool.emplace_back(worker, i);
+  worker(0);
+  for (auto& t : pool) t.join();
+  for (size_t k = 0; k < 4 * size_t(nMat); ++k)
+  {
+    double sum = 0.0;
+    for (int t = 0; t < threads; ++t) sum += part[t][k];
+    grad[k] = sum;
+  }
+  return loss * norm;
+}
+
+// Value and parameter derivatives (FP64) of the BSDF on given frames and directions, with the
+// parameter rows of orc_bsdf: n x [value rgb, d value / d albedo rgb, d value / d alpha rgb].
+void orc_bsdf_grad(int n, const float* luts, const float* params, const float* geo, double* out)
+{
+  const Luts& L = *(const Luts*)luts;
+  for (int i = 0; i < n; ++i)
+  {
+    const float* pr = params + 11 * i;
+    const float* gg = geo + 18 * i;
+    Ctx ctx;
+    ctx.f.U = V3(gg[0], gg[1], gg[2]); ctx.f.V = V3(gg[3], gg[4], gg[5]); ctx.f.N = V3(gg[6], gg[7], gg[8]);
+    ctx.Ng = V3(gg[9], gg[10], gg[11]);
+    V3 wo(gg[12], gg[13], gg[14]), wi(gg[15], gg[16], gg[17]);
+    Material m{};
+    m.kind = int(pr[0]); m.r = pr[1]; m.g = pr[2]; m.b = pr[3]; m.er = pr[4]; m.eg = pr[5]; m.eb = pr[6]; m.roughness = pr[7];
+    m.ior = 1.0f / pr[8]; m.thickness = pr[9]; m.weight = pr[10];
+    Bsdf b = makeBsdf(L, m, ctx, wo, 0, -1);
+    b.fr = ctx.f;
+    GradD g = bsdfEvalGradD(L, b, ctx, wo, wi);
+    double* o = out + 9 * size_t(i);
+    for (int k = 0; k < 3; ++k) { o[k] = g.v[k]; o[3 + k] = g.dA[k]; o[6 + k] = g.dAl[k]; }
+  }
 }
 
 // ---- unit-level entry points used by the golden-vector tests --------------------------
