@@ -193,6 +193,34 @@ int prt_adjoint(prt_context* ctx, const float* target, int32_t spp, double* loss
  * (planar [3][H*W]); equal to the primal sample's radiance (the replay follows the same path). */
 int prt_debug_adjoint_replay(prt_context* ctx, uint32_t sample_index, float* radiance);
 
+/* --- persistent batch rendering (dataset generation) -------------------------------------
+ * No counterpart in the reference's renderer API (its batch mode, main/render/render_window.cpp:
+ * 630-698, renders preset after preset through Device::render); added to render many frames of
+ * uneven resolution from several scenes in one persistent launch, captured in a HIP graph.
+ * Each frame names a scene context (prt_context with geometry, BVH, materials, lights and
+ * renderer parameters set; it must outlive the batch), a camera, a size (multiples of 8, at most
+ * 4096), a Sobol-Burley seed and a sample count; sample indices 0 .. spp-1 are rendered in
+ * order, as prt_render would, and the mean is kept, plus the means after snapshot_spp[k]
+ * samples (ascending, at most 4) and, when aov_mask has PRT_AOV_ALBEDO1 or PRT_AOV_NORMAL1,
+ * the first-hit albedo and normal means.  prt_batch_tune times every (tiles per fetch, blocks
+ * per CU) candidate on the batch with calibration_spp samples per frame and keeps the fastest.
+ * prt_batch_render: *ms = device time of the launch, *samples = path samples rendered.
+ * prt_batch_get copies one buffer of a frame as H x W x 3 floats (prt_blit layout). */
+#define PRT_BATCH_COLOR 0
+#define PRT_BATCH_ALBEDO1 1
+#define PRT_BATCH_NORMAL1 2
+#define PRT_BATCH_SNAPSHOT 16 /* + k: the mean after snapshot_spp[k] samples */
+typedef struct prt_batch prt_batch;
+int prt_batch_create(int32_t device_ordinal, prt_batch** out);
+int prt_batch_destroy(prt_batch* batch);
+int prt_batch_add_frame(prt_batch* batch, prt_context* scene, const prt_camera* cam, int32_t width, int32_t height,
+                        uint32_t seed, int32_t spp, const int32_t* snapshot_spp, int32_t n_snapshots, uint32_t aov_mask,
+                        int32_t* frame_id);
+int prt_batch_tune(prt_batch* batch, int32_t calibration_spp, int32_t* tiles_per_fetch, int32_t* blocks_per_cu,
+                   double* best_ms);
+int prt_batch_render(prt_batch* batch, double* ms, double* samples);
+int prt_batch_get(prt_batch* batch, int32_t frame, int32_t buffer, float* dst);
+
 /* --- per-sample debug outputs of the last pass (parity checks) ------------------------- */
 /* When enabled, each pass also stores per pixel: the clamped sample radiance (planar
  * 3 x W*H), a hash of the path's hit / lobe sequence and the first-hit primitive ID. */

@@ -847,22 +847,12 @@ int prt_debug_enable(prt_context* c, int32_t enable)
   return PRT_OK;
 }
 
-static RenderArgs makeArgs(prt_context* c)
+// Camera (camera/pinhole_camera.h:72-90 and thin_lens_camera.h:155-161) and frame size of the
+// kernel arguments.
+static void setCameraArgs(RenderArgs& a, const prt_camera& k, int W, int H)
 {
-  RenderArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.lut = c->lut.p;
-  a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
-  a.nLights = int(c->lights.size());
-  a.lightOffset = c->areaCount > 0 ? 1 : 0;
-  for (int i = 0; i < a.nLights; ++i) a.lights[i] = c->lights[i];
-  for (int i = 0; i <= a.nLights + a.lightOffset; ++i) a.lightCdf[i] = c->lightCdf[i];
-  a.areaTris = c->areaTris.p; a.areaCdf = c->areaCdf.p; a.areaMap = c->areaMap.p; a.areaCount = c->areaCount;
-  a.env.texels = c->envTex.p; a.env.xDists = c->envX.p; a.env.yDist = c->envY.p;
-  a.env.W = c->envW; a.env.H = c->envH; a.env.LW = c->envLW; a.env.LH = c->envLH;
-  // camera/pinhole_camera.h:72-90 and thin_lens_camera.h:155-161
-  const prt_camera& k = c->cam;
-  const float aspect = float(c->W) / float(c->H);
+  const float aspect = float(W) / float(H);
+
   const float hh = std::tan(0.5f * k.fov), hw = hh * aspect;
   H3 U = hmk(k.U[0], k.U[1], k.U[2]), V = hmk(k.V[0], k.V[1], k.V[2]), N = hmk(k.N[0], k.N[1], k.N[2]);
   auto basisMul = [&](float x, float y, float z) { return hadd(hadd(hscale(U, x), hscale(V, y)), hscale(N, z)); };
@@ -876,9 +866,25 @@ static RenderArgs makeArgs(prt_context* c)
   a.nearClip = k.nearClip;
   a.nearClipScale = k.nearClip / k.focalDistance;
   a.pinhole = k.lensRadius == 0.f;  // render/device_cpu.cpp:117-128
-  a.W = c->W; a.H = c->H; a.npix = c->npix;
-  a.invW = 1.0f / float(c->W);
-  a.invH = 1.0f / float(c->H);
+  a.W = W; a.H = H; a.npix = W * H;
+  a.invW = 1.0f / float(W);
+  a.invH = 1.0f / float(H);
+}
+
+static RenderArgs makeArgs(prt_context* c)
+{
+  RenderArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.lut = c->lut.p;
+  a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
+  a.nLights = int(c->lights.size());
+  a.lightOffset = c->areaCount > 0 ? 1 : 0;
+  for (int i = 0; i < a.nLights; ++i) a.lights[i] = c->lights[i];
+  for (int i = 0; i <= a.nLights + a.lightOffset; ++i) a.lightCdf[i] = c->lightCdf[i];
+  a.areaTris = c->areaTris.p; a.areaCdf = c->areaCdf.p; a.areaMap = c->areaMap.p; a.areaCount = c->areaCount;
+  a.env.texels = c->envTex.p; a.env.xDists = c->envX.p; a.env.yDist = c->envY.p;
+  a.env.W = c->envW; a.env.H = c->envH; a.env.LW = c->envLW; a.env.LH = c->envLH;
+  setCameraArgs(a, c->cam, c->W, c->H);
   prt_frame_dev& f = a.fb;
   f.color = c->color.p; f.weight = c->weight.p; f.color2 = c->color2.p;
   f.albedo = c->albedo.p; f.albedoW = c->albedoW.p; f.albedo1 = c->albedo1.p; f.albedo1W = c->albedo1W.p;
@@ -1239,6 +1245,274 @@ int prt_debug_kernel_ms(prt_context* c, double* ms4, double* counters6, double* 
   if (ms4) for (int k = 0; k < 4; ++k) ms4[k] = c->kernelMs[k];
   if (counters6) for (int k = 0; k < 6; ++k) counters6[k] = c->lastStats[k];
   if (launches2) { launches2[0] = c->launches[0]; launches2[1] = c->launches[1]; }
+  return PRT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Persistent batch renderer (dataset generation): see include/prt_hip.h and batch.hip.
+// ---------------------------------------------------------------------------------------
+struct prt_batch
+{
+  int device = 0, cus = 0;
+  hipStream_t stream = nullptr;
+  struct Frame
+  {
+    prt_context* scene;
+    prt_camera cam;
+    int W, H, spp, nSnap;
+    uint32_t seed, aov;
+    int snap[PRT_BATCH_MAX_SNAPSHOTS];
+  };
+  std::vector<Frame> frames;
+  std::vector<size_t> outOffset;  // per frame: first float of its outputs in out
+  std::vector<BatchFrameDev> desc;
+  DevBuf<float> out;
+  DevBuf<RenderArgs> args;
+  DevBuf<BatchFrameDev> fdev;
+  DevBuf<int> counter, spill;
+  DevBuf<unsigned long long> samples;
+  int spillStride = 0, nTiles = 0;
+  int tilesPerFetch = 1, blocksPerCu = 2;
+  bool prepared = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+constexpr int kBatchMaxBlocksPerCu = 4;
+
+void dropGraph(prt_batch* b)
+{
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Outputs, kernel arguments and frame descriptors of every frame on the device.
+int batchPrepare(prt_batch* b)
+{
+  if (b->frames.empty()) return fail(PRT_ERR_STATE, "the batch has no frames");
+  size_t total = 0;
+  int tiles = 0;
+  b->outOffset.clear();
+  for (const auto& f : b->frames)
+  {
+    const size_t n = size_t(f.W) * f.H;
+    b->outOffset.push_back(total);
+    total += 3 * n * (1 + f.nSnap + ((f.aov & (PRT_AOV_ALBEDO1 | PRT_AOV_NORMAL1)) ? 2 : 0));
+    tiles += (f.W / 8) * (f.H / 8);
+  }
+  HIP_TRY(b->out.alloc(total));
+  HIP_TRY(b->args.alloc(b->frames.size()));
+  HIP_TRY(b->fdev.alloc(b->frames.size()));
+  std::vector<RenderArgs> args(b->frames.size());
+  b->desc.assign(b->frames.size(), BatchFrameDev{});
+  int tileBegin = 0;
+  for (size_t i = 0; i < b->frames.size(); ++i)
+  {
+    const auto& f = b->frames[i];
+    RenderArgs a = makeArgs(f.scene);
+    setCameraArgs(a, f.cam, f.W, f.H);
+    a.seed = f.seed;
+    std::memset(&a.fb, 0, sizeof(a.fb));
+    args[i] = a;
+    BatchFrameDev& d = b->desc[i];
+    const size_t n = size_t(f.W) * f.H;
+    float* base = b->out.p + b->outOffset[i];
+    d.args = b->args.p + i;
+    d.color = base;
+    d.snap = base + 3 * n;
+    const bool aovs = (f.aov & (PRT_AOV_ALBEDO1 | PRT_AOV_NORMAL1)) != 0;
+    d.albedo1 = aovs ? base + 3 * n * (1 + f.nSnap) : nullptr;
+    d.normal1 = aovs ? base + 3 * n * (2 + f.nSnap) : nullptr;
+    d.W = f.W; d.H = f.H; d.npix = int(n); d.spp = f.spp; d.nSnap = f.nSnap;
+    for (int k = 0; k < PRT_BATCH_MAX_SNAPSHOTS; ++k) d.snapSpp[k] = k < f.nSnap ? f.snap[k] : 0;
+    d.tileBegin = tileBegin;
+    d.tilesX = f.W / 8;
+    tileBegin += (f.W / 8) * (f.H / 8);
+  }
+  b->nTiles = tiles;
+  HIP_TRY(hipMemcpy(b->args.p, args.data(), sizeof(RenderArgs) * args.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->fdev.p, b->desc.data(), sizeof(BatchFrameDev) * b->desc.size(), hipMemcpyHostToDevice));
+  b->spillStride = kBatchMaxBlocksPerCu * b->cus * kBatchBlockSize;
+  HIP_TRY(b->spill.alloc(2 * size_t(PRT_MAX_STACK - PRT_LDS_STACK) * b->spillStride));
+  HIP_TRY(b->counter.alloc(1));
+  HIP_TRY(b->samples.alloc(1));
+  b->prepared = true;
+  dropGraph(b);
+  return PRT_OK;
+}
+
+void batchLaunch(prt_batch* b)
+{
+  launchBatch(b->fdev.p, int(b->frames.size()), b->nTiles, b->counter.p, b->tilesPerFetch, b->blocksPerCu * b->cus, b->spill.p,
+              b->spillStride, b->samples.p, b->stream);
+}
+}  // namespace
+
+extern "C" {
+
+int prt_batch_create(int32_t device_ordinal, prt_batch** out)
+{
+  if (!out) return fail(PRT_ERR_INVALID, "out is null");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device_ordinal < 0 || device_ordinal >= n) return fail(PRT_ERR_INVALID, "invalid device ordinal");
+  HIP_TRY(hipSetDevice(device_ordinal));
+  prt_batch* b = new prt_batch();
+  b->device = device_ordinal;
+  hipDeviceProp_t props{};
+  hipError_t e = hipGetDeviceProperties(&props, device_ordinal);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev[1]);
+  if (e != hipSuccess) { prt_batch_destroy(b); return fail(PRT_ERR_HIP, hipGetErrorString(e)); }
+  b->cus = props.multiProcessorCount;
+  *out = b;
+  return PRT_OK;
+}
+
+int prt_batch_destroy(prt_batch* b)
+{
+  if (!b) return PRT_OK;
+  hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  dropGraph(b);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PRT_OK;
+}
+
+int prt_batch_add_frame(prt_batch* b, prt_context* scene, const prt_camera* cam, int32_t width, int32_t height, uint32_t seed,
+                        int32_t spp, const int32_t* snapshot_spp, int32_t n_snapshots, uint32_t aov_mask, int32_t* frame_id)
+{
+  if (!b || !scene || !cam) return fail(PRT_ERR_INVALID, "null argument");
+  if (scene->device != b->device) return fail(PRT_ERR_INVALID, "scene context on another device");
+  if (!scene->bvhReady || !scene->lightsReady || !scene->mats.p || !scene->rendererReady)
+    return fail(PRT_ERR_STATE, "scene context needs geometry, BVH, materials, lights and renderer parameters");
+  if (scene->maxMaterialId >= int(scene->materials.size())) return fail(PRT_ERR_INVALID, "triangle material ID beyond the material table");
+  if (width <= 0 || height <= 0 || width % 8 != 0 || height % 8 != 0 || width > 4096 || height > 4096)
+    return fail(PRT_ERR_INVALID, "batch frame size must be a multiple of 8 and at most 4096");
+  if (!(cam->fov > 0.f && cam->fov < 3.14159265f)) return fail(PRT_ERR_INVALID, "invalid fov");
+  if (spp < 1) return fail(PRT_ERR_INVALID, "spp must be >= 1");
+  if (n_snapshots < 0 || n_snapshots > PRT_BATCH_MAX_SNAPSHOTS || (n_snapshots > 0 && !snapshot_spp))
+    return fail(PRT_ERR_INVALID, "0..4 snapshots");
+  prt_batch::Frame f{};
+  f.scene = scene;
+  f.cam = *cam;
+  f.W = width; f.H = height; f.spp = spp; f.nSnap = n_snapshots;
+  f.seed = seed;
+  f.aov = aov_mask;
+  for (int k = 0; k < n_snapshots; ++k)
+  {
+    if (snapshot_spp[k] < 1 || snapshot_spp[k] > spp || (k > 0 && snapshot_spp[k] <= snapshot_spp[k - 1]))
+      return fail(PRT_ERR_INVALID, "snapshot sample counts must increase within 1..spp");
+    f.snap[k] = snapshot_spp[k];
+  }
+  b->frames.push_back(f);
+  b->prepared = false;
+  if (frame_id) *frame_id = int(b->frames.size()) - 1;
+  return PRT_OK;
+}
+
+int prt_batch_tune(prt_batch* b, int32_t calibration_spp, int32_t* tiles_per_fetch, int32_t* blocks_per_cu, double* best_ms)
+{
+  if (!b) return fail(PRT_ERR_INVALID, "batch is null");
+  if (calibration_spp < 1) return fail(PRT_ERR_INVALID, "calibration_spp must be >= 1");
+  HIP_TRY(hipSetDevice(b->device));
+  if (!b->prepared)
+  {
+    int rc = batchPrepare(b);
+    if (rc != PRT_OK) return rc;
+  }
+  // the frames' sample counts are lowered to calibration_spp while timing each candidate
+  std::vector<BatchFrameDev> cal = b->desc;
+  for (auto& d : cal) { d.spp = std::min(d.spp, int(calibration_spp)); d.nSnap = 0; }
+  HIP_TRY(hipMemcpy(b->fdev.p, cal.data(), sizeof(BatchFrameDev) * cal.size(), hipMemcpyHostToDevice));
+  const int fetches[4] = {1, 2, 4, 8};
+  float best = 3.4e38f;
+  int bestF = b->tilesPerFetch, bestB = b->blocksPerCu;
+  for (int bp = 1; bp <= kBatchMaxBlocksPerCu; ++bp)
+    for (int fi = 0; fi < 4; ++fi)
+    {
+      b->blocksPerCu = bp;
+      b->tilesPerFetch = fetches[fi];
+      batchLaunch(b);  // warm-up of this shape
+      HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+      batchLaunch(b);
+      HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+      HIP_TRY(hipEventSynchronize(b->ev[1]));
+      HIP_TRY(hipGetLastError());
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+      if (ms < best) { best = ms; bestF = fetches[fi]; bestB = bp; }
+    }
+  b->tilesPerFetch = bestF;
+  b->blocksPerCu = bestB;
+  HIP_TRY(hipMemcpy(b->fdev.p, b->desc.data(), sizeof(BatchFrameDev) * b->desc.size(), hipMemcpyHostToDevice));
+  dropGraph(b);
+  if (tiles_per_fetch) *tiles_per_fetch = bestF;
+  if (blocks_per_cu) *blocks_per_cu = bestB;
+  if (best_ms) *best_ms = best;
+  return PRT_OK;
+}
+
+int prt_batch_render(prt_batch* b, double* ms, double* samples)
+{
+  if (!b) return fail(PRT_ERR_INVALID, "batch is null");
+  HIP_TRY(hipSetDevice(b->device));
+  if (!b->prepared)
+  {
+    int rc = batchPrepare(b);
+    if (rc != PRT_OK) return rc;
+  }
+  if (!b->exec)
+  {
+    // capture the reset and the persistent launch once; later calls replay the graph
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    batchLaunch(b);
+    hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+    if (e != hipSuccess) return fail(PRT_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  unsigned long long n = 0;
+  HIP_TRY(hipMemcpyAsync(&n, b->samples.p, sizeof(n), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
+  if (ms) *ms = t;
+  if (samples) *samples = double(n);
+  return PRT_OK;
+}
+
+int prt_batch_get(prt_batch* b, int32_t frame, int32_t buffer, float* dst)
+{
+  if (!b || !dst) return fail(PRT_ERR_INVALID, "null argument");
+  if (frame < 0 || frame >= int(b->frames.size())) return fail(PRT_ERR_INVALID, "invalid frame");
+  if (!b->prepared) return fail(PRT_ERR_STATE, "batch not rendered");
+  HIP_TRY(hipSetDevice(b->device));
+  const BatchFrameDev& d = b->desc[frame];
+  const float* src = nullptr;
+  if (buffer == PRT_BATCH_COLOR) src = d.color;
+  else if (buffer >= PRT_BATCH_SNAPSHOT && buffer < PRT_BATCH_SNAPSHOT + d.nSnap) src = d.snap + size_t(buffer - PRT_BATCH_SNAPSHOT) * 3 * d.npix;
+  else if (buffer == PRT_BATCH_ALBEDO1) src = d.albedo1;
+  else if (buffer == PRT_BATCH_NORMAL1) src = d.normal1;
+  if (!src) return fail(PRT_ERR_INVALID, "buffer not produced for this frame");
+  const size_t n = size_t(d.npix);
+  std::vector<float> h(3 * n);
+  HIP_TRY(hipMemcpy(h.data(), src, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) dst[3 * i + k] = h[k * n + i];
   return PRT_OK;
 }
 

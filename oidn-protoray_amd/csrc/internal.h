@@ -77,6 +77,25 @@ struct RenderArgs
   unsigned long long* stats;  // closest hit: [0] rays, [1] inner-node visits, [2] triangle tests; shadow: [3..5]
 };
 
+// One frame of a persistent batch (batch.hip): its scene and camera as kernel arguments, its
+// outputs (planar [3][npix] means), its sample count and snapshot counts, and its place in the
+// batch's tile list (8x8 tiles, row-major, tilesX per row).
+#define PRT_BATCH_MAX_SNAPSHOTS 4
+struct BatchFrameDev
+{
+  const RenderArgs* args;
+  float* color;    // mean of all spp samples
+  float* snap;     // [nSnap][3][npix]: means after snapSpp[k] samples
+  float* albedo1;  // first-hit albedo / normal means (both null when not requested)
+  float* normal1;
+  int W, H, npix, spp, nSnap;
+  int snapSpp[PRT_BATCH_MAX_SNAPSHOTS];
+  int tileBegin, tilesX;
+};
+constexpr int kBatchBlockSize = 256;
+void launchBatch(const BatchFrameDev* frames, int nFrames, int nTiles, int* counter, int tilesPerFetch, int grid, int* spill,
+                 int spillStride, unsigned long long* samples, hipStream_t s);
+
 // Kernel launchers (render.hip)
 struct LaunchCfg
 {

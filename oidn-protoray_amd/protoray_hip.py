@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libprt_hip.so")
 PRT_OK, PRT_ERR_INVALID, PRT_ERR_HIP, PRT_ERR_NOMEM, PRT_ERR_STATE = 0, -1, -2, -3, -4
 AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR = 1, 2, 4, 8, 16, 32
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
+BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_SNAPSHOT = 0, 1, 2, 16
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
@@ -73,6 +74,12 @@ SIGNATURES = {
     "prt_debug_sampler": [_vp, _i, _vp, _i, _vp, _vp, _vp],
     "prt_debug_luts": [_vp, _vp, ctypes.c_int32, _vp],
     "prt_adjoint": [_vp, _vp, _i, _vp, _vp, _i, _vp],
+    "prt_batch_create": [_i, _vp],
+    "prt_batch_destroy": [_vp],
+    "prt_batch_add_frame": [_vp, _vp, _vp, _i, _i, _u, _i, _vp, _i, _u, _vp],
+    "prt_batch_tune": [_vp, _i, _vp, _vp, _vp],
+    "prt_batch_render": [_vp, _vp, _vp],
+    "prt_batch_get": [_vp, _i, _i, _vp],
     "prt_debug_adjoint_replay": [_vp, _u, _vp],
     "prt_debug_bvh_stats": [_vp, _vp],
     "prt_debug_kernel_ms": [_vp, _vp, _vp, _vp],
@@ -291,3 +298,49 @@ def load_scene(dev, scene):
     dev.initRenderer(maxDepth=scene["max_depth"])
     dev.initSampler(scene.get("seed", 0))
     dev.initCamera(scene["camera"])
+
+
+class BatchHip:
+    """Persistent batch renderer: frames of uneven size from several scenes in one launch."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.prt_batch_create(device, ctypes.byref(h)))
+        self.h = h
+        self.frames = []
+        self.scenes = []  # keeps the scene devices alive while the batch exists
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.prt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def addFrame(self, scene_dev, camera, width, height, seed, spp, snapshots=(), aov_mask=0):
+        cam = np.ascontiguousarray(camera, CAMERA_DTYPE)
+        snap = np.ascontiguousarray(snapshots, np.int32)
+        fid = ctypes.c_int32(0)
+        _check(_lib.prt_batch_add_frame(self.h, scene_dev.h, _ptr(cam), width, height, seed, spp,
+                                        _ptr(snap) if snap.size else None, int(snap.size), aov_mask, ctypes.byref(fid)))
+        self.frames.append((width, height, tuple(int(x) for x in snapshots), aov_mask))
+        self.scenes.append(scene_dev)
+        return fid.value
+
+    def tune(self, calibration_spp=1):
+        f, b, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_double(0.0)
+        _check(_lib.prt_batch_tune(self.h, calibration_spp, ctypes.byref(f), ctypes.byref(b), ctypes.byref(ms)))
+        return {"tiles_per_fetch": f.value, "blocks_per_cu": b.value, "ms": ms.value}
+
+    def render(self):
+        ms, n = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(_lib.prt_batch_render(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return {"ms": ms.value, "samples": n.value}
+
+    def get(self, frame, buffer=BATCH_COLOR):
+        W, H = self.frames[frame][:2]
+        out = np.zeros((H, W, 3), np.float32)
+        _check(_lib.prt_batch_get(self.h, frame, buffer, _ptr(out)))
+        return out
