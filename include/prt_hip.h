@@ -221,6 +221,41 @@ int prt_batch_tune(prt_batch* batch, int32_t calibration_spp, int32_t* tiles_per
 int prt_batch_render(prt_batch* batch, double* ms, double* samples);
 int prt_batch_get(prt_batch* batch, int32_t frame, int32_t buffer, float* dst);
 
+/* --- learned denoiser ---------------------------------------------------------------------
+ * The Denoiser plug-in (image/denoiser.h:10-41): init with the colour / albedo / normal /
+ * output images (H x W x 3 floats each), execute (returns the denoising time in seconds),
+ * version.  The filter is a U-Net (4 levels, 3x3 convolutions, 32 / 48 / 64 / 96 / 128 channels,
+ * ReLU, max pooling, nearest upsampling with skips) on BF16 matrix cores with FP32 accumulation.
+ * Weights: param_count FP32 values (per layer: weights [cout][cin][3][3] then bias [cout]; the
+ * layers enc0..enc4, dec3..dec0, out).  Training (no counterpart in the reference, which calls
+ * OIDN's pretrained filters): train_step runs forward, backward and one Adam step on a batch of
+ * n images h x w (multiples of 16) given as input (n*h*w*9: colour, albedo, normal) and target
+ * (n*h*w*3), with the L1 loss of log(1 + colour); gradient returns the loss and the FP32
+ * parameter gradient of a batch without updating. */
+typedef struct prt_denoiser prt_denoiser;
+typedef struct prt_denoiser_params {   /* Denoiser::Params (image/denoiser.h:14-29) */
+  const float* color;
+  const float* albedo;
+  const float* normal;
+  float* output;
+  int32_t width, height;
+  int32_t hdr;
+  float hdrScale;                      /* <= 0 or NaN: 1 */
+} prt_denoiser_params;
+int prt_denoiser_create(int32_t device_ordinal, prt_denoiser** out);
+int prt_denoiser_destroy(prt_denoiser* dn);
+int prt_denoiser_init(prt_denoiser* dn, const prt_denoiser_params* params);
+int prt_denoiser_execute(prt_denoiser* dn, double* seconds);
+int prt_denoiser_get_version(prt_denoiser* dn, char* buf, int32_t len);
+int prt_denoiser_kernel_ms(prt_denoiser* dn, double* ms);
+int prt_denoiser_param_count(prt_denoiser* dn, int64_t* n);
+int prt_denoiser_get_weights(prt_denoiser* dn, float* weights, int64_t n);
+int prt_denoiser_set_weights(prt_denoiser* dn, const float* weights, int64_t n);
+int prt_denoiser_train_step(prt_denoiser* dn, const float* input, const float* target, int32_t n, int32_t h, int32_t w,
+                            float lr, double* loss, double* ms);
+int prt_denoiser_gradient(prt_denoiser* dn, const float* input, const float* target, int32_t n, int32_t h, int32_t w,
+                          double* loss, float* grad, int64_t grad_count);
+
 /* --- per-sample debug outputs of the last pass (parity checks) ------------------------- */
 /* When enabled, each pass also stores per pixel: the clamped sample radiance (planar
  * 3 x W*H), a hash of the path's hit / lobe sequence and the first-hit primitive ID. */

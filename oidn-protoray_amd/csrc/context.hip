@@ -309,6 +309,10 @@ struct prt_context
     if (e_ != hipSuccess) return fail(PRT_ERR_HIP, hipGetErrorString(e_)); \
   } while (0)
 
+namespace prt {
+int setLastError(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace prt
+
 extern "C" {
 
 const char* prt_last_error(void) { return gLastError.c_str(); }

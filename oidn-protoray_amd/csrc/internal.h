@@ -21,6 +21,9 @@ struct BvhData
   double buildMs = 0.0;
 };
 
+// Records the message returned by prt_last_error() and returns code (context.hip).
+int setLastError(int code, const std::string& msg);
+
 hipError_t buildBvh(const float* dPos, const int* dIdx, int ntris, BvhData& out, hipStream_t stream, std::string& err);
 void freeBvh(BvhData& b);
 

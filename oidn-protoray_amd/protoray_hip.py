@@ -74,6 +74,17 @@ SIGNATURES = {
     "prt_debug_sampler": [_vp, _i, _vp, _i, _vp, _vp, _vp],
     "prt_debug_luts": [_vp, _vp, ctypes.c_int32, _vp],
     "prt_adjoint": [_vp, _vp, _i, _vp, _vp, _i, _vp],
+    "prt_denoiser_create": [_i, _vp],
+    "prt_denoiser_destroy": [_vp],
+    "prt_denoiser_init": [_vp, _vp],
+    "prt_denoiser_execute": [_vp, _vp],
+    "prt_denoiser_get_version": [_vp, ctypes.c_char_p, _i],
+    "prt_denoiser_kernel_ms": [_vp, _vp],
+    "prt_denoiser_param_count": [_vp, _vp],
+    "prt_denoiser_get_weights": [_vp, _vp, ctypes.c_int64],
+    "prt_denoiser_set_weights": [_vp, _vp, ctypes.c_int64],
+    "prt_denoiser_train_step": [_vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp],
+    "prt_denoiser_gradient": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, ctypes.c_int64],
     "prt_batch_create": [_i, _vp],
     "prt_batch_destroy": [_vp],
     "prt_batch_add_frame": [_vp, _vp, _vp, _i, _i, _u, _i, _vp, _i, _u, _vp],
@@ -344,3 +355,89 @@ class BatchHip:
         out = np.zeros((H, W, 3), np.float32)
         _check(_lib.prt_batch_get(self.h, frame, buffer, _ptr(out)))
         return out
+
+
+class DenoiserParams(ctypes.Structure):  # prt_denoiser_params = Denoiser::Params (image/denoiser.h:14-29)
+    _fields_ = [("color", ctypes.c_void_p), ("albedo", ctypes.c_void_p), ("normal", ctypes.c_void_p),
+                ("output", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("hdr", ctypes.c_int32), ("hdrScale", ctypes.c_float)]
+
+
+class DenoiserHip:
+    """The learned denoiser behind the Denoiser interface (init / execute / getVersion)."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.prt_denoiser_create(device, ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int64(0)
+        _check(_lib.prt_denoiser_param_count(self.h, ctypes.byref(n)))
+        self.n_params = n.value
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.prt_denoiser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def getVersion(self):
+        buf = ctypes.create_string_buffer(128)
+        _check(_lib.prt_denoiser_get_version(self.h, buf, 128))
+        return buf.value.decode()
+
+    def init(self, color, albedo, normal, output, hdr=True, hdrScale=float("nan")):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (color, albedo, normal)]
+        if not (output.dtype == np.float32 and output.flags["C_CONTIGUOUS"]):
+            raise ValueError("output must be a C-contiguous float32 array")
+        H, W = arrs[0].shape[:2]
+        self._keep = arrs + [output]
+        p = DenoiserParams(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, output.ctypes.data, W, H,
+                           int(hdr), hdrScale)
+        _check(_lib.prt_denoiser_init(self.h, ctypes.byref(p)))
+
+    def execute(self):
+        t = ctypes.c_double(0.0)
+        _check(_lib.prt_denoiser_execute(self.h, ctypes.byref(t)))
+        return t.value
+
+    def kernelMs(self):
+        t = ctypes.c_double(0.0)
+        _check(_lib.prt_denoiser_kernel_ms(self.h, ctypes.byref(t)))
+        return t.value
+
+    def denoise(self, color, albedo, normal):
+        out = np.zeros(np.shape(color), np.float32)
+        self.init(color, albedo, normal, out)
+        self.execute()
+        return out
+
+    def getWeights(self):
+        w = np.zeros(self.n_params, np.float32)
+        _check(_lib.prt_denoiser_get_weights(self.h, _ptr(w), w.size))
+        return w
+
+    def setWeights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _check(_lib.prt_denoiser_set_weights(self.h, _ptr(w), w.size))
+
+    def trainStep(self, inp, target, lr):
+        """inp: n x h x w x 9 (colour, albedo, normal); target: n x h x w x 3.  Returns (loss, device ms)."""
+        inp = np.ascontiguousarray(inp, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        n, h, w = inp.shape[:3]
+        loss, ms = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(_lib.prt_denoiser_train_step(self.h, _ptr(inp), _ptr(target), n, h, w, lr, ctypes.byref(loss), ctypes.byref(ms)))
+        return loss.value, ms.value
+
+    def gradient(self, inp, target):
+        inp = np.ascontiguousarray(inp, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        n, h, w = inp.shape[:3]
+        loss = ctypes.c_double(0.0)
+        g = np.zeros(self.n_params, np.float32)
+        _check(_lib.prt_denoiser_gradient(self.h, _ptr(inp), _ptr(target), n, h, w, ctypes.byref(loss), _ptr(g), g.size))
+        return loss.value, g
