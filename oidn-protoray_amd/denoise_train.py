@@ -1,0 +1,190 @@
+"""Training and evaluation of the learned denoiser on data the renderer produces.
+
+Training set: frames of the batched dataset configuration's scenes (sun/sky spheres, diffuse /
+conductor / dielectric / coated materials), each rendered twice in one persistent batch launch:
+a 4-sample noisy frame with its 4-sample first-hit albedo and normal, and a reference frame of
+the same camera and seed with ref_spp samples.  Training: random 256x256 crops, batch 16,
+Adam, L1 loss of log(1 + colour) (the kernels of csrc/denoise.hip).
+
+Evaluation: relMSE = mean over pixels and channels of (x - y)^2 / (y^2 + 1e-2) against a
+high-sample reference of the 1920x1080 sun/sky workload (20,971,522 triangles, depth 8), for
+plain rendering at increasing sample counts and for the denoised 4-, 8- and 16-sample images;
+the time to reach relMSE 1e-3 is interpolated in log-log space from those points.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+for _p in (HERE, os.path.join(HERE, "scenes")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np
+
+import dataset as D
+import protoray_hip as P
+import scenegen as G
+
+
+def training_set(n_frames=24, size=(384, 384), ref_spp=4096, n_scenes=4, max_tris=1_000_000, seed=11, device=0, log=print):
+    scenes = D.dataset_scenes(n_scenes, max_tris=max_tris)
+    devs = []
+    for sc in scenes:
+        d = P.DeviceHip(device)
+        P.load_scene(d, sc)
+        devs.append(d)
+    rng = np.random.default_rng(seed)
+    batch = P.BatchHip(device)
+    W, H = size
+    for i in range(n_frames):
+        pos = (rng.uniform(-6, 6), rng.uniform(0.5, 5), rng.uniform(-4, 2))
+        tgt = (rng.uniform(-6, 6), rng.uniform(0, 4), rng.uniform(12, 24))
+        cam = D.look_at(pos, tgt, fov_deg=float(rng.uniform(35, 60)), lens_radius=float(rng.choice([0.0, 0.05])),
+                        focal=float(np.linalg.norm(np.subtract(tgt, pos))))
+        d = devs[i % n_scenes]
+        batch.addFrame(d, cam, W, H, 1000 + i, 4, aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1)  # noisy input
+        batch.addFrame(d, cam, W, H, 1000 + i, ref_spp)                                   # reference
+    t0 = time.perf_counter()
+    r = batch.render()
+    log(f"training set: {n_frames} frames {W}x{H}, 4 spp + {ref_spp} spp references, {r['samples'] / 1e9:.2f} G samples "
+        f"in {r['ms'] / 1e3:.1f} s ({time.perf_counter() - t0:.1f} s wall)")
+    inputs = np.zeros((n_frames, H, W, 9), np.float32)
+    targets = np.zeros((n_frames, H, W, 3), np.float32)
+    for i in range(n_frames):
+        inputs[i, ..., 0:3] = batch.get(2 * i)
+        inputs[i, ..., 3:6] = batch.get(2 * i, P.BATCH_ALBEDO1)
+        inputs[i, ..., 6:9] = batch.get(2 * i, P.BATCH_NORMAL1)
+        targets[i] = batch.get(2 * i + 1)
+    batch.close()
+    for d in devs:
+        d.close()
+    return inputs, targets, r
+
+
+def train(dn, inputs, targets, steps=1500, batch=16, crop=256, lr=1e-3, seed=5, log=print):
+    rng = np.random.default_rng(seed)
+    n, H, W, _ = inputs.shape
+    losses, dev_ms = [], 0.0
+    t0 = time.perf_counter()
+    for step in range(steps):
+        idx = rng.integers(0, n, batch)
+        ys = rng.integers(0, H - crop + 1, batch)
+        xs = rng.integers(0, W - crop + 1, batch)
+        bi = np.stack([inputs[i, y:y + crop, x:x + crop] for i, y, x in zip(idx, ys, xs)])
+        bt = np.stack([targets[i, y:y + crop, x:x + crop] for i, y, x in zip(idx, ys, xs)])
+        rate = lr * 0.5 * (1 + math.cos(math.pi * step / steps))  # cosine decay
+        loss, ms = dn.trainStep(bi, bt, rate)
+        losses.append(loss)
+        dev_ms += ms
+        if step % 250 == 0 or step == steps - 1:
+            log(f"step {step}: loss {np.mean(losses[-50:]):.4f}, {dev_ms / (step + 1):.1f} ms per step on the device")
+    return {"steps": steps, "final_loss": float(np.mean(losses[-50:])), "first_loss": float(losses[0]),
+            "device_ms_per_step": dev_ms / steps, "wall_s": time.perf_counter() - t0}
+
+
+def relmse(x, ref):
+    return G.relmse(x, ref)
+
+
+def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4, 8, 16), device=0, log=print):
+    """relMSE against a ref_spp render of the 1080p sun/sky workload for plain and denoised images."""
+    sc = G.spheres_sunsky()
+    dev = P.DeviceHip(device)
+    P.load_scene(dev, sc)
+    W, H = sc["width"], sc["height"]
+    dev.initFrame(W, H, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+    # reference: sample indices of a different seed than the evaluated images
+    dev.initSampler(7)
+    dev.clearFrame()
+    t0 = time.perf_counter()
+    for _ in range(ref_spp // 64):
+        dev.render(64)
+    ref = dev.blitFrame()
+    log(f"reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    res = {"plain": [], "denoised": []}
+    dev.initSampler(0)
+    dev.clearFrame()
+    done = 0
+    want = sorted(set(plain_spp) | set(denoised_spp))
+    for spp in want:
+        dev.synchronize()
+        t0 = time.perf_counter()
+        dev.render(spp - done)
+        dev.synchronize()
+        done = spp
+        img = dev.blitFrame()
+        if spp in plain_spp:
+            res["plain"].append({"spp": spp, "relmse": relmse(img, ref)})
+        if spp in denoised_spp:
+            alb, nrm = dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1)
+            out = np.zeros_like(img)
+            dn.init(img, alb, nrm, out)
+            secs = dn.execute()
+            res["denoised"].append({"spp": spp, "relmse": relmse(out, ref), "denoise_s": secs, "denoise_kernel_ms": dn.kernelMs(),
+                                    "noisy_relmse": relmse(img, ref)})
+    # time per sample pass of this frame
+    dev.synchronize()
+    t0 = time.perf_counter()
+    dev.render(16)
+    dev.synchronize()
+    ms_per_spp = (time.perf_counter() - t0) * 1e3 / 16
+    dev.close()
+    for r in res["plain"]:
+        r["ms"] = r["spp"] * ms_per_spp
+    for r in res["denoised"]:
+        r["ms"] = r["spp"] * ms_per_spp + r["denoise_s"] * 1e3
+    res["ms_per_spp"] = ms_per_spp
+    res["ref_spp"] = ref_spp
+    res["time_to_relmse_1e-3_ms"] = {"plain": time_to(res["plain"], 1e-3), "denoised": time_to(res["denoised"], 1e-3)}
+    log(json.dumps(res))
+    return res
+
+
+def time_to(points, target):
+    """Time at which relMSE reaches target, interpolated in log-log space between the measured
+    points (None when no measured point reaches it, or extrapolated from the last two, marked)."""
+    pts = sorted((p["ms"], p["relmse"]) for p in points)
+    for (t0, e0), (t1, e1) in zip(pts, pts[1:]):
+        if e0 > target >= e1:
+            a = (math.log(target) - math.log(e0)) / (math.log(e1) - math.log(e0))
+            return math.exp(math.log(t0) + a * (math.log(t1) - math.log(t0)))
+    if pts and pts[0][1] <= target:
+        return pts[0][0]
+    if len(pts) >= 2 and pts[-1][1] < pts[-2][1]:
+        (t0, e0), (t1, e1) = pts[-2], pts[-1]
+        slope = (math.log(e1) - math.log(e0)) / (math.log(t1) - math.log(t0))
+        return {"extrapolated": math.exp(math.log(t1) + (math.log(target) - math.log(e1)) / slope)}
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description="train the denoiser on rendered data and evaluate relMSE at 1080p")
+    ap.add_argument("--frames", type=int, default=24)
+    ap.add_argument("--train-ref-spp", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--eval-ref-spp", type=int, default=4096)
+    ap.add_argument("--weights-out", default=os.path.join(HERE, "weights", "unet_sunsky.npy"))
+    ap.add_argument("--json-out", default=None)
+    args = ap.parse_args()
+    log = lambda m: print(m, flush=True)
+    inputs, targets, r = training_set(args.frames, ref_spp=args.train_ref_spp, log=log)
+    dn = P.DenoiserHip(0)
+    tr = train(dn, inputs, targets, steps=args.steps, log=log)
+    os.makedirs(os.path.dirname(args.weights_out), exist_ok=True)
+    np.save(args.weights_out, dn.getWeights())
+    ev = evaluate(dn, ref_spp=args.eval_ref_spp, log=log)
+    out = {"training_data": {"frames": args.frames, "ref_spp": args.train_ref_spp, "render_ms": r["ms"], "samples": r["samples"]},
+           "training": tr, "evaluation": ev}
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            json.dump(out, f, indent=1)
+    log(json.dumps(out))
+    dn.close()
+
+
+if __name__ == "__main__":
+    main()
