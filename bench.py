@@ -9,9 +9,10 @@ frame.  Scene generation, upload and the BVH build happen before the timed regio
 build time is reported separately, as Embree's buildMs is).
 
 Output: one JSON line with the metric, the roofline of the dominant kernel (algorithmic bytes
-from the kernels' own counters over the HIP-event time of that kernel) and a CPU baseline: the
-oracle (the project's C++ restatement of the reference path) timed on the host cores on a
-bounded sample of the same frame.
+from the kernels' own counters over the HIP-event time of that kernel), a CPU baseline (the
+oracle, the project's C++ restatement of the reference path, timed on the host cores on a
+bounded sample of the same frame) and the time-to-relMSE comparison of plain rendering against
+the learned denoiser on the same frame ("relmse"; --no-relmse skips it).
 """
 import argparse
 import json
@@ -55,6 +56,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--small", action="store_true", help="tiny scene, for profiling smoke runs only")
+    p.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE comparison (plain vs denoised)")
+    p.add_argument("--relmse-ref-spp", type=int, default=2048, help="samples of the relMSE reference render")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="before the timed region, run one device-to-device copy of this many MiB (a copy "
                         "kernel of known traffic that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -85,6 +88,33 @@ def cpu_baseline(scene, target_s):
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"rows {rows[0]}-{rows[1]} of the {W}x{H} frame x {spp} spp ({samples} samples, "
                       f"{dt:.1f} s, {rays / dt / 1e6:.2f} Mray/s); CPU BVH build {build_s:.1f} s not timed"}
+
+
+def relmse_leg(ref_spp):
+    """Second half of the metric: relMSE against a ref_spp render of the same frame (a different
+    seed) for plain rendering at 4..1024 samples and for the learned denoiser (committed weights,
+    trained on 4-sample / 4096-sample pairs the batch renderer produced) on 4, 8 and 16 samples,
+    with the render (+ denoise) time of each point and the time to relMSE 1e-3 where reached."""
+    import protoray_hip as P
+    import denoise_train as T
+    dn = P.DenoiserHip(0)
+    dn.setWeights(np.load(os.path.join(ROOT, "oidn-protoray_amd", "weights", "unet_sunsky.npy")))
+    ev = T.evaluate(dn, ref_spp=ref_spp, log=lambda m: print(m, file=sys.stderr, flush=True))
+    dn.close()
+    best_dn = min(ev["denoised"], key=lambda r: r["relmse"])
+    first_dn = ev["denoised"][0]
+    return {
+        "definition": "mean over pixels and channels of (x - y)^2 / (y^2 + 1e-2), y = reference",
+        "reference_spp": ref_spp,
+        "ms_per_spp": round(ev["ms_per_spp"], 3),
+        "plain": [{"spp": r["spp"], "ms": round(r["ms"], 1), "relmse": round(r["relmse"], 5)} for r in ev["plain"]],
+        "denoised": [{"spp": r["spp"], "ms": round(r["ms"], 1), "relmse": round(r["relmse"], 5),
+                      "denoise_kernel_ms": round(r["denoise_kernel_ms"], 2)} for r in ev["denoised"]],
+        "time_to_relmse_1e-3_ms": ev["time_to_relmse_1e-3_ms"],
+        "time_to_denoised_4spp_quality_ms": {"denoised": round(first_dn["ms"], 1),
+                                             "plain": T.time_to(ev["plain"], first_dn["relmse"])},
+        "best_denoised_relmse": round(best_dn["relmse"], 5),
+    }
 
 
 def main():
@@ -181,6 +211,8 @@ def main():
         },
     }
     dev.close()
+    if not args.no_relmse and not args.small:
+        result["relmse"] = relmse_leg(args.relmse_ref_spp)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
         result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)

@@ -146,7 +146,8 @@ def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4
 
 def time_to(points, target):
     """Time at which relMSE reaches target, interpolated in log-log space between the measured
-    points (None when no measured point reaches it, or extrapolated from the last two, marked)."""
+    points; beyond them, extrapolated from the last two (marked) while the curve still falls at
+    least as t^-1/2, else marked not reached."""
     pts = sorted((p["ms"], p["relmse"]) for p in points)
     for (t0, e0), (t1, e1) in zip(pts, pts[1:]):
         if e0 > target >= e1:
@@ -157,7 +158,11 @@ def time_to(points, target):
     if len(pts) >= 2 and pts[-1][1] < pts[-2][1]:
         (t0, e0), (t1, e1) = pts[-2], pts[-1]
         slope = (math.log(e1) - math.log(e0)) / (math.log(t1) - math.log(t0))
-        return {"extrapolated": math.exp(math.log(t1) + (math.log(target) - math.log(e1)) / slope)}
+        # extrapolate only a curve still falling at least as fast as t^-1/2 (Monte Carlo noise
+        # falls as t^-1); a flattening curve (bias, or the reference's own noise) is not extrapolated
+        if slope <= -0.5:
+            return {"extrapolated": math.exp(math.log(t1) + (math.log(target) - math.log(e1)) / slope)}
+        return {"not_reached": "the curve flattens (slope %.2f in log-log)" % slope}
     return None
 
 
