@@ -14,6 +14,15 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
 
 
+def results_dir():
+    """Directory for the JSON / npz records GPU tests leave behind (PRT_RESULTS_DIR), or None:
+    then nothing is written."""
+    d = os.environ.get("PRT_RESULTS_DIR")
+    if d:
+        os.makedirs(d, exist_ok=True)
+    return d
+
+
 @pytest.fixture(scope="session")
 def golden():
     import numpy as np

@@ -18,6 +18,7 @@ import pytest
 import oracle as O
 import protoray_hip as P
 import scenegen as G
+from conftest import results_dir
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -160,9 +161,10 @@ def test_differentiable_rendering_configuration(dev):
                                              "albedo_b": float(rel[:, 2].max()), "alpha": float(rel[:, 3].max())},
            "central_differences": fd_rows}
     print(json.dumps(res, indent=1))
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "adjoint_config.json"), "w") as f:
-        json.dump(res, f, indent=1)
+    out = results_dir()
+    if out:
+        with open(os.path.join(out, "adjoint_config.json"), "w") as f:
+            json.dump(res, f, indent=1)
     assert abs(loss - oloss) <= 1e-4 * oloss
     assert rel.max() < 5e-3
     for r in fd_rows:  # sampling-dependent parameters: the detached gradient agrees in expectation

@@ -13,6 +13,7 @@ import pytest
 import dataset as D
 import protoray_hip as P
 import scenegen as G
+from conftest import results_dir
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -78,9 +79,10 @@ def test_dataset_configuration_batch(tmp_path):
     # with a 64-sample reference instead of 4096 to keep the test short; two frames are written
     s = D.render_dataset(outdir=None, ref_spp=64)
     print(json.dumps(s))
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "batch_config.json"), "w") as f:
-        json.dump(s, f, indent=1)
+    out = results_dir()
+    if out:
+        with open(os.path.join(out, "batch_config.json"), "w") as f:
+            json.dump(s, f, indent=1)
     assert s["samples"] == s["pixels"] * 64
     assert min(s["triangles"]) >= 90_000 and max(s["triangles"]) <= 4_200_000
     D.render_dataset(outdir=str(tmp_path), ref_spp=4, n_frames=2, n_scenes=2, max_tris=200_000, tune=False)

@@ -12,6 +12,7 @@ import pytest
 import oracle as O
 import protoray_hip as P
 import scenegen as G
+from conftest import results_dir
 
 pytestmark = pytest.mark.gpu
 
@@ -132,9 +133,10 @@ def _parity(dev, scene, W, H, spp, min_match):
         ref = acc["_sampleL"]
         bad = same & ~np.isclose(L, ref, rtol=RTOL, atol=ATOL).all(axis=0)
         if bad.any():
-            os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-            np.savez(os.path.join(ROOT, "gpurun_out", f"parity_fail_{scene['name']}_{s}.npz"), L=L, hsh=hsh,
-                     first=first, refL=ref, refHash=acc["_hash"], refFirst=ref_first, bad=np.flatnonzero(bad))
+            out = results_dir()
+            if out:
+                np.savez(os.path.join(out, f"parity_fail_{scene['name']}_{s}.npz"), L=L, hsh=hsh,
+                         first=first, refL=ref, refHash=acc["_hash"], refFirst=ref_first, bad=np.flatnonzero(bad))
             print("mismatched pixels", np.flatnonzero(bad), L[:, bad].T, ref[:, bad].T)
         np.testing.assert_allclose(L[:, same], ref[:, same], rtol=RTOL, atol=ATOL)
     gs, gw = dev.getAccum()
