@@ -256,6 +256,36 @@ int prt_denoiser_train_step(prt_denoiser* dn, const float* input, const float* t
 int prt_denoiser_gradient(prt_denoiser* dn, const float* input, const float* target, int32_t n, int32_t h, int32_t w,
                           double* loss, float* grad, int64_t grad_count);
 
+/* --- scene files ------------------------------------------------------------------------
+ * The loaders of Scene::Scene(path, props) (render/scene.cpp:64-372): .rkscene (JSON + .rkbin,
+ * rkscene/rkscene.h:847-903), .gltf (rkscene/rkscene_gltf.h; also node matrices and data-URI
+ * buffers) and .mesh (TriangleMesh blob, geometry/triangle_mesh.cpp:362-380, sys/blob.h:13-37,
+ * with the .mtl of the same base name, material/mtl_loader.cpp).  The file becomes one
+ * world-space triangle mesh: instanced objects are flattened (each instance's copy
+ * transformed; normals by the inverse transpose), as the reference does for emissive objects
+ * or with its `flatten` prop (render/scene.cpp:343-363).  Material IDs follow the reference:
+ * 0 = environment, emissive materials first, the default material last (.mesh files carry
+ * their IDs).  Standard material parameters map to the records above; a material using
+ * textures, sheen, subsurface, thin walls, opacity, partial metalness/transmission, smooth
+ * (roughness 0) lobes or a coat over a specular layer is mapped to its nearest record and
+ * counted in approximated_materials.  Host only: no device is touched.  prt_scene_file_get
+ * copies the arrays (sizes from the info; normals only when has_normals). */
+typedef struct prt_scene_file prt_scene_file;
+typedef struct prt_scene_file_info {
+  int64_t vertex_count, triangle_count;
+  int32_t material_count;          /* including ID 0 */
+  int32_t has_normals;
+  int32_t has_sun_sky;             /* an rkscene sunSky light: its direction below */
+  float sun_direction[3];
+  int32_t approximated_materials;
+  int32_t instance_count;          /* instances flattened */
+} prt_scene_file_info;
+int prt_scene_file_load(const char* path, prt_scene_file** out, prt_scene_file_info* info);
+int prt_scene_file_get(prt_scene_file* f, float* positions, float* normals, int32_t* indices, int32_t* tri_material,
+                       prt_material* materials);
+int prt_scene_file_material_name(prt_scene_file* f, int32_t id, char* buf, int32_t len);
+int prt_scene_file_destroy(prt_scene_file* f);
+
 /* --- per-sample debug outputs of the last pass (parity checks) ------------------------- */
 /* When enabled, each pass also stores per pixel: the clamped sample radiance (planar
  * 3 x W*H), a hash of the path's hit / lobe sequence and the first-hit primitive ID. */

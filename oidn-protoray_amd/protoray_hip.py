@@ -94,6 +94,10 @@ SIGNATURES = {
     "prt_debug_adjoint_replay": [_vp, _u, _vp],
     "prt_debug_bvh_stats": [_vp, _vp],
     "prt_debug_kernel_ms": [_vp, _vp, _vp, _vp],
+    "prt_scene_file_load": [ctypes.c_char_p, _vp, _vp],
+    "prt_scene_file_get": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "prt_scene_file_material_name": [_vp, _i, ctypes.c_char_p, _i],
+    "prt_scene_file_destroy": [_vp],
 }
 
 
@@ -302,9 +306,45 @@ class DeviceHip:
                 "intersect_launches": ln[0], "occluded_launches": ln[1]}
 
 
+class SceneFileInfo(ctypes.Structure):  # prt_scene_file_info
+    _fields_ = [("vertex_count", ctypes.c_int64), ("triangle_count", ctypes.c_int64), ("material_count", ctypes.c_int32),
+                ("has_normals", ctypes.c_int32), ("has_sun_sky", ctypes.c_int32), ("sun_direction", ctypes.c_float * 3),
+                ("approximated_materials", ctypes.c_int32), ("instance_count", ctypes.c_int32)]
+
+
+def load_scene_file(path):
+    """Scene::Scene(path) for .rkscene / .gltf / .mesh files (render/scene.cpp:64-372): the
+    flattened world-space mesh, per-triangle material IDs and material records as a scene dict
+    (positions, normals or None, indices, tri_material, materials, material_names, info).
+    Host-side parsing only; lights, camera and renderer settings are the caller's."""
+    L = load_library()
+    h = ctypes.c_void_p()
+    info = SceneFileInfo()
+    _check(L.prt_scene_file_load(os.fsencode(str(path)), ctypes.byref(h), ctypes.byref(info)))
+    try:
+        nv, nt, nm = info.vertex_count, info.triangle_count, info.material_count
+        pos = np.zeros((nv, 3), np.float32)
+        nrm = np.zeros((nv, 3), np.float32) if info.has_normals else None
+        idx = np.zeros((nt, 3), np.int32)
+        tm = np.zeros(nt, np.int32)
+        mats = np.zeros(nm, MATERIAL_DTYPE)
+        _check(L.prt_scene_file_get(h, _ptr(pos), _ptr(nrm), _ptr(idx), _ptr(tm), _ptr(mats)))
+        names = []
+        buf = ctypes.create_string_buffer(1024)
+        for i in range(nm):
+            _check(L.prt_scene_file_material_name(h, i, buf, 1024))
+            names.append(buf.value.decode(errors="replace"))
+    finally:
+        L.prt_scene_file_destroy(h)
+    return {"positions": pos, "normals": nrm, "indices": idx, "tri_material": tm, "materials": mats,
+            "material_names": names,
+            "info": {"approximated_materials": info.approximated_materials, "instance_count": info.instance_count,
+                     "has_sun_sky": bool(info.has_sun_sky), "sun_direction": tuple(info.sun_direction)}}
+
+
 def load_scene(dev, scene):
     """Uploads a scene dict from scenes/scenegen.py and configures renderer, camera and frame."""
-    dev.initScene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"])
+    dev.initScene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"], scene.get("normals"))
     dev.initLights(scene["lights"], scene.get("env"))
     dev.initRenderer(maxDepth=scene["max_depth"])
     dev.initSampler(scene.get("seed", 0))

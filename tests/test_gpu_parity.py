@@ -4,6 +4,7 @@ Bars: integer work (sampler values, closest-hit primitive IDs and barycentrics) 
 per-pixel FP32 radiance within 1e-3 relative (1e-5 absolute floor) on identical paths, i.e.
 samples whose hit / lobe event hash matches the oracle's.
 """
+import math
 import os
 
 import numpy as np
@@ -115,7 +116,7 @@ def _parity(dev, scene, W, H, spp, min_match):
     dev.enableDebug(True)
     P.load_scene(dev, scene)
     dev.initFrame(W, H, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
-    osc = O.Scene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"])
+    osc = O.Scene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"], scene.get("normals"))
     osc.set_lights(scene["lights"], scene.get("env"))
     acc = None
     for s in range(spp):
@@ -172,6 +173,53 @@ def test_render_parity_area_lights(dev, ambient):
     # emissive spheres (one TriangleLight over their triangles) with and without an ambient light
     scene = G.spheres_ambient(n_spheres=24, subdiv=2, emissive=6, ambient=ambient)
     _parity(dev, scene, 64, 64, 4, 0.98)
+
+
+def test_render_parity_scene_file(dev, tmp_path):
+    # a glTF file (smooth-normal spheres instanced by a node tree, a ground quad, an emissive quad,
+    # Standard / metal / glass / emissive materials) through prt_scene_file_load, rendered with
+    # interpolated vertex normals, against the oracle on the same arrays
+    import scene_files as S
+    w = S.GltfWriter()
+    w.doc["materials"] = [
+        {"name": "plastic", "pbrMetallicRoughness": {"baseColorFactor": [0.7, 0.3, 0.2, 1], "metallicFactor": 0.0,
+                                                     "roughnessFactor": 0.4}},
+        {"name": "metal", "pbrMetallicRoughness": {"baseColorFactor": [0.9, 0.8, 0.5, 1], "metallicFactor": 1.0,
+                                                   "roughnessFactor": 0.3}},
+        {"name": "glass", "pbrMetallicRoughness": {"metallicFactor": 0.0, "roughnessFactor": 0.15},
+         "extensions": {"KHR_materials_transmission": {"transmissionFactor": 1.0},
+                        "KHR_materials_volume": {"thicknessFactor": 1.0}}},
+        {"name": "lamp", "emissiveFactor": [1.0, 0.9, 0.8],
+         "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 6.0}}},
+        {"name": "floor", "pbrMetallicRoughness": {"baseColorFactor": [0.5, 0.5, 0.5, 1], "metallicFactor": 0.0,
+                                                   "roughnessFactor": 0.8},
+         "extensions": {"KHR_materials_specular": {"specularFactor": 0.0}}},
+    ]
+    sp, si = S.icosphere(2)
+    pa, na, ia = w.accessor(sp, 5126, "VEC3"), w.accessor(sp, 5126, "VEC3"), w.accessor(si.ravel(), 5125, "SCALAR")
+    spheres = [w.mesh([{"attributes": {"POSITION": pa, "NORMAL": na}, "indices": ia, "material": m}]) for m in range(3)]
+    quad = np.array([[-1, 0, -1], [1, 0, -1], [-1, 0, 1], [1, 0, 1]], np.float32)
+    qa, qi = w.accessor(quad, 5126, "VEC3"), w.accessor(np.array([0, 2, 1, 1, 2, 3]), 5125, "SCALAR")
+    floor = w.mesh([{"attributes": {"POSITION": qa}, "indices": qi, "material": 4}])
+    lamp = w.mesh([{"attributes": {"POSITION": qa}, "indices": qi, "material": 3}])
+    rng = np.random.default_rng(3)
+    kids = [w.node(mesh=spheres[i % 3], translation=[float(rng.uniform(-4, 4)), float(rng.uniform(0, 2.5)),
+                                                     float(rng.uniform(6, 14))],
+                   scale=[float(rng.uniform(0.4, 1.0))] * 3) for i in range(12)]
+    w.node(children=kids, root=True, rotation=S.quaternion((0, 1, 0), 0.2))
+    w.node(mesh=floor, root=True, translation=[0, -1, 10], scale=[20, 1, 20])
+    w.node(mesh=lamp, root=True, translation=[0, 6, 10], rotation=S.quaternion((1, 0, 0), math.pi), scale=[2, 1, 2])
+    path = tmp_path / "parity.gltf"
+    w.save(str(path))
+    sc = P.load_scene_file(path)
+    assert sc["normals"] is not None and sc["info"]["instance_count"] == 14
+    lights = np.zeros(2, G.LIGHT_DTYPE)
+    d = np.array([-0.4, 0.8, -0.45])
+    d /= np.linalg.norm(d)
+    lights[0] = (G.LIGHT_AMBIENT, 0.3, 0.35, 0.4, 0, 0, 0, 0)
+    lights[1] = (G.LIGHT_DISTANT, 20.0, 19.0, 17.0, d[0], d[1], d[2], math.radians(2.0))
+    sc.update(name="gltf_file", lights=lights, env=None, camera=G._camera(), max_depth=5, seed=0)
+    _parity(dev, sc, 64, 64, 3, 0.97)
 
 
 def test_render_stats_and_error(dev):

@@ -1,0 +1,314 @@
+"""Scene files (.rkscene, .gltf, .mesh + .mtl) through the library's loader, host side only.
+
+Pinned against the reference's own code where it builds here (oracle/_ref/ref_scene, compiled
+from rkscene/rkscene.h, rkscene/rkscene_gltf.h and geometry/triangle_mesh.cpp):
+* the parsed geometry, instance transforms and material parameters equal what the reference's
+  .rkscene and glTF parsers read from the same files (flattening by x' = M x + t done here);
+* a .mesh blob written by the reference's TriangleMesh stream operator reads back exactly;
+* instance copies (positions, and normals by the inverse transpose) equal
+  TriangleMesh::clone(Affine3f).
+The material numbering (environment 0, emissive first, default last) and the record mapping
+follow render/scene.cpp:200-262 and material/standard.cpp, which do not build here (TBB); they
+are pinned by hand-worked cases.
+"""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import protoray_hip as P
+import scene_files as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_scene")
+needs_ref = pytest.mark.skipif(not os.path.exists(REF), reason="reference scene harness not built (oracle/Makefile.ref)")
+
+
+def _ref(mode, *args):
+    out = subprocess.run([REF, mode] + [str(a) for a in args], check=True, capture_output=True, text=True).stdout
+    return None if mode == "mesh" else json.loads(out)  # the blob writer logs to stdout
+
+
+def _flatten_dump(d):
+    """Flattened positions / normals / indices / per-triangle file material of a reference dump."""
+    pos, nrm, idx, mat = [], [], [], []
+    base = 0
+    for o in d["objects"]:
+        xs = o["instances"] or [None]
+        for x in xs:
+            for gi in o["geometries"]:
+                g = d["geometries"][gi]
+                p = np.array(g["positions"], np.float64).reshape(-1, 3)
+                n = np.array(g["normals"], np.float64).reshape(-1, 3) if g["normals"] else None
+                if x is not None:
+                    c = np.array(x, np.float64).reshape(4, 3)  # columns vx, vy, vz, p
+                    L = c[:3].T
+                    p = p @ L.T + c[3]
+                    if n is not None:
+                        n = n @ np.linalg.inv(L)   # (L^-T n) as rows
+                pos.append(p)
+                nrm.append(n)
+                i = np.array(g["indices"], np.int64).reshape(-1, 3)
+                idx.append(i + base)
+                mat += [g["material"]] * len(i)
+                base += len(p)
+    return np.concatenate(pos), nrm, np.concatenate(idx), np.array(mat)
+
+
+def _sample_rkscene(tmp_path):
+    rng = np.random.default_rng(4)
+    w = S.RkWriter()
+    w.doc["materials"] = [
+        {"name": "red", "baseColor": [0.8, 0.1, 0.1], "specularRoughness": 0.4},
+        {"name": "lamp", "emissionLuminance": 5.0, "emissionColor": [1.0, 0.9, 0.7]},
+        {"name": "gold", "metalness": 1.0, "baseColor": [0.9, 0.6, 0.2], "specularRoughness": 0.2},
+        {"name": "glass", "transmissionWeight": 1.0, "specularIOR": 1.45, "specularRoughness": 0.05},
+        {"name": "tex", "baseColor": {"image": "wood.png", "factor": [0.5, 0.5, 0.5]}},
+    ]
+    sp, si = S.icosphere(1)
+    g0 = w.geometry(sp, si, material=0, normals=sp)
+    g1 = w.geometry([[0, 0, 0], [1, 0, 0], [0, 0, 1], [1, 0, 1]], [[0, 2, 1], [1, 2, 3]], material=1)
+    g2 = w.geometry(sp * 0.5, si, material=2, normals=sp)
+    g3 = w.geometry(rng.random((5, 3)), [[0, 1, 2], [2, 3, 4]])           # no material: the default
+    g4 = w.geometry(sp * 0.3, si, material=3, normals=sp)
+    w.object([g1, g3])                                                     # not instanced
+    w.object([g0], [S.trs_matrix((1, 2, 3), S.quaternion((0, 1, 0), 0.7), (1, 2, 0.5)),
+                    S.trs_matrix((-2, 0, 5), S.quaternion((1, 1, 0), 1.9), (0.7, 0.7, 0.7))])
+    w.object([g2, g4], [S.trs_matrix((0, 1, 0), S.quaternion((0, 0, 1), 0.3), (2, 1, 1))])
+    w.doc["lights"] = [{"type": "sunSky", "direction": [0.2, -0.9, 0.3]}]
+    path = tmp_path / "scene.rkscene"
+    w.save(str(path))
+    return path
+
+
+def test_rkscene_numbering_and_records(tmp_path):
+    sc = P.load_scene_file(_sample_rkscene(tmp_path))
+    # render/scene.cpp:200-262: [env, red, lamp, gold, glass, tex, default] -> emissive first
+    assert sc["material_names"] == ["", "lamp", "red", "gold", "glass", "tex", ""]
+    m = sc["materials"]
+    assert m["kind"][0] == 0
+    lamp, red, gold, glass, tex, dflt = m[1], m[2], m[3], m[4], m[5], m[6]
+    assert np.allclose(lamp["emission"], [5.0, 4.5, 3.5])
+    # the Standard default: a rough dielectric layer (IOR 1.5, weight 1) over the diffuse base
+    assert red["kind"] == 3 and np.allclose([red["r"], red["g"], red["b"]], [0.8, 0.1, 0.1])
+    assert np.isclose(red["ior"], 1.5) and np.isclose(red["roughness"], 0.4) and red["weight"] == 1.0
+    assert gold["kind"] == 1 and np.isclose(gold["roughness"], 0.2) and np.allclose([gold["er"], gold["eg"], gold["eb"]], 1)
+    assert glass["kind"] == 2 and np.isclose(glass["ior"], 1.45) and np.isclose(glass["roughness"], 0.05)
+    assert tex["kind"] == 3 and np.allclose([tex["r"], tex["g"], tex["b"]], 0.5)
+    assert dflt["kind"] == 0 and np.allclose([dflt["r"], dflt["g"], dflt["b"]], 0.8)
+    assert sc["info"]["approximated_materials"] == 1         # the textured one
+    assert sc["info"]["instance_count"] == 3
+    assert sc["info"]["has_sun_sky"] and np.allclose(sc["info"]["sun_direction"], [0.2, -0.9, 0.3])
+    # objects in order: [quad (lamp), random (default)], 2 x sphere (red), sphere (gold) + sphere (glass)
+    tm = sc["tri_material"]
+    nq, ns = 2, 80
+    expect = [1] * nq + [6] * 2 + [2] * (2 * ns) + [3] * ns + [4] * ns
+    assert tm.tolist() == expect
+
+
+@needs_ref
+def test_rkscene_matches_reference_parser(tmp_path):
+    path = _sample_rkscene(tmp_path)
+    d = _ref("dump", path)
+    sc = P.load_scene_file(path)
+    pos, nrm, idx, fmat = _flatten_dump(d)
+    np.testing.assert_allclose(sc["positions"], pos, rtol=1e-6, atol=1e-6)
+    assert np.array_equal(sc["indices"], idx)
+    # one consistent file material -> ID map; names follow it
+    ids = {}
+    for f, i in zip(fmat, sc["tri_material"]):
+        assert ids.setdefault(int(f), int(i)) == int(i)
+    for f, i in ids.items():
+        want = d["materials"][f]["name"] if f >= 0 else ""
+        assert sc["material_names"][i] == want
+    # normals: geometries without normals get zeros in the shared array
+    off = 0
+    got = sc["normals"]
+    k = 0
+    for o in d["objects"]:
+        for _ in (o["instances"] or [None]):
+            for gi in o["geometries"]:
+                nv = len(d["geometries"][gi]["positions"]) // 3
+                ref_n = nrm[k]
+                if ref_n is None:
+                    assert np.all(got[off:off + nv] == 0)
+                else:
+                    np.testing.assert_allclose(got[off:off + nv], ref_n, rtol=1e-5, atol=1e-6)
+                off += nv
+                k += 1
+    # parsed material parameters the records use
+    m = sc["materials"]
+    for f, i in ids.items():
+        if f < 0:
+            continue
+        rm = d["materials"][f]
+        if rm["metalness"] == 1:
+            assert np.allclose([m[i]["r"], m[i]["g"], m[i]["b"]], rm["baseColor"])
+        if rm["emissionLuminance"] > 0:
+            assert np.allclose(m[i]["emission"], np.array(rm["emissionColor"]) * rm["emissionLuminance"])
+    assert np.allclose(d["lights"][0]["direction"], sc["info"]["sun_direction"])
+
+
+def _sample_gltf(tmp_path, data_uri):
+    w = S.GltfWriter()
+    w.doc["materials"] = [
+        {"name": "plastic", "pbrMetallicRoughness": {"baseColorFactor": [0.2, 0.5, 0.8, 1.0], "metallicFactor": 0.0,
+                                                     "roughnessFactor": 0.35}},
+        {"name": "metal", "pbrMetallicRoughness": {"baseColorFactor": [0.95, 0.64, 0.54, 1.0], "metallicFactor": 1.0,
+                                                   "roughnessFactor": 0.25}},
+        {"name": "glass", "pbrMetallicRoughness": {"metallicFactor": 0.0, "roughnessFactor": 0.1},
+         "extensions": {"KHR_materials_transmission": {"transmissionFactor": 1.0}, "KHR_materials_ior": {"ior": 1.33},
+                        "KHR_materials_volume": {"thicknessFactor": 1.0}}},
+        {"name": "light", "emissiveFactor": [1.0, 0.5, 0.25],
+         "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 8.0}}},
+        {"name": "varnish", "pbrMetallicRoughness": {"baseColorFactor": [0.3, 0.2, 0.1, 1.0], "metallicFactor": 0.0},
+         "extensions": {"KHR_materials_clearcoat": {"clearcoatFactor": 1.0, "clearcoatRoughnessFactor": 0.1}}},
+    ]
+    sp, si = S.icosphere(1)
+    pa = w.accessor(sp, 5126, "VEC3", stride=16)          # interleaved stride
+    na = w.accessor(sp, 5126, "VEC3")
+    ia = w.accessor(si.ravel(), 5123, "SCALAR")           # uint16 indices
+    quad = np.array([[-1, 0, -1], [1, 0, -1], [-1, 0, 1], [1, 0, 1]], np.float32)
+    qa = w.accessor(quad, 5126, "VEC3")
+    qi = w.accessor(np.array([0, 2, 1, 1, 2, 3]), 5121, "SCALAR")   # uint8 indices
+    sphere = w.mesh([{"attributes": {"POSITION": pa, "NORMAL": na}, "indices": ia, "material": 0}])
+    lamp = w.mesh([{"attributes": {"POSITION": qa}, "indices": qi, "material": 3}])
+    two = w.mesh([{"attributes": {"POSITION": pa, "NORMAL": na}, "indices": ia, "material": 1},
+                  {"attributes": {"POSITION": pa}, "indices": ia, "material": 2}])
+    var = w.mesh([{"attributes": {"POSITION": qa}, "indices": qi, "material": 4}])
+    c0 = w.node(mesh=sphere, translation=[0.5, 1.0, 2.0], rotation=S.quaternion((0, 1, 1), 0.4))
+    c1 = w.node(mesh=two, scale=[0.5, 0.5, 0.5], translation=[0, 0.5, 0])
+    w.node(children=[c0, c1], root=True, translation=[1, 0, 0], rotation=S.quaternion((1, 0, 0), 0.3), scale=[1, 2, 1])
+    w.node(mesh=sphere, root=True, translation=[-3, 1, 4])          # the sphere again: instanced
+    w.node(mesh=lamp, root=True, translation=[0, 5, 3])
+    w.node(mesh=var, root=True)
+    path = tmp_path / ("scene_uri.gltf" if data_uri else "scene.gltf")
+    w.save(str(path), data_uri=data_uri)
+    return path
+
+
+@pytest.mark.parametrize("data_uri", [False, True])
+def test_gltf_numbering_and_records(tmp_path, data_uri):
+    sc = P.load_scene_file(_sample_gltf(tmp_path, data_uri))
+    assert sc["material_names"] == ["", "light", "metal", "glass", "plastic", "varnish", ""]
+    m = sc["materials"]
+    light, metal, glass, plastic, varnish = m[1], m[2], m[3], m[4], m[5]
+    assert np.allclose(light["emission"], [8.0, 4.0, 2.0])
+    assert metal["kind"] == 1 and np.allclose([metal["r"], metal["g"], metal["b"]], [0.95, 0.64, 0.54])
+    assert glass["kind"] == 2 and np.isclose(glass["ior"], 1.33) and np.isclose(glass["roughness"], 0.1)
+    assert plastic["kind"] == 3 and np.isclose(plastic["roughness"], 0.35) and np.isclose(plastic["ior"], 1.5)
+    # KHR_materials_clearcoat over a specular layer: mapped to the specular layer, counted
+    assert varnish["kind"] == 3
+    assert sc["info"]["approximated_materials"] >= 1
+    assert sc["info"]["instance_count"] == 5
+    assert sc["positions"].shape[0] == 42 * 4 + 4 * 2
+
+
+@needs_ref
+@pytest.mark.parametrize("data_uri", [False, True])
+def test_gltf_matches_reference_parser(tmp_path, data_uri):
+    # the reference reads external buffers only: a data-URI copy of the same document is
+    # compared with the reference's reading of the external-buffer file
+    d = _ref("dump", _sample_gltf(tmp_path, False))
+    sc = P.load_scene_file(_sample_gltf(tmp_path, data_uri))
+    pos, nrm, idx, fmat = _flatten_dump(d)
+    # the reference converts meshes to objects in the order the node walk meets them and drops
+    # the identity instance of objects used once; the walk here is depth first as well
+    np.testing.assert_allclose(sc["positions"], pos, rtol=1e-5, atol=2e-6)
+    assert np.array_equal(sc["indices"], idx)
+    ids = {}
+    for f, i in zip(fmat, sc["tri_material"]):
+        assert ids.setdefault(int(f), int(i)) == int(i)
+    for f, i in ids.items():
+        assert sc["material_names"][i] == d["materials"][f]["name"]
+    rm = {x["name"]: x for x in d["materials"]}
+    assert math.isclose(rm["glass"]["specularIor"], 1.33, rel_tol=1e-6)
+    assert rm["glass"]["thinWalled"] == 0 and rm["plastic"]["thinWalled"] == 1
+
+
+def _mesh_case(rng):
+    nv, nt = 40, 60
+    pos = rng.standard_normal((nv, 3)).astype(np.float32)
+    nrm = rng.standard_normal((nv, 3)).astype(np.float32)
+    idx = rng.integers(0, nv, (nt, 3)).astype(np.int32)
+    mat = rng.integers(0, 4, nt).astype(np.int32)
+    names = ["", "Lamp", "Paint", "Unknown"]
+    return pos, nrm, idx, mat, names
+
+
+def _mtl(path):
+    with open(path, "w") as f:
+        f.write("# materials\nnewmtl Lamp\ntype Luminous\ncolor 4 3 2\n\n"
+                "newmtl Paint\ntype Standard\nbaseColor 0.1 0.6 0.2\nspecularRoughness 0.25\nmetalness 0\n")
+
+
+@needs_ref
+def test_mesh_blob_written_by_reference(tmp_path):
+    pos, nrm, idx, mat, names = _mesh_case(np.random.default_rng(8))
+    desc = tmp_path / "m.txt"
+    S.write_mesh_desc(desc, pos, idx, mat, names, nrm)
+    _ref("mesh", desc, tmp_path / "scene.mesh")
+    _mtl(tmp_path / "scene.mtl")
+    sc = P.load_scene_file(tmp_path / "scene.mesh")
+    assert np.array_equal(sc["positions"], pos) and np.array_equal(sc["normals"], nrm)
+    assert np.array_equal(sc["indices"], idx) and np.array_equal(sc["tri_material"], mat)
+    assert sc["material_names"] == names
+
+
+def test_mesh_blob_and_mtl(tmp_path):
+    pos, nrm, idx, mat, names = _mesh_case(np.random.default_rng(9))
+    S.write_mesh_blob(tmp_path / "scene.mesh", pos, idx, mat, names, nrm)
+    _mtl(tmp_path / "scene.mtl")
+    sc = P.load_scene_file(tmp_path / "scene.mesh")
+    assert np.array_equal(sc["positions"], pos) and np.array_equal(sc["tri_material"], mat)
+    m = sc["materials"]
+    assert np.allclose(m[1]["emission"], [4, 3, 2])                               # Luminous
+    assert m[2]["kind"] == 3 and np.allclose([m[2]["r"], m[2]["g"], m[2]["b"]], [0.1, 0.6, 0.2])
+    assert m[3]["kind"] == 0 and np.allclose([m[3]["r"], m[3]["g"], m[3]["b"]], 0.8)  # not in the .mtl: default
+    # without normals and without names: one default material
+    S.write_mesh_blob(tmp_path / "bare.mesh", pos, idx, np.zeros_like(mat), [])
+    sc = P.load_scene_file(tmp_path / "bare.mesh")
+    assert sc["normals"] is None and len(sc["materials"]) == 1
+
+
+@needs_ref
+def test_instance_copy_matches_reference_clone(tmp_path):
+    rng = np.random.default_rng(5)
+    sp, si = S.icosphere(1)
+    nrm = (sp + 0.1 * rng.standard_normal(sp.shape)).astype(np.float32)
+    M = S.trs_matrix((0.3, -1, 2), S.quaternion((1, 2, 3), 1.1), (0.5, 2.0, 1.5))
+    cols = np.concatenate([M[:3, 0], M[:3, 1], M[:3, 2], M[:3, 3]]).astype(np.float32)
+    desc = tmp_path / "c.txt"
+    S.write_mesh_desc(desc, sp, si.astype(np.int32), np.zeros(len(si), np.int32), [""], nrm)
+    ref = _ref("clone", desc, *["%.9g" % v for v in cols])
+    w = S.RkWriter()
+    g = w.geometry(sp, si, normals=nrm)
+    w.object([g], [M.astype(np.float32).astype(np.float64)])
+    path = tmp_path / "one.rkscene"
+    w.save(str(path))
+    sc = P.load_scene_file(path)
+    np.testing.assert_allclose(sc["positions"], np.array(ref["positions"]).reshape(-1, 3), rtol=1e-6, atol=1e-6)
+    # the reference's inverse uses its own reciprocal: a few ulp apart
+    np.testing.assert_allclose(sc["normals"], np.array(ref["normals"]).reshape(-1, 3), rtol=1e-5, atol=1e-6)
+
+
+def test_errors(tmp_path):
+    with pytest.raises(ValueError, match="unknown scene format"):
+        P.load_scene_file(tmp_path / "x.obj")
+    with pytest.raises(ValueError, match="failed to open"):
+        P.load_scene_file(tmp_path / "missing.rkscene")
+    (tmp_path / "bad.gltf").write_text("{\"asset\": {\"version\": \"2.0\"}, \"buffers\": [}")
+    with pytest.raises(ValueError, match="JSON"):
+        P.load_scene_file(tmp_path / "bad.gltf")
+    (tmp_path / "bad.mesh").write_bytes(b"\x00\x01\x02\x03")
+    with pytest.raises(ValueError, match="blob ID"):
+        P.load_scene_file(tmp_path / "bad.mesh")
+    w = S.RkWriter()
+    w.geometry([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 5]])
+    w.object([0])
+    w.save(str(tmp_path / "oob.rkscene"))
+    with pytest.raises(ValueError, match="out of range"):
+        P.load_scene_file(tmp_path / "oob.rkscene")
