@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--small", action="store_true", help="tiny scene, for profiling smoke runs only")
     p.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE comparison (plain vs denoised)")
-    p.add_argument("--relmse-ref-spp", type=int, default=2048, help="samples of the relMSE reference render")
+    p.add_argument("--relmse-ref-spp", type=int, default=4096, help="samples of the relMSE reference render")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="before the timed region, run one device-to-device copy of this many MiB (a copy "
                         "kernel of known traffic that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -79,7 +79,13 @@ def cpu_baseline(scene, target_s):
     t0 = time.time()
     osc.render(scene["camera"], W, H, spp_start=0, spp_count=1, max_depth=scene["max_depth"], threads=threads, rows=rows)
     t1 = time.time() - t0
-    spp = max(1, min(scene["spp"], int(target_s / max(t1, 1e-3))))
+    # budget in units of one sample over the 32-row band: all of the workload's samples first,
+    # then a taller band (rows in steps of 32 around the middle of the frame)
+    budget = target_s / max(t1, 1e-3)
+    spp = max(1, min(scene["spp"], int(budget)))
+    if budget >= 2 * scene["spp"]:
+        band = min(H, 32 * int(budget // scene["spp"]))
+        rows = (max(0, H // 2 - band // 2), max(0, H // 2 - band // 2) + band)
     t0 = time.time()
     _, rays = osc.render(scene["camera"], W, H, spp_start=0, spp_count=spp, max_depth=scene["max_depth"],
                          threads=threads, rows=rows)

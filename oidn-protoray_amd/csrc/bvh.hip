@@ -6,7 +6,8 @@
 // sorted cluster list (smallest surface area of the merged box, ties to the smaller index),
 // merges mutual nearest neighbours into a new binary node and compacts the list with a prefix
 // sum, until one cluster is left.  The binary tree is then collapsed top-down into 4-wide
-// 128-byte nodes (greedy surface-area opening); subtrees of <= PRT_LEAF_MAX triangles become
+// 128-byte nodes (greedy surface-area opening; the inner children of a node get consecutive
+// node indices); subtrees of <= PRT_LEAF_MAX triangles become
 // leaves whose triangles are written contiguously as (v0, e1, e2) records.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -259,6 +260,28 @@ __device__ __forceinline__ void writeTri(const float* pos, const int* idx, const
   tris[3 * size_t(o) + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.f);
 }
 
+// Writes a 4-wide node: the n child boxes (padded) and references; empty slots get an empty box.
+__device__ void writeNode(prt_bvh4_node* dst, const Box* bb, const int* ref, int n)
+{
+  prt_bvh4_node nd;
+  for (int q = 0; q < 4; ++q)
+  {
+    nd.pad[q] = 0;
+    if (q >= n)
+    {
+      nd.lox[q] = nd.loy[q] = nd.loz[q] = 3.4e38f;
+      nd.hix[q] = nd.hiy[q] = nd.hiz[q] = -3.4e38f;
+      nd.child[q] = -1;  // empty slot: never visited (traversal masks child == -1)
+      continue;
+    }
+    nd.lox[q] = bb[q].lo[0]; nd.hix[q] = bb[q].hi[0];
+    nd.loy[q] = bb[q].lo[1]; nd.hiy[q] = bb[q].hi[1];
+    nd.loz[q] = bb[q].lo[2]; nd.hiz[q] = bb[q].hi[2];
+    nd.child[q] = ref[q];
+  }
+  *dst = nd;
+}
+
 // Top-down collapse of the binary tree into 4-wide nodes, one level of the wide tree per
 // launch.  A frontier entry is (binary node, wide node, first triangle slot of its subtree).
 // Each wide node starts from the two children of its binary node and repeatedly opens the
@@ -297,29 +320,28 @@ __global__ void __launch_bounds__(kBlock) kCollapse(const int4* __restrict__ fro
     codes[best] = childL[c];
     codes[n++] = childR[c];
   }
-  prt_bvh4_node nd;
-  for (int q = 0; q < 4; ++q)
+  // child boxes (padded) and references; inner children get consecutive wide-node indices
+  Box bb[4];
+  int ref[4];
+  int nInner = 0;
+  for (int q = 0; q < n; ++q)
   {
-    nd.pad[q] = 0;
-    if (q >= n)
-    {
-      nd.lox[q] = nd.loy[q] = nd.loz[q] = 3.4e38f;
-      nd.hix[q] = nd.hiy[q] = nd.hiz[q] = -3.4e38f;
-      nd.child[q] = -1;  // empty slot: never visited (traversal masks child == -1)
-      continue;
-    }
-    int c = codes[q];
-    Box bb;
+    const int c = codes[q];
     const float* src = c < 0 ? leafBox + 6 * size_t(~c) : innerBox + 6 * size_t(c);
-    for (int a = 0; a < 3; ++a) { bb.lo[a] = src[a]; bb.hi[a] = src[3 + a]; }
-    padBox(bb);
-    nd.lox[q] = bb.lo[0]; nd.hix[q] = bb.hi[0];
-    nd.loy[q] = bb.lo[1]; nd.hiy[q] = bb.hi[1];
-    nd.loz[q] = bb.lo[2]; nd.hiz[q] = bb.hi[2];
+    for (int a = 0; a < 3; ++a) { bb[q].lo[a] = src[a]; bb[q].hi[a] = src[3 + a]; }
+    padBox(bb[q]);
+    if (openable(c)) ++nInner;
+  }
+  const int base = nInner ? atomicAdd(nodeCounter, nInner) : 0;
+  const int fbase = nInner ? atomicAdd(nextCount, nInner) : 0;
+  int inner = 0;
+  for (int q = 0; q < n; ++q)
+  {
+    int c = codes[q];
     const int cnt = c < 0 ? 1 : nodeCnt[c];
     if (!openable(c))
     {
-      nd.child[q] = leafCode(slot, cnt);
+      ref[q] = leafCode(slot, cnt);
       // gather the subtree's triangles (at most PRT_LEAF_MAX leaves, 2 PRT_LEAF_MAX - 1 nodes)
       int stk[2 * PRT_LEAF_MAX];
       int sp = 0, o = slot;
@@ -333,13 +355,14 @@ __global__ void __launch_bounds__(kBlock) kCollapse(const int4* __restrict__ fro
     }
     else
     {
-      int j = atomicAdd(nodeCounter, 1);
-      next[atomicAdd(nextCount, 1)] = make_int4(c, j, slot, 0);
-      nd.child[q] = j;
+      const int j = base + inner;
+      next[fbase + inner] = make_int4(c, j, slot, 0);
+      ++inner;
+      ref[q] = j;
     }
     slot += cnt;
   }
-  nodes[k] = nd;
+  writeNode(nodes + k, bb, ref, n);
 }
 
 // One-triangle scene: the root is a leaf over triangle slot 0.

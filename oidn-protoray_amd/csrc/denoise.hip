@@ -9,12 +9,13 @@
 // weights, weight gradients and Adam state.  Tensors are NHWC with channels padded to multiples
 // of 16 (one MFMA K step).
 //
-// Convolutions are implicit GEMMs.  Forward and data gradient: M = pixels (64 consecutive pixels
-// of one image row per wave, as 4 MFMA row blocks), N = output channels (16 per MFMA column
-// block), K = 9 taps x input channels; the data gradient is the forward kernel run on the output
+// Convolutions are implicit GEMMs on LDS-staged tiles.  Forward and data gradient (kConvTile):
+// a block computes a 16 x 32 pixel tile for up to 64 output channels, M = output channels,
+// N = pixels, K = 9 taps x input channels; the data gradient is the same kernel on the output
 // gradient with the weights transposed and flipped, and its output is masked by the ReLU of the
-// layer input.  Weight gradient: M = output channels, N = input channels, K = pixels (16 image
-// rows per wave), reduced with FP32 atomics.
+// layer input.  Weight gradient (kWgradTile): M = output channels, N = input channels, K =
+// pixels, fragments transposed on the matrix cores, reduced with FP32 atomics; the bias
+// gradient comes out of the same kernel.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
@@ -65,74 +66,180 @@ struct ConvArgs
   const uint16_t* mask;  // optional: outputs where mask <= 0 are zeroed (mask has y's layout)
 };
 
-// One wave: 64 pixels of one row (4 MFMA row blocks) x 16*NB output channels.
-template <int NB>
-__global__ void __launch_bounds__(kBlock) kConv(ConvArgs a)
+// 8 BF16 values <-> one 16-byte word
+struct Bf8
 {
-  const int lane = threadIdx.x & 63;
-  const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const int groups = a.coutPad / (16 * NB);
-  const int segs = (a.W + 63) / 64;
-  const int total = a.N * a.H * segs * groups;
-  if (wave >= total) return;  // whole waves leave together
-  int t = wave;
-  const int g = t % groups; t /= groups;
-  const int xs = t % segs; t /= segs;
-  const int y = t % a.H;
-  const int n = t / a.H;
-  const int x0 = xs * 64, co0 = g * 16 * NB;
+  uint16_t h[8];
+};
+__device__ inline Bf8 loadBf8(const uint16_t* p)
+{
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  Bf8 r;
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+  {
+    r.h[2 * k] = uint16_t(w[k] & 0xffffu);
+    r.h[2 * k + 1] = uint16_t(w[k] >> 16);
+  }
+  return r;
+}
+__device__ inline void storeBf8(uint16_t* p, const Bf8& r)
+{
+  uint4 v;
+  v.x = uint32_t(r.h[0]) | uint32_t(r.h[1]) << 16;
+  v.y = uint32_t(r.h[2]) | uint32_t(r.h[3]) << 16;
+  v.z = uint32_t(r.h[4]) | uint32_t(r.h[5]) << 16;
+  v.w = uint32_t(r.h[6]) | uint32_t(r.h[7]) << 16;
+  *reinterpret_cast<uint4*>(p) = v;
+}
+
+// Tiled convolution.  A block (8 waves) computes a 16 x 32 pixel tile of one image for 16*NB
+// output channels.  Per 16-channel slice of the input, the haloed input tile (18 x 34 pixels)
+// and the weights of that slice (9 taps x 16*NB outputs) are staged in LDS (40-byte slots:
+// 16 channels + 4 padding, so the 16 lanes of an MFMA operand hit distinct banks); each wave
+// computes two image rows (4 MFMA pixel blocks of 16) for all output blocks.  The MFMA takes the
+// weights as the row operand and the pixels as the column operand, so each lane ends up with 4
+// consecutive output channels of one pixel and stores them as one 8-byte word.
+constexpr int kCtY = 16, kCtX = 32, kCtIY = kCtY + 2, kCtIX = kCtX + 2;
+constexpr int kCtThreads = 512;
+constexpr int kSlot = 20;
+
+template <int NB>
+__global__ void __launch_bounds__(kCtThreads) kConvTile(ConvArgs a)
+{
+  __shared__ __align__(16) uint16_t xs[kCtIY * kCtIX * kSlot];
+  __shared__ __align__(16) uint16_t ws[9 * 16 * NB * kSlot];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tilesX = (a.W + kCtX - 1) / kCtX, tilesY = (a.H + kCtY - 1) / kCtY;
+  int t = blockIdx.x;
+  const int tx = t % tilesX; t /= tilesX;
+  const int ty = t % tilesY;
+  const int n = t / tilesY;
+  const int co0 = blockIdx.y * 16 * NB;
+  const int nb = min(NB, (a.coutPad - co0) / 16);
+  const int x0 = tx * kCtX, y0 = ty * kCtY;
   const int li = lane & 15, lk = (lane >> 4) * 4;
   f32x4 acc[4][NB];
 #pragma unroll
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[p][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const bf16x4 zero = {0, 0, 0, 0};
-  for (int tap = 0; tap < 9; ++tap)
-  {
-    const int yy = y + tap / 3 - 1, dx = tap % 3 - 1;
-    const bool rowOk = yy >= 0 && yy < a.H;
-    const uint16_t* xrow = a.x + (size_t(n) * a.H + (rowOk ? yy : 0)) * a.W * size_t(a.xC);
-    const uint16_t* wt = a.w + size_t(tap) * a.coutPad * a.cinPad;
-    for (int c0 = 0; c0 < a.cinPad; c0 += 16)
+  // Staging: each thread owns fixed slots of the input tile and of the weight slice; the next
+  // slice's global loads are issued before the current slice's MFMAs, so their latency is hidden.
+  constexpr int kXItems = (kCtIY * kCtIX * 2 + kCtThreads - 1) / kCtThreads;
+  constexpr int kWItems = (9 * 16 * NB * 2 + kCtThreads - 1) / kCtThreads;
+  uint4 xr[kXItems], wr[kWItems];
+  auto fetch = [&](int c0) {
+#pragma unroll
+    for (int k = 0; k < kXItems; ++k)
     {
-      bf16x4 A[4], B[NB];
+      const int i = tid + k * kCtThreads, half = i & 1, pix = i >> 1;
+      const int yy = y0 + pix / kCtIX - 1, xx = x0 + pix % kCtIX - 1;
+      xr[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (i < kCtIY * kCtIX * 2 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+        xr[k] = *reinterpret_cast<const uint4*>(a.x + ((size_t(n) * a.H + yy) * a.W + xx) * a.xC + c0 + 8 * half);
+    }
+#pragma unroll
+    for (int k = 0; k < kWItems; ++k)
+    {
+      const int i = tid + k * kCtThreads, half = i & 1, r = i >> 1;
+      const int co = co0 + r % (16 * NB), tap = r / (16 * NB);
+      wr[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (i < 9 * 16 * NB * 2 && co < a.coutPad)
+        wr[k] = *reinterpret_cast<const uint4*>(a.w + (size_t(tap) * a.coutPad + co) * a.cinPad + c0 + 8 * half);
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int k = 0; k < kXItems; ++k)
+    {
+      const int i = tid + k * kCtThreads;
+      if (i < kCtIY * kCtIX * 2)
+      {
+        uint2* d = reinterpret_cast<uint2*>(xs + (i >> 1) * kSlot + 8 * (i & 1));
+        d[0] = make_uint2(xr[k].x, xr[k].y);
+        d[1] = make_uint2(xr[k].z, xr[k].w);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kWItems; ++k)
+    {
+      const int i = tid + k * kCtThreads;
+      if (i < 9 * 16 * NB * 2)
+      {
+        uint2* d = reinterpret_cast<uint2*>(ws + (i >> 1) * kSlot + 8 * (i & 1));
+        d[0] = make_uint2(wr[k].x, wr[k].y);
+        d[1] = make_uint2(wr[k].z, wr[k].w);
+      }
+    }
+  };
+  fetch(0);
+  for (int c0 = 0; c0 < a.cinPad; c0 += 16)
+  {
+    __syncthreads();  // the previous slice's readers are done
+    stage();
+    __syncthreads();
+    if (c0 + 16 < a.cinPad) fetch(c0 + 16);
+#pragma unroll 3
+    for (int tap = 0; tap < 9; ++tap)
+    {
+      const int oy = tap / 3, ox = tap % 3;
+      bf16x4 B[4];
 #pragma unroll
       for (int p = 0; p < 4; ++p)
       {
-        const int xx = x0 + 16 * p + li + dx;
-        A[p] = (rowOk && xx >= 0 && xx < a.W) ? *reinterpret_cast<const bf16x4*>(xrow + size_t(xx) * a.xC + c0 + lk) : zero;
+        const int ly = 2 * wave + (p >> 1) + oy, lx = 16 * (p & 1) + li + ox;
+        B[p] = *reinterpret_cast<const bf16x4*>(xs + (ly * kCtIX + lx) * kSlot + lk);
       }
 #pragma unroll
       for (int b = 0; b < NB; ++b)
-        B[b] = *reinterpret_cast<const bf16x4*>(wt + size_t(co0 + 16 * b + li) * a.cinPad + c0 + lk);
+        if (b < nb)
+        {
+          const bf16x4 A = *reinterpret_cast<const bf16x4*>(ws + ((tap * NB + b) * 16 + li) * kSlot + lk);
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc[p][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[p], B[b], acc[p][b], 0, 0, 0);
+          for (int p = 0; p < 4; ++p) acc[p][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B[p], acc[p][b], 0, 0, 0);
+        }
     }
   }
-  // D of block (p, b): lane holds output channel co0 + 16 b + li for pixels x0 + 16 p + lk + j
+  // D of (p, b): lane holds pixel li of pixel block p, output channels co0 + 16 b + lk + j
 #pragma unroll
-  for (int b = 0; b < NB; ++b)
+  for (int p = 0; p < 4; ++p)
   {
-    const int co = co0 + 16 * b + li;
-    const float bias = (a.bias && co < a.coutReal) ? a.bias[co] : 0.f;
+    const int y = y0 + 2 * wave + (p >> 1), x = x0 + 16 * (p & 1) + li;
+    if (y >= a.H || x >= a.W) continue;
+    const size_t pix = (size_t(n) * a.H + y) * a.W + x;
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+    for (int b = 0; b < NB; ++b)
+    {
+      if (b >= nb) continue;
+      const int co = co0 + 16 * b + lk;
+      float v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
       {
-        const int x = x0 + 16 * p + lk + j;
-        if (x >= a.W || co >= a.coutPad) continue;
-        const size_t pix = (size_t(n) * a.H + y) * a.W + x;
-        float v = acc[p][b][j] + bias;
-        if (a.relu) v = fmaxf(v, 0.f);
-        if (co >= a.coutReal) v = 0.f;
-        if (a.mask && !(fromBf16(a.mask[pix * a.yC + a.yOff + co]) > 0.f)) v = 0.f;
-        if (a.y) a.y[pix * a.yC + a.yOff + co] = toBf16(v);
-        if (a.yf && co < a.coutReal) a.yf[pix * a.coutReal + co] = v;
+        float u = acc[p][b][j];
+        if (a.bias && co + j < a.coutReal) u += a.bias[co + j];
+        if (a.relu) u = fmaxf(u, 0.f);
+        v[j] = co + j < a.coutReal ? u : 0.f;
       }
+      const size_t o = pix * a.yC + a.yOff + co;
+      if (a.mask)
+      {
+        const uint2 m = *reinterpret_cast<const uint2*>(a.mask + o);
+        const uint32_t mw[2] = {m.x, m.y};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (!(fromBf16(uint16_t(mw[j >> 1] >> (16 * (j & 1)))) > 0.f)) v[j] = 0.f;
+      }
+      if (a.y)
+        *reinterpret_cast<uint2*>(a.y + o) = make_uint2(uint32_t(toBf16(v[0])) | uint32_t(toBf16(v[1])) << 16,
+                                                         uint32_t(toBf16(v[2])) | uint32_t(toBf16(v[3])) << 16);
+      if (a.yf)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (co + j < a.coutReal) a.yf[pix * a.coutReal + co + j] = v[j];
+    }
   }
 }
 
@@ -141,170 +248,335 @@ struct WgradArgs
   const uint16_t* dy;  // [pixels][coutPad] (masked output gradient)
   const uint16_t* x;   // [pixels][cinPad] layer input
   int N, H, W, cinPad, coutPad, rows;
-  float* dw;  // [9][coutPad][cinPad] FP32, accumulated
+  float* part;   // per (image, row chunk) partial sums: [N * chunks][9][coutPad][cinPad]
+  float* partB;  // per (image, row chunk) partial bias sums: [N * chunks][coutPad] (null: no bias)
 };
 
-// One wave: one tap, 32 output x 32 input channels (2 x 2 MFMA blocks, or fewer), `rows` image
-// rows of one image; K runs over the pixels 16 at a time.
-__global__ void __launch_bounds__(kBlock) kWgrad(WgradArgs a)
+// 16 x 16 BF16 fragment transpose on the matrix cores: with the identity as the column operand,
+// D = A, and D's lane layout (column = lane % 16, rows 4 (lane / 16) + j) is the transpose of the
+// row operand's (row = lane % 16, k = 4 (lane / 16) + j).  Exact: one product x * 1 per output.
+__device__ inline bf16x4 transposeFrag(const bf16x4 a, const bf16x4 ident)
 {
-  const int lane = threadIdx.x & 63;
-  const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const int mBlocks = (a.coutPad + 31) / 32, nBlocks = (a.cinPad + 31) / 32;
+  const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ident, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  bf16x4 r;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) r[j] = short(__float_as_uint(d[j]) >> 16);
+  return r;
+}
+
+// Weight gradient dW[tap][co][ci] = sum over pixels p of dy[p][co] * x[p + tap offset][ci].
+// A block (6 waves) owns 64 output x 32 input channels of all 9 taps and `rows` image rows of
+// one image.  Per 64-pixel row segment it copies dy (64 pixels x 64 channels) and the three
+// input rows the taps read (66 pixels x 32 channels each) into LDS in their NHWC layout (padded
+// pixel strides, so the 16 pixels of a fragment hit distinct banks); the next segment's global
+// loads are issued before the current segment's MFMAs.  The MFMA wants K = pixels in both
+// operands: each 16 x 16 fragment is read with channels contiguous and transposed on the matrix
+// cores (transposeFrag).  Wave w computes the three taps of kernel row w % 3 for the input
+// channels 16 (w / 3) .. +15 of the group; the blocks of the first input-channel group also sum
+// dy into the bias gradient (a product with ones).  Pixels beyond the image width have dy = 0.
+// Each block stores its sums in its own slice of a partial buffer (no atomics: thousands of
+// blocks would contend for the same few thousand weights); kWgradReduce adds the slices.
+constexpr int kWgThreads = 384, kWgSeg = 64;
+constexpr int kWgDyStride = 68, kWgXStride = 36;  // LDS pixel strides in BF16 elements
+constexpr int kWgDyItems = (kWgSeg * 8 + kWgThreads - 1) / kWgThreads;
+constexpr int kWgXItems = (3 * (kWgSeg + 2) * 4 + kWgThreads - 1) / kWgThreads;
+
+__global__ void __launch_bounds__(kWgThreads) __attribute__((amdgpu_waves_per_eu(3))) kWgradTile(WgradArgs a)
+{
+  __shared__ __align__(16) uint16_t dyS[kWgSeg * kWgDyStride];
+  __shared__ __align__(16) uint16_t xS[3 * (kWgSeg + 2) * kWgXStride];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int krow = wave % 3, half = wave / 3;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  const int coGroups = (a.coutPad + 63) / 64, ciGroups = (a.cinPad + 31) / 32;
   const int chunks = (a.H + a.rows - 1) / a.rows;
-  const int total = 9 * mBlocks * nBlocks * a.N * chunks;
-  if (wave >= total) return;
-  int t = wave;
-  const int nb = t % nBlocks; t /= nBlocks;
-  const int mb = t % mBlocks; t /= mBlocks;
-  const int tap = t % 9; t /= 9;
+  int t = blockIdx.x;
+  const int cg = t % coGroups; t /= coGroups;
+  const int ig = t % ciGroups; t /= ciGroups;
   const int ch = t % chunks;
   const int n = t / chunks;
-  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-  const int li = lane & 15, lk = (lane >> 4) * 4;
-  f32x4 acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int y0 = ch * a.rows, y1 = min(a.H, y0 + a.rows);
-  for (int y = y0; y < y1; ++y)
+  const int co0 = cg * 64, ci0 = ig * 32;
+  const int ncb = min(4, (a.coutPad - co0) / 16);
+  const bool active = ci0 + 16 * half < a.cinPad;  // this wave's 16 input channels exist
+  const bool doBias = a.partB && ig == 0 && wave == 0;
+  bf16x4 ident, ones;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
   {
-    const int yy = y + dy;
-    const bool rowOk = yy >= 0 && yy < a.H;
-    const size_t rowOut = (size_t(n) * a.H + y) * a.W;
-    const size_t rowIn = (size_t(n) * a.H + (rowOk ? yy : 0)) * a.W;
-    for (int xb = 0; xb < a.W; xb += 16)
+    ident[j] = short(lk + j == li ? 0x3F80 : 0);
+    ones[j] = short(0x3F80);
+  }
+  f32x4 acc[3][4], accB[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+  {
+    accB[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 3; ++s) acc[s][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  const int yBeg = ch * a.rows, yEnd = min(a.H, yBeg + a.rows);
+  const int segsPerRow = (a.W + kWgSeg - 1) / kWgSeg;
+  const int nSeg = (yEnd - yBeg) * segsPerRow;
+  uint4 dr[kWgDyItems], xr[kWgXItems];
+  auto fetch = [&](int sIdx) {
+    const int y = yBeg + sIdx / segsPerRow, x0 = (sIdx % segsPerRow) * kWgSeg;
+#pragma unroll
+    for (int k = 0; k < kWgDyItems; ++k)
     {
-      bf16x4 A[2], B[2];
-      for (int i = 0; i < 2; ++i)
+      const int i = tid + k * kWgThreads, px = i >> 3, oct = i & 7, x = x0 + px;
+      dr[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (i < kWgSeg * 8 && x < a.W && co0 + 8 * oct < a.coutPad)
+        dr[k] = *reinterpret_cast<const uint4*>(a.dy + ((size_t(n) * a.H + y) * a.W + x) * a.coutPad + co0 + 8 * oct);
+    }
+#pragma unroll
+    for (int k = 0; k < kWgXItems; ++k)
+    {
+      const int i = tid + k * kWgThreads, oct = i & 3, r = i >> 2;
+      const int row = r / (kWgSeg + 2), q = r % (kWgSeg + 2);
+      const int yy = y + row - 1, xx = x0 - 1 + q;
+      xr[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (i < 3 * (kWgSeg + 2) * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ci0 + 8 * oct < a.cinPad)
+        xr[k] = *reinterpret_cast<const uint4*>(a.x + ((size_t(n) * a.H + yy) * a.W + xx) * a.cinPad + ci0 + 8 * oct);
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int k = 0; k < kWgDyItems; ++k)
+    {
+      const int i = tid + k * kWgThreads;
+      if (i < kWgSeg * 8)
       {
-        const int co = mb * 32 + 16 * i + li;
-        for (int j = 0; j < 4; ++j)
-        {
-          const int x = xb + lk + j;
-          A[i][j] = (co < a.coutPad && x < a.W) ? short(a.dy[(rowOut + x) * a.coutPad + co]) : short(0);
-        }
-        const int ci = nb * 32 + 16 * i + li;
-        for (int j = 0; j < 4; ++j)
-        {
-          const int xx = xb + lk + j + dx;
-          B[i][j] = (ci < a.cinPad && rowOk && xx >= 0 && xx < a.W && xb + lk + j < a.W)
-                        ? short(a.x[(rowIn + xx) * a.cinPad + ci]) : short(0);
-        }
+        uint2* d = reinterpret_cast<uint2*>(dyS + (i >> 3) * kWgDyStride + 8 * (i & 7));
+        d[0] = make_uint2(dr[k].x, dr[k].y);
+        d[1] = make_uint2(dr[k].z, dr[k].w);
       }
-      for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[i], B[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < kWgXItems; ++k)
+    {
+      const int i = tid + k * kWgThreads;
+      if (i < 3 * (kWgSeg + 2) * 4)
+      {
+        uint2* d = reinterpret_cast<uint2*>(xS + (i >> 2) * kWgXStride + 8 * (i & 3));
+        d[0] = make_uint2(xr[k].x, xr[k].y);
+        d[1] = make_uint2(xr[k].z, xr[k].w);
+      }
+    }
+  };
+  if (nSeg > 0) fetch(0);
+  for (int sIdx = 0; sIdx < nSeg; ++sIdx)
+  {
+    __syncthreads();
+    stage();
+    __syncthreads();
+    if (sIdx + 1 < nSeg) fetch(sIdx + 1);
+    if (!active) continue;
+    for (int k0 = 0; k0 < kWgSeg; k0 += 16)
+    {
+      bf16x4 A[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c < ncb)
+        {
+          A[c] = transposeFrag(*reinterpret_cast<const bf16x4*>(dyS + (k0 + li) * kWgDyStride + 16 * c + lk), ident);
+          if (doBias) accB[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[c], ones, accB[c], 0, 0, 0);
+        }
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+      {
+        const bf16x4 B = transposeFrag(
+            *reinterpret_cast<const bf16x4*>(xS + (krow * (kWgSeg + 2) + k0 + li + s) * kWgXStride + 16 * half + lk), ident);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (c < ncb) acc[s][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[c], B, acc[s][c], 0, 0, 0);
+      }
     }
   }
-  // D of block (i, j): lane holds input channel nb*32 + 16 j + li for output channels mb*32 + 16 i + lk + r
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
+  // D of (s, c): lane holds input channel ci0 + 16 half + li for output channels co0 + 16 c + lk + r
+  const size_t pIdx = size_t(n) * chunks + ch;
+  if (active)
+  {
+    float* dst = a.part + pIdx * (size_t(9) * a.coutPad * a.cinPad);
+    const int ci = ci0 + 16 * half + li;
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
     {
-      const int ci = nb * 32 + 16 * j + li;
-      for (int r = 0; r < 4; ++r)
+      const int tap = krow * 3 + s;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
       {
-        const int co = mb * 32 + 16 * i + lk + r;
-        if (co < a.coutPad && ci < a.cinPad && acc[i][j][r] != 0.f)
-          atomicAdd(&a.dw[(size_t(tap) * a.coutPad + co) * a.cinPad + ci], acc[i][j][r]);
+        if (c >= ncb) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          const int co = co0 + 16 * c + lk + r;
+          dst[(size_t(tap) * a.coutPad + co) * a.cinPad + ci] = acc[s][c][r];
+        }
       }
     }
+  }
+  // bias: every column of accB holds the sum over pixels; column 0 (lanes 0, 16, 32, 48) stores it
+  if (doBias && li == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < ncb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a.partB[pIdx * a.coutPad + co0 + 16 * c + lk + r] = accB[c][r];
 }
 
-// Bias gradient: per output channel, the sum of the (masked) output gradient over the pixels.
-__global__ void __launch_bounds__(kBlock) kBiasGrad(const uint16_t* __restrict__ dy, int npix, int cPad, int cReal,
-                                                    float* __restrict__ db)
+// Sum of the per-block partial weight (and bias) gradients.  A block takes 32 consecutive
+// entries (weights, then biases); its 8 thread rows each sum every 8th partial, and the rows are
+// added in LDS in a fixed order.
+constexpr int kRedEl = 32, kRedRows = kBlock / kRedEl;
+
+__global__ void __launch_bounds__(kBlock) kWgradReduce(const float* __restrict__ part, const float* __restrict__ partB, int nPart,
+                                                       size_t size, int coutPad, int coutReal, float* __restrict__ dw,
+                                                       float* __restrict__ db)
 {
-  const int chunk = 1024;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  const int c = t % cReal, k = t / cReal;
-  const int p0 = k * chunk;
-  if (p0 >= npix) return;
-  float s = 0.f;
-  for (int p = p0; p < min(npix, p0 + chunk); ++p) s += fromBf16(dy[size_t(p) * cPad + c]);
-  atomicAdd(&db[c], s);
+  __shared__ float red[kRedRows][kRedEl];
+  const int el = threadIdx.x % kRedEl, row = threadIdx.x / kRedEl;
+  const size_t e = size_t(blockIdx.x) * kRedEl + el;
+  const bool isW = e < size, isB = !isW && partB && e < size + size_t(coutReal);
+  float t = 0.f;
+  if (isW)
+  {
+#pragma unroll 4
+    for (int p = row; p < nPart; p += kRedRows) t += part[size_t(p) * size + e];
+  }
+  else if (isB)
+  {
+    const int co = int(e - size);
+#pragma unroll 4
+    for (int p = row; p < nPart; p += kRedRows) t += partB[size_t(p) * coutPad + co];
+  }
+  red[row][el] = t;
+  __syncthreads();
+  if (row == 0 && (isW || isB))
+  {
+    float s = 0.f;
+    for (int r = 0; r < kRedRows; ++r) s += red[r][el];
+    if (isW) dw[e] = s;
+    else db[e - size] = s;
+  }
 }
 
-// 2x2 max pooling (NHWC, H and W even).
+// 2x2 max pooling (NHWC, H and W even); a thread takes 8 channels of one output pixel.
 __global__ void __launch_bounds__(kBlock) kPool(const uint16_t* __restrict__ x, int N, int H, int W, int C, uint16_t* __restrict__ y)
 {
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  const int Ho = H / 2, Wo = W / 2;
-  if (i >= size_t(N) * Ho * Wo * C) return;
-  const int c = int(i % C);
-  size_t p = i / C;
+  const int Ho = H / 2, Wo = W / 2, octs = C / 8;
+  if (i >= size_t(N) * Ho * Wo * octs) return;
+  const int c = int(i % octs) * 8;
+  size_t p = i / octs;
   const int xo = int(p % Wo); p /= Wo;
   const int yo = int(p % Ho);
   const int n = int(p / Ho);
-  auto at = [&](int yy, int xx) { return fromBf16(x[((size_t(n) * H + yy) * W + xx) * C + c]); };
-  float m = fmaxf(fmaxf(at(2 * yo, 2 * xo), at(2 * yo, 2 * xo + 1)), fmaxf(at(2 * yo + 1, 2 * xo), at(2 * yo + 1, 2 * xo + 1)));
-  y[i] = toBf16(m);
+  Bf8 m = loadBf8(x + ((size_t(n) * H + 2 * yo) * W + 2 * xo) * C + c);
+  for (int k = 1; k < 4; ++k)
+  {
+    const Bf8 v = loadBf8(x + ((size_t(n) * H + 2 * yo + k / 2) * W + 2 * xo + k % 2) * C + c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (fromBf16(v.h[j]) > fromBf16(m.h[j])) m.h[j] = v.h[j];
+  }
+  storeBf8(y + i * 8, m);
 }
 
 // Encoder level backward: de = (pool gradient routed to the first maximum of each 2x2 window
-// + the skip slice of the decoder's concat gradient), masked by ReLU(e) > 0.
+// + the skip slice of the decoder's concat gradient), masked by ReLU(e) > 0.  A thread takes
+// 8 channels of one 2x2 window.
 __global__ void __launch_bounds__(kBlock) kPoolBwd(const uint16_t* __restrict__ e, const uint16_t* __restrict__ dp,
                                                    const uint16_t* __restrict__ dcat, int catC, int catOff, int N, int H, int W, int C,
                                                    uint16_t* __restrict__ de)
 {
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= size_t(N) * H * W * C) return;
-  const int c = int(i % C);
-  size_t p = i / C;
-  const int x = int(p % W); p /= W;
-  const int y = int(p % H);
-  const int n = int(p / H);
-  const float v = fromBf16(e[i]);
-  float g = 0.f;
-  if (v > 0.f)
+  const int Ho = H / 2, Wo = W / 2, octs = C / 8;
+  if (i >= size_t(N) * Ho * Wo * octs) return;
+  const int c = int(i % octs) * 8;
+  size_t p = i / octs;
+  const int xo = int(p % Wo); p /= Wo;
+  const int yo = int(p % Ho);
+  const int n = int(p / Ho);
+  Bf8 v[4];
+  size_t pix[4];
+  for (int k = 0; k < 4; ++k)
   {
-    const int xo = x / 2, yo = y / 2;
-    // the window's first maximum in order (0,0), (0,1), (1,0), (1,1)
-    int best = -1;
-    float bv = -3.4e38f;
-    for (int k = 0; k < 4; ++k)
-    {
-      const int yy = 2 * yo + k / 2, xx = 2 * xo + k % 2;
-      const float w = fromBf16(e[((size_t(n) * H + yy) * W + xx) * C + c]);
-      if (w > bv) { bv = w; best = k; }
-    }
-    if (best == (y - 2 * yo) * 2 + (x - 2 * xo) && dp)
-      g += fromBf16(dp[((size_t(n) * (H / 2) + yo) * (W / 2) + xo) * C + c]);
-    if (dcat) g += fromBf16(dcat[((size_t(n) * H + y) * W + x) * catC + catOff + c]);
+    pix[k] = (size_t(n) * H + 2 * yo + k / 2) * W + 2 * xo + k % 2;
+    v[k] = loadBf8(e + pix[k] * C + c);
   }
-  de[i] = toBf16(g);
+  Bf8 g{};
+  if (dp) g = loadBf8(dp + ((size_t(n) * Ho + yo) * Wo + xo) * C + c);
+  int first[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+  {
+    int best = 0;
+    float bv = fromBf16(v[0].h[j]);
+    for (int k = 1; k < 4; ++k)
+      if (fromBf16(v[k].h[j]) > bv) { bv = fromBf16(v[k].h[j]); best = k; }
+    first[j] = best;
+  }
+  for (int k = 0; k < 4; ++k)
+  {
+    Bf8 sk{};
+    if (dcat) sk = loadBf8(dcat + pix[k] * catC + catOff + c);
+    Bf8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+      float t = 0.f;
+      if (fromBf16(v[k].h[j]) > 0.f)
+      {
+        if (dp && first[j] == k) t += fromBf16(g.h[j]);
+        if (dcat) t += fromBf16(sk.h[j]);
+      }
+      o.h[j] = toBf16(t);
+    }
+    storeBf8(de + pix[k] * C + c, o);
+  }
 }
 
-// Decoder input: cat[.., 0:Cl] = nearest 2x upsampling of the lower level, cat[.., Cl:] = skip.
+// Decoder input: cat[.., 0:Cl] = nearest 2x upsampling of the lower level, cat[.., Cl:] = skip;
+// a thread copies 8 channels.
 __global__ void __launch_bounds__(kBlock) kUpcat(const uint16_t* __restrict__ low, int Cl, const uint16_t* __restrict__ skip, int Cs,
                                                  int N, int H, int W, uint16_t* __restrict__ cat)
 {
-  const int C = Cl + Cs;
+  const int C = Cl + Cs, octs = C / 8;
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= size_t(N) * H * W * C) return;
-  const int c = int(i % C);
-  size_t p = i / C;
+  if (i >= size_t(N) * H * W * octs) return;
+  const int c = int(i % octs) * 8;
+  size_t p = i / octs;
   const int x = int(p % W); p /= W;
   const int y = int(p % H);
   const int n = int(p / H);
-  cat[i] = c < Cl ? low[((size_t(n) * (H / 2) + y / 2) * (W / 2) + x / 2) * Cl + c]
-                  : skip[((size_t(n) * H + y) * W + x) * Cs + (c - Cl)];
+  const uint16_t* src = c < Cl ? low + ((size_t(n) * (H / 2) + y / 2) * (W / 2) + x / 2) * Cl + c
+                               : skip + ((size_t(n) * H + y) * W + x) * Cs + (c - Cl);
+  *reinterpret_cast<uint4*>(cat + i * 8) = *reinterpret_cast<const uint4*>(src);
 }
 
 // Gradient of the upsampled part of a concat: the sum over each 2x2 block (already masked).
 __global__ void __launch_bounds__(kBlock) kUpBwd(const uint16_t* __restrict__ dcat, int catC, int Cl, int N, int H, int W,
                                                  uint16_t* __restrict__ dlow)
 {
-  const int Ho = H / 2, Wo = W / 2;
+  const int Ho = H / 2, Wo = W / 2, octs = Cl / 8;
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= size_t(N) * Ho * Wo * Cl) return;
-  const int c = int(i % Cl);
-  size_t p = i / Cl;
+  if (i >= size_t(N) * Ho * Wo * octs) return;
+  const int c = int(i % octs) * 8;
+  size_t p = i / octs;
   const int xo = int(p % Wo); p /= Wo;
   const int yo = int(p % Ho);
   const int n = int(p / Ho);
-  float s = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int k = 0; k < 4; ++k)
-    s += fromBf16(dcat[((size_t(n) * H + 2 * yo + k / 2) * W + 2 * xo + k % 2) * catC + c]);
-  dlow[i] = toBf16(s);
+  {
+    const Bf8 v = loadBf8(dcat + ((size_t(n) * H + 2 * yo + k / 2) * W + 2 * xo + k % 2) * catC + c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] += fromBf16(v.h[j]);
+  }
+  Bf8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o.h[j] = toBf16(s[j]);
+  storeBf8(dlow + i * 8, o);
 }
 
 // Network input: log(1 + colour), albedo, normal -> 16 BF16 channels (9 used).
@@ -312,37 +584,47 @@ __global__ void __launch_bounds__(kBlock) kInput(const float* __restrict__ in9, 
 {
   const size_t p = size_t(blockIdx.x) * kBlock + threadIdx.x;
   if (p >= npix) return;
-  for (int c = 0; c < 16; ++c)
+  Bf8 lo{}, hi{};
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
   {
-    float v = 0.f;
-    if (c < 3) v = log1pf(fmaxf(in9[p * 9 + c] * hdrScale, 0.f));
-    else if (c < 9) v = in9[p * 9 + c];
-    x[p * 16 + c] = toBf16(v);
+    const float v = c < 3 ? log1pf(fmaxf(in9[p * 9 + c] * hdrScale, 0.f)) : in9[p * 9 + c];
+    if (c < 8) lo.h[c] = toBf16(v);
+    else hi.h[0] = toBf16(v);
   }
+  storeBf8(x + p * 16, lo);
+  storeBf8(x + p * 16 + 8, hi);
 }
 
-// L1 loss in log(1 + colour) space: loss sum and the BF16 output gradient.
+// L1 loss in log(1 + colour) space: loss sum (one atomic per block) and the BF16 output gradient.
 __global__ void __launch_bounds__(kBlock) kLoss(const float* __restrict__ pred, const float* __restrict__ target, size_t npix,
                                                 float scale, uint16_t* __restrict__ dout, float* __restrict__ loss)
 {
+  __shared__ float part[kBlock / 64];
   const size_t p = size_t(blockIdx.x) * kBlock + threadIdx.x;
   float l = 0.f;
   if (p < npix)
   {
-    for (int c = 0; c < 16; ++c)
+    Bf8 g{}, z{};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
     {
-      float g = 0.f;
-      if (c < 3)
-      {
-        const float d = pred[p * 3 + c] - log1pf(fmaxf(target[p * 3 + c], 0.f));
-        l += fabsf(d);
-        g = d > 0.f ? scale : (d < 0.f ? -scale : 0.f);
-      }
-      dout[p * 16 + c] = toBf16(g);
+      const float d = pred[p * 3 + c] - log1pf(fmaxf(target[p * 3 + c], 0.f));
+      l += fabsf(d);
+      g.h[c] = toBf16(d > 0.f ? scale : (d < 0.f ? -scale : 0.f));
     }
+    storeBf8(dout + p * 16, g);
+    storeBf8(dout + p * 16 + 8, z);
   }
   for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
-  if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    float t = 0.f;
+    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+    if (t != 0.f) atomicAdd(loss, t);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) kOutput(const float* __restrict__ pred, size_t npix, float invScale, float* __restrict__ out)
@@ -427,6 +709,7 @@ struct prt_denoiser
   uint16_t* wf[10] = {};
   uint16_t* wt[10] = {};
   float* dwp[10] = {};  // padded FP32 weight gradients
+  float* wgPart = nullptr;  // partial weight / bias gradients of the weight-gradient blocks
   int step = 0;
   // activation buffers for the current (N, H, W)
   int N = 0, H = 0, W = 0;
@@ -465,6 +748,21 @@ hipError_t actAlloc(prt_denoiser* dn, T*& p, size_t count)
   return e;
 }
 
+// Image rows per weight-gradient block: about 1024 blocks per launch.
+int wgradRows(int N, int H, int coutPad, int cinPad)
+{
+  const size_t groups = size_t((coutPad + 63) / 64) * ((cinPad + 31) / 32);
+  return int(std::min<size_t>(size_t(H), std::max<size_t>(1, (size_t(N) * H * groups + 1023) / 1024)));
+}
+
+// Floats of partial sums the weight-gradient blocks of a layer at one level need.
+size_t wgradPartFloats(int N, int H, int coutPad, int cinPad)
+{
+  const int rows = wgradRows(N, H, coutPad, cinPad);
+  const size_t nPart = size_t(N) * ((H + rows - 1) / rows);
+  return nPart * (size_t(9) * coutPad * cinPad + coutPad);
+}
+
 // Activation and gradient buffers for N images of H x W (multiples of 16).
 int ensureActs(prt_denoiser* dn, int N, int H, int W)
 {
@@ -495,8 +793,24 @@ int ensureActs(prt_denoiser* dn, int N, int H, int W)
     DN_TRY(actAlloc(dn, dn->d[l], np(l) * decC[l]));
     DN_TRY(actAlloc(dn, dn->dd[l], np(l) * decC[l]));
   }
+  size_t part = 0;
+  for (int i = 0; i < 10; ++i)
+  {
+    const int lvl = i < 5 ? i : (i < 9 ? 8 - i : 0);  // enc0..enc4, dec3..dec0, out
+    part = std::max(part, wgradPartFloats(N, H >> lvl, dn->L[i].coutPad, dn->L[i].cinPad));
+  }
+  DN_TRY(actAlloc(dn, dn->wgPart, part));
   dn->N = N; dn->H = H; dn->W = W;
   return PRT_OK;
+}
+
+void launchConv(prt_denoiser* dn, const ConvArgs& a)
+{
+  const unsigned tiles = unsigned(a.N * ((a.H + kCtY - 1) / kCtY) * ((a.W + kCtX - 1) / kCtX));
+  if (a.coutPad <= 32)
+    kConvTile<2><<<dim3(tiles, (a.coutPad + 31) / 32), kCtThreads, 0, dn->stream>>>(a);
+  else
+    kConvTile<4><<<dim3(tiles, (a.coutPad + 63) / 64), kCtThreads, 0, dn->stream>>>(a);
 }
 
 void conv(prt_denoiser* dn, int layer, const uint16_t* x, int N, int H, int W, uint16_t* y, int yC, int yOff, bool relu, float* yf)
@@ -506,8 +820,7 @@ void conv(prt_denoiser* dn, int layer, const uint16_t* x, int N, int H, int W, u
   a.x = x; a.xC = l.cinPad; a.w = dn->wf[layer]; a.bias = dn->w + l.wOff + size_t(l.cout) * l.cin * 9;
   a.N = N; a.H = H; a.W = W; a.cinPad = l.cinPad; a.coutPad = l.coutPad; a.coutReal = l.cout;
   a.relu = relu ? 1 : 0; a.y = y; a.yC = yC; a.yOff = yOff; a.yf = yf; a.mask = nullptr;
-  const size_t waves = size_t(N) * H * ((W + 63) / 64) * (l.coutPad / 16);
-  kConv<1><<<blocksFor(waves * 64), kBlock, 0, dn->stream>>>(a);
+  launchConv(dn, a);
 }
 
 // data gradient of layer `layer`: dx (layout of its input, cinPad channels) from dy (coutPad),
@@ -519,19 +832,21 @@ void dgrad(prt_denoiser* dn, int layer, const uint16_t* dy, int N, int H, int W,
   a.x = dy; a.xC = l.coutPad; a.w = dn->wt[layer]; a.bias = nullptr;
   a.N = N; a.H = H; a.W = W; a.cinPad = l.coutPad; a.coutPad = l.cinPad; a.coutReal = l.cinPad;
   a.relu = 0; a.y = dx; a.yC = l.cinPad; a.yOff = 0; a.yf = nullptr; a.mask = mask;
-  const size_t waves = size_t(N) * H * ((W + 63) / 64) * (l.cinPad / 16);
-  kConv<1><<<blocksFor(waves * 64), kBlock, 0, dn->stream>>>(a);
+  launchConv(dn, a);
 }
 
 void wgrad(prt_denoiser* dn, int layer, const uint16_t* dy, const uint16_t* x, int N, int H, int W)
 {
   const DenoiseLayer& l = dn->L[layer];
-  WgradArgs a{dy, x, N, H, W, l.cinPad, l.coutPad, 16, dn->dwp[layer]};
-  const size_t waves = size_t(9) * ((l.coutPad + 31) / 32) * ((l.cinPad + 31) / 32) * N * ((H + 15) / 16);
-  kWgrad<<<blocksFor(waves * 64), kBlock, 0, dn->stream>>>(a);
-  const size_t npix = size_t(N) * H * W;
-  const size_t threads = ((npix + 1023) / 1024) * l.cout;
-  kBiasGrad<<<blocksFor(threads), kBlock, 0, dn->stream>>>(dy, int(npix), l.coutPad, l.cout, dn->g + l.wOff + size_t(l.cout) * l.cin * 9);
+  const int rows = wgradRows(N, H, l.coutPad, l.cinPad);
+  const int chunks = (H + rows - 1) / rows;
+  const int nPart = N * chunks;
+  const size_t size = size_t(9) * l.coutPad * l.cinPad;
+  WgradArgs a{dy, x, N, H, W, l.cinPad, l.coutPad, rows, dn->wgPart, dn->wgPart + size_t(nPart) * size};
+  const size_t groups = size_t((l.coutPad + 63) / 64) * ((l.cinPad + 31) / 32);
+  kWgradTile<<<unsigned(groups * nPart), kWgThreads, 0, dn->stream>>>(a);
+  kWgradReduce<<<unsigned((size + l.cout + kRedEl - 1) / kRedEl), kBlock, 0, dn->stream>>>(a.part, a.partB, nPart, size, l.coutPad, l.cout,
+                                                                     dn->dwp[layer], dn->g + l.wOff + size_t(l.cout) * l.cin * 9);
 }
 
 void forward(prt_denoiser* dn, int N, int H, int W)
@@ -544,7 +859,7 @@ void forward(prt_denoiser* dn, int N, int H, int W)
     conv(dn, l, in, N, H >> s, W >> s, dn->e[l], encC[l], 0, true, nullptr);
     if (l < 4)
     {
-      const size_t n = size_t(N) * (H >> (s + 1)) * (W >> (s + 1)) * encC[l];
+      const size_t n = size_t(N) * (H >> (s + 1)) * (W >> (s + 1)) * (encC[l] / 8);
       kPool<<<blocksFor(n), kBlock, 0, dn->stream>>>(dn->e[l], N, H >> s, W >> s, encC[l], dn->p[l]);
       in = dn->p[l];
     }
@@ -555,7 +870,7 @@ void forward(prt_denoiser* dn, int N, int H, int W)
   for (int l = 3; l >= 0; --l)
   {
     const int h = H >> l, w = W >> l;
-    const size_t n = size_t(N) * h * w * (lowC + encC[l]);
+    const size_t n = size_t(N) * h * w * ((lowC + encC[l]) / 8);
     kUpcat<<<blocksFor(n), kBlock, 0, dn->stream>>>(low, lowC, dn->e[l], encC[l], N, h, w, dn->cat[l]);
     conv(dn, 8 - l, dn->cat[l], N, h, w, dn->d[l], decC[l], 0, true, nullptr);  // dec3 = layer 5 ... dec0 = layer 8
     low = dn->d[l];
@@ -579,7 +894,7 @@ void backward(prt_denoiser* dn, int N, int H, int W)
     dgrad(dn, layer, dn->dd[l], N, h, w, dn->dcat[l], dn->cat[l]);
     const int lowC = l == 3 ? 128 : decC[l + 1];
     uint16_t* dlow = l == 3 ? dn->de[4] : dn->dd[l + 1];
-    const size_t n = size_t(N) * (h / 2) * (w / 2) * lowC;
+    const size_t n = size_t(N) * (h / 2) * (w / 2) * (lowC / 8);
     kUpBwd<<<blocksFor(n), kBlock, 0, dn->stream>>>(dn->dcat[l], catC, lowC, N, h, w, dlow);
   }
   // encoder: de[4] is the masked gradient of e[4]
@@ -593,7 +908,7 @@ void backward(prt_denoiser* dn, int N, int H, int W)
       dgrad(dn, l, dn->de[l], N, h, w, dn->dp[l - 1], nullptr);
       const int catC = (l - 1 == 3 ? 128 : decC[l]) + encC[l - 1];
       const int lowC = l - 1 == 3 ? 128 : decC[l];
-      const size_t n = size_t(N) * (h * 2) * (w * 2) * encC[l - 1];
+      const size_t n = size_t(N) * h * w * (encC[l - 1] / 8);  // 2x2 windows of level l - 1
       kPoolBwd<<<blocksFor(n), kBlock, 0, dn->stream>>>(dn->e[l - 1], dn->dp[l - 1], dn->dcat[l - 1], catC, lowC, N, h * 2, w * 2,
                                                         encC[l - 1], dn->de[l - 1]);
     }
@@ -619,8 +934,6 @@ int lossAndGrad(prt_denoiser* dn, int N, int H, int W, double* lossOut)
   const size_t npix = size_t(N) * H * W;
   DN_TRY(hipMemsetAsync(dn->loss, 0, sizeof(float), dn->stream));
   DN_TRY(hipMemsetAsync(dn->g, 0, sizeof(float) * dn->nParams, dn->stream));
-  for (int i = 0; i < 10; ++i)
-    DN_TRY(hipMemsetAsync(dn->dwp[i], 0, sizeof(float) * 9 * dn->L[i].coutPad * dn->L[i].cinPad, dn->stream));
   kInput<<<blocksFor(npix), kBlock, 0, dn->stream>>>(dn->in9, npix, 1.f, dn->x);
   forward(dn, N, H, W);
   kLoss<<<blocksFor(npix), kBlock, 0, dn->stream>>>(dn->pred, dn->tgt, npix, 1.0f / float(npix * 3), dn->dout, dn->loss);

@@ -66,21 +66,10 @@ struct prt_light_dev
   float pad;
 };
 
-// BVH node (64 bytes): the bounding boxes of both children (so one node fetch tests two
-// boxes) and the two child references.  child >= 0: inner node index; child < 0: leaf,
-// (child & 0x7fffffff) = first_triangle << 3 | (count - 1).
-struct prt_bvh_node
-{
-  float lox[2], hix[2];  // [child0, child1] ... packed as x: lo0, hi0, lo1, hi1
-  float loy[2], hiy[2];
-  float loz[2], hiz[2];
-  int32_t child[2];
-  int32_t pad[2];
-};
-
 // 4-wide BVH node (128 bytes, one L2 line): the boxes of up to four children as SoA (lo/hi per
-// axis for children 0..3) and four child references with the same encoding as above.  Unused
-// slots hold an empty box (lo = +inf, hi = -inf) that no ray hits.
+// axis for children 0..3) and four child references: child >= 0 is an inner node index,
+// child < 0 a leaf with (child & 0x7fffffff) = first_triangle << 3 | (count - 1), -1 an unused
+// slot (which also holds an empty box, lo = +inf, hi = -inf).
 struct prt_bvh4_node
 {
   float lox[4], hix[4];

@@ -166,8 +166,30 @@ def time_to(points, target):
     return None
 
 
+def kernel_bench(inferences=4, train_steps=6, width=1920, height=1080, batch=16, crop=256, log=print):
+    """Device time of the denoiser's kernels on synthetic inputs: 1080p inferences with the
+    committed weights, then training steps on batch x crop x crop (the training configuration)."""
+    dn = P.DenoiserHip(0)
+    dn.setWeights(np.load(os.path.join(HERE, "weights", "unet_sunsky.npy")))
+    rng = np.random.default_rng(0)
+    col, alb, nrm = (rng.random((height, width, 3), np.float32) for _ in range(3))
+    res = {"inference_ms": [], "train_step_ms": []}
+    for _ in range(inferences):
+        dn.denoise(col, alb, nrm)
+        res["inference_ms"].append(dn.kernelMs())
+    inp = rng.random((batch, crop, crop, 9), np.float32)
+    tgt = rng.random((batch, crop, crop, 3), np.float32)
+    for _ in range(train_steps):
+        res["train_step_ms"].append(dn.trainStep(inp, tgt, 1e-4)[1])
+    dn.close()
+    log(json.dumps(res))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description="train the denoiser on rendered data and evaluate relMSE at 1080p")
+    ap.add_argument("--kernel-bench", action="store_true",
+                    help="only time the kernels on synthetic inputs (1080p inference, 16 x 256^2 training steps)")
     ap.add_argument("--frames", type=int, default=24)
     ap.add_argument("--train-ref-spp", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=1500)
@@ -176,6 +198,9 @@ def main():
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
     log = lambda m: print(m, flush=True)
+    if args.kernel_bench:
+        kernel_bench(log=log)
+        return
     inputs, targets, r = training_set(args.frames, ref_spp=args.train_ref_spp, log=log)
     dn = P.DenoiserHip(0)
     tr = train(dn, inputs, targets, steps=args.steps, log=log)
