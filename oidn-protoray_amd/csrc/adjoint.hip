@@ -53,10 +53,10 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
     if (nParams > 0)
     {
       auto hook = [&](const PathVertex& pv) {
-        if (pv.depth >= PRT_ADJ_MAX_VERTICES || !(pv.m->kind == PRT_KIND_DIFFUSE || pv.m->kind == PRT_KIND_COATED)) return;
+        if (pv.depth >= PRT_ADJ_MAX_VERTICES) return;
         float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
         auto add = [&](f3 dir, f3 radiance) {
-          BsdfGrad g = bsdfEvalGrad(a.lut, *pv.b, *pv.ctx, pv.wo, dir);
+          BsdfGrad g = bsdfEvalGrad(a.lut, *pv.b, *pv.m, *pv.ctx, pv.wo, dir);
           f3 wgt = dl * radiance;
           if (g.value.x > 0.f) { gv.x += wgt.x * g.dAlbedo.x / g.value.x; gv.w += wgt.x * g.dAlpha.x / g.value.x; }
           if (g.value.y > 0.f) { gv.y += wgt.y * g.dAlbedo.y / g.value.y; gv.w += wgt.y * g.dAlpha.y / g.value.y; }

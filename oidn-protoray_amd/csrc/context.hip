@@ -297,6 +297,7 @@ struct prt_context
   int spillStride = 0;
   // timing
   hipEvent_t ev[8] = {};
+  std::vector<hipEvent_t> evPass[2];  // per-pass kernel timing events, double-buffered
   double kernelMs[4] = {0, 0, 0, 0};
   double lastStats[6] = {0, 0, 0, 0, 0, 0};
   double launches[2] = {0, 0};
@@ -358,6 +359,8 @@ int prt_context_destroy(prt_context* c)
   freeBvh(c->bvh);
   if (c->argsHost) hipHostFree(c->argsHost);
   for (auto& ev : c->ev) if (ev) hipEventDestroy(ev);
+  for (auto& set : c->evPass)
+    for (auto& ev : set) if (ev) hipEventDestroy(ev);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return PRT_OK;
@@ -497,6 +500,8 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
                      0.278708f * r3 + 0.19744f * g * rr + 0.0360605f * g2 * rr - 0.2586f * g * r2;
         q.c2[j] = hsqr(Favg) * q.Eavg / (1.f - Favg * (1.f - q.Eavg));  // rough_conductor.h:127-129
         q.rr[j] = rgb[j];
+        q.edge[j] = g;
+        q.Favg[j] = Favg;
       }
     }
     else if (s.kind == PRT_KIND_COATED)
@@ -539,6 +544,8 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
         q.EavgEta[o] = Ee;
         q.EavgRcpEta[o] = Er;
         q.fmsRatio[o] = 1.f - x * (1.f - Fe);  // rough_dielectric.h:283-291
+        q.FavgEta[o] = Fe;
+        q.FavgRcpEta[o] = Fr;
       }
       // media deduplicated by (color, IOR), ID 0 = ambient (material/material_factory.cpp:20-21,159-175)
       int id = -1;
@@ -929,47 +936,75 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
   HIP_TRY(hipMemsetAsync(c->stats.p, 0, sizeof(unsigned long long) * 8, c->stream));
   for (int k = 0; k < 4; ++k) c->kernelMs[k] = 0.0;
   c->launches[0] = c->launches[1] = 0;
+  // The first kFreeDepths depths of a pass are enqueued without waiting (a depth with no queued
+  // paths launches kernels that find an empty queue and return); deeper ones check on the host
+  // whether any path survives, so a large maxDepth does not launch hundreds of empty kernels.
+  // The kernel times of pass s are read after pass s + 1 is enqueued (two event sets).
+  constexpr int kFreeDepths = 9;
+  const size_t nEv = 2 + 3 * size_t(a.maxDepth + 1);
+  for (auto& set : c->evPass)
+    while (set.size() < nEv)
+    {
+      hipEvent_t e = nullptr;
+      HIP_TRY(hipEventCreate(&e));
+      set.push_back(e);
+    }
+  int depths[2] = {0, 0};  // depths enqueued in each event set's pass
+  auto readTimes = [&](int set) -> hipError_t {
+    const std::vector<hipEvent_t>& ev = c->evPass[set];
+    const int nd = depths[set];
+    hipError_t e = hipEventSynchronize(ev[1 + 3 * nd]);
+    if (e != hipSuccess) return e;
+    float m = 0.f;
+    hipEventElapsedTime(&m, ev[0], ev[1]);
+    c->kernelMs[0] += m;
+    for (int d = 0; d < nd; ++d)
+      for (int k = 0; k < 3; ++k)
+      {
+        hipEventElapsedTime(&m, ev[1 + 3 * d + k], ev[2 + 3 * d + k]);
+        c->kernelMs[1 + k] += m;
+      }
+    c->launches[0] += nd;
+    c->launches[1] += nd;
+    return hipSuccess;
+  };
   auto t0 = std::chrono::high_resolution_clock::now();
   for (int s = 0; s < n_spp; ++s)
   {
+    const std::vector<hipEvent_t>& ev = c->evPass[s & 1];
     // FrameBuffer::begin: checkpoint copy at power-of-two sample counts (accum_buffer.h:252-260)
     if (isPowerOfTwo(c->count))
     {
       HIP_TRY(hipMemcpyAsync(c->colorC.p, c->color.p, sizeof(float) * 3 * c->npix, hipMemcpyDeviceToDevice, c->stream));
       HIP_TRY(hipMemcpyAsync(c->weightC.p, c->weight.p, sizeof(float) * c->npix, hipMemcpyDeviceToDevice, c->stream));
     }
-    hipEventRecord(c->ev[0], c->stream);
+    HIP_TRY(hipEventRecord(ev[0], c->stream));
     launchGenerate(ad, a.npix, c->sampleIndex, c->stream);
-    hipEventRecord(c->ev[1], c->stream);
-    float msGen = 0.f, msI = 0.f, msS = 0.f, msO = 0.f;
-    for (int depth = 0; depth <= a.maxDepth; ++depth)
+    HIP_TRY(hipEventRecord(ev[1], c->stream));
+    int depth = 0;
+    for (; depth <= a.maxDepth; ++depth)
     {
-      hipEventRecord(c->ev[2], c->stream);
       launchIntersect(ad, depth, c->launch, c->stream);
-      hipEventRecord(c->ev[3], c->stream);
+      HIP_TRY(hipEventRecord(ev[2 + 3 * depth], c->stream));
       launchShade(ad, depth, c->sampleIndex, c->launch, c->stream);
-      hipEventRecord(c->ev[4], c->stream);
+      HIP_TRY(hipEventRecord(ev[3 + 3 * depth], c->stream));
       launchOccluded(ad, depth, c->launch, c->stream);
-      hipEventRecord(c->ev[5], c->stream);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipEventSynchronize(c->ev[5]));
-      float m1 = 0, m2 = 0, m3 = 0;
-      hipEventElapsedTime(&m1, c->ev[2], c->ev[3]);
-      hipEventElapsedTime(&m2, c->ev[3], c->ev[4]);
-      hipEventElapsedTime(&m3, c->ev[4], c->ev[5]);
-      msI += m1; msS += m2; msO += m3;
-      c->launches[0] += 1; c->launches[1] += 1;
-      // stop early once no path survives (checked on the host after each depth)
-      int alive = 0;
-      HIP_TRY(hipMemcpyAsync(&alive, c->counters.p + 2 * (depth + 1), sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIP_TRY(hipStreamSynchronize(c->stream));
-      if (alive == 0) break;
+      HIP_TRY(hipEventRecord(ev[4 + 3 * depth], c->stream));
+      if (depth + 1 >= kFreeDepths && depth < a.maxDepth)
+      {
+        int alive = 0;
+        HIP_TRY(hipMemcpyAsync(&alive, c->counters.p + 2 * (depth + 1), sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (alive == 0) { ++depth; break; }
+      }
     }
-    hipEventElapsedTime(&msGen, c->ev[0], c->ev[1]);
-    c->kernelMs[0] += msGen; c->kernelMs[1] += msI; c->kernelMs[2] += msS; c->kernelMs[3] += msO;
+    depths[s & 1] = depth;
+    HIP_TRY(hipGetLastError());
+    if (s > 0) HIP_TRY(readTimes((s - 1) & 1));
     c->sampleIndex++;  // (:185-186)
     c->count++;        // FrameBuffer::finish (accum_buffer.h:263-267)
   }
+  if (n_spp > 0) HIP_TRY(readTimes((n_spp - 1) & 1));
   HIP_TRY(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::high_resolution_clock::now();
   unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};

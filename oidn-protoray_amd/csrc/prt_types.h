@@ -50,7 +50,11 @@ struct prt_material_constants
   int32_t mediumInt;  // interior medium ID (0 for non-dielectrics)
   float emission[3];  // emissionColor * emissionLuminance
   int32_t emissive;   // hits end the path as area-light hits (render/path_renderer_stream.h:419-582)
-  int32_t pad[2];
+  // read by the adjoint's parameter derivatives only
+  float edge[3];       // conductor edge tint g
+  float Favg[3];       // conductor Kulla-Conty average Fresnel
+  float FavgEta[2];    // dielectric average Fresnel at eta, outside [0] / inside [1]
+  float FavgRcpEta[2]; // dielectric average Fresnel at 1 / eta
 };
 
 // Environment light record as the kernels read it.  64 bytes.

@@ -126,6 +126,20 @@ def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4
             secs = dn.execute()
             res["denoised"].append({"spp": spp, "relmse": relmse(out, ref), "denoise_s": secs, "denoise_kernel_ms": dn.kernelMs(),
                                     "noisy_relmse": relmse(img, ref)})
+    # the batched form: the same 4-sample frame (seed 0) from the persistent batch renderer in one
+    # launch (it equals the wavefront renderer's image), then the denoiser
+    batch = P.BatchHip(device)
+    batch.addFrame(dev, sc["camera"], W, H, 0, 4, aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+    batch.render()  # first launch builds the graph
+    br = batch.render()
+    bimg, balb, bnrm = batch.get(0), batch.get(0, P.BATCH_ALBEDO1), batch.get(0, P.BATCH_NORMAL1)
+    batch.close()
+    bout = np.zeros_like(bimg)
+    dn.init(bimg, balb, bnrm, bout)
+    bsecs = dn.execute()
+    res["denoised_batched"] = {"spp": 4, "render_ms": br["ms"], "denoise_s": bsecs, "denoise_kernel_ms": dn.kernelMs(),
+                               "relmse": relmse(bout, ref), "ms": br["ms"] + bsecs * 1e3,
+                               "ms_device": br["ms"] + dn.kernelMs()}
     # time per sample pass of this frame
     dev.synchronize()
     t0 = time.perf_counter()
@@ -137,6 +151,7 @@ def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4
         r["ms"] = r["spp"] * ms_per_spp
     for r in res["denoised"]:
         r["ms"] = r["spp"] * ms_per_spp + r["denoise_s"] * 1e3
+        r["ms_device"] = r["spp"] * ms_per_spp + r["denoise_kernel_ms"]
     res["ms_per_spp"] = ms_per_spp
     res["ref_spp"] = ref_spp
     res["time_to_relmse_1e-3_ms"] = {"plain": time_to(res["plain"], 1e-3), "denoised": time_to(res["denoised"], 1e-3)}
