@@ -183,8 +183,10 @@ int prt_query_rays_device(prt_context* ctx, int32_t n, const float* rays_device,
  * (indices 0 .. spp-1 of the sampler seed); forms the loss mean over pixels and channels of
  * (I - target)^2 with target an H x W x 3 image in prt_blit layout; returns *loss and
  * dLoss / d(albedo r, g, b, GGX alpha) per material in grad[4 * material + k] (grad_capacity
- * floats, >= 4 x the material count).  Differentiated: the diffuse colour of kinds 0 and 3 and
- * the coat alpha (= roughness^2) of kind 3; other entries are 0.  The gradient is that of the
+ * floats, >= 4 x the material count).  Differentiated: the diffuse colour of kinds 0 and 3, the
+ * reflectivity (baseColor) of kind 1, and alpha (= roughness^2) of kinds 1, 2 and 3 (the
+ * microfacet lobe, through D, G2 and the albedo tables of the multiple-scattering term); kind 0
+ * has no alpha and kind 2 no albedo, so those entries are 0.  The gradient is that of the
  * primal estimator with its samples fixed (detached sampling); paths are replayed from the
  * seed, not stored.  Requires maxDepth <= 8.  *ms: device time of the replay kernels. */
 int prt_adjoint(prt_context* ctx, const float* target, int32_t spp, double* loss, float* grad, int32_t grad_capacity,

@@ -56,6 +56,7 @@ struct prt_material_constants
   float FavgEta[2];    // dielectric average Fresnel at eta, outside [0] / inside [1]
   float FavgRcpEta[2]; // dielectric average Fresnel at 1 / eta
 };
+static_assert(sizeof(prt_material_constants) == 160, "material constants: 160 bytes");
 
 // Environment light record as the kernels read it.  64 bytes.
 struct prt_light_dev

@@ -188,17 +188,25 @@ def spheres_ambient(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=
                 camera=_camera(), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)
 
 
-def spheres_adjoint(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=32, max_depth=4, n_materials=64):
+def spheres_adjoint(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=32, max_depth=4, n_materials=64,
+                    mixed=False):
     """Differentiable-rendering scene (config-0 style): the ground (diffuse 0.5) plus n_spheres
     icospheres sharing n_materials coated-diffuse materials (material IDs 2 .. n_materials+1;
     albedo U[0.2,0.8]^3, coat IOR 1.5, coat alpha U[0.05,0.4], i.e. roughness sqrt(alpha)),
-    under the ambient light and a distant light."""
+    under the ambient light and a distant light.  mixed=True makes every third material a rough
+    conductor (reflectivity = the albedo, edge tint 0.5) and every third, starting at the second,
+    a rough dielectric (IOR 1.5), with the same alpha draw."""
     rng = Pcg32(seed)
     mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
-    for _ in range(n_materials):
+    for i in range(n_materials):
         rgb = (rng.f(0.2, 0.8), rng.f(0.2, 0.8), rng.f(0.2, 0.8))
         alpha = rng.f(0.05, 0.4)
-        mats.append(_material(KIND_COATED, rgb, (1, 1, 1), math.sqrt(alpha), 1.5, thickness=1.0, weight=1.0))
+        if mixed and i % 3 == 1:
+            mats.append(_material(KIND_CONDUCTOR, rgb, (0.5, 0.5, 0.5), math.sqrt(alpha)))
+        elif mixed and i % 3 == 2:
+            mats.append(_material(KIND_DIELECTRIC, (1, 1, 1), (0, 0, 0), math.sqrt(alpha), 1.5))
+        else:
+            mats.append(_material(KIND_COATED, rgb, (1, 1, 1), math.sqrt(alpha), 1.5, thickness=1.0, weight=1.0))
     pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, lambda i: 2 + i % n_materials)
     lights = np.zeros(2, LIGHT_DTYPE)
     d = np.array([-0.5, 0.8, -0.3])
@@ -212,13 +220,14 @@ def spheres_adjoint(seed=0, n_spheres=256, subdiv=3, width=512, height=512, spp=
 
 def perturbed_materials(materials, seed=1, d_albedo=0.1, d_alpha=0.05):
     """Target parameters of the differentiable-rendering configuration: every coated material's
-    albedo moved by +-d_albedo per channel and its alpha by +-d_alpha (signs from PCG32 seed)."""
+    albedo moved by +-d_albedo per channel and its alpha by +-d_alpha (signs from PCG32 seed);
+    conductors likewise (reflectivity and alpha), dielectrics in alpha only."""
     rng = Pcg32(seed)
     m = materials.copy()
     for i in range(m.shape[0]):
-        if m[i]["kind"] != KIND_COATED:
+        if m[i]["kind"] not in (KIND_COATED, KIND_CONDUCTOR, KIND_DIELECTRIC):
             continue
-        for ch in ("r", "g", "b"):
+        for ch in ("r", "g", "b") if m[i]["kind"] != KIND_DIELECTRIC else ():
             m[i][ch] = min(max(m[i][ch] + (d_albedo if rng.f() < 0.5 else -d_albedo), 0.02), 0.98)
         alpha = float(m[i]["roughness"]) ** 2 + (d_alpha if rng.f() < 0.5 else -d_alpha)
         m[i]["roughness"] = math.sqrt(min(max(alpha, 0.01), 0.9))

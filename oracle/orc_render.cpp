@@ -452,6 +452,169 @@ static D3 d3(V3 a) { return D3{a.x, a.y, a.z}; }
 static double dotd(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static D3 normd(D3 a) { double l = std::sqrt(dotd(a, a)); return D3{a.x / l, a.y / l, a.z / l}; }
 
+static double interp1Dd(double x, const float* f, int n, bool slope)
+{
+  if (slope && !(x > 0.0 && x < double(n - 1))) return 0.0;
+  double xc = clampd(x, 0, n - 1);
+  int x0 = std::min(int(xc), n - 1), x1 = std::min(x0 + 1, n - 1);
+  return slope ? double(f[x1]) - f[x0] : lerpd(f[x0], f[x1], xc - std::floor(xc));
+}
+
+// GGX D and the Smith masking term in FP64, with their alpha derivatives (ggx_distribution.h:24-102).
+struct GgxD { double D, G, dD, dG; };
+static GgxD ggxD(double a, D3 o0, D3 i0, D3 h0)
+{
+  double sh = h0.x * h0.x + h0.y * h0.y, z2 = h0.z * h0.z, q = a * a * z2 + sh;
+  auto lam = [&](D3 w) { double t2 = (w.x * w.x + w.y * w.y) / (w.z * w.z); return 0.5 * (-1 + std::sqrt(1 + a * a * t2)); };
+  auto dlam = [&](D3 w) { double t2 = (w.x * w.x + w.y * w.y) / (w.z * w.z); return 0.5 * a * t2 / std::sqrt(1 + a * a * t2); };
+  GgxD r;
+  r.D = a * a / (M_PI * q * q);
+  r.G = 1.0 / (1.0 + lam(o0) + lam(i0));
+  r.dD = a > 0.001 ? (2 * a / (M_PI * q * q)) * (1 - 2 * a * a * z2 / q) : 0.0;
+  r.dG = a > 0.001 ? -r.G * r.G * (dlam(o0) + dlam(i0)) : 0.0;
+  return r;
+}
+static double fresnelDielD(double cosI, double eta)  // optics.h:84-99: cosI clamped at 0
+{
+  double ci = std::max(cosI, 0.0), kk = 1.0 - eta * eta * (1.0 - ci * ci);
+  if (kk < 0.0) return 1.0;
+  double ct = std::sqrt(kk);
+  double Rper = (eta * ci - ct) / (eta * ci + ct), Rpar = (ci - eta * ct) / (ci + eta * ct);
+  return 0.5 * (Rpar * Rpar + Rper * Rper);
+}
+static double fresnelCondD(double cosI, double eta, double k)  // optics.h:179-190
+{
+  double c = std::fabs(cosI), c2 = c * c, tmp = eta * eta + k * k, e2 = 2 * eta * c;
+  return 0.5 * ((tmp * c2 - e2 + 1) / (tmp * c2 + e2 + 1) + (tmp - e2 + c2) / (tmp + e2 + c2));
+}
+static double artisticFresnelD(double cosI, double r, double g)  // optics.h:214-226
+{
+  r = std::min(r, 0.99);
+  double nMin = (1 - r) / (1 + r), nMax = (1 + std::sqrt(r)) / (1 - std::sqrt(r)), n = g * nMin + (1 - g) * nMax;
+  double k2 = ((n + 1) * (n + 1) * r - (n - 1) * (n - 1)) / (1 - r);
+  return fresnelCondD(cosI, n, std::sqrt(std::max(k2, 0.0)));
+}
+static double artisticAvgD(double r, double g)  // optics.h:228-240
+{
+  return 0.087237 + 0.0230685 * g - 0.0864902 * g * g + 0.0774594 * g * g * g + 0.782654 * r - 0.136432 * r * r +
+         0.278708 * r * r * r + 0.19744 * g * r + 0.0360605 * g * g * r - 0.2586 * g * r * r;
+}
+static double fresnelDielAvgD(double eta)  // optics.h:144-175
+{
+  double re = 1.0 / eta;
+  if (re >= 1.0) return (re - 1.0) / (4.08567 + 1.00071 * re);
+  return 0.997118 + 0.1014 * re - 0.965241 * re * re - 0.130607 * re * re * re;
+}
+// dielectric albedo tables and their roughness slopes (albedo_tables.h:82-154)
+static double dielDirD(const Luts& L, double cosO, double eta, double r, bool slope)
+{
+  if (eta <= 1.0) return interp3Dd(cosO * 15, (eta - 1.0 / 3.0) / (2.0 / 3.0) * 15, r * 15, L.dielEtaDir, 16, slope) * (slope ? 15 : 1);
+  return interp3Dd(cosO * 15, (eta - 1.0) / 2.0 * 15, r * 15, L.dielRcpEtaDir, 16, slope) * (slope ? 15 : 1);
+}
+static double dielAvgD(const Luts& L, double eta, double r, bool slope)
+{
+  if (eta <= 1.0) return interp2Dd((eta - 1.0 / 3.0) / (2.0 / 3.0) * 15, r * 15, L.dielEtaAvg, 16, slope) * (slope ? 15 : 1);
+  return interp2Dd((eta - 1.0) / 2.0 * 15, r * 15, L.dielRcpEtaAvg, 16, slope) * (slope ? 15 : 1);
+}
+
+// Rough conductor: value and derivatives per channel, albedo = reflectivity r (the mapping to n, k
+// by a central difference of width 2e-3, as the device does), alpha through D, G2 and the tables.
+static GradD conductorGradD(const Luts& L, const Bsdf& b, D3 N, D3 U, D3 Vv, D3 o, D3 i)
+{
+  GradD g;
+  const double cosO = dotd(o, N), cosI = dotd(i, N);
+  if (!(cosO > 0.0 && cosI > 0.0)) return g;
+  const D3 h = normd(D3{o.x + i.x, o.y + i.y, o.z + i.z});
+  const double cosOH = dotd(o, h);
+  const D3 o0{dotd(o, U), dotd(o, Vv), dotd(o, N)}, i0{dotd(i, U), dotd(i, Vv), dotd(i, N)}, h0{dotd(h, U), dotd(h, Vv), dotd(h, N)};
+  const double r = b.roughness, a = std::max(r * r, 0.001);
+  const GgxD m = ggxD(a, o0, i0, h0);
+  const double kF = m.D * m.G / (4 * cosO), dkF = (m.dD * m.G + m.D * m.dG) / (4 * cosO);
+  const double Eo = interp2Dd(cosO * 31, r * 31, L.ggxDir, 32, false), Ei = interp2Dd(cosI * 31, r * 31, L.ggxDir, 32, false);
+  const double dEo = 31 * interp2Dd(cosO * 31, r * 31, L.ggxDir, 32, true), dEi = 31 * interp2Dd(cosI * 31, r * 31, L.ggxDir, 32, true);
+  const double E = interp1Dd(r * 31, L.ggxAvg, 32, false), dE = 31 * interp1Dd(r * 31, L.ggxAvg, 32, true);
+  const double A = (1 - Eo) * (1 - Ei), B = 1 / (1 - E), dA = -dEo * (1 - Ei) - (1 - Eo) * dEi, dB = dE * B * B;
+  const double drda = (a > 0.001 && r > 0) ? 0.5 / r : 0.0, w = cosI / M_PI;
+  const double rr[3] = {b.rr.x, b.rr.y, b.rr.z}, gg[3] = {b.gg.x, b.gg.y, b.gg.z};
+  for (int k = 0; k < 3; ++k)
+  {
+    const double F = artisticFresnelD(cosOH, rr[k], gg[k]);
+    const double Fa = artisticAvgD(rr[k], gg[k]), den = 1 - Fa * (1 - E);
+    const double S = Fa * Fa * E / den, dSdE = Fa * Fa * (1 - Fa) / (den * den);
+    const double dSdF = (2 * Fa * E * den + Fa * Fa * E * (1 - E)) / (den * den);
+    g.v[k] = F * kF + S * A * B * w;
+    g.dAl[k] = F * dkF + (dSdE * dE * A * B + S * dA * B + S * A * dB) * w * drda;
+    double dF = 0.0;
+    if (rr[k] < 0.99)
+    {
+      const double lo = std::max(rr[k] - 1e-3, 0.0), hi = std::min(rr[k] + 1e-3, 0.99);
+      dF = (artisticFresnelD(cosOH, hi, gg[k]) - artisticFresnelD(cosOH, lo, gg[k])) / (hi - lo);
+    }
+    const double dFa = 0.782654 - 2 * 0.136432 * rr[k] + 3 * 0.278708 * rr[k] * rr[k] + 0.19744 * gg[k] +
+                       0.0360605 * gg[k] * gg[k] - 2 * 0.2586 * gg[k] * rr[k];
+    g.dA[k] = dF * kF + dSdF * dFa * A * B * w;
+  }
+  return g;
+}
+
+// Rough dielectric: value and alpha derivative (no albedo), single-scattering lobe through D, G2,
+// multiple-scattering lobe through the tables and fmsRatio.
+static GradD dielectricGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, D3 N, D3 U, D3 Vv, D3 o, D3 i)
+{
+  GradD g;
+  const double cosO = dotd(o, N);
+  if (!(cosO > 0.0)) return g;
+  const double cosI = dotd(i, N);
+  if (!(cosI * dotd(i, d3(ctx.Ng)) > 0.0)) return g;
+  const double eta = b.deta, r = b.roughness, a = std::max(r * r, 0.001);
+  const bool refl = cosI > 0.0;
+  D3 h = refl ? D3{o.x + i.x, o.y + i.y, o.z + i.z} : D3{eta * o.x + i.x, eta * o.y + i.y, eta * o.z + i.z};
+  h = normd(h);
+  if (dotd(h, N) < 0.0) h = D3{-h.x, -h.y, -h.z};
+  const double cosOH = dotd(o, h), cosIH = dotd(i, h);
+  const double F = fresnelDielD(cosOH, eta);
+  const D3 o0{dotd(o, U), dotd(o, Vv), dotd(o, N)}, i0{dotd(i, U), dotd(i, Vv), dotd(i, N)}, h0{dotd(h, U), dotd(h, Vv), dotd(h, N)};
+  const GgxD m = ggxD(a, o0, i0, h0);
+  double pre;
+  if (refl) pre = F / (4 * cosO);
+  else
+  {
+    const double den = cosOH + cosIH / eta;
+    pre = (1 - F) / (eta * eta) * std::fabs(cosIH) / (den * den) * std::fabs(cosOH) / cosO;
+  }
+  const double single = pre * m.D * m.G, dSingle = pre * (m.dD * m.G + m.D * m.dG);
+  const double Fe = fresnelDielAvgD(eta), Fr = fresnelDielAvgD(1 / eta);
+  const double Ee = dielAvgD(L, eta, r, false), Er = dielAvgD(L, 1 / eta, r, false);
+  const double dEe = dielAvgD(L, eta, r, true), dEr = dielAvgD(L, 1 / eta, r, true);
+  const double aa = (1 - Fe) / (1 - Er), bb = (1 - Fr) / (1 - Ee), x = bb / (aa + bb), ratio = 1 - x * (1 - Fe);
+  const double daa = (1 - Fe) / ((1 - Er) * (1 - Er)) * dEr, dbb = (1 - Fr) / ((1 - Ee) * (1 - Ee)) * dEe;
+  const double dRatio = -(1 - Fe) * (dbb * aa - bb * daa) / ((aa + bb) * (aa + bb));
+  const double Eo = dielDirD(L, cosO, eta, r, false), dEo = dielDirD(L, cosO, eta, r, true);
+  double fms, dFms;
+  if (refl)
+  {
+    const double Ei = dielDirD(L, cosI, eta, r, false), dEi = dielDirD(L, cosI, eta, r, true);
+    const double A = (1 - Eo) * (1 - Ei), dA = -dEo * (1 - Ei) - (1 - Eo) * dEi, B = 1 / (1 - Ee), dB = dEe * B * B;
+    fms = ratio * A * B * cosI / M_PI;
+    dFms = (dRatio * A * B + ratio * dA * B + ratio * A * dB) * cosI / M_PI;
+  }
+  else
+  {
+    const double ac = std::fabs(cosI);
+    const double Ei = dielDirD(L, ac, 1 / eta, r, false), dEi = dielDirD(L, ac, 1 / eta, r, true);
+    const double A = (1 - Eo) * (1 - Ei), dA = -dEo * (1 - Ei) - (1 - Eo) * dEi, B = 1 / (1 - Er), dB = dEr * B * B;
+    fms = (1 - ratio) * A * B * ac / M_PI;
+    dFms = (-dRatio * A * B + (1 - ratio) * dA * B + (1 - ratio) * A * dB) * ac / M_PI;
+  }
+  const double drda = (a > 0.001 && r > 0) ? 0.5 / r : 0.0;
+  for (int k = 0; k < 3; ++k)
+  {
+    g.v[k] = single + fms;
+    g.dAl[k] = dSingle + dFms * drda;
+  }
+  return g;
+}
+
 static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, const float* coatParams = nullptr)
 {
   GradD g;
@@ -464,6 +627,8 @@ static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, 
     for (int k = 0; k < 3; ++k) { g.v[k] = rho[k] / M_PI * c; g.dA[k] = c / M_PI; }
     return g;
   }
+  if (b.kind == kConductor) return conductorGradD(L, b, N, U, Vv, o, i);
+  if (b.kind == kDielectric) return dielectricGradD(L, b, ctx, N, U, Vv, o, i);
   if (b.kind != kCoated) return g;
   const double cosO = dotd(o, N);
   if (!(cosO > 0.0)) return g;
@@ -748,7 +913,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
         if (m.kind == kDielectric && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
           mediumId = (mediumId == mediumIntId) ? 0 : mediumIntId;
       }
-      if (adj && (m.kind == kDiffuse || m.kind == kCoated))
+      if (adj)
       {
         haveAv = true;
         av.mat = matId;
@@ -1081,6 +1246,7 @@ void orc_bsdf_grad(int n, const float* luts, const float* params, const float* g
     m.ior = 1.0f / pr[8]; m.thickness = pr[9]; m.weight = pr[10];
     Bsdf b = makeBsdf(L, m, ctx, wo, 0, -1);
     b.fr = ctx.f;
+    if (b.kind == kDielectric) b.deta = pr[8];  // the supplied eta as is, as orc_bsdf does
     GradD g = bsdfEvalGradD(L, b, ctx, wo, wi);
     double* o = out + 9 * size_t(i);
     for (int k = 0; k < 3; ++k) { o[k] = g.v[k]; o[3 + k] = g.dA[k]; o[6 + k] = g.dAl[k]; }

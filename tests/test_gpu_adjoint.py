@@ -87,6 +87,56 @@ def test_gradient_matches_fp64_oracle(dev):
     assert np.all(err <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
 
 
+def test_gradient_matches_fp64_oracle_conductor_dielectric(dev):
+    # coated, rough conductor and rough dielectric materials in turn: reflectivity and alpha of
+    # the conductors, alpha of the dielectrics (refraction into the spheres and out again)
+    W, H, spp = 64, 64, 4
+    sc = G.spheres_adjoint(n_spheres=24, subdiv=2, n_materials=9, width=W, height=H, mixed=True)
+    _load(dev, sc, G.perturbed_materials(sc["materials"]))
+    target = _render(dev, W, H, spp)
+    _load(dev, sc)
+    loss, grad, ms = dev.adjoint(target.reshape(H, W, 3), spp, sc["materials"].shape[0])
+    osc = O.Scene(sc["positions"], sc["indices"], sc["tri_material"], sc["materials"])
+    osc.set_lights(sc["lights"])
+    oloss, ograd = osc.adjoint(sc["camera"], W, H, target, spp, max_depth=4)
+    kinds = sc["materials"]["kind"]
+    print("loss", loss, oloss, "ms", ms)
+    assert abs(loss - oloss) <= 1e-4 * oloss
+    assert np.all(grad[kinds == G.KIND_DIELECTRIC, :3] == 0)
+    scale = np.abs(ograd).max(axis=0)
+    err = np.abs(grad - ograd)
+    for k in (G.KIND_CONDUCTOR, G.KIND_DIELECTRIC, G.KIND_COATED):
+        sel = kinds == k
+        print("kind", k, "max abs err / max |g| per (r, g, b, alpha):", err[sel].max(axis=0) / scale)
+        assert np.all(np.abs(ograd[sel, 3]) > 0)
+    assert np.all(err <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
+
+
+def test_conductor_reflectivity_matches_central_differences(dev):
+    # the conductor's sampling does not depend on its reflectivity: the adjoint equals central
+    # differences of the GPU-rendered loss
+    W, H, spp = 64, 64, 8
+    sc = G.spheres_adjoint(n_spheres=24, subdiv=2, n_materials=9, width=W, height=H, mixed=True)
+    _load(dev, sc, G.perturbed_materials(sc["materials"]))
+    target = _render(dev, W, H, spp)
+    _load(dev, sc)
+    _, grad, _ = dev.adjoint(target.reshape(H, W, 3), spp, sc["materials"].shape[0])
+    h = 1e-3
+    for mi in np.where(sc["materials"]["kind"] == G.KIND_CONDUCTOR)[0]:
+        for c, ch in enumerate(("r", "g", "b")):
+            mp, mm = sc["materials"].copy(), sc["materials"].copy()
+            mp[mi][ch] += h
+            mm[mi][ch] -= h
+            step = float(mp[mi][ch]) - float(mm[mi][ch])
+            _load(dev, sc, mp)
+            lp = np.mean((_render(dev, W, H, spp) - target) ** 2)
+            _load(dev, sc, mm)
+            lm = np.mean((_render(dev, W, H, spp) - target) ** 2)
+            fd = (lp - lm) / step
+            print(f"conductor {mi} {ch}: adjoint {grad[mi, c]:.6e} central difference {fd:.6e}")
+            assert abs(fd - grad[mi, c]) <= 1e-2 * abs(grad[mi, c]) + 1e-7
+
+
 def test_ground_albedo_matches_central_differences(dev):
     W, H, spp = 64, 64, 8
     sc = G.spheres_adjoint(n_spheres=24, subdiv=2, n_materials=8, width=W, height=H)
