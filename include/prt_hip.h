@@ -27,7 +27,7 @@ extern "C" {
 
 typedef struct prt_context prt_context;
 
-/* Material record (56 bytes).  Uniform StandardMaterial parameter sets of the stream path
+/* Material record (84 bytes; a zero field is the reference's default except where noted).  Uniform StandardMaterial parameter sets of the stream path
  * (material/standard.cpp:21-183 ctor, :409-597 SIMD lobes), each a MultiBsdf with one lobe:
  *   kind 0 diffuse    : baseColor = (r,g,b), specularWeight 0          -> DiffuseBrdf
  *   kind 1 conductor  : metalness 1, baseColor = (r,g,b), specularColor = (er,eg,eb),
@@ -42,7 +42,20 @@ typedef struct prt_context prt_context;
  *                       weight = w, ior = specularIOR (standard.cpp:512-518); a coat layer
  *                       over a diffuse base (specularWeight 0, coatWeight w) takes coatColor,
  *                       coatThickness, coatRoughness and coatIOR (standard.cpp:580-597).
- * thickness and weight are read by kind 3 only.
+ *   kind 4 thin dielectric : thinWalled 1, transmissionWeight 1, specularIOR = ior,
+ *                       specularRoughness = roughness, transmissionColor = (r,g,b),
+ *                       thickness / transmissionDepth = thickness -> ThinRoughDielectricBsdf
+ *                       (thin_rough_dielectric.h:170-345): glossy reflection plus an unbent
+ *                       transmission (a continuation lobe: the ray goes straight through).
+ * thickness is read by kinds 3 and 4, weight by kind 3 only.
+ * transparency = 1 - opacity (standard.cpp:152-157, :412-420, :613-627), any kind: a value
+ * above 1e-5 mixes the lobe (scale opacity) with a transparent BTDF (scale transparency,
+ * transparent.h) in a MultiBsdf.  A material with transparency > 0, and every kind 4, is
+ * transparent: shadow rays pass through it, attenuated by its transparency
+ * (render/transparent_filter.h:42-93).
+ * diffuse_roughness (kinds 0 and 3): above 0 the diffuse base is the Oren-Nayar BRDF of
+ * rough_diffuse.h (Fujii's form) instead of the Lambertian one (standard.cpp:446-453).
+ * sheen_weight, sheen_color, sheen_roughness: the sheen layer (standard.cpp:599-611, sheen.h).
  * emission = emissionColor * emissionLuminance (standard.cpp:71-74, :663-666), any kind: a
  * non-zero emission makes the material emissive, its triangles form the scene's area light
  * (light/triangle_light.h, render/scene.cpp:83-116) and a path that hits one ends there.
@@ -56,6 +69,11 @@ typedef struct prt_material {
   float thickness;
   float weight;
   float emission[3];
+  float transparency;
+  float diffuse_roughness;
+  float sheen_weight;
+  float sheen_color[3];
+  float sheen_roughness;
 } prt_material;
 
 /* Environment light record (32 bytes): light/ambient_light.h (kind 0, Ke = rgb),

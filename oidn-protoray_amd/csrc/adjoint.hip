@@ -63,7 +63,9 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
           if (g.value.z > 0.f) { gv.z += wgt.z * g.dAlbedo.z / g.value.z; gv.w += wgt.z * g.dAlpha.z / g.value.z; }
         };
         if (pv.nee) add(pv.wiL, pv.dirL);           // the next-event estimate made here
-        if (pv.ext) add(pv.wi, Ltotal - pv.L);      // everything gathered after it
+        // everything gathered after it, through a smooth lobe (a delta or continuation lobe's
+        // weight holds no differentiated parameter)
+        if (pv.ext && !(pv.type & BsdfType::Delta)) add(pv.wi, Ltotal - pv.L);
         recMat[pv.depth][threadIdx.x] = pv.matId;
         recG[pv.depth][threadIdx.x] = gv;
       };

@@ -244,6 +244,8 @@ bool isPowerOfTwo(int x) { return (x & (x - 1)) == 0; }
 
 struct prt_context
 {
+  int anyTransparent = 0;  // some material lets shadow rays through
+  int extMaterials = 0;    // some material is a thin dielectric, has opacity or an Oren-Nayar base
   int device = 0;
   hipStream_t stream = nullptr;
   hipDeviceProp_t props{};
@@ -465,7 +467,15 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
     prt_material_constants& q = k[i];
     std::memset(&q, 0, sizeof(q));
     const prt_material& s = m[i];
-    if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_COATED) return fail(PRT_ERR_INVALID, "unknown material kind");
+    if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_THIN_DIELECTRIC) return fail(PRT_ERR_INVALID, "unknown material kind");
+    if (s.sheen_weight != 0.f) return fail(PRT_ERR_INVALID, "the sheen layer is not supported");
+    if (!(s.transparency >= 0.f && s.transparency <= 1.f)) return fail(PRT_ERR_INVALID, "transparency must be in [0, 1]");
+    // opacity (material/standard.cpp:412-420, :613-627, eps = 1e-5, standard.h:79)
+    q.opacity = hclamp(1.f - s.transparency, 0.f, 1.f);
+    q.transp = 1.f - q.opacity;
+    q.mix = q.transp > 1e-5f ? (q.opacity > 1e-5f ? 1 : 2) : 0;
+    q.transparent = (s.transparency != 0.f || s.kind == PRT_KIND_THIN_DIELECTRIC) ? 1 : 0;  // :152-166
+    q.diffuseRoughness = hclamp(s.diffuse_roughness, 0.f, 1.f);
     q.kind = s.kind;
     q.roughness = hclamp(s.roughness, 0.f, 1.f);
     q.alpha = std::max(hsqr(q.roughness), 0.001f);  // bsdf/microfacet.h:10-24
@@ -528,6 +538,25 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
       q.Eavg = dielReflAvg(c->luts, eta, q.roughness) * scale;  // Ravg
       q.importance = std::max(1.f, baseImportance);              // rough_coating.h:562-565
     }
+    else if (s.kind == PRT_KIND_THIN_DIELECTRIC)
+    {
+      // ThinRoughDielectricBsdf (bsdf/thin_rough_dielectric.h:180-192); thin-walled, so always
+      // seen from outside: eta = ambientIor / specularIor (material/standard.cpp:429, :433-434)
+      float ior = s.ior < 1.f ? 1.0f / s.ior : s.ior;
+      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "dielectric IOR must differ from the ambient IOR 1");
+      float eta = hclamp(1.0f / ior, 1.f / 3.f, 3.f);
+      float thickness = std::max(s.thickness, 0.f);
+      q.Eavg = lutInterp1(q.roughness * 31.f, c->luts.data() + PRT_LUT_GGX_AVG, 32);
+      float Favg = fresnelAvg(eta);
+      q.c1[0] = Favg * (1.f - q.Eavg) / (1.f - Favg * q.Eavg);  // fmsScale
+      q.c2[0] = thickness;
+      q.eta[0] = q.eta[1] = eta;
+      for (int j = 0; j < 3; ++j)
+      {
+        q.c0[j] = rgb[j];
+        q.rr[j] = std::pow(rgb[j], thickness);  // getAlbedo (:327-330)
+      }
+    }
     else
     {
       float ior = s.ior < 1.f ? 1.0f / s.ior : s.ior;
@@ -553,6 +582,12 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
       if (id < 0) { iors.push_back(ior); id = int(iors.size()); }
       q.mediumInt = id;
     }
+  }
+  c->anyTransparent = c->extMaterials = 0;
+  for (int i = 0; i < n; ++i)
+  {
+    c->anyTransparent |= k[i].transparent;
+    c->extMaterials |= (k[i].kind == PRT_KIND_THIN_DIELECTRIC || k[i].mix != 0 || k[i].diffuseRoughness > 0.f) ? 1 : 0;
   }
   HIP_TRY(c->mats.alloc(n));
   HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * n, hipMemcpyHostToDevice));
@@ -887,6 +922,7 @@ static RenderArgs makeArgs(prt_context* c)
   RenderArgs a;
   std::memset(&a, 0, sizeof(a));
   a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.lut = c->lut.p;
+  a.anyTransparent = c->anyTransparent;
   a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
   a.nLights = int(c->lights.size());
   a.lightOffset = c->areaCount > 0 ? 1 : 0;
@@ -986,9 +1022,9 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
     {
       launchIntersect(ad, depth, c->launch, c->stream);
       HIP_TRY(hipEventRecord(ev[2 + 3 * depth], c->stream));
-      launchShade(ad, depth, c->sampleIndex, c->launch, c->stream);
+      launchShade(ad, depth, c->sampleIndex, c->launch, c->extMaterials != 0, c->stream);
       HIP_TRY(hipEventRecord(ev[3 + 3 * depth], c->stream));
-      launchOccluded(ad, depth, c->launch, c->stream);
+      launchOccluded(ad, depth, c->launch, c->anyTransparent != 0, c->stream);
       HIP_TRY(hipEventRecord(ev[4 + 3 * depth], c->stream));
       if (depth + 1 >= kFreeDepths && depth < a.maxDepth)
       {

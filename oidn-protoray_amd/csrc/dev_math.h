@@ -403,20 +403,57 @@ struct Bsdf
   float Eavg;     // conductor: average albedo
   float eta, EavgEta, EavgRcpEta, fmsRatio;  // dielectric (oriented)
   float importance;
+  float diffRough;            // kinds 0 and 3: Oren-Nayar base when > 0
+  int mix;                    // opacity mix: 0 lobe, 1 lobe + transparent BTDF, 2 transparent
+  float opacity, transp;
+  // false: the scene has no thin dielectric, opacity or Oren-Nayar base; a compile-time
+  // constant in the kernels built for such scenes, where it folds those branches away
+  bool ext;
 };
 
-PRT_DEV f3 bsdfAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
+// The lobe's albedo (multi_bsdf.h:314-337 with one child).
+PRT_DEV f3 lobeAlbedo(const Bsdf& b)
 {
   if (b.kind == PRT_KIND_DIFFUSE) return b.c0;
-  return (b.kind == PRT_KIND_DIELECTRIC) ? f3(1.f) : b.rr;  // conductor reflectivity | coated albedo
+  return (b.kind == PRT_KIND_DIELECTRIC) ? f3(1.f) : b.rr;  // conductor reflectivity | coated, thin albedo
+}
+// The albedo AOV of the whole BSDF: the opacity mix ignores the transparent BTDF's negative
+// albedo and rescales (multi_bsdf.h:339-362); transparent alone returns -1 (transparent.h:52-55).
+PRT_DEV f3 bsdfAlbedo(const Bsdf& b)
+{
+  if (b.mix == 2) return f3(-1.f);
+  if (b.mix == 1 && !(b.importance * b.opacity > 0.f)) return f3(0.f);
+  return lobeAlbedo(b);
+}
+// The normal AOV: the first child with a non-zero importance (multi_bsdf.h:408-428); the
+// transparent BTDF has no frame.
+PRT_DEV f3 bsdfNormal(const Bsdf& b)
+{
+  return (b.mix != 2 && b.importance * (b.mix == 1 ? b.opacity : 1.f) > 0.f) ? b.fr.N : f3(0.f);
 }
 
-// bsdf/diffuse.h:55-68
-PRT_DEV f3 diffuseEval(const Bsdf& b, f3 wi, float& pdf)
+// bsdf/diffuse.h:55-68, or with diffRough > 0 the Oren-Nayar BRDF in Fujii's form
+// (bsdf/rough_diffuse.h:65-75)
+PRT_DEV f3 diffuseEval(const Bsdf& b, f3 wo, f3 wi, float& pdf)
 {
   float c = fmaxf(dot(wi, b.fr.N), 0.f);
   pdf = c * (1.f / kPi);
-  return b.c0 * (1.f / kPi) * c;
+  if (!(b.diffRough > 0.f)) return b.c0 * (1.f / kPi) * c;
+  float cO = fmaxf(dot(wo, b.fr.N), 0.f);
+  float s = dot(wo, wi) - cO * c;
+  float sInvt = s > 0.f ? s * rcpSafe(fmaxf(cO, c)) : s;
+  return b.c0 * (c * (1.f + b.diffRough * sInvt) * (1.0f / (kPi + (kPi / 2.f - 2.f / 3.f) * b.diffRough)));
+}
+// Cosine sampling of the diffuse base (diffuse.h:61-67, rough_diffuse.h:77-87): the weight,
+// zero below the geometric surface.
+PRT_DEV f3 diffuseSample(const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy)
+{
+  wi = toGlobal(b.fr, cosineSampleHemisphere(pdf, sx, sy));
+  if (!(dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  type = BsdfType::DiffuseReflection;
+  if (!(b.diffRough > 0.f)) return b.c0;
+  f3 v = diffuseEval(b, wo, wi, pdf);
+  return v * (1.0f / pdf);
 }
 // bsdf/rough_conductor.h:132-202
 PRT_DEV f3 conductorF(const Bsdf& b, float c)
@@ -600,7 +637,7 @@ PRT_DEV f3 coatEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi,
   float cosI = dot(wi, b.fr.N);
   if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
   float basePdf;
-  f3 baseValue = diffuseEval(b, wi, basePdf);
+  f3 baseValue = diffuseEval(b, wo, wi, basePdf);
   float Ro = dielReflAlbedo(lut, cosO, b.eta, b.roughness) * b.c2.y;
   baseValue = coatBase(lut, b, baseValue, cosO, cosI, Ro);
   float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
@@ -630,15 +667,14 @@ PRT_DEV f3 coatSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
     wi = reflect(wo, wh, dot(wo, wh));
     if (!(dot(wi, b.fr.N) > 0.f)) return f3(0.f);
     type = BsdfType::GlossyReflection;
-    baseValue = diffuseEval(b, wi, basePdf);
+    baseValue = diffuseEval(b, wo, wi, basePdf);
   }
   else
   {
     // the diffuse base reads only the 2D part of the remapped sample (:424-427)
-    wi = toGlobal(b.fr, cosineSampleHemisphere(basePdf, sx, sy));
-    if (!(dot(wi, c.Ng) > 0.f && fmaxf(b.c0.x, fmaxf(b.c0.y, b.c0.z)) > 0.f)) return f3(0.f);
-    type = BsdfType::DiffuseReflection;
-    baseValue = b.c0 * basePdf;
+    f3 bw = diffuseSample(b, c, wo, wi, basePdf, type, sx, sy);
+    if (!(reduceMax(bw) > 0.f)) return f3(0.f);
+    baseValue = bw * basePdf;
     wh = normalize(wo + wi);
   }
   float cosI = dot(wi, b.fr.N);
@@ -655,40 +691,187 @@ PRT_DEV f3 coatSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
   return (f3(coatValue) + baseValue) * (1.0f / pdf);
 }
 
-// MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210)
-PRT_DEV f3 bsdfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+// --- thin rough dielectric (bsdf/thin_rough_dielectric.h:170-345): c0 = transmission color,
+// c1.x = fmsScale, c2.x = thickness, Eavg = GGX average albedo, eta <= 1.  Reflection is a GGX
+// lobe with energy compensation; transmission goes straight through (unbent, a continuation
+// lobe) with probability 1 - R.
+PRT_DEV float thinReflectance(const float* lut, const Bsdf& b, float cosO, float& Eo)
+{
+  Eo = ggxAlbedo(lut, cosO, b.roughness);
+  return dielReflAlbedo(lut, cosO, b.eta, b.roughness) + b.c1.x * (1.f - Eo);  // :262-264
+}
+PRT_DEV f3 thinTransmittance(const Bsdf& b, float cosO)  // pow(color, thickness / cosT) (:319)
+{
+  float e = (1.0f / refractCos(cosO, b.eta)) * b.c2.x;
+  return f3(powf(b.c0.x, e), powf(b.c0.y, e), powf(b.c0.z, e));
+}
+PRT_DEV f3 thinEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)  // :194-240
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f) || !(cosI > 0.f)) return f3(0.f);
+  f3 wh = normalize(wi + wo);
+  float cosOH = dot(wo, wh);
+  f3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  float F = fresnelDielectric(cosOH, b.eta);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  float Eo, Ei = ggxAlbedo(lut, cosI, b.roughness);
+  float R = thinReflectance(lut, b, cosO, Eo);
+  float fms = b.c1.x * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.Eavg))) * cosI);
+  pdf = whPdf * (1.0f / (4.f * fabsf(cosOH))) * R;
+  return f3(F * D * G * (1.0f / (4.f * cosO)) + fms);
+}
+PRT_DEV f3 thinSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)  // :242-305
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float Eo;
+  float R = thinReflectance(lut, b, cosO, Eo);
+  pdf = 0.f;
+  wi = f3(0.f);
+  if (sc <= R)
+  {
+    f3 wo0 = toLocal(b.fr, wo);
+    float whPdf;
+    f3 wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+    wi = reflect(wo, wh);
+    float cosI = dot(wi, b.fr.N);
+    if (!(cosI > 0.f && dot(wi, c.Ng) > 0.f)) return f3(0.f);
+    float cosOH = dot(wo, wh);
+    f3 wi0 = toLocal(b.fr, wi);
+    float F = fresnelDielectric(cosOH, b.eta);
+    float G = b.ggx.G2(wo0, wi0);
+    float Ei = ggxAlbedo(lut, cosI, b.roughness);
+    float fms = b.c1.x * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.Eavg))) * cosI);
+    type = BsdfType::GlossyReflection;
+    pdf = whPdf * (1.0f / (4.f * fabsf(cosOH))) * R;
+    return f3(F * G * rcpSafe(b.ggx.G1(wo0)) * (1.0f / R) + fms * (1.0f / pdf));
+  }
+  wi = -wo;
+  if (!(dot(wi, c.Ng) < 0.f)) return f3(0.f);
+  type = BsdfType::UnbentTransmission;
+  pdf = 1.f - R;
+  return thinTransmittance(b, cosO);
+}
+// getTransparency (:307-321): what a shadow ray keeps through the surface
+PRT_DEV f3 thinTransparency(const float* lut, const Bsdf& b, f3 wo)
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float Eo;
+  float R = thinReflectance(lut, b, cosO, Eo);
+  return thinTransmittance(b, cosO) * (1.f - R);
+}
+
+// The lobe: a MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210)
+PRT_DEV f3 lobeEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
 {
   f3 v(0.f);
   pdf = 0.f;
   if (b.importance > 0.f)
   {
     float cp;
-    if (b.kind == PRT_KIND_DIFFUSE) v = diffuseEval(b, wi, cp);
+    if (b.kind == PRT_KIND_DIFFUSE) v = diffuseEval(b, wo, wi, cp);
     else if (b.kind == PRT_KIND_CONDUCTOR) v = conductorEval(lut, b, wo, wi, cp);
     else if (b.kind == PRT_KIND_COATED) v = coatEval(lut, b, c, wo, wi, cp);
+    else if (b.ext && b.kind == PRT_KIND_THIN_DIELECTRIC) v = thinEval(lut, b, c, wo, wi, cp);
     else v = dielEval(lut, b, c, wo, wi, cp);
     pdf = cp * b.importance;
   }
   pdf *= 1.0f / b.importance;
   return v;
 }
-PRT_DEV f3 bsdfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+PRT_DEV f3 lobeSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
 {
   if (!(b.importance > 0.f)) return f3(0.f);
-  if (b.kind == PRT_KIND_DIFFUSE)  // bsdf/diffuse.h:61-67
-  {
-    wi = toGlobal(b.fr, cosineSampleHemisphere(pdf, sx, sy));
-    type = BsdfType::DiffuseReflection;
-    return dot(wi, c.Ng) > 0.f ? b.c0 : f3(0.f);
-  }
+  if (b.kind == PRT_KIND_DIFFUSE) return diffuseSample(b, c, wo, wi, pdf, type, sx, sy);
   if (b.kind == PRT_KIND_CONDUCTOR) return conductorSample(lut, b, c, wo, wi, pdf, type, sx, sy);
   if (b.kind == PRT_KIND_COATED) return coatSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+  if (b.ext && b.kind == PRT_KIND_THIN_DIELECTRIC) return thinSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
   return dielSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+}
+
+// The whole BSDF with the opacity mix (material/standard.cpp:613-627): MultiBsdf{lobe x opacity,
+// transparent BTDF x transp} (multi_bsdf.h:166-287, transparent.h:40-45).  The transparent
+// BTDF evaluates to zero and has importance 1.
+PRT_DEV f3 bsdfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  if (b.mix == 0) return lobeEval(lut, b, c, wo, wi, pdf);
+  pdf = 0.f;
+  if (b.mix == 2) return f3(0.f);
+  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  f3 v(0.f);
+  if (imp0 > 0.f)
+  {
+    float cp;
+    v = lobeEval(lut, b, c, wo, wi, cp) * b.opacity;
+    pdf = cp * imp0;
+  }
+  pdf *= 1.0f / sum;
+  return v;
+}
+PRT_DEV f3 bsdfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  if (b.mix == 0) return lobeSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+  if (b.mix == 2)
+  {
+    wi = -wo;
+    type = BsdfType::Transparency;
+    pdf = 1.f;
+    return f3(1.f);
+  }
+  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  if (!(sum > 0.f)) return f3(0.f);
+  // choose a child by importance and remap the sample (:219-245)
+  const float x = sc * sum;
+  const bool lobe = !(x > imp0);
+  const float imp = lobe ? imp0 : b.transp, prefix = lobe ? imp0 : sum;
+  const float sc1 = (x + imp - prefix) * (1.0f / imp);
+  f3 w(0.f);
+  wi = f3(0.f);
+  pdf = 0.f;
+  type = 0;
+  if (lobe)
+  {
+    f3 cwi(0.f);
+    float cpdf = 0.f;
+    int ctype = 0;
+    f3 cw = lobeSample(lut, b, c, wo, cwi, cpdf, ctype, sx, sy, sc1);
+    if (reduceMax(cw) > 0.f) { w = cw * b.opacity; wi = cwi; pdf = cpdf; type = ctype; }
+  }
+  else
+  {
+    w = f3(b.transp);
+    wi = -wo;
+    pdf = 1.f;
+    type = BsdfType::Transparency;
+  }
+  if (!(reduceMax(w) > 0.f && pdf > 0.f)) return f3(0.f);
+  if (type & BsdfType::Delta) return w * (sum * (1.0f / imp));  // (:262-272)
+  // a smooth sample of the lobe: the transparent BTDF adds nothing to value or pdf (:274-287)
+  f3 value = w * pdf;
+  pdf = pdf * imp * (1.0f / sum);
+  return value * (1.0f / pdf);
+}
+// getTransparency of a material whose shadow rays pass through (material/standard.cpp:633-661):
+// a thin dielectric's lobe transmits, every other lobe is opaque, the transparent BTDF passes 1.
+PRT_DEV f3 bsdfTransparency(const float* lut, const Bsdf& b, f3 wo)
+{
+  if (b.mix == 2) return f3(1.f);
+  f3 t = (b.kind == PRT_KIND_THIN_DIELECTRIC && b.importance > 0.f) ? thinTransparency(lut, b, wo) : f3(0.f);
+  if (b.mix == 0) return t;
+  return t * b.opacity + f3(b.transp);
 }
 
 // Builds the BSDF for a hit from the material constants (prt_material_constants, computed on
 // the host from the formulas of rough_conductor.h:121-130 and rough_dielectric.h:279-291).
-PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool inside)
+// ext = false (for scenes without thin dielectrics, opacity or Oren-Nayar bases) drops those
+// paths at compile time where the argument is a constant.
+PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool inside, bool ext = true)
 {
   Bsdf b;
   b.fr = shadingFrame(c, wo);
@@ -706,6 +889,11 @@ PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool
   b.EavgEta = m.EavgEta[o];
   b.EavgRcpEta = m.EavgRcpEta[o];
   b.fmsRatio = m.fmsRatio[o];
+  b.ext = ext;
+  b.diffRough = ext ? m.diffuseRoughness : 0.f;
+  b.mix = ext ? m.mix : 0;
+  b.opacity = m.opacity;
+  b.transp = m.transp;
   return b;
 }
 

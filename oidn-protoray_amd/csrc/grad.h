@@ -2,7 +2,8 @@
 // BSDF b and directions (wo, wi), bsdfEvalGrad returns the value bsdfEval returns (f * cos)
 // and its partial derivatives with respect to the material's albedo (per colour channel: the
 // diffuse base colour) and its GGX alpha (alpha = roughness^2).  Differentiated lobes:
-//   diffuse (bsdf/diffuse.h:55-68): value = rho / pi * cos.
+//   diffuse (bsdf/diffuse.h:55-68): value = rho / pi * cos, or with a diffuse roughness the
+//     Oren-Nayar value (bsdf/rough_diffuse.h:65-75), also rho times a factor of the directions.
 //   coated diffuse (bsdf/rough_coating.h:297-378 over bsdf/diffuse.h): value =
 //     F s D G2 / (4 cos_o) [cos_i > 0]  +  rho / pi * cos_i * absorb * (1 - Ro)(1 - Ri) / (1 - Ravg)
 //     with Ro, Ri, Ravg from the dielectric reflection albedo tables (linear in roughness
@@ -15,6 +16,7 @@
 //     analytically), alpha through D, G2 and the albedo tables Eo, Ei, Eavg.
 //   rough dielectric (bsdf/rough_dielectric.h:296-526): alpha only (no albedo), through D, G2 of
 //     the single-scattering lobe and the tables and fmsRatio of the multiple-scattering lobe.
+//   thin dielectric: not differentiated (zero derivatives).
 // Table derivatives are those of the linear interpolation in roughness (zero where clamped).
 #pragma once
 
@@ -215,15 +217,20 @@ PRT_DEV BsdfGrad bsdfEvalGrad(const float* lut, const Bsdf& b, const prt_materia
   BsdfGrad g;
   g.value = g.dAlbedo = g.dAlpha = f3(0.f);
   if (!(b.importance > 0.f)) return g;
+  // the diffuse base per unit albedo: Lambert, or Oren-Nayar (linear in the albedo too)
+  Bsdf unitBase = b;
+  unitBase.c0 = f3(1.f);
+  float basePdf;
+  const float kBase = diffuseEval(unitBase, wo, wi, basePdf).x;
   if (b.kind == PRT_KIND_DIFFUSE)
   {
-    float cosI = fmaxf(dot(wi, b.fr.N), 0.f);
-    g.value = b.c0 * (1.f / kPi) * cosI;
-    g.dAlbedo = f3((1.f / kPi) * cosI);
+    g.value = b.c0 * kBase;
+    g.dAlbedo = f3(kBase);
     return g;
   }
   if (b.kind == PRT_KIND_CONDUCTOR) return conductorEvalGrad(lut, b, m, wo, wi);
   if (b.kind == PRT_KIND_DIELECTRIC) return dielectricEvalGrad(lut, b, m, c, wo, wi);
+  if (b.kind != PRT_KIND_COATED) return g;  // thin dielectric: not differentiated
   const float cosO = dot(wo, b.fr.N);
   if (!(cosO > 0.f)) return g;
   const float cosI = dot(wi, b.fr.N);
@@ -247,7 +254,7 @@ PRT_DEV BsdfGrad bsdfEvalGrad(const float* lut, const Bsdf& b, const prt_materia
   const float dTdr = (-dRo * (1.f - Ri) - (1.f - Ro) * dRi) * inv + (1.f - Ro) * (1.f - Ri) * dRavg * inv * inv;
   // alpha = roughness^2 (alpha above its 0.001 floor): d/dalpha = d/droughness / (2 roughness)
   const float drda = (a > 0.001f && r > 0.f) ? 0.5f / r : 0.f;
-  const f3 unit = absorb * ((1.f / kPi) * fmaxf(cosI, 0.f));
+  const f3 unit = absorb * kBase;
   g.value = b.c0 * unit * T;
   g.dAlbedo = unit * T;
   g.dAlpha = b.c0 * unit * (dTdr * drda);

@@ -38,6 +38,7 @@ struct RenderArgs
   const prt_material_constants* mats;
   const float* lut;
   BvhView bvh;
+  int anyTransparent;  // some material lets shadow rays through (render/scene.cpp:452-462)
   prt_light_dev lights[PRT_MAX_LIGHTS];  // environment lights
   float lightCdf[PRT_MAX_LIGHTS + 2];    // power distribution over [area light,] environment lights
   int nLights;                           // environment lights
@@ -109,8 +110,8 @@ struct LaunchCfg
 // ~1 KB record by value would make every lane copy it to scratch.
 void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStream_t s);
 void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s);
-void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, hipStream_t s);
-void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s);
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, bool ext, hipStream_t s);
+void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, bool filter, hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
                  int* spill, int spillStride, int grid, hipStream_t s);
 // Path-replay adjoint (adjoint.hip): partial is [blocks][nParams] (zeroed by the caller before the

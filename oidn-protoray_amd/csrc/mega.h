@@ -30,6 +30,7 @@ struct PathVertex
   f3 wiL;     // its light direction
   bool ext;   // the path continues along wi
   f3 wi;
+  int type;   // the sampled lobe's type (BsdfType)
   f3 L;       // radiance gathered so far, through this vertex's next-event estimate
 };
 
@@ -91,6 +92,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     // next-event estimate (:659-718) with its shadow ray (:951-1002)
     f3 dirL(0.f), wiL(0.f);
     bool nee = false;
+    if (b.mix != 2)  // a smooth lobe (all but the transparent BTDF)
     {
       float s0, s1, s2;
       smp.get3D(sampleDim + 3, s0, s1, s2);
@@ -112,7 +114,13 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
           float w = misWeight(lightPdf, bsdfPdf);
           dirL = dirT * w * lw;
           float ts, us = 0.f, vs = 0.f;
-          nee = traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris) < 0;
+          if (a.anyTransparent)
+          {
+            ShadowFilter flt{&a, ctx.p, wiL, f3(1.f)};
+            nee = traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris, flt) < 0;
+            dirL = dirL * flt.T;
+          }
+          else nee = traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris) < 0;
         }
       }
     }
@@ -128,7 +136,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     if (mAov1)
     {
       aov1 = true;
-      if (fh) { fh->albedo = bsdfAlbedo(b); fh->normal = b.fr.N; }
+      if (fh) { fh->albedo = bsdfAlbedo(b); fh->normal = bsdfNormal(b); }
     }
     mAov2 = mAov2 && (extType & BsdfType::Delta) == 0;
     if (mAov2) aov2 = true;
@@ -143,7 +151,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     }
     // the shadow ray's radiance joins after the shading of this depth, as in the wavefront kernels
     if (nee) L = L + dirL;
-    PathVertex pv{depth, matId, &m, &b, &ctx, wo, nee, dirL, wiL, ext, wi, L};
+    PathVertex pv{depth, matId, &m, &b, &ctx, wo, nee, dirL, wiL, ext, wi, type, L};
     hook(pv);
     if (!ext) break;
     o = ctx.p;

@@ -120,6 +120,25 @@ PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, fl
   return c;
 }
 
+// Transparent-shadow filter (render/transparent_filter.h:42-93): a hit on a transparent
+// material multiplies the shadow ray's transmittance T by the material's transparency seen
+// along -d and lets the ray go on until T falls below 1e-5; a hit on any other material
+// occludes.  The product does not depend on the order in which the hits are found.
+struct ShadowFilter
+{
+  const RenderArgs* a;
+  f3 o, d, T;
+  PRT_DEV bool operator()(int prim, float t, float u, float v)
+  {
+    const prt_material_constants& m = a->mats[a->triMat[prim]];
+    if (!m.transparent) { T = f3(0.f); return true; }
+    Ctx c = postIntersect(*a, o, d, t, prim, u, v);
+    Bsdf b = makeBsdf(m, c, -d, false);
+    T = T * bsdfTransparency(a->lut, b, -d);
+    return reduceMax(T) < 1e-5f;
+  }
+};
+
 
 // Camera ray of pixel (x, y) for sample sampleIndex (path_renderer_stream.h:209-268): pixel
 // jitter from dimension set 0, lens sample from set 2.

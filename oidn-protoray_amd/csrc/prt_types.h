@@ -7,6 +7,7 @@
 #define PRT_KIND_CONDUCTOR 1
 #define PRT_KIND_DIELECTRIC 2
 #define PRT_KIND_COATED 3
+#define PRT_KIND_THIN_DIELECTRIC 4
 
 #define PRT_LIGHT_AMBIENT 0
 #define PRT_LIGHT_DISTANT 1
@@ -38,12 +39,14 @@ struct prt_material_constants
   float roughness;   // clamped specularRoughness
   float alpha;       // max(roughness^2, 0.001)
   float importance;  // MultiBsdf importance of the single lobe (diffuse: max(color))
-  float c0[3];       // diffuse color | conductor eta | coated: diffuse color
-  float c1[3];       // conductor k | coated: coat color (transmittance)
+  float c0[3];       // diffuse color | conductor eta | coated: diffuse color | thin: color
+  float c1[3];       // conductor k | coated: coat color (transmittance) | thin: fmsScale in x
   float c2[3];       // conductor fmsScale | coated: thickness, weight (scale), diffuse importance
-  float rr[3];       // conductor reflectivity | coated: albedo (the albedo AOV value)
-  float Eavg;        // conductor average albedo | coated: Ravg (average coat reflectance x scale)
-  float eta[2];      // dielectric eta seen from outside [0] / inside [1] | coated: eta (<= 1)
+                     // | thin: thickness in x
+  float rr[3];       // conductor reflectivity | coated, thin: albedo (the albedo AOV value)
+  float Eavg;        // conductor, thin: GGX average albedo | coated: Ravg (average coat
+                     // reflectance x scale)
+  float eta[2];      // dielectric eta seen from outside [0] / inside [1] | coated, thin: eta (<= 1)
   float EavgEta[2];
   float EavgRcpEta[2];
   float fmsRatio[2];
@@ -55,8 +58,15 @@ struct prt_material_constants
   float Favg[3];       // conductor Kulla-Conty average Fresnel
   float FavgEta[2];    // dielectric average Fresnel at eta, outside [0] / inside [1]
   float FavgRcpEta[2]; // dielectric average Fresnel at 1 / eta
+  // opacity mix (material/standard.cpp:412-420, :613-627): mix 0 = the lobe alone, 1 = the lobe
+  // (scale opacity) and a transparent BTDF (scale transp) in a MultiBsdf, 2 = transparent only
+  int32_t mix;
+  float opacity, transp;
+  int32_t transparent;     // shadow rays pass through, attenuated (render/transparent_filter.h)
+  float diffuseRoughness;  // kinds 0 and 3: Oren-Nayar base when > 0 (bsdf/rough_diffuse.h)
+  float pad[3];
 };
-static_assert(sizeof(prt_material_constants) == 160, "material constants: 160 bytes");
+static_assert(sizeof(prt_material_constants) == 192, "material constants: 192 bytes");
 
 // Environment light record as the kernels read it.  64 bytes.
 struct prt_light_dev

@@ -136,6 +136,9 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __re
 #ifndef PRT_SHADE_WAVES_PER_EU
 #define PRT_SHADE_WAVES_PER_EU 4
 #endif
+// kExt: the scene has thin dielectrics, opacity or Oren-Nayar bases (the kernel without them
+// keeps the register budget of the four original lobes).
+template <bool kExt>
 __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(PRT_SHADE_WAVES_PER_EU)))
 kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
 {
@@ -232,13 +235,14 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
         f3 wo = -d;
         int mediumId = fl >> PRT_F_MEDIUM_SHIFT;
         bool inside = (m.kind == PRT_KIND_DIELECTRIC) && (mediumId == m.mediumInt);
-        Bsdf b = makeBsdf(m, ctx, wo, inside);
+        Bsdf b = makeBsdf(m, ctx, wo, inside, kExt);
         Sampler smp;
         smp.set(a.seed, sampleIndex, pixelId);  // resetSample (:641-642)
         bool mAov1 = !(fl & PRT_F_AOV1), mAov2 = !(fl & PRT_F_AOV2);
         bool mRR = !(mAov1 || mAov2);
-        // direct light (:659-718); every supported lobe is Smooth
+        // direct light (:659-718) when the BSDF has a smooth lobe: all but the transparent BTDF
         f3 dirL(0.f);
+        if (b.mix != 2)
         {
           float s0, s1, s2;
           smp.get3D(sampleDim + 3, s0, s1, s2);
@@ -286,7 +290,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
         if (mAov1)  // first-hit AOVs (:739-819)
         {
           addAov3(a.fb.albedo1, a.fb.albedo1W, a.npix, pix, bsdfAlbedo(b));
-          addAov3(a.fb.normal1, a.fb.normal1W, a.npix, pix, b.fr.N);
+          addAov3(a.fb.normal1, a.fb.normal1W, a.npix, pix, bsdfNormal(b));
           if (a.fb.depth)
           {
             a.fb.depth[pix] += dot(ctx.p - ld3(a.camPos), -ld3(a.camN));
@@ -298,7 +302,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
         if (mAov2)
         {
           addAov3(a.fb.albedo, a.fb.albedoW, a.npix, pix, bsdfAlbedo(b));
-          addAov3(a.fb.normal, a.fb.normalW, a.npix, pix, b.fr.N);
+          addAov3(a.fb.normal, a.fb.normalW, a.npix, pix, bsdfNormal(b));
           fl |= PRT_F_AOV2;
         }
         T = T * bw;
@@ -339,6 +343,10 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
   }
 }
 
+// Shadow rays of depth `depth`.  kFilter: the scene has transparent materials, and the shadow
+// rays gather the transmittance of the transparent surfaces they cross (path_renderer_stream.h:
+// 951-1002 with the TransparentShadowFilter of render/transparent_filter.h).
+template <bool kFilter>
 __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth)
 {
   const RenderArgs& a = *ap;
@@ -352,14 +360,23 @@ __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __res
     int slot = a.shadowQ[i];
     float4 so = a.shO[slot], sd = a.shD[slot];
     float t, u, v;
-    int prim = traceRay<true>(a.bvh, f3(so.x, so.y, so.z), f3(sd.x, sd.y, sd.z), so.w, sd.w, t, u, v, st, nodes, tris);
+    const f3 o(so.x, so.y, so.z), d(sd.x, sd.y, sd.z);
+    int prim;
+    f3 Tsh(1.f);
+    if (kFilter)
+    {
+      ShadowFilter flt{&a, o, d, f3(1.f)};
+      prim = traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris, flt);
+      Tsh = flt.T;
+    }
+    else prim = traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris);
     ++rays;
     float4 lr = a.Lr[slot];
     f3 L(lr.x, lr.y, lr.z);
-    if (prim < 0)  // unoccluded: add the direct light (:991-1002)
+    if (prim < 0)  // unoccluded: add the direct light times the shadow transmittance (:991-1002)
     {
       float4 dl = a.dirL[slot];
-      L = L + f3(dl.x, dl.y, dl.z);
+      L = L + (kFilter ? f3(dl.x, dl.y, dl.z) * Tsh : f3(dl.x, dl.y, dl.z));
       a.Lr[slot] = make_float4(L.x, L.y, L.z, 0.f);
     }
     uint32_t hsh = 0u;
@@ -425,13 +442,15 @@ void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, hipStre
 {
   kIntersect<<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
 }
-void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, hipStream_t s)
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, bool ext, hipStream_t s)
 {
-  kShade<<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex);
+  if (ext) kShade<true><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex);
+  else kShade<false><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex);
 }
-void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s)
+void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, bool filter, hipStream_t s)
 {
-  kOccluded<<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  if (filter) kOccluded<true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  else kOccluded<false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
 }
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit, int* spill,
                  int spillStride, int grid, hipStream_t s)

@@ -978,8 +978,9 @@ MeshFile loadMesh(const std::string& path)
 
 // ---------------------------------------------------------------------------------------
 // Standard parameters -> material record.  The record models one lobe set per material
-// (diffuse, rough conductor, rough dielectric, coated diffuse; emission on any); parameters
-// outside that set are dropped and the material counted as approximated.
+// (diffuse or Oren-Nayar, rough conductor, rough dielectric, coated diffuse, thin rough
+// dielectric; opacity and emission on any); parameters outside that set are dropped and the
+// material counted as approximated.
 // ---------------------------------------------------------------------------------------
 bool emissive(const StdMat& m)
 {
@@ -1012,8 +1013,10 @@ prt_material toRecord(const StdMat& m, bool& approx)
   const float ior = m.specularIor > 0.f && m.specularIor < 1.f ? 1.f / m.specularIor : m.specularIor;
   const bool mixedMetal = m.metalness > 0.f && m.metalness < 1.f;
   const bool mixedTrans = m.transmissionWeight > 0.f && m.transmissionWeight < 1.f;
-  approx = m.textured || mixedMetal || mixedTrans || m.sheenWeight > 0.f || m.subsurfaceWeight > 0.f || m.opacity < 1.f ||
-           m.diffuseRoughness > 0.f || (m.baseWeight != 1.f && m.metalness < 1.f);
+  approx = m.textured || mixedMetal || mixedTrans || m.sheenWeight > 0.f || m.subsurfaceWeight > 0.f ||
+           (m.baseWeight != 1.f && m.metalness < 1.f);
+  // opacity (material/standard.cpp:152-157, :613-627) applies to every lobe set
+  r.transparency = 1.f - std::min(std::max(m.opacity, 0.f), 1.f);
   if (m.metalness >= 0.5f)
   {
     r.kind = 1;
@@ -1023,6 +1026,18 @@ prt_material toRecord(const StdMat& m, bool& approx)
     approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f;
     return r;
   }
+  if (m.transmissionWeight >= 0.5f && m.thinWalled)
+  {
+    // thin-walled transmission: ThinRoughDielectricBsdf with transmissionColor and thickness /
+    // transmissionDepth, thickness 1 (material/standard.cpp:534-543, :48-52, :79-80)
+    r.kind = 4;
+    r.ior = std::fabs(ior - 1.f) > 1e-6f ? ior : 1.5f;
+    r.roughness = m.specularRoughness;
+    r.r = m.transmissionColor[0]; r.g = m.transmissionColor[1]; r.b = m.transmissionColor[2];
+    r.thickness = m.transmissionDepth > 0.f ? 1.f / m.transmissionDepth : 3.4e38f;
+    approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f || std::fabs(ior - 1.f) <= 1e-6f;
+    return r;
+  }
   if (m.transmissionWeight >= 0.5f)
   {
     r.kind = 2;
@@ -1030,13 +1045,14 @@ prt_material toRecord(const StdMat& m, bool& approx)
     r.roughness = m.specularRoughness;
     // the medium inside: transmissionColor ^ transmissionDepth... kept white (not modelled here)
     r.r = r.g = r.b = 1.f;
-    approx = approx || m.thinWalled || m.coatWeight > 0.f || m.specularRoughness <= 0.f || std::fabs(ior - 1.f) <= 1e-6f ||
+    approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f || std::fabs(ior - 1.f) <= 1e-6f ||
              m.transmissionColor[0] != 1.f || m.transmissionColor[1] != 1.f || m.transmissionColor[2] != 1.f;
     return r;
   }
   r.r = m.baseColor[0] * m.baseWeight;
   r.g = m.baseColor[1] * m.baseWeight;
   r.b = m.baseColor[2] * m.baseWeight;
+  r.diffuse_roughness = std::min(std::max(m.diffuseRoughness, 0.f), 1.f);  // Oren-Nayar base (:446-453)
   const bool specLayer = m.specularWeight > 0.f && std::fabs(ior - 1.f) > 1e-6f;
   const bool coatLayer = m.coatWeight > 0.f && std::fabs(m.coatIor - 1.f) > 1e-6f;
   if (specLayer)

@@ -120,12 +120,20 @@ PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int 
   return st;
 }
 
+// Any-hit filter that accepts every hit (a plain occlusion test).
+struct AcceptHit
+{
+  PRT_DEV bool operator()(int, float, float, float) const { return true; }
+};
+
 // Closest hit (kAnyHit = false) or any hit.  Ties in t go to the smaller primitive ID, so the
 // result does not depend on the traversal order.  Returns the primitive ID or -1; nodeVisits
-// counts wide-node fetches.
-template <bool kAnyHit>
+// counts wide-node fetches.  An any-hit traversal offers each hit in [tnear, tfar] to
+// filter(prim, t, u, v), which accepts it (the ray is occluded) or lets the ray go on, as
+// Embree's filter callbacks do (accel/embree_intersector.cpp:154-215).
+template <bool kAnyHit, class Filter = AcceptHit>
 PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, float& tHit, float& uHit, float& vHit,
-                     TraceStack& st, uint32_t& nodeVisits, uint32_t& triTests)
+                     TraceStack& st, uint32_t& nodeVisits, uint32_t& triTests, Filter&& filter = Filter())
 {
   f3 id(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   int best = -1;
@@ -168,7 +176,11 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
         if (intersectTriangle(a, b, cc, o, d, tnear, tfar, th, u, v))
         {
           int prim = __float_as_int(a.w);
-          if (kAnyHit) { tHit = th; return prim; }
+          if (kAnyHit)
+          {
+            if (filter(prim, th, u, v)) { tHit = th; return prim; }
+            continue;
+          }
           if (th < tfar || best < 0 || prim < best)
           {
             tfar = th; best = prim; uHit = u; vHit = v;

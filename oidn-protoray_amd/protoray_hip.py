@@ -24,7 +24,9 @@ BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_SNAPSHOT = 0, 1, 2, 16
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
-                           ("emission", "<f4", 3)])
+                           ("emission", "<f4", 3), ("transparency", "<f4"),
+                           ("diffuse_roughness", "<f4"), ("sheen_weight", "<f4"), ("sheen_color", "<f4", 3),
+                           ("sheen_roughness", "<f4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),

@@ -37,13 +37,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
-                           ("emission", "<f4", 3)])
+                           ("emission", "<f4", 3), ("transparency", "<f4"),
+                           ("diffuse_roughness", "<f4"), ("sheen_weight", "<f4"), ("sheen_color", "<f4", 3),
+                           ("sheen_roughness", "<f4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
                          ("fov", "<f4"), ("nearClip", "<f4"), ("lensRadius", "<f4"),
                          ("focalDistance", "<f4")])
-KIND_DIFFUSE, KIND_CONDUCTOR, KIND_DIELECTRIC, KIND_COATED = 0, 1, 2, 3
+KIND_DIFFUSE, KIND_CONDUCTOR, KIND_DIELECTRIC, KIND_COATED, KIND_THIN_DIELECTRIC = 0, 1, 2, 3, 4
 LIGHT_AMBIENT, LIGHT_DISTANT, LIGHT_IMAGE = 0, 1, 2
 
 
@@ -121,8 +123,11 @@ def gold_artistic(rng):
     return r, g
 
 
-def _material(kind, rgb=(0, 0, 0), edge=(0, 0, 0), roughness=0.0, ior=1.5, thickness=0.0, weight=0.0):
+def _material(kind, rgb=(0, 0, 0), edge=(0, 0, 0), roughness=0.0, ior=1.5, thickness=0.0, weight=0.0,
+              transparency=0.0, diffuse_roughness=0.0):
     m = np.zeros((), MATERIAL_DTYPE)
+    m["transparency"] = transparency
+    m["diffuse_roughness"] = diffuse_roughness
     m["kind"] = kind
     m["r"], m["g"], m["b"] = rgb
     m["er"], m["eg"], m["eb"] = edge
@@ -286,6 +291,48 @@ def spheres_sunsky(seed=0, n_spheres=4096, subdiv=4, width=1920, height=1080, sp
                 materials=np.array(mats, MATERIAL_DTYPE), lights=lights, env=ss["sky"],
                 camera=_camera(lens_radius=0.05, focal=15.0), width=width, height=height, spp=spp,
                 max_depth=max_depth, seed=0)
+
+
+def spheres_lobes(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4, max_depth=8, palette=24):
+    """The sun/sky scene of spheres_sunsky with the remaining StandardMaterial lobes: rough
+    diffuse (Oren-Nayar) bases, coats over them, thin rough dielectrics (thin-walled bubbles with
+    tinted transmission), and opacity < 1 on diffuse, conductor and coated materials (one fully
+    transparent); shadow rays pass through the thin and transparent ones."""
+    rng = Pcg32(seed)
+    mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5), diffuse_roughness=0.5)]
+    for i in range(palette):
+        v = i % 6
+        rgb = (rng.f(0.1, 0.9), rng.f(0.1, 0.9), rng.f(0.1, 0.9))
+        rough = math.sqrt(rng.f(0.02, 0.4))
+        if v == 0:
+            mats.append(_material(KIND_DIFFUSE, rgb, diffuse_roughness=rng.f(0.2, 1.0)))
+        elif v == 1:
+            mats.append(_material(KIND_COATED, rgb, (1, 1, 1), rough, rng.f(1.3, 1.8), thickness=1.0, weight=1.0,
+                                  diffuse_roughness=rng.f(0.2, 1.0)))
+        elif v == 2:
+            tint = (rng.f(0.4, 1.0), rng.f(0.4, 1.0), rng.f(0.4, 1.0))
+            mats.append(_material(KIND_THIN_DIELECTRIC, tint, roughness=rough, ior=rng.f(1.3, 1.8),
+                                  thickness=rng.f(0.2, 2.0)))
+        elif v == 3:
+            mats.append(_material(KIND_DIFFUSE, rgb, transparency=rng.f(0.2, 0.8)))
+        elif v == 4:
+            r, g = gold_artistic(rng)
+            mats.append(_material(KIND_CONDUCTOR, r, g, rough, transparency=1.0 if i == 4 else rng.f(0.2, 0.8)))
+        else:
+            mats.append(_material(KIND_COATED, rgb, (1, 1, 1), rough, 1.5, thickness=1.0, weight=1.0,
+                                  transparency=rng.f(0.2, 0.8)))
+    pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, lambda i: 2 + rng.u32() % palette)
+    ss = load_sunsky()
+    d = np.array([-0.6, 0.7, -0.4])
+    d /= np.linalg.norm(d)
+    lights = np.zeros(3, LIGHT_DTYPE)
+    lights[0] = (LIGHT_DISTANT, 300.0, 280.0, 250.0, d[0], d[1], d[2], 0.5 * math.radians(2.0))
+    sr, sd = ss["sun_radiance"], ss["sun_dir"]
+    lights[1] = (LIGHT_DISTANT, sr[0], sr[1], sr[2], sd[0], sd[1], sd[2], ss["sun_radius"])
+    lights[2] = (LIGHT_IMAGE, 1.0, 1.0, 1.0, 0, 0, 0, 0)
+    return dict(name="spheres_lobes", positions=pos, indices=idx, tri_material=tri_mat,
+                materials=np.array(mats, MATERIAL_DTYPE), lights=lights, env=ss["sky"],
+                camera=_camera(), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)
 
 
 def relmse(x, ref):

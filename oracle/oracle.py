@@ -32,7 +32,9 @@ _lib = None
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
-                           ("emission", "<f4", 3)])
+                           ("emission", "<f4", 3), ("transparency", "<f4"),
+                           ("diffuse_roughness", "<f4"), ("sheen_weight", "<f4"), ("sheen_color", "<f4", 3),
+                           ("sheen_roughness", "<f4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
@@ -83,6 +85,7 @@ def lib():
         L.orc_adjoint.restype = ctypes.c_double
         L.orc_adjoint.argtypes = [vp, vp, i, i, i, u, i, i, f, vp, i, vp]
         L.orc_bsdf_grad.argtypes = [i, vp, vp, vp, vp]
+        L.orc_bsdf_mat.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -170,6 +173,22 @@ def bsdf(luts_flat, params, geo, s):
     lib().orc_bsdf(n, c32(luts_flat).ctypes.data, params.ctypes.data, geo.ctypes.data, s.ctypes.data,
                    so.ctypes.data, st.ctypes.data, ev.ctypes.data)
     return so, st, ev
+
+
+def bsdf_mat(luts_flat, mats, geo, s):
+    """The renderer's whole BSDF for material records (MATERIAL_DTYPE) on supplied frames:
+    (sample n x 7 [weight wi pdf], type n, eval n x 4 [value pdf], aux n x 9 [albedo, normal,
+    transparency of a transparent material])."""
+    mats = np.ascontiguousarray(mats, MATERIAL_DTYPE)
+    geo, s = c32(geo), c32(s)
+    n = mats.shape[0]
+    so = np.zeros((n, 7), np.float32)
+    st = np.zeros(n, np.int32)
+    ev = np.zeros((n, 4), np.float32)
+    aux = np.zeros((n, 9), np.float32)
+    lib().orc_bsdf_mat(n, c32(luts_flat).ctypes.data, mats.ctypes.data, geo.ctypes.data, s.ctypes.data,
+                       so.ctypes.data, st.ctypes.data, ev.ctypes.data, aux.ctypes.data)
+    return so, st, ev, aux
 
 
 def bsdf_grad(luts_flat, params, geo):

@@ -541,9 +541,14 @@ enum : int {
 //                                                           -> RoughConductorBrdf<FresnelConductorArtistic>
 //   2 dielectric : transmissionWeight = 1, specularIOR, specularRoughness
 //                                                           -> RoughDielectricBsdf + media interface
-enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2, kCoated = 3 };
+//   4 thin dielectric : thinWalled, transmissionWeight = 1, specularIOR, specularRoughness,
+//                      transmissionColor = r, thickness / transmissionDepth = thickness
+//                                                           -> ThinRoughDielectricBsdf
+// Any kind: transparency = 1 - opacity mixes in a TransparentBtdf; diffuseRoughness (kinds 0
+// and 3) selects the Oren-Nayar base.
+enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2, kCoated = 3, kThinDielectric = 4 };
 
-struct Material  // 32-byte record shared with the HIP path's material table
+struct Material  // 84-byte record, the prt_material layout of the HIP path
 {
   int kind;
   float r, g, b;     // albedo (diffuse) / reflectivity (conductor)
@@ -553,7 +558,12 @@ struct Material  // 32-byte record shared with the HIP path's material table
   float thickness;   // coated: coat thickness (absorption path scale)
   float weight;      // coated: coat weight (specularWeight / coatWeight)
   float emission[3]; // emissionColor * emissionLuminance; non-zero makes the material an area light
+  float transparency;      // 1 - opacity
+  float diffuseRoughness;  // Oren-Nayar base of kinds 0 and 3 when > 0
+  float sheenWeight, sheenColor[3], sheenRoughness;  // the sheen layer (not built)
 };
+// MaterialType::Transparent (material/standard.cpp:152-166): shadow rays pass through
+inline bool isTransparent(const Material& m) { return m.transparency != 0.f || m.kind == kThinDielectric; }
 
 inline bool isEmissive(const Material& m) { return m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f; }
 
@@ -609,6 +619,11 @@ struct Bsdf
   // eta <= 1, Ravg = average coat reflectance * scale, importance of the diffuse base
   V3 coatColor;
   float thickness, scale, ceta, Ravg, baseImportance;
+  // thin dielectric: color = transmission color, eta <= 1, GGX Eavg, fmsScale, thickness
+  float teta, tEavg, tfms, tthick;
+  float diffRough;  // Oren-Nayar base when > 0
+  int mix;          // 0 the lobe, 1 lobe x opacity + transparent BTDF x transp, 2 transparent only
+  float opacity, transp;
   float roughness;
   GGX ggx{0.001f};
   float importance;
@@ -622,6 +637,12 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
   float roughness = clampf(m.roughness, 0.f, 1.f);
   b.roughness = roughness;
   b.ggx = GGX(roughnessToAlpha(roughness));
+  b.diffRough = clampf(m.diffuseRoughness, 0.f, 1.f);
+  // opacity (material/standard.cpp:412-420, :613-627; eps = 1e-5, standard.h:79)
+  b.opacity = clampf(1.f - m.transparency, 0.f, 1.f);
+  b.transp = 1.f - b.opacity;
+  b.mix = b.transp > 1e-5f ? (b.opacity > 1e-5f ? 1 : 2) : 0;
+  b.teta = b.tEavg = b.tfms = b.tthick = 0.f;
   if (m.kind == kDiffuse)
   {
     b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
@@ -665,7 +686,7 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
     b.flags = BsdfType::GlossyReflection | BsdfType::GlossyTransmission;
     b.importance = 1.f;
   }
-  else
+  else if (m.kind == kCoated)
   {
     // coated diffuse (material/standard.cpp:512-518, 580-597): RoughCoatingBsdf over a diffuse
     // base; opaque materials are never media interfaces, so fromOutside holds and
@@ -683,21 +704,44 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
     b.flags = BsdfType::DiffuseReflection | BsdfType::GlossyReflection;
     b.importance = std::max(1.f, b.baseImportance);  // rough_coating.h:562-565
   }
+  else if (m.kind == kThinDielectric)
+  {
+    // thin_rough_dielectric.h:180-192; thin-walled, always seen from outside (standard.cpp:429)
+    float ior = m.ior < 1.f ? 1.0f / m.ior : m.ior;
+    b.teta = clampf(1.0f / ior, 1.f / 3.f, 3.f);
+    b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
+    b.tthick = std::max(m.thickness, 0.f);
+    b.tEavg = ggxAlbedoAvg(L, roughness);
+    float Favg = fresnelDielectricAvg(b.teta);
+    b.tfms = Favg * (1.f - b.tEavg) / (1.f - Favg * b.tEavg);
+    b.flags = BsdfType::GlossyReflection | BsdfType::UnbentTransmission;
+    b.importance = 1.f;
+  }
+  if (b.mix == 1) b.flags |= BsdfType::Transparency;
+  if (b.mix == 2) b.flags = BsdfType::Transparency;
   return b;
 }
 
-// --- diffuse (bsdf/diffuse.h:55-68)
-inline V3 diffuseEval(const Bsdf& b, V3 wi, float& pdf)
+// --- diffuse (bsdf/diffuse.h:55-68), or the Oren-Nayar BRDF in Fujii's form when diffRough > 0
+// (bsdf/rough_diffuse.h:65-87)
+inline V3 diffuseEval(const Bsdf& b, V3 wo, V3 wi, float& pdf)
 {
   float c = std::max(dot(wi, b.fr.N), 0.f);
   pdf = cosineSampleHemispherePdf(c);
-  return b.color * (1.f / kPi) * c;
+  if (!(b.diffRough > 0.f)) return b.color * (1.f / kPi) * c;
+  float cO = std::max(dot(wo, b.fr.N), 0.f);
+  float s = dot(wo, wi) - cO * c;
+  float sInvt = s > 0.f ? s * rcpSafe(std::max(cO, c)) : s;
+  return b.color * (c * (1.f + b.diffRough * sInvt) * (1.0f / (kPi + (kPi / 2.f - 2.f / 3.f) * b.diffRough)));
 }
-inline V3 diffuseSample(const Bsdf& b, const Ctx& ctx, V3& wi, float& pdf, int& type, float sx, float sy)
+inline V3 diffuseSample(const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy)
 {
   wi = toGlobal(b.fr, cosineSampleHemisphere(pdf, sx, sy));
+  if (!(dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
   type = BsdfType::DiffuseReflection;
-  return dot(wi, ctx.Ng) > 0.f ? b.color : V3(0.f);
+  if (!(b.diffRough > 0.f)) return b.color;
+  V3 v = diffuseEval(b, wo, wi, pdf);
+  return v * (1.0f / pdf);
 }
 
 // --- rough conductor (bsdf/rough_conductor.h:132-202)
@@ -896,7 +940,7 @@ inline V3 coatEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, f
   float cosI = dot(wi, b.fr.N);
   if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
   float basePdf;
-  V3 baseValue = diffuseEval(b, wi, basePdf);
+  V3 baseValue = diffuseEval(b, wo, wi, basePdf);
   float Ro = dielReflAlbedo(L, cosO, b.ceta, b.roughness) * b.scale;
   baseValue = coatBase(L, b, baseValue, cosO, cosI, Ro);
   float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
@@ -929,7 +973,7 @@ inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
     wi = reflect(wo, wh, dot(wo, wh));
     if (!(dot(wi, b.fr.N) > 0.f)) return V3(0.f);
     type = BsdfType::GlossyReflection;
-    baseValue = diffuseEval(b, wi, basePdf);
+    baseValue = diffuseEval(b, wo, wi, basePdf);
   }
   else
   {
@@ -937,7 +981,7 @@ inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
     V3 baseWi;
     float basePdf2;
     int baseType;
-    V3 baseWeight = diffuseSample(b, ctx, baseWi, basePdf2, baseType, sx, sy);
+    V3 baseWeight = diffuseSample(b, ctx, wo, baseWi, basePdf2, baseType, sx, sy);
     if (!(reduceMax(baseWeight) > 0.f)) return V3(0.f);
     wi = baseWi;
     basePdf = basePdf2;
@@ -959,34 +1003,108 @@ inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
   return (V3(coatValue) + baseValue) * (1.0f / pdf);
 }
 
-// MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210): eval pdf is scaled by
-// importance / importanceSum; sample returns the child's weight times the child scale (1);
-// a zero importance sum disables sampling.
-inline V3 bsdfEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+// --- thin rough dielectric (bsdf/thin_rough_dielectric.h:170-345)
+inline float thinReflectance(const Luts& L, const Bsdf& b, float cosO, float& Eo)  // :262-264
+{
+  Eo = ggxAlbedo(L, cosO, b.roughness);
+  return dielReflAlbedo(L, cosO, b.teta, b.roughness) + b.tfms * (1.f - Eo);
+}
+inline V3 thinTransmittance(const Bsdf& b, float cosO)  // :319
+{
+  float e = (1.0f / refractCos(cosO, b.teta)) * b.tthick;
+  return V3(std::pow(b.color.x, e), std::pow(b.color.y, e), std::pow(b.color.z, e));
+}
+inline V3 thinEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)  // :194-240
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f) || !(cosI > 0.f)) return V3(0.f);
+  V3 wh = normalize(wi + wo);
+  float cosOH = dot(wo, wh);
+  V3 wo0 = toLocal(b.fr, wo), wi0 = toLocal(b.fr, wi), wh0 = toLocal(b.fr, wh);
+  float F = fresnelDielectric(cosOH, b.teta);
+  float whPdf;
+  float D = b.ggx.evalVisible(wh0, wo0, cosOH, whPdf);
+  float G = b.ggx.G2(wo0, wi0);
+  float Eo, Ei = ggxAlbedo(L, cosI, b.roughness);
+  float R = thinReflectance(L, b, cosO, Eo);
+  float fms = b.tfms * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.tEavg))) * cosI);
+  pdf = whPdf * (1.0f / (4.f * std::fabs(cosOH))) * R;
+  return V3(F * D * G * (1.0f / (4.f * cosO)) + fms);
+}
+inline V3 thinSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)  // :242-305
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float Eo;
+  float R = thinReflectance(L, b, cosO, Eo);
+  pdf = 0.f;
+  wi = V3(0.f);
+  if (sc <= R)
+  {
+    V3 wo0 = toLocal(b.fr, wo);
+    float whPdf;
+    V3 wh = toGlobal(b.fr, b.ggx.sampleVisible(wo0, whPdf, sx, sy));
+    wi = reflect(wo, wh);
+    float cosI = dot(wi, b.fr.N);
+    if (!(cosI > 0.f && dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+    float cosOH = dot(wo, wh);
+    V3 wi0 = toLocal(b.fr, wi);
+    float F = fresnelDielectric(cosOH, b.teta);
+    float G = b.ggx.G2(wo0, wi0);
+    float Ei = ggxAlbedo(L, cosI, b.roughness);
+    float fms = b.tfms * ((1.f - Eo) * (1.f - Ei) * (1.0f / (kPi * (1.f - b.tEavg))) * cosI);
+    type = BsdfType::GlossyReflection;
+    pdf = whPdf * (1.0f / (4.f * std::fabs(cosOH))) * R;
+    return V3(F * G * rcpSafe(b.ggx.G1(wo0)) * (1.0f / R) + fms * (1.0f / pdf));
+  }
+  wi = -wo;
+  if (!(dot(wi, ctx.Ng) < 0.f)) return V3(0.f);
+  type = BsdfType::UnbentTransmission;
+  pdf = 1.f - R;
+  return thinTransmittance(b, cosO);
+}
+inline V3 thinTransparency(const Luts& L, const Bsdf& b, V3 wo)  // :307-321
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float Eo;
+  float R = thinReflectance(L, b, cosO, Eo);
+  return thinTransmittance(b, cosO) * (1.f - R);
+}
+
+// The lobe: MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210): eval pdf is
+// scaled by importance / importanceSum; sample returns the child's weight times the child scale
+// (1); a zero importance sum disables sampling.
+inline V3 lobeEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
 {
   V3 v(0.f);
   pdf = 0.f;
   if (b.importance > 0.f)
   {
     float cp = 0.f;
-    if (b.kind == kDiffuse) v = diffuseEval(b, wi, cp);
+    if (b.kind == kDiffuse) v = diffuseEval(b, wo, wi, cp);
     else if (b.kind == kConductor) v = conductorEval(L, b, wo, wi, cp);
     else if (b.kind == kCoated) v = coatEval(L, b, ctx, wo, wi, cp);
+    else if (b.kind == kThinDielectric) v = thinEval(L, b, ctx, wo, wi, cp);
     else v = dielEval(L, b, ctx, wo, wi, cp);
     pdf = cp * b.importance;
   }
   pdf *= 1.0f / b.importance;
   return v;
 }
-inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+inline V3 lobeSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
 {
   if (!(b.importance > 0.f)) return V3(0.f);
-  if (b.kind == kDiffuse) return diffuseSample(b, ctx, wi, pdf, type, sx, sy);
+  if (b.kind == kDiffuse) return diffuseSample(b, ctx, wo, wi, pdf, type, sx, sy);
   if (b.kind == kConductor) return conductorSample(L, b, ctx, wo, wi, pdf, type, sx, sy);
   if (b.kind == kCoated) return coatSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
+  if (b.kind == kThinDielectric) return thinSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
   return dielSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
 }
-inline V3 bsdfAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
+inline V3 lobeAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
 {
   if (b.kind == kDiffuse) return b.color;
   if (b.kind == kConductor) return b.rr;
@@ -996,7 +1114,91 @@ inline V3 bsdfAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
                  b.color.z * std::pow(b.coatColor.z, b.thickness));
     return b.color + b.scale * (absorbed - b.color);
   }
+  if (b.kind == kThinDielectric)  // thin_rough_dielectric.h:324-327
+    return V3(std::pow(b.color.x, b.tthick), std::pow(b.color.y, b.tthick), std::pow(b.color.z, b.tthick));
   return V3(1.f);
+}
+
+// The whole BSDF with the opacity mix (material/standard.cpp:613-627): MultiBsdf{lobe x opacity,
+// TransparentBtdf x transp} (multi_bsdf.h:166-287, transparent.h:40-45); the transparent BTDF
+// evaluates to zero with importance 1; alone (opacity <= 1e-5) it passes every ray straight on.
+inline V3 bsdfEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  if (b.mix == 0) return lobeEval(L, b, ctx, wo, wi, pdf);
+  pdf = 0.f;
+  if (b.mix == 2) return V3(0.f);
+  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  V3 v(0.f);
+  if (imp0 > 0.f)
+  {
+    float cp;
+    v = lobeEval(L, b, ctx, wo, wi, cp) * b.opacity;
+    pdf = cp * imp0;
+  }
+  pdf *= 1.0f / sum;
+  return v;
+}
+inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  if (b.mix == 0) return lobeSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
+  if (b.mix == 2)
+  {
+    wi = -wo;
+    type = BsdfType::Transparency;
+    pdf = 1.f;
+    return V3(1.f);
+  }
+  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  if (!(sum > 0.f)) return V3(0.f);
+  // pick a child by importance, remap the sample (:219-245)
+  const float x = sc * sum;
+  const bool lobe = !(x > imp0);
+  const float imp = lobe ? imp0 : b.transp, prefix = lobe ? imp0 : sum;
+  const float sc1 = (x + imp - prefix) * (1.0f / imp);
+  V3 w(0.f);
+  wi = V3(0.f);
+  pdf = 0.f;
+  type = 0;
+  if (lobe)
+  {
+    V3 cwi(0.f);
+    float cpdf = 0.f;
+    int ctype = 0;
+    V3 cw = lobeSample(L, b, ctx, wo, cwi, cpdf, ctype, sx, sy, sc1);
+    if (reduceMax(cw) > 0.f) { w = cw * b.opacity; wi = cwi; pdf = cpdf; type = ctype; }
+  }
+  else
+  {
+    w = V3(b.transp);
+    wi = -wo;
+    pdf = 1.f;
+    type = BsdfType::Transparency;
+  }
+  if (!(reduceMax(w) > 0.f && pdf > 0.f)) return V3(0.f);
+  if (type & BsdfType::Delta) return w * (sum * (1.0f / imp));  // (:262-272)
+  V3 value = w * pdf;  // (:274-287): the transparent BTDF adds nothing
+  pdf = pdf * imp * (1.0f / sum);
+  return value * (1.0f / pdf);
+}
+// getAlbedo (multi_bsdf.h:339-362): negative albedos (the transparent BTDF) are left out
+inline V3 bsdfAlbedo(const Bsdf& b)
+{
+  if (b.mix == 2) return V3(-1.f);
+  if (b.mix == 1 && !(b.importance * b.opacity > 0.f)) return V3(0.f);
+  return lobeAlbedo(b);
+}
+// getNormal (multi_bsdf.h:408-428): the first child with a non-zero importance
+inline V3 bsdfNormal(const Bsdf& b)
+{
+  return (b.mix != 2 && b.importance * (b.mix == 1 ? b.opacity : 1.f) > 0.f) ? b.fr.N : V3(0.f);
+}
+// getTransparency of a transparent material (material/standard.cpp:633-661)
+inline V3 bsdfTransparency(const Luts& L, const Bsdf& b, V3 wo)
+{
+  if (b.mix == 2) return V3(1.f);
+  V3 t = (b.kind == kThinDielectric && b.importance > 0.f) ? thinTransparency(L, b, wo) : V3(0.f);
+  if (b.mix == 0) return t;
+  return t * b.opacity + V3(b.transp);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -41,12 +41,15 @@ struct Scene
   // clear ambient medium (material/material_factory.cpp:20-21,159-175); transmission color is
   // 1 for every supported dielectric, so distinct IORs get IDs 1, 2, ... in material order.
   std::vector<int> matMedium;
+  bool anyTransparent = false;  // render/scene.cpp:452-462: shadow rays use the transparent filter
 };
 
 static void assignMedia(Scene& s)
 {
   std::vector<float> iors;
   s.matMedium.assign(s.mats.size(), 0);
+  s.anyTransparent = false;
+  for (size_t m = 0; m < s.mats.size(); ++m) s.anyTransparent = s.anyTransparent || isTransparent(s.mats[m]);
   for (size_t m = 0; m < s.mats.size(); ++m)
   {
     if (s.mats[m].kind != kDielectric) continue;
@@ -353,6 +356,51 @@ static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u,
   return c;
 }
 
+// Any-hit traversal with the transparent-shadow filter (render/transparent_filter.h:42-93):
+// each hit on a transparent material multiplies T by the material's transparency along -d and
+// the ray goes on until T falls below 1e-5; a hit on any other material occludes (T = 0).
+static void traverseShadow(const Scene& s, RayHit& r, V3& T)
+{
+  T = V3(1.f);
+  V3 inv(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  r.prim = -1;
+  while (sp)
+  {
+    const Node& n = s.nodes[stack[--sp]];
+    float te;
+    if (!boxHit(n, r.o, inv, r.tnear, r.tfar, te)) continue;
+    if (n.count > 0)
+    {
+      for (int i = 0; i < n.count; ++i)
+      {
+        const Tri& tr = s.tris[n.first + i];
+        float t, u, v;
+        if (!intersectTri(tr, r.o, r.d, r.tnear, r.tfar, t, u, v)) continue;
+        const Material& m = s.mats[s.triMat[tr.prim]];
+        if (isTransparent(m))
+        {
+          Ctx c = postIntersect(s, r.o, r.d, t, tr.prim, u, v);
+          Bsdf b = makeBsdf(s.luts, m, c, -r.d, 0, -1);
+          T *= bsdfTransparency(s.luts, b, -r.d);
+          if (!(reduceMax(T) < 1e-5f)) continue;
+        }
+        else T = V3(0.f);
+        r.prim = tr.prim;
+        r.tfar = t;
+        return;
+      }
+    }
+    else
+    {
+      stack[sp++] = n.first + 1;
+      stack[sp++] = n.first;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Integrator
 // ---------------------------------------------------------------------------------------
@@ -620,11 +668,23 @@ static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, 
   GradD g;
   if (!(b.importance > 0.f)) return g;
   const D3 N = d3(b.fr.N), U = d3(b.fr.U), Vv = d3(b.fr.V), o = d3(wo), i = d3(wi);
+  // the diffuse base per unit albedo: Lambert, or Oren-Nayar in Fujii's form (rough_diffuse.h:65-75)
+  double kBase;
+  {
+    const double c = std::max(dotd(i, N), 0.0);
+    if (!(b.diffRough > 0.f)) kBase = c / M_PI;
+    else
+    {
+      const double cO = std::max(dotd(o, N), 0.0), sr = b.diffRough;
+      const double sdot = dotd(o, i) - cO * c;
+      const double sInvt = sdot > 0.0 ? sdot / std::max(cO, c) : sdot;
+      kBase = c * (1.0 + sr * sInvt) / (M_PI + (M_PI / 2.0 - 2.0 / 3.0) * sr);
+    }
+  }
   if (b.kind == kDiffuse)
   {
-    double c = std::max(dotd(i, N), 0.0);
     double rho[3] = {b.color.x, b.color.y, b.color.z};
-    for (int k = 0; k < 3; ++k) { g.v[k] = rho[k] / M_PI * c; g.dA[k] = c / M_PI; }
+    for (int k = 0; k < 3; ++k) { g.v[k] = rho[k] * kBase; g.dA[k] = kBase; }
     return g;
   }
   if (b.kind == kConductor) return conductorGradD(L, b, N, U, Vv, o, i);
@@ -653,7 +713,7 @@ static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, 
   const double drda = (a > 0.001 && r > 0) ? 0.5 / r : 0.0;
   for (int k = 0; k < 3; ++k)
   {
-    double unit = (1.0 + s * (std::pow(coat[k], t) - 1.0)) * std::max(cosI, 0.0) / M_PI;
+    double unit = (1.0 + s * (std::pow(coat[k], t) - 1.0)) * kBase;
     g.v[k] = rho[k] * unit * T;
     g.dA[k] = unit * T;
     g.dAl[k] = rho[k] * unit * dTdr * drda;
@@ -884,7 +944,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       if (mAov1)  // first-hit AOVs (:739-819)
       {
         add3(A.albedo1, A.albedo1W, npix, pix, bsdfAlbedo(b));
-        add3(A.normal1, A.normal1W, npix, pix, b.fr.N);
+        add3(A.normal1, A.normal1W, npix, pix, bsdfNormal(b));
         if (A.depth) { A.depth[pix] += dot(ctx.p - V3(cam.pos[0], cam.pos[1], cam.pos[2]), -camN); A.depthW[pix] += 1.f; }
         aov1Done = true;
       }
@@ -892,7 +952,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       if (mAov2)
       {
         add3(A.albedo, A.albedoW, npix, pix, bsdfAlbedo(b));
-        add3(A.normal, A.normalW, npix, pix, b.fr.N);
+        add3(A.normal, A.normalW, npix, pix, bsdfNormal(b));
         aov2Done = true;
       }
 
@@ -917,9 +977,10 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       {
         haveAv = true;
         av.mat = matId;
-        av.ext = ext;
         av.dirL = dirL;
-        if (ext) av.gExt = bsdfEvalGradD(s.luts, b, ctx, wo, wi);
+        // a delta (continuation) sample carries no differentiated BSDF value
+        av.ext = ext && !(type & BsdfType::Delta);
+        if (av.ext) av.gExt = bsdfEvalGradD(s.luts, b, ctx, wo, wi);
       }
     }
 
@@ -928,10 +989,16 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
     {
       RayHit sr;
       sr.o = shadowO; sr.d = shadowD; sr.tnear = shadowNear; sr.tfar = shadowFar;
-      traverse(s, sr, true, nullptr);
+      V3 Tsh(1.f);
+      if (s.anyTransparent) traverseShadow(s, sr, Tsh);
+      else traverse(s, sr, true, nullptr);
       st.rays++;
       hash = hashStep(hash, sr.prim < 0 ? 1u : 2u);  // the visibility event is part of the path
-      if (sr.prim < 0) L += dirL;
+      if (sr.prim < 0)
+      {
+        if (s.anyTransparent) { dirL *= Tsh; av.dirL = dirL; }  // the shadow ray's transmittance
+        L += dirL;
+      }
       av.nee = sr.prim < 0;
     }
     if (haveAv)
@@ -1405,6 +1472,42 @@ void orc_bsdf(int n, const float* luts, const float* params, const float* geo, c
     V3 alb = bsdfAlbedo(built);
     float* e = outEval + 7 * i;
     e[0] = ev.x; e[1] = ev.y; e[2] = ev.z; e[3] = epdf; e[4] = alb.x; e[5] = alb.y; e[6] = alb.z;
+  }
+}
+
+// The whole BSDF of a material record as the renderer builds it (makeBsdf from outside, the
+// opacity mix included) on a supplied frame: mats n x Material, geo n x 18, s n x 3.
+// outSample n x 7 [weight wi pdf], outType n, outEval n x 4 [value pdf], outAux n x 9
+// [albedo, normal, transparency].
+void orc_bsdf_mat(int n, const float* luts, const void* mats, const float* geo, const float* s, float* outSample, int* outType,
+                  float* outEval, float* outAux)
+{
+  const Luts& L = *(const Luts*)luts;
+  const Material* ms = (const Material*)mats;
+  for (int i = 0; i < n; ++i)
+  {
+    const float* g = geo + 18 * i;
+    Ctx ctx;
+    ctx.f.U = V3(g[0], g[1], g[2]); ctx.f.V = V3(g[3], g[4], g[5]); ctx.f.N = V3(g[6], g[7], g[8]);
+    ctx.Ng = V3(g[9], g[10], g[11]);
+    V3 wo(g[12], g[13], g[14]), wi(g[15], g[16], g[17]);
+    Bsdf b = makeBsdf(L, ms[i], ctx, wo, 0, -1);
+    b.fr = ctx.f;
+    V3 swi(0.f);
+    float spdf = 0.f;
+    int stype = 0;
+    V3 w = bsdfSample(L, b, ctx, wo, swi, spdf, stype, s[3 * i], s[3 * i + 1], s[3 * i + 2]);
+    float* o = outSample + 7 * i;
+    o[0] = w.x; o[1] = w.y; o[2] = w.z; o[3] = swi.x; o[4] = swi.y; o[5] = swi.z; o[6] = spdf;
+    outType[i] = stype;
+    float epdf;
+    V3 ev = bsdfEval(L, b, ctx, wo, wi, epdf);
+    float* e = outEval + 4 * i;
+    e[0] = ev.x; e[1] = ev.y; e[2] = ev.z; e[3] = epdf;
+    V3 alb = bsdfAlbedo(b), nrm = bsdfNormal(b);
+    V3 tr = isTransparent(ms[i]) ? bsdfTransparency(L, b, wo) : V3(0.f);
+    float* x = outAux + 9 * i;
+    x[0] = alb.x; x[1] = alb.y; x[2] = alb.z; x[3] = nrm.x; x[4] = nrm.y; x[5] = nrm.z; x[6] = tr.x; x[7] = tr.y; x[8] = tr.z;
   }
 }
 

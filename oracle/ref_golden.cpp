@@ -16,6 +16,9 @@
 //   bsdf      bsdf/diffuse.h:45-89, bsdf/rough_conductor.h:106-228,
 //             bsdf/rough_dielectric.h:265-548, bsdf/multi_bsdf.h:151-260, bsdf/optics.h
 //   coating   bsdf/rough_coating.h:270-593 over bsdf/diffuse.h:45-89, in a MultiBsdfSimd
+//   lobes     bsdf/rough_diffuse.h:55-106, bsdf/thin_rough_dielectric.h:170-345,
+//             bsdf/transparent.h:26-71, bsdf/multi_bsdf.h:151-459, composed as
+//             material/standard.cpp:409-627 composes them (opacity mix, coat over rough diffuse)
 //   postisect geometry/triangle_mesh.cpp:152-257, core/ray_simd.h:63-68,
 //             core/shading_context.h:144-191 (makeFrame)
 
@@ -37,6 +40,9 @@
 #include "bsdf/rough_dielectric.h"
 #include "bsdf/multi_bsdf.h"
 #include "bsdf/rough_coating.h"
+#include "bsdf/rough_diffuse.h"
+#include "bsdf/thin_rough_dielectric.h"
+#include "bsdf/transparent.h"
 #include "geometry/triangle_mesh.h"
 
 using namespace prt;
@@ -530,6 +536,138 @@ static void goldenPostIntersect()
   }
 }
 
+// The remaining lobes, composed as StandardMaterial composes them (material/standard.cpp:
+// 409-627): the lobe in a MultiBsdfSimd of weight 1; with opacity < 1 - 1e-5 that in turn in a
+// MultiBsdfSimd with a TransparentBtdfSimd (scales opacity, 1 - opacity); opacity <= 1e-5 leaves
+// the TransparentBtdfSimd alone.  Variants by batch: 0 rough diffuse, 1 coating over a rough
+// diffuse base, 2 thin rough dielectric, 3 rough diffuse with transparency, 4 thin dielectric
+// with transparency, 5 rough conductor with transparency (fully transparent every 4th batch).
+// params: kind, r,g,b, er,eg,eb, roughness, ior, thickness, weight, transparency, diffuseRoughness
+// (kinds as the HIP path numbers them: 0 diffuse, 1 conductor, 3 coated, 4 thin dielectric).
+// transparency: what StandardMaterial::getTransparency returns (:633-661) for the same inputs.
+static void goldenLobes()
+{
+  InputRng rng(6262);
+  const int nTotal = 768;
+  std::vector<float> params, geo, samp, outSample, outEval, outAlb, outNormal, outTransp;
+  std::vector<int32_t> outType;
+  int n = 0;
+  for (int i = 0; i < nTotal; i += simdSize)
+  {
+    const int batch = i / simdSize, variant = batch % 6;
+    const int kind = (variant == 0 || variant == 3) ? 0 : (variant == 1 ? 3 : (variant == 5 ? 1 : 4));
+    Vec3f col(0.05f + 0.9f * rng.f(), 0.05f + 0.9f * rng.f(), 0.05f + 0.9f * rng.f());
+    Vec3f edge(rng.f(), rng.f(), rng.f());
+    Vec3f coatCol(0.3f + 0.7f * rng.f(), 0.3f + 0.7f * rng.f(), 0.3f + 0.7f * rng.f());
+    float roughness = std::sqrt(0.01f + 0.5f * rng.f());
+    float diffRough = (kind == 0 || kind == 3) ? 0.05f + 0.95f * rng.f() : 0.f;
+    const float iors[3] = {1.5f, 1.33f, 2.2f};
+    float ior = iors[batch % 3];
+    float thickness = kind == 4 ? 0.2f + 1.8f * rng.f() : (kind == 3 ? 2.f * rng.f() : 0.f);
+    float weight = kind == 3 ? 0.3f + 0.7f * rng.f() : 0.f;
+    float transparency = variant >= 3 ? 0.1f + 0.8f * rng.f() : 0.f;
+    if (variant == 5 && (batch / 6) % 4 == 0) transparency = 1.f;
+    const float eta = std::min(std::max(1.f / ior, 1.f / 3.f), 3.f);
+
+    Basis3vf frame;
+    ShadingContextSimd* ctx = new ShadingContextSimd();
+    Vec3vf wo, wi, s3;
+    for (int k = 0; k < simdSize; ++k)
+    {
+      Vec3f N = rng.dir();
+      Vec3f U, V;
+      makeFrame(U, V, N);
+      Vec3f Ng = N;
+      if (rng.f() < 0.3f) { Ng = normalize(N + 0.2f * rng.dir()); if (dot(Ng, N) < 0.2f) Ng = N; }
+      Vec3f o = rng.dir();
+      if (rng.f() < 0.9f && dot(o, N) < 0.f) o = -o;
+      Vec3f w = rng.dir();
+      if (rng.f() < 0.7f && dot(w, N) < 0.f) w = -w;
+      frame.U.x[k] = U.x; frame.U.y[k] = U.y; frame.U.z[k] = U.z;
+      frame.V.x[k] = V.x; frame.V.y[k] = V.y; frame.V.z[k] = V.z;
+      frame.N.x[k] = N.x; frame.N.y[k] = N.y; frame.N.z[k] = N.z;
+      ctx->Ng.x[k] = Ng.x; ctx->Ng.y[k] = Ng.y; ctx->Ng.z[k] = Ng.z;
+      wo.x[k] = o.x; wo.y[k] = o.y; wo.z[k] = o.z;
+      wi.x[k] = w.x; wi.y[k] = w.y; wi.z[k] = w.z;
+      s3.x[k] = rng.f(); s3.y[k] = rng.f(); s3.z[k] = rng.f();
+      const Vec3f e3 = kind == 3 ? coatCol : edge;
+      params.insert(params.end(), {float(kind), col.x, col.y, col.z, e3.x, e3.y, e3.z, roughness, ior, thickness, weight,
+                                   transparency, diffRough});
+      geo.insert(geo.end(), {U.x, U.y, U.z, V.x, V.y, V.z, N.x, N.y, N.z, Ng.x, Ng.y, Ng.z, o.x, o.y, o.z, w.x, w.y, w.z});
+      samp.insert(samp.end(), {s3.x[k], s3.y[k], s3.z[k]});
+    }
+    vbool m = one;
+    BsdfSimd* base = nullptr;
+    BsdfSimd* child = nullptr;
+    if (kind == 0)
+      child = new RoughDiffuseBrdfSimd(m, &frame, Vec3vf(col), vfloat(diffRough));
+    else if (kind == 3)
+    {
+      base = new RoughDiffuseBrdfSimd(m, &frame, Vec3vf(col), vfloat(diffRough));
+      child = new RoughCoatingBsdfSimd(m, &frame, base, vfloat(eta), Vec3vf(coatCol), vfloat(thickness), vfloat(roughness),
+                                       vfloat(0.f), vfloat(weight));
+    }
+    else if (kind == 1)
+      child = new RoughConductorBrdfSimd<FresnelConductorArtistic<vfloat>>(m, &frame, FresnelConductorArtistic<vfloat>(Vec3vf(col), Vec3vf(edge)), vfloat(roughness), vfloat(0.f));
+    else
+      child = new ThinRoughDielectricBsdfSimd(m, &frame, vfloat(eta), vfloat(roughness), vfloat(0.f), Vec3vf(col), vfloat(thickness));
+    const float opacity = std::min(std::max(1.f - transparency, 0.f), 1.f), transp = 1.f - opacity;
+    const float epsM = 1e-5f;  // material/standard.h:79
+    MultiBsdfSimd inner, outer;
+    TransparentBtdfSimd transparent;
+    const BsdfSimd* bsdf = nullptr;
+    if (opacity > epsM)
+    {
+      inner.add(m, child, vfloat(1.f));
+      bsdf = &inner;
+    }
+    if (transp > epsM)
+    {
+      if (bsdf)
+      {
+        outer.add(m, &inner, vfloat(opacity));
+        outer.add(m, &transparent, vfloat(transp));
+        bsdf = &outer;
+      }
+      else bsdf = &transparent;
+    }
+
+    Vec3vf swi = zero;
+    vfloat spdf = zero;
+    vint stype = zero;
+    Vec3vf sw = bsdf->sample(m, *ctx, wo, swi, spdf, stype, BsdfSampleSimd(s3));
+    vfloat epdf;
+    Vec3vf ev = bsdf->eval(m, *ctx, wo, wi, epdf);
+    Vec3vf alb = bsdf->getAlbedo(m);
+    Vec3vf nrm = bsdf->getNormal(m);
+    // StandardMaterial::getTransparency: a thin-walled transmissive material evaluates its
+    // lobes (here the same composition), any other returns 1 - opacity
+    Vec3vf tr = kind == 4 ? bsdf->getTransparency(m, *ctx, wo) : Vec3vf(vfloat(1.f - opacity));
+    for (int k = 0; k < simdSize; ++k)
+    {
+      outSample.insert(outSample.end(), {sw.x[k], sw.y[k], sw.z[k], swi.x[k], swi.y[k], swi.z[k], spdf[k]});
+      outType.push_back(stype[k]);
+      outEval.insert(outEval.end(), {ev.x[k], ev.y[k], ev.z[k], epdf[k]});
+      outAlb.insert(outAlb.end(), {alb.x[k], alb.y[k], alb.z[k]});
+      outNormal.insert(outNormal.end(), {nrm.x[k], nrm.y[k], nrm.z[k]});
+      outTransp.insert(outTransp.end(), {tr.x[k], tr.y[k], tr.z[k]});
+    }
+    delete child;
+    delete base;
+    delete ctx;
+    n += simdSize;
+  }
+  dump("lobe_params", "f4", params, {n, 13});
+  dump("lobe_geo", "f4", geo, {n, 18});
+  dump("lobe_s", "f4", samp, {n, 3});
+  dump("lobe_sample", "f4", outSample, {n, 7});
+  dump("lobe_sample_type", "i4", outType, {n});
+  dump("lobe_eval", "f4", outEval, {n, 4});
+  dump("lobe_albedo", "f4", outAlb, {n, 3});
+  dump("lobe_normal", "f4", outNormal, {n, 3});
+  dump("lobe_transparency", "f4", outTransp, {n, 3});
+}
+
 int main(int argc, char** argv)
 {
   if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
@@ -546,6 +684,7 @@ int main(int argc, char** argv)
   goldenBsdf();
   goldenCoating();
   goldenPostIntersect();
+  goldenLobes();
   fprintf(gManifest, "\n}\n");
   fclose(gManifest);
   return 0;

@@ -112,6 +112,37 @@ def test_gradient_matches_fp64_oracle_conductor_dielectric(dev):
     assert np.all(err <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
 
 
+def test_gradient_matches_fp64_oracle_remaining_lobes(dev):
+    # Oren-Nayar bases, thin dielectrics (not differentiated), opacity mixes (continuations carry
+    # no gradient) and shadow rays through transparent surfaces
+    W, H, spp = 64, 64, 4
+    sc = G.spheres_lobes(width=W, height=H)
+    mats = sc["materials"]
+    target_mats = mats.copy()
+    for i in range(2, len(mats)):
+        for ch in ("r", "g", "b"):
+            target_mats[i][ch] = min(max(mats[i][ch] + (0.1 if (i + ord(ch)) % 2 else -0.1), 0.02), 0.98)
+    dev.initScene(sc["positions"], sc["indices"], sc["tri_material"], target_mats)
+    dev.initLights(sc["lights"], sc["env"])
+    dev.initRenderer(maxDepth=4)
+    dev.initSampler(0)
+    dev.initCamera(sc["camera"])
+    target = _render(dev, W, H, spp)
+    dev.initScene(sc["positions"], sc["indices"], sc["tri_material"], mats)
+    dev.initLights(sc["lights"], sc["env"])
+    loss, grad, _ = dev.adjoint(target.reshape(H, W, 3), spp, mats.shape[0])
+    osc = O.Scene(sc["positions"], sc["indices"], sc["tri_material"], mats)
+    osc.set_lights(sc["lights"], sc["env"])
+    oloss, ograd = osc.adjoint(sc["camera"], W, H, target, spp, max_depth=4)
+    print("loss", loss, oloss)
+    assert abs(loss - oloss) <= 1e-4 * oloss
+    assert np.all(grad[mats["kind"] == G.KIND_THIN_DIELECTRIC] == 0)
+    scale = np.abs(ograd).max(axis=0)
+    err = np.abs(grad - ograd)
+    print("max abs err / max |g| per (r, g, b, alpha):", err.max(axis=0) / scale)
+    assert np.all(err <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
+
+
 def test_conductor_reflectivity_matches_central_differences(dev):
     # the conductor's sampling does not depend on its reflectivity: the adjoint equals central
     # differences of the GPU-rendered loss

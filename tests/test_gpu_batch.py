@@ -42,9 +42,11 @@ def _wavefront(dev, cam, W, H, seed, spp):
 def test_batch_equals_wavefront_renderer():
     scA = G.spheres_sunsky(seed=1, n_spheres=64, subdiv=2, palette=32, coated=0.5)
     scB = G.spheres_ambient(seed=2, n_spheres=48, subdiv=2, emissive=4, ambient=0.3)
-    dA, dB = _scene_dev(scA), _scene_dev(scB, max_depth=5)
+    scC = G.spheres_lobes(seed=3)  # thin dielectrics, transparency, Oren-Nayar bases
+    dA, dB, dC = _scene_dev(scA), _scene_dev(scB, max_depth=5), _scene_dev(scC)
     cams = [D.look_at((0, 2, -2), (1, 1, 15), 50), D.look_at((-3, 1, 0), (0, 2, 20), 40, lens_radius=0.05, focal=15)]
-    frames = [(dA, cams[0], 96, 64, 3, 6), (dB, cams[1], 64, 48, 11, 5), (dA, cams[1], 128, 32, 0, 4)]
+    frames = [(dA, cams[0], 96, 64, 3, 6), (dB, cams[1], 64, 48, 11, 5), (dA, cams[1], 128, 32, 0, 4),
+              (dC, cams[0], 64, 64, 5, 4)]
     batch = P.BatchHip(0)
     for d, cam, W, H, seed, spp in frames:
         batch.addFrame(d, cam, W, H, seed, spp, snapshots=(1, 4), aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1)
@@ -72,6 +74,7 @@ def test_batch_equals_wavefront_renderer():
     batch.close()
     dA.close()
     dB.close()
+    dC.close()
 
 
 def test_dataset_configuration_batch(tmp_path):

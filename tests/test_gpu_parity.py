@@ -168,6 +168,16 @@ def test_render_parity_sunsky_coated(dev):
     _parity(dev, scene, 96, 64, 3, 0.97)
 
 
+def test_render_parity_remaining_lobes(dev):
+    # Oren-Nayar bases (alone and under a coat), thin rough dielectrics, opacity mixes with the
+    # transparent BTDF (one material fully transparent), and shadow rays through the thin and
+    # transparent surfaces (the transparent-shadow filter)
+    scene = G.spheres_lobes()
+    kinds = scene["materials"]["kind"]
+    assert (kinds == G.KIND_THIN_DIELECTRIC).sum() >= 3 and (scene["materials"]["transparency"] > 0).sum() >= 9
+    _parity(dev, scene, 96, 64, 4, 0.97)
+
+
 @pytest.mark.parametrize("ambient", [0.5, None])
 def test_render_parity_area_lights(dev, ambient):
     # emissive spheres (one TriangleLight over their triangles) with and without an ambient light

@@ -146,6 +146,52 @@ def test_coated_diffuse_sample_eval(golden):
     np.testing.assert_allclose(ev[:, 4:7], ref_e[:, 4:7], rtol=1e-5)
 
 
+def lobe_materials(golden):
+    """Material records of the golden lobe rows (MATERIAL_DTYPE)."""
+    p = golden["lobe_params"]
+    m = np.zeros(p.shape[0], O.MATERIAL_DTYPE)
+    m["kind"] = p[:, 0].astype(np.int32)
+    for k, f in enumerate(("r", "g", "b", "er", "eg", "eb", "roughness", "ior", "thickness", "weight", "transparency",
+                           "diffuse_roughness")):
+        m[f] = p[:, 1 + k]
+    return m
+
+
+@pytest.mark.parametrize("variant", range(6))
+def test_remaining_lobes_sample_eval(golden, variant):
+    # bsdf/rough_diffuse.h (Oren-Nayar, Fujii), a coating over it, bsdf/thin_rough_dielectric.h,
+    # and the opacity mix with bsdf/transparent.h in MultiBsdfSimd (multi_bsdf.h:151-459), composed
+    # as material/standard.cpp:409-627 composes them; variants: 0 rough diffuse, 1 coated rough
+    # diffuse, 2 thin dielectric, 3 rough diffuse + transparency, 4 thin dielectric +
+    # transparency, 5 conductor + transparency (every 4th batch fully transparent)
+    flat, _ = O.compute_luts()
+    mats, geo, s = lobe_materials(golden), golden["lobe_geo"], golden["lobe_s"]
+    so, st, ev, aux = O.bsdf_mat(flat, mats, geo, s)
+    sel = (np.arange(len(mats)) // 8) % 6 == variant
+    ref_s, ref_t, ref_e = golden["lobe_sample"][sel], golden["lobe_sample_type"][sel], golden["lobe_eval"][sel]
+    so, st, ev, aux = so[sel], st[sel], ev[sel], aux[sel]
+    valid_ref = np.abs(ref_s[:, :3]).max(1) > 0
+    valid = np.abs(so[:, :3]).max(1) > 0
+    assert valid.mean() > 0.5
+    assert np.array_equal(valid, valid_ref)
+    assert np.array_equal(st, ref_t)
+    # the reference's SIMD code leaves the reflected direction in rejected lanes; a zero weight
+    # ends the path either way, so directions are compared where the weight is non-zero
+    np.testing.assert_allclose(so[valid, :3], ref_s[valid, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(so[valid, 3:6], ref_s[valid, 3:6], atol=2e-5)
+    np.testing.assert_allclose(so[:, 6], ref_s[:, 6], rtol=1e-3, atol=1e-7)
+    np.testing.assert_allclose(ev[:, :3], ref_e[:, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(ev[:, 3], ref_e[:, 3], rtol=1e-3, atol=1e-6)
+    np.testing.assert_allclose(aux[:, 0:3], golden["lobe_albedo"][sel], rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(aux[:, 3:6], golden["lobe_normal"][sel], atol=1e-6)
+    if variant in (2, 4):  # thin-walled: what a shadow ray keeps through the surface
+        np.testing.assert_allclose(aux[:, 6:9], golden["lobe_transparency"][sel], rtol=5e-4, atol=1e-6)
+    if variant >= 3:
+        types = set(np.unique(st[valid]))
+        assert 128 in types  # the transparent BTDF gets sampled (BsdfType::Transparency)
+        assert len(types) > 1 or variant == 5
+
+
 @pytest.mark.parametrize("suffix", ["", "_n"])
 def test_post_intersect_and_frame(golden, suffix):
     # geometry/triangle_mesh.cpp:152-257, core/ray_simd.h:63-68, core/shading_context.h:144-191
