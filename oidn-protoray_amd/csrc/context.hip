@@ -169,6 +169,47 @@ float lutAvg(const float* f, int size)
   }
   return std::min(2.f * (sum / float(n - 1)), 1.f);
 }
+// Sheen microfiber distribution (bsdf/sheen_distribution.h:10-78, Conty and Kulla 2017): host
+// copy for the sheen albedo table
+struct HostSheen
+{
+  float r;  // alpha, in (0, 1]
+  float D(float c) const
+  {
+    float inv = 1.0f / r;
+    return (2.f + inv) * std::pow(std::sqrt(std::max(1.f - c * c, 0.f)), inv) * (1.f / (2.f * 3.14159265358979323846f));
+  }
+  float L(float x) const
+  {
+    float w0 = hsqr(1.f - r), w1 = 1.f - w0;
+    float a = w0 * 25.3245f + w1 * 21.5473f, b = w0 * 3.32435f + w1 * 3.82987f, c = w0 * 0.16801f + w1 * 0.19823f;
+    float d = w0 * -1.27393f + w1 * -1.97760f, e = w0 * -4.85967f + w1 * -4.32054f;
+    return a / (1.f + b * std::pow(x, c)) + d * x + e;
+  }
+  float lambda(float c) const { return c < 0.5f ? std::exp(L(c)) : std::exp(2.f * L(0.5f) - L(1.f - c)); }
+  float lambdaI(float c) const { return std::pow(lambda(c), 1.f + 2.f * hsqr(hsqr(hsqr(1.f - c)))); }
+  float G2(float co, float ci) const { return 1.0f / (1.f + lambda(co) + lambdaI(ci)); }
+};
+// bsdf/albedo.h:287-330: uniform hemisphere samples on a 32 x 32 midpoint grid
+float sheenIntegrate(float cosO, const HostSheen& m)
+{
+  const int n = 32;
+  const float pi = 3.14159265358979323846f;
+  float c = std::max(cosO, 1e-6f);
+  H3 wo = hmk(std::sqrt(std::max(1.f - c * c, 0.f)), 0.f, c);
+  float sum = 0.f;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+    {
+      float sx = std::min((float(i) + 0.5f) / float(n), 1.f - 1e-6f), sy = std::min((float(j) + 0.5f) / float(n), 1.f - 1e-6f);
+      float phi = 2.f * pi * sx, st = std::sqrt(std::max(1.f - sy * sy, 0.f));
+      H3 wi = hmk(std::cos(phi) * st, std::sin(phi) * st, sy);
+      H3 wh = hnorm(hadd(wo, wi));
+      sum += m.D(wh.z) * m.G2(c, wi.z) * (1.0f / (4.f * c * (1.f / (2.f * pi))));
+    }
+  return std::min(sum / float(n * n), 1.f);
+}
+
 void computeLuts(std::vector<float>& L)
 {
   L.assign(PRT_LUT_FLOATS, 0.f);
@@ -181,6 +222,12 @@ void computeLuts(std::vector<float>& L)
   }
   // MicrofacetDielectricAlbedoTable (eta in [1/3, 1] and [1, 3]) and
   // MicrofacetDielectricReflectionAlbedoTable (eta in [1/3, 1]), albedo_tables.cpp:94-116
+  // MicrofacetSheenAlbedoTable (albedo_tables.cpp:155-170)
+  for (int j = 0; j < 16; ++j)
+  {
+    HostSheen m{std::max(hsqr(float(j) / 15.f), 0.001f)};
+    for (int i = 0; i < 16; ++i) L[PRT_LUT_SHEEN_DIR + j * 16 + i] = sheenIntegrate(float(i) / 15.f, m);
+  }
   const int dirOff[3] = {PRT_LUT_DIEL_ETA_DIR, PRT_LUT_DIEL_RCPETA_DIR, PRT_LUT_DIEL_REFL_DIR};
   const int avgOff[3] = {PRT_LUT_DIEL_ETA_AVG, PRT_LUT_DIEL_RCPETA_AVG, PRT_LUT_DIEL_REFL_AVG};
   for (int table = 0; table < 3; ++table)
@@ -468,7 +515,6 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
     std::memset(&q, 0, sizeof(q));
     const prt_material& s = m[i];
     if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_THIN_DIELECTRIC) return fail(PRT_ERR_INVALID, "unknown material kind");
-    if (s.sheen_weight != 0.f) return fail(PRT_ERR_INVALID, "the sheen layer is not supported");
     if (!(s.transparency >= 0.f && s.transparency <= 1.f)) return fail(PRT_ERR_INVALID, "transparency must be in [0, 1]");
     // opacity (material/standard.cpp:412-420, :613-627, eps = 1e-5, standard.h:79)
     q.opacity = hclamp(1.f - s.transparency, 0.f, 1.f);
@@ -476,6 +522,12 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
     q.mix = q.transp > 1e-5f ? (q.opacity > 1e-5f ? 1 : 2) : 0;
     q.transparent = (s.transparency != 0.f || s.kind == PRT_KIND_THIN_DIELECTRIC) ? 1 : 0;  // :152-166
     q.diffuseRoughness = hclamp(s.diffuse_roughness, 0.f, 1.f);
+    // sheen (standard.cpp:599-611): colour and roughness clamped, alpha = roughnessToAlpha
+    q.sheenScale = hclamp(s.sheen_weight, 0.f, 1.f);
+    q.sheen = q.sheenScale > 1e-5f ? 1 : 0;
+    for (int j = 0; j < 3; ++j) q.sheenColor[j] = hclamp(s.sheen_color[j], 0.f, 1.f);
+    q.sheenRoughness = hclamp(s.sheen_roughness, 0.f, 1.f);
+    q.sheenAlpha = std::max(hsqr(q.sheenRoughness), 0.001f);
     q.kind = s.kind;
     q.roughness = hclamp(s.roughness, 0.f, 1.f);
     q.alpha = std::max(hsqr(q.roughness), 0.001f);  // bsdf/microfacet.h:10-24
@@ -587,7 +639,8 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
   for (int i = 0; i < n; ++i)
   {
     c->anyTransparent |= k[i].transparent;
-    c->extMaterials |= (k[i].kind == PRT_KIND_THIN_DIELECTRIC || k[i].mix != 0 || k[i].diffuseRoughness > 0.f) ? 1 : 0;
+    c->extMaterials |=
+        (k[i].kind == PRT_KIND_THIN_DIELECTRIC || k[i].mix != 0 || k[i].diffuseRoughness > 0.f || k[i].sheen) ? 1 : 0;
   }
   HIP_TRY(c->mats.alloc(n));
   HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * n, hipMemcpyHostToDevice));

@@ -324,6 +324,30 @@ PRT_DEV float interp3D(float px, float py, float pz, const float* f, int n)
   float f11 = lerpf(f[x0 + y1 * ny + z1 * nz], f[x1 + y1 * ny + z1 * nz], sx);
   return lerpf(lerpf(f00, f01, sy), lerpf(f10, f11, sy), sz);
 }
+PRT_DEV float sheenAlbedo(const float* lut, float cosO, float roughness)  // MicrofacetSheenAlbedoTable::eval
+{
+  return interp2D(cosO * 15.f, roughness * 15.f, lut + PRT_LUT_SHEEN_DIR, 16);
+}
+// Sheen microfiber distribution (bsdf/sheen_distribution.h:10-78; Conty and Kulla 2017), r = alpha
+struct SheenDist
+{
+  float r;
+  PRT_DEV float D(float cosTheta) const
+  {
+    float inv = 1.0f / r;
+    return (2.f + inv) * powf(cos2sin(cosTheta), inv) * (1.f / (2.f * kPi));
+  }
+  PRT_DEV float L(float x) const  // the fitted exponent of Lambda
+  {
+    float w0 = sqr(1.f - r), w1 = 1.f - w0;
+    float a = w0 * 25.3245f + w1 * 21.5473f, b = w0 * 3.32435f + w1 * 3.82987f, c = w0 * 0.16801f + w1 * 0.19823f;
+    float d = w0 * -1.27393f + w1 * -1.97760f, e = w0 * -4.85967f + w1 * -4.32054f;
+    return a / (1.f + b * powf(x, c)) + d * x + e;
+  }
+  PRT_DEV float lambda(float c) const { return c < 0.5f ? expf(L(c)) : expf(2.f * L(0.5f) - L(1.f - c)); }
+  PRT_DEV float lambdaI(float c) const { return powf(lambda(c), 1.f + 2.f * sqr(sqr(sqr(1.f - c)))); }
+  PRT_DEV float G2(float cosO, float cosI) const { return 1.0f / (1.f + lambda(cosO) + lambdaI(cosI)); }
+};
 PRT_DEV float ggxAlbedo(const float* lut, float cosO, float roughness)  // MicrofacetAlbedoTable::eval
 {
   return interp2D(cosO * 31.f, roughness * 31.f, lut + PRT_LUT_GGX_DIR, 32);
@@ -406,11 +430,20 @@ struct Bsdf
   float diffRough;            // kinds 0 and 3: Oren-Nayar base when > 0
   int mix;                    // opacity mix: 0 lobe, 1 lobe + transparent BTDF, 2 transparent
   float opacity, transp;
-  // false: the scene has no thin dielectric, opacity or Oren-Nayar base; a compile-time
+  bool sheen;                 // a sheen layer over the lobe (bsdf/sheen.h)
+  f3 sheenColor;
+  float sheenScale, sheenRough, sheenA;
+  // false: the scene has no thin dielectric, opacity, Oren-Nayar base or sheen; a compile-time
   // constant in the kernels built for such scenes, where it folds those branches away
   bool ext;
 };
 
+// Importance of the surface: the lobe's, or under a sheen layer the larger of the sheen
+// colour's and the lobe's (sheen.h:403-406).
+PRT_DEV float surfImportance(const Bsdf& b)
+{
+  return b.sheen ? fmaxf(reduceMax(b.sheenColor), b.importance) : b.importance;
+}
 // The lobe's albedo (multi_bsdf.h:314-337 with one child).
 PRT_DEV f3 lobeAlbedo(const Bsdf& b)
 {
@@ -422,14 +455,14 @@ PRT_DEV f3 lobeAlbedo(const Bsdf& b)
 PRT_DEV f3 bsdfAlbedo(const Bsdf& b)
 {
   if (b.mix == 2) return f3(-1.f);
-  if (b.mix == 1 && !(b.importance * b.opacity > 0.f)) return f3(0.f);
-  return lobeAlbedo(b);
+  if (b.mix == 1 && !(surfImportance(b) * b.opacity > 0.f)) return f3(0.f);
+  return lobeAlbedo(b);  // the sheen layer passes the base's albedo (sheen.h:408-411)
 }
 // The normal AOV: the first child with a non-zero importance (multi_bsdf.h:408-428); the
 // transparent BTDF has no frame.
 PRT_DEV f3 bsdfNormal(const Bsdf& b)
 {
-  return (b.mix != 2 && b.importance * (b.mix == 1 ? b.opacity : 1.f) > 0.f) ? b.fr.N : f3(0.f);
+  return (b.mix != 2 && b.importance > 0.f && (b.mix == 0 || surfImportance(b) * b.opacity > 0.f)) ? b.fr.N : f3(0.f);
 }
 
 // bsdf/diffuse.h:55-68, or with diffRough > 0 the Oren-Nayar BRDF in Fujii's form
@@ -795,20 +828,118 @@ PRT_DEV f3 lobeSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
   return dielSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
 }
 
+// --- sheen layer over the lobe (bsdf/sheen.h:199-451): a microfiber reflection picked by
+// importance against the lobe, which it attenuates by min(1 - Ro, 1 - Ri)
+PRT_DEV void sheenPicks(const Bsdf& b, float Ro, float& coatPick, float& basePick)
+{
+  float coatImp = Ro * reduceMax(b.sheenColor), baseImp = (1.f - Ro) * b.importance;
+  coatPick = coatImp * (1.0f / (coatImp + baseImp));
+  basePick = 1.f - coatPick;
+}
+PRT_DEV f3 sheenReflection(const Bsdf& b, f3 wo, f3 wi, float cosO, float cosI)  // :253-265, :372-384
+{
+  f3 wh = normalize(wo + wi);
+  SheenDist sd{b.sheenA};
+  float D = sd.D(dot(wh, b.fr.N)), G = sd.G2(cosO, cosI);
+  return b.sheenColor * (D * G * (1.0f / (4.f * cosO)) * b.sheenScale);
+}
+PRT_DEV f3 sheenEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)  // :218-279
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  float basePdf;
+  f3 baseValue = lobeEval(lut, b, c, wo, wi, basePdf);
+  float Ro = sheenAlbedo(lut, cosO, b.sheenRough) * b.sheenScale;
+  float Ri = sheenAlbedo(lut, fabsf(cosI), b.sheenRough) * b.sheenScale;
+  baseValue = baseValue * fminf(1.f - Ro, 1.f - Ri);
+  float coatPick, basePick;
+  sheenPicks(b, Ro, coatPick, basePick);
+  if (cosI > 0.f)
+  {
+    pdf = coatPick * (1.f / (2.f * kPi)) + basePick * basePdf;
+    return sheenReflection(b, wo, wi, cosO, cosI) + baseValue;
+  }
+  pdf = basePdf * basePick;
+  return baseValue;
+}
+PRT_DEV f3 sheenSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)  // :281-391
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return f3(0.f);
+  float Ro = sheenAlbedo(lut, cosO, b.sheenRough) * b.sheenScale;
+  float coatPick, basePick;
+  sheenPicks(b, Ro, coatPick, basePick);
+  float basePdf = 0.f;
+  f3 baseValue(0.f);
+  wi = f3(0.f);
+  pdf = 0.f;
+  type = 0;
+  if (sc <= coatPick)
+  {
+    type = BsdfType::DiffuseReflection;
+    float phi = 2.f * kPi * sx, sp, cp;
+    sincosf(phi, &sp, &cp);
+    const float st = cos2sin(sy);
+    wi = toGlobal(b.fr, f3(cp * st, sp * st, sy));  // uniformSampleHemisphere (shape_sampler.h:54-61)
+    baseValue = lobeEval(lut, b, c, wo, wi, basePdf);
+  }
+  else
+  {
+    f3 bwi(0.f);
+    float bpdf = 0.f;
+    int btype = 0;
+    f3 bw = lobeSample(lut, b, c, wo, bwi, bpdf, btype, sx, sy, (sc - coatPick) * (1.0f / basePick));
+    if (!(reduceMax(bw) > 0.f)) return f3(0.f);
+    wi = bwi;
+    basePdf = bpdf;
+    type = btype;
+    baseValue = bw * ((type & BsdfType::Delta) ? 1.f : basePdf);
+  }
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
+  float Ri = sheenAlbedo(lut, fabsf(cosI), b.sheenRough) * b.sheenScale;
+  baseValue = baseValue * fminf(1.f - Ro, 1.f - Ri);
+  if (type & BsdfType::Delta)
+  {
+    pdf = basePdf;
+    return baseValue * (1.0f / basePick);
+  }
+  if (cosI <= 0.f)
+  {
+    pdf = basePick * basePdf;
+    return baseValue * (1.0f / pdf);
+  }
+  pdf = coatPick * (1.f / (2.f * kPi)) + basePick * basePdf;
+  return (sheenReflection(b, wo, wi, cosO, cosI) + baseValue) * (1.0f / pdf);
+}
+
+// The surface: the lobe, under the sheen layer where there is one.
+PRT_DEV f3 surfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  return b.sheen ? sheenEval(lut, b, c, wo, wi, pdf) : lobeEval(lut, b, c, wo, wi, pdf);
+}
+PRT_DEV f3 surfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  return b.sheen ? sheenSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc) : lobeSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+}
+
 // The whole BSDF with the opacity mix (material/standard.cpp:613-627): MultiBsdf{lobe x opacity,
 // transparent BTDF x transp} (multi_bsdf.h:166-287, transparent.h:40-45).  The transparent
 // BTDF evaluates to zero and has importance 1.
 PRT_DEV f3 bsdfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
 {
-  if (b.mix == 0) return lobeEval(lut, b, c, wo, wi, pdf);
+  if (b.mix == 0) return surfEval(lut, b, c, wo, wi, pdf);
   pdf = 0.f;
   if (b.mix == 2) return f3(0.f);
-  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  const float imp0 = surfImportance(b) * b.opacity, sum = imp0 + b.transp;
   f3 v(0.f);
   if (imp0 > 0.f)
   {
     float cp;
-    v = lobeEval(lut, b, c, wo, wi, cp) * b.opacity;
+    v = surfEval(lut, b, c, wo, wi, cp) * b.opacity;
     pdf = cp * imp0;
   }
   pdf *= 1.0f / sum;
@@ -816,7 +947,7 @@ PRT_DEV f3 bsdfEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi,
 }
 PRT_DEV f3 bsdfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
 {
-  if (b.mix == 0) return lobeSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+  if (b.mix == 0) return surfSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
   if (b.mix == 2)
   {
     wi = -wo;
@@ -824,7 +955,7 @@ PRT_DEV f3 bsdfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
     pdf = 1.f;
     return f3(1.f);
   }
-  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  const float imp0 = surfImportance(b) * b.opacity, sum = imp0 + b.transp;
   if (!(sum > 0.f)) return f3(0.f);
   // choose a child by importance and remap the sample (:219-245)
   const float x = sc * sum;
@@ -840,7 +971,7 @@ PRT_DEV f3 bsdfSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
     f3 cwi(0.f);
     float cpdf = 0.f;
     int ctype = 0;
-    f3 cw = lobeSample(lut, b, c, wo, cwi, cpdf, ctype, sx, sy, sc1);
+    f3 cw = surfSample(lut, b, c, wo, cwi, cpdf, ctype, sx, sy, sc1);
     if (reduceMax(cw) > 0.f) { w = cw * b.opacity; wi = cwi; pdf = cpdf; type = ctype; }
   }
   else
@@ -863,6 +994,11 @@ PRT_DEV f3 bsdfTransparency(const float* lut, const Bsdf& b, f3 wo)
 {
   if (b.mix == 2) return f3(1.f);
   f3 t = (b.kind == PRT_KIND_THIN_DIELECTRIC && b.importance > 0.f) ? thinTransparency(lut, b, wo) : f3(0.f);
+  if (b.sheen)  // sheen.h:393-401
+  {
+    float cosO = dot(wo, b.fr.N);
+    t = cosO > 0.f ? t * (1.f - sheenAlbedo(lut, cosO, b.sheenRough) * b.sheenScale) : f3(0.f);
+  }
   if (b.mix == 0) return t;
   return t * b.opacity + f3(b.transp);
 }
@@ -894,6 +1030,11 @@ PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool
   b.mix = ext ? m.mix : 0;
   b.opacity = m.opacity;
   b.transp = m.transp;
+  b.sheen = ext && m.sheen;
+  b.sheenColor = f3(m.sheenColor[0], m.sheenColor[1], m.sheenColor[2]);
+  b.sheenScale = m.sheenScale;
+  b.sheenRough = m.sheenRoughness;
+  b.sheenA = m.sheenAlpha;
   return b;
 }
 

@@ -17,6 +17,8 @@
 //   rough dielectric (bsdf/rough_dielectric.h:296-526): alpha only (no albedo), through D, G2 of
 //     the single-scattering lobe and the tables and fmsRatio of the multiple-scattering lobe.
 //   thin dielectric: not differentiated (zero derivatives).
+//   a sheen layer attenuates the lobe's derivatives by its T = min(1 - Ro, 1 - Ri); its own
+//     parameters are not differentiated.
 // Table derivatives are those of the linear interpolation in roughness (zero where clamped).
 #pragma once
 
@@ -212,7 +214,7 @@ PRT_DEV BsdfGrad dielectricEvalGrad(const float* lut, const Bsdf& b, const prt_m
   return g;
 }
 
-PRT_DEV BsdfGrad bsdfEvalGrad(const float* lut, const Bsdf& b, const prt_material_constants& m, const Ctx& c, f3 wo, f3 wi)
+PRT_DEV BsdfGrad lobeEvalGrad(const float* lut, const Bsdf& b, const prt_material_constants& m, const Ctx& c, f3 wo, f3 wi)
 {
   BsdfGrad g;
   g.value = g.dAlbedo = g.dAlpha = f3(0.f);
@@ -271,6 +273,28 @@ PRT_DEV BsdfGrad bsdfEvalGrad(const float* lut, const Bsdf& b, const prt_materia
     g.value = g.value + f3(k * D * G);
     g.dAlpha = g.dAlpha + f3(k * (dD * G + D * dG));
   }
+  return g;
+}
+
+// Under a sheen layer (bsdf/sheen.h:218-279) the lobe's value and derivatives are attenuated by
+// T = min(1 - Ro, 1 - Ri) and the sheen reflection (not differentiated) joins the value.
+PRT_DEV BsdfGrad bsdfEvalGrad(const float* lut, const Bsdf& b, const prt_material_constants& m, const Ctx& c, f3 wo, f3 wi)
+{
+  BsdfGrad g = lobeEvalGrad(lut, b, m, c, wo, wi);
+  if (!b.sheen) return g;
+  const float cosO = dot(wo, b.fr.N), cosI = dot(wi, b.fr.N);
+  if (!(cosO > 0.f) || !(cosI * dot(wi, c.Ng) > 0.f))
+  {
+    g.value = g.dAlbedo = g.dAlpha = f3(0.f);
+    return g;
+  }
+  const float Ro = sheenAlbedo(lut, cosO, b.sheenRough) * b.sheenScale;
+  const float Ri = sheenAlbedo(lut, fabsf(cosI), b.sheenRough) * b.sheenScale;
+  const float T = fminf(1.f - Ro, 1.f - Ri);
+  g.value = g.value * T;
+  if (cosI > 0.f) g.value = g.value + sheenReflection(b, wo, wi, cosO, cosI);
+  g.dAlbedo = g.dAlbedo * T;
+  g.dAlpha = g.dAlpha * T;
   return g;
 }
 

@@ -116,7 +116,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
           float ts, us = 0.f, vs = 0.f;
           if (a.anyTransparent)
           {
-            ShadowFilter flt{&a, ctx.p, wiL, f3(1.f)};
+            ShadowFilter flt{&a, ctx.p, wiL, f3(1.f), 0u};
             nee = traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris, flt) < 0;
             dirL = dirL * flt.T;
           }

@@ -128,10 +128,12 @@ struct ShadowFilter
 {
   const RenderArgs* a;
   f3 o, d, T;
+  uint32_t crossed;  // transparent surfaces passed (part of the path's identity in debug hashes)
   PRT_DEV bool operator()(int prim, float t, float u, float v)
   {
     const prt_material_constants& m = a->mats[a->triMat[prim]];
     if (!m.transparent) { T = f3(0.f); return true; }
+    ++crossed;
     Ctx c = postIntersect(*a, o, d, t, prim, u, v);
     Bsdf b = makeBsdf(m, c, -d, false);
     T = T * bsdfTransparency(a->lut, b, -d);

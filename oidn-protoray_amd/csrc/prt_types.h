@@ -28,7 +28,9 @@
 #define PRT_LUT_DIEL_RCPETA_AVG 9504
 #define PRT_LUT_DIEL_REFL_DIR 9760
 #define PRT_LUT_DIEL_REFL_AVG 13856
-#define PRT_LUT_FLOATS 14112
+//   MicrofacetSheenAlbedoTable dir [16 roughness][16 cos] (bsdf/albedo_tables.h:281-300)
+#define PRT_LUT_SHEEN_DIR 14112
+#define PRT_LUT_FLOATS 14368
 
 // Per-material constants derived on the host at upload (the per-hit constructor work of
 // RoughConductorBrdfSimd, rough_conductor.h:121-130, and RoughDielectricBsdfSimd,
@@ -64,9 +66,12 @@ struct prt_material_constants
   float opacity, transp;
   int32_t transparent;     // shadow rays pass through, attenuated (render/transparent_filter.h)
   float diffuseRoughness;  // kinds 0 and 3: Oren-Nayar base when > 0 (bsdf/rough_diffuse.h)
-  float pad[3];
+  // sheen layer over the lobe (material/standard.cpp:599-611, bsdf/sheen.h:199-451)
+  int32_t sheen;           // sheen weight above 1e-5
+  float sheenColor[3];
+  float sheenScale, sheenRoughness, sheenAlpha;
 };
-static_assert(sizeof(prt_material_constants) == 192, "material constants: 192 bytes");
+static_assert(sizeof(prt_material_constants) == 208, "material constants: 208 bytes");
 
 // Environment light record as the kernels read it.  64 bytes.
 struct prt_light_dev

@@ -363,11 +363,13 @@ __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __res
     const f3 o(so.x, so.y, so.z), d(sd.x, sd.y, sd.z);
     int prim;
     f3 Tsh(1.f);
+    uint32_t crossed = 0;
     if (kFilter)
     {
-      ShadowFilter flt{&a, o, d, f3(1.f)};
+      ShadowFilter flt{&a, o, d, f3(1.f), 0u};
       prim = traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris, flt);
       Tsh = flt.T;
+      crossed = flt.crossed;
     }
     else prim = traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris);
     ++rays;
@@ -382,7 +384,7 @@ __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __res
     uint32_t hsh = 0u;
     if (a.hash)  // the visibility event is part of the path's identity
     {
-      hsh = hashMix(a.hash[slot], prim < 0 ? 1u : 2u);
+      hsh = hashMix(a.hash[slot], prim < 0 ? 1u + 4u * crossed : 2u);  // + the transparent surfaces crossed
       a.hash[slot] = hsh;
     }
     if (a.flags[slot] & PRT_F_TERMINATE) finalizePath(a, slot, L, hsh);

@@ -266,7 +266,7 @@ struct StdMat
   float transmissionWeight = 0.f, transmissionColor[3] = {1.f, 1.f, 1.f}, transmissionDepth = 1.f;
   float subsurfaceWeight = 0.f;
   float coatWeight = 0.f, coatColor[3] = {1.f, 1.f, 1.f}, coatRoughness = 0.f, coatIor = 1.6f;
-  float sheenWeight = 0.f;
+  float sheenWeight = 0.f, sheenColor[3] = {1.f, 1.f, 1.f}, sheenRoughness = 0.3f;
   float emissionLuminance = 0.f, emissionColor[3] = {1.f, 1.f, 1.f};
   bool thinWalled = false;
   float opacity = 1.f;
@@ -360,6 +360,8 @@ StdMat rkMaterial(const Json& jm)
   matParam<1>(jm, "coatRoughness", &m.coatRoughness, tx);
   if (const Json* v = jm.get("coatIOR")) m.coatIor = float(v->num());
   matParam<1>(jm, "sheenWeight", &m.sheenWeight, tx);
+  matParam<3>(jm, "sheenColor", m.sheenColor, tx);
+  matParam<1>(jm, "sheenRoughness", &m.sheenRoughness, tx);
   matParam<1>(jm, "emissionLuminance", &m.emissionLuminance, tx);
   matParam<3>(jm, "emissionColor", m.emissionColor, tx);
   if (const Json* v = jm.get("thinWalled")) m.thinWalled = v->num() != 0.0;
@@ -608,7 +610,7 @@ StdMat gltfMaterial(const Json& gm)
   float emissive[3] = {0.f, 0.f, 0.f}, emissiveStrength = 1.f;
   if (const Json* e = gm.get("emissiveFactor")) readF3(*e, emissive);
   float specular = 1.f, specColor[3] = {1.f, 1.f, 1.f}, ior = 1.5f, transmission = 0.f, thickness = 0.f;
-  float sheenColor[3] = {0.f, 0.f, 0.f}, attDist = INFINITY, attColor[3] = {1.f, 1.f, 1.f}, clearcoat = 0.f, clearcoatRough = 0.f;
+  float sheenColor[3] = {0.f, 0.f, 0.f}, sheenRough = 0.f, attDist = INFINITY, attColor[3] = {1.f, 1.f, 1.f}, clearcoat = 0.f, clearcoatRough = 0.f;
   float diffTrans = 0.f;
   if (const Json* ext = gm.get("extensions"))
   {
@@ -620,7 +622,11 @@ StdMat gltfMaterial(const Json& gm)
     if (const Json* e = ext->get("KHR_materials_ior")) ior = jnum(*e, "ior", 1.5f);
     if (const Json* e = ext->get("KHR_materials_transmission")) transmission = jnum(*e, "transmissionFactor", 0.f);
     if (const Json* e = ext->get("KHR_materials_sheen"))
+    {
       if (const Json* c = e->get("sheenColorFactor")) readF3(*c, sheenColor);
+      sheenRough = jnum(*e, "sheenRoughnessFactor", 0.f);
+      if (e->get("sheenColorTexture") || e->get("sheenRoughnessTexture")) m.textured = true;
+    }
     if (const Json* e = ext->get("KHR_materials_diffuse_transmission")) diffTrans = jnum(*e, "diffuseTransmissionFactor", 0.f);
     if (const Json* e = ext->get("KHR_materials_volume"))
     {
@@ -640,7 +646,10 @@ StdMat gltfMaterial(const Json& gm)
   m.specularIor = ior;
   m.transmissionWeight = transmission;
   if (transmission != 0.f) std::copy(base, base + 3, m.transmissionColor);
+  // KHR_materials_sheen (rkscene/rkscene_gltf.h:975-978): weight 1 for a non-black colour
   if (sheenColor[0] != 0.f || sheenColor[1] != 0.f || sheenColor[2] != 0.f) m.sheenWeight = 1.f;
+  std::copy(sheenColor, sheenColor + 3, m.sheenColor);
+  m.sheenRoughness = sheenRough;
   m.subsurfaceWeight = diffTrans;
   if (thickness != 0.f && std::isfinite(attDist))
   {
@@ -830,6 +839,8 @@ StdMat mtlMaterial(const MtlProps& p)
     m.coatRoughness = p.f("coatRoughness", 0.f);
     m.coatIor = p.f("coatIOR", 1.6f);
     m.sheenWeight = p.f("sheenWeight", 0.f);
+    p.f3("sheenColor", m.sheenColor);
+    m.sheenRoughness = p.f("sheenRoughness", 0.3f);
     m.emissionLuminance = p.f("emissionLuminance", 0.f);
     p.f3("emissionColor", m.emissionColor);
     m.thinWalled = p.f("thinWalled", 0.f) != 0.f;
@@ -879,7 +890,13 @@ StdMat mtlMaterial(const MtlProps& p)
     }
   }
   m.coatWeight = p.f("coat", 0.f);
+  // legacy sheen (material/standard.cpp:133-137, :605-607): roughness 0.2, colour lerped to the
+  // base colour by sheenTint
   m.sheenWeight = p.f("sheen", 0.f);
+  p.f3("sheenColor", m.sheenColor);
+  m.sheenRoughness = p.f("sheenRoughness", 0.2f);
+  const float tint = std::min(std::max(p.f("sheenTint", 0.f), 0.f), 1.f);
+  for (int c = 0; c < 3; ++c) m.sheenColor[c] += tint * (m.baseColor[c] - m.sheenColor[c]);
   m.thinWalled = p.f("thin", 0.f) != 0.f;
   m.opacity = p.f("opacity", 1.f);
   for (const char* k : {"map_Kd", "baseColorMap", "map_Bump", "map_bump", "bumpMap", "normalMap", "map_d"})
@@ -979,8 +996,8 @@ MeshFile loadMesh(const std::string& path)
 // ---------------------------------------------------------------------------------------
 // Standard parameters -> material record.  The record models one lobe set per material
 // (diffuse or Oren-Nayar, rough conductor, rough dielectric, coated diffuse, thin rough
-// dielectric; opacity and emission on any); parameters outside that set are dropped and the
-// material counted as approximated.
+// dielectric; opacity, sheen and emission on any); parameters outside that set are dropped and
+// the material counted as approximated.
 // ---------------------------------------------------------------------------------------
 bool emissive(const StdMat& m)
 {
@@ -1013,8 +1030,11 @@ prt_material toRecord(const StdMat& m, bool& approx)
   const float ior = m.specularIor > 0.f && m.specularIor < 1.f ? 1.f / m.specularIor : m.specularIor;
   const bool mixedMetal = m.metalness > 0.f && m.metalness < 1.f;
   const bool mixedTrans = m.transmissionWeight > 0.f && m.transmissionWeight < 1.f;
-  approx = m.textured || mixedMetal || mixedTrans || m.sheenWeight > 0.f || m.subsurfaceWeight > 0.f ||
-           (m.baseWeight != 1.f && m.metalness < 1.f);
+  approx = m.textured || mixedMetal || mixedTrans || m.subsurfaceWeight > 0.f || (m.baseWeight != 1.f && m.metalness < 1.f);
+  // the sheen layer over any lobe set (material/standard.cpp:599-611)
+  r.sheen_weight = m.sheenWeight;
+  for (int c = 0; c < 3; ++c) r.sheen_color[c] = m.sheenColor[c];
+  r.sheen_roughness = m.sheenRoughness;
   // opacity (material/standard.cpp:152-157, :613-627) applies to every lobe set
   r.transparency = 1.f - std::min(std::max(m.opacity, 0.f), 1.f);
   if (m.metalness >= 0.5f)

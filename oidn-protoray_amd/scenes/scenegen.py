@@ -296,8 +296,9 @@ def spheres_sunsky(seed=0, n_spheres=4096, subdiv=4, width=1920, height=1080, sp
 def spheres_lobes(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4, max_depth=8, palette=24):
     """The sun/sky scene of spheres_sunsky with the remaining StandardMaterial lobes: rough
     diffuse (Oren-Nayar) bases, coats over them, thin rough dielectrics (thin-walled bubbles with
-    tinted transmission), and opacity < 1 on diffuse, conductor and coated materials (one fully
-    transparent); shadow rays pass through the thin and transparent ones."""
+    tinted transmission), opacity < 1 on diffuse, conductor and coated materials (one fully
+    transparent), and a sheen layer on every other material (rough dielectrics among them);
+    shadow rays pass through the thin and transparent ones."""
     rng = Pcg32(seed)
     mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5), diffuse_roughness=0.5)]
     for i in range(palette):
@@ -321,6 +322,14 @@ def spheres_lobes(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4, ma
         else:
             mats.append(_material(KIND_COATED, rgb, (1, 1, 1), rough, 1.5, thickness=1.0, weight=1.0,
                                   transparency=rng.f(0.2, 0.8)))
+    for i in range(palette // 4):  # a few rough dielectrics (media interfaces) with sheen
+        mats.append(_material(KIND_DIELECTRIC, roughness=math.sqrt(rng.f(0.02, 0.3)), ior=1.5))
+    srng = Pcg32(seed + 101)
+    for i in range(2, len(mats), 2):
+        mats[i]["sheen_weight"] = srng.f(0.3, 1.0)
+        mats[i]["sheen_color"] = (srng.f(0.3, 1.0), srng.f(0.3, 1.0), srng.f(0.3, 1.0))
+        mats[i]["sheen_roughness"] = srng.f(0.1, 0.9)
+    palette = len(mats) - 2
     pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, lambda i: 2 + rng.u32() % palette)
     ss = load_sunsky()
     d = np.array([-0.6, 0.7, -0.4])

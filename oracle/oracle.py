@@ -42,7 +42,7 @@ CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("
                          ("focalDistance", "<f4")])
 LUT_SIZES = [("ggx_dir", 32 * 32), ("ggx_avg", 32), ("diel_eta_dir", 16 ** 3), ("diel_eta_avg", 16 ** 2),
              ("diel_rcpeta_dir", 16 ** 3), ("diel_rcpeta_avg", 16 ** 2), ("diel_refl_dir", 16 ** 3),
-             ("diel_refl_avg", 16 ** 2)]
+             ("diel_refl_avg", 16 ** 2), ("sheen_dir", 16 ** 2)]
 LUT_FLOATS = sum(n for _, n in LUT_SIZES)
 
 

@@ -472,6 +472,7 @@ struct Luts
   float dielRcpEtaAvg[16 * 16];
   float dielReflDir[16 * 16 * 16];  // MicrofacetDielectricReflectionAlbedoTable::etaDir, eta in [1/3,1]
   float dielReflAvg[16 * 16];
+  float sheenDir[16 * 16];          // MicrofacetSheenAlbedoTable::dir [roughness][cosThetaO]
 };
 
 // albedo_tables.h:20-42
@@ -517,6 +518,33 @@ inline float dielReflAlbedoAvg(const Luts& L, float eta, float roughness)
   float e = (eta - minEta) / (maxEta - minEta);
   return interp2D(e * 15.f, roughness * 15.f, L.dielReflAvg, 16, 16);
 }
+
+// MicrofacetSheenAlbedoTable::eval (albedo_tables.h:287-291)
+inline float sheenAlbedo(const Luts& L, float cosThetaO, float roughness)
+{
+  return interp2D(cosThetaO * 15.f, roughness * 15.f, L.sheenDir, 16, 16);
+}
+
+// Sheen microfiber distribution (bsdf/sheen_distribution.h:10-78; Conty and Kulla 2017), r = alpha
+struct SheenDist
+{
+  float r;
+  float D(float cosTheta) const
+  {
+    float inv = 1.0f / r;
+    return (2.f + inv) * std::pow(cos2sin(cosTheta), inv) * (1.f / (2.f * kPi));
+  }
+  float L(float x) const  // the fitted exponent of Lambda
+  {
+    float w0 = sqr(1.f - r), w1 = 1.f - w0;
+    float a = w0 * 25.3245f + w1 * 21.5473f, b = w0 * 3.32435f + w1 * 3.82987f, c = w0 * 0.16801f + w1 * 0.19823f;
+    float d = w0 * -1.27393f + w1 * -1.97760f, e = w0 * -4.85967f + w1 * -4.32054f;
+    return a / (1.f + b * std::pow(x, c)) + d * x + e;
+  }
+  float lambda(float c) const { return c < 0.5f ? std::exp(L(c)) : std::exp(2.f * L(0.5f) - L(1.f - c)); }
+  float lambdaI(float c) const { return std::pow(lambda(c), 1.f + 2.f * sqr(sqr(sqr(1.f - c)))); }  // softer terminator
+  float G2(float cosO, float cosI) const { return 1.0f / (1.f + lambda(cosO) + lambdaI(cosI)); }
+};
 
 void precomputeLuts(Luts& L);  // orc_luts.cpp
 
@@ -624,6 +652,10 @@ struct Bsdf
   float diffRough;  // Oren-Nayar base when > 0
   int mix;          // 0 the lobe, 1 lobe x opacity + transparent BTDF x transp, 2 transparent only
   float opacity, transp;
+  bool sheen;       // a sheen layer over the lobe
+  V3 sheenColor;
+  float sheenScale, sheenRough;
+  SheenDist sheenDist{0.001f};
   float roughness;
   GGX ggx{0.001f};
   float importance;
@@ -643,6 +675,12 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
   b.transp = 1.f - b.opacity;
   b.mix = b.transp > 1e-5f ? (b.opacity > 1e-5f ? 1 : 2) : 0;
   b.teta = b.tEavg = b.tfms = b.tthick = 0.f;
+  // sheen (material/standard.cpp:599-611; sheenTint 0 keeps sheenColor)
+  b.sheenScale = clampf(m.sheenWeight, 0.f, 1.f);
+  b.sheen = b.sheenScale > 1e-5f;
+  b.sheenColor = V3(clampf(m.sheenColor[0], 0.f, 1.f), clampf(m.sheenColor[1], 0.f, 1.f), clampf(m.sheenColor[2], 0.f, 1.f));
+  b.sheenRough = clampf(m.sheenRoughness, 0.f, 1.f);
+  b.sheenDist = SheenDist{roughnessToAlpha(b.sheenRough)};
   if (m.kind == kDiffuse)
   {
     b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
@@ -717,6 +755,7 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
     b.flags = BsdfType::GlossyReflection | BsdfType::UnbentTransmission;
     b.importance = 1.f;
   }
+  if (b.sheen) b.flags |= BsdfType::DiffuseReflection;  // sheen.h:210-211
   if (b.mix == 1) b.flags |= BsdfType::Transparency;
   if (b.mix == 2) b.flags = BsdfType::Transparency;
   return b;
@@ -1119,20 +1158,117 @@ inline V3 lobeAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
   return V3(1.f);
 }
 
+// --- sheen layer over the lobe (bsdf/sheen.h:199-451)
+inline void sheenPicks(const Bsdf& b, float Ro, float& coatPick, float& basePick)
+{
+  float coatImp = Ro * reduceMax(b.sheenColor), baseImp = (1.f - Ro) * b.importance;
+  coatPick = coatImp * (1.0f / (coatImp + baseImp));
+  basePick = 1.f - coatPick;
+}
+inline V3 sheenReflection(const Bsdf& b, V3 wo, V3 wi, float cosO, float cosI)  // :253-265
+{
+  V3 wh = normalize(wo + wi);
+  float D = b.sheenDist.D(dot(wh, b.fr.N)), G = b.sheenDist.G2(cosO, cosI);
+  return b.sheenColor * (D * G * (1.0f / (4.f * cosO)) * b.sheenScale);
+}
+inline V3 sheenEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)  // :218-279
+{
+  pdf = 0.f;
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+  float basePdf;
+  V3 baseValue = lobeEval(L, b, ctx, wo, wi, basePdf);
+  float Ro = sheenAlbedo(L, cosO, b.sheenRough) * b.sheenScale;
+  float Ri = sheenAlbedo(L, std::fabs(cosI), b.sheenRough) * b.sheenScale;
+  baseValue *= std::min(1.f - Ro, 1.f - Ri);
+  float coatPick, basePick;
+  sheenPicks(b, Ro, coatPick, basePick);
+  if (cosI > 0.f)
+  {
+    pdf = coatPick * (1.f / (2.f * kPi)) + basePick * basePdf;
+    return sheenReflection(b, wo, wi, cosO, cosI) + baseValue;
+  }
+  pdf = basePdf * basePick;
+  return baseValue;
+}
+inline V3 sheenSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)  // :281-391
+{
+  float cosO = dot(wo, b.fr.N);
+  if (!(cosO > 0.f)) return V3(0.f);
+  float Ro = sheenAlbedo(L, cosO, b.sheenRough) * b.sheenScale;
+  float coatPick, basePick;
+  sheenPicks(b, Ro, coatPick, basePick);
+  float basePdf = 0.f;
+  V3 baseValue(0.f);
+  wi = V3(0.f);
+  pdf = 0.f;
+  type = 0;
+  if (sc <= coatPick)
+  {
+    type = BsdfType::DiffuseReflection;
+    float phi = 2.f * kPi * sx;
+    wi = toGlobal(b.fr, V3(std::cos(phi) * cos2sin(sy), std::sin(phi) * cos2sin(sy), sy));  // shape_sampler.h:54-61
+    baseValue = lobeEval(L, b, ctx, wo, wi, basePdf);
+  }
+  else
+  {
+    V3 bwi(0.f);
+    float bpdf = 0.f;
+    int btype = 0;
+    V3 bw = lobeSample(L, b, ctx, wo, bwi, bpdf, btype, sx, sy, (sc - coatPick) * (1.0f / basePick));
+    if (!(reduceMax(bw) > 0.f)) return V3(0.f);
+    wi = bwi;
+    basePdf = bpdf;
+    type = btype;
+    baseValue = bw * ((type & BsdfType::Delta) ? 1.f : basePdf);
+  }
+  float cosI = dot(wi, b.fr.N);
+  if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
+  float Ri = sheenAlbedo(L, std::fabs(cosI), b.sheenRough) * b.sheenScale;
+  baseValue *= std::min(1.f - Ro, 1.f - Ri);
+  if (type & BsdfType::Delta)
+  {
+    pdf = basePdf;
+    return baseValue * (1.0f / basePick);
+  }
+  if (cosI <= 0.f)
+  {
+    pdf = basePick * basePdf;
+    return baseValue * (1.0f / pdf);
+  }
+  pdf = coatPick * (1.f / (2.f * kPi)) + basePick * basePdf;
+  return (sheenReflection(b, wo, wi, cosO, cosI) + baseValue) * (1.0f / pdf);
+}
+// The surface: the lobe, under the sheen layer where there is one.
+inline float surfImportance(const Bsdf& b)  // sheen.h:403-406
+{
+  return b.sheen ? std::max(reduceMax(b.sheenColor), b.importance) : b.importance;
+}
+inline V3 surfEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  return b.sheen ? sheenEval(L, b, ctx, wo, wi, pdf) : lobeEval(L, b, ctx, wo, wi, pdf);
+}
+inline V3 surfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  return b.sheen ? sheenSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc) : lobeSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
+}
+
 // The whole BSDF with the opacity mix (material/standard.cpp:613-627): MultiBsdf{lobe x opacity,
 // TransparentBtdf x transp} (multi_bsdf.h:166-287, transparent.h:40-45); the transparent BTDF
 // evaluates to zero with importance 1; alone (opacity <= 1e-5) it passes every ray straight on.
 inline V3 bsdfEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
 {
-  if (b.mix == 0) return lobeEval(L, b, ctx, wo, wi, pdf);
+  if (b.mix == 0) return surfEval(L, b, ctx, wo, wi, pdf);
   pdf = 0.f;
   if (b.mix == 2) return V3(0.f);
-  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  const float imp0 = surfImportance(b) * b.opacity, sum = imp0 + b.transp;
   V3 v(0.f);
   if (imp0 > 0.f)
   {
     float cp;
-    v = lobeEval(L, b, ctx, wo, wi, cp) * b.opacity;
+    v = surfEval(L, b, ctx, wo, wi, cp) * b.opacity;
     pdf = cp * imp0;
   }
   pdf *= 1.0f / sum;
@@ -1140,7 +1276,7 @@ inline V3 bsdfEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, f
 }
 inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
 {
-  if (b.mix == 0) return lobeSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
+  if (b.mix == 0) return surfSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
   if (b.mix == 2)
   {
     wi = -wo;
@@ -1148,7 +1284,7 @@ inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
     pdf = 1.f;
     return V3(1.f);
   }
-  const float imp0 = b.importance * b.opacity, sum = imp0 + b.transp;
+  const float imp0 = surfImportance(b) * b.opacity, sum = imp0 + b.transp;
   if (!(sum > 0.f)) return V3(0.f);
   // pick a child by importance, remap the sample (:219-245)
   const float x = sc * sum;
@@ -1164,7 +1300,7 @@ inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
     V3 cwi(0.f);
     float cpdf = 0.f;
     int ctype = 0;
-    V3 cw = lobeSample(L, b, ctx, wo, cwi, cpdf, ctype, sx, sy, sc1);
+    V3 cw = surfSample(L, b, ctx, wo, cwi, cpdf, ctype, sx, sy, sc1);
     if (reduceMax(cw) > 0.f) { w = cw * b.opacity; wi = cwi; pdf = cpdf; type = ctype; }
   }
   else
@@ -1184,19 +1320,24 @@ inline V3 bsdfSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
 inline V3 bsdfAlbedo(const Bsdf& b)
 {
   if (b.mix == 2) return V3(-1.f);
-  if (b.mix == 1 && !(b.importance * b.opacity > 0.f)) return V3(0.f);
-  return lobeAlbedo(b);
+  if (b.mix == 1 && !(surfImportance(b) * b.opacity > 0.f)) return V3(0.f);
+  return lobeAlbedo(b);  // the sheen layer passes the base's albedo (sheen.h:408-411)
 }
 // getNormal (multi_bsdf.h:408-428): the first child with a non-zero importance
 inline V3 bsdfNormal(const Bsdf& b)
 {
-  return (b.mix != 2 && b.importance * (b.mix == 1 ? b.opacity : 1.f) > 0.f) ? b.fr.N : V3(0.f);
+  return (b.mix != 2 && b.importance > 0.f && (b.mix == 0 || surfImportance(b) * b.opacity > 0.f)) ? b.fr.N : V3(0.f);
 }
 // getTransparency of a transparent material (material/standard.cpp:633-661)
 inline V3 bsdfTransparency(const Luts& L, const Bsdf& b, V3 wo)
 {
   if (b.mix == 2) return V3(1.f);
   V3 t = (b.kind == kThinDielectric && b.importance > 0.f) ? thinTransparency(L, b, wo) : V3(0.f);
+  if (b.sheen)  // sheen.h:393-401
+  {
+    float cosO = dot(wo, b.fr.N);
+    t = cosO > 0.f ? t * (1.f - sheenAlbedo(L, cosO, b.sheenRough) * b.sheenScale) : V3(0.f);
+  }
   if (b.mix == 0) return t;
   return t * b.opacity + V3(b.transp);
 }

@@ -103,8 +103,29 @@ static void dielTable(float minEta, float maxEta, float* dir, float* avg)  // al
   }
 }
 
+// albedo.h:287-330: the sheen lobe's albedo, uniform hemisphere samples
+static float sheenAlbedoSample(float cosThetaO, const SheenDist& m, float sx, float sy)
+{
+  cosThetaO = std::max(cosThetaO, 1e-6f);
+  V3 wo(cos2sin(cosThetaO), 0.f, cosThetaO);
+  float phi = 2.f * kPi * sx;
+  V3 wi(std::cos(phi) * cos2sin(sy), std::sin(phi) * cos2sin(sy), sy);
+  V3 wh = normalize(wo + wi);
+  return m.D(wh.z) * m.G2(cosThetaO, wi.z) * (1.0f / (4.f * cosThetaO * (1.f / (2.f * kPi))));
+}
+
 void precomputeLuts(Luts& L)
 {
+  // albedo_tables.cpp:155-170
+  for (int j = 0; j < 16; j++)
+  {
+    SheenDist m{roughnessToAlpha(float(j) / 15.f)};
+    for (int i = 0; i < 16; ++i)
+    {
+      const float c = float(i) / 15.f;
+      L.sheenDir[j * 16 + i] = integrateStratified([&](float sx, float sy) { return sheenAlbedoSample(c, m, sx, sy); });
+    }
+  }
   // albedo_tables.cpp:38-59
   for (int j = 0; j < 32; j++)
   {

@@ -359,9 +359,10 @@ static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u,
 // Any-hit traversal with the transparent-shadow filter (render/transparent_filter.h:42-93):
 // each hit on a transparent material multiplies T by the material's transparency along -d and
 // the ray goes on until T falls below 1e-5; a hit on any other material occludes (T = 0).
-static void traverseShadow(const Scene& s, RayHit& r, V3& T)
+static void traverseShadow(const Scene& s, RayHit& r, V3& T, uint32_t& crossed)
 {
   T = V3(1.f);
+  crossed = 0;
   V3 inv(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
   int stack[128];
   int sp = 0;
@@ -385,6 +386,7 @@ static void traverseShadow(const Scene& s, RayHit& r, V3& T)
           Ctx c = postIntersect(s, r.o, r.d, t, tr.prim, u, v);
           Bsdf b = makeBsdf(s.luts, m, c, -r.d, 0, -1);
           T *= bsdfTransparency(s.luts, b, -r.d);
+          ++crossed;
           if (!(reduceMax(T) < 1e-5f)) continue;
         }
         else T = V3(0.f);
@@ -663,7 +665,7 @@ static GradD dielectricGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, D3 N,
   return g;
 }
 
-static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, const float* coatParams = nullptr)
+static GradD lobeEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi)
 {
   GradD g;
   if (!(b.importance > 0.f)) return g;
@@ -740,6 +742,28 @@ static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, 
     double dG = a > 0.001 ? -G * G * (dlam(o0) + dlam(i0)) : 0.0;
     double kf = F / (4 * cosO);
     for (int k = 0; k < 3; ++k) { g.v[k] += kf * D * G; g.dAl[k] += kf * (dD * G + D * dG); }
+  }
+  return g;
+}
+
+// Under a sheen layer (sheen.h:218-279): the lobe attenuated by T = min(1 - Ro, 1 - Ri) plus the
+// sheen reflection, whose own parameters are not differentiated.
+static GradD bsdfEvalGradD(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi)
+{
+  GradD g = lobeEvalGradD(L, b, ctx, wo, wi);
+  if (!b.sheen) return g;
+  const float cosO = dot(wo, b.fr.N), cosI = dot(wi, b.fr.N);
+  if (!(cosO > 0.f) || !(cosI * dot(wi, ctx.Ng) > 0.f)) return GradD();
+  const double Ro = double(sheenAlbedo(L, cosO, b.sheenRough)) * b.sheenScale;
+  const double Ri = double(sheenAlbedo(L, std::fabs(cosI), b.sheenRough)) * b.sheenScale;
+  const double T = std::min(1.0 - Ro, 1.0 - Ri);
+  V3 sr = cosI > 0.f ? sheenReflection(b, wo, wi, cosO, cosI) : V3(0.f);
+  const double srv[3] = {sr.x, sr.y, sr.z};
+  for (int k = 0; k < 3; ++k)
+  {
+    g.v[k] = g.v[k] * T + srv[k];
+    g.dA[k] *= T;
+    g.dAl[k] *= T;
   }
   return g;
 }
@@ -990,10 +1014,12 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       RayHit sr;
       sr.o = shadowO; sr.d = shadowD; sr.tnear = shadowNear; sr.tfar = shadowFar;
       V3 Tsh(1.f);
-      if (s.anyTransparent) traverseShadow(s, sr, Tsh);
+      uint32_t crossed = 0;
+      if (s.anyTransparent) traverseShadow(s, sr, Tsh, crossed);
       else traverse(s, sr, true, nullptr);
       st.rays++;
-      hash = hashStep(hash, sr.prim < 0 ? 1u : 2u);  // the visibility event is part of the path
+      // the visibility event, with the transparent surfaces crossed, is part of the path
+      hash = hashStep(hash, sr.prim < 0 ? 1u + 4u * crossed : 2u);
       if (sr.prim < 0)
       {
         if (s.anyTransparent) { dirL *= Tsh; av.dirL = dirL; }  // the shadow ray's transmittance

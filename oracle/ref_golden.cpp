@@ -16,6 +16,7 @@
 //   bsdf      bsdf/diffuse.h:45-89, bsdf/rough_conductor.h:106-228,
 //             bsdf/rough_dielectric.h:265-548, bsdf/multi_bsdf.h:151-260, bsdf/optics.h
 //   coating   bsdf/rough_coating.h:270-593 over bsdf/diffuse.h:45-89, in a MultiBsdfSimd
+//   sheen     bsdf/sheen.h:199-451 + bsdf/sheen_distribution.h over each lobe kind
 //   lobes     bsdf/rough_diffuse.h:55-106, bsdf/thin_rough_dielectric.h:170-345,
 //             bsdf/transparent.h:26-71, bsdf/multi_bsdf.h:151-459, composed as
 //             material/standard.cpp:409-627 composes them (opacity mix, coat over rough diffuse)
@@ -43,6 +44,7 @@
 #include "bsdf/rough_diffuse.h"
 #include "bsdf/thin_rough_dielectric.h"
 #include "bsdf/transparent.h"
+#include "bsdf/sheen.h"
 #include "geometry/triangle_mesh.h"
 
 using namespace prt;
@@ -285,6 +287,7 @@ static void goldenLuts()
   dump("lut_diel_rcpeta_avg", "f4", v(MicrofacetDielectricAlbedoTable::rcpEtaAvg, 16 * 16), {16, 16});
   dump("lut_diel_refl_dir", "f4", v(MicrofacetDielectricReflectionAlbedoTable::etaDir, 16 * 16 * 16), {16, 16, 16});
   dump("lut_diel_refl_avg", "f4", v(MicrofacetDielectricReflectionAlbedoTable::etaAvg, 16 * 16), {16, 16});
+  dump("lut_sheen_dir", "f4", v(MicrofacetSheenAlbedoTable::dir, 16 * 16), {16, 16});
 }
 
 // BSDF kinds: 0 diffuse, 1 rough conductor (artistic Fresnel), 2 rough dielectric.  Each is
@@ -668,6 +671,130 @@ static void goldenLobes()
   dump("lobe_transparency", "f4", outTransp, {n, 3});
 }
 
+// The sheen layer (material/standard.cpp:599-611): SheenBsdfSimd over the lobe's MultiBsdfSimd,
+// under the opacity mix when there is one.  Variants by batch: 0 diffuse, 1 rough conductor,
+// 2 coated diffuse, 3 thin rough dielectric, 4 rough diffuse with transparency, 5 rough
+// dielectric (from outside).  params: the 13 columns of the lobe rows, then sheen weight, sheen
+// color rgb, sheen roughness.
+static void goldenSheen()
+{
+  InputRng rng(7373);
+  const int nTotal = 768;
+  std::vector<float> params, geo, samp, outSample, outEval, outAlb, outNormal, outTransp;
+  std::vector<int32_t> outType;
+  int n = 0;
+  for (int i = 0; i < nTotal; i += simdSize)
+  {
+    const int batch = i / simdSize, variant = batch % 6;
+    const int kinds[6] = {0, 1, 3, 4, 0, 2};
+    const int kind = kinds[variant];
+    Vec3f col(0.05f + 0.9f * rng.f(), 0.05f + 0.9f * rng.f(), 0.05f + 0.9f * rng.f());
+    Vec3f edge(rng.f(), rng.f(), rng.f());
+    Vec3f coatCol(1.f);
+    float roughness = std::sqrt(0.01f + 0.5f * rng.f());
+    float diffRough = variant == 4 ? 0.1f + 0.9f * rng.f() : 0.f;
+    float ior = (batch % 2) ? 1.5f : 1.4f;
+    float thickness = kind == 4 ? 0.2f + 1.8f * rng.f() : 1.f;
+    float weight = kind == 3 ? 1.f : 0.f;
+    float transparency = variant == 4 ? 0.1f + 0.8f * rng.f() : 0.f;
+    float sheenW = 0.2f + 0.8f * rng.f();
+    Vec3f sheenCol(0.2f + 0.8f * rng.f(), 0.2f + 0.8f * rng.f(), 0.2f + 0.8f * rng.f());
+    float sheenR = 0.05f + 0.95f * rng.f();
+    const float eta = std::min(std::max(1.f / ior, 1.f / 3.f), 3.f);
+
+    Basis3vf frame;
+    ShadingContextSimd* ctx = new ShadingContextSimd();
+    Vec3vf wo, wi, s3;
+    for (int k = 0; k < simdSize; ++k)
+    {
+      Vec3f N = rng.dir();
+      Vec3f U, V;
+      makeFrame(U, V, N);
+      Vec3f Ng = N;
+      if (rng.f() < 0.3f) { Ng = normalize(N + 0.2f * rng.dir()); if (dot(Ng, N) < 0.2f) Ng = N; }
+      Vec3f o = rng.dir();
+      if (rng.f() < 0.9f && dot(o, N) < 0.f) o = -o;
+      Vec3f w = rng.dir();
+      if (rng.f() < 0.7f && dot(w, N) < 0.f) w = -w;
+      frame.U.x[k] = U.x; frame.U.y[k] = U.y; frame.U.z[k] = U.z;
+      frame.V.x[k] = V.x; frame.V.y[k] = V.y; frame.V.z[k] = V.z;
+      frame.N.x[k] = N.x; frame.N.y[k] = N.y; frame.N.z[k] = N.z;
+      ctx->Ng.x[k] = Ng.x; ctx->Ng.y[k] = Ng.y; ctx->Ng.z[k] = Ng.z;
+      wo.x[k] = o.x; wo.y[k] = o.y; wo.z[k] = o.z;
+      wi.x[k] = w.x; wi.y[k] = w.y; wi.z[k] = w.z;
+      s3.x[k] = rng.f(); s3.y[k] = rng.f(); s3.z[k] = rng.f();
+      const Vec3f e3 = kind == 3 ? coatCol : edge;
+      params.insert(params.end(), {float(kind), col.x, col.y, col.z, e3.x, e3.y, e3.z, roughness, ior, thickness, weight,
+                                   transparency, diffRough, sheenW, sheenCol.x, sheenCol.y, sheenCol.z, sheenR});
+      geo.insert(geo.end(), {U.x, U.y, U.z, V.x, V.y, V.z, N.x, N.y, N.z, Ng.x, Ng.y, Ng.z, o.x, o.y, o.z, w.x, w.y, w.z});
+      samp.insert(samp.end(), {s3.x[k], s3.y[k], s3.z[k]});
+    }
+    vbool m = one;
+    BsdfSimd* base = nullptr;
+    BsdfSimd* child = nullptr;
+    if (kind == 0)
+    {
+      if (diffRough > 0.f) child = new RoughDiffuseBrdfSimd(m, &frame, Vec3vf(col), vfloat(diffRough));
+      else child = new DiffuseBrdfSimd(m, &frame, Vec3vf(col));
+    }
+    else if (kind == 1)
+      child = new RoughConductorBrdfSimd<FresnelConductorArtistic<vfloat>>(m, &frame, FresnelConductorArtistic<vfloat>(Vec3vf(col), Vec3vf(edge)), vfloat(roughness), vfloat(0.f));
+    else if (kind == 2)
+      child = new RoughDielectricBsdfSimd(m, &frame, vfloat(eta), vfloat(roughness), vfloat(0.f));
+    else if (kind == 3)
+    {
+      base = new DiffuseBrdfSimd(m, &frame, Vec3vf(col));
+      child = new RoughCoatingBsdfSimd(m, &frame, base, vfloat(eta), Vec3vf(coatCol), vfloat(thickness), vfloat(roughness),
+                                       vfloat(0.f), vfloat(weight));
+    }
+    else
+      child = new ThinRoughDielectricBsdfSimd(m, &frame, vfloat(eta), vfloat(roughness), vfloat(0.f), Vec3vf(col), vfloat(thickness));
+    MultiBsdfSimd inner, outer;
+    inner.add(m, child, vfloat(1.f));
+    SheenBsdfSimd sheen(m, &frame, &inner, Vec3vf(sheenCol), vfloat(sheenR), vfloat(sheenW));
+    TransparentBtdfSimd transparent;
+    const BsdfSimd* bsdf = &sheen;
+    const float opacity = 1.f - transparency;
+    if (transparency > 1e-5f)
+    {
+      outer.add(m, &sheen, vfloat(opacity));
+      outer.add(m, &transparent, vfloat(transparency));
+      bsdf = &outer;
+    }
+    Vec3vf swi = zero;
+    vfloat spdf = zero;
+    vint stype = zero;
+    Vec3vf sw = bsdf->sample(m, *ctx, wo, swi, spdf, stype, BsdfSampleSimd(s3));
+    vfloat epdf;
+    Vec3vf ev = bsdf->eval(m, *ctx, wo, wi, epdf);
+    Vec3vf alb = bsdf->getAlbedo(m);
+    Vec3vf nrm = bsdf->getNormal(m);
+    Vec3vf tr = kind == 4 ? bsdf->getTransparency(m, *ctx, wo) : Vec3vf(vfloat(1.f - opacity));
+    for (int k = 0; k < simdSize; ++k)
+    {
+      outSample.insert(outSample.end(), {sw.x[k], sw.y[k], sw.z[k], swi.x[k], swi.y[k], swi.z[k], spdf[k]});
+      outType.push_back(stype[k]);
+      outEval.insert(outEval.end(), {ev.x[k], ev.y[k], ev.z[k], epdf[k]});
+      outAlb.insert(outAlb.end(), {alb.x[k], alb.y[k], alb.z[k]});
+      outNormal.insert(outNormal.end(), {nrm.x[k], nrm.y[k], nrm.z[k]});
+      outTransp.insert(outTransp.end(), {tr.x[k], tr.y[k], tr.z[k]});
+    }
+    delete child;
+    delete base;
+    delete ctx;
+    n += simdSize;
+  }
+  dump("sheen_params", "f4", params, {n, 18});
+  dump("sheen_geo", "f4", geo, {n, 18});
+  dump("sheen_s", "f4", samp, {n, 3});
+  dump("sheen_sample", "f4", outSample, {n, 7});
+  dump("sheen_sample_type", "i4", outType, {n});
+  dump("sheen_eval", "f4", outEval, {n, 4});
+  dump("sheen_albedo", "f4", outAlb, {n, 3});
+  dump("sheen_normal", "f4", outNormal, {n, 3});
+  dump("sheen_transparency", "f4", outTransp, {n, 3});
+}
+
 int main(int argc, char** argv)
 {
   if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
@@ -685,6 +812,7 @@ int main(int argc, char** argv)
   goldenCoating();
   goldenPostIntersect();
   goldenLobes();
+  goldenSheen();
   fprintf(gManifest, "\n}\n");
   fclose(gManifest);
   return 0;

@@ -50,6 +50,7 @@ def test_luts(dev, golden):
         np.testing.assert_allclose(lut[off:off + n], orc[name], atol=1e-6, err_msg=name)
         off += n
     np.testing.assert_allclose(lut[:1024], golden["lut_ggx_dir"].ravel(), atol=5e-6)
+    np.testing.assert_allclose(lut[-256:], golden["lut_sheen_dir"].ravel(), atol=5e-6)
 
 
 def _random_scene(seed, ntri=3000):
@@ -170,11 +171,12 @@ def test_render_parity_sunsky_coated(dev):
 
 def test_render_parity_remaining_lobes(dev):
     # Oren-Nayar bases (alone and under a coat), thin rough dielectrics, opacity mixes with the
-    # transparent BTDF (one material fully transparent), and shadow rays through the thin and
-    # transparent surfaces (the transparent-shadow filter)
+    # transparent BTDF (one material fully transparent), sheen layers over every kind, and shadow
+    # rays through the thin and transparent surfaces (the transparent-shadow filter)
     scene = G.spheres_lobes()
     kinds = scene["materials"]["kind"]
     assert (kinds == G.KIND_THIN_DIELECTRIC).sum() >= 3 and (scene["materials"]["transparency"] > 0).sum() >= 9
+    assert (scene["materials"]["sheen_weight"] > 0).sum() >= 10
     _parity(dev, scene, 96, 64, 4, 0.97)
 
 

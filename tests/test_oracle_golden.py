@@ -96,6 +96,8 @@ def test_energy_compensation_luts(golden):
     refl_dir[:, 15, 0] = ref_refl_dir[:, 15, 0] = 0
     np.testing.assert_allclose(refl_dir, ref_refl_dir, atol=2e-5)
     np.testing.assert_allclose(L["diel_refl_avg"], golden["lut_diel_refl_avg"].ravel(), atol=1e-3)
+    # MicrofacetSheenAlbedoTable (albedo_tables.cpp:155-170)
+    np.testing.assert_allclose(L["sheen_dir"], golden["lut_sheen_dir"].ravel(), atol=5e-6)
 
 
 @pytest.mark.parametrize("kind", [0, 1, 2])
@@ -190,6 +192,48 @@ def test_remaining_lobes_sample_eval(golden, variant):
         types = set(np.unique(st[valid]))
         assert 128 in types  # the transparent BTDF gets sampled (BsdfType::Transparency)
         assert len(types) > 1 or variant == 5
+
+
+def sheen_materials(golden):
+    """Material records of the golden sheen rows (MATERIAL_DTYPE)."""
+    p = golden["sheen_params"]
+    m = np.zeros(p.shape[0], O.MATERIAL_DTYPE)
+    m["kind"] = p[:, 0].astype(np.int32)
+    for k, f in enumerate(("r", "g", "b", "er", "eg", "eb", "roughness", "ior", "thickness", "weight", "transparency",
+                           "diffuse_roughness", "sheen_weight")):
+        m[f] = p[:, 1 + k]
+    m["sheen_color"] = p[:, 14:17]
+    m["sheen_roughness"] = p[:, 17]
+    return m
+
+
+@pytest.mark.parametrize("variant", range(6))
+def test_sheen_layer_sample_eval(golden, variant):
+    # bsdf/sheen.h:199-451 (Conty-Kulla microfiber distribution, sheen_distribution.h) over the
+    # lobe of each kind, as material/standard.cpp:599-611 layers it; variants: 0 diffuse,
+    # 1 conductor, 2 coated diffuse, 3 thin dielectric, 4 rough diffuse + transparency,
+    # 5 rough dielectric
+    flat, _ = O.compute_luts()
+    mats = sheen_materials(golden)
+    so, st, ev, aux = O.bsdf_mat(flat, mats, golden["sheen_geo"], golden["sheen_s"])
+    sel = (np.arange(len(mats)) // 8) % 6 == variant
+    ref_s, ref_t, ref_e = golden["sheen_sample"][sel], golden["sheen_sample_type"][sel], golden["sheen_eval"][sel]
+    so, st, ev, aux = so[sel], st[sel], ev[sel], aux[sel]
+    valid_ref = np.abs(ref_s[:, :3]).max(1) > 0
+    valid = np.abs(so[:, :3]).max(1) > 0
+    assert valid.mean() > 0.5
+    assert np.array_equal(valid, valid_ref)
+    assert np.array_equal(st, ref_t)
+    assert (st[valid] == 1).any()  # the sheen reflection gets sampled (BsdfType::DiffuseReflection)
+    np.testing.assert_allclose(so[valid, :3], ref_s[valid, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(so[valid, 3:6], ref_s[valid, 3:6], atol=2e-5)
+    np.testing.assert_allclose(so[:, 6], ref_s[:, 6], rtol=1e-3, atol=1e-7)
+    np.testing.assert_allclose(ev[:, :3], ref_e[:, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(ev[:, 3], ref_e[:, 3], rtol=1e-3, atol=1e-6)
+    np.testing.assert_allclose(aux[:, 0:3], golden["sheen_albedo"][sel], rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(aux[:, 3:6], golden["sheen_normal"][sel], atol=1e-6)
+    if variant == 3:
+        np.testing.assert_allclose(aux[:, 6:9], golden["sheen_transparency"][sel], rtol=5e-4, atol=1e-6)
 
 
 @pytest.mark.parametrize("suffix", ["", "_n"])
