@@ -55,7 +55,9 @@ typedef struct prt_context prt_context;
  * (render/transparent_filter.h:42-93).
  * diffuse_roughness (kinds 0 and 3): above 0 the diffuse base is the Oren-Nayar BRDF of
  * rough_diffuse.h (Fujii's form) instead of the Lambertian one (standard.cpp:446-453).
- * sheen_weight, sheen_color, sheen_roughness: the sheen layer (standard.cpp:599-611, sheen.h).
+ * sheen_weight, sheen_color, sheen_roughness: the sheen layer over any kind (standard.cpp:
+ * 599-611, bsdf/sheen.h), active above a weight of 1e-5; the reference's default sheen
+ * roughness is 0.3 (a zero here means roughness 0, alpha 0.001).
  * emission = emissionColor * emissionLuminance (standard.cpp:71-74, :663-666), any kind: a
  * non-zero emission makes the material emissive, its triangles form the scene's area light
  * (light/triangle_light.h, render/scene.cpp:83-116) and a path that hits one ends there.
@@ -204,7 +206,9 @@ int prt_query_rays_device(prt_context* ctx, int32_t n, const float* rays_device,
  * floats, >= 4 x the material count).  Differentiated: the diffuse colour of kinds 0 and 3, the
  * reflectivity (baseColor) of kind 1, and alpha (= roughness^2) of kinds 1, 2 and 3 (the
  * microfacet lobe, through D, G2 and the albedo tables of the multiple-scattering term); kind 0
- * has no alpha and kind 2 no albedo, so those entries are 0.  The gradient is that of the
+ * has no alpha, kind 2 no albedo and kind 4 is not differentiated, so those entries are 0.  A
+ * sheen layer attenuates the derivatives of the lobe under it; opacity scales value and
+ * derivatives alike.  The gradient is that of the
  * primal estimator with its samples fixed (detached sampling); paths are replayed from the
  * seed, not stored.  Requires maxDepth <= 8.  *ms: device time of the replay kernels. */
 int prt_adjoint(prt_context* ctx, const float* target, int32_t spp, double* loss, float* grad, int32_t grad_capacity,
