@@ -191,8 +191,27 @@ int prt_frame_error(prt_context* ctx, float* error);
 /* --- queries (Renderer::queryRay / queryOcclusionRay, render/renderer.h:47-128) ------- */
 /* rays: n x [ox oy oz dx dy dz near far]; closest hit per ray: prim (-1 = miss), t, u, v. */
 int prt_query_rays(prt_context* ctx, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v);
-/* any hit in [near, far]: occluded[i] = 1 / 0. */
+/* any hit in [near, far]: occluded[i] = 1 / 0.  In a scene with transparent materials the ray
+ * passes them as a shadow ray does (renderer_stream.h:64-97 with the transparent-shadow
+ * filter): occluded once the transmittance falls below 1e-5. */
 int prt_query_occlusion(prt_context* ctx, int32_t n, const float* rays, int32_t* occluded);
+/* One hit of a pixel query (Renderer::queryPixel, render/renderer.h:50-121, with the hit
+ * properties of RendererStream::queryRay, render/renderer_stream.h:16-62). */
+typedef struct prt_pixel_hit {
+  int32_t prim, material;
+  int32_t material_type;  /* MaterialType bits (material/material.h:17-25): 1 transparent,
+                             4 transmissive, 8 media interface, 16 emissive */
+  float p[3], Ng[3], N[3], U[3], V[3];  /* hit point, oriented geometric normal, shading frame */
+  float uv[2];            /* the hit's barycentric (u, v): the uv the reference reports without
+                             texture coordinates (geometry/triangle_mesh.cpp:131) */
+  float T[3];             /* transparency along the ray (0 for an opaque material) */
+  float eps, dist, depth; /* hit offset, distance along the ray, distance along the view axis */
+} prt_pixel_hit;
+/* The ray through image point (px, py) in [0, 1]^2 (x right, y down; (x + 0.5) / W,
+ * (y + 0.5) / H for pixel (x, y)) of the current camera with the lens at its centre: up to
+ * max_hits (<= 32) hits, stepping past each transparent one (T >= 1e-5) by 16 x its offset.
+ * Needs the scene, BVH, camera and frame size. */
+int prt_query_pixel(prt_context* ctx, float px, float py, int32_t max_hits, prt_pixel_hit* hits, int32_t* count);
 /* Same two queries on device buffers already in HBM, on the context stream (asynchronous). */
 int prt_query_rays_device(prt_context* ctx, int32_t n, const float* rays_device, int32_t* prim_device,
                           float* t_device, float* u_device, float* v_device);

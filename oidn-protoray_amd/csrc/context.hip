@@ -1310,9 +1310,50 @@ int prt_query_rays(prt_context* c, int32_t n, const float* rays, int32_t* prim, 
 
 int prt_query_occlusion(prt_context* c, int32_t n, const float* rays, int32_t* occluded)
 {
+  CTX_CHECK(c);
+  if (c->anyTransparent && c->bvhReady && n > 0 && rays && occluded)
+  {
+    // the transparent-shadow filter needs the materials: the renderer's argument block
+    RenderArgs a = makeArgs(c);
+    DevBuf<RenderArgs> da;
+    DevBuf<float> dr;
+    DevBuf<int> docc;
+    HIP_TRY(da.alloc(1));
+    HIP_TRY(dr.alloc(size_t(8) * n));
+    HIP_TRY(docc.alloc(n));
+    HIP_TRY(hipMemcpy(da.p, &a, sizeof(a), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dr.p, rays, sizeof(float) * 8 * n, hipMemcpyHostToDevice));
+    launchQueryOcclusionFilter(da.p, n, dr.p, docc.p, c->launch.gridTrace, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(occluded, docc.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    return PRT_OK;
+  }
   int rc = queryHost(c, n, rays, occluded, nullptr, nullptr, nullptr, 1);
   if (rc != PRT_OK) return rc;
   for (int i = 0; i < n; ++i) occluded[i] = occluded[i] >= 0 ? 1 : 0;
+  return PRT_OK;
+}
+
+int prt_query_pixel(prt_context* c, float px, float py, int32_t max_hits, prt_pixel_hit* hits, int32_t* count)
+{
+  CTX_CHECK(c);
+  if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
+  if (c->W <= 0 || c->H <= 0) return fail(PRT_ERR_STATE, "no frame: the pixel query needs the image size");
+  if (max_hits < 1 || max_hits > 32 || !hits || !count) return fail(PRT_ERR_INVALID, "invalid pixel query arguments");
+  RenderArgs a = makeArgs(c);
+  DevBuf<RenderArgs> da;
+  DevBuf<prt_pixel_hit> dh;
+  DevBuf<int> dn;
+  HIP_TRY(da.alloc(1));
+  HIP_TRY(dh.alloc(max_hits));
+  HIP_TRY(dn.alloc(1));
+  HIP_TRY(hipMemcpy(da.p, &a, sizeof(a), hipMemcpyHostToDevice));
+  launchQueryPixel(da.p, px, py, max_hits, dh.p, dn.p, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(count, dn.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (*count > 0) HIP_TRY(hipMemcpy(hits, dh.p, sizeof(prt_pixel_hit) * size_t(*count), hipMemcpyDeviceToHost));
   return PRT_OK;
 }
 

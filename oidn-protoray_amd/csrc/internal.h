@@ -8,6 +8,8 @@
 #include "dev_math.h"
 #include "trace.h"
 
+struct prt_pixel_hit;  // include/prt_hip.h
+
 namespace prt {
 
 struct BvhData
@@ -112,6 +114,8 @@ void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStre
 void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s);
 void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, bool ext, hipStream_t s);
 void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, bool filter, hipStream_t s);
+void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s);
+void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
                  int* spill, int spillStride, int grid, hipStream_t s);
 // Path-replay adjoint (adjoint.hip): partial is [blocks][nParams] (zeroed by the caller before the

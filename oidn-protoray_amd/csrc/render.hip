@@ -14,6 +14,7 @@
 // one sample per pass, so the FP32 accumulators are updated without atomics and in sample
 // order, as AccumBuffer3f::addSerial does (image/accum_buffer.h:439-486).
 #include "path.h"
+#include "../../include/prt_hip.h"
 
 namespace prt {
 
@@ -420,6 +421,79 @@ __global__ void __launch_bounds__(kTraceBlock) kQuery(BvhView bvh, int n, const 
   }
 }
 
+// Shadow-style occlusion queries in a scene with transparent materials: the transparent-shadow
+// filter decides (render/renderer_stream.h:64-97).
+__global__ void __launch_bounds__(kTraceBlock) kQueryOcclusionFilter(const RenderArgs* __restrict__ ap, int n,
+                                                                     const float* __restrict__ rays, int* __restrict__ occ)
+{
+  const RenderArgs& a = *ap;
+  __shared__ TraceLds<kTraceBlock> lds;
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  TraceStack st = makeStack(lds, a.spill, a.spillStride, gtid);
+  uint32_t nodes = 0, tris = 0;
+  for (int i = gtid; i < n; i += gridDim.x * blockDim.x)
+  {
+    const float* r = rays + 8 * i;
+    f3 o(r[0], r[1], r[2]), d(r[3], r[4], r[5]);
+    float t, u, v;
+    ShadowFilter flt{&a, o, d, f3(1.f), 0u};
+    occ[i] = traceRay<true>(a.bvh, o, d, r[6], r[7], t, u, v, st, nodes, tris, flt) >= 0 ? 1 : 0;
+  }
+}
+
+// Renderer::queryPixel (render/renderer.h:82-115): one lane walks the hits of the pixel's ray.
+__global__ void __launch_bounds__(64) kQueryPixel(const RenderArgs* __restrict__ ap, float px, float py, int maxHits,
+                                                  prt_pixel_hit* __restrict__ out, int* __restrict__ count)
+{
+  const RenderArgs& a = *ap;
+  __shared__ TraceLds<64> lds;
+  TraceStack st = makeStack(lds, a.spill, a.spillStride, threadIdx.x);
+  if (threadIdx.x != 0) return;
+  // camera ray with the lens sample at the centre (pinhole_camera.h:128-133, thin_lens_camera.h:175-182)
+  auto ray = [&](float x, float y, f3& d, float& tnear) {
+    d = ld3(a.imageO) + x * ld3(a.imageDx) + y * ld3(a.imageDy);
+    if (!a.pinhole) d = a.focal * d;
+    float len = length(d);
+    d = d * (1.0f / len);
+    tnear = (a.pinhole ? a.nearClip : a.nearClipScale) * len;
+  };
+  f3 dc, d;
+  float tc, tnear;
+  ray(0.5f, 0.5f, dc, tc);
+  ray(fminf(fmaxf(px, 0.f), 1.f), fminf(fmaxf(py, 0.f), 1.f), d, tnear);
+  const f3 o = ld3(a.camPos);
+  uint32_t nodes = 0, tris = 0;
+  int n = 0;
+  for (; n < maxHits; ++n)
+  {
+    float t, u = 0.f, v = 0.f;
+    const int prim = traceRay<false>(a.bvh, o, d, tnear, kPosMax, t, u, v, st, nodes, tris);
+    if (prim < 0) break;
+    const Ctx c = postIntersect(a, o, d, t, prim, u, v);
+    const int matId = a.triMat[prim];
+    const prt_material_constants& m = a.mats[matId];
+    const bool transmissive = m.kind == PRT_KIND_DIELECTRIC || m.kind == PRT_KIND_THIN_DIELECTRIC;
+    f3 T(0.f);
+    if (m.transparent) T = bsdfTransparency(a.lut, makeBsdf(m, c, -d, false), -d);
+    prt_pixel_hit& h = out[n];
+    h.prim = prim;
+    h.material = matId;
+    h.material_type = (m.transparent ? 1 : 0) | (transmissive ? 4 : 0) | (m.kind == PRT_KIND_DIELECTRIC ? 8 : 0) |
+                      (m.emissive ? 16 : 0);
+    const f3 vals[6] = {c.p, c.Ng, c.f.N, c.f.U, c.f.V, T};
+    float* dst[6] = {h.p, h.Ng, h.N, h.U, h.V, h.T};
+    for (int k = 0; k < 6; ++k) { dst[k][0] = vals[k].x; dst[k][1] = vals[k].y; dst[k][2] = vals[k].z; }
+    h.uv[0] = u;
+    h.uv[1] = v;
+    h.eps = c.eps;
+    h.dist = t;
+    h.depth = t * dot(d, dc);
+    if (!(reduceMax(T) >= 1e-5f)) { ++n; break; }
+    tnear = t + c.eps * 16.f;  // the next hit (renderer.h:111-113)
+  }
+  *count = n;
+}
+
 __global__ void kDebugSampler(int n, const uint32_t* __restrict__ in, int ndim, float* o1, float* o2, float* o3)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -458,6 +532,14 @@ void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float*
                  int spillStride, int grid, hipStream_t s)
 {
   kQuery<<<grid, kTraceBlock, 0, s>>>(bvh, n, rays, prim, t, u, v, anyHit, spill, spillStride);
+}
+void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s)
+{
+  kQueryOcclusionFilter<<<grid, kTraceBlock, 0, s>>>(a, n, rays, occ);
+}
+void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, hipStream_t s)
+{
+  kQueryPixel<<<1, 64, 0, s>>>(a, px, py, maxHits, out, count);
 }
 void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s)
 {

@@ -27,6 +27,9 @@ MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<
                            ("emission", "<f4", 3), ("transparency", "<f4"),
                            ("diffuse_roughness", "<f4"), ("sheen_weight", "<f4"), ("sheen_color", "<f4", 3),
                            ("sheen_roughness", "<f4")])
+PIXEL_HIT_DTYPE = np.dtype([("prim", "<i4"), ("material", "<i4"), ("material_type", "<i4"), ("p", "<f4", 3),
+                            ("Ng", "<f4", 3), ("N", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("uv", "<f4", 2),
+                            ("T", "<f4", 3), ("eps", "<f4"), ("dist", "<f4"), ("depth", "<f4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
@@ -70,6 +73,7 @@ SIGNATURES = {
     "prt_frame_error": [_vp, ctypes.POINTER(ctypes.c_float)],
     "prt_query_rays": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_query_occlusion": [_vp, _i, _vp, _vp],
+    "prt_query_pixel": [_vp, ctypes.c_float, ctypes.c_float, _i, _vp, _vp],
     "prt_query_rays_device": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_debug_enable": [_vp, _i],
     "prt_debug_get": [_vp, _vp, _vp, _vp],
@@ -238,6 +242,17 @@ class DeviceHip:
         occ = np.zeros(r.shape[0], np.int32)
         _check(_lib.prt_query_occlusion(self.h, r.shape[0], _ptr(r), _ptr(occ)))
         return occ
+
+    def queryPixel(self, x, y, max_hits=32):
+        """Renderer::queryPixel (render/renderer.h:82-121): the hits of the ray through the centre
+        of pixel (x, y) with the lens at its centre, stepping past transparent ones; a
+        PIXEL_HIT_DTYPE array (prim, material, material_type, p, Ng, N, U, V, uv, T, eps, dist,
+        depth)."""
+        W, H = self.size
+        hits = np.zeros(max_hits, PIXEL_HIT_DTYPE)
+        n = ctypes.c_int32(0)
+        _check(_lib.prt_query_pixel(self.h, (x + 0.5) / W, (y + 0.5) / H, max_hits, _ptr(hits), ctypes.byref(n)))
+        return hits[:n.value]
 
     # --- debug / parity hooks ---------------------------------------------------------------
     def enableDebug(self, enable=True):
