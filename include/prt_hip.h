@@ -170,6 +170,14 @@ int prt_renderer_init(prt_context* ctx, const prt_renderer_params* params);
 int prt_sampler_init(prt_context* ctx, uint32_t seed);
 /* Device::initCamera (render/device_cpu.cpp:130-134). */
 int prt_set_camera(prt_context* ctx, const prt_camera* camera);
+/* The camera's matrices for an image of width x height (camera/pinhole_camera.h:41-68, the same
+ * for the thin lens; Device::getWorldToViewD3D / getViewToClipD3D / getWorldToRaster,
+ * render/device.h:85-87), each 16 floats row-major: worldToViewD3D (row-vector convention, the
+ * translation in the last row), viewToClipD3D (reverse infinite perspective), worldToRaster
+ * (column-vector convention: raster = M [p 1], divided by its w).  Host only: no device is
+ * touched.  Any output may be null. */
+int prt_camera_matrices(const prt_camera* camera, int32_t width, int32_t height, float* world_to_view_d3d,
+                        float* view_to_clip_d3d, float* world_to_raster);
 /* Device::initFrame / clearFrame (render/device_cpu.cpp:178-232, FrameBuffer::clear).
  * Width must be a multiple of 32 and height of 8, each <= 4096
  * (render/path_renderer_stream.h:109-112). */

@@ -970,6 +970,45 @@ static void setCameraArgs(RenderArgs& a, const prt_camera& k, int W, int H)
   a.invH = 1.0f / float(H);
 }
 
+// Camera matrices (camera/pinhole_camera.h:41-68) in the reference's operation order; Mat4 is
+// stored by columns and multiplied column by column (math/mat4.h:64-74).
+int prt_camera_matrices(const prt_camera* k, int32_t width, int32_t height, float* wv, float* vc, float* wr)
+{
+  if (!k || width <= 0 || height <= 0) return fail(PRT_ERR_INVALID, "invalid camera or image size");
+  const float W = float(width), H = float(height), aspect = W / H;
+  const float hh = std::tan(0.5f * k->fov), hw = hh * aspect;
+  const H3 P = hmk(k->pos[0], k->pos[1], k->pos[2]);
+  const H3 U = hmk(k->U[0], k->U[1], k->U[2]), V = hmk(k->V[0], k->V[1], k->V[2]), N = hmk(k->N[0], k->N[1], k->N[2]);
+  // row-major values as written in the constructors
+  const float worldToView[16] = {U.x, U.y, U.z, -hdot(U, P), V.x, V.y, V.z, -hdot(V, P), N.x, N.y, N.z, -hdot(N, P), 0, 0, 0, 1};
+  const float viewToRaster[16] = {W / (2.f * hw), 0, -W / 2.f, 0, 0, -H / (2.f * hh), -H / 2.f, 0, 0, 0, 0, k->nearClip, 0, 0, -1, 0};
+  if (wr)
+  {
+    // (a * b) column j = a[0] b[j].x + a[1] b[j].y + a[2] b[j].z + a[3] b[j].w, left to right
+    for (int j = 0; j < 4; ++j)
+      for (int r = 0; r < 4; ++r)
+      {
+        float acc = viewToRaster[r * 4 + 0] * worldToView[0 * 4 + j];
+        acc = acc + viewToRaster[r * 4 + 1] * worldToView[1 * 4 + j];
+        acc = acc + viewToRaster[r * 4 + 2] * worldToView[2 * 4 + j];
+        acc = acc + viewToRaster[r * 4 + 3] * worldToView[3 * 4 + j];
+        wr[r * 4 + j] = acc;
+      }
+  }
+  const H3 F = hmk(-N.x, -N.y, -N.z);
+  if (wv)
+  {
+    const float m[16] = {U.x, V.x, F.x, 0, U.y, V.y, F.y, 0, U.z, V.z, F.z, 0, -hdot(U, P), -hdot(V, P), -hdot(F, P), 1};
+    std::memcpy(wv, m, sizeof(m));
+  }
+  if (vc)
+  {
+    const float m[16] = {1.f / hw, 0, 0, 0, 0, 1.f / hh, 0, 0, 0, 0, 0, 1, 0, 0, k->nearClip, 0};
+    std::memcpy(vc, m, sizeof(m));
+  }
+  return PRT_OK;
+}
+
 static RenderArgs makeArgs(prt_context* c)
 {
   RenderArgs a;

@@ -74,6 +74,7 @@ SIGNATURES = {
     "prt_query_rays": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_query_occlusion": [_vp, _i, _vp, _vp],
     "prt_query_pixel": [_vp, ctypes.c_float, ctypes.c_float, _i, _vp, _vp],
+    "prt_camera_matrices": [_vp, _i, _i, _vp, _vp, _vp],
     "prt_query_rays_device": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_debug_enable": [_vp, _i],
     "prt_debug_get": [_vp, _vp, _vp, _vp],
@@ -133,6 +134,16 @@ def _check(rc):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def camera_matrices(camera, width, height):
+    """(worldToViewD3D, viewToClipD3D, worldToRaster) of a CAMERA_DTYPE record for an image of
+    width x height, each 4 x 4 in row-major notation (camera/pinhole_camera.h:41-68).  Host only."""
+    cam = np.ascontiguousarray(camera, CAMERA_DTYPE)
+    out = np.zeros((3, 16), np.float32)
+    _check(load_library().prt_camera_matrices(_ptr(cam), int(width), int(height), _ptr(out[0]), _ptr(out[1]),
+                                              _ptr(out[2])))
+    return tuple(m.reshape(4, 4) for m in out)
 
 
 class DeviceHip:

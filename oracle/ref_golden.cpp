@@ -539,6 +539,45 @@ static void goldenPostIntersect()
   }
 }
 
+// Camera matrices of PinholeCamera (camera/pinhole_camera.h:41-68; ThinLensCamera inherits them):
+// worldToViewD3D, viewToClipD3D and worldToRaster, written row-major (m[r][c] = column c, row r).
+// in: [width, height, fov, nearClip, pos xyz, U xyz, V xyz, N xyz]
+static void goldenCameraMatrices()
+{
+  InputRng rng(4321);
+  std::vector<float> in, out;
+  const int nCam = 6;
+  for (int i = 0; i < nCam; ++i)
+  {
+    const int w = 32 * (8 + int(rng.f() * 52)), h = 8 * (8 + int(rng.f() * 120));
+    const float fov = (20.f + 70.f * rng.f()) * float(pi) / 180.f, nearClip = 0.005f + 0.1f * rng.f();
+    const Vec3f pos(20.f * rng.f() - 10.f, 20.f * rng.f() - 10.f, 20.f * rng.f() - 10.f);
+    Vec3f N = i == 0 ? Vec3f(0.f, 0.f, -1.f) : rng.dir();
+    Vec3f U, V;
+    if (i == 0) { U = Vec3f(-1.f, 0.f, 0.f); V = Vec3f(0.f, 1.f, 0.f); }
+    else makeFrame(U, V, N);
+    const Basis3f basis(U, V, N);
+    Props props;
+    props.set("position", pos);
+    props.set("basis", basis);
+    props.set("nearClip", nearClip);
+    props.set("fov", fov);
+    props.set("width", w);
+    props.set("height", h);
+    PinholeCamera cam(props);
+    in.insert(in.end(), {float(w), float(h), fov, nearClip, pos.x, pos.y, pos.z, U.x, U.y, U.z, V.x, V.y, V.z, N.x, N.y, N.z});
+    for (const Mat4f* m : {&cam.worldToViewD3D, &cam.viewToClipD3D, &cam.worldToRaster})
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c)
+        {
+          const Vec4f& col = (*m)[c];
+          out.push_back(r == 0 ? col.x : (r == 1 ? col.y : (r == 2 ? col.z : col.w)));
+        }
+  }
+  dump("cam_mat_in", "f4", in, {nCam, 16});
+  dump("cam_mat_out", "f4", out, {nCam, 3, 16});
+}
+
 // The remaining lobes, composed as StandardMaterial composes them (material/standard.cpp:
 // 409-627): the lobe in a MultiBsdfSimd of weight 1; with opacity < 1 - 1e-5 that in turn in a
 // MultiBsdfSimd with a TransparentBtdfSimd (scales opacity, 1 - opacity); opacity <= 1e-5 leaves
@@ -813,6 +852,7 @@ int main(int argc, char** argv)
   goldenPostIntersect();
   goldenLobes();
   goldenSheen();
+  goldenCameraMatrices();
   fprintf(gManifest, "\n}\n");
   fclose(gManifest);
   return 0;

@@ -30,3 +30,24 @@ def test_last_error_is_callable_without_a_device():
     # a null context is rejected with PRT_ERR_INVALID, no GPU needed
     assert lib.prt_frame_clear(None) == P.PRT_ERR_INVALID
     assert b"null context" in lib.prt_last_error()
+
+
+def test_camera_matrices_match_reference(golden):
+    # camera/pinhole_camera.h:41-68 through the C ABI (host only): worldToViewD3D, viewToClipD3D,
+    # worldToRaster against the reference's PinholeCamera for axis-aligned and rotated bases
+    import numpy as np
+    import protoray_hip as P
+    cin, cout = golden["cam_mat_in"], golden["cam_mat_out"]
+    for row, ref in zip(cin, cout):
+        cam = np.zeros((), P.CAMERA_DTYPE)
+        cam["fov"], cam["nearClip"] = row[2], row[3]
+        cam["pos"], cam["U"], cam["V"], cam["N"] = row[4:7], row[7:10], row[10:13], row[13:16]
+        mats = P.camera_matrices(cam, int(row[0]), int(row[1]))
+        for m, r in zip(mats, ref):
+            np.testing.assert_allclose(m.ravel(), r, rtol=1e-6, atol=1e-5 * np.abs(r).max())
+        # a world point projects to the raster position of the ray through it
+        W, H = int(row[0]), int(row[1])
+        p = row[4:7] - 5.0 * row[13:16] + 0.3 * row[7:10] - 0.2 * row[10:13]
+        q = mats[2] @ np.append(p, 1.0)
+        px, py = q[0] / q[3], q[1] / q[3]
+        assert 0 < px < W and 0 < py < H
