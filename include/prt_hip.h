@@ -346,9 +346,9 @@ int prt_debug_get(prt_context* ctx, float* sample_l, uint32_t* path_hash, int32_
  * n x ndim x 3 (sampling/sobol_burley_sampler_simd.h:36-85). */
 int prt_debug_sampler(prt_context* ctx, int32_t n, const uint32_t* seed_pid_idx, int32_t ndim, float* out1,
                       float* out2, float* out3);
-/* The energy-compensation LUTs computed at context creation (14112 floats: GGX dir 32x32,
+/* The energy-compensation LUTs computed at context creation (14368 floats: GGX dir 32x32,
  * GGX avg 32, dielectric eta dir 16^3, eta avg 16^2, rcp-eta dir 16^3, rcp-eta avg 16^2,
- * dielectric reflection dir 16^3, avg 16^2; bsdf/albedo_tables.h:14-205).  *count receives the
+ * dielectric reflection dir 16^3, avg 16^2, sheen dir 16^2; bsdf/albedo_tables.h:14-300).  *count receives the
  * number of floats; out (capacity floats) may be NULL to query it.  PRT_ERR_INVALID if
  * capacity is too small. */
 int prt_debug_luts(prt_context* ctx, float* out, int32_t capacity, int32_t* count);

@@ -33,8 +33,9 @@
 #define PRT_LUT_FLOATS 14368
 
 // Per-material constants derived on the host at upload (the per-hit constructor work of
-// RoughConductorBrdfSimd, rough_conductor.h:121-130, and RoughDielectricBsdfSimd,
-// rough_dielectric.h:279-291, depends on the material only).  128 bytes.
+// RoughConductorBrdfSimd, rough_conductor.h:121-130, RoughDielectricBsdfSimd,
+// rough_dielectric.h:279-291, and ThinRoughDielectricBsdfSimd, thin_rough_dielectric.h:180-192,
+// depends on the material only).  208 bytes.
 struct prt_material_constants
 {
   int32_t kind;

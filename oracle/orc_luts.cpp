@@ -1,8 +1,9 @@
 // Oracle: energy-compensation LUT precompute.  TEST INFRASTRUCTURE ONLY.
 // Restates bsdf/albedo.h:16-120 (stratified 32x32 midpoint quadrature of the GGX and
 // dielectric single-scattering albedo) and bsdf/albedo_tables.cpp:23-107 (table layout and
-// trapezoid average).  Only the tables the stream path's BSDFs read are built: GGX, dielectric
-// (both eta ranges) and dielectric reflection (the coating's).
+// trapezoid average), and albedo.h:287-330 / albedo_tables.cpp:155-170 for the sheen table.
+// Only the tables the stream path's BSDFs read are built: GGX, dielectric (both eta ranges),
+// dielectric reflection (the coating's and the thin dielectric's) and sheen.
 #include "orc_core.h"
 
 namespace orc {
