@@ -98,3 +98,44 @@ def test_zero_samples_and_empty_queries(dev):
     prim, t, u, v = dev.queryRays(np.zeros((0, 8), np.float32))
     assert prim.shape == (0,)
     assert dev.queryOcclusion(np.zeros((0, 8), np.float32)).shape == (0,)
+
+
+def _shadow_arrays(transparency):
+    # floor at y = -1 (diffuse 0.5) under a distant light from straight above, a black quad of
+    # the given transparency at y = 1 between the floor and the light (above the camera)
+    v = np.array([[-100, -1, -10], [100, -1, -10], [100, -1, 100], [-100, -1, 100],
+                  [-100, 1, -10], [100, 1, -10], [100, 1, 100], [-100, 1, 100]], np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7]], np.int32)
+    mats = np.zeros(3, G.MATERIAL_DTYPE)
+    mats[1] = G._material(G.KIND_DIFFUSE, (0.5, 0.5, 0.5))
+    mats[2] = G._material(G.KIND_DIFFUSE, (0, 0, 0), transparency=transparency)
+    lights = np.zeros(1, G.LIGHT_DTYPE)
+    lights[0] = (G.LIGHT_DISTANT, 3.0, 3.0, 3.0, 0, 1, 0, np.radians(1.0))
+    return v, idx, np.array([1, 1, 2, 2], np.int32), mats, lights
+
+
+def test_transparent_shadow_scales_direct_light(dev):
+    # the transparent-shadow filter in kOccluded: a shadow ray through a surface of transparency t
+    # keeps t of the light, so every floor pixel is exactly t times its value under a fully
+    # transparent quad (bounce rays leave through the black quad into an empty sky either way)
+    W, H, spp = 64, 32, 4
+
+    def render(t):
+        v, idx, tm, mats, lights = _shadow_arrays(t)
+        _load(dev, v, idx, tm, mats, lights, max_depth=4)
+        dev.enableDebug(True)
+        dev.initFrame(W, H)
+        dev.clearFrame()
+        dev.render(spp)
+        _, _, first = dev.debugOutputs()
+        dev.enableDebug(False)
+        return dev.blitFrame().reshape(-1, 3), first
+
+    base, first = render(1.0)
+    floor = (first >= 0) & (first < 2)
+    assert floor.sum() > 300 and base[floor].min() > 0
+    for t in (0.37, 0.8):
+        img, _ = render(t)
+        np.testing.assert_allclose(img[floor], t * base[floor], rtol=2e-6)
+    img, _ = render(0.0)
+    assert np.all(img[floor] == 0)

@@ -252,3 +252,77 @@ def test_light_pick_proportional_to_power():
     area_power = np.pi * 2 * (20.0 * 20.0) * Le
     amb_power = 4 * np.pi ** 2 * s.radius ** 2 * 0.7
     np.testing.assert_allclose(cdf[1], area_power / (area_power + amb_power), rtol=1e-5)
+
+
+def _shadow_scene(transparency, thin=False):
+    """Floor at y = -1 (diffuse 0.5) under a distant light from straight above, with a black
+    quad at y = 1 between the floor and the light (the camera at the origin looks along +z, so
+    its rays to the floor do not cross the quad)."""
+    v = np.array([[-100, -1, -10], [100, -1, -10], [100, -1, 100], [-100, -1, 100],
+                  [-100, 1, -10], [100, 1, -10], [100, 1, 100], [-100, 1, 100]], np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7]], np.int32)
+    mats = np.zeros(3, O.MATERIAL_DTYPE)
+    mats[1]["kind"] = G.KIND_DIFFUSE
+    mats[1]["r"] = mats[1]["g"] = mats[1]["b"] = 0.5
+    if thin:
+        mats[2]["kind"] = G.KIND_THIN_DIELECTRIC
+        mats[2]["r"], mats[2]["g"], mats[2]["b"] = 0.9, 0.6, 0.3
+        mats[2]["roughness"], mats[2]["ior"], mats[2]["thickness"] = 0.3, 1.5, 0.5
+    else:
+        mats[2]["kind"] = G.KIND_DIFFUSE  # black: its lobe has no importance, only the transparent BTDF is sampled
+        mats[2]["transparency"] = transparency
+    s = O.Scene(v, idx, np.array([1, 1, 2, 2], np.int32), mats)
+    lights = np.zeros(1, O.LIGHT_DTYPE)
+    lights[0] = (G.LIGHT_DISTANT, 3.0, 3.0, 3.0, 0, 1, 0, np.radians(1.0))
+    s.set_lights(lights)
+    return s
+
+
+def test_transparent_shadow_scales_direct_light():
+    # render/transparent_filter.h:42-93: a shadow ray crossing a transparent (non-thin) surface
+    # keeps 1 - opacity of the light.  The floor's bounce rays leave through the black quad into
+    # an empty sky either way, so every floor pixel is exactly t times its unshadowed value.
+    W, H, spp = 64, 32, 4
+    cam = G._camera()
+    open_sky = _shadow_scene(0.0)
+    ref, _ = _shadow_scene(1.0).render(cam, W, H, 0, spp, max_depth=4, record=True)  # quad fully transparent
+    for t in (0.37, 0.8):
+        acc, _ = _shadow_scene(t).render(cam, W, H, 0, spp, max_depth=4, record=True)
+        floor = (acc["_first"] >= 0) & (acc["_first"] < 2)
+        assert floor.sum() > 300
+        img, base = acc["color"][:, floor], ref["color"][:, floor]
+        assert base.min() > 0
+        np.testing.assert_allclose(img, t * base, rtol=2e-6)
+    # an opaque quad (transparency 0) blocks the direct light completely
+    acc, _ = open_sky.render(cam, W, H, 0, spp, max_depth=4, record=True)
+    floor = (acc["_first"] >= 0) & (acc["_first"] < 2)
+    assert np.all(acc["color"][:, floor] == 0)
+
+
+def test_thin_dielectric_shadow_transmittance():
+    # a thin rough dielectric between the floor and a light from straight above: the shadow ray
+    # keeps (1 - R(cos 1)) pow(color, thickness / cos_t) (thin_rough_dielectric.h:307-321), the
+    # same value the golden-pinned transparency gives at normal incidence
+    W, H, spp = 64, 32, 4
+    cam = G._camera()
+    ref, _ = _shadow_scene(1.0).render(cam, W, H, 0, spp, max_depth=1, record=True)
+    acc, _ = _shadow_scene(0.0, thin=True).render(cam, W, H, 0, spp, max_depth=1, record=True)
+    floor = (acc["_first"] >= 0) & (acc["_first"] < 2)
+    ratio = acc["color"][:, floor] / ref["color"][:, floor]
+    # the light's cone is 1 degree: the ratio varies by well under 1e-3 around its mean
+    expect = ratio.mean(axis=1, keepdims=True)
+    np.testing.assert_allclose(ratio, np.broadcast_to(expect, ratio.shape), rtol=2e-3)
+    flat, _ = O.compute_luts()
+    m = np.zeros(1, O.MATERIAL_DTYPE)
+    m[0] = _thin_record()
+    geo = np.array([[1, 0, 0, 0, 0, 1, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, -1, 0]], np.float32)  # N = Ng = wo = +y
+    _, _, _, aux = O.bsdf_mat(flat, m, geo, np.zeros((1, 3), np.float32))
+    np.testing.assert_allclose(expect[:, 0], aux[0, 6:9], rtol=2e-3)
+
+
+def _thin_record():
+    r = np.zeros((), O.MATERIAL_DTYPE)
+    r["kind"] = G.KIND_THIN_DIELECTRIC
+    r["r"], r["g"], r["b"] = 0.9, 0.6, 0.3
+    r["roughness"], r["ior"], r["thickness"] = 0.3, 1.5, 0.5
+    return r
