@@ -21,6 +21,7 @@ import pytest
 
 import protoray_hip as P
 import scene_files as S
+import scenegen as G
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_scene")
@@ -312,3 +313,35 @@ def test_errors(tmp_path):
     w.save(str(tmp_path / "oob.rkscene"))
     with pytest.raises(ValueError, match="out of range"):
         P.load_scene_file(tmp_path / "oob.rkscene")
+
+
+def test_remaining_lobe_parameters_map_to_records(tmp_path):
+    # thin-walled transmission -> kind 4 (standard.cpp:534-543: transmissionColor, thickness /
+    # transmissionDepth), opacity -> transparency, diffuseRoughness -> the Oren-Nayar base, the
+    # sheen layer (standard.cpp:599-611) on any kind; none of these counts as approximated
+    w = S.RkWriter()
+    w.doc["materials"] = [
+        {"name": "thin", "transmissionWeight": 1.0, "thinWalled": 1, "specularIOR": 1.4, "specularRoughness": 0.2,
+         "transmissionColor": [0.9, 0.5, 0.2], "transmissionDepth": 2.0},
+        {"name": "cutout", "baseColor": [0.2, 0.7, 0.3], "specularWeight": 0.0, "opacity": 0.35},
+        {"name": "clay", "baseColor": [0.6, 0.5, 0.4], "specularWeight": 0.0, "diffuseRoughness": 0.7},
+        {"name": "velvet", "baseColor": [0.4, 0.1, 0.1], "sheenWeight": 0.8, "sheenColor": [0.9, 0.8, 0.7],
+         "sheenRoughness": 0.45},
+    ]
+    sp, si = S.icosphere(1)
+    geoms = [w.geometry(sp * (0.5 + 0.1 * k), si, material=k) for k in range(4)]
+    w.object(geoms)
+    path = tmp_path / "lobes.rkscene"
+    w.save(str(path))
+    sc = P.load_scene_file(path)
+    assert sc["info"]["approximated_materials"] == 0
+    m = {n: r for n, r in zip(sc["material_names"], sc["materials"])}
+    thin, cutout, clay, velvet = m["thin"], m["cutout"], m["clay"], m["velvet"]
+    assert thin["kind"] == G.KIND_THIN_DIELECTRIC and np.isclose(thin["ior"], 1.4) and np.isclose(thin["roughness"], 0.2)
+    assert np.allclose([thin["r"], thin["g"], thin["b"]], [0.9, 0.5, 0.2]) and np.isclose(thin["thickness"], 0.5)
+    assert cutout["kind"] == G.KIND_DIFFUSE and np.isclose(cutout["transparency"], 0.65)
+    assert clay["kind"] == G.KIND_DIFFUSE and np.isclose(clay["diffuse_roughness"], 0.7)
+    assert velvet["kind"] == G.KIND_COATED and np.isclose(velvet["sheen_weight"], 0.8)
+    assert np.allclose(velvet["sheen_color"], [0.9, 0.8, 0.7]) and np.isclose(velvet["sheen_roughness"], 0.45)
+    for r in (thin, cutout, clay):
+        assert r["sheen_weight"] == 0
