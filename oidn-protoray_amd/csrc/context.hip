@@ -314,6 +314,7 @@ struct prt_context
   float areaPower = 0.f;
   DevBuf<float4> envTex;
   DevBuf<float> envX, envY;
+  DevBuf<int> envXG, envYG;  // guide tables of the sampling CDFs
   int envW = 0, envH = 0, envLW = 0, envLH = 0;
   float sceneRadius = 0.f;
   bool lightsReady = false;
@@ -774,6 +775,29 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
     for (int y = 0; y < env_h; ++y)
       for (int x = 0; x < env_w; ++x) s += lum(x, y);
     envMean = s / float(env_w * env_h);
+    // guide tables: the index the device's full search (distribution1d.cpp:61-80) returns at
+    // each bin start g / PRT_ENV_GUIDE, and at 1 for the end
+    auto lookup = [](const float* cdf, int size, float v) {
+      int first = 0, len = size;
+      while (len > 0)
+      {
+        int half = len >> 1, mid = first + half;
+        if (v < cdf[mid]) len = half;
+        else { first = mid + 1; len = len - half - 1; }
+      }
+      return first - 1 > 0 ? first - 1 : 0;
+    };
+    std::vector<int> xg(size_t(lh) * (PRT_ENV_GUIDE + 1)), yg(PRT_ENV_GUIDE + 1);
+    for (int g = 0; g <= PRT_ENV_GUIDE; ++g)
+    {
+      const float v = float(g) / float(PRT_ENV_GUIDE);
+      yg[g] = lookup(yd.data(), lh, v);
+      for (int y = 0; y < lh; ++y) xg[size_t(y) * (PRT_ENV_GUIDE + 1) + g] = lookup(xd.data() + size_t(y) * (lw + 1), lw, v);
+    }
+    HIP_TRY(c->envXG.alloc(xg.size()));
+    HIP_TRY(c->envYG.alloc(yg.size()));
+    HIP_TRY(hipMemcpy(c->envXG.p, xg.data(), sizeof(int) * xg.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->envYG.p, yg.data(), sizeof(int) * yg.size(), hipMemcpyHostToDevice));
     HIP_TRY(c->envTex.alloc(tex.size()));
     HIP_TRY(c->envX.alloc(xd.size()));
     HIP_TRY(c->envY.alloc(yd.size()));
@@ -1022,6 +1046,7 @@ static RenderArgs makeArgs(prt_context* c)
   for (int i = 0; i <= a.nLights + a.lightOffset; ++i) a.lightCdf[i] = c->lightCdf[i];
   a.areaTris = c->areaTris.p; a.areaCdf = c->areaCdf.p; a.areaMap = c->areaMap.p; a.areaCount = c->areaCount;
   a.env.texels = c->envTex.p; a.env.xDists = c->envX.p; a.env.yDist = c->envY.p;
+  a.env.xGuide = c->envXG.p; a.env.yGuide = c->envYG.p;
   a.env.W = c->envW; a.env.H = c->envH; a.env.LW = c->envLW; a.env.LH = c->envLH;
   setCameraArgs(a, c->cam, c->W, c->H);
   prt_frame_dev& f = a.fb;

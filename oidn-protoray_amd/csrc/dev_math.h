@@ -211,6 +211,31 @@ PRT_DEV int dist1dSampleReuse(const float* cdf, int size, float& pdf, float& s)
   return i;
 }
 PRT_DEV float dist1dPdf(const float* cdf, int size, int i) { return (i < 0 || i >= size) ? 0.f : cdf[i + 1] - cdf[i]; }
+// dist1dLookup with a guide table (the cutpoint method): guide[g] is the index the full search
+// returns for s = g / PRT_ENV_GUIDE.  The full search's result does not decrease with s, so for
+// s in bin g it lies in [guide[g], guide[g + 1]]: the last i there with cdf[i] <= s, found by
+// searching cdf[guide[g] + 1 .. guide[g + 1]] only -- the same index in a few steps.
+PRT_DEV int dist1dLookupGuided(const float* cdf, const int* guide, float s)
+{
+  const int g = min(int(s * float(PRT_ENV_GUIDE)), PRT_ENV_GUIDE - 1);
+  const int lo = guide[g];
+  int first = lo + 1, len = guide[g + 1] - lo;
+  while (len > 0)
+  {
+    int half = len >> 1, mid = first + half;
+    if (s < cdf[mid]) len = half;
+    else { first = mid + 1; len = len - half - 1; }
+  }
+  return first - 1;
+}
+PRT_DEV int dist1dSampleReuseGuided(const float* cdf, const int* guide, float& pdf, float& s)
+{
+  int i = dist1dLookupGuided(cdf, guide, s);
+  float c0 = cdf[i], c1 = cdf[i + 1];
+  pdf = c1 - c0;
+  s = (s - c0) * (1.0f / pdf);
+  return i;
+}
 
 // ---------------------------------------------------------------------------------------
 // GGX (bsdf/ggx_distribution.h) -- isotropic alpha = max(roughness^2, 0.001)
@@ -1046,6 +1071,8 @@ struct EnvView
   const float4* texels;  // [H][W] rgb + pad, texture row order
   const float* xDists;   // [LH][LW+1]
   const float* yDist;    // [LH+1]
+  const int* xGuide;     // [LH][PRT_ENV_GUIDE+1] guide tables of the row CDFs
+  const int* yGuide;     // [PRT_ENV_GUIDE+1] guide table of the row-selection CDF
   int W, H, LW, LH;
 };
 
@@ -1110,8 +1137,8 @@ PRT_DEV f3 envSampleOne(const prt_light_dev& l, const EnvView& e, f3& wi, float&
   }
   // image_light.h:112-134 (Distribution2D::sampleReuse, distribution2d.h:53-64)
   float yPdf, xPdf;
-  int iy = dist1dSampleReuse(e.yDist, e.LH, yPdf, sy);
-  int ix = dist1dSampleReuse(e.xDists + iy * (e.LW + 1), e.LW, xPdf, sx);
+  int iy = dist1dSampleReuseGuided(e.yDist, e.yGuide, yPdf, sy);
+  int ix = dist1dSampleReuseGuided(e.xDists + iy * (e.LW + 1), e.xGuide + iy * (PRT_ENV_GUIDE + 1), xPdf, sx);
   pdf = xPdf * yPdf;
   float u = (float(ix) + sx) * (1.0f / float(e.LW));
   float v = (float(iy) + sy) * (1.0f / float(e.LH));

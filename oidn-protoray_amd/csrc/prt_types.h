@@ -9,6 +9,10 @@
 #define PRT_KIND_COATED 3
 #define PRT_KIND_THIN_DIELECTRIC 4
 
+// Bins of the guide tables that narrow the searches of the environment's sampling CDFs (a power
+// of two, so s * PRT_ENV_GUIDE is exact in float)
+#define PRT_ENV_GUIDE 32
+
 #define PRT_LIGHT_AMBIENT 0
 #define PRT_LIGHT_DISTANT 1
 #define PRT_LIGHT_IMAGE 2
