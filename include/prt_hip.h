@@ -162,6 +162,9 @@ int prt_scene_set_lights(prt_context* ctx, int32_t n, const prt_light* lights, i
 /* BVH build on the device (replaces the Embree build, accel/embree_intersector.cpp:11-90).
  * build_ms receives the build time (Embree's buildMs stat). */
 int prt_bvh_build(prt_context* ctx, double* build_ms);
+/* Device::getSceneBounds (render/device.h:65): the box of the vertices the triangles reference
+ * (empty: lo > hi). */
+int prt_scene_bounds(prt_context* ctx, float lo[3], float hi[3]);
 
 /* --- renderer (Device::initRenderer -> RendererFactoryStream::make "pathStream",
  *     render/renderer_factory_stream.cpp:17-76) --------------------------------------- */

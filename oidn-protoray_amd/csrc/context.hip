@@ -291,6 +291,7 @@ bool isPowerOfTwo(int x) { return (x & (x - 1)) == 0; }
 
 struct prt_context
 {
+  float boundsLo[3] = {1.f, 1.f, 1.f}, boundsHi[3] = {-1.f, -1.f, -1.f};  // scene box (empty: lo > hi)
   int anyTransparent = 0;  // some material lets shadow rays through
   int extMaterials = 0;    // some material is a thin dielectric, has opacity or an Oren-Nayar base
   int device = 0;
@@ -496,6 +497,7 @@ int prt_scene_set_geometry(prt_context* c, int32_t nverts, const float* position
       lo[k] = std::min(lo[k], v);
       hi[k] = std::max(hi[k], v);
     }
+  for (int k = 0; k < 3; ++k) { c->boundsLo[k] = lo[k]; c->boundsHi[k] = hi[k]; }
   H3 ext = ntris ? hmk(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) : hmk(0, 0, 0);
   c->sceneRadius = std::sqrt(hdot(ext, ext)) * 0.5f;
   c->bvhReady = false;
@@ -1396,6 +1398,14 @@ int prt_query_occlusion(prt_context* c, int32_t n, const float* rays, int32_t* o
   int rc = queryHost(c, n, rays, occluded, nullptr, nullptr, nullptr, 1);
   if (rc != PRT_OK) return rc;
   for (int i = 0; i < n; ++i) occluded[i] = occluded[i] >= 0 ? 1 : 0;
+  return PRT_OK;
+}
+
+int prt_scene_bounds(prt_context* c, float lo[3], float hi[3])
+{
+  CTX_CHECK(c);
+  if (!lo || !hi) return fail(PRT_ERR_INVALID, "null bounds arrays");
+  for (int k = 0; k < 3; ++k) { lo[k] = c->boundsLo[k]; hi[k] = c->boundsHi[k]; }
   return PRT_OK;
 }
 

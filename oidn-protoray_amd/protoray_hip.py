@@ -75,6 +75,7 @@ SIGNATURES = {
     "prt_query_occlusion": [_vp, _i, _vp, _vp],
     "prt_query_pixel": [_vp, ctypes.c_float, ctypes.c_float, _i, _vp, _vp],
     "prt_camera_matrices": [_vp, _i, _i, _vp, _vp, _vp],
+    "prt_scene_bounds": [_vp, _vp, _vp],
     "prt_query_rays_device": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_debug_enable": [_vp, _i],
     "prt_debug_get": [_vp, _vp, _vp, _vp],
@@ -253,6 +254,12 @@ class DeviceHip:
         occ = np.zeros(r.shape[0], np.int32)
         _check(_lib.prt_query_occlusion(self.h, r.shape[0], _ptr(r), _ptr(occ)))
         return occ
+
+    def sceneBounds(self):
+        """Device::getSceneBounds: (lo, hi) of the vertices the triangles reference."""
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        _check(_lib.prt_scene_bounds(self.h, _ptr(lo), _ptr(hi)))
+        return lo, hi
 
     def queryPixel(self, x, y, max_hits=32):
         """Renderer::queryPixel (render/renderer.h:82-121): the hits of the ray through the centre

@@ -87,6 +87,20 @@ def test_largest_frame(dev):
     dev.initFrame(64, 16)  # release the large frame
 
 
+def test_scene_bounds(dev):
+    # Device::getSceneBounds: the box of the referenced vertices (an unreferenced far vertex is
+    # left out), empty for a scene without triangles
+    pos = np.array([[-3, -2, 6], [3, -2, 6], [0, 3, 7], [100, 100, 100]], np.float32)
+    mats = np.zeros(2, G.MATERIAL_DTYPE)
+    mats[1] = G._material(G.KIND_DIFFUSE, (0.5, 0.5, 0.5))
+    _load(dev, pos, np.array([[0, 1, 2]], np.int32), np.array([1], np.int32), mats, _ambient(1.0))
+    lo, hi = dev.sceneBounds()
+    assert np.array_equal(lo, [-3, -2, 6]) and np.array_equal(hi, [3, 3, 7])
+    _load(dev, np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32), np.zeros(0, np.int32), mats[:1], _ambient(1.0))
+    lo, hi = dev.sceneBounds()
+    assert np.all(lo > hi)
+
+
 def test_zero_samples_and_empty_queries(dev):
     scene = G.spheres_ambient(n_spheres=8, subdiv=1)
     P.load_scene(dev, scene)
