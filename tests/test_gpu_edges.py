@@ -153,3 +153,21 @@ def test_transparent_shadow_scales_direct_light(dev):
         np.testing.assert_allclose(img[floor], t * base[floor], rtol=2e-6)
     img, _ = render(0.0)
     assert np.all(img[floor] == 0)
+
+
+@pytest.mark.parametrize("field,value", [("kind", 5), ("transparency", 1.5), ("transparency", -0.1)])
+def test_invalid_material_records(dev, field, value):
+    # prt_scene_set_materials rejects an unknown kind and a transparency outside [0, 1]
+    mats = np.zeros(2, G.MATERIAL_DTYPE)
+    mats[1] = G._material(G.KIND_DIFFUSE, (0.5, 0.5, 0.5))
+    mats[1][field] = value
+    with pytest.raises(ValueError):
+        dev.initScene(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.int32), np.array([1], np.int32), mats)
+
+
+def test_thin_dielectric_needs_an_interface(dev):
+    # an IOR equal to the ambient 1 would make it a transparent BTDF (standard.cpp:547-550): rejected
+    mats = np.zeros(2, G.MATERIAL_DTYPE)
+    mats[1] = G._material(G.KIND_THIN_DIELECTRIC, (1, 1, 1), roughness=0.3, ior=1.0)
+    with pytest.raises(ValueError):
+        dev.initScene(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.int32), np.array([1], np.int32), mats)
