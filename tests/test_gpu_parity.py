@@ -169,6 +169,17 @@ def test_render_parity_sunsky_coated(dev):
     _parity(dev, scene, 96, 64, 3, 0.97)
 
 
+def test_render_parity_default_depth(dev):
+    # the reference's default maxDepth of 100 (render/path_renderer_stream.h:114): Russian roulette from
+    # depth 8 on, depths beyond 9 enqueued only while paths survive, and sampler dimensions up to
+    # 4 + 7 * 100
+    scene = G.spheres_sunsky(n_spheres=48, subdiv=2, palette=24)
+    scene["max_depth"] = 100
+    _parity(dev, scene, 96, 64, 3, 0.97)
+    st = dev.render(1)
+    assert st["spp"] >= 1
+
+
 def test_render_parity_remaining_lobes(dev):
     # Oren-Nayar bases (alone and under a coat), thin rough dielectrics, opacity mixes with the
     # transparent BTDF (one material fully transparent), sheen layers over every kind, and shadow
