@@ -385,7 +385,8 @@ int prt_context_create(int device_ordinal, prt_context** out)
   c->cus = c->props.multiProcessorCount;
   c->launch.gridTrace = c->cus * PRT_TRACE_BLOCKS_PER_CU;
   c->launch.gridShade = c->cus * 8;
-  c->spillStride = c->launch.gridTrace * kTraceBlock;
+  c->launch.gridOccluded = c->cus * PRT_OCCLUDED_BLOCKS_PER_CU;
+  c->spillStride = std::max(c->launch.gridTrace, c->launch.gridOccluded) * kTraceBlock;
   for (auto& ev : c->ev) hipEventCreate(&ev);
   // albedo LUT precompute, as DeviceCpu's constructor does (render/device_cpu.cpp:20-23)
   computeLuts(c->luts);

@@ -107,6 +107,7 @@ struct LaunchCfg
 {
   int gridTrace;  // blocks of the persistent traversal kernels
   int gridShade;  // blocks of the persistent shading kernel
+  int gridOccluded;  // blocks of the opaque-scene shadow-ray kernel
 };
 // The kernels read RenderArgs from device memory (one copy per render call); passing the
 // ~1 KB record by value would make every lane copy it to scratch.

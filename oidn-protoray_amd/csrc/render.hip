@@ -526,7 +526,7 @@ void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const Lau
 void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, bool filter, hipStream_t s)
 {
   if (filter) kOccluded<true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
-  else kOccluded<false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  else kOccluded<false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth);
 }
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit, int* spill,
                  int spillStride, int grid, hipStream_t s)
