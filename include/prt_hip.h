@@ -123,6 +123,9 @@ typedef struct prt_stats {
 #define PRT_AOV_NORMAL1  (1u << 3)  /* aovNormal1 */
 #define PRT_AOV_DEPTH    (1u << 4)  /* aovDepth   : linear depth of the first hit */
 #define PRT_AOV_VAR      (1u << 5)  /* aovHdrVar  : color sample variance (sum of squares) */
+#define PRT_AOV_HALF     (1u << 6)  /* AccumFlag::HalfBuffer of the color buffer: the class-B samples
+                                       (image/accum_buffer.h:36-47, 455-470), read by prt_blit_half */
+#define PRT_AOV_ALL      0x7fu
 
 /* Buffer selectors for prt_blit. */
 #define PRT_BUF_COLOR   0
@@ -193,6 +196,28 @@ int prt_render(prt_context* ctx, int32_t n_spp, prt_stats* stats);
 /* Device::blitFrame (render/device_cpu.cpp:316-376, AccumBuffer3f::blit accum_buffer.h:607-649):
  * mean = sum * rcpSafe(weight), interleaved RGB (depth: one float).  var may be NULL. */
 int prt_blit(prt_context* ctx, int32_t buffer, float* dst, float* var);
+/* The half buffer of AccumBuffer3f::blit (destHalf, accum_buffer.h:633-643): the mean of the
+ * class-B samples (sample counts with an odd number of bits set in count & 0xaaaaaaaa),
+ * interleaved RGB.  Needs PRT_AOV_HALF in prt_frame_init. */
+int prt_blit_half(prt_context* ctx, float* dst);
+/* Device::setJitter / disableJitter / hasJitter / getJitter (render/device.h:90-94,
+ * image/frame_buffer.h:75-78): with enable != 0 every pixel's sample position is (jx, jy) in
+ * [0, 1]^2 instead of the sampler's dimension-set 0 value (path_renderer_stream.h:228-231);
+ * enable = 0 disables it.  The jitter belongs to the frame: prt_frame_init resets it. */
+int prt_set_jitter(prt_context* ctx, int32_t enable, float jx, float jy);
+int prt_get_jitter(prt_context* ctx, int32_t* has, float* jxy);
+/* Device::initFilter (render/device_cpu.cpp:235-282) in PixelFilterMode::Sample: the pixel
+ * sample is warped through the filter's tabulated distribution (image/pixel_filter.h:30-66).
+ * width <= 0 takes the reference default (triangle 2, Gaussian and Blackman-Harris 3); sigma <= 0
+ * the Gaussian default width / 6; bias: subtract the Gaussian's value at the radius.  Without
+ * a frame the call does nothing (as DeviceCpu::initFilter); prt_frame_init resets the filter to
+ * the box.  prt_get_filter_width: Device::getFilterWidth (0 for the box). */
+#define PRT_FILTER_BOX 0
+#define PRT_FILTER_TRIANGLE 1
+#define PRT_FILTER_GAUSSIAN 2
+#define PRT_FILTER_BLACKMAN_HARRIS 3
+int prt_set_pixel_filter(prt_context* ctx, int32_t type, float width, float sigma, int32_t bias);
+int prt_get_filter_width(prt_context* ctx, float* width);
 /* Raw FP32 accumulator sums (planar, 3 x W*H) and weights (W*H) of the color buffer. */
 int prt_get_accum(prt_context* ctx, float* sum, float* weight);
 /* Device::getFrameError (AccumBuffer3f::getError, accum_buffer.h:752-805) against the
@@ -355,6 +380,10 @@ int prt_debug_sampler(prt_context* ctx, int32_t n, const uint32_t* seed_pid_idx,
  * number of floats; out (capacity floats) may be NULL to query it.  PRT_ERR_INVALID if
  * capacity is too small. */
 int prt_debug_luts(prt_context* ctx, float* out, int32_t capacity, int32_t* count);
+/* The light-power distribution the renderer picks lights from (Scene::lightDistribution,
+ * render/scene.cpp:440-445): *count = lights + 1 CDF entries (the area light first, if any, then
+ * the environment lights in order); out (capacity floats) may be NULL to query the count. */
+int prt_debug_light_cdf(prt_context* ctx, float* out, int32_t capacity, int32_t* count);
 /* BVH statistics: inner node count, triangle count, max depth, scene triangle count. */
 int prt_debug_bvh_stats(prt_context* ctx, int64_t* stats4);
 /* Device-side kernel timing of the last prt_render call, per kernel class
@@ -363,6 +392,16 @@ int prt_debug_bvh_stats(prt_context* ctx, int64_t* stats4);
  * counters6 = closest-hit rays, inner-node visits, triangle tests, then the same three for the
  * shadow (any-hit) rays; launches2 = number of intersect and occluded launches. */
 int prt_debug_kernel_ms(prt_context* ctx, double* ms4, double* counters6, double* launches2);
+/* Renders with enable != 0 use builds of the traversal kernels that also count their loop
+ * iterations (prt_debug_trace_steps); the default builds keep their registers and occupancy. */
+int prt_debug_count_steps(prt_context* ctx, int32_t enable);
+/* SIMD utilisation of the traversal kernels in the last prt_render call (counted when enabled by
+ * prt_debug_count_steps, zero otherwise): steps4 = traversal loop
+ * iterations (inner-node and leaf visits) summed over lanes, then summed over waves (per ray
+ * batch, the longest lane's count), for closest-hit then shadow rays.  Active-lane fraction =
+ * lane steps / (64 x wave steps) (the analogue of RayStats' active-lane counts,
+ * core/ray_stats.h:10-45). */
+int prt_debug_trace_steps(prt_context* ctx, double* steps4);
 
 #ifdef __cplusplus
 }

@@ -331,12 +331,20 @@ struct prt_context
   int W = 0, H = 0, npix = 0;
   uint32_t aovMask = 0;
   int count = 0;
-  DevBuf<float> color, weight, color2, colorC, weightC;
+  DevBuf<float> color, weight, color2, colorC, weightC, colorB, weightB;
+  // pixel sample: jitter (FrameBuffer::setJitter) and pixel filter (DeviceCpu::initFilter); both
+  // belong to the frame and are reset by prt_frame_init, as a new FrameBuffer resets them
+  bool jitter = false;
+  float jitterXY[2] = {0.f, 0.f};
+  DevBuf<float> filterCdf;
+  int filterSize = 0;
+  float filterWidth = 0.f;
   DevBuf<float> albedo, albedoW, albedo1, albedo1W, normal, normalW, normal1, normal1W, depth, depthW;
   DevBuf<float> dbgL;
   DevBuf<uint32_t> dbgHash;
   DevBuf<int> dbgFirst;
   bool debug = false;
+  bool countSteps = false;  // traversal kernels built with the SIMD-utilisation counters
   // path state
   DevBuf<float4> rayO, rayD, hit, TP, Lr, shO, shD, dirL;
   DevBuf<int> flags, q0, q1, shadowQ, counters, spill;
@@ -350,7 +358,7 @@ struct prt_context
   hipEvent_t ev[8] = {};
   std::vector<hipEvent_t> evPass[2];  // per-pass kernel timing events, double-buffered
   double kernelMs[4] = {0, 0, 0, 0};
-  double lastStats[6] = {0, 0, 0, 0, 0, 0};
+  double lastStats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   double launches[2] = {0, 0};
 };
 
@@ -393,7 +401,7 @@ int prt_context_create(int device_ordinal, prt_context** out)
   if (c->lut.alloc(PRT_LUT_FLOATS) != hipSuccess ||
       hipMemcpy(c->lut.p, c->luts.data(), sizeof(float) * PRT_LUT_FLOATS, hipMemcpyHostToDevice) != hipSuccess ||
       c->spill.alloc(2 * size_t(PRT_MAX_STACK - PRT_LDS_STACK) * c->spillStride) != hipSuccess ||
-      c->stats.alloc(8) != hipSuccess || c->argsDev.alloc(1) != hipSuccess ||
+      c->stats.alloc(kStatCount) != hipSuccess || c->argsDev.alloc(1) != hipSuccess ||
       hipHostMalloc((void**)&c->argsHost, sizeof(RenderArgs), hipHostMallocDefault) != hipSuccess)
   {
     prt_context_destroy(c);
@@ -736,7 +744,6 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
   if (needImage && (!env_rgb || env_w <= 0 || env_h <= 0)) return fail(PRT_ERR_INVALID, "image light without an image");
   const float pi = 3.14159265358979323846f;
   std::vector<float> sumL;
-  double envMean = 0.0;
   if (env_rgb && env_w > 0 && env_h > 0)
   {
     // light/image_light.h:20-54: importance LUT with 2x bucketing down to lut_max
@@ -774,10 +781,6 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
       init1d(xd.data() + size_t(y) * (lw + 1), w.data() + size_t(y) * lw, lw);
     }
     init1d(yd.data(), fy.data(), lh);
-    float s = 0.f;  // light/image_light.h:137-148
-    for (int y = 0; y < env_h; ++y)
-      for (int x = 0; x < env_w; ++x) s += lum(x, y);
-    envMean = s / float(env_w * env_h);
     // guide tables: the index the device's full search (distribution1d.cpp:61-80) returns at
     // each bin start g / PRT_ENV_GUIDE, and at 1 for the end
     auto lookup = [](const float* cdf, int size, float v) {
@@ -835,7 +838,16 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
       power[i] = 4.f * hsqr(pi) * hsqr(std::sin(0.5f * l.angle)) * hsqr(R) * lumC;  // distant_light.h:73-77
     }
     else if (l.kind == PRT_LIGHT_IMAGE)
-      power[i] = 4.f * hsqr(pi) * hsqr(R) * float(envMean) * lumC;
+    {
+      // light/image_light.h:137-148 over the image this light emits (the texels times its scale)
+      float sumL = 0.f;
+      for (size_t t = 0; t < size_t(env_w) * env_h; ++t)
+      {
+        const float* p = env_rgb + 3 * t;
+        sumL += 0.212671f * (p[0] * l.r) + 0.715160f * (p[1] * l.g) + 0.072169f * (p[2] * l.b);
+      }
+      power[i] = 4.f * hsqr(pi) * hsqr(R) * (sumL / float(env_w * env_h));
+    }
     else
       return fail(PRT_ERR_INVALID, "unknown light kind");
   }
@@ -907,6 +919,7 @@ static hipError_t allocFrame(prt_context* c)
   };
   if ((e = A(c->color, 3 * n, true)) || (e = A(c->weight, n, true)) || (e = A(c->colorC, 3 * n, true)) ||
       (e = A(c->weightC, n, true)) || (e = A(c->color2, 3 * n, c->aovMask & PRT_AOV_VAR)) ||
+      (e = A(c->colorB, 3 * n, c->aovMask & PRT_AOV_HALF)) || (e = A(c->weightB, n, c->aovMask & PRT_AOV_HALF)) ||
       (e = A(c->albedo, 3 * n, c->aovMask & PRT_AOV_ALBEDO)) || (e = A(c->albedoW, n, c->aovMask & PRT_AOV_ALBEDO)) ||
       (e = A(c->albedo1, 3 * n, c->aovMask & PRT_AOV_ALBEDO1)) || (e = A(c->albedo1W, n, c->aovMask & PRT_AOV_ALBEDO1)) ||
       (e = A(c->normal, 3 * n, c->aovMask & PRT_AOV_NORMAL)) || (e = A(c->normalW, n, c->aovMask & PRT_AOV_NORMAL)) ||
@@ -925,7 +938,7 @@ int prt_frame_clear(prt_context* c)
   CTX_CHECK(c);
   if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
   const size_t n = size_t(c->npix);
-  DevBuf<float>* bufs[] = {&c->color, &c->weight, &c->color2, &c->colorC, &c->weightC, &c->albedo, &c->albedoW, &c->albedo1,
+  DevBuf<float>* bufs[] = {&c->color, &c->weight, &c->color2, &c->colorC, &c->weightC, &c->colorB, &c->weightB, &c->albedo, &c->albedoW, &c->albedo1,
                            &c->albedo1W, &c->normal, &c->normalW, &c->normal1, &c->normal1W, &c->depth, &c->depthW};
   for (auto* b : bufs)
     if (b->p) HIP_TRY(hipMemsetAsync(b->p, 0, sizeof(float) * b->n, c->stream));
@@ -942,10 +955,16 @@ int prt_frame_init(prt_context* c, int32_t width, int32_t height, uint32_t aov_m
   if (width <= 0 || height <= 0 || width % 32 != 0 || height % 8 != 0)
     return fail(PRT_ERR_INVALID, "image size is not divisible by the tile size");
   if (width > 4096 || height > 4096) return fail(PRT_ERR_INVALID, "image size is too large");
+  if (aov_mask & ~uint32_t(PRT_AOV_ALL)) return fail(PRT_ERR_INVALID, "unknown AOV bits");
   c->W = width;
   c->H = height;
   c->npix = width * height;
   c->aovMask = aov_mask;
+  // a new frame has no jitter and the box filter (FrameBuffer ctor, image/frame_buffer.cpp:9-14)
+  c->jitter = false;
+  c->filterCdf.release();
+  c->filterSize = 0;
+  c->filterWidth = 0.f;
   hipError_t e = allocFrame(c);
   if (e != hipSuccess) return fail(PRT_ERR_NOMEM, hipGetErrorString(e));
   if (c->debug)
@@ -1052,8 +1071,15 @@ static RenderArgs makeArgs(prt_context* c)
   a.env.xGuide = c->envXG.p; a.env.yGuide = c->envYG.p;
   a.env.W = c->envW; a.env.H = c->envH; a.env.LW = c->envLW; a.env.LH = c->envLH;
   setCameraArgs(a, c->cam, c->W, c->H);
+  a.jitter = c->jitter ? 1 : 0;
+  a.jitterX = c->jitterXY[0];
+  a.jitterY = c->jitterXY[1];
+  a.filterCdf = c->filterSize ? c->filterCdf.p : nullptr;
+  a.filterSize = c->filterSize;
+  a.filterWidth = c->filterWidth;
   prt_frame_dev& f = a.fb;
   f.color = c->color.p; f.weight = c->weight.p; f.color2 = c->color2.p;
+  f.colorB = c->colorB.p; f.weightB = c->weightB.p;
   f.albedo = c->albedo.p; f.albedoW = c->albedoW.p; f.albedo1 = c->albedo1.p; f.albedo1W = c->albedo1W.p;
   f.normal = c->normal.p; f.normalW = c->normalW.p; f.normal1 = c->normal1.p; f.normal1W = c->normal1W.p;
   f.depth = c->depth.p; f.depthW = c->depthW.p;
@@ -1089,7 +1115,7 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
   *c->argsHost = a;
   HIP_TRY(hipMemcpyAsync(c->argsDev.p, c->argsHost, sizeof(RenderArgs), hipMemcpyHostToDevice, c->stream));
   const RenderArgs* ad = c->argsDev.p;
-  HIP_TRY(hipMemsetAsync(c->stats.p, 0, sizeof(unsigned long long) * 8, c->stream));
+  HIP_TRY(hipMemsetAsync(c->stats.p, 0, sizeof(unsigned long long) * kStatCount, c->stream));
   for (int k = 0; k < 4; ++k) c->kernelMs[k] = 0.0;
   c->launches[0] = c->launches[1] = 0;
   // The first kFreeDepths depths of a pass are enqueued without waiting (a depth with no queued
@@ -1134,17 +1160,19 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
       HIP_TRY(hipMemcpyAsync(c->colorC.p, c->color.p, sizeof(float) * 3 * c->npix, hipMemcpyDeviceToDevice, c->stream));
       HIP_TRY(hipMemcpyAsync(c->weightC.p, c->weight.p, sizeof(float) * c->npix, hipMemcpyDeviceToDevice, c->stream));
     }
+    // AccumBuffer::finish sets classB for the next sample from the sample count (accum_buffer.h:36-47)
+    const bool classB = (__builtin_popcount(uint32_t(c->count) & 0xaaaaaaaau) & 1) != 0;
     HIP_TRY(hipEventRecord(ev[0], c->stream));
     launchGenerate(ad, a.npix, c->sampleIndex, c->stream);
     HIP_TRY(hipEventRecord(ev[1], c->stream));
     int depth = 0;
     for (; depth <= a.maxDepth; ++depth)
     {
-      launchIntersect(ad, depth, c->launch, c->stream);
+      launchIntersect(ad, depth, c->launch, c->countSteps, c->stream);
       HIP_TRY(hipEventRecord(ev[2 + 3 * depth], c->stream));
-      launchShade(ad, depth, c->sampleIndex, c->launch, c->extMaterials != 0, c->stream);
+      launchShade(ad, depth, c->sampleIndex, classB, c->launch, c->extMaterials != 0, c->stream);
       HIP_TRY(hipEventRecord(ev[3 + 3 * depth], c->stream));
-      launchOccluded(ad, depth, c->launch, c->anyTransparent != 0, c->stream);
+      launchOccluded(ad, depth, classB, c->launch, c->anyTransparent != 0, c->countSteps, c->stream);
       HIP_TRY(hipEventRecord(ev[4 + 3 * depth], c->stream));
       if (depth + 1 >= kFreeDepths && depth < a.maxDepth)
       {
@@ -1163,9 +1191,9 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
   if (n_spp > 0) HIP_TRY(readTimes((n_spp - 1) & 1));
   HIP_TRY(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::high_resolution_clock::now();
-  unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long st[kStatCount] = {};
   HIP_TRY(hipMemcpy(st, c->stats.p, sizeof(st), hipMemcpyDeviceToHost));
-  for (int k = 0; k < 6; ++k) c->lastStats[k] = double(st[k]);
+  for (int k = 0; k < 10; ++k) c->lastStats[k] = double(st[k]);
   if (stats)
   {
     double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -1333,6 +1361,94 @@ int prt_frame_error(prt_context* c, float* error)
   return PRT_OK;
 }
 
+int prt_blit_half(prt_context* c, float* dst)
+{
+  CTX_CHECK(c);
+  if (!dst) return fail(PRT_ERR_INVALID, "dst is null");
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  if (!c->colorB.p) return fail(PRT_ERR_INVALID, "half buffer not enabled for the buffer");
+  const size_t n = size_t(c->npix);
+  std::vector<float> s(3 * n), w(n);
+  HIP_TRY(hipMemcpy(s.data(), c->colorB.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(w.data(), c->weightB.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i)  // accum_buffer.h:633-643
+  {
+    const float r = w[i] == 0.f ? 3.402823466e38f : 1.0f / w[i];
+    for (int k = 0; k < 3; ++k) dst[i * 3 + k] = s[k * n + i] * r;
+  }
+  return PRT_OK;
+}
+
+int prt_set_jitter(prt_context* c, int32_t enable, float jx, float jy)
+{
+  CTX_CHECK(c);
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  if (enable && !(jx >= 0.f && jx <= 1.f && jy >= 0.f && jy <= 1.f))
+    return fail(PRT_ERR_INVALID, "jitter outside [0, 1]");
+  c->jitter = enable != 0;
+  if (c->jitter) { c->jitterXY[0] = jx; c->jitterXY[1] = jy; }
+  return PRT_OK;
+}
+
+int prt_get_jitter(prt_context* c, int32_t* has, float* jxy)
+{
+  CTX_CHECK(c);
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  if (has) *has = c->jitter ? 1 : 0;
+  if (jxy && c->jitter) { jxy[0] = c->jitterXY[0]; jxy[1] = c->jitterXY[1]; }
+  return PRT_OK;
+}
+
+// The filter's lookup table and its Distribution1D (image/pixel_filter.h:30-44 with the
+// TriangleFilter, GaussianFilter and BlackmanHarrisFilter of :77-139; distribution1d.cpp:20-35).
+int prt_set_pixel_filter(prt_context* c, int32_t type, float width, float sigma, int32_t bias)
+{
+  CTX_CHECK(c);
+  if (!c->npix) return PRT_OK;  // DeviceCpu::initFilter returns without a frame buffer
+  if (type == PRT_FILTER_BOX)
+  {
+    c->filterCdf.release();
+    c->filterSize = 0;
+    c->filterWidth = 0.f;
+    return PRT_OK;
+  }
+  if (type != PRT_FILTER_TRIANGLE && type != PRT_FILTER_GAUSSIAN && type != PRT_FILTER_BLACKMAN_HARRIS)
+    return fail(PRT_ERR_INVALID, "invalid pixel filter type");
+  if (!(width > 0.f)) width = type == PRT_FILTER_TRIANGLE ? 2.f : 3.f;  // device_cpu.cpp:247-262 defaults
+  if (type == PRT_FILTER_GAUSSIAN && !(sigma > 0.f)) sigma = width / 6.f;
+  const float radius = width / 2.f;
+  const float alpha = type == PRT_FILTER_GAUSSIAN ? -1.f / (2.f * sigma * sigma) : 0.f;
+  const float beta = (type == PRT_FILTER_GAUSSIAN && bias) ? std::exp(alpha * radius * radius) : 0.f;
+  auto eval = [&](float x) -> float {
+    if (type == PRT_FILTER_TRIANGLE) return std::max(radius - std::fabs(x), 0.f);
+    if (type == PRT_FILTER_GAUSSIAN) return std::max(std::exp(alpha * x * x) - beta, 0.f);
+    x /= radius;
+    if (x < -1.f || x > 1.f) return 0.f;
+    const float t = (x + 1.f) * 3.14159265358979323846f;
+    return 0.35875f - 0.48829f * std::cos(t) + 0.14128f * std::cos(2.f * t) - 0.01168f * std::cos(3.f * t);
+  };
+  const int n = 2048, size = 2 * n + 1;
+  std::vector<float> cdf(size + 1);
+  cdf[0] = 0.f;
+  for (int i = 1; i <= size; ++i) cdf[i] = cdf[i - 1] + eval((float(i - 1 - n) / float(n)) * radius);
+  const float norm = cdf[size] == 0.f ? 0.f : 1.0f / cdf[size];
+  for (int i = 1; i <= size; ++i) cdf[i] *= norm;
+  cdf[size] = 1.f;
+  HIP_TRY(c->filterCdf.alloc(cdf.size()));
+  HIP_TRY(hipMemcpy(c->filterCdf.p, cdf.data(), sizeof(float) * cdf.size(), hipMemcpyHostToDevice));
+  c->filterSize = size;
+  c->filterWidth = width;
+  return PRT_OK;
+}
+
+int prt_get_filter_width(prt_context* c, float* width)
+{
+  CTX_CHECK(c);
+  if (!width) return fail(PRT_ERR_INVALID, "width is null");
+  *width = c->filterSize ? c->filterWidth : 0.f;  // device_cpu.h:73-78
+  return PRT_OK;
+}
+
 int prt_query_rays_device(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v)
 {
   CTX_CHECK(c);
@@ -1469,6 +1585,33 @@ int prt_debug_luts(prt_context* c, float* out, int32_t capacity, int32_t* count)
   if (!out) return PRT_OK;
   if (capacity < PRT_LUT_FLOATS) return fail(PRT_ERR_INVALID, "LUT buffer too small");
   std::memcpy(out, c->luts.data(), sizeof(float) * PRT_LUT_FLOATS);
+  return PRT_OK;
+}
+
+int prt_debug_count_steps(prt_context* c, int32_t enable)
+{
+  CTX_CHECK(c);
+  c->countSteps = enable != 0;
+  return PRT_OK;
+}
+
+int prt_debug_trace_steps(prt_context* c, double* steps4)
+{
+  CTX_CHECK(c);
+  if (!steps4) return fail(PRT_ERR_INVALID, "steps4 is null");
+  for (int k = 0; k < 4; ++k) steps4[k] = c->lastStats[6 + k];
+  return PRT_OK;
+}
+
+int prt_debug_light_cdf(prt_context* c, float* out, int32_t capacity, int32_t* count)
+{
+  CTX_CHECK(c);
+  if (!c->lightsReady) return fail(PRT_ERR_STATE, "lights not set");
+  const int n = int(c->lightCdf.size());
+  if (count) *count = n;
+  if (!out) return PRT_OK;
+  if (capacity < n) return fail(PRT_ERR_INVALID, "light CDF buffer too small");
+  std::memcpy(out, c->lightCdf.data(), sizeof(float) * n);
   return PRT_OK;
 }
 

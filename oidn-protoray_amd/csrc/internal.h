@@ -56,6 +56,14 @@ struct RenderArgs
   float camPos[3], imageO[3], imageDx[3], imageDy[3], camU[3], camV[3], camN[3];
   float lensRadius, focal, nearClip, nearClipScale;
   int pinhole;
+  // pixel sample (path_renderer_stream.h:228-236): a fixed sub-pixel jitter for every pixel
+  // (FrameBuffer::setJitter), else the sampler's dimension-set 0 value, warped by the pixel
+  // filter's tabulated CDF when one is set (image/pixel_filter.h:30-66, PixelFilterMode::Sample)
+  int jitter;
+  float jitterX, jitterY;
+  const float* filterCdf;  // filterSize + 1 entries, or null (box filter)
+  int filterSize;
+  float filterWidth;
   // frame
   int W, H, npix;
   float invW, invH;
@@ -80,7 +88,10 @@ struct RenderArgs
   int* counters;  // [2*d] paths queued for depth d (d >= 1), [2*d+1] shadow rays of depth d
   int* spill;     // traversal stack overflow, 2 x (PRT_MAX_STACK - PRT_LDS_STACK) x spillStride (nodes, distances)
   int spillStride;
-  unsigned long long* stats;  // closest hit: [0] rays, [1] inner-node visits, [2] triangle tests; shadow: [3..5]
+  // closest hit: [0] rays, [1] inner-node visits, [2] triangle tests; shadow: [3..5]; traversal
+  // loop iterations summed over lanes and over waves (the longest lane's per call): closest hit
+  // [6], [7], shadow [8], [9]
+  unsigned long long* stats;
 };
 
 // One frame of a persistent batch (batch.hip): its scene and camera as kernel arguments, its
@@ -112,9 +123,9 @@ struct LaunchCfg
 // The kernels read RenderArgs from device memory (one copy per render call); passing the
 // ~1 KB record by value would make every lane copy it to scratch.
 void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStream_t s);
-void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s);
-void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, bool ext, hipStream_t s);
-void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, bool filter, hipStream_t s);
+void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool count, hipStream_t s);
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, bool ext, hipStream_t s);
+void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, bool filter, bool count, hipStream_t s);
 void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s);
 void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
@@ -128,6 +139,7 @@ void launchAdjointReduce(const float* partial, int nBlocks, int nParams, float* 
 void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s);
 
 constexpr int kTraceBlock = 256;
+constexpr int kStatCount = 12;
 constexpr int kShadeBlock = 256;
 
 // Slot <-> pixel mapping: slot = (tile * 64) + (ly * 8 + lx), tiles row-major over W/8 columns.

@@ -109,9 +109,15 @@ PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, fl
   f3 N = Ng;
   if (a.nrm)
   {
+    // a triangle of a geometry without normals carries zero vertex normals in the shared array:
+    // it keeps N = Ng, as a TriangleMesh without normals does (geometry/triangle_mesh.cpp:176-189)
     float b0 = 1.f - u - v;
-    N = normalize(b0 * ld3(a.nrm + 3 * i0) + u * ld3(a.nrm + 3 * i1) + v * ld3(a.nrm + 3 * i2));
-    if (dot(N, Ng) < 0.f) N = -N;
+    f3 Ns = b0 * ld3(a.nrm + 3 * i0) + u * ld3(a.nrm + 3 * i1) + v * ld3(a.nrm + 3 * i2);
+    if (lengthSqr(Ns) > 0.f)
+    {
+      N = normalize(Ns);
+      if (dot(N, Ng) < 0.f) N = -N;
+    }
   }
   makeFrame(c.f.U, c.f.V, N);
   if (dot(Ng, d) > 0.f) { Ng = -Ng; N = -N; }
@@ -142,15 +148,40 @@ struct ShadowFilter
 };
 
 
-// Camera ray of pixel (x, y) for sample sampleIndex (path_renderer_stream.h:209-268): pixel
-// jitter from dimension set 0, lens sample from set 2.
+// PixelFilter::sample for one coordinate (image/pixel_filter.h:57-63): a continuous sample of
+// the filter's tabulated distribution (Distribution1D::sampleContinuous,
+// sampling/distribution1d.h:104-108), centred and scaled to the filter width.
+PRT_DEV float filterSample(const RenderArgs& a, float s)
+{
+  float pdf;
+  const int i = dist1dSampleReuse(a.filterCdf, a.filterSize, pdf, s);
+  const float x = (float(i) + s) * (1.0f / float(a.filterSize));
+  return (x - 0.5f) * a.filterWidth;
+}
+
+// Camera ray of pixel (x, y) for sample sampleIndex (path_renderer_stream.h:209-268): the pixel
+// sample from the jitter or from dimension set 0 (through the pixel filter), the lens sample
+// from set 2.
 PRT_DEV void cameraRay(const RenderArgs& a, int x, int y, uint32_t sampleIndex, f3& org, f3& dir, float& tnear)
 {
   uint32_t pixelId = uint32_t(x) | (uint32_t(y) << 16);  // path_renderer_stream.h:219-220
   Sampler smp;
   smp.set(a.seed, sampleIndex, pixelId);
   float psx, psy, lsx, lsy;
-  smp.get2D(0, psx, psy);  // sampleDimPixel (integrator.h:52)
+  if (a.jitter)  // (:228-231)
+  {
+    psx = a.jitterX;
+    psy = a.jitterY;
+  }
+  else
+  {
+    smp.get2D(0, psx, psy);  // sampleDimPixel (integrator.h:52)
+    if (a.filterCdf)         // (:234-235)
+    {
+      psx = filterSample(a, psx) + 0.5f;
+      psy = filterSample(a, psy) + 0.5f;
+    }
+  }
   smp.get2D(2, lsx, lsy);  // sampleDimLens (integrator.h:53)
   float imx = (float(x) + psx) * a.invW, imy = (float(y) + psy) * a.invH;
   f3 imageO = ld3(a.imageO), imageDx = ld3(a.imageDx), imageDy = ld3(a.imageDy);

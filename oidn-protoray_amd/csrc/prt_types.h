@@ -141,6 +141,8 @@ struct prt_frame_dev  // device pointers of the frame accumulators (FP32 sums, p
   float* color;   // 3 x npix
   float* weight;  // npix
   float* color2;  // 3 x npix (sum of squares) or null
+  float* colorB;  // 3 x npix: sums of the class-B samples only (the half buffer) or null
+  float* weightB; // npix
   float* albedo; float* albedoW;
   float* albedo1; float* albedo1W;
   float* normal; float* normalW;

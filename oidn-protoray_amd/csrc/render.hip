@@ -41,6 +41,12 @@ PRT_DEV unsigned long long waveSum(unsigned long long v)
   return v;
 }
 
+PRT_DEV uint32_t waveMax(uint32_t v)
+{
+  for (int off = 32; off > 0; off >>= 1) v = max(v, uint32_t(__shfl_xor(int(v), off)));
+  return v;
+}
+
 
 // Coarse direction cell (1/32 per axis) folded into the debug path hash.
 PRT_DEV uint32_t dirCell(f3 d)
@@ -59,8 +65,10 @@ PRT_DEV void addAov3(float* buf, float* w, int npix, int pix, f3 c)
 }
 
 
-// Adds the path's radiance as this pass's sample of its pixel (path_renderer_stream.h:323,1026).
-PRT_DEV void finalizePath(const RenderArgs& a, int slot, f3 L, uint32_t hash)
+// Adds the path's radiance as this pass's sample of its pixel (path_renderer_stream.h:323,1026);
+// classB: the pass is a class-B sample and also goes to the half buffer (image/accum_buffer.h:
+// 36-47, 455-470).
+PRT_DEV void finalizePath(const RenderArgs& a, int slot, f3 L, uint32_t hash, bool classB)
 {
   int x, y;
   slotToPixel(slot, a.W, x, y);
@@ -71,6 +79,13 @@ PRT_DEV void finalizePath(const RenderArgs& a, int slot, f3 L, uint32_t hash)
   fb.color[a.npix + pix] += c.y;
   fb.color[2 * a.npix + pix] += c.z;
   fb.weight[pix] += 1.f;
+  if (fb.colorB && classB)
+  {
+    fb.colorB[pix] += c.x;
+    fb.colorB[a.npix + pix] += c.y;
+    fb.colorB[2 * a.npix + pix] += c.z;
+    fb.weightB[pix] += 1.f;
+  }
   if (fb.color2)
   {
     fb.color2[pix] += c.x * c.x;
@@ -104,6 +119,10 @@ __global__ void __launch_bounds__(256) kGenerate(const RenderArgs* __restrict__ 
   if (a.hash) a.hash[slot] = 0x9e3779b9u;
 }
 
+// Closest hits of the queued paths of depth `depth`.  kCount: also count the traversal loop
+// iterations per lane and per wave (the SIMD utilisation), in a separate build that profiling
+// passes use, so the timed kernel keeps its registers and occupancy.
+template <bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __restrict__ ap, int depth)
 {
   const RenderArgs& a = *ap;
@@ -112,22 +131,42 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __re
   TraceStack st = makeStack(lds, a.spill, a.spillStride, gtid);
   const int count = depth == 0 ? a.npix : a.counters[2 * depth];
   const int* q = a.queue[depth & 1];
-  uint32_t nodes = 0, tris = 0, rays = 0;
-  for (int i = gtid; i < count; i += gridDim.x * blockDim.x)
-  {
+  uint32_t nodes = 0, tris = 0, rays = 0, waveSteps = 0;
+  auto one = [&](int i) {
     int slot = depth == 0 ? i : q[i];
     float4 ro = a.rayO[slot], rd = a.rayD[slot];
     float t, u = 0.f, v = 0.f;
     int prim = traceRay<false>(a.bvh, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, st, nodes, tris);
     a.hit[slot] = make_float4(t, u, v, __int_as_float(prim));
     ++rays;
-  }
+  };
+  const int stride = gridDim.x * blockDim.x;
+  if (!kCount)
+    for (int i = gtid; i < count; i += stride) one(i);
+  else
+    // the lanes of a wave run the same number of iterations here, so the wave's steps for a batch
+    // of rays (its longest lane's) are a cross-lane maximum
+    for (int base = gtid - laneId(); base < count; base += stride)
+    {
+      const uint32_t before = st.steps;
+      if (base + laneId() < count) one(base + laneId());
+      waveSteps += waveMax(st.steps - before);
+    }
   unsigned long long r = waveSum(rays), nv = waveSum(nodes), tt = waveSum(tris);
   if (laneId() == 0 && r)
   {
     atomicAdd(&a.stats[0], r);
     atomicAdd(&a.stats[1], nv);
     atomicAdd(&a.stats[2], tt);
+  }
+  if (kCount)
+  {
+    unsigned long long ls = waveSum(st.steps);
+    if (laneId() == 0 && r)
+    {
+      atomicAdd(&a.stats[6], ls);
+      atomicAdd(&a.stats[7], (unsigned long long)waveSteps);
+    }
   }
 }
 
@@ -141,7 +180,7 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __re
 // keeps the register budget of the four original lobes).
 template <bool kExt>
 __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(PRT_SHADE_WAVES_PER_EU)))
-kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
+kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool classB)
 {
   const RenderArgs& a = *ap;
   const int count = depth == 0 ? a.npix : a.counters[2 * depth];
@@ -198,7 +237,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
           addAov3(a.fb.albedo, a.fb.albedoW, a.npix, pix, f3(0.f));
           addAov3(a.fb.normal, a.fb.normalW, a.npix, pix, f3(0.f));
         }
-        finalizePath(a, slot, L, hsh);
+        finalizePath(a, slot, L, hsh, classB);
       }
       else if (a.mats[matId].emissive)
       {
@@ -223,11 +262,11 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
           addAov3(a.fb.albedo, a.fb.albedoW, a.npix, pix, f3(1.f));
           addAov3(a.fb.normal, a.fb.normalW, a.npix, pix, Nv);
         }
-        finalizePath(a, slot, L, hsh);
+        finalizePath(a, slot, L, hsh, classB);
       }
       else if (depth >= a.maxDepth)
       {
-        finalizePath(a, slot, L, hsh);  // no shading beyond maxDepth (:615)
+        finalizePath(a, slot, L, hsh, classB);  // no shading beyond maxDepth (:615)
       }
       else
       {
@@ -331,7 +370,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
         if (!ext)
         {
           if (wantShadow) fl |= PRT_F_TERMINATE;  // accumulate after the shadow test
-          else finalizePath(a, slot, L, hsh);
+          else finalizePath(a, slot, L, hsh, classB);
         }
       }
       a.flags[slot] = fl;
@@ -344,19 +383,10 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex)
   }
 }
 
-// Shadow rays of depth `depth`.  kFilter: the scene has transparent materials, and the shadow
-// rays gather the transmittance of the transparent surfaces they cross (path_renderer_stream.h:
-// 951-1002 with the TransparentShadowFilter of render/transparent_filter.h).
+// One shadow ray of the queue (path_renderer_stream.h:951-1065).
 template <bool kFilter>
-__global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth)
+PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& nodes, uint32_t& tris, uint32_t& rays, bool classB)
 {
-  const RenderArgs& a = *ap;
-  __shared__ TraceLds<kTraceBlock> lds;
-  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  TraceStack st = makeStack(lds, a.spill, a.spillStride, gtid);
-  const int count = a.counters[2 * depth + 1];
-  uint32_t nodes = 0, tris = 0, rays = 0;
-  for (int i = gtid; i < count; i += gridDim.x * blockDim.x)
   {
     int slot = a.shadowQ[i];
     float4 so = a.shO[slot], sd = a.shD[slot];
@@ -388,14 +418,47 @@ __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __res
       hsh = hashMix(a.hash[slot], prim < 0 ? 1u + 4u * crossed : 2u);  // + the transparent surfaces crossed
       a.hash[slot] = hsh;
     }
-    if (a.flags[slot] & PRT_F_TERMINATE) finalizePath(a, slot, L, hsh);
+    if (a.flags[slot] & PRT_F_TERMINATE) finalizePath(a, slot, L, hsh, classB);
   }
+}
+
+// Shadow rays of depth `depth`.  kFilter: the scene has transparent materials, and the shadow
+// rays gather the transmittance of the transparent surfaces they cross (path_renderer_stream.h:
+// 951-1002 with the TransparentShadowFilter of render/transparent_filter.h).
+template <bool kFilter, bool kCount>
+__global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth, bool classB)
+{
+  const RenderArgs& a = *ap;
+  __shared__ TraceLds<kTraceBlock> lds;
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  TraceStack st = makeStack(lds, a.spill, a.spillStride, gtid);
+  const int count = a.counters[2 * depth + 1];
+  uint32_t nodes = 0, tris = 0, rays = 0, waveSteps = 0;
+  const int stride = gridDim.x * blockDim.x;
+  if (!kCount)
+    for (int i = gtid; i < count; i += stride) occludedRay<kFilter>(a, i, st, nodes, tris, rays, classB);
+  else
+    for (int base = gtid - laneId(); base < count; base += stride)
+    {
+      const uint32_t before = st.steps;
+      if (base + laneId() < count) occludedRay<kFilter>(a, base + laneId(), st, nodes, tris, rays, classB);
+      waveSteps += waveMax(st.steps - before);
+    }
   unsigned long long r = waveSum(rays), nv = waveSum(nodes), tt = waveSum(tris);
   if (laneId() == 0 && r)
   {
     atomicAdd(&a.stats[3], r);
     atomicAdd(&a.stats[4], nv);
     atomicAdd(&a.stats[5], tt);
+  }
+  if (kCount)
+  {
+    unsigned long long ls = waveSum(st.steps);
+    if (laneId() == 0 && r)
+    {
+      atomicAdd(&a.stats[8], ls);
+      atomicAdd(&a.stats[9], (unsigned long long)waveSteps);
+    }
   }
 }
 
@@ -514,19 +577,25 @@ void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStre
 {
   kGenerate<<<(npix + 255) / 256, 256, 0, s>>>(a, sampleIndex);
 }
-void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, hipStream_t s)
+void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool count, hipStream_t s)
 {
-  kIntersect<<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  if (count) kIntersect<true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  else kIntersect<false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
 }
-void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, const LaunchCfg& c, bool ext, hipStream_t s)
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, bool ext, hipStream_t s)
 {
-  if (ext) kShade<true><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex);
-  else kShade<false><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex);
+  if (ext) kShade<true><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
+  else kShade<false><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
 }
-void launchOccluded(const RenderArgs* a, int depth, const LaunchCfg& c, bool filter, hipStream_t s)
+void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, bool filter, bool count, hipStream_t s)
 {
-  if (filter) kOccluded<true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
-  else kOccluded<false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth);
+  if (count)
+  {
+    if (filter) kOccluded<true, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    else kOccluded<false, true><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
+  }
+  else if (filter) kOccluded<true, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+  else kOccluded<false, false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
 }
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit, int* spill,
                  int spillStride, int grid, hipStream_t s)

@@ -77,6 +77,7 @@ struct TraceStack
   int* spill;     // this lane's overflow column: spill[k * stride]
   int stride;
   int sp;
+  uint32_t steps;  // traversal loop iterations of this lane (inner-node and leaf visits)
   // The build rejects trees deeper than PRT_MAX_STACK, so the last check never drops an entry;
   // it only keeps every store inside the allocated overflow area.
   PRT_DEV void push(int v, float t)
@@ -117,6 +118,7 @@ PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int 
   st.spill = spill + gtid;
   st.stride = stride;
   st.sp = 0;
+  st.steps = 0;
   return st;
 }
 
@@ -142,6 +144,7 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
   const float inf = __int_as_float(0x7f800000);
   for (;;)
   {
+    ++st.steps;
     if (node >= 0)
     {
       float t[4];

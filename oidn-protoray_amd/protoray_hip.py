@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libprt_hip.so")
 
 PRT_OK, PRT_ERR_INVALID, PRT_ERR_HIP, PRT_ERR_NOMEM, PRT_ERR_STATE = 0, -1, -2, -3, -4
-AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR = 1, 2, 4, 8, 16, 32
+AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR, AOV_HALF = 1, 2, 4, 8, 16, 32, 64
+FILTER_BOX, FILTER_TRIANGLE, FILTER_GAUSSIAN, FILTER_BLACKMAN_HARRIS = 0, 1, 2, 3
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
 BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_SNAPSHOT = 0, 1, 2, 16
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
@@ -70,6 +71,11 @@ SIGNATURES = {
     "prt_render": [_vp, _i, ctypes.POINTER(PrtStats)],
     "prt_blit": [_vp, _i, _vp, _vp],
     "prt_get_accum": [_vp, _vp, _vp],
+    "prt_blit_half": [_vp, _vp],
+    "prt_set_jitter": [_vp, _i, _f, _f],
+    "prt_get_jitter": [_vp, _vp, _vp],
+    "prt_set_pixel_filter": [_vp, _i, _f, _f, _i],
+    "prt_get_filter_width": [_vp, ctypes.POINTER(ctypes.c_float)],
     "prt_frame_error": [_vp, ctypes.POINTER(ctypes.c_float)],
     "prt_query_rays": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_query_occlusion": [_vp, _i, _vp, _vp],
@@ -81,6 +87,9 @@ SIGNATURES = {
     "prt_debug_get": [_vp, _vp, _vp, _vp],
     "prt_debug_sampler": [_vp, _i, _vp, _i, _vp, _vp, _vp],
     "prt_debug_luts": [_vp, _vp, ctypes.c_int32, _vp],
+    "prt_debug_light_cdf": [_vp, _vp, ctypes.c_int32, _vp],
+    "prt_debug_trace_steps": [_vp, _vp],
+    "prt_debug_count_steps": [_vp, _i],
     "prt_adjoint": [_vp, _vp, _i, _vp, _vp, _i, _vp],
     "prt_denoiser_create": [_i, _vp],
     "prt_denoiser_destroy": [_vp],
@@ -227,6 +236,42 @@ class DeviceHip:
         _check(_lib.prt_blit(self.h, buffer, _ptr(out), _ptr(var)))
         return (out, var) if variance else out
 
+    def blitHalf(self):
+        """Mean of the class-B samples (AccumBuffer3f::blit destHalf); needs AOV_HALF."""
+        W, H = self.size
+        out = np.zeros((H, W, 3), np.float32)
+        _check(_lib.prt_blit_half(self.h, _ptr(out)))
+        return out
+
+    def setJitter(self, jitter):
+        """Device::setJitter: the sub-pixel position (x, y) in [0, 1]^2 of every pixel's sample."""
+        _check(_lib.prt_set_jitter(self.h, 1, float(jitter[0]), float(jitter[1])))
+
+    def disableJitter(self):
+        _check(_lib.prt_set_jitter(self.h, 0, 0.0, 0.0))
+
+    def hasJitter(self):
+        has = ctypes.c_int32(0)
+        _check(_lib.prt_get_jitter(self.h, ctypes.byref(has), None))
+        return bool(has.value)
+
+    def getJitter(self):
+        has = ctypes.c_int32(0)
+        j = np.zeros(2, np.float32)
+        _check(_lib.prt_get_jitter(self.h, ctypes.byref(has), _ptr(j)))
+        if not has.value:
+            raise ValueError("no jitter set")
+        return float(j[0]), float(j[1])
+
+    def initFilter(self, filter_type, width=0.0, sigma=0.0, bias=True):
+        """Device::initFilter (sample mode): FILTER_BOX / TRIANGLE / GAUSSIAN / BLACKMAN_HARRIS."""
+        _check(_lib.prt_set_pixel_filter(self.h, int(filter_type), float(width), float(sigma), 1 if bias else 0))
+
+    def getFilterWidth(self):
+        w = ctypes.c_float(0)
+        _check(_lib.prt_get_filter_width(self.h, ctypes.byref(w)))
+        return w.value
+
     def getAccum(self):
         W, H = self.size
         s = np.zeros((3, W * H), np.float32)
@@ -314,6 +359,24 @@ class DeviceHip:
         _check(_lib.prt_debug_luts(self.h, _ptr(out), out.size, ctypes.byref(n)))
         return out
 
+    def countSteps(self, enable=True):
+        """Renders from now on use the traversal kernels that count their loop iterations."""
+        _check(_lib.prt_debug_count_steps(self.h, 1 if enable else 0))
+
+    def traceSteps(self):
+        """Traversal loop iterations of the last render: lane sums and wave sums (closest, shadow)."""
+        s = np.zeros(4, np.float64)
+        _check(_lib.prt_debug_trace_steps(self.h, _ptr(s)))
+        return {"closest_lane_steps": s[0], "closest_wave_steps": s[1], "shadow_lane_steps": s[2],
+                "shadow_wave_steps": s[3]}
+
+    def lightCdf(self):
+        n = ctypes.c_int32(0)
+        _check(_lib.prt_debug_light_cdf(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        _check(_lib.prt_debug_light_cdf(self.h, _ptr(out), out.size, ctypes.byref(n)))
+        return out
+
     def bvhStats(self):
         s = np.zeros(4, np.int64)
         _check(_lib.prt_debug_bvh_stats(self.h, _ptr(s)))
@@ -381,7 +444,7 @@ def load_scene(dev, scene):
     """Uploads a scene dict from scenes/scenegen.py and configures renderer, camera and frame."""
     dev.initScene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"], scene.get("normals"))
     dev.initLights(scene["lights"], scene.get("env"))
-    dev.initRenderer(maxDepth=scene["max_depth"])
+    dev.initRenderer(maxDepth=scene["max_depth"], rrDepth=scene.get("rr_depth", 8))
     dev.initSampler(scene.get("seed", 0))
     dev.initCamera(scene["camera"])
 

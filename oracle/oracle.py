@@ -50,10 +50,22 @@ def _p(a, t=ctypes.c_float):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
 
 
+_lib_name = "liboracle.so"
+
+
+def select_library(name):
+    """Chooses the build to load before the first call: liboracle.so (the parity checker, FP
+    contraction off) or liboracle_bench.so (the timed CPU baseline, AVX2 + FMA)."""
+    global _lib_name
+    if _lib is not None and name != _lib_name:
+        raise RuntimeError("the oracle library is already loaded")
+    _lib_name = name
+
+
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        path = os.path.join(HERE, _lib_name)
         if not os.path.exists(path):
             raise RuntimeError("oracle library not built: run `make -C oracle`")
         L = ctypes.CDLL(path)
@@ -86,6 +98,8 @@ def lib():
         L.orc_adjoint.argtypes = [vp, vp, i, i, i, u, i, i, f, vp, i, vp]
         L.orc_bsdf_grad.argtypes = [i, vp, vp, vp, vp]
         L.orc_bsdf_mat.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_set_pixel_sample.argtypes = [vp, i, f, f, i, f, f, i]
+        L.orc_pixel_sample.argtypes = [vp, i, vp, vp]
         _lib = L
     return _lib
 
@@ -238,6 +252,19 @@ class Scene:
         if rc != 0:
             raise ValueError("invalid light list")
 
+    def set_pixel_sample(self, jitter=None, filter_type=0, width=0.0, sigma=0.0, bias=1):
+        """Pixel sample of later renders: a fixed jitter (jx, jy), or the pixel filter (0 box,
+        1 triangle, 2 Gaussian, 3 Blackman-Harris) applied to the sampler's value."""
+        jx, jy = (0.0, 0.0) if jitter is None else jitter
+        lib().orc_set_pixel_sample(self.h, 0 if jitter is None else 1, jx, jy, filter_type, width, sigma, bias)
+
+    def pixel_sample(self, s2):
+        """Pixel-sample positions (n x 2) for sampler values s2 (n x 2) under the jitter / filter."""
+        s2 = c32(s2)
+        out = np.zeros_like(s2)
+        lib().orc_pixel_sample(self.h, s2.shape[0], s2.ctypes.data, out.ctypes.data)
+        return out
+
     @property
     def radius(self):
         return lib().orc_scene_radius(self.h)
@@ -291,13 +318,17 @@ class Scene:
         if accum is None:
             accum = {"color": np.zeros((3, npix), np.float32), "weight": np.zeros(npix, np.float32)}
             for a in aovs:
+                if a == "color2":  # sums of squares of the color samples (AccumFlag::SampleVar)
+                    accum[a] = np.zeros((3, npix), np.float32)
+                    continue
                 accum[a] = np.zeros((1, npix) if a == "depth" else (3, npix), np.float32)
                 accum[a + "W"] = np.zeros(npix, np.float32)
         names = ["color", "weight", "albedo", "albedoW", "albedo1", "albedo1W", "normal", "normalW", "normal1",
                  "normal1W", "depth", "depthW"]
-        bufs = (ctypes.c_void_p * 13)()
+        bufs = (ctypes.c_void_p * 14)()
         for k, nm in enumerate(names):
             bufs[k] = accum[nm].ctypes.data if nm in accum else None
+        bufs[13] = accum["color2"].ctypes.data if "color2" in accum else None
         sampleL = hashes = first = None
         if record:
             sampleL = np.zeros((3, npix), np.float32)

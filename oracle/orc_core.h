@@ -1450,9 +1450,9 @@ inline float lightPower(const Light& l, const EnvImage* env, float sceneRadius)
   V3 c(l.r, l.g, l.b);
   if (l.kind == kAmbient) return 4.f * sqr(kPi) * sqr(sceneRadius) * luminance(c);
   if (l.kind == kDistant) return 4.f * sqr(kPi) * sqr(std::sin(0.5f * l.angle)) * sqr(sceneRadius) * luminance(c);
-  float sumL = 0.f;
+  float sumL = 0.f;  // over the image the light emits: the texels times its scale
   for (int y = 0; y < env->H; ++y)
-    for (int x = 0; x < env->W; ++x) sumL += luminance(envTexel(*env, x, y));
+    for (int x = 0; x < env->W; ++x) sumL += luminance(envTexel(*env, x, y) * c);
   return 4.f * sqr(kPi) * sqr(sceneRadius) * (sumL / float(env->W * env->H));
 }
 

@@ -42,6 +42,12 @@ struct Scene
   // 1 for every supported dielectric, so distinct IORs get IDs 1, 2, ... in material order.
   std::vector<int> matMedium;
   bool anyTransparent = false;  // render/scene.cpp:452-462: shadow rays use the transparent filter
+  // pixel sample (path_renderer_stream.h:228-236): a fixed jitter, or the sampler's value warped
+  // by the pixel filter's tabulated CDF (image/pixel_filter.h:30-66) when filterCdf is not empty
+  bool jitter = false;
+  float jitterXY[2] = {0.f, 0.f};
+  std::vector<float> filterCdf;
+  float filterWidth = 0.f;
 };
 
 static void assignMedia(Scene& s)
@@ -344,9 +350,15 @@ static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u,
   V3 N = Ng;
   if (!s.nrm.empty())
   {
+    // zero vertex normals (a geometry without normals in a mixed file) keep N = Ng, as a
+    // TriangleMesh without normals does (geometry/triangle_mesh.cpp:176-189)
     float b0 = 1.f - u - v;
-    N = normalize(b0 * s.nrm[i0] + u * s.nrm[i1] + v * s.nrm[i2]);
-    if (dot(N, Ng) < 0.f) N = -N;
+    V3 Ns = b0 * s.nrm[i0] + u * s.nrm[i1] + v * s.nrm[i2];
+    if (dot(Ns, Ns) > 0.f)
+    {
+      N = normalize(Ns);
+      if (dot(N, Ng) < 0.f) N = -N;
+    }
   }
   c.Ng = Ng;
   c.f.N = N;
@@ -429,6 +441,7 @@ struct Accum  // sums are FP32, added in sample order (image/accum_buffer.h:439-
   float* normal; float* normalW;
   float* normal1; float* normal1W;
   float* depth; float* depthW;
+  float* color2;  // [3][W*H] sums of squares (AccumFlag::SampleVar) or null
   // per-sample debug outputs of the last pass (nullable)
   float* sampleL;      // [3][W*H]
   uint32_t* pathHash;  // [W*H]
@@ -440,6 +453,17 @@ static inline void add3(float* buf, float* w, int npix, int pix, V3 c)
   if (!buf) return;
   buf[pix] += c.x; buf[npix + pix] += c.y; buf[2 * npix + pix] += c.z;
   w[pix] += 1.f;
+}
+
+// AccumBuffer3f::addSerial (image/accum_buffer.h:439-486): sum, weight and, with SampleVar, the
+// sum of squares
+static inline void addColor(Accum& A, int npix, int pix, V3 c)
+{
+  add3(A.color, A.weight, npix, pix, c);
+  if (A.color2)
+  {
+    A.color2[pix] += c.x * c.x; A.color2[npix + pix] += c.y * c.y; A.color2[2 * npix + pix] += c.z * c.z;
+  }
 }
 
 // integrator.h:27-46
@@ -461,6 +485,29 @@ static inline uint32_t dirCell(V3 d)
 }
 
 struct Stats { long long rays = 0; long long nodes = 0; };
+
+// The pixel sample (path_renderer_stream.h:228-236) from the sampler's dimension-set 0 value:
+// the jitter when one is set, else the value warped by the pixel filter (PixelFilter::sample +
+// 0.5, image/pixel_filter.h:57-63, Distribution1D::sampleContinuous, distribution1d.h:104-108).
+static void pixelSample(const Scene& s, float& psx, float& psy)
+{
+  if (s.jitter)
+  {
+    psx = s.jitterXY[0];
+    psy = s.jitterXY[1];
+    return;
+  }
+  if (s.filterCdf.empty()) return;
+  const int size = int(s.filterCdf.size()) - 1;
+  auto warp = [&](float u) {
+    float pdf;
+    int i = dist1dSampleReuse(s.filterCdf.data(), size, pdf, u);
+    float x = (float(i) + u) * (1.0f / float(size));
+    return (x - 0.5f) * s.filterWidth + 0.5f;
+  };
+  psx = warp(psx);
+  psy = warp(psy);
+}
 
 // One path of one sample (path_renderer_stream.h:209-268 init, :276-1071 bounce loop)
 // ---- FP64 parameter derivatives of the BSDF values (the adjoint's checker) --------------
@@ -784,6 +831,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
   smp.set(P.seed, sampleIndex, pixelId);
   float psx, psy, lsx, lsy;
   smp.get2D(0, psx, psy);  // sampleDimPixel
+  pixelSample(s, psx, psy);
   float imx = (float(x) + psx) * (1.0f / float(P.W));
   float imy = (float(y) + psy) * (1.0f / float(P.H));
   smp.get2D(2, lsx, lsy);  // sampleDimLens
@@ -847,7 +895,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
       L += T * w * Le;
       V3 c = clampL(L, P.maxRadiance);
-      add3(A.color, A.weight, npix, pix, c);
+      addColor(A, npix, pix, c);
       if (!aov1Done)
       {
         add3(A.albedo1, A.albedo1W, npix, pix, V3(0.f));
@@ -876,7 +924,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
       L += T * w * Le;
       V3 c = clampL(L, P.maxRadiance);
-      add3(A.color, A.weight, npix, pix, c);
+      addColor(A, npix, pix, c);
       V3 Nv = normalize(-dir);
       if (!aov1Done)
       {
@@ -1036,7 +1084,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
     if (!ext)
     {
       V3 c = clampL(L, P.maxRadiance);
-      add3(A.color, A.weight, npix, pix, c);
+      addColor(A, npix, pix, c);
       if (recordSample)
       {
         if (A.sampleL) { A.sampleL[pix] = c.x; A.sampleL[npix + pix] = c.y; A.sampleL[2 * npix + pix] = c.z; }
@@ -1192,7 +1240,7 @@ void orc_intersect(void* p, int n, const float* rays, int mode, int* prim, float
 // bounded band of the frame).  Returns the number of rays traced.
 long long orc_render(void* p, const void* camp, int W, int H, int rowBegin, int rowEnd, int sppStart, int sppCount,
                      uint32_t seed, int maxDepth, int rrDepth, float minWeight, float maxRadiance, int threads,
-                     float** buffers /* color, weight, albedo, albedoW, albedo1, albedo1W, normal, normalW, normal1, normal1W, depth, depthW, sampleL */,
+                     float** buffers /* color, weight, albedo, albedoW, albedo1, albedo1W, normal, normalW, normal1, normal1W, depth, depthW, sampleL, color2 */,
                      uint32_t* pathHash, int* firstPrim)
 {
   Scene& s = *(Scene*)p;
@@ -1206,6 +1254,7 @@ long long orc_render(void* p, const void* camp, int W, int H, int rowBegin, int 
   A.normal1 = buffers[8]; A.normal1W = buffers[9];
   A.depth = buffers[10]; A.depthW = buffers[11];
   A.sampleL = buffers[12];
+  A.color2 = buffers[13];
   A.pathHash = pathHash; A.firstPrim = firstPrim;
   if (threads < 1) threads = 1;
   rowBegin = std::max(rowBegin, 0);
@@ -1237,6 +1286,63 @@ long long orc_render(void* p, const void* camp, int W, int H, int rowBegin, int 
   return rays.load();
 }
 
+// The pixel sample of later renders: jitter (enable, jx, jy) or the pixel filter of
+// DeviceCpu::initFilter (render/device_cpu.cpp:235-282; type 0 box, 1 triangle, 2 Gaussian,
+// 3 Blackman-Harris; width <= 0 and sigma <= 0 take the reference defaults), its lookup table
+// restated from image/pixel_filter.h:30-44, 77-139 and distribution1d.cpp:20-35.
+void orc_set_pixel_sample(void* p, int jitter, float jx, float jy, int type, float width, float sigma, int bias)
+{
+  Scene& s = *(Scene*)p;
+  s.jitter = jitter != 0;
+  s.jitterXY[0] = jx;
+  s.jitterXY[1] = jy;
+  s.filterCdf.clear();
+  s.filterWidth = 0.f;
+  if (type < 1 || type > 3) return;
+  if (!(width > 0.f)) width = type == 1 ? 2.f : 3.f;
+  if (type == 2 && !(sigma > 0.f)) sigma = width / 6.f;
+  const float radius = width / 2.f;
+  auto eval = [&](float x) -> float {
+    switch (type)
+    {
+      case 1: return std::max(radius - std::fabs(x), 0.f);
+      case 2:
+      {
+        const float a = -1.f / (2.f * sigma * sigma);
+        const float b = bias ? std::exp(a * radius * radius) : 0.f;
+        return std::max(std::exp(a * x * x) - b, 0.f);
+      }
+      default:
+      {
+        const float u = x / radius;
+        if (u < -1.f || u > 1.f) return 0.f;
+        const float t = (u + 1.f) * kPi;
+        return 0.35875f - 0.48829f * std::cos(t) + 0.14128f * std::cos(2.f * t) - 0.01168f * std::cos(3.f * t);
+      }
+    }
+  };
+  const int n = 2048, size = 2 * n + 1;
+  std::vector<float> lut(size);
+  for (int i = 0; i < size; ++i) lut[i] = eval((float(i - n) / float(n)) * radius);
+  s.filterCdf.resize(size + 1);
+  dist1dInit(s.filterCdf.data(), lut.data(), size);
+  s.filterWidth = width;
+}
+
+// The pixel-sample positions the renderer takes for sampler values s (n x 2) with the scene's
+// jitter / pixel filter: out n x 2.
+void orc_pixel_sample(void* p, int n, const float* sv, float* out)
+{
+  const Scene& s = *(const Scene*)p;
+  for (int i = 0; i < n; ++i)
+  {
+    float x = sv[2 * i], y = sv[2 * i + 1];
+    pixelSample(s, x, y);
+    out[2 * i] = x;
+    out[2 * i + 1] = y;
+  }
+}
+
 // FP64 gradient of the loss mean((I - target)^2) over pixels and channels with respect to
 // (albedo r, g, b, alpha) of every material, for the image of sample passes 0 .. spp-1; the
 // estimator of the HIP adjoint (detached sampling, suffix radiance times df/f at each vertex),
@@ -1248,8 +1354,8 @@ double orc_adjoint(void* p, const void* camp, int W, int H, int spp, uint32_t se
   const Camera& cam = *(const Camera*)camp;
   const int npix = W * H;
   std::vector<float> color(3 * size_t(npix), 0.f), weight(npix, 0.f);
-  float* bufs[13] = {color.data(), weight.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, nullptr, nullptr};
+  float* bufs[14] = {color.data(), weight.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr};
   orc_render(p, camp, W, H, 0, H, 0, spp, seed, maxDepth, rrDepth, minWeight, 3.402823466e38f, threads, bufs, nullptr, nullptr);
   const double norm = 1.0 / (3.0 * npix);
   std::vector<double> delta(3 * size_t(npix));

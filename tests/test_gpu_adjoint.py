@@ -135,17 +135,20 @@ def test_gradient_matches_fp64_oracle_remaining_lobes(dev):
     osc.set_lights(sc["lights"], sc["env"])
     oloss, ograd = osc.adjoint(sc["camera"], W, H, target, spp, max_depth=4)
     print("loss", loss, oloss)
-    # a shadow ray at a grazing angle can cross its own transparent surface just beyond the
-    # self-intersection offset on one side and not the other (one-ulp differences in the hit
-    # point): such paths keep their decisions but not their transmittance, so the loss over all
-    # pixels agrees to 1e-3 here rather than 1e-4
-    assert abs(loss - oloss) <= 1e-3 * oloss
+    out = results_dir()
+    if out:
+        scale_ = np.abs(ograd).max(axis=0)
+        with open(os.path.join(out, "adjoint_remaining_lobes.json"), "w") as f:
+            json.dump({"loss_rel_err": abs(loss - oloss) / oloss,
+                       "grad_err_over_scale": (np.abs(grad - ograd).max(axis=0) / scale_).tolist()}, f)
+    # camera rays, hit points and frames round alike on both sides (no FP contraction in either),
+    # so shadow rays that graze their own transparent surface decide alike too
+    assert abs(loss - oloss) <= 1e-4 * oloss
     assert np.all(grad[mats["kind"] == G.KIND_THIN_DIELECTRIC] == 0)
     scale = np.abs(ograd).max(axis=0)
     err = np.abs(grad - ograd)
     print("max abs err / max |g| per (r, g, b, alpha):", err.max(axis=0) / scale)
-    # the same paths move the gradient of the materials they shade: 5e-3 of the largest here
-    assert np.all(err <= 2e-3 * np.abs(ograd) + 5e-3 * scale)
+    assert np.all(err <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
 
 
 def test_conductor_reflectivity_matches_central_differences(dev):
