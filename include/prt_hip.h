@@ -251,6 +251,15 @@ int prt_query_pixel(prt_context* ctx, float px, float py, int32_t max_hits, prt_
 /* Same two queries on device buffers already in HBM, on the context stream (asynchronous). */
 int prt_query_rays_device(prt_context* ctx, int32_t n, const float* rays_device, int32_t* prim_device,
                           float* t_device, float* u_device, float* v_device);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops): prt_query_rays_device with the same results, whose traversal
+ * stack overflows into a fixed 16 MiB static device array (128 blocks x 256 lanes x 64 entries)
+ * instead of the context's buffer; it calls no hipMalloc / hipFree.  Trees of more than 24 levels
+ * are rejected (PRT_ERR_INVALID).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other contexts or streams) race; calls on one context are ordered by its
+ * stream.  Asynchronous on the context stream. */
+int prt_query_rays_device_noalloc(prt_context* ctx, int32_t n, const float* rays_device, int32_t* prim_device,
+                                  float* t_device, float* u_device, float* v_device);
 
 /* --- differentiable rendering ----------------------------------------------------------
  * Path-replay adjoint.  No counterpart in the reference (it renders forward only); added for

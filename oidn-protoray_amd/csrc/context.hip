@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -289,6 +290,39 @@ bool isPowerOfTwo(int x) { return (x & (x - 1)) == 0; }
 
 }  // namespace
 
+// ---- the static scratch of prt_query_rays_device_noalloc -----------------------------------
+// Traversal-stack overflow for a query that allocates nothing: kScratchBlocks persistent blocks of
+// kTraceBlock lanes, kScratchOverflow (node, distance) entries per lane beyond the LDS stack:
+// 128 x 256 x 2 x 64 ints = 16 MiB, held by every GPU the library runs on until the process ends.
+// In host code `scratch` names a host-side stand-in; the device address, per device, comes from
+// hipGetSymbolAddress at context creation (setupScratch) and only that address reaches kernels.
+constexpr int kScratchBlocks = 128;
+constexpr int kScratchOverflow = 64;
+constexpr size_t kScratchInts = size_t(kScratchBlocks) * kTraceBlock * 2 * kScratchOverflow;
+static __device__ __align__(256) int scratch[kScratchInts];
+constexpr int kMaxDevices = 64;
+static std::mutex gScratchMutex;
+static int* gScratchPtr[kMaxDevices] = {};
+
+// Records the device address of `scratch` on `device` (the current device) once.
+static int setupScratch(int device)
+{
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (device < 0 || device >= kMaxDevices) return fail(PRT_ERR_INVALID, "device ordinal beyond the scratch table");
+  if (gScratchPtr[device]) return PRT_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return fail(PRT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  gScratchPtr[device] = static_cast<int*>(p);
+  return PRT_OK;
+}
+
+static int* scratchOf(int device)
+{
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  return (device >= 0 && device < kMaxDevices) ? gScratchPtr[device] : nullptr;
+}
+
 struct prt_context
 {
   float boundsLo[3] = {1.f, 1.f, 1.f}, boundsHi[3] = {-1.f, -1.f, -1.f};  // scene box (empty: lo > hi)
@@ -385,6 +419,8 @@ int prt_context_create(int device_ordinal, prt_context** out)
   HIP_TRY(hipGetDeviceCount(&n));
   if (device_ordinal < 0 || device_ordinal >= n) return fail(PRT_ERR_INVALID, "invalid device ordinal");
   HIP_TRY(hipSetDevice(device_ordinal));
+  const int rcScratch = setupScratch(device_ordinal);
+  if (rcScratch != PRT_OK) return rcScratch;
   prt_context* c = new prt_context();
   c->device = device_ordinal;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1456,7 +1492,26 @@ int prt_query_rays_device(prt_context* c, int32_t n, const float* rays, int32_t*
   if (n < 0 || (n > 0 && (!rays || !prim))) return fail(PRT_ERR_INVALID, "invalid query arrays");
   if (n == 0) return PRT_OK;
   BvhView b{c->bvh.nodes, c->bvh.tris, c->bvh.root};
-  launchQuery(b, n, rays, prim, t, u, v, 0, c->spill.p, c->spillStride, c->launch.gridTrace, c->stream);
+  launchQuery(b, n, rays, prim, t, u, v, 0, c->spill.p, c->spillStride, PRT_MAX_STACK - PRT_LDS_STACK, c->launch.gridTrace,
+              c->stream);
+  HIP_TRY(hipGetLastError());
+  return PRT_OK;
+}
+
+int prt_query_rays_device_noalloc(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v)
+{
+  CTX_CHECK(c);
+  if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
+  if (n < 0 || (n > 0 && (!rays || !prim))) return fail(PRT_ERR_INVALID, "invalid query arrays");
+  // a wide node pushes at most three children: the stack of a tree of L levels holds < 3 L entries
+  if (3 * c->bvh.maxDepth >= PRT_LDS_STACK + kScratchOverflow)
+    return fail(PRT_ERR_INVALID, "BVH of " + std::to_string(c->bvh.maxDepth) + " levels exceeds the static traversal scratch (at most " +
+                                     std::to_string((PRT_LDS_STACK + kScratchOverflow - 1) / 3) + " levels)");
+  int* sp = scratchOf(c->device);
+  if (!sp) return fail(PRT_ERR_STATE, "static scratch not set up on this device");
+  if (n == 0) return PRT_OK;
+  BvhView b{c->bvh.nodes, c->bvh.tris, c->bvh.root};
+  launchQuery(b, n, rays, prim, t, u, v, 0, sp, kScratchBlocks * kTraceBlock, kScratchOverflow, kScratchBlocks, c->stream);
   HIP_TRY(hipGetLastError());
   return PRT_OK;
 }
@@ -1476,7 +1531,8 @@ static int queryHost(prt_context* c, int32_t n, const float* rays, int32_t* prim
   HIP_TRY(dv.alloc(n));
   HIP_TRY(hipMemcpy(dr.p, rays, sizeof(float) * 8 * n, hipMemcpyHostToDevice));
   BvhView b{c->bvh.nodes, c->bvh.tris, c->bvh.root};
-  launchQuery(b, n, dr.p, dp.p, dt.p, du.p, dv.p, anyHit, c->spill.p, c->spillStride, c->launch.gridTrace, c->stream);
+  launchQuery(b, n, dr.p, dp.p, dt.p, du.p, dv.p, anyHit, c->spill.p, c->spillStride, PRT_MAX_STACK - PRT_LDS_STACK,
+              c->launch.gridTrace, c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(prim, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost));

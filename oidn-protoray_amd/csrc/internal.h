@@ -129,7 +129,7 @@ void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg
 void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s);
 void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
-                 int* spill, int spillStride, int grid, hipStream_t s);
+                 int* spill, int spillStride, int spillCap, int grid, hipStream_t s);
 // Path-replay adjoint (adjoint.hip): partial is [blocks][nParams] (zeroed by the caller before the
 // first sample pass), grad [nParams]; delta is the per-pixel dLoss/dI (planar rgb).
 int adjointBlocks(int npix, int spillStride);

@@ -464,11 +464,11 @@ __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __res
 
 __global__ void __launch_bounds__(kTraceBlock) kQuery(BvhView bvh, int n, const float* __restrict__ rays, int* __restrict__ prim,
                                                       float* __restrict__ tOut, float* __restrict__ uOut, float* __restrict__ vOut,
-                                                      int anyHit, int* spill, int spillStride)
+                                                      int anyHit, int* spill, int spillStride, int spillCap)
 {
   __shared__ TraceLds<kTraceBlock> lds;
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  TraceStack st = makeStack(lds, spill, spillStride, gtid);
+  TraceStack st = makeStack(lds, spill, spillStride, gtid, spillCap);
   uint32_t nodes = 0, tris = 0;
   for (int i = gtid; i < n; i += gridDim.x * blockDim.x)
   {
@@ -598,9 +598,9 @@ void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg
   else kOccluded<false, false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
 }
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit, int* spill,
-                 int spillStride, int grid, hipStream_t s)
+                 int spillStride, int spillCap, int grid, hipStream_t s)
 {
-  kQuery<<<grid, kTraceBlock, 0, s>>>(bvh, n, rays, prim, t, u, v, anyHit, spill, spillStride);
+  kQuery<<<grid, kTraceBlock, 0, s>>>(bvh, n, rays, prim, t, u, v, anyHit, spill, spillStride, spillCap);
 }
 void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s)
 {

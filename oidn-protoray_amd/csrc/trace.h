@@ -68,25 +68,27 @@ PRT_DEV void cswap(float& ta, int& ca, float& tb, int& cb)
 
 // Per-lane traversal stack of (node, entry distance): the first PRT_LDS_STACK entries in LDS
 // (two lane-interleaved columns, so a wave's accesses hit 64 distinct banks), deeper entries in
-// the lane's overflow column in HBM (nodes in rows [0, PRT_MAX_STACK - PRT_LDS_STACK), the
-// distances in the rows after them).
+// the lane's overflow column in HBM (nodes in rows [0, cap), the distances in the rows after
+// them; cap = PRT_MAX_STACK - PRT_LDS_STACK except for the static-scratch query).
 struct TraceStack
 {
   int* lds;       // this lane's node column: lds[k * 64]
   float* ldsT;    // this lane's distance column: ldsT[k * 64]
   int* spill;     // this lane's overflow column: spill[k * stride]
   int stride;
+  int cap;        // overflow entries per lane: nodes in rows [0, cap), distances in [cap, 2 cap)
   int sp;
   uint32_t steps;  // traversal loop iterations of this lane (inner-node and leaf visits)
-  // The build rejects trees deeper than PRT_MAX_STACK, so the last check never drops an entry;
-  // it only keeps every store inside the allocated overflow area.
+  // The build rejects trees deeper than PRT_MAX_STACK (and the static-scratch query trees deeper
+  // than its capacity), so the last check never drops an entry; it only keeps every store inside
+  // the overflow area.
   PRT_DEV void push(int v, float t)
   {
     if (sp < PRT_LDS_STACK) { lds[sp * 64] = v; ldsT[sp * 64] = t; }
-    else if (sp < PRT_MAX_STACK)
+    else if (sp < PRT_LDS_STACK + cap)
     {
       spill[(sp - PRT_LDS_STACK) * stride] = v;
-      spill[(sp - PRT_LDS_STACK + (PRT_MAX_STACK - PRT_LDS_STACK)) * stride] = __float_as_int(t);
+      spill[(sp - PRT_LDS_STACK + cap) * stride] = __float_as_int(t);
     }
     else return;
     ++sp;
@@ -95,7 +97,7 @@ struct TraceStack
   {
     --sp;
     if (sp < PRT_LDS_STACK) { t = ldsT[sp * 64]; return lds[sp * 64]; }
-    t = __int_as_float(spill[(sp - PRT_LDS_STACK + (PRT_MAX_STACK - PRT_LDS_STACK)) * stride]);
+    t = __int_as_float(spill[(sp - PRT_LDS_STACK + cap) * stride]);
     return spill[(sp - PRT_LDS_STACK) * stride];
   }
 };
@@ -109,7 +111,8 @@ struct TraceLds
 };
 // This lane's stack over the block's LDS and the overflow area (gtid-th column of spill).
 template <int kThreads>
-PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int gtid)
+PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int gtid,
+                             int cap = PRT_MAX_STACK - PRT_LDS_STACK)
 {
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   TraceStack st;
@@ -117,6 +120,7 @@ PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int 
   st.ldsT = l.dist + wave * (PRT_LDS_STACK * 64) + lane;
   st.spill = spill + gtid;
   st.stride = stride;
+  st.cap = cap;
   st.sp = 0;
   st.steps = 0;
   return st;

@@ -83,6 +83,7 @@ SIGNATURES = {
     "prt_camera_matrices": [_vp, _i, _i, _vp, _vp, _vp],
     "prt_scene_bounds": [_vp, _vp, _vp],
     "prt_query_rays_device": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "prt_query_rays_device_noalloc": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_debug_enable": [_vp, _i],
     "prt_debug_get": [_vp, _vp, _vp, _vp],
     "prt_debug_sampler": [_vp, _i, _vp, _i, _vp, _vp, _vp],
@@ -293,6 +294,12 @@ class DeviceHip:
         t, u, v = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
         _check(_lib.prt_query_rays(self.h, n, _ptr(r), _ptr(prim), _ptr(t), _ptr(u), _ptr(v)))
         return prim, t, u, v
+
+    def queryRaysDevice(self, n, rays_ptr, prim_ptr, t_ptr, u_ptr, v_ptr, noalloc=False):
+        """Closest hits of n rays already in device memory (raw device addresses), asynchronous
+        on the context stream; noalloc selects prt_query_rays_device_noalloc."""
+        fn = _lib.prt_query_rays_device_noalloc if noalloc else _lib.prt_query_rays_device
+        _check(fn(self.h, int(n), rays_ptr, prim_ptr, t_ptr, u_ptr, v_ptr))
 
     def queryOcclusion(self, rays):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
