@@ -62,6 +62,11 @@ typedef struct prt_context prt_context;
  * non-zero emission makes the material emissive, its triangles form the scene's area light
  * (light/triangle_light.h, render/scene.cpp:83-116) and a path that hits one ends there.
  * Material ID 0 is reserved for the environment (render/scene.cpp:204-207). */
+#define PRT_KIND_DIFFUSE 0
+#define PRT_KIND_CONDUCTOR 1
+#define PRT_KIND_DIELECTRIC 2
+#define PRT_KIND_COATED 3
+#define PRT_KIND_THIN_DIELECTRIC 4
 typedef struct prt_material {
   int32_t kind;
   float r, g, b;
@@ -83,6 +88,9 @@ typedef struct prt_material {
  * light/image_light.h (kind 2, the lat-long image given to prt_scene_set_lights, scaled by rgb).
  * A sun/sky light (light/sun_sky_light.cpp) is passed as its baked pair: a distant sun plus
  * an image sky, in that order. */
+#define PRT_LIGHT_AMBIENT 0
+#define PRT_LIGHT_DISTANT 1
+#define PRT_LIGHT_IMAGE 2
 typedef struct prt_light {
   int32_t kind;
   float r, g, b;
@@ -223,6 +231,9 @@ int prt_get_accum(prt_context* ctx, float* sum, float* weight);
 /* Device::getFrameError (AccumBuffer3f::getError, accum_buffer.h:752-805) against the
  * checkpoint copy taken at the last power-of-two sample count. */
 int prt_frame_error(prt_context* ctx, float* error);
+/* getError(toneMapper) with a tone mapper: both images scaled by its exposure first
+ * (accum_buffer.h:787-792). */
+int prt_frame_error_exposure(prt_context* ctx, float exposure, float* error);
 
 /* --- queries (Renderer::queryRay / queryOcclusionRay, render/renderer.h:47-128) ------- */
 /* rays: n x [ox oy oz dx dy dz near far]; closest hit per ray: prim (-1 = miss), t, u, v. */
@@ -248,6 +259,11 @@ typedef struct prt_pixel_hit {
  * max_hits (<= 32) hits, stepping past each transparent one (T >= 1e-5) by 16 x its offset.
  * Needs the scene, BVH, camera and frame size. */
 int prt_query_pixel(prt_context* ctx, float px, float py, int32_t max_hits, prt_pixel_hit* hits, int32_t* count);
+/* Renderer::queryRay with the hit properties of RendererStream::queryRay (render/renderer_stream.h:
+ * 16-62) for one ray [ox oy oz dx dy dz near far]: up to max_hits (<= 32) hits along it, stepping
+ * past each transparent one as prt_query_pixel does (max_hits = 1: the closest hit only); depth
+ * is the distance along the ray.  *count = 0 for a miss. */
+int prt_query_ray_hits(prt_context* ctx, const float* ray, int32_t max_hits, prt_pixel_hit* hits, int32_t* count);
 /* Same two queries on device buffers already in HBM, on the context stream (asynchronous). */
 int prt_query_rays_device(prt_context* ctx, int32_t n, const float* rays_device, int32_t* prim_device,
                           float* t_device, float* u_device, float* v_device);
@@ -290,13 +306,20 @@ int prt_debug_adjoint_replay(prt_context* ctx, uint32_t sample_index, float* rad
  * 4096), a Sobol-Burley seed and a sample count; sample indices 0 .. spp-1 are rendered in
  * order, as prt_render would, and the mean is kept, plus the means after snapshot_spp[k]
  * samples (ascending, at most 4) and, when aov_mask has PRT_AOV_ALBEDO1 or PRT_AOV_NORMAL1,
- * the first-hit albedo and normal means.  prt_batch_tune times every (tiles per fetch, blocks
+ * the first-hit albedo and normal means, with PRT_AOV_DEPTH the first-hit depth mean (distance
+ * along the view axis, +inf where a sample missed).  prt_batch_tune times every (tiles per fetch, blocks
  * per CU) candidate on the batch with calibration_spp samples per frame and keeps the fastest.
  * prt_batch_render: *ms = device time of the launch, *samples = path samples rendered.
- * prt_batch_get copies one buffer of a frame as H x W x 3 floats (prt_blit layout). */
+ * prt_batch_get copies one buffer of a frame as H x W x 3 floats (prt_blit layout; the depth as
+ * H x W).  A scene
+ * context changed after its frames were prepared (geometry, materials, lights, BVH or renderer
+ * parameters) makes the next prt_batch_render / prt_batch_tune rebuild every frame's kernel
+ * arguments and the captured graph, so they never read the context's released buffers; the
+ * outputs of earlier renders are released with them. */
 #define PRT_BATCH_COLOR 0
 #define PRT_BATCH_ALBEDO1 1
 #define PRT_BATCH_NORMAL1 2
+#define PRT_BATCH_DEPTH 3
 #define PRT_BATCH_SNAPSHOT 16 /* + k: the mean after snapshot_spp[k] samples */
 typedef struct prt_batch prt_batch;
 int prt_batch_create(int32_t device_ordinal, prt_batch** out);
@@ -311,8 +334,9 @@ int prt_batch_get(prt_batch* batch, int32_t frame, int32_t buffer, float* dst);
 
 /* --- learned denoiser ---------------------------------------------------------------------
  * The Denoiser plug-in (image/denoiser.h:10-41): init with the colour / albedo / normal /
- * output images (H x W x 3 floats each), execute (returns the denoising time in seconds),
- * version.  The filter is a U-Net (4 levels, 3x3 convolutions, 32 / 48 / 64 / 96 / 128 channels,
+ * output images (H x W x 3 floats each; albedo and normal may be NULL, as Params allows, but a
+ * normal needs an albedo: missing images enter the network as zero channels, so set the weights
+ * trained for that input set), execute (returns the denoising time in seconds), version.  The filter is a U-Net (4 levels, 3x3 convolutions, 32 / 48 / 64 / 96 / 128 channels,
  * ReLU, max pooling, nearest upsampling with skips) on BF16 matrix cores with FP32 accumulation.
  * Weights: param_count FP32 values (per layer: weights [cout][cin][3][3] then bias [cout]; the
  * layers enc0..enc4, dec3..dec0, out).  Training (no counterpart in the reference, which calls

@@ -48,13 +48,16 @@ __global__ void __launch_bounds__(kBatchBlock) kBatch(const BatchFrameDev* __res
       const int x = (lt % f.tilesX) * 8 + (lane & 7), y = (lt / f.tilesX) * 8 + (lane >> 3);
       const int pix = y * f.W + x;
       f3 sum(0.f), alb(0.f), nrm(0.f);
+      float dep = 0.f;
       int k = 0;
       FirstHit fh;
+      const bool wantFirst = f.albedo1 || f.depth;
       for (int s = 0; s < f.spp; ++s)
       {
-        f3 L = tracePathMega(a, st, x, y, uint32_t(s), f.albedo1 ? &fh : nullptr, NoVertexHook(), nodes, tris);
+        f3 L = tracePathMega(a, st, x, y, uint32_t(s), wantFirst ? &fh : nullptr, NoVertexHook(), nodes, tris);
         sum = sum + clampL(L, a.maxRadiance);
         if (f.albedo1) { alb = alb + fh.albedo; nrm = nrm + fh.normal; }
+        if (f.depth) dep += fh.depth;
         if (k < f.nSnap && s + 1 == f.snapSpp[k])
         {
           const float r = 1.0f / float(s + 1);
@@ -70,6 +73,7 @@ __global__ void __launch_bounds__(kBatchBlock) kBatch(const BatchFrameDev* __res
         f.albedo1[pix] = alb.x * r; f.albedo1[f.npix + pix] = alb.y * r; f.albedo1[2 * f.npix + pix] = alb.z * r;
         f.normal1[pix] = nrm.x * r; f.normal1[f.npix + pix] = nrm.y * r; f.normal1[2 * f.npix + pix] = nrm.z * r;
       }
+      if (f.depth) f.depth[pix] = dep * r;
       done += f.spp;
     }
   }

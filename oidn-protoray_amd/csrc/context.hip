@@ -378,7 +378,10 @@ struct prt_context
   DevBuf<uint32_t> dbgHash;
   DevBuf<int> dbgFirst;
   bool debug = false;
-  bool countSteps = false;  // traversal kernels built with the SIMD-utilisation counters
+  bool countSteps = false;
+  // bumped by every call that changes the scene, lights, BVH or renderer parameters (the device
+  // pointers and values a batch frame copied into its kernel arguments)
+  uint64_t generation = 0;  // traversal kernels built with the SIMD-utilisation counters
   // path state
   DevBuf<float4> rayO, rayD, hit, TP, Lr, shO, shD, dirL;
   DevBuf<int> flags, q0, q1, shadowQ, counters, spill;
@@ -504,6 +507,7 @@ int prt_scene_set_geometry(prt_context* c, int32_t nverts, const float* position
                            const int32_t* indices, const int32_t* tri_material)
 {
   CTX_CHECK(c);
+  ++c->generation;
   if (nverts < 0 || ntris < 0 || (nverts > 0 && !positions) || (ntris > 0 && (!indices || !tri_material)))
     return fail(PRT_ERR_INVALID, "invalid geometry arrays");
   if (ntris >= (1 << 28)) return fail(PRT_ERR_INVALID, "too many triangles (limit 2^28)");
@@ -553,6 +557,7 @@ int prt_scene_set_geometry(prt_context* c, int32_t nverts, const float* position
 int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
 {
   CTX_CHECK(c);
+  ++c->generation;
   if (n < 1 || !m) return fail(PRT_ERR_INVALID, "need at least the reserved material 0");
   c->materials.assign(m, m + n);
   std::vector<prt_material_constants> k(n);
@@ -760,6 +765,7 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
                          int32_t lut_max)
 {
   CTX_CHECK(c);
+  ++c->generation;
   if (n < 0 || n > PRT_MAX_LIGHTS || (n > 0 && !lights)) return fail(PRT_ERR_INVALID, "0..8 environment lights");
   if (c->materials.empty()) return fail(PRT_ERR_STATE, "materials must be set before the lights");
   {
@@ -903,6 +909,7 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
 int prt_bvh_build(prt_context* c, double* build_ms)
 {
   CTX_CHECK(c);
+  ++c->generation;
   if (!c->pos.p && c->ntris > 0) return fail(PRT_ERR_STATE, "no geometry");
   std::string err;
   hipError_t e = buildBvh(c->pos.p, c->idx.p, c->ntris, c->bvh, c->stream, err);
@@ -915,6 +922,7 @@ int prt_bvh_build(prt_context* c, double* build_ms)
 int prt_renderer_init(prt_context* c, const prt_renderer_params* p)
 {
   CTX_CHECK(c);
+  ++c->generation;
   if (p)
   {
     if (p->maxDepth < 0 || p->maxDepth > 1000 || p->rrDepth < 0) return fail(PRT_ERR_INVALID, "invalid depth parameters");
@@ -1372,7 +1380,9 @@ int prt_get_accum(prt_context* c, float* sum, float* weight)
   return PRT_OK;
 }
 
-int prt_frame_error(prt_context* c, float* error)
+int prt_frame_error(prt_context* c, float* error) { return prt_frame_error_exposure(c, 1.f, error); }
+
+int prt_frame_error_exposure(prt_context* c, float exposure, float* error)
 {
   CTX_CHECK(c);
   if (!error) return fail(PRT_ERR_INVALID, "error is null");
@@ -1389,6 +1399,8 @@ int prt_frame_error(prt_context* c, float* error)
     float r = w[i] == 0.f ? 3.402823466e38f : 1.0f / w[i], rc = wc[i] == 0.f ? 3.402823466e38f : 1.0f / wc[i];
     float I[3], B[3];
     for (int k = 0; k < 3; ++k) { I[k] = s[k * n + i] * r; B[k] = sc[k * n + i] * rc; }
+    if (exposure != 1.f)  // the tone mapper's exposure (accum_buffer.h:787-792)
+      for (int k = 0; k < 3; ++k) { I[k] *= exposure; B[k] *= exposure; }
     float norm = I[0] + I[1] + I[2];
     if (norm < 1.f) norm = std::sqrt(norm);
     err += (std::fabs(I[0] - B[0]) + std::fabs(I[1] - B[1]) + std::fabs(I[2] - B[2])) / (norm + 1e-4f);
@@ -1582,26 +1594,46 @@ int prt_scene_bounds(prt_context* c, float lo[3], float hi[3])
   return PRT_OK;
 }
 
+static int queryHits(prt_context* c, float px, float py, const float* ray8, int32_t max_hits, prt_pixel_hit* hits, int32_t* count)
+{
+  RenderArgs a = makeArgs(c);
+  DevBuf<RenderArgs> da;
+  DevBuf<prt_pixel_hit> dh;
+  DevBuf<int> dn;
+  DevBuf<float> dr;
+  HIP_TRY(da.alloc(1));
+  HIP_TRY(dh.alloc(max_hits));
+  HIP_TRY(dn.alloc(1));
+  HIP_TRY(hipMemcpy(da.p, &a, sizeof(a), hipMemcpyHostToDevice));
+  if (ray8)
+  {
+    HIP_TRY(dr.alloc(8));
+    HIP_TRY(hipMemcpy(dr.p, ray8, sizeof(float) * 8, hipMemcpyHostToDevice));
+  }
+  launchQueryPixel(da.p, px, py, max_hits, dh.p, dn.p, dr.p, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(count, dn.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (*count > 0) HIP_TRY(hipMemcpy(hits, dh.p, sizeof(prt_pixel_hit) * size_t(*count), hipMemcpyDeviceToHost));
+  return PRT_OK;
+}
+
 int prt_query_pixel(prt_context* c, float px, float py, int32_t max_hits, prt_pixel_hit* hits, int32_t* count)
 {
   CTX_CHECK(c);
   if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
   if (c->W <= 0 || c->H <= 0) return fail(PRT_ERR_STATE, "no frame: the pixel query needs the image size");
   if (max_hits < 1 || max_hits > 32 || !hits || !count) return fail(PRT_ERR_INVALID, "invalid pixel query arguments");
-  RenderArgs a = makeArgs(c);
-  DevBuf<RenderArgs> da;
-  DevBuf<prt_pixel_hit> dh;
-  DevBuf<int> dn;
-  HIP_TRY(da.alloc(1));
-  HIP_TRY(dh.alloc(max_hits));
-  HIP_TRY(dn.alloc(1));
-  HIP_TRY(hipMemcpy(da.p, &a, sizeof(a), hipMemcpyHostToDevice));
-  launchQueryPixel(da.p, px, py, max_hits, dh.p, dn.p, c->stream);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(count, dn.p, sizeof(int), hipMemcpyDeviceToHost));
-  if (*count > 0) HIP_TRY(hipMemcpy(hits, dh.p, sizeof(prt_pixel_hit) * size_t(*count), hipMemcpyDeviceToHost));
-  return PRT_OK;
+  return queryHits(c, px, py, nullptr, max_hits, hits, count);
+}
+
+int prt_query_ray_hits(prt_context* c, const float* ray, int32_t max_hits, prt_pixel_hit* hits, int32_t* count)
+{
+  CTX_CHECK(c);
+  if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
+  if (!c->mats.p) return fail(PRT_ERR_STATE, "materials not set");
+  if (!ray || max_hits < 1 || max_hits > 32 || !hits || !count) return fail(PRT_ERR_INVALID, "invalid ray query arguments");
+  return queryHits(c, 0.f, 0.f, ray, max_hits, hits, count);
 }
 
 int prt_debug_get(prt_context* c, float* sample_l, uint32_t* path_hash, int32_t* first_prim)
@@ -1707,6 +1739,7 @@ struct prt_batch
     int W, H, spp, nSnap;
     uint32_t seed, aov;
     int snap[PRT_BATCH_MAX_SNAPSHOTS];
+    uint64_t generation;  // the scene context's generation when the frame's arguments were made
   };
   std::vector<Frame> frames;
   std::vector<size_t> outOffset;  // per frame: first float of its outputs in out
@@ -1736,9 +1769,22 @@ void dropGraph(prt_batch* b)
 }
 
 // Outputs, kernel arguments and frame descriptors of every frame on the device.
+int sceneReady(const prt_context* scene)
+{
+  if (!scene->bvhReady || !scene->lightsReady || !scene->mats.p || !scene->rendererReady)
+    return fail(PRT_ERR_STATE, "scene context needs geometry, BVH, materials, lights and renderer parameters");
+  if (scene->maxMaterialId >= int(scene->materials.size())) return fail(PRT_ERR_INVALID, "triangle material ID beyond the material table");
+  return PRT_OK;
+}
+
 int batchPrepare(prt_batch* b)
 {
   if (b->frames.empty()) return fail(PRT_ERR_STATE, "the batch has no frames");
+  for (const auto& f : b->frames)  // a changed scene context must be complete again
+  {
+    const int rc = sceneReady(f.scene);
+    if (rc != PRT_OK) return rc;
+  }
   size_t total = 0;
   int tiles = 0;
   b->outOffset.clear();
@@ -1746,7 +1792,8 @@ int batchPrepare(prt_batch* b)
   {
     const size_t n = size_t(f.W) * f.H;
     b->outOffset.push_back(total);
-    total += 3 * n * (1 + f.nSnap + ((f.aov & (PRT_AOV_ALBEDO1 | PRT_AOV_NORMAL1)) ? 2 : 0));
+    total += 3 * n * (1 + f.nSnap + ((f.aov & (PRT_AOV_ALBEDO1 | PRT_AOV_NORMAL1)) ? 2 : 0)) +
+             ((f.aov & PRT_AOV_DEPTH) ? n : 0);
     tiles += (f.W / 8) * (f.H / 8);
   }
   HIP_TRY(b->out.alloc(total));
@@ -1757,11 +1804,16 @@ int batchPrepare(prt_batch* b)
   int tileBegin = 0;
   for (size_t i = 0; i < b->frames.size(); ++i)
   {
-    const auto& f = b->frames[i];
+    auto& f = b->frames[i];
     RenderArgs a = makeArgs(f.scene);
+    f.generation = f.scene->generation;
     setCameraArgs(a, f.cam, f.W, f.H);
     a.seed = f.seed;
     std::memset(&a.fb, 0, sizeof(a.fb));
+    // the jitter and pixel filter belong to the scene context's own frame, not to batch frames
+    a.jitter = 0;
+    a.filterCdf = nullptr;
+    a.filterSize = 0;
     args[i] = a;
     BatchFrameDev& d = b->desc[i];
     const size_t n = size_t(f.W) * f.H;
@@ -1772,6 +1824,7 @@ int batchPrepare(prt_batch* b)
     const bool aovs = (f.aov & (PRT_AOV_ALBEDO1 | PRT_AOV_NORMAL1)) != 0;
     d.albedo1 = aovs ? base + 3 * n * (1 + f.nSnap) : nullptr;
     d.normal1 = aovs ? base + 3 * n * (2 + f.nSnap) : nullptr;
+    d.depth = (f.aov & PRT_AOV_DEPTH) ? base + 3 * n * (1 + f.nSnap + (aovs ? 2 : 0)) : nullptr;
     d.W = f.W; d.H = f.H; d.npix = int(n); d.spp = f.spp; d.nSnap = f.nSnap;
     for (int k = 0; k < PRT_BATCH_MAX_SNAPSHOTS; ++k) d.snapSpp[k] = k < f.nSnap ? f.snap[k] : 0;
     d.tileBegin = tileBegin;
@@ -1788,6 +1841,15 @@ int batchPrepare(prt_batch* b)
   b->prepared = true;
   dropGraph(b);
   return PRT_OK;
+}
+
+// A frame's scene context changed since its kernel arguments were made (its device buffers may
+// have been reallocated): the arguments and the captured graph are made again.
+bool batchStale(const prt_batch* b)
+{
+  for (const auto& f : b->frames)
+    if (f.generation != f.scene->generation) return true;
+  return false;
 }
 
 void batchLaunch(prt_batch* b)
@@ -1838,9 +1900,8 @@ int prt_batch_add_frame(prt_batch* b, prt_context* scene, const prt_camera* cam,
 {
   if (!b || !scene || !cam) return fail(PRT_ERR_INVALID, "null argument");
   if (scene->device != b->device) return fail(PRT_ERR_INVALID, "scene context on another device");
-  if (!scene->bvhReady || !scene->lightsReady || !scene->mats.p || !scene->rendererReady)
-    return fail(PRT_ERR_STATE, "scene context needs geometry, BVH, materials, lights and renderer parameters");
-  if (scene->maxMaterialId >= int(scene->materials.size())) return fail(PRT_ERR_INVALID, "triangle material ID beyond the material table");
+  const int rcScene = sceneReady(scene);
+  if (rcScene != PRT_OK) return rcScene;
   if (width <= 0 || height <= 0 || width % 8 != 0 || height % 8 != 0 || width > 4096 || height > 4096)
     return fail(PRT_ERR_INVALID, "batch frame size must be a multiple of 8 and at most 4096");
   if (!(cam->fov > 0.f && cam->fov < 3.14159265f)) return fail(PRT_ERR_INVALID, "invalid fov");
@@ -1870,7 +1931,7 @@ int prt_batch_tune(prt_batch* b, int32_t calibration_spp, int32_t* tiles_per_fet
   if (!b) return fail(PRT_ERR_INVALID, "batch is null");
   if (calibration_spp < 1) return fail(PRT_ERR_INVALID, "calibration_spp must be >= 1");
   HIP_TRY(hipSetDevice(b->device));
-  if (!b->prepared)
+  if (!b->prepared || batchStale(b))
   {
     int rc = batchPrepare(b);
     if (rc != PRT_OK) return rc;
@@ -1911,7 +1972,7 @@ int prt_batch_render(prt_batch* b, double* ms, double* samples)
 {
   if (!b) return fail(PRT_ERR_INVALID, "batch is null");
   HIP_TRY(hipSetDevice(b->device));
-  if (!b->prepared)
+  if (!b->prepared || batchStale(b))
   {
     int rc = batchPrepare(b);
     if (rc != PRT_OK) return rc;
@@ -1950,6 +2011,12 @@ int prt_batch_get(prt_batch* b, int32_t frame, int32_t buffer, float* dst)
   else if (buffer >= PRT_BATCH_SNAPSHOT && buffer < PRT_BATCH_SNAPSHOT + d.nSnap) src = d.snap + size_t(buffer - PRT_BATCH_SNAPSHOT) * 3 * d.npix;
   else if (buffer == PRT_BATCH_ALBEDO1) src = d.albedo1;
   else if (buffer == PRT_BATCH_NORMAL1) src = d.normal1;
+  else if (buffer == PRT_BATCH_DEPTH)
+  {
+    if (!d.depth) return fail(PRT_ERR_INVALID, "buffer not produced for this frame");
+    HIP_TRY(hipMemcpy(dst, d.depth, sizeof(float) * size_t(d.npix), hipMemcpyDeviceToHost));
+    return PRT_OK;
+  }
   if (!src) return fail(PRT_ERR_INVALID, "buffer not produced for this frame");
   const size_t n = size_t(d.npix);
   std::vector<float> h(3 * n);

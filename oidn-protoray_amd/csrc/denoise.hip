@@ -1128,7 +1128,10 @@ int prt_denoiser_init(prt_denoiser* dn, const prt_denoiser_params* p)
 {
   if (!dn || !p) return dfail(PRT_ERR_INVALID, "null argument");
   if (!p->color || !p->output || p->width < 1 || p->height < 1) return dfail(PRT_ERR_INVALID, "colour, output and size are required");
-  if (!p->albedo || !p->normal) return dfail(PRT_ERR_INVALID, "this filter needs the albedo and normal images");
+  // Denoiser::Params makes the auxiliary images optional (image/denoiser.h:14-29); as with OIDN a
+  // normal image needs an albedo image (image/oidn_denoiser.cpp).  Missing ones enter the network
+  // as zero channels, the input the weights for that input set were trained on.
+  if (p->normal && !p->albedo) return dfail(PRT_ERR_INVALID, "a normal image needs an albedo image");
   dn->params = *p;
   dn->inited = true;
   return PRT_OK;
@@ -1150,8 +1153,8 @@ int prt_denoiser_execute(prt_denoiser* dn, double* seconds)
       for (int c = 0; c < 3; ++c)
       {
         in9[d * 9 + c] = p.color[s * 3 + c];
-        in9[d * 9 + 3 + c] = p.albedo[s * 3 + c];
-        in9[d * 9 + 6 + c] = p.normal[s * 3 + c];
+        in9[d * 9 + 3 + c] = p.albedo ? p.albedo[s * 3 + c] : 0.f;
+        in9[d * 9 + 6 + c] = p.normal ? p.normal[s * 3 + c] : 0.f;
       }
     }
   int rc = ensureActs(dn, 1, Hp, Wp);

@@ -105,6 +105,7 @@ struct BatchFrameDev
   float* snap;     // [nSnap][3][npix]: means after snapSpp[k] samples
   float* albedo1;  // first-hit albedo / normal means (both null when not requested)
   float* normal1;
+  float* depth;    // first-hit view-axis depth mean (1 x npix, null when not requested)
   int W, H, npix, spp, nSnap;
   int snapSpp[PRT_BATCH_MAX_SNAPSHOTS];
   int tileBegin, tilesX;
@@ -127,7 +128,8 @@ void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool co
 void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, bool ext, hipStream_t s);
 void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, bool filter, bool count, hipStream_t s);
 void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s);
-void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, hipStream_t s);
+void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, const float* ray8,
+                      hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
                  int* spill, int spillStride, int spillCap, int grid, hipStream_t s);
 // Path-replay adjoint (adjoint.hip): partial is [blocks][nParams] (zeroed by the caller before the

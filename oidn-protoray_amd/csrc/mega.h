@@ -9,11 +9,13 @@
 
 namespace prt {
 
-// First-hit AOVs of a sample (path_renderer_stream.h:301-330, :475-519, :739-819): albedo and
-// shading normal of the first shaded vertex; zero for a miss; 1 and -direction for an emitter.
+// First-hit AOVs of a sample (path_renderer_stream.h:301-330, :475-519, :739-819): albedo,
+// shading normal and view-axis depth of the first shaded vertex; zero (and an infinite depth) for
+// a miss; 1 and -direction for an emitter.
 struct FirstHit
 {
   f3 albedo, normal;
+  float depth;  // distance of the first hit along the view axis (+inf for a miss)
 };
 
 // Everything a vertex hook may read about a shaded vertex, after its next-event estimate and
@@ -53,7 +55,11 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
   float lastPdf = 1.f;
   int lastType = BsdfType::SpecularReflection, mediumId = 0;
   bool aov1 = false, aov2 = false;
-  if (fh) fh->albedo = fh->normal = f3(0.f);
+  if (fh)
+  {
+    fh->albedo = fh->normal = f3(0.f);
+    fh->depth = __int_as_float(0x7f800000);
+  }
   for (int depth = 0;; ++depth)
   {
     const int sampleDim = 4 + 7 * depth;
@@ -76,7 +82,12 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
       f3 Le = evalArea(a, d, th, prim, lightPdf);
       float w = ((lastType & BsdfType::Smooth) == 0) ? 1.f : misWeight(lastPdf, lightPdf);
       L = L + T * w * Le;
-      if (fh && !aov1) { fh->albedo = f3(1.f); fh->normal = normalize(-d); }
+      if (fh && !aov1)
+      {
+        fh->albedo = f3(1.f);
+        fh->normal = normalize(-d);
+        fh->depth = dot((o + d * th) - ld3(a.camPos), -ld3(a.camN));
+      }
       break;
     }
     if (depth >= a.maxDepth) break;
@@ -136,7 +147,12 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     if (mAov1)
     {
       aov1 = true;
-      if (fh) { fh->albedo = bsdfAlbedo(b); fh->normal = bsdfNormal(b); }
+      if (fh)
+      {
+        fh->albedo = bsdfAlbedo(b);
+        fh->normal = bsdfNormal(b);
+        fh->depth = dot(ctx.p - ld3(a.camPos), -ld3(a.camN));
+      }
     }
     mAov2 = mAov2 && (extType & BsdfType::Delta) == 0;
     if (mAov2) aov2 = true;

@@ -504,9 +504,12 @@ __global__ void __launch_bounds__(kTraceBlock) kQueryOcclusionFilter(const Rende
   }
 }
 
-// Renderer::queryPixel (render/renderer.h:82-115): one lane walks the hits of the pixel's ray.
+// Renderer::queryPixel (render/renderer.h:82-115): one lane walks the hits of the pixel's ray;
+// with ray8 (origin, direction, near, far) the hits of that ray instead (RendererStream::queryRay,
+// render/renderer_stream.h:16-62), their depth being the distance along the ray.
 __global__ void __launch_bounds__(64) kQueryPixel(const RenderArgs* __restrict__ ap, float px, float py, int maxHits,
-                                                  prt_pixel_hit* __restrict__ out, int* __restrict__ count)
+                                                  prt_pixel_hit* __restrict__ out, int* __restrict__ count,
+                                                  const float* __restrict__ ray8)
 {
   const RenderArgs& a = *ap;
   __shared__ TraceLds<64> lds;
@@ -520,17 +523,28 @@ __global__ void __launch_bounds__(64) kQueryPixel(const RenderArgs* __restrict__
     d = d * (1.0f / len);
     tnear = (a.pinhole ? a.nearClip : a.nearClipScale) * len;
   };
-  f3 dc, d;
-  float tc, tnear;
-  ray(0.5f, 0.5f, dc, tc);
-  ray(fminf(fmaxf(px, 0.f), 1.f), fminf(fmaxf(py, 0.f), 1.f), d, tnear);
-  const f3 o = ld3(a.camPos);
+  f3 dc, d, o;
+  float tc, tnear, tfar = kPosMax;
+  if (ray8)
+  {
+    o = f3(ray8[0], ray8[1], ray8[2]);
+    d = f3(ray8[3], ray8[4], ray8[5]);
+    dc = d;
+    tnear = ray8[6];
+    tfar = ray8[7];
+  }
+  else
+  {
+    ray(0.5f, 0.5f, dc, tc);
+    ray(fminf(fmaxf(px, 0.f), 1.f), fminf(fmaxf(py, 0.f), 1.f), d, tnear);
+    o = ld3(a.camPos);
+  }
   uint32_t nodes = 0, tris = 0;
   int n = 0;
   for (; n < maxHits; ++n)
   {
     float t, u = 0.f, v = 0.f;
-    const int prim = traceRay<false>(a.bvh, o, d, tnear, kPosMax, t, u, v, st, nodes, tris);
+    const int prim = traceRay<false>(a.bvh, o, d, tnear, tfar, t, u, v, st, nodes, tris);
     if (prim < 0) break;
     const Ctx c = postIntersect(a, o, d, t, prim, u, v);
     const int matId = a.triMat[prim];
@@ -606,9 +620,10 @@ void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, i
 {
   kQueryOcclusionFilter<<<grid, kTraceBlock, 0, s>>>(a, n, rays, occ);
 }
-void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, hipStream_t s)
+void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, const float* ray8,
+                      hipStream_t s)
 {
-  kQueryPixel<<<1, 64, 0, s>>>(a, px, py, maxHits, out, count);
+  kQueryPixel<<<1, 64, 0, s>>>(a, px, py, maxHits, out, count, ray8);
 }
 void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s)
 {

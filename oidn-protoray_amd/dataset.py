@@ -7,6 +7,7 @@ with the reference's keys.  Names follow the OIDN training layout:
     <name>_0001.hdr.exr, <name>_0004.hdr.exr   noisy colour after 1 and 4 samples
     <name>_ref.hdr.exr                          reference colour
     <name>.alb.exr, <name>.nrm.exr              first-hit albedo and normal
+    <name>.z.exr                                first-hit depth along the view axis (FP32 Y)
     <name>.json                                 metadata
 
 The OpenEXR encoder below is written from the file-format specification (single-part scanline
@@ -142,10 +143,29 @@ def read_exr(path):
     return out, [c for c, _ in chans]
 
 
-def camera_metadata(camera, width, height, spp, max_depth, exposure=1.0):
-    """The reference's per-image JSON keys (main/render/render_window.cpp:1406-1444)."""
+def scene_scale(positions):
+    """RenderWindow's automatic sceneScale (main/render/render_window.cpp:153-154): a tenth of the
+    smallest extent of the scene's bounding box."""
+    p = np.asarray(positions, np.float64).reshape(-1, 3)
+    return float((p.max(axis=0) - p.min(axis=0)).min() * 0.1)
+
+
+def camera_metadata(camera, width, height, spp, max_depth, exposure=1.0, scene_scale_=None, jitter=None,
+                    pixel_filter="box", pixel_filter_width=0.0):
+    """The reference's per-image JSON keys (main/render/render_window.cpp:1406-1444): the pixel
+    filter (or the jitter offset), the sampler, the camera frame and lens, worldToViewMatrix and
+    viewToClipMatrix (Device::getWorldToViewD3D / getViewToClipD3D, rows as toJSON(Mat4f) writes
+    them), sceneScale, exposure, path depth and sample count."""
+    import protoray_hip as P
     U, V, N = (np.asarray(camera[k], float) for k in ("U", "V", "N"))
-    return {
+    world_to_view, view_to_clip, _ = P.camera_matrices(camera, width, height)
+    meta = {}
+    if jitter is not None:
+        meta["jitterOffset"] = [0.5 - float(jitter[0]), 0.5 - float(jitter[1])]
+    else:
+        meta["pixelFilter"] = pixel_filter
+        meta["pixelFilterWidth"] = float(pixel_filter_width)
+    meta.update({
         "sampler": "sobolBurley",
         "cameraPosition": [float(x) for x in camera["pos"]],
         "cameraForward": [float(-x) for x in N],
@@ -156,14 +176,18 @@ def camera_metadata(camera, width, height, spp, max_depth, exposure=1.0):
         "cameraLensRadius": float(camera["lensRadius"]),
         "cameraFocalDistance": float(camera["focalDistance"]),
         "cameraNear": float(camera["nearClip"]),
-        "exposure": exposure,
-        "maxPathDepth": max_depth,
-        "samplesPerPixel": spp,
-    }
+        "worldToViewMatrix": [[float(x) for x in row] for row in world_to_view],
+        "viewToClipMatrix": [[float(x) for x in row] for row in view_to_clip],
+    })
+    if scene_scale_ is not None:
+        meta["sceneScale"] = scene_scale_
+    meta.update({"exposure": exposure, "maxPathDepth": max_depth, "samplesPerPixel": spp})
+    return meta
 
 
-def write_frame(outdir, name, noisy, ref, albedo, normal, meta):
-    """noisy: {spp: H x W x 3}; writes the files listed in the module docstring."""
+def write_frame(outdir, name, noisy, ref, albedo, normal, meta, depth=None):
+    """noisy: {spp: H x W x 3}; depth: H x W or None; writes the files listed in the module
+    docstring (the depth as FP32, as the reference saves it: saveImageSafe(..., 1, 32))."""
     os.makedirs(outdir, exist_ok=True)
     paths = []
     for spp, img in sorted(noisy.items()):
@@ -173,6 +197,9 @@ def write_frame(outdir, name, noisy, ref, albedo, normal, meta):
         if img is not None:
             paths.append(os.path.join(outdir, name + suffix + ".exr"))
             write_exr(paths[-1], img)
+    if depth is not None:
+        paths.append(os.path.join(outdir, name + ".z.exr"))
+        write_exr(paths[-1], depth, half=False)
     paths.append(os.path.join(outdir, name + ".json"))
     with open(paths[-1], "w") as f:
         json.dump(meta, f, indent=2)
@@ -245,7 +272,8 @@ def render_dataset(outdir=None, ref_spp=4096, n_frames=64, n_scenes=8, max_tris=
     batch = P.BatchHip(device)
     frames = dataset_frames(n_frames, n_scenes)
     for si, cam, W, H, seed in frames:
-        batch.addFrame(devs[si], cam, W, H, seed, ref_spp, snapshots=(1, 4), aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+        batch.addFrame(devs[si], cam, W, H, seed, ref_spp, snapshots=(1, 4),
+                       aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1 | P.AOV_DEPTH)
     tuned = batch.tune(1) if tune else None
     r = batch.render()
     summary = {"frames": n_frames, "scenes": n_scenes, "triangles": [int(sc["indices"].shape[0]) for sc in scenes],
@@ -253,10 +281,12 @@ def render_dataset(outdir=None, ref_spp=4096, n_frames=64, n_scenes=8, max_tris=
                "samples": r["samples"], "msamples_per_s": r["samples"] / (r["ms"] * 1e3), "tuned": tuned}
     if outdir:
         for i, (si, cam, W, H, seed) in enumerate(frames):
-            meta = camera_metadata(cam, W, H, ref_spp, scenes[si]["max_depth"])
+            meta = camera_metadata(cam, W, H, ref_spp, scenes[si]["max_depth"],
+                                   scene_scale_=scene_scale(scenes[si]["positions"]))
             meta.update({"scene": si, "seed": seed})
             write_frame(outdir, "frame%02d" % i, {1: batch.get(i, P.BATCH_SNAPSHOT), 4: batch.get(i, P.BATCH_SNAPSHOT + 1)},
-                        batch.get(i, P.BATCH_COLOR), batch.get(i, P.BATCH_ALBEDO1), batch.get(i, P.BATCH_NORMAL1), meta)
+                        batch.get(i, P.BATCH_COLOR), batch.get(i, P.BATCH_ALBEDO1), batch.get(i, P.BATCH_NORMAL1), meta,
+                        depth=batch.get(i, P.BATCH_DEPTH))
     batch.close()
     for d in devs:
         d.close()

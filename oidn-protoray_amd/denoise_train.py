@@ -65,7 +65,20 @@ def training_set(n_frames=24, size=(384, 384), ref_spp=4096, n_scenes=4, max_tri
     return inputs, targets, r
 
 
-def train(dn, inputs, targets, steps=1500, batch=16, crop=256, lr=1e-3, seed=5, log=print):
+# Input sets of the filter (Denoiser::Params with and without the auxiliary images): the channels of
+# the 9-channel input each one keeps, and the weights file trained for it.
+INPUT_SETS = {"full": 9, "color_albedo": 6, "color": 3}
+
+
+def weights_path(inputs="full", directory=os.path.join(HERE, "weights")):
+    suffix = "" if inputs == "full" else "_" + inputs
+    return os.path.join(directory, "unet_sunsky%s.npy" % suffix)
+
+
+def train(dn, inputs, targets, steps=1500, batch=16, crop=256, lr=1e-3, seed=5, log=print, input_set="full"):
+    """input_set: the auxiliary channels beyond it are zero in every batch (the filter's input
+    when the caller gives no albedo / normal image)."""
+    keep = INPUT_SETS[input_set]
     rng = np.random.default_rng(seed)
     n, H, W, _ = inputs.shape
     losses, dev_ms = [], 0.0
@@ -75,6 +88,7 @@ def train(dn, inputs, targets, steps=1500, batch=16, crop=256, lr=1e-3, seed=5, 
         ys = rng.integers(0, H - crop + 1, batch)
         xs = rng.integers(0, W - crop + 1, batch)
         bi = np.stack([inputs[i, y:y + crop, x:x + crop] for i, y, x in zip(idx, ys, xs)])
+        bi[..., keep:] = 0.0
         bt = np.stack([targets[i, y:y + crop, x:x + crop] for i, y, x in zip(idx, ys, xs)])
         rate = lr * 0.5 * (1 + math.cos(math.pi * step / steps))  # cosine decay
         loss, ms = dn.trainStep(bi, bt, rate)
@@ -209,7 +223,11 @@ def main():
     ap.add_argument("--train-ref-spp", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--eval-ref-spp", type=int, default=4096)
-    ap.add_argument("--weights-out", default=os.path.join(HERE, "weights", "unet_sunsky.npy"))
+    ap.add_argument("--inputs", default="full",
+                    help="comma-separated input sets to train for (full, color_albedo, color; all: every set), "
+                         "each from the same rendered training set")
+    ap.add_argument("--weights-dir", default=os.path.join(HERE, "weights"))
+    ap.add_argument("--no-eval", action="store_true", help="skip the 1080p relMSE evaluation")
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
     log = lambda m: print(m, flush=True)
@@ -217,13 +235,21 @@ def main():
         kernel_bench(log=log)
         return
     inputs, targets, r = training_set(args.frames, ref_spp=args.train_ref_spp, log=log)
-    dn = P.DenoiserHip(0)
-    tr = train(dn, inputs, targets, steps=args.steps, log=log)
-    os.makedirs(os.path.dirname(args.weights_out), exist_ok=True)
-    np.save(args.weights_out, dn.getWeights())
-    ev = evaluate(dn, ref_spp=args.eval_ref_spp, log=log)
+    sets = sorted(INPUT_SETS) if args.inputs == "all" else args.inputs.split(",")
+    for name in sets:
+        if name not in INPUT_SETS:
+            raise SystemExit(f"unknown input set {name}")
     out = {"training_data": {"frames": args.frames, "ref_spp": args.train_ref_spp, "render_ms": r["ms"], "samples": r["samples"]},
-           "training": tr, "evaluation": ev}
+           "training": {}}
+    dn = P.DenoiserHip(0)
+    for name in sets:
+        dn.setWeights(dn.initialWeights)
+        out["training"][name] = train(dn, inputs, targets, steps=args.steps, log=log, input_set=name)
+        os.makedirs(args.weights_dir, exist_ok=True)
+        np.save(weights_path(name, args.weights_dir), dn.getWeights())
+    if not args.no_eval:
+        dn.setWeights(np.load(weights_path("full", args.weights_dir)))
+        out["evaluation"] = evaluate(dn, ref_spp=args.eval_ref_spp, log=log)
     if args.json_out:
         with open(args.json_out, "w") as f:
             json.dump(out, f, indent=1)

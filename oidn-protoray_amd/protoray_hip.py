@@ -21,7 +21,7 @@ PRT_OK, PRT_ERR_INVALID, PRT_ERR_HIP, PRT_ERR_NOMEM, PRT_ERR_STATE = 0, -1, -2, 
 AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR, AOV_HALF = 1, 2, 4, 8, 16, 32, 64
 FILTER_BOX, FILTER_TRIANGLE, FILTER_GAUSSIAN, FILTER_BLACKMAN_HARRIS = 0, 1, 2, 3
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
-BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_SNAPSHOT = 0, 1, 2, 16
+BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_DEPTH, BATCH_SNAPSHOT = 0, 1, 2, 3, 16
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
@@ -80,6 +80,8 @@ SIGNATURES = {
     "prt_query_rays": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "prt_query_occlusion": [_vp, _i, _vp, _vp],
     "prt_query_pixel": [_vp, ctypes.c_float, ctypes.c_float, _i, _vp, _vp],
+    "prt_query_ray_hits": [_vp, _vp, _i, _vp, _vp],
+    "prt_frame_error_exposure": [_vp, _f, ctypes.POINTER(ctypes.c_float)],
     "prt_camera_matrices": [_vp, _i, _i, _vp, _vp, _vp],
     "prt_scene_bounds": [_vp, _vp, _vp],
     "prt_query_rays_device": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
@@ -280,10 +282,18 @@ class DeviceHip:
         _check(_lib.prt_get_accum(self.h, _ptr(s), _ptr(w)))
         return s, w
 
-    def getFrameError(self):
+    def getFrameError(self, exposure=1.0):
         e = ctypes.c_float(0)
-        _check(_lib.prt_frame_error(self.h, ctypes.byref(e)))
+        _check(_lib.prt_frame_error_exposure(self.h, float(exposure), ctypes.byref(e)))
         return e.value
+
+    def queryRayHits(self, ray, max_hits=1):
+        """Hit properties along one ray [ox oy oz dx dy dz near far] (Renderer::queryRay)."""
+        r = np.ascontiguousarray(ray, np.float32)
+        hits = np.zeros(max_hits, PIXEL_HIT_DTYPE)
+        n = ctypes.c_int32(0)
+        _check(_lib.prt_query_ray_hits(self.h, _ptr(r), max_hits, _ptr(hits), ctypes.byref(n)))
+        return hits[:n.value]
 
     # --- queries (Renderer::queryRay / queryOcclusionRay) ---------------------------------
     def queryRays(self, rays):
@@ -497,7 +507,7 @@ class BatchHip:
 
     def get(self, frame, buffer=BATCH_COLOR):
         W, H = self.frames[frame][:2]
-        out = np.zeros((H, W, 3), np.float32)
+        out = np.zeros((H, W) if buffer == BATCH_DEPTH else (H, W, 3), np.float32)
         _check(_lib.prt_batch_get(self.h, frame, buffer, _ptr(out)))
         return out
 
@@ -520,6 +530,7 @@ class DenoiserHip:
         _check(_lib.prt_denoiser_param_count(self.h, ctypes.byref(n)))
         self.n_params = n.value
         self._keep = None
+        self.initialWeights = self.getWeights()  # the initializer's, before any training or loading
 
     def close(self):
         if getattr(self, "h", None):
@@ -535,13 +546,14 @@ class DenoiserHip:
         return buf.value.decode()
 
     def init(self, color, albedo, normal, output, hdr=True, hdrScale=float("nan")):
-        arrs = [np.ascontiguousarray(a, np.float32) for a in (color, albedo, normal)]
+        """Denoiser::init; albedo and normal may be None (colour-only or colour + albedo input)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (color, albedo, normal)]
         if not (output.dtype == np.float32 and output.flags["C_CONTIGUOUS"]):
             raise ValueError("output must be a C-contiguous float32 array")
         H, W = arrs[0].shape[:2]
         self._keep = arrs + [output]
-        p = DenoiserParams(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, output.ctypes.data, W, H,
-                           int(hdr), hdrScale)
+        ptr = lambda a: None if a is None else a.ctypes.data
+        p = DenoiserParams(ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), output.ctypes.data, W, H, int(hdr), hdrScale)
         _check(_lib.prt_denoiser_init(self.h, ctypes.byref(p)))
 
     def execute(self):

@@ -59,13 +59,33 @@ def test_write_frame_files(tmp_path):
     import scenegen as G
     cam = D.look_at((0, 1, 0), (0, 1, 10))
     img = np.ones((8, 16, 3), np.float32)
-    meta = D.camera_metadata(cam, 16, 8, 64, 8)
-    paths = D.write_frame(tmp_path, "f", {1: img, 4: img}, img, img, img, meta)
+    meta = D.camera_metadata(cam, 16, 8, 64, 8, scene_scale_=D.scene_scale([[0, 0, 0], [3, 2, 5]]))
+    depth = np.linspace(1, 9, 8 * 16, dtype=np.float32).reshape(8, 16)
+    depth[0, 0] = np.inf  # a pixel whose samples all missed
+    paths = D.write_frame(tmp_path, "f", {1: img, 4: img}, img, img, img, meta, depth=depth)
     names = sorted(p.split("/")[-1] for p in paths)
-    assert names == ["f.alb.exr", "f.json", "f.nrm.exr", "f_0001.hdr.exr", "f_0004.hdr.exr", "f_ref.hdr.exr"]
+    assert names == ["f.alb.exr", "f.json", "f.nrm.exr", "f.z.exr", "f_0001.hdr.exr", "f_0004.hdr.exr",
+                     "f_ref.hdr.exr"]
+    z, zn = D.read_exr(tmp_path / "f.z.exr")
+    assert zn == ["Y"] and np.array_equal(z[:, :, 0], depth)  # FP32, exact
     m = json.load(open(tmp_path / "f.json"))
     np.testing.assert_allclose(m["cameraForward"], [0, 0, 1], atol=1e-7)
     assert m["cameraAspectRatio"] == 2.0 and m["samplesPerPixel"] == 64 and m["maxPathDepth"] == 8
+    # the keys of main/render/render_window.cpp:1408-1441, in its order
+    assert list(m) == ["pixelFilter", "pixelFilterWidth", "sampler", "cameraPosition", "cameraForward", "cameraUp",
+                       "cameraRight", "cameraAspectRatio", "cameraFovVertical", "cameraLensRadius",
+                       "cameraFocalDistance", "cameraNear", "worldToViewMatrix", "viewToClipMatrix", "sceneScale",
+                       "exposure", "maxPathDepth", "samplesPerPixel"]
+    assert m["pixelFilter"] == "box" and m["pixelFilterWidth"] == 0.0 and abs(m["sceneScale"] - 0.2) < 1e-12
+    # worldToViewD3D maps the camera position to the origin and a point ahead to +z (row vectors,
+    # D3D view space looks along +z)
+    wv = np.array(m["worldToViewMatrix"])
+    assert wv.shape == (4, 4)
+    np.testing.assert_allclose(np.array([0, 1, 0, 1]) @ wv, [0, 0, 0, 1], atol=1e-6)
+    np.testing.assert_allclose(np.array([0, 1, 5, 1]) @ wv, [0, 0, 5, 1], atol=1e-5)
+    assert np.array(m["viewToClipMatrix"]).shape == (4, 4)
+    jm = D.camera_metadata(cam, 16, 8, 64, 8, jitter=(0.25, 0.5))
+    assert jm["jitterOffset"] == [0.25, 0.0] and "pixelFilter" not in jm
 
 
 def test_dataset_configuration_shape():

@@ -28,7 +28,7 @@ def _scene_dev(sc, max_depth=8):
 
 def _wavefront(dev, cam, W, H, seed, spp):
     dev.initCamera(cam)
-    dev.initFrame(W, H, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+    dev.initFrame(W, H, P.AOV_ALBEDO1 | P.AOV_NORMAL1 | P.AOV_DEPTH)
     dev.initSampler(seed)
     dev.clearFrame()
     out = {}
@@ -36,7 +36,7 @@ def _wavefront(dev, cam, W, H, seed, spp):
         dev.render(1)
         if s in (1, 4, spp):
             out[s] = dev.blitFrame()
-    return out, dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1)
+    return out, dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1), dev.blitFrame(P.BUF_DEPTH)
 
 
 def test_batch_equals_wavefront_renderer():
@@ -49,17 +49,18 @@ def test_batch_equals_wavefront_renderer():
               (dC, cams[0], 64, 64, 5, 4)]
     batch = P.BatchHip(0)
     for d, cam, W, H, seed, spp in frames:
-        batch.addFrame(d, cam, W, H, seed, spp, snapshots=(1, 4), aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+        batch.addFrame(d, cam, W, H, seed, spp, snapshots=(1, 4), aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1 | P.AOV_DEPTH)
     r1 = batch.render()
     assert r1["samples"] == sum(W * H * spp for _, _, W, H, _, spp in frames)
     first = [batch.get(i) for i in range(len(frames))]
     for i, (d, cam, W, H, seed, spp) in enumerate(frames):
-        ref, alb, nrm = _wavefront(d, cam, W, H, seed, spp)
+        ref, alb, nrm, dep = _wavefront(d, cam, W, H, seed, spp)
         np.testing.assert_allclose(batch.get(i), ref[spp], rtol=1e-5, atol=1e-7)
         np.testing.assert_allclose(batch.get(i, P.BATCH_SNAPSHOT), ref[1], rtol=1e-5, atol=1e-7)
         np.testing.assert_allclose(batch.get(i, P.BATCH_SNAPSHOT + 1), ref[4], rtol=1e-5, atol=1e-7)
         np.testing.assert_allclose(batch.get(i, P.BATCH_ALBEDO1), alb, rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(batch.get(i, P.BATCH_NORMAL1), nrm, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(batch.get(i, P.BATCH_DEPTH), dep, rtol=1e-5)
     # the captured graph replays: same work, same result
     r2 = batch.render()
     assert r2["samples"] == r1["samples"]
@@ -75,6 +76,36 @@ def test_batch_equals_wavefront_renderer():
     dA.close()
     dB.close()
     dC.close()
+
+
+def test_batch_follows_a_changed_scene():
+    # geometry, materials and BVH of a frame's scene context replaced after the batch rendered:
+    # the next render rebuilds the frame's kernel arguments (the old device buffers are released)
+    # and equals a fresh batch of the changed scene
+    scA = G.spheres_ambient(seed=4, n_spheres=24, subdiv=2)
+    scB = G.spheres_ambient(seed=5, n_spheres=40, subdiv=2)
+    d = _scene_dev(scA, max_depth=5)
+    cam = D.look_at((0, 2, -2), (1, 1, 15), 50)
+    batch = P.BatchHip(0)
+    batch.addFrame(d, cam, 64, 64, 1, 3)
+    batch.render()
+    before = batch.get(0)
+    mats = scB["materials"].copy()
+    mats["r"] = 1.0 - mats["r"]
+    d.initScene(scB["positions"], scB["indices"], scB["tri_material"], mats)
+    with pytest.raises(ValueError):
+        batch.render()  # new geometry needs its lights set again (the area light depends on it)
+    d.initLights(scB["lights"])
+    batch.render()
+    after = batch.get(0)
+    assert not np.array_equal(after, before)
+    fresh = P.BatchHip(0)
+    fresh.addFrame(d, cam, 64, 64, 1, 3)
+    fresh.render()
+    assert np.array_equal(after, fresh.get(0))
+    fresh.close()
+    batch.close()
+    d.close()
 
 
 def test_dataset_configuration_batch(tmp_path):

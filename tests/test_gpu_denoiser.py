@@ -76,14 +76,18 @@ def test_training_reduces_the_loss(dn):
     assert losses[-1] < 0.5 * losses[0]
 
 
-def test_trained_network_beats_the_noisy_render(dn):
-    # the weights trained on rendered data (oidn-protoray_amd/weights/unet_sunsky.npy) on a scene
-    # and camera outside the training set: 4 samples, denoised vs not, against 256 samples
-    import os
+@pytest.mark.parametrize("inputs", ["full", "color_albedo", "color"])
+def test_trained_network_beats_the_noisy_render(dn, inputs):
+    # the weights trained on rendered data for each input set (Denoiser::Params with and without the
+    # albedo and normal images; oidn-protoray_amd/weights/unet_sunsky*.npy) on a scene and camera
+    # outside the training set: 4 samples, denoised vs not, against 256 samples
     import scenegen as G
     import dataset as D
-    w = np.load(os.path.join(os.path.dirname(P.__file__), "weights", "unet_sunsky.npy"))
-    dn.setWeights(w)
+    import denoise_train as T
+    import os
+    if not os.path.exists(T.weights_path(inputs)):
+        pytest.skip(f"no weights trained for the {inputs} input set")
+    dn.setWeights(np.load(T.weights_path(inputs)))
     sc = G.spheres_sunsky(seed=77, n_spheres=300, subdiv=3, palette=64, coated=0.3, width=256, height=192)
     dev = P.DeviceHip(0)
     P.load_scene(dev, sc)
@@ -98,7 +102,7 @@ def test_trained_network_beats_the_noisy_render(dn):
     dev.render(4)
     noisy, alb, nrm = dev.blitFrame(), dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1)
     dev.close()
-    out = dn.denoise(noisy, alb, nrm)
+    out = dn.denoise(noisy, alb if inputs != "color" else None, nrm if inputs == "full" else None)
     e_noisy, e_out = G.relmse(noisy, ref), G.relmse(out, ref)
     print(f"relMSE vs 256 spp: 4 spp {e_noisy:.4f}, denoised 4 spp {e_out:.4f}")
     assert e_out < 0.25 * e_noisy
