@@ -27,14 +27,16 @@ extern "C" {
 
 typedef struct prt_context prt_context;
 
-/* Material record (84 bytes; a zero field is the reference's default except where noted).  Uniform StandardMaterial parameter sets of the stream path
+/* Material record (148 bytes; a zero field is the reference's default except where noted).  Uniform StandardMaterial parameter sets of the stream path
  * (material/standard.cpp:21-183 ctor, :409-597 SIMD lobes), each a MultiBsdf with one lobe:
  *   kind 0 diffuse    : baseColor = (r,g,b), specularWeight 0          -> DiffuseBrdf
  *   kind 1 conductor  : metalness 1, baseColor = (r,g,b), specularColor = (er,eg,eb),
  *                       specularRoughness = roughness                  -> RoughConductorBrdf
  *                                                                         <FresnelConductorArtistic>
  *   kind 2 dielectric : transmissionWeight 1, specularIOR = ior,
- *                       specularRoughness = roughness                  -> RoughDielectricBsdf
+ *                       specularRoughness = roughness                  -> RoughDielectricBsdf;
+ *                       its interior medium absorbs by transmission_color ^ transmission_depth
+ *                       per unit length (see the fields below)
  *   kind 3 coated     : a rough dielectric layer over a diffuse base   -> RoughCoatingBsdf
  *                       (rough_coating.h:270-593) over DiffuseBrdf.  The StandardMaterial
  *                       default (metalness 0, specularWeight w > 0, specularRoughness > 0) is
@@ -81,6 +83,27 @@ typedef struct prt_material {
   float sheen_weight;
   float sheen_color[3];
   float sheen_roughness;
+  /* kinds 2 and 5: the medium inside a transmissive, non-thin-walled surface, transmission_color ^
+   * transmission_depth (material/standard.cpp:169-173); a path segment that ends at a hit inside
+   * it attenuates its throughput by that colour ^ hit distance (path_renderer_stream.h:652-657).
+   * The reference's default is depth 1 with colour 1 (clear); depth 0 (a zeroed record) is clear
+   * as well, since colour ^ 0 = 1. */
+  float transmission_color[3];
+  float transmission_depth;
+  /* kind 5 only: the StandardMaterial parameters read by no single-lobe kind (zero is NOT the
+   * default here: set every field).  The fields above keep their meaning: (r,g,b) baseColor,
+   * (er,eg,eb) specularColor, roughness specularRoughness, ior specularIOR, thickness (thin-walled
+   * transmission), emission, transparency, diffuse_roughness, sheen_*; weight is unused. */
+  float base_weight;          /* baseWeight, default 1 */
+  float metalness;            /* default 0 */
+  float specular_weight;      /* specularWeight, default 1 */
+  float transmission_weight;  /* default 0 */
+  float coat_weight;          /* default 0 */
+  float coat_color[3];        /* default 1 */
+  float coat_roughness;       /* default 0 (> 0 required where the coat is present) */
+  float coat_ior;             /* coatIOR, default 1.6 */
+  float coat_thickness;       /* default 0.5 (StandardMaterial's fixed coat thickness) */
+  int32_t thin_walled;        /* default 0 */
 } prt_material;
 
 /* Environment light record (32 bytes): light/ambient_light.h (kind 0, Ke = rgb),

@@ -338,6 +338,7 @@ struct prt_context
   DevBuf<int> idx, triMat;
   std::vector<prt_material> materials;
   DevBuf<prt_material_constants> mats;
+  DevBuf<float4> media;  // interior media: (r, g, b, ior) per medium ID, ID 0 clear
   std::vector<float> luts;
   DevBuf<float> lut;
   std::vector<prt_light_dev> lights;
@@ -554,6 +555,21 @@ int prt_scene_set_geometry(prt_context* c, int32_t nverts, const float* position
   return PRT_OK;
 }
 
+// The interior medium of a transmissive surface that is not thin-walled: colour
+// transmission_color ^ transmission_depth (material/standard.cpp:48-49, :169-173) and IOR.  Media
+// are deduplicated by (colour, IOR), ID 0 being the clear ambient medium
+// (material/material_factory.cpp:20-21, :159-175); media holds (r, g, b, ior) per ID.
+static int mediumOf(std::vector<float4>& media, float ior, const prt_material& s)
+{
+  const float d = std::max(s.transmission_depth, 0.f);
+  float col[3];
+  for (int j = 0; j < 3; ++j) col[j] = std::pow(hclamp(s.transmission_color[j], 0.f, 1.f), d);
+  for (size_t t = 1; t < media.size(); ++t)
+    if (media[t].x == col[0] && media[t].y == col[1] && media[t].z == col[2] && media[t].w == ior) return int(t);
+  media.push_back(make_float4(col[0], col[1], col[2], ior));
+  return int(media.size()) - 1;
+}
+
 int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
 {
   CTX_CHECK(c);
@@ -561,7 +577,7 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
   if (n < 1 || !m) return fail(PRT_ERR_INVALID, "need at least the reserved material 0");
   c->materials.assign(m, m + n);
   std::vector<prt_material_constants> k(n);
-  std::vector<float> iors;
+  std::vector<float4> media(1, make_float4(1.f, 1.f, 1.f, 1.f));  // ID 0: the clear ambient medium
   for (int i = 0; i < n; ++i)
   {
     prt_material_constants& q = k[i];
@@ -681,11 +697,7 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
         q.FavgEta[o] = Fe;
         q.FavgRcpEta[o] = Fr;
       }
-      // media deduplicated by (color, IOR), ID 0 = ambient (material/material_factory.cpp:20-21,159-175)
-      int id = -1;
-      for (size_t t = 0; t < iors.size(); ++t) if (iors[t] == ior) id = int(t) + 1;
-      if (id < 0) { iors.push_back(ior); id = int(iors.size()); }
-      q.mediumInt = id;
+      q.mediumInt = mediumOf(media, ior, s);
     }
   }
   c->anyTransparent = c->extMaterials = 0;
@@ -697,6 +709,8 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
   }
   HIP_TRY(c->mats.alloc(n));
   HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * n, hipMemcpyHostToDevice));
+  HIP_TRY(c->media.alloc(media.size()));
+  HIP_TRY(hipMemcpy(c->media.p, media.data(), sizeof(float4) * media.size(), hipMemcpyHostToDevice));
   c->lightsReady = false;  // the area light depends on the materials: set the lights again
   return PRT_OK;
 }
@@ -1103,7 +1117,7 @@ static RenderArgs makeArgs(prt_context* c)
 {
   RenderArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.lut = c->lut.p;
+  a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.media = c->media.p; a.lut = c->lut.p;
   a.anyTransparent = c->anyTransparent;
   a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
   a.nLights = int(c->lights.size());

@@ -38,6 +38,7 @@ struct RenderArgs
   const int* idx;
   const int* triMat;
   const prt_material_constants* mats;
+  const float4* media;  // interior media (r, g, b, ior) by medium ID; ID 0 is clear
   const float* lut;
   BvhView bvh;
   int anyTransparent;  // some material lets shadow rays through (render/scene.cpp:452-462)

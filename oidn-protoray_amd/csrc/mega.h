@@ -97,6 +97,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     Bsdf b = makeBsdf(m, ctx, wo, inside);
     Sampler smp;
     smp.set(a.seed, sampleIndex, pixelId);
+    if (mediumId > 0) T = T * mediumTransmittance(a.media[mediumId], th);  // volume transmission (:652-657)
     const bool mAov1 = !aov1;
     bool mAov2 = !aov2;
     const bool mRR = !(mAov1 || mAov2);

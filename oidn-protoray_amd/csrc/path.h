@@ -98,6 +98,13 @@ PRT_DEV f3 sampleLight(const RenderArgs& a, f3 p, f3& wi, float& pdf, float& dis
 }
 
 // geometry/triangle_mesh.cpp:152-213 + render/scene.h:61-72
+// Beer-Lambert transmittance of a path segment of length t through a medium of colour (r, g, b)
+// (path_renderer_stream.h:652-657, pow(mediumColor, ray.far)).
+PRT_DEV f3 mediumTransmittance(float4 medium, float t)
+{
+  return f3(powf(medium.x, t), powf(medium.y, t), powf(medium.z, t));
+}
+
 PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, float u, float v)
 {
   Ctx c;

@@ -278,6 +278,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
         Bsdf b = makeBsdf(m, ctx, wo, inside, kExt);
         Sampler smp;
         smp.set(a.seed, sampleIndex, pixelId);  // resetSample (:641-642)
+        if (mediumId > 0) T = T * mediumTransmittance(a.media[mediumId], h.x);  // volume transmission (:652-657)
         bool mAov1 = !(fl & PRT_F_AOV1), mAov2 = !(fl & PRT_F_AOV2);
         bool mRR = !(mAov1 || mAov2);
         // direct light (:659-718) when the BSDF has a smooth lobe: all but the transparent BTDF

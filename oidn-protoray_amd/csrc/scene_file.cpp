@@ -1063,10 +1063,11 @@ prt_material toRecord(const StdMat& m, bool& approx)
     r.kind = 2;
     r.ior = std::fabs(ior - 1.f) > 1e-6f ? ior : 1.5f;
     r.roughness = m.specularRoughness;
-    // the medium inside: transmissionColor ^ transmissionDepth... kept white (not modelled here)
+    // the medium inside: transmissionColor ^ transmissionDepth (material/standard.cpp:169-173)
     r.r = r.g = r.b = 1.f;
-    approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f || std::fabs(ior - 1.f) <= 1e-6f ||
-             m.transmissionColor[0] != 1.f || m.transmissionColor[1] != 1.f || m.transmissionColor[2] != 1.f;
+    for (int c = 0; c < 3; ++c) r.transmission_color[c] = m.transmissionColor[c];
+    r.transmission_depth = m.transmissionDepth;
+    approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f || std::fabs(ior - 1.f) <= 1e-6f;
     return r;
   }
   r.r = m.baseColor[0] * m.baseWeight;

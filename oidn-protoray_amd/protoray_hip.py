@@ -27,7 +27,12 @@ MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
                            ("emission", "<f4", 3), ("transparency", "<f4"),
                            ("diffuse_roughness", "<f4"), ("sheen_weight", "<f4"), ("sheen_color", "<f4", 3),
-                           ("sheen_roughness", "<f4")])
+                           ("sheen_roughness", "<f4"),
+                           ("transmission_color", "<f4", 3), ("transmission_depth", "<f4"),
+                           ("base_weight", "<f4"), ("metalness", "<f4"), ("specular_weight", "<f4"),
+                           ("transmission_weight", "<f4"), ("coat_weight", "<f4"), ("coat_color", "<f4", 3),
+                           ("coat_roughness", "<f4"), ("coat_ior", "<f4"), ("coat_thickness", "<f4"),
+                           ("thin_walled", "<i4")])
 PIXEL_HIT_DTYPE = np.dtype([("prim", "<i4"), ("material", "<i4"), ("material_type", "<i4"), ("p", "<f4", 3),
                             ("Ng", "<f4", 3), ("N", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("uv", "<f4", 2),
                             ("T", "<f4", 3), ("eps", "<f4"), ("dist", "<f4"), ("depth", "<f4")])

@@ -39,7 +39,12 @@ MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
                            ("emission", "<f4", 3), ("transparency", "<f4"),
                            ("diffuse_roughness", "<f4"), ("sheen_weight", "<f4"), ("sheen_color", "<f4", 3),
-                           ("sheen_roughness", "<f4")])
+                           ("sheen_roughness", "<f4"),
+                           ("transmission_color", "<f4", 3), ("transmission_depth", "<f4"),
+                           ("base_weight", "<f4"), ("metalness", "<f4"), ("specular_weight", "<f4"),
+                           ("transmission_weight", "<f4"), ("coat_weight", "<f4"), ("coat_color", "<f4", 3),
+                           ("coat_roughness", "<f4"), ("coat_ior", "<f4"), ("coat_thickness", "<f4"),
+                           ("thin_walled", "<i4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),
@@ -349,3 +354,60 @@ def relmse(x, ref):
     x = np.asarray(x, np.float64)
     y = np.asarray(ref, np.float64)
     return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))
+
+
+def glass_slabs(slabs=(((0.25, 0.5, 0.8), 0.5), ((0.5, 0.5, 0.5), 2.0)), thickness=2.0, ior=1.0001, z0=5.0,
+                ambient=1.0, width=64, height=32, max_depth=6):
+    """Media absorption with a closed form: side by side (slab k covers x in [xs[k], xs[k + 1]], xs
+    evenly from -40 to 40), slabs of one IOR but their own transmission colour and depth, each two
+    facing quads at z = z0 and
+    z0 + thickness of a smooth dielectric so close to the ambient IOR that the rays pass almost
+    straight and unreflected (Fresnel 2.5e-9 at normal incidence), under an ambient light; the
+    camera at the origin looks along +z.  A camera ray through slab k returns ambient x
+    (colour_k ^ depth_k) ^ (thickness / cos theta), theta its angle to the z axis
+    (path_renderer_stream.h:652-657, material/standard.cpp:169-173).  slabs: ((colour, depth), ...)
+    in order of increasing x."""
+    n = len(slabs)
+    pos, idx, tri_mat = [], [], []
+    xs = np.linspace(-40.0, 40.0, n + 1)
+    for k in range(n):
+        x0, x1 = xs[k + 1], xs[k]  # the camera's U is -x: slab 0 fills the left of the image
+        for z in (z0, z0 + thickness):
+            b = len(pos)
+            pos += [[x0, -40, z], [x1, -40, z], [x1, 40, z], [x0, 40, z]]
+            idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+            tri_mat += [1 + k, 1 + k]
+    mats = np.zeros(1 + n, MATERIAL_DTYPE)
+    for k, (col, depth) in enumerate(slabs):
+        m = mats[1 + k]
+        m["kind"] = KIND_DIELECTRIC
+        m["ior"] = ior
+        m["roughness"] = 0.0
+        m["transmission_color"] = col
+        m["transmission_depth"] = depth
+    lights = np.zeros(1, LIGHT_DTYPE)
+    lights[0] = (LIGHT_AMBIENT, ambient, ambient, ambient, 0, 0, 0, 0)
+    return dict(name="glass_slabs", positions=np.array(pos, np.float32), indices=np.array(idx, np.int32),
+                tri_material=np.array(tri_mat, np.int32), materials=mats, lights=lights, env=None,
+                camera=_camera(fov_deg=60.0), width=width, height=height, spp=1, max_depth=max_depth, seed=0,
+                rr_depth=100, slab_thickness=thickness, slab_z0=z0,
+                slab_media=[(np.array(c, np.float64), d) for c, d in slabs])
+
+
+def glass_slab_radiance(scene, dirs):
+    """The closed form of glass_slabs for camera directions dirs (n x 3): the ambient radiance times
+    the slab's medium colour to the power of the path length through it, for the slab the ray
+    enters (returned as k per ray)."""
+    d = np.asarray(dirs, np.float64)
+    amb = float(scene["lights"][0]["r"])
+    cos = d[:, 2] / np.linalg.norm(d, axis=1)
+    media = scene["slab_media"]
+    n = len(media)
+    x_hit = d[:, 0] / d[:, 2] * scene["slab_z0"]
+    xs = np.linspace(-40.0, 40.0, n + 1)
+    k = np.clip(np.searchsorted(xs, x_hit) - 1, 0, n - 1)
+    out = np.zeros((d.shape[0], 3))
+    for j, (col, depth) in enumerate(media):
+        sel = k == j
+        out[sel] = amb * (col[None] ** depth) ** (scene["slab_thickness"] / cos[sel, None])
+    return out, k

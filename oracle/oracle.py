@@ -34,7 +34,12 @@ MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
                            ("emission", "<f4", 3), ("transparency", "<f4"),
                            ("diffuse_roughness", "<f4"), ("sheen_weight", "<f4"), ("sheen_color", "<f4", 3),
-                           ("sheen_roughness", "<f4")])
+                           ("sheen_roughness", "<f4"),
+                           ("transmission_color", "<f4", 3), ("transmission_depth", "<f4"),
+                           ("base_weight", "<f4"), ("metalness", "<f4"), ("specular_weight", "<f4"),
+                           ("transmission_weight", "<f4"), ("coat_weight", "<f4"), ("coat_color", "<f4", 3),
+                           ("coat_roughness", "<f4"), ("coat_ior", "<f4"), ("coat_thickness", "<f4"),
+                           ("thin_walled", "<i4")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                         ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("angle", "<f4")])
 CAMERA_DTYPE = np.dtype([("pos", "<f4", 3), ("U", "<f4", 3), ("V", "<f4", 3), ("N", "<f4", 3),

@@ -574,9 +574,9 @@ enum : int {
 //                                                           -> ThinRoughDielectricBsdf
 // Any kind: transparency = 1 - opacity mixes in a TransparentBtdf; diffuseRoughness (kinds 0
 // and 3) selects the Oren-Nayar base.
-enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2, kCoated = 3, kThinDielectric = 4 };
+enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2, kCoated = 3, kThinDielectric = 4, kStandard = 5 };
 
-struct Material  // 84-byte record, the prt_material layout of the HIP path
+struct Material  // 148-byte record, the prt_material layout of the HIP path
 {
   int kind;
   float r, g, b;     // albedo (diffuse) / reflectivity (conductor)
@@ -588,10 +588,32 @@ struct Material  // 84-byte record, the prt_material layout of the HIP path
   float emission[3]; // emissionColor * emissionLuminance; non-zero makes the material an area light
   float transparency;      // 1 - opacity
   float diffuseRoughness;  // Oren-Nayar base of kinds 0 and 3 when > 0
-  float sheenWeight, sheenColor[3], sheenRoughness;  // the sheen layer (not built)
+  float sheenWeight, sheenColor[3], sheenRoughness;  // the sheen layer
+  // kinds 2 and 5: the interior medium's colour = transmissionColor ^ transmissionDepth
+  // (material/standard.cpp:169-173); depth 0 leaves it clear
+  float transmissionColor[3], transmissionDepth;
+  // kind 5: the StandardMaterial parameters no single-lobe kind reads (standard.cpp:27-81)
+  float baseWeight, metalness, specularWeight, transmissionWeight;
+  float coatWeight, coatColor[3], coatRoughness, coatIor, coatThickness;
+  int thinWalled;
 };
+static_assert(sizeof(Material) == 148, "the prt_material layout");
 // MaterialType::Transparent (material/standard.cpp:152-166): shadow rays pass through
-inline bool isTransparent(const Material& m) { return m.transparency != 0.f || m.kind == kThinDielectric; }
+inline bool isTransparent(const Material& m)
+{
+  if (m.transparency != 0.f || m.kind == kThinDielectric) return true;
+  return m.kind == kStandard && m.transmissionWeight > 0.f && m.thinWalled;
+}
+
+// MaterialType::MediaInterface (material/standard.cpp:159-175): a transmissive surface that is not
+// thin-walled separates the ambient medium from its interior one.
+inline bool isMediaInterface(const Material& m)
+{
+  if (m.kind == kDielectric) return true;
+  return m.kind == kStandard && m.transmissionWeight > 0.f && !m.thinWalled;
+}
+// The IOR of that interface, as the medium records it (>= 1).
+inline float interfaceIor(const Material& m) { return m.ior < 1.f ? 1.0f / m.ior : m.ior; }
 
 inline bool isEmissive(const Material& m) { return m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f; }
 

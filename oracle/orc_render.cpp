@@ -37,10 +37,11 @@ struct Scene
   std::vector<Node> nodes;
   std::vector<Tri> tris;
   Luts luts;
-  // Interior medium ID per material.  Media are deduplicated by (color, IOR) with ID 0 the
-  // clear ambient medium (material/material_factory.cpp:20-21,159-175); transmission color is
-  // 1 for every supported dielectric, so distinct IORs get IDs 1, 2, ... in material order.
+  // Interior medium ID per material and the media's colours.  Media are deduplicated by
+  // (colour, IOR) with ID 0 the clear ambient medium (material/material_factory.cpp:20-21,
+  // 159-175); new media get IDs 1, 2, ... in material order.
   std::vector<int> matMedium;
+  std::vector<V3> mediumColor;
   bool anyTransparent = false;  // render/scene.cpp:452-462: shadow rays use the transparent filter
   // pixel sample (path_renderer_stream.h:228-236): a fixed jitter, or the sampler's value warped
   // by the pixel filter's tabulated CDF (image/pixel_filter.h:30-66) when filterCdf is not empty
@@ -50,19 +51,38 @@ struct Scene
   float filterWidth = 0.f;
 };
 
+// The colour of a material's interior medium: pow(clamp(transmissionColor), transmissionDepth)
+// (material/standard.cpp:48-49, :169-173).
+static V3 interiorColor(const Material& m)
+{
+  const float d = std::max(m.transmissionDepth, 0.f);
+  auto ch = [&](int k) { return std::pow(std::min(std::max(m.transmissionColor[k], 0.f), 1.f), d); };
+  return V3(ch(0), ch(1), ch(2));
+}
+
 static void assignMedia(Scene& s)
 {
   std::vector<float> iors;
   s.matMedium.assign(s.mats.size(), 0);
+  s.mediumColor.assign(1, V3(1.f));
   s.anyTransparent = false;
   for (size_t m = 0; m < s.mats.size(); ++m) s.anyTransparent = s.anyTransparent || isTransparent(s.mats[m]);
   for (size_t m = 0; m < s.mats.size(); ++m)
   {
-    if (s.mats[m].kind != kDielectric) continue;
-    float ior = s.mats[m].ior < 1.f ? 1.0f / s.mats[m].ior : s.mats[m].ior;
+    if (!isMediaInterface(s.mats[m])) continue;
+    const float ior = interfaceIor(s.mats[m]);
+    const V3 col = interiorColor(s.mats[m]);
     int id = -1;
-    for (size_t k = 0; k < iors.size(); ++k) if (iors[k] == ior) id = int(k) + 1;
-    if (id < 0) { iors.push_back(ior); id = int(iors.size()); }
+    for (size_t k = 0; k < iors.size(); ++k)
+      if (iors[k] == ior && s.mediumColor[k + 1].x == col.x && s.mediumColor[k + 1].y == col.y &&
+          s.mediumColor[k + 1].z == col.z)
+        id = int(k) + 1;
+    if (id < 0)
+    {
+      iors.push_back(ior);
+      s.mediumColor.push_back(col);
+      id = int(iors.size());
+    }
     s.matMedium[m] = id;
   }
 }
@@ -967,7 +987,13 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       // sampler.resetSample with the same index and pixel (:641-642) leaves the state unchanged
       bool mAov1 = !aov1Done, mAov2 = !aov2Done;
       bool mRR = !(mAov1 || mAov2);
-      // media of the configs are clear (transmission color 1): pow(1, far) == 1 (:653-657)
+      // volume transmission (:652-657): a segment that ends here inside a medium is attenuated
+      // by the medium's colour to the power of its length
+      if (mediumId > 0)
+      {
+        const V3 mc = s.mediumColor[mediumId];
+        T = T * V3(std::pow(mc.x, r.tfar), std::pow(mc.y, r.tfar), std::pow(mc.z, r.tfar));
+      }
 
       // direct light (:659-718)
       if (b.flags & BsdfType::Smooth)

@@ -320,3 +320,15 @@ def two_quad_samples(scene, W, H, sample_index, sampler_fn, margin_eps=1e-4, gra
         out[:, i] = np.maximum(L, 0.0)
         ok[i] = margin >= margin_eps and not graze
     return out, ok
+
+
+def pinhole_dirs(cam, W, H, jx, jy):
+    """Unit directions of the pinhole camera's rays through (x + jx, y + jy) of every pixel, row
+    by row (camera/pinhole_camera.h:86-117; the view looks along -N, image rows go down V)."""
+    U, V, N = (np.asarray(cam[k], np.float64) for k in ("U", "V", "N"))
+    hh = math.tan(0.5 * float(cam["fov"]))
+    hw = hh * W / H
+    ys, xs = np.divmod(np.arange(W * H), W)
+    d = (-hw * U + hh * V - N)[None] + ((xs + jx) / W)[:, None] * (2 * hw * U)[None] \
+        + ((ys + jy) / H)[:, None] * (-2 * hh * V)[None]
+    return d / np.linalg.norm(d, axis=1, keepdims=True)

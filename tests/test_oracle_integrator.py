@@ -30,7 +30,11 @@ def plane_scene(kind, rgb=(0.6, 0.6, 0.6), edge=(1, 1, 1), roughness=0.5, ior=1.
     v = np.array([[-1000, 0, -1000], [1000, 0, -1000], [1000, 0, 1000], [-1000, 0, 1000]], np.float32)
     idx = np.array([[0, 2, 1], [0, 3, 2]], np.int32)
     mats = np.zeros(2, O.MATERIAL_DTYPE)
-    mats[1] = (kind, *rgb, *edge, roughness, ior, thickness, weight, (0, 0, 0), 0, 0, 0, (0, 0, 0), 0)
+    m = mats[1]
+    m["kind"] = kind
+    m["r"], m["g"], m["b"] = rgb
+    m["er"], m["eg"], m["eb"] = edge
+    m["roughness"], m["ior"], m["thickness"], m["weight"] = roughness, ior, thickness, weight
     s = O.Scene(v, idx, np.ones(2, np.int32), mats)
     lights = np.zeros(1, O.LIGHT_DTYPE)
     lights[0] = (0, 1.0, 1.0, 1.0, 0, 0, 0, 0)
