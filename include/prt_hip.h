@@ -49,7 +49,20 @@ typedef struct prt_context prt_context;
  *                       thickness / transmissionDepth = thickness -> ThinRoughDielectricBsdf
  *                       (thin_rough_dielectric.h:170-345): glossy reflection plus an unbent
  *                       transmission (a continuation lobe: the ray goes straight through).
- * thickness is read by kinds 3 and 4, weight by kind 3 only.
+ *   kind 5 standard : any uniform StandardMaterial parameter set (material/standard.cpp:409-611
+ *                       for ambientIOR 1): the opaque base (diffuse or Oren-Nayar, under a
+ *                       specular coating of weight specular_weight when specularIOR differs from
+ *                       1), the translucent base (rough dielectric, thin rough dielectric when
+ *                       thin_walled, the transparent BTDF at IOR 1) and the metal base (rough
+ *                       conductor, artistic Fresnel of baseColor * base_weight and specularColor)
+ *                       in one MultiBsdf weighted (1 - metalness)(1 - transmission),
+ *                       (1 - metalness) transmission specular and metalness specular; a rough
+ *                       coat (coat_*) over that mix; then sheen and opacity as for every kind.
+ *                       Reads the kind-5 fields below; a transmissive, non-thin-walled one is a
+ *                       media interface like kind 2.
+ * thickness is read by kinds 3, 4 and 5 (thin-walled transmission), weight by kind 3 only.
+ * Zero roughness is modelled by the rough lobes at the minimum GGX alpha (0.001), where the
+ * reference switches to its smooth lobes.
  * transparency = 1 - opacity (standard.cpp:152-157, :412-420, :613-627), any kind: a value
  * above 1e-5 mixes the lobe (scale opacity) with a transparent BTDF (scale transparency,
  * transparent.h) in a MultiBsdf.  A material with transparency > 0, and every kind 4, is
@@ -69,6 +82,7 @@ typedef struct prt_context prt_context;
 #define PRT_KIND_DIELECTRIC 2
 #define PRT_KIND_COATED 3
 #define PRT_KIND_THIN_DIELECTRIC 4
+#define PRT_KIND_STANDARD 5
 typedef struct prt_material {
   int32_t kind;
   float r, g, b;

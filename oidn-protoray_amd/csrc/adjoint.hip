@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
     const int pix = y * a.W + x;
     const f3 dl(delta[pix] * invSpp, delta[a.npix + pix] * invSpp, delta[2 * a.npix + pix] * invSpp);
     // pass 1 recovers the sample's radiance; pass 2 replays the same path with the gradient hook
-    const f3 Ltotal = tracePathMega(a, st, x, y, sampleIndex, nullptr, NoVertexHook(), nodes, tris);
+    const f3 Ltotal = tracePathMega<true>(a, st, x, y, sampleIndex, nullptr, NoVertexHook(), nodes, tris);
     if (nParams > 0)
     {
       auto hook = [&](const PathVertex& pv) {
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
         recMat[pv.depth][threadIdx.x] = pv.matId;
         recG[pv.depth][threadIdx.x] = gv;
       };
-      tracePathMega(a, st, x, y, sampleIndex, nullptr, hook, nodes, tris);
+      tracePathMega<true>(a, st, x, y, sampleIndex, nullptr, hook, nodes, tris);
     }
     if (replayL)
     {

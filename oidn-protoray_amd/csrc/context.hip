@@ -327,7 +327,9 @@ struct prt_context
 {
   float boundsLo[3] = {1.f, 1.f, 1.f}, boundsHi[3] = {-1.f, -1.f, -1.f};  // scene box (empty: lo > hi)
   int anyTransparent = 0;  // some material lets shadow rays through
-  int extMaterials = 0;    // some material is a thin dielectric, has opacity or an Oren-Nayar base
+  int extMaterials = 0;    // some material is a thin dielectric, has opacity, an Oren-Nayar base,
+                           // sheen or a coloured interior medium
+  int stdMaterials = 0;    // some material is of kind 5
   int device = 0;
   hipStream_t stream = nullptr;
   hipDeviceProp_t props{};
@@ -564,11 +566,119 @@ static int mediumOf(std::vector<float4>& media, float ior, const prt_material& s
   const float d = std::max(s.transmission_depth, 0.f);
   float col[3];
   for (int j = 0; j < 3; ++j) col[j] = std::pow(hclamp(s.transmission_color[j], 0.f, 1.f), d);
-  for (size_t t = 1; t < media.size(); ++t)
+  for (size_t t = 0; t < media.size(); ++t)
     if (media[t].x == col[0] && media[t].y == col[1] && media[t].z == col[2] && media[t].w == ior) return int(t);
   media.push_back(make_float4(col[0], col[1], col[2], ior));
   return int(media.size()) - 1;
 }
+
+// One lobe's constants (the per-hit constructor work of its reference class for the given,
+// already clamped parameters).  kind: PRT_KIND_DIFFUSE .. PRT_KIND_THIN_DIELECTRIC or
+// PRT_KIND_TRANSPARENT; etaArg[o]: the eta the dielectric, coating or thin dielectric is built
+// with seen from outside (o = 0) and inside (o = 1), clamped to [1/3, 3]; coat*: the coating of
+// the coated diffuse kind.
+struct LobeParams
+{
+  int kind = PRT_KIND_DIFFUSE;
+  float rgb[3] = {0.f, 0.f, 0.f}, edge[3] = {0.f, 0.f, 0.f};
+  float roughness = 0.f, diffRough = 0.f;
+  float etaArg[2] = {1.f, 1.f};
+  float coatColor[3] = {1.f, 1.f, 1.f}, thickness = 0.f, scale = 0.f;
+};
+
+static void lobeConstants(const std::vector<float>& luts, const LobeParams& p, prt_material_constants& q)
+{
+  q.kind = p.kind;
+  q.roughness = p.roughness;
+  q.alpha = std::max(hsqr(q.roughness), 0.001f);  // bsdf/microfacet.h:10-24
+  q.importance = 1.f;
+  q.diffuseRoughness = p.diffRough;
+  q.smooth = 1;
+  const float* rgb = p.rgb;
+  if (p.kind == PRT_KIND_DIFFUSE)
+  {
+    for (int j = 0; j < 3; ++j) q.c0[j] = rgb[j];
+    q.importance = std::max(std::max(rgb[0], rgb[1]), rgb[2]);  // bsdf/diffuse.h:72-75
+  }
+  else if (p.kind == PRT_KIND_CONDUCTOR)
+  {
+    q.Eavg = lutInterp1(q.roughness * 31.f, luts.data() + PRT_LUT_GGX_AVG, 32);
+    for (int j = 0; j < 3; ++j)
+    {
+      // bsdf/optics.h:214-240 (Gulbrandsen + Kulla-Conty average)
+      float r = std::min(rgb[j], 0.99f), g = p.edge[j];
+      float nMin = (1.f - r) / (1.f + r), nMax = (1.f + std::sqrt(r)) / (1.f - std::sqrt(r));
+      float nn = g * nMin + (1.f - g) * nMax;
+      float k2 = (hsqr(nn + 1.f) * r - hsqr(nn - 1.f)) / (1.f - r);
+      q.c0[j] = nn;
+      q.c1[j] = std::sqrt(std::max(k2, 0.f));
+      float rr = rgb[j], r2 = rr * rr, r3 = r2 * rr, g2 = g * g, g3 = g2 * g;
+      float Favg = 0.087237f + 0.0230685f * g - 0.0864902f * g2 + 0.0774594f * g3 + 0.782654f * rr - 0.136432f * r2 +
+                   0.278708f * r3 + 0.19744f * g * rr + 0.0360605f * g2 * rr - 0.2586f * g * r2;
+      q.c2[j] = hsqr(Favg) * q.Eavg / (1.f - Favg * (1.f - q.Eavg));  // rough_conductor.h:127-129
+      q.rr[j] = rgb[j];
+      q.edge[j] = g;
+      q.Favg[j] = Favg;
+    }
+  }
+  else if (p.kind == PRT_KIND_COATED)
+  {
+    // RoughCoatingBsdf over a diffuse base (bsdf/rough_coating.h:287-295): the member eta is
+    // folded to <= 1, Ravg uses the argument as given
+    float baseImportance = std::max(std::max(rgb[0], rgb[1]), rgb[2]);  // bsdf/diffuse.h:70-73
+    for (int j = 0; j < 3; ++j)
+    {
+      q.c0[j] = rgb[j];
+      q.c1[j] = p.coatColor[j];
+      // getAlbedo (rough_coating.h:567-571): lerp(base, base * coat^thickness, scale)
+      q.rr[j] = rgb[j] + p.scale * (rgb[j] * std::pow(p.coatColor[j], p.thickness) - rgb[j]);
+    }
+    q.c2[0] = p.thickness;
+    q.c2[1] = p.scale;
+    q.c2[2] = baseImportance;
+    for (int o = 0; o < 2; ++o) q.eta[o] = p.etaArg[o] <= 1.f ? p.etaArg[o] : 1.0f / p.etaArg[o];
+    q.Eavg = dielReflAvg(luts, p.etaArg[0], q.roughness) * p.scale;  // Ravg (one orientation per record)
+    q.importance = std::max(1.f, baseImportance);                     // rough_coating.h:562-565
+  }
+  else if (p.kind == PRT_KIND_THIN_DIELECTRIC)
+  {
+    // ThinRoughDielectricBsdf (bsdf/thin_rough_dielectric.h:180-192)
+    const float eta = p.etaArg[0];
+    q.Eavg = lutInterp1(q.roughness * 31.f, luts.data() + PRT_LUT_GGX_AVG, 32);
+    float Favg = fresnelAvg(eta);
+    q.c1[0] = Favg * (1.f - q.Eavg) / (1.f - Favg * q.Eavg);  // fmsScale
+    q.c2[0] = p.thickness;
+    q.eta[0] = q.eta[1] = eta;
+    for (int j = 0; j < 3; ++j)
+    {
+      q.c0[j] = rgb[j];
+      q.rr[j] = std::pow(rgb[j], p.thickness);  // getAlbedo (:327-330)
+    }
+  }
+  else if (p.kind == PRT_KIND_DIELECTRIC)
+  {
+    for (int o = 0; o < 2; ++o)
+    {
+      const float eta = p.etaArg[o];
+      float Ee = dielAvg(luts, eta, q.roughness), Er = dielAvg(luts, 1.0f / eta, q.roughness);
+      float Fe = fresnelAvg(eta), Fr = fresnelAvg(1.0f / eta);
+      float a = (1.f - Fe) / (1.f - Er), b = (1.f - Fr) / (1.f - Ee);
+      float x = b / (a + b);
+      q.eta[o] = eta;
+      q.EavgEta[o] = Ee;
+      q.EavgRcpEta[o] = Er;
+      q.fmsRatio[o] = 1.f - x * (1.f - Fe);  // rough_dielectric.h:283-291
+      q.FavgEta[o] = Fe;
+      q.FavgRcpEta[o] = Fr;
+    }
+  }
+  else  // the transparent BTDF (bsdf/transparent.h:32-71): importance 1, no smooth lobe
+  {
+    q.smooth = 0;
+  }
+}
+
+static float foldIor(float ior) { return ior < 1.f ? 1.0f / ior : ior; }
 
 int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
 {
@@ -576,139 +686,216 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
   ++c->generation;
   if (n < 1 || !m) return fail(PRT_ERR_INVALID, "need at least the reserved material 0");
   c->materials.assign(m, m + n);
-  std::vector<prt_material_constants> k(n);
+  // the n materials' records, then the base records of kind-5 materials
+  std::vector<prt_material_constants> k(n), extra;
   std::vector<float4> media(1, make_float4(1.f, 1.f, 1.f, 1.f));  // ID 0: the clear ambient medium
+  const float eps = 1e-5f;  // material/standard.h:79
   for (int i = 0; i < n; ++i)
   {
     prt_material_constants& q = k[i];
     std::memset(&q, 0, sizeof(q));
     const prt_material& s = m[i];
-    if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_THIN_DIELECTRIC) return fail(PRT_ERR_INVALID, "unknown material kind");
+    if (s.kind < PRT_KIND_DIFFUSE || s.kind > PRT_KIND_STANDARD) return fail(PRT_ERR_INVALID, "unknown material kind");
     if (!(s.transparency >= 0.f && s.transparency <= 1.f)) return fail(PRT_ERR_INVALID, "transparency must be in [0, 1]");
+    for (int j = 0; j < 3; ++j)
+      if (!(s.emission[j] >= 0.f)) return fail(PRT_ERR_INVALID, "emission must be non-negative");
+    LobeParams p;
+    p.kind = s.kind;
+    for (int j = 0; j < 3; ++j)
+    {
+      p.rgb[j] = hclamp((&s.r)[j], 0.f, 1.f);
+      p.edge[j] = hclamp((&s.er)[j], 0.f, 1.f);
+    }
+    p.roughness = hclamp(s.roughness, 0.f, 1.f);
+    p.diffRough = hclamp(s.diffuse_roughness, 0.f, 1.f);
+    const float ior = foldIor(s.ior);
+    bool mediaInterface = false;
+    if (s.kind == PRT_KIND_COATED)
+    {
+      // material/standard.cpp:512-518, 580-597; opaque, so never a media interface: eta = 1 / ior
+      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "coat IOR must differ from the ambient IOR 1");
+      p.etaArg[0] = p.etaArg[1] = hclamp(1.0f / ior, 1.f / 3.f, 3.f);
+      p.thickness = std::max(s.thickness, 0.f);
+      p.scale = hclamp(s.weight, 0.f, 1.f);
+      for (int j = 0; j < 3; ++j) p.coatColor[j] = p.edge[j];
+    }
+    else if (s.kind == PRT_KIND_THIN_DIELECTRIC)
+    {
+      // thin-walled, so always seen from outside: eta = ambientIor / specularIor (standard.cpp:429)
+      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "dielectric IOR must differ from the ambient IOR 1");
+      p.etaArg[0] = p.etaArg[1] = hclamp(1.0f / ior, 1.f / 3.f, 3.f);
+      p.thickness = std::max(s.thickness, 0.f);
+    }
+    else if (s.kind == PRT_KIND_DIELECTRIC)
+    {
+      // material/standard.cpp:436-437: outside eta = 1/ior, inside eta = ior, clamped to [1/3, 3]
+      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "dielectric IOR must differ from the ambient IOR 1");
+      p.etaArg[0] = hclamp(1.0f / ior, 1.f / 3.f, 3.f);
+      p.etaArg[1] = hclamp(ior, 1.f / 3.f, 3.f);
+      mediaInterface = true;
+    }
+    if (s.kind != PRT_KIND_STANDARD) lobeConstants(c->luts, p, q);
+    else
+    {
+      // StandardMaterial's bases (material/standard.cpp:409-566, ambient IOR 1), one set of base
+      // records per orientation (fromOutside = thinWalled || !backfacing)
+      if (!(s.ior > 0.f) || !(s.coat_ior > 0.f)) return fail(PRT_ERR_INVALID, "IORs must be positive");
+      const bool thin = s.thin_walled != 0;
+      const float base = hclamp(s.base_weight, 0.f, 1.f), metalness = hclamp(s.metalness, 0.f, 1.f);
+      const float specular = hclamp(s.specular_weight, 0.f, 1.f);
+      const float transmission = hclamp(s.transmission_weight, 0.f, 1.f);
+      mediaInterface = s.transmission_weight != 0.f && !thin;  // MaterialType::MediaInterface (:159-175)
+      q.kind = PRT_KIND_STANDARD;
+      q.importance = 0.f;
+      int flagsDiffuseOnly = 1;
+      for (int o = 0; o < 2; ++o)
+      {
+        const bool fromOutside = thin || o == 0;
+        const float eta = hclamp(fromOutside ? 1.0f / ior : ior, 1.f / 3.f, 3.f);
+        std::vector<std::pair<LobeParams, float>> kids;
+        const float dielectric = 1.f - metalness;
+        if (dielectric > eps)
+        {
+          const float opaque = dielectric * (1.f - transmission);
+          if (opaque > eps)  // (:428-505): diffuse, under a specular coating where the IOR differs
+          {
+            LobeParams c0 = p;
+            for (int j = 0; j < 3; ++j) c0.rgb[j] = p.rgb[j] * base;
+            if (std::fabs(ior - 1.f) > eps && specular > eps)
+            {
+              c0.kind = PRT_KIND_COATED;
+              c0.etaArg[0] = c0.etaArg[1] = eta;
+              c0.thickness = 1.f;
+              c0.scale = specular;
+              c0.coatColor[0] = c0.coatColor[1] = c0.coatColor[2] = 1.f;
+            }
+            else c0.kind = PRT_KIND_DIFFUSE;
+            kids.push_back({c0, opaque});
+          }
+          const float translucent = dielectric * transmission * specular;
+          if (translucent > eps)  // (:507-530)
+          {
+            LobeParams c1 = p;
+            if (std::fabs(ior - 1.f) > eps)
+            {
+              if (thin)
+              {
+                c1.kind = PRT_KIND_THIN_DIELECTRIC;
+                c1.etaArg[0] = c1.etaArg[1] = eta;
+                for (int j = 0; j < 3; ++j) c1.rgb[j] = hclamp(s.transmission_color[j], 0.f, 1.f);
+                c1.thickness = std::max(s.thickness, 0.f) / std::max(s.transmission_depth, 0.f);
+              }
+              else
+              {
+                c1.kind = PRT_KIND_DIELECTRIC;
+                c1.etaArg[0] = c1.etaArg[1] = eta;
+              }
+            }
+            else c1.kind = PRT_KIND_TRANSPARENT;
+            kids.push_back({c1, translucent});
+          }
+        }
+        const float metal = metalness * specular;
+        if (metal > eps)  // (:533-546)
+        {
+          LobeParams c2 = p;
+          c2.kind = PRT_KIND_CONDUCTOR;
+          for (int j = 0; j < 3; ++j) c2.rgb[j] = p.rgb[j] * base;
+          kids.push_back({c2, metal});
+        }
+        q.nKids = int(kids.size());
+        const size_t first = extra.size();
+        q.kidBase[o] = int(n + first) - i;  // relative to this record
+        float impSum = 0.f;
+        int anySmooth = 0;
+        for (size_t t = 0; t < kids.size(); ++t)
+        {
+          prt_material_constants kq;
+          std::memset(&kq, 0, sizeof(kq));
+          lobeConstants(c->luts, kids[t].first, kq);
+          kq.opacity = 1.f;
+          q.kidScale[t] = kids[t].second;
+          q.kidImp[t] = kq.importance * kids[t].second;
+          impSum += q.kidImp[t];
+          anySmooth |= kq.smooth;
+          if (kids[t].first.kind != PRT_KIND_DIFFUSE) flagsDiffuseOnly = 0;
+          extra.push_back(kq);
+        }
+        q.impSum = impSum;
+        q.smooth = anySmooth;
+        // the coat over the mix (:551-566)
+        const float coat = hclamp(s.coat_weight, 0.f, 1.f), coatIor = foldIor(s.coat_ior);
+        q.coat = (coat > eps && std::fabs(coatIor - 1.f) > eps) ? 1 : 0;
+        if (q.coat)
+        {
+          const float etaArg = hclamp(fromOutside ? 1.0f / coatIor : coatIor, 1.f / 3.f, 3.f);
+          q.coatRoughness = hclamp(s.coat_roughness, 0.f, 1.f);
+          q.coatAlpha = std::max(hsqr(q.coatRoughness), 0.001f);
+          q.coatEta[o] = etaArg <= 1.f ? etaArg : 1.0f / etaArg;
+          q.coatRavg[o] = dielReflAvg(c->luts, etaArg, q.coatRoughness) * coat;
+          q.coatScale = coat;
+          q.coatThickness = std::max(s.coat_thickness, 0.f);
+          for (int j = 0; j < 3; ++j) q.coatColor[j] = hclamp(s.coat_color[j], 0.f, 1.f);
+          q.smooth = 1;
+        }
+        q.importance = q.coat ? std::max(1.f, impSum) : impSum;  // rough_coating.h:562-565
+        // the albedo AOV of the mix (multi_bsdf.h:314-337), coated (rough_coating.h:567-571)
+        float alb[3] = {0.f, 0.f, 0.f}, scaleSum = 0.f, total = 0.f;
+        for (size_t t = 0; t < kids.size(); ++t)
+        {
+          const prt_material_constants& kq = extra[first + t];
+          if (!(q.kidImp[t] > 0.f)) continue;
+          total += q.kidScale[t];
+          if (kq.kind == PRT_KIND_TRANSPARENT) continue;
+          for (int j = 0; j < 3; ++j) alb[j] += (kq.kind == PRT_KIND_DIFFUSE ? kq.c0[j] : (kq.kind == PRT_KIND_DIELECTRIC ? 1.f : kq.rr[j])) * q.kidScale[t];
+          scaleSum += q.kidScale[t];
+        }
+        for (int j = 0; j < 3; ++j)
+        {
+          float a = alb[j] * (total * (scaleSum == 0.f ? 3.402823466e38f : 1.0f / scaleSum));
+          if (q.coat) a = a + q.coatScale * (a * std::pow(q.coatColor[j], q.coatThickness) - a);
+          if (o == 0) q.rr[j] = a;  // the same in both orientations
+        }
+      }
+      q.mixDiffuseOnly = flagsDiffuseOnly;
+      q.roughness = q.coatRoughness;
+      q.alpha = q.coat ? q.coatAlpha : 0.001f;
+      q.diffuseRoughness = p.diffRough;
+    }
     // opacity (material/standard.cpp:412-420, :613-627, eps = 1e-5, standard.h:79)
     q.opacity = hclamp(1.f - s.transparency, 0.f, 1.f);
     q.transp = 1.f - q.opacity;
     q.mix = q.transp > 1e-5f ? (q.opacity > 1e-5f ? 1 : 2) : 0;
-    q.transparent = (s.transparency != 0.f || s.kind == PRT_KIND_THIN_DIELECTRIC) ? 1 : 0;  // :152-166
-    q.diffuseRoughness = hclamp(s.diffuse_roughness, 0.f, 1.f);
+    // MaterialType::Transparent (:152-166): opacity below 1, or thin-walled transmission
+    q.transparent = (s.transparency != 0.f || s.kind == PRT_KIND_THIN_DIELECTRIC ||
+                     (s.kind == PRT_KIND_STANDARD && s.thin_walled && s.transmission_weight != 0.f)) ? 1 : 0;
     // sheen (standard.cpp:599-611): colour and roughness clamped, alpha = roughnessToAlpha
     q.sheenScale = hclamp(s.sheen_weight, 0.f, 1.f);
     q.sheen = q.sheenScale > 1e-5f ? 1 : 0;
     for (int j = 0; j < 3; ++j) q.sheenColor[j] = hclamp(s.sheen_color[j], 0.f, 1.f);
     q.sheenRoughness = hclamp(s.sheen_roughness, 0.f, 1.f);
     q.sheenAlpha = std::max(hsqr(q.sheenRoughness), 0.001f);
-    q.kind = s.kind;
-    q.roughness = hclamp(s.roughness, 0.f, 1.f);
-    q.alpha = std::max(hsqr(q.roughness), 0.001f);  // bsdf/microfacet.h:10-24
-    q.importance = 1.f;
-    for (int j = 0; j < 3; ++j)
-    {
-      if (!(s.emission[j] >= 0.f)) return fail(PRT_ERR_INVALID, "emission must be non-negative");
-      q.emission[j] = s.emission[j];
-    }
+    if (q.sheen) q.smooth = 1;
+    for (int j = 0; j < 3; ++j) q.emission[j] = s.emission[j];
     q.emissive = (s.emission[0] > 0.f || s.emission[1] > 0.f || s.emission[2] > 0.f) ? 1 : 0;
-    float rgb[3] = {hclamp(s.r, 0.f, 1.f), hclamp(s.g, 0.f, 1.f), hclamp(s.b, 0.f, 1.f)};
-    if (s.kind == PRT_KIND_DIFFUSE)
-    {
-      for (int j = 0; j < 3; ++j) q.c0[j] = rgb[j];
-      q.importance = std::max(std::max(rgb[0], rgb[1]), rgb[2]);  // bsdf/diffuse.h:72-75
-    }
-    else if (s.kind == PRT_KIND_CONDUCTOR)
-    {
-      float edge[3] = {hclamp(s.er, 0.f, 1.f), hclamp(s.eg, 0.f, 1.f), hclamp(s.eb, 0.f, 1.f)};
-      q.Eavg = lutInterp1(q.roughness * 31.f, c->luts.data() + PRT_LUT_GGX_AVG, 32);
-      for (int j = 0; j < 3; ++j)
-      {
-        // bsdf/optics.h:214-240 (Gulbrandsen + Kulla-Conty average)
-        float r = std::min(rgb[j], 0.99f), g = edge[j];
-        float nMin = (1.f - r) / (1.f + r), nMax = (1.f + std::sqrt(r)) / (1.f - std::sqrt(r));
-        float nn = g * nMin + (1.f - g) * nMax;
-        float k2 = (hsqr(nn + 1.f) * r - hsqr(nn - 1.f)) / (1.f - r);
-        q.c0[j] = nn;
-        q.c1[j] = std::sqrt(std::max(k2, 0.f));
-        float rr = rgb[j], r2 = rr * rr, r3 = r2 * rr, g2 = g * g, g3 = g2 * g;
-        float Favg = 0.087237f + 0.0230685f * g - 0.0864902f * g2 + 0.0774594f * g3 + 0.782654f * rr - 0.136432f * r2 +
-                     0.278708f * r3 + 0.19744f * g * rr + 0.0360605f * g2 * rr - 0.2586f * g * r2;
-        q.c2[j] = hsqr(Favg) * q.Eavg / (1.f - Favg * (1.f - q.Eavg));  // rough_conductor.h:127-129
-        q.rr[j] = rgb[j];
-        q.edge[j] = g;
-        q.Favg[j] = Favg;
-      }
-    }
-    else if (s.kind == PRT_KIND_COATED)
-    {
-      // RoughCoatingBsdf over a diffuse base (material/standard.cpp:512-518, 580-597,
-      // bsdf/rough_coating.h:287-295).  Opaque, so never a media interface: eta = 1 / ior.
-      float ior = s.ior < 1.f ? 1.0f / s.ior : s.ior;
-      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "coat IOR must differ from the ambient IOR 1");
-      float eta = hclamp(1.0f / ior, 1.f / 3.f, 3.f);
-      float thickness = std::max(s.thickness, 0.f), scale = hclamp(s.weight, 0.f, 1.f);
-      float coat[3] = {hclamp(s.er, 0.f, 1.f), hclamp(s.eg, 0.f, 1.f), hclamp(s.eb, 0.f, 1.f)};
-      float baseImportance = std::max(std::max(rgb[0], rgb[1]), rgb[2]);  // bsdf/diffuse.h:70-73
-      for (int j = 0; j < 3; ++j)
-      {
-        q.c0[j] = rgb[j];
-        q.c1[j] = coat[j];
-        // getAlbedo (rough_coating.h:567-571): lerp(base, base * coat^thickness, scale)
-        q.rr[j] = rgb[j] + scale * (rgb[j] * std::pow(coat[j], thickness) - rgb[j]);
-      }
-      q.c2[0] = thickness;
-      q.c2[1] = scale;
-      q.c2[2] = baseImportance;
-      q.eta[0] = q.eta[1] = eta <= 1.f ? eta : 1.0f / eta;
-      q.Eavg = dielReflAvg(c->luts, eta, q.roughness) * scale;  // Ravg
-      q.importance = std::max(1.f, baseImportance);              // rough_coating.h:562-565
-    }
-    else if (s.kind == PRT_KIND_THIN_DIELECTRIC)
-    {
-      // ThinRoughDielectricBsdf (bsdf/thin_rough_dielectric.h:180-192); thin-walled, so always
-      // seen from outside: eta = ambientIor / specularIor (material/standard.cpp:429, :433-434)
-      float ior = s.ior < 1.f ? 1.0f / s.ior : s.ior;
-      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "dielectric IOR must differ from the ambient IOR 1");
-      float eta = hclamp(1.0f / ior, 1.f / 3.f, 3.f);
-      float thickness = std::max(s.thickness, 0.f);
-      q.Eavg = lutInterp1(q.roughness * 31.f, c->luts.data() + PRT_LUT_GGX_AVG, 32);
-      float Favg = fresnelAvg(eta);
-      q.c1[0] = Favg * (1.f - q.Eavg) / (1.f - Favg * q.Eavg);  // fmsScale
-      q.c2[0] = thickness;
-      q.eta[0] = q.eta[1] = eta;
-      for (int j = 0; j < 3; ++j)
-      {
-        q.c0[j] = rgb[j];
-        q.rr[j] = std::pow(rgb[j], thickness);  // getAlbedo (:327-330)
-      }
-    }
-    else
-    {
-      float ior = s.ior < 1.f ? 1.0f / s.ior : s.ior;
-      if (!(ior > 1.f)) return fail(PRT_ERR_INVALID, "dielectric IOR must differ from the ambient IOR 1");
-      for (int o = 0; o < 2; ++o)
-      {
-        // material/standard.cpp:436-437: outside eta = 1/ior, inside eta = ior, clamped to [1/3, 3]
-        float eta = hclamp(o == 0 ? 1.0f / ior : ior, 1.f / 3.f, 3.f);
-        float Ee = dielAvg(c->luts, eta, q.roughness), Er = dielAvg(c->luts, 1.0f / eta, q.roughness);
-        float Fe = fresnelAvg(eta), Fr = fresnelAvg(1.0f / eta);
-        float a = (1.f - Fe) / (1.f - Er), b = (1.f - Fr) / (1.f - Ee);
-        float x = b / (a + b);
-        q.eta[o] = eta;
-        q.EavgEta[o] = Ee;
-        q.EavgRcpEta[o] = Er;
-        q.fmsRatio[o] = 1.f - x * (1.f - Fe);  // rough_dielectric.h:283-291
-        q.FavgEta[o] = Fe;
-        q.FavgRcpEta[o] = Fr;
-      }
-      q.mediumInt = mediumOf(media, ior, s);
-    }
+    q.mediaInterface = mediaInterface ? 1 : 0;
+    q.transmissive = (s.kind == PRT_KIND_DIELECTRIC || s.kind == PRT_KIND_THIN_DIELECTRIC ||
+                      (s.kind == PRT_KIND_STANDARD && s.transmission_weight != 0.f)) ? 1 : 0;
+    if (mediaInterface) q.mediumInt = mediumOf(media, ior, s);
   }
-  c->anyTransparent = c->extMaterials = 0;
+  c->anyTransparent = c->extMaterials = c->stdMaterials = 0;
   for (int i = 0; i < n; ++i)
   {
     c->anyTransparent |= k[i].transparent;
-    c->extMaterials |=
-        (k[i].kind == PRT_KIND_THIN_DIELECTRIC || k[i].mix != 0 || k[i].diffuseRoughness > 0.f || k[i].sheen) ? 1 : 0;
+    c->stdMaterials |= k[i].kind == PRT_KIND_STANDARD ? 1 : 0;
+    c->extMaterials |= (k[i].kind == PRT_KIND_THIN_DIELECTRIC || k[i].kind == PRT_KIND_STANDARD || k[i].mix != 0 ||
+                        k[i].diffuseRoughness > 0.f || k[i].sheen) ? 1 : 0;
   }
-  HIP_TRY(c->mats.alloc(n));
-  HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * n, hipMemcpyHostToDevice));
+  for (size_t t = 1; t < media.size(); ++t)  // a coloured medium absorbs (the base kernel skips that)
+    if (media[t].x != 1.f || media[t].y != 1.f || media[t].z != 1.f) c->extMaterials = 1;
+  k.insert(k.end(), extra.begin(), extra.end());
+  HIP_TRY(c->mats.alloc(k.size()));
+  HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * k.size(), hipMemcpyHostToDevice));
   HIP_TRY(c->media.alloc(media.size()));
   HIP_TRY(hipMemcpy(c->media.p, media.data(), sizeof(float4) * media.size(), hipMemcpyHostToDevice));
   c->lightsReady = false;  // the area light depends on the materials: set the lights again
@@ -1228,9 +1415,9 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
     {
       launchIntersect(ad, depth, c->launch, c->countSteps, c->stream);
       HIP_TRY(hipEventRecord(ev[2 + 3 * depth], c->stream));
-      launchShade(ad, depth, c->sampleIndex, classB, c->launch, c->extMaterials != 0, c->stream);
+      launchShade(ad, depth, c->sampleIndex, classB, c->launch, c->stdMaterials ? 2 : (c->extMaterials ? 1 : 0), c->stream);
       HIP_TRY(hipEventRecord(ev[3 + 3 * depth], c->stream));
-      launchOccluded(ad, depth, classB, c->launch, c->anyTransparent != 0, c->countSteps, c->stream);
+      launchOccluded(ad, depth, classB, c->launch, c->anyTransparent ? (c->stdMaterials ? 2 : 1) : 0, c->countSteps, c->stream);
       HIP_TRY(hipEventRecord(ev[4 + 3 * depth], c->stream));
       if (depth + 1 >= kFreeDepths && depth < a.maxDepth)
       {
@@ -1868,8 +2055,10 @@ bool batchStale(const prt_batch* b)
 
 void batchLaunch(prt_batch* b)
 {
+  bool std = false;
+  for (const auto& f : b->frames) std = std || f.scene->stdMaterials != 0;
   launchBatch(b->fdev.p, int(b->frames.size()), b->nTiles, b->counter.p, b->tilesPerFetch, b->blocksPerCu * b->cus, b->spill.p,
-              b->spillStride, b->samples.p, b->stream);
+              b->spillStride, b->samples.p, std, b->stream);
 }
 }  // namespace
 

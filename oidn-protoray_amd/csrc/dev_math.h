@@ -458,9 +458,17 @@ struct Bsdf
   bool sheen;                 // a sheen layer over the lobe (bsdf/sheen.h)
   f3 sheenColor;
   float sheenScale, sheenRough, sheenA;
-  // false: the scene has no thin dielectric, opacity, Oren-Nayar base or sheen; a compile-time
-  // constant in the kernels built for such scenes, where it folds those branches away
+  // false: the scene has no thin dielectric, opacity, Oren-Nayar base, sheen or kind-5 material; a
+  // compile-time constant in the kernels built for such scenes, where it folds those branches away
   bool ext;
+  bool stdk;    // false: no kind-5 material in the scene (compile-time in the kernels built without them)
+  bool smooth;  // a diffuse or glossy lobe: next-event estimation applies
+  // kind 5: the mix of bases (the records kids[0 .. stdm->nKids) with stdm's scales and
+  // importances) and whether the coat covers it (the coated fields hold the coat: c1 colour,
+  // c2 = (thickness, scale, mix importance), Eavg = Ravg, eta, roughness, ggx); rr the albedo
+  const prt_material_constants* stdm;
+  const prt_material_constants* kids;
+  bool coat;
 };
 
 // Importance of the surface: the lobe's, or under a sheen layer the larger of the sheen
@@ -487,7 +495,14 @@ PRT_DEV f3 bsdfAlbedo(const Bsdf& b)
 // transparent BTDF has no frame.
 PRT_DEV f3 bsdfNormal(const Bsdf& b)
 {
-  return (b.mix != 2 && b.importance > 0.f && (b.mix == 0 || surfImportance(b) * b.opacity > 0.f)) ? b.fr.N : f3(0.f);
+  bool lobe = b.importance > 0.f;
+  if (b.stdk && b.kind == PRT_KIND_STANDARD)  // kind 5: a base with an importance and a frame
+  {
+    lobe = false;
+    for (int t = 0; t < b.stdm->nKids; ++t)
+      lobe = lobe || (b.stdm->kidImp[t] > 0.f && b.kids[t].kind != PRT_KIND_TRANSPARENT);
+  }
+  return (b.mix != 2 && lobe && (b.mix == 0 || surfImportance(b) * b.opacity > 0.f)) ? b.fr.N : f3(0.f);
 }
 
 // bsdf/diffuse.h:55-68, or with diffRough > 0 the Oren-Nayar BRDF in Fujii's form
@@ -658,9 +673,10 @@ PRT_DEV float refractCos(float cosI, float eta)  // optics.h:50-56
 {
   return sqrtf(fmaxf(1.f - eta * eta * (1.f - cosI * cosI), 0.f));
 }
-// absorption through the coat along the refracted path and energy conservation of a diffuse
-// base [Kelemen and Szirmay-Kalos 2001] (:331-350)
-PRT_DEV f3 coatBase(const float* lut, const Bsdf& b, f3 baseValue, float cosO, float cosI, float Ro)
+// absorption through the coat along the refracted path and energy conservation (:331-350): the
+// term of Kelemen and Szirmay-Kalos 2001 for a base with diffuse lobes only, min(1 - Ro, 1 - Ri)
+// for any other base (Kulla and Conty 2017)
+PRT_DEV f3 coatBase(const float* lut, const Bsdf& b, f3 baseValue, float cosO, float cosI, float Ro, bool diffuseOnly)
 {
   float length = 1.0f / refractCos(cosO, b.eta) + 1.0f / refractCos(fabsf(cosI), b.eta);
   if (cosI <= 0.f) length *= 0.5f;
@@ -668,7 +684,7 @@ PRT_DEV f3 coatBase(const float* lut, const Bsdf& b, f3 baseValue, float cosO, f
   f3 absorbed(baseValue.x * powf(b.c1.x, t), baseValue.y * powf(b.c1.y, t), baseValue.z * powf(b.c1.z, t));
   baseValue = baseValue + b.c2.y * (absorbed - baseValue);
   float Ri = dielReflAlbedo(lut, fabsf(cosI), b.eta, b.roughness) * b.c2.y;
-  return baseValue * ((1.f - Ro) * (1.f - Ri) * (1.0f / (1.f - b.Eavg)));
+  return baseValue * (diffuseOnly ? (1.f - Ro) * (1.f - Ri) * (1.0f / (1.f - b.Eavg)) : fminf(1.f - Ro, 1.f - Ri));
 }
 PRT_DEV float coatPickProb(const Bsdf& b, float Ro)
 {
@@ -687,7 +703,10 @@ PRT_DEV float coatReflection(const Bsdf& b, f3 wo, f3 wi, f3 wh, float cosO, flo
   coatPdf = whPdf * (1.0f / (4.f * fabsf(cosOH)));
   return F * D * G * (1.0f / (4.f * cosO));
 }
-PRT_DEV f3 coatEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+// RoughCoatingBsdfSimd over a base (:297-537).  Base: eval(wo, wi, pdf), sample(wo, wi, pdf, type,
+// sx, sy, sc), diffuseOnly(), and kDelta (whether a base sample can be a delta).
+template <class Base>
+PRT_DEV f3 coatEvalOver(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf, const Base& base)
 {
   pdf = 0.f;
   float cosO = dot(wo, b.fr.N);
@@ -695,9 +714,9 @@ PRT_DEV f3 coatEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi,
   float cosI = dot(wi, b.fr.N);
   if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
   float basePdf;
-  f3 baseValue = diffuseEval(b, wo, wi, basePdf);
+  f3 baseValue = base.eval(wo, wi, basePdf);
   float Ro = dielReflAlbedo(lut, cosO, b.eta, b.roughness) * b.c2.y;
-  baseValue = coatBase(lut, b, baseValue, cosO, cosI, Ro);
+  baseValue = coatBase(lut, b, baseValue, cosO, cosI, Ro, base.diffuseOnly());
   float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
   if (cosI > 0.f)
   {
@@ -709,7 +728,9 @@ PRT_DEV f3 coatEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi,
   pdf = basePdf * basePick;
   return baseValue;
 }
-PRT_DEV f3 coatSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+template <class Base>
+PRT_DEV f3 coatSampleOver(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy,
+                          float sc, const Base& base)
 {
   float cosO = dot(wo, b.fr.N);
   if (!(cosO > 0.f)) return f3(0.f);
@@ -725,19 +746,23 @@ PRT_DEV f3 coatSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
     wi = reflect(wo, wh, dot(wo, wh));
     if (!(dot(wi, b.fr.N) > 0.f)) return f3(0.f);
     type = BsdfType::GlossyReflection;
-    baseValue = diffuseEval(b, wo, wi, basePdf);
+    baseValue = base.eval(wo, wi, basePdf);
   }
   else
   {
-    // the diffuse base reads only the 2D part of the remapped sample (:424-427)
-    f3 bw = diffuseSample(b, c, wo, wi, basePdf, type, sx, sy);
+    f3 bw = base.sample(wo, wi, basePdf, type, sx, sy, (sc - coatPick) * (1.0f / basePick));
     if (!(reduceMax(bw) > 0.f)) return f3(0.f);
-    baseValue = bw * basePdf;
+    baseValue = (Base::kDelta && (type & BsdfType::Delta)) ? bw : bw * basePdf;
     wh = normalize(wo + wi);
   }
   float cosI = dot(wi, b.fr.N);
   if (!(cosI * dot(wi, c.Ng) > 0.f)) return f3(0.f);
-  baseValue = coatBase(lut, b, baseValue, cosO, cosI, Ro);
+  baseValue = coatBase(lut, b, baseValue, cosO, cosI, Ro, base.diffuseOnly());
+  if (Base::kDelta && (type & BsdfType::Delta))  // a delta base sample: no coat reflection (:484-490)
+  {
+    pdf = basePdf;
+    return baseValue * (1.0f / basePick);
+  }
   if (cosI <= 0.f)
   {
     pdf = basePick * basePdf;
@@ -747,6 +772,27 @@ PRT_DEV f3 coatSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& 
   float coatValue = coatReflection(b, wo, wi, wh, cosO, coatPdf);
   pdf = coatPick * coatPdf + basePick * basePdf;
   return (f3(coatValue) + baseValue) * (1.0f / pdf);
+}
+// the coated kind's diffuse base: it reads only the 2D part of the remapped sample (:424-427)
+struct DiffuseBase
+{
+  static constexpr bool kDelta = false;
+  const Bsdf& b;
+  const Ctx& c;
+  PRT_DEV f3 eval(f3 wo, f3 wi, float& pdf) const { return diffuseEval(b, wo, wi, pdf); }
+  PRT_DEV f3 sample(f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float) const
+  {
+    return diffuseSample(b, c, wo, wi, pdf, type, sx, sy);
+  }
+  PRT_DEV bool diffuseOnly() const { return true; }
+};
+PRT_DEV f3 coatEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  return coatEvalOver(lut, b, c, wo, wi, pdf, DiffuseBase{b, c});
+}
+PRT_DEV f3 coatSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  return coatSampleOver(lut, b, c, wo, wi, pdf, type, sx, sy, sc, DiffuseBase{b, c});
 }
 
 // --- thin rough dielectric (bsdf/thin_rough_dielectric.h:170-345): c0 = transmission color,
@@ -825,19 +871,154 @@ PRT_DEV f3 thinTransparency(const float* lut, const Bsdf& b, f3 wo)
   return thinTransmittance(b, cosO) * (1.f - R);
 }
 
-// The lobe: a MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210)
+// One lobe of a single kind, or the transparent BTDF (bsdf/transparent.h:32-71: evaluates to zero,
+// samples -wo with pdf 1).
+PRT_DEV f3 leafEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  pdf = 0.f;
+  if (b.kind == PRT_KIND_DIFFUSE) return diffuseEval(b, wo, wi, pdf);
+  if (b.kind == PRT_KIND_CONDUCTOR) return conductorEval(lut, b, wo, wi, pdf);
+  if (b.kind == PRT_KIND_COATED) return coatEval(lut, b, c, wo, wi, pdf);
+  if (b.ext && b.kind == PRT_KIND_THIN_DIELECTRIC) return thinEval(lut, b, c, wo, wi, pdf);
+  if (b.ext && b.kind == PRT_KIND_TRANSPARENT) return f3(0.f);
+  return dielEval(lut, b, c, wo, wi, pdf);
+}
+PRT_DEV f3 leafSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  if (b.kind == PRT_KIND_DIFFUSE) return diffuseSample(b, c, wo, wi, pdf, type, sx, sy);
+  if (b.kind == PRT_KIND_CONDUCTOR) return conductorSample(lut, b, c, wo, wi, pdf, type, sx, sy);
+  if (b.kind == PRT_KIND_COATED) return coatSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+  if (b.ext && b.kind == PRT_KIND_THIN_DIELECTRIC) return thinSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+  if (b.ext && b.kind == PRT_KIND_TRANSPARENT)
+  {
+    wi = -wo;
+    type = BsdfType::Transparency;
+    pdf = 1.f;
+    return f3(1.f);
+  }
+  return dielSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+}
+
+// Base t of a kind-5 mix as a lobe on the parent's frame.
+PRT_DEV Bsdf kidBsdf(const Bsdf& parent, int t)
+{
+  const prt_material_constants& k = parent.kids[t];
+  Bsdf b;
+  b.fr = parent.fr;
+  b.kind = k.kind;
+  b.roughness = k.roughness;
+  b.ggx.a = k.alpha;
+  b.importance = k.importance;
+  b.c0 = f3(k.c0[0], k.c0[1], k.c0[2]);
+  b.c1 = f3(k.c1[0], k.c1[1], k.c1[2]);
+  b.c2 = f3(k.c2[0], k.c2[1], k.c2[2]);
+  b.rr = f3(k.rr[0], k.rr[1], k.rr[2]);
+  b.Eavg = k.Eavg;
+  b.eta = k.eta[0];  // each orientation has its own base records
+  b.EavgEta = k.EavgEta[0];
+  b.EavgRcpEta = k.EavgRcpEta[0];
+  b.fmsRatio = k.fmsRatio[0];
+  b.diffRough = k.diffuseRoughness;
+  b.mix = 0;
+  b.sheen = false;
+  b.ext = true;
+  b.stdk = false;
+  b.smooth = k.smooth != 0;
+  b.stdm = nullptr;
+  b.kids = nullptr;
+  b.coat = false;
+  return b;
+}
+
+// kind 5's weighted mix of bases (multi_bsdf.h:166-287): eval sums the bases' values times their
+// scales and their pdfs times their importances; sample picks a base by importance with the
+// remapped sample, then (unless the pick is a delta) adds the other bases' values and pdfs.
+PRT_DEV f3 mixEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
+{
+  const prt_material_constants& m = *b.stdm;
+  f3 value(0.f);
+  pdf = 0.f;
+  for (int t = 0; t < m.nKids; ++t)
+  {
+    if (!(m.kidImp[t] > 0.f)) continue;
+    float cp;
+    f3 cv = leafEval(lut, kidBsdf(b, t), c, wo, wi, cp);
+    value = value + cv * m.kidScale[t];
+    pdf += cp * m.kidImp[t];
+  }
+  pdf *= 1.0f / m.impSum;
+  return value;
+}
+PRT_DEV f3 mixSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  const prt_material_constants& m = *b.stdm;
+  if (!(m.impSum > 0.f)) return f3(0.f);
+  if (m.nKids == 1) return leafSample(lut, kidBsdf(b, 0), c, wo, wi, pdf, type, sx, sy, sc) * m.kidScale[0];
+  const float x = sc * m.impSum;
+  int choice = 0;
+  float prefix = m.kidImp[0], imp = m.kidImp[0], scale = m.kidScale[0];
+  for (int t = 1; t < m.nKids; ++t)
+  {
+    if (!(x > prefix)) break;
+    choice = t;
+    prefix += m.kidImp[t];
+    imp = m.kidImp[t];
+    scale = m.kidScale[t];
+  }
+  f3 cwi(0.f);
+  float cpdf = 0.f;
+  int ctype = 0;
+  f3 cw = leafSample(lut, kidBsdf(b, choice), c, wo, cwi, cpdf, ctype, sx, sy, (x + imp - prefix) * (1.0f / imp));
+  f3 w(0.f);
+  wi = f3(0.f);
+  pdf = 0.f;
+  type = 0;
+  if (reduceMax(cw) > 0.f) { w = cw; wi = cwi; pdf = cpdf; type = ctype; }
+  w = w * scale;
+  if (!(reduceMax(w) > 0.f && pdf > 0.f)) return f3(0.f);
+  if (type & BsdfType::Delta) return w * (m.impSum * (1.0f / imp));
+  f3 value = w * pdf;
+  pdf = pdf * imp;
+  for (int t = 0; t < m.nKids; ++t)
+  {
+    if (t == choice || !(m.kidImp[t] > 0.f)) continue;
+    float cp;
+    f3 cv = leafEval(lut, kidBsdf(b, t), c, wo, wi, cp);
+    value = value + cv * m.kidScale[t];
+    pdf += cp * m.kidImp[t];
+  }
+  pdf *= 1.0f / m.impSum;
+  return value * (1.0f / pdf);
+}
+struct MixBase
+{
+  static constexpr bool kDelta = true;
+  const float* lut;
+  const Bsdf& b;
+  const Ctx& c;
+  PRT_DEV f3 eval(f3 wo, f3 wi, float& pdf) const { return mixEval(lut, b, c, wo, wi, pdf); }
+  PRT_DEV f3 sample(f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc) const
+  {
+    return mixSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+  }
+  PRT_DEV bool diffuseOnly() const { return b.stdm->mixDiffuseOnly != 0; }  // rough_coating.h:349
+};
+
+// The material's lobe: a MultiBsdf with one child of weight 1 for kinds 0-4
+// (bsdf/multi_bsdf.h:166-210), kind 5's mix of bases under its coat when it has one.
 PRT_DEV f3 lobeEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi, float& pdf)
 {
+  if (b.stdk && b.kind == PRT_KIND_STANDARD)
+  {
+    const MixBase base{lut, b, c};
+    return b.coat ? coatEvalOver(lut, b, c, wo, wi, pdf, base) : base.eval(wo, wi, pdf);
+  }
   f3 v(0.f);
   pdf = 0.f;
   if (b.importance > 0.f)
   {
     float cp;
-    if (b.kind == PRT_KIND_DIFFUSE) v = diffuseEval(b, wo, wi, cp);
-    else if (b.kind == PRT_KIND_CONDUCTOR) v = conductorEval(lut, b, wo, wi, cp);
-    else if (b.kind == PRT_KIND_COATED) v = coatEval(lut, b, c, wo, wi, cp);
-    else if (b.ext && b.kind == PRT_KIND_THIN_DIELECTRIC) v = thinEval(lut, b, c, wo, wi, cp);
-    else v = dielEval(lut, b, c, wo, wi, cp);
+    v = leafEval(lut, b, c, wo, wi, cp);
     pdf = cp * b.importance;
   }
   pdf *= 1.0f / b.importance;
@@ -845,12 +1026,13 @@ PRT_DEV f3 lobeEval(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3 wi,
 }
 PRT_DEV f3 lobeSample(const float* lut, const Bsdf& b, const Ctx& c, f3 wo, f3& wi, float& pdf, int& type, float sx, float sy, float sc)
 {
+  if (b.stdk && b.kind == PRT_KIND_STANDARD)
+  {
+    const MixBase base{lut, b, c};
+    return b.coat ? coatSampleOver(lut, b, c, wo, wi, pdf, type, sx, sy, sc, base) : base.sample(wo, wi, pdf, type, sx, sy, sc);
+  }
   if (!(b.importance > 0.f)) return f3(0.f);
-  if (b.kind == PRT_KIND_DIFFUSE) return diffuseSample(b, c, wo, wi, pdf, type, sx, sy);
-  if (b.kind == PRT_KIND_CONDUCTOR) return conductorSample(lut, b, c, wo, wi, pdf, type, sx, sy);
-  if (b.kind == PRT_KIND_COATED) return coatSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
-  if (b.ext && b.kind == PRT_KIND_THIN_DIELECTRIC) return thinSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
-  return dielSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
+  return leafSample(lut, b, c, wo, wi, pdf, type, sx, sy, sc);
 }
 
 // --- sheen layer over the lobe (bsdf/sheen.h:199-451): a microfiber reflection picked by
@@ -1019,6 +1201,30 @@ PRT_DEV f3 bsdfTransparency(const float* lut, const Bsdf& b, f3 wo)
 {
   if (b.mix == 2) return f3(1.f);
   f3 t = (b.kind == PRT_KIND_THIN_DIELECTRIC && b.importance > 0.f) ? thinTransparency(lut, b, wo) : f3(0.f);
+  if (b.stdk && b.kind == PRT_KIND_STANDARD)
+  {
+    // the mix (multi_bsdf.h:289-305): thin dielectrics transmit, the transparent BTDF passes 1
+    const prt_material_constants& m = *b.stdm;
+    t = f3(0.f);
+    for (int k = 0; k < m.nKids; ++k)
+    {
+      if (!(m.kidImp[k] > 0.f)) continue;
+      const int kind = b.kids[k].kind;
+      if (kind == PRT_KIND_THIN_DIELECTRIC) t = t + thinTransparency(lut, kidBsdf(b, k), wo) * m.kidScale[k];
+      else if (kind == PRT_KIND_TRANSPARENT) t = t + f3(m.kidScale[k]);
+    }
+    if (b.coat)  // the coat's absorption along the refracted direction and its reflectance (rough_coating.h:539-560)
+    {
+      const float cosO = dot(wo, b.fr.N);
+      if (!(cosO > 0.f)) t = f3(0.f);
+      else
+      {
+        const float e = b.c2.x * (1.0f / refractCos(cosO, b.eta));
+        f3 absorbed(t.x * powf(b.c1.x, e), t.y * powf(b.c1.y, e), t.z * powf(b.c1.z, e));
+        t = (t + b.c2.y * (absorbed - t)) * (1.f - dielReflAlbedo(lut, cosO, b.eta, b.roughness) * b.c2.y);
+      }
+    }
+  }
   if (b.sheen)  // sheen.h:393-401
   {
     float cosO = dot(wo, b.fr.N);
@@ -1031,8 +1237,8 @@ PRT_DEV f3 bsdfTransparency(const float* lut, const Bsdf& b, f3 wo)
 // Builds the BSDF for a hit from the material constants (prt_material_constants, computed on
 // the host from the formulas of rough_conductor.h:121-130 and rough_dielectric.h:279-291).
 // ext = false (for scenes without thin dielectrics, opacity or Oren-Nayar bases) drops those
-// paths at compile time where the argument is a constant.
-PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool inside, bool ext = true)
+// paths at compile time where the argument is a constant; stdk = false likewise drops kind 5.
+PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool inside, bool ext = true, bool stdk = true)
 {
   Bsdf b;
   b.fr = shadingFrame(c, wo);
@@ -1060,6 +1266,24 @@ PRT_DEV Bsdf makeBsdf(const prt_material_constants& m, const Ctx& c, f3 wo, bool
   b.sheenScale = m.sheenScale;
   b.sheenRough = m.sheenRoughness;
   b.sheenA = m.sheenAlpha;
+  b.stdk = ext && stdk;
+  b.smooth = !b.stdk || m.smooth != 0;
+  b.stdm = nullptr;
+  b.kids = nullptr;
+  b.coat = false;
+  if (b.stdk && m.kind == PRT_KIND_STANDARD)
+  {
+    // the mix of bases of this orientation, and the coat over it
+    b.stdm = &m;
+    b.kids = &m + m.kidBase[o];
+    b.coat = m.coat != 0;
+    b.c1 = f3(m.coatColor[0], m.coatColor[1], m.coatColor[2]);
+    b.c2 = f3(m.coatThickness, m.coatScale, m.impSum);
+    b.Eavg = m.coatRavg[o];
+    b.eta = m.coatEta[o];
+    b.roughness = m.coatRoughness;
+    b.ggx.a = m.coatAlpha;
+  }
   return b;
 }
 

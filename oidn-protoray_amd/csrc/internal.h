@@ -113,7 +113,7 @@ struct BatchFrameDev
 };
 constexpr int kBatchBlockSize = 256;
 void launchBatch(const BatchFrameDev* frames, int nFrames, int nTiles, int* counter, int tilesPerFetch, int grid, int* spill,
-                 int spillStride, unsigned long long* samples, hipStream_t s);
+                 int spillStride, unsigned long long* samples, bool std, hipStream_t s);
 
 // Kernel launchers (render.hip)
 struct LaunchCfg
@@ -126,8 +126,10 @@ struct LaunchCfg
 // ~1 KB record by value would make every lane copy it to scratch.
 void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStream_t s);
 void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool count, hipStream_t s);
-void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, bool ext, hipStream_t s);
-void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, bool filter, bool count, hipStream_t s);
+// level: 0 the original lobes, 1 the extended lobes and coloured media, 2 kind-5 materials too;
+// filter: 0 opaque scene, 1 transparent materials, 2 transparent materials and kind-5 materials
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, int level, hipStream_t s);
+void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, int filter, bool count, hipStream_t s);
 void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s);
 void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, const float* ray8,
                       hipStream_t s);

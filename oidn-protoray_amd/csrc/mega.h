@@ -42,8 +42,9 @@ struct NoVertexHook
 };
 
 // Traces sample sampleIndex of pixel (x, y); returns the path's radiance (unclamped).  fh, when
-// not null, receives the first-hit AOVs; hook(vertex) runs at every shaded vertex.
-template <class Hook>
+// not null, receives the first-hit AOVs; hook(vertex) runs at every shaded vertex.  kStd: the
+// scene may hold kind-5 materials (false keeps their code out of the kernel).
+template <bool kStd, class Hook>
 PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint32_t sampleIndex, FirstHit* fh, Hook&& hook,
                          uint32_t& nodes, uint32_t& tris)
 {
@@ -93,8 +94,8 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     if (depth >= a.maxDepth) break;
     Ctx ctx = postIntersect(a, o, d, th, prim, u, v);
     const f3 wo = -d;
-    const bool inside = (m.kind == PRT_KIND_DIELECTRIC) && (mediumId == m.mediumInt);
-    Bsdf b = makeBsdf(m, ctx, wo, inside);
+    const bool inside = m.mediaInterface && (mediumId == m.mediumInt);  // material/material.h:54-61
+    Bsdf b = makeBsdf(m, ctx, wo, inside, true, kStd);
     Sampler smp;
     smp.set(a.seed, sampleIndex, pixelId);
     if (mediumId > 0) T = T * mediumTransmittance(a.media[mediumId], th);  // volume transmission (:652-657)
@@ -104,7 +105,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     // next-event estimate (:659-718) with its shadow ray (:951-1002)
     f3 dirL(0.f), wiL(0.f);
     bool nee = false;
-    if (b.mix != 2)  // a smooth lobe (all but the transparent BTDF)
+    if (b.mix != 2 && b.smooth)  // a smooth lobe (all but the transparent BTDF)
     {
       float s0, s1, s2;
       smp.get3D(sampleDim + 3, s0, s1, s2);
@@ -128,7 +129,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
           float ts, us = 0.f, vs = 0.f;
           if (a.anyTransparent)
           {
-            ShadowFilter flt{&a, ctx.p, wiL, f3(1.f), 0u};
+            ShadowFilterT<kStd> flt{&a, ctx.p, wiL, f3(1.f), 0u};
             nee = traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris, flt) < 0;
             dirL = dirL * flt.T;
           }
@@ -175,7 +176,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     tnear = ctx.eps;
     d = wi;
     if ((type & BsdfType::Continuation) == 0) { lastPdf = pdf; lastType = type; }
-    if (m.kind == PRT_KIND_DIELECTRIC && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
+    if (m.mediaInterface && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
       mediumId = (mediumId == m.mediumInt) ? 0 : m.mediumInt;
   }
   return L;

@@ -136,8 +136,10 @@ PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, fl
 // Transparent-shadow filter (render/transparent_filter.h:42-93): a hit on a transparent
 // material multiplies the shadow ray's transmittance T by the material's transparency seen
 // along -d and lets the ray go on until T falls below 1e-5; a hit on any other material
-// occludes.  The product does not depend on the order in which the hits are found.
-struct ShadowFilter
+// occludes.  The product does not depend on the order in which the hits are found.  kStd: the
+// scene has kind-5 materials.
+template <bool kStd = true>
+struct ShadowFilterT
 {
   const RenderArgs* a;
   f3 o, d, T;
@@ -148,11 +150,12 @@ struct ShadowFilter
     if (!m.transparent) { T = f3(0.f); return true; }
     ++crossed;
     Ctx c = postIntersect(*a, o, d, t, prim, u, v);
-    Bsdf b = makeBsdf(m, c, -d, false);
+    Bsdf b = makeBsdf(m, c, -d, false, true, kStd);
     T = T * bsdfTransparency(a->lut, b, -d);
     return reduceMax(T) < 1e-5f;
   }
 };
+using ShadowFilter = ShadowFilterT<true>;
 
 
 // PixelFilter::sample for one coordinate (image/pixel_filter.h:57-63): a continuous sample of

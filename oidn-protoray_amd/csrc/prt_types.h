@@ -8,6 +8,9 @@
 #define PRT_KIND_DIELECTRIC 2
 #define PRT_KIND_COATED 3
 #define PRT_KIND_THIN_DIELECTRIC 4
+#define PRT_KIND_STANDARD 5
+// the transparent BTDF as a base of a kind-5 material's mix (a constants record, never a material kind)
+#define PRT_KIND_TRANSPARENT 6
 
 // Bins of the guide tables that narrow the searches of the environment's sampling CDFs (a power
 // of two, so s * PRT_ENV_GUIDE is exact in float)
@@ -39,7 +42,7 @@
 // Per-material constants derived on the host at upload (the per-hit constructor work of
 // RoughConductorBrdfSimd, rough_conductor.h:121-130, RoughDielectricBsdfSimd,
 // rough_dielectric.h:279-291, and ThinRoughDielectricBsdfSimd, thin_rough_dielectric.h:180-192,
-// depends on the material only).  208 bytes.
+// depends on the material only).  312 bytes.
 struct prt_material_constants
 {
   int32_t kind;
@@ -75,8 +78,24 @@ struct prt_material_constants
   int32_t sheen;           // sheen weight above 1e-5
   float sheenColor[3];
   float sheenScale, sheenRoughness, sheenAlpha;
+  int32_t transmissive;    // MaterialType::Transmissive (material/standard.cpp:159-160)
+  int32_t mediaInterface;  // a transmissive surface that is not thin-walled (material/standard.cpp:159-175)
+  int32_t smooth;          // a diffuse or glossy lobe: next-event estimation applies (path_renderer_stream.h:659)
+  // kind 5, StandardMaterial's bases (material/standard.cpp:409-566): the base records seen from
+  // outside [0] and inside [1] (consecutive records at this record's index + kidBase[o]), their scales and
+  // importances (base importance x scale), the mix's importance, whether every base is diffuse,
+  // and the coat over the mix (eta folded to <= 1 and Ravg per orientation); rr holds the albedo
+  int32_t nKids;
+  int32_t kidBase[2];
+  float kidScale[3], kidImp[3];
+  float impSum;
+  int32_t mixDiffuseOnly;
+  int32_t coat;
+  float coatEta[2], coatRavg[2];
+  float coatColor[3];
+  float coatThickness, coatScale, coatRoughness, coatAlpha;
 };
-static_assert(sizeof(prt_material_constants) == 208, "material constants: 208 bytes");
+static_assert(sizeof(prt_material_constants) == 312, "material constants: 312 bytes");
 
 // Environment light record as the kernels read it.  64 bytes.
 struct prt_light_dev

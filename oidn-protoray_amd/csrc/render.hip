@@ -176,9 +176,10 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __re
 #ifndef PRT_SHADE_WAVES_PER_EU
 #define PRT_SHADE_WAVES_PER_EU 4
 #endif
-// kExt: the scene has thin dielectrics, opacity or Oren-Nayar bases (the kernel without them
-// keeps the register budget of the four original lobes).
-template <bool kExt>
+// kLevel: 0 the four original lobes and clear media (this kernel keeps their register budget);
+// 1 the scene also has thin dielectrics, opacity, Oren-Nayar bases, sheen or coloured media;
+// 2 it also has kind-5 materials (StandardMaterial's weighted bases and coats).
+template <int kLevel>
 __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(PRT_SHADE_WAVES_PER_EU)))
 kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool classB)
 {
@@ -274,16 +275,16 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
         Ctx ctx = postIntersect(a, o, d, h.x, prim, h.y, h.z);
         f3 wo = -d;
         int mediumId = fl >> PRT_F_MEDIUM_SHIFT;
-        bool inside = (m.kind == PRT_KIND_DIELECTRIC) && (mediumId == m.mediumInt);
-        Bsdf b = makeBsdf(m, ctx, wo, inside, kExt);
+        bool inside = m.mediaInterface && (mediumId == m.mediumInt);  // material/material.h:54-61
+        Bsdf b = makeBsdf(m, ctx, wo, inside, kLevel >= 1, kLevel >= 2);
         Sampler smp;
         smp.set(a.seed, sampleIndex, pixelId);  // resetSample (:641-642)
-        if (mediumId > 0) T = T * mediumTransmittance(a.media[mediumId], h.x);  // volume transmission (:652-657)
+        if (kLevel >= 1 && mediumId > 0) T = T * mediumTransmittance(a.media[mediumId], h.x);  // volume transmission (:652-657)
         bool mAov1 = !(fl & PRT_F_AOV1), mAov2 = !(fl & PRT_F_AOV2);
         bool mRR = !(mAov1 || mAov2);
         // direct light (:659-718) when the BSDF has a smooth lobe: all but the transparent BTDF
         f3 dirL(0.f);
-        if (b.mix != 2)
+        if (b.mix != 2 && b.smooth)
         {
           float s0, s1, s2;
           smp.get3D(sampleDim + 3, s0, s1, s2);
@@ -361,7 +362,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
           a.rayO[slot] = make_float4(ctx.p.x, ctx.p.y, ctx.p.z, ctx.eps);
           a.rayD[slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
           if ((type & BsdfType::Continuation) == 0) { lastPdf = pdf; lastType = type; }
-          if (m.kind == PRT_KIND_DIELECTRIC && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
+          if (m.mediaInterface && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)  // material/material.h:84-95
             mediumId = (mediumId == m.mediumInt) ? 0 : m.mediumInt;
           a.TP[slot] = make_float4(T.x, T.y, T.z, lastPdf);
           wantExt = true;
@@ -384,8 +385,9 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
   }
 }
 
-// One shadow ray of the queue (path_renderer_stream.h:951-1065).
-template <bool kFilter>
+// One shadow ray of the queue (path_renderer_stream.h:951-1065).  kFilter: 0 opaque scene, 1
+// transparent materials, 2 transparent materials and kind-5 materials.
+template <int kFilter>
 PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& nodes, uint32_t& tris, uint32_t& rays, bool classB)
 {
   {
@@ -398,7 +400,7 @@ PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& n
     uint32_t crossed = 0;
     if (kFilter)
     {
-      ShadowFilter flt{&a, o, d, f3(1.f), 0u};
+      ShadowFilterT<(kFilter >= 2)> flt{&a, o, d, f3(1.f), 0u};
       prim = traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris, flt);
       Tsh = flt.T;
       crossed = flt.crossed;
@@ -425,8 +427,9 @@ PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& n
 
 // Shadow rays of depth `depth`.  kFilter: the scene has transparent materials, and the shadow
 // rays gather the transmittance of the transparent surfaces they cross (path_renderer_stream.h:
-// 951-1002 with the TransparentShadowFilter of render/transparent_filter.h).
-template <bool kFilter, bool kCount>
+// 951-1002 with the TransparentShadowFilter of render/transparent_filter.h); 2: some of them are
+// kind-5 materials.
+template <int kFilter, bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth, bool classB)
 {
   const RenderArgs& a = *ap;
@@ -550,14 +553,12 @@ __global__ void __launch_bounds__(64) kQueryPixel(const RenderArgs* __restrict__
     const Ctx c = postIntersect(a, o, d, t, prim, u, v);
     const int matId = a.triMat[prim];
     const prt_material_constants& m = a.mats[matId];
-    const bool transmissive = m.kind == PRT_KIND_DIELECTRIC || m.kind == PRT_KIND_THIN_DIELECTRIC;
     f3 T(0.f);
     if (m.transparent) T = bsdfTransparency(a.lut, makeBsdf(m, c, -d, false), -d);
     prt_pixel_hit& h = out[n];
     h.prim = prim;
     h.material = matId;
-    h.material_type = (m.transparent ? 1 : 0) | (transmissive ? 4 : 0) | (m.kind == PRT_KIND_DIELECTRIC ? 8 : 0) |
-                      (m.emissive ? 16 : 0);
+    h.material_type = (m.transparent ? 1 : 0) | (m.transmissive ? 4 : 0) | (m.mediaInterface ? 8 : 0) | (m.emissive ? 16 : 0);
     const f3 vals[6] = {c.p, c.Ng, c.f.N, c.f.U, c.f.V, T};
     float* dst[6] = {h.p, h.Ng, h.N, h.U, h.V, h.T};
     for (int k = 0; k < 6; ++k) { dst[k][0] = vals[k].x; dst[k][1] = vals[k].y; dst[k][2] = vals[k].z; }
@@ -597,20 +598,23 @@ void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool co
   if (count) kIntersect<true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
   else kIntersect<false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
 }
-void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, bool ext, hipStream_t s)
+void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, int level, hipStream_t s)
 {
-  if (ext) kShade<true><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
-  else kShade<false><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
+  if (level >= 2) kShade<2><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
+  else if (level == 1) kShade<1><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
+  else kShade<0><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
 }
-void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, bool filter, bool count, hipStream_t s)
+void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, int filter, bool count, hipStream_t s)
 {
   if (count)
   {
-    if (filter) kOccluded<true, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
-    else kOccluded<false, true><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
+    if (filter >= 2) kOccluded<2, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    else if (filter == 1) kOccluded<1, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    else kOccluded<0, true><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
   }
-  else if (filter) kOccluded<true, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
-  else kOccluded<false, false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
+  else if (filter >= 2) kOccluded<2, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+  else if (filter == 1) kOccluded<1, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+  else kOccluded<0, false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
 }
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit, int* spill,
                  int spillStride, int spillCap, int grid, hipStream_t s)

@@ -994,10 +994,10 @@ MeshFile loadMesh(const std::string& path)
 }
 
 // ---------------------------------------------------------------------------------------
-// Standard parameters -> material record.  The record models one lobe set per material
-// (diffuse or Oren-Nayar, rough conductor, rough dielectric, coated diffuse, thin rough
-// dielectric; opacity, sheen and emission on any); parameters outside that set are dropped and
-// the material counted as approximated.
+// Standard parameters -> material record: a single-lobe kind where StandardMaterial's composition
+// reduces to one lobe of weight 1, kind 5 (the weighted bases and the coat) otherwise.  Textures
+// (their factors are kept), subsurface and zero roughness (the smooth lobes, modelled by the
+// rough ones at the minimum roughness) count the material as approximated.
 // ---------------------------------------------------------------------------------------
 bool emissive(const StdMat& m)
 {
@@ -1028,79 +1028,105 @@ prt_material toRecord(const StdMat& m, bool& approx)
   if (m.emissionLuminance > 0.f)
     for (int c = 0; c < 3; ++c) r.emission[c] = m.emissionColor[c] * m.emissionLuminance;
   const float ior = m.specularIor > 0.f && m.specularIor < 1.f ? 1.f / m.specularIor : m.specularIor;
-  const bool mixedMetal = m.metalness > 0.f && m.metalness < 1.f;
-  const bool mixedTrans = m.transmissionWeight > 0.f && m.transmissionWeight < 1.f;
-  approx = m.textured || mixedMetal || mixedTrans || m.subsurfaceWeight > 0.f || (m.baseWeight != 1.f && m.metalness < 1.f);
-  // the sheen layer over any lobe set (material/standard.cpp:599-611)
+  const float coatIor = m.coatIor > 0.f && m.coatIor < 1.f ? 1.f / m.coatIor : m.coatIor;
+  auto cl = [](float x) { return std::min(std::max(x, 0.f), 1.f); };
+  const float eps = 1e-5f;  // material/standard.h:79
+  const float base = cl(m.baseWeight), metal = cl(m.metalness), spec = cl(m.specularWeight);
+  const float trans = cl(m.transmissionWeight), coat = cl(m.coatWeight);
+  const bool specIor = std::fabs(ior - 1.f) > eps, hasCoat = coat > eps && std::fabs(coatIor - 1.f) > eps;
+  // textures keep their factors and subsurface is not modelled; roughness 0 (the smooth lobes) is
+  // modelled by the rough lobes at the minimum roughness
+  approx = m.textured || m.subsurfaceWeight > 0.f;
+  // the sheen layer and opacity over any lobe set (material/standard.cpp:599-627)
   r.sheen_weight = m.sheenWeight;
   for (int c = 0; c < 3; ++c) r.sheen_color[c] = m.sheenColor[c];
   r.sheen_roughness = m.sheenRoughness;
-  // opacity (material/standard.cpp:152-157, :613-627) applies to every lobe set
-  r.transparency = 1.f - std::min(std::max(m.opacity, 0.f), 1.f);
-  if (m.metalness >= 0.5f)
+  r.transparency = 1.f - cl(m.opacity);
+  r.diffuse_roughness = cl(m.diffuseRoughness);
+  for (int c = 0; c < 3; ++c)
   {
-    r.kind = 1;
-    r.r = m.baseColor[0]; r.g = m.baseColor[1]; r.b = m.baseColor[2];
+    r.transmission_color[c] = m.transmissionColor[c];
+    r.coat_color[c] = m.coatColor[c];
+  }
+  r.transmission_depth = m.transmissionDepth;
+  // the single-lobe kinds where the composition reduces to one lobe of weight 1 (their kernels
+  // are the leanest), kind 5 for every other parameter set
+  if (metal >= 1.f && spec >= 1.f && !hasCoat)
+  {
+    r.kind = 1;  // metal base alone (:533-546)
+    r.r = m.baseColor[0] * base; r.g = m.baseColor[1] * base; r.b = m.baseColor[2] * base;
     r.er = m.specularColor[0]; r.eg = m.specularColor[1]; r.eb = m.specularColor[2];
     r.roughness = m.specularRoughness;
-    approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f;
+    approx = approx || m.specularRoughness <= 0.f;
     return r;
   }
-  if (m.transmissionWeight >= 0.5f && m.thinWalled)
+  if (metal <= 0.f && trans >= 1.f && spec >= 1.f && specIor && !hasCoat)
   {
-    // thin-walled transmission: ThinRoughDielectricBsdf with transmissionColor and thickness /
-    // transmissionDepth, thickness 1 (material/standard.cpp:534-543, :48-52, :79-80)
-    r.kind = 4;
-    r.ior = std::fabs(ior - 1.f) > 1e-6f ? ior : 1.5f;
-    r.roughness = m.specularRoughness;
-    r.r = m.transmissionColor[0]; r.g = m.transmissionColor[1]; r.b = m.transmissionColor[2];
-    r.thickness = m.transmissionDepth > 0.f ? 1.f / m.transmissionDepth : 3.4e38f;
-    approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f || std::fabs(ior - 1.f) <= 1e-6f;
-    return r;
-  }
-  if (m.transmissionWeight >= 0.5f)
-  {
-    r.kind = 2;
-    r.ior = std::fabs(ior - 1.f) > 1e-6f ? ior : 1.5f;
-    r.roughness = m.specularRoughness;
-    // the medium inside: transmissionColor ^ transmissionDepth (material/standard.cpp:169-173)
-    r.r = r.g = r.b = 1.f;
-    for (int c = 0; c < 3; ++c) r.transmission_color[c] = m.transmissionColor[c];
-    r.transmission_depth = m.transmissionDepth;
-    approx = approx || m.coatWeight > 0.f || m.specularRoughness <= 0.f || std::fabs(ior - 1.f) <= 1e-6f;
-    return r;
-  }
-  r.r = m.baseColor[0] * m.baseWeight;
-  r.g = m.baseColor[1] * m.baseWeight;
-  r.b = m.baseColor[2] * m.baseWeight;
-  r.diffuse_roughness = std::min(std::max(m.diffuseRoughness, 0.f), 1.f);  // Oren-Nayar base (:446-453)
-  const bool specLayer = m.specularWeight > 0.f && std::fabs(ior - 1.f) > 1e-6f;
-  const bool coatLayer = m.coatWeight > 0.f && std::fabs(m.coatIor - 1.f) > 1e-6f;
-  if (specLayer)
-  {
-    // the default Standard material: a rough dielectric layer over the diffuse base
-    r.kind = 3;
-    r.er = r.eg = r.eb = 1.f;
-    r.thickness = 1.f;
-    r.weight = m.specularWeight;
     r.ior = ior;
     r.roughness = m.specularRoughness;
-    approx = approx || coatLayer || m.specularRoughness <= 0.f || m.specularColor[0] != 1.f || m.specularColor[1] != 1.f ||
-             m.specularColor[2] != 1.f;
+    approx = approx || m.specularRoughness <= 0.f;
+    if (m.thinWalled)
+    {
+      // thin-walled transmission: ThinRoughDielectricBsdf with transmissionColor and thickness /
+      // transmissionDepth, thickness 1 (material/standard.cpp:534-543, :48-52, :79-80)
+      r.kind = 4;
+      r.r = m.transmissionColor[0]; r.g = m.transmissionColor[1]; r.b = m.transmissionColor[2];
+      r.thickness = m.transmissionDepth > 0.f ? 1.f / m.transmissionDepth : 3.4e38f;
+      return r;
+    }
+    // the medium inside: transmissionColor ^ transmissionDepth (material/standard.cpp:169-173)
+    r.kind = 2;
+    r.r = r.g = r.b = 1.f;
     return r;
   }
-  if (coatLayer)
+  if (metal <= 0.f && trans <= 0.f && !(specIor && spec > eps && hasCoat))
   {
-    r.kind = 3;
-    r.er = m.coatColor[0]; r.eg = m.coatColor[1]; r.eb = m.coatColor[2];
-    r.thickness = 1.f;
-    r.weight = m.coatWeight;
-    r.ior = m.coatIor < 1.f ? 1.f / m.coatIor : m.coatIor;
-    r.roughness = m.coatRoughness;
-    approx = approx || m.coatRoughness <= 0.f;
+    r.r = m.baseColor[0] * base; r.g = m.baseColor[1] * base; r.b = m.baseColor[2] * base;
+    if (specIor && spec > eps)
+    {
+      // the default Standard material: a rough dielectric layer of weight specularWeight over the
+      // diffuse base (:499-503)
+      r.kind = 3;
+      r.er = r.eg = r.eb = 1.f;
+      r.thickness = 1.f;
+      r.weight = spec;
+      r.ior = ior;
+      r.roughness = m.specularRoughness;
+      approx = approx || m.specularRoughness <= 0.f;
+      return r;
+    }
+    if (hasCoat)
+    {
+      // the coat over a diffuse base (:551-566), StandardMaterial's coat thickness 0.5 (:66)
+      r.kind = 3;
+      r.er = m.coatColor[0]; r.eg = m.coatColor[1]; r.eb = m.coatColor[2];
+      r.thickness = 0.5f;
+      r.weight = coat;
+      r.ior = coatIor;
+      r.roughness = m.coatRoughness;
+      approx = approx || m.coatRoughness <= 0.f;
+      return r;
+    }
+    r.kind = 0;
     return r;
   }
-  r.kind = 0;
+  // kind 5: the weighted bases and the coat (material/standard.cpp:409-611)
+  r.kind = PRT_KIND_STANDARD;
+  r.r = m.baseColor[0]; r.g = m.baseColor[1]; r.b = m.baseColor[2];
+  r.er = m.specularColor[0]; r.eg = m.specularColor[1]; r.eb = m.specularColor[2];
+  r.roughness = m.specularRoughness;
+  r.ior = m.specularIor > 0.f ? m.specularIor : 1.5f;
+  r.base_weight = m.baseWeight;
+  r.metalness = m.metalness;
+  r.specular_weight = m.specularWeight;
+  r.transmission_weight = m.transmissionWeight;
+  r.thin_walled = m.thinWalled ? 1 : 0;
+  r.thickness = 1.f;  // thicknessPar.scale (:79)
+  r.coat_weight = m.coatWeight;
+  r.coat_roughness = m.coatRoughness;
+  r.coat_ior = m.coatIor > 0.f ? m.coatIor : 1.6f;
+  r.coat_thickness = 0.5f;  // coatThicknessPar.scale (:66)
+  approx = approx || (m.specularRoughness <= 0.f && spec > eps) || (hasCoat && m.coatRoughness <= 0.f);
   return r;
 }
 

@@ -411,3 +411,103 @@ def glass_slab_radiance(scene, dirs):
         sel = k == j
         out[sel] = amb * (col[None] ** depth) ** (scene["slab_thickness"] / cos[sel, None])
     return out, k
+
+
+KIND_STANDARD = 5
+
+
+def standard_material(base_color=(0.8, 0.8, 0.8), base_weight=1.0, metalness=0.0, diffuse_roughness=0.0,
+                      specular_weight=1.0, specular_color=(1, 1, 1), specular_roughness=0.3, specular_ior=1.5,
+                      transmission_weight=0.0, transmission_color=(1, 1, 1), transmission_depth=1.0, thin_walled=0,
+                      thickness=1.0, coat_weight=0.0, coat_color=(1, 1, 1), coat_roughness=0.1, coat_ior=1.6,
+                      coat_thickness=0.5, sheen_weight=0.0, sheen_color=(1, 1, 1), sheen_roughness=0.3, opacity=1.0,
+                      emission=(0, 0, 0)):
+    """A kind-5 record: StandardMaterial's uniform parameters with the reference's defaults
+    (material/standard.cpp:27-81; coat roughness 0.1 instead of 0, the rough coat being the one
+    modelled)."""
+    m = np.zeros((), MATERIAL_DTYPE)
+    m["kind"] = KIND_STANDARD
+    m["r"], m["g"], m["b"] = base_color
+    m["base_weight"] = base_weight
+    m["metalness"] = metalness
+    m["diffuse_roughness"] = diffuse_roughness
+    m["specular_weight"] = specular_weight
+    m["er"], m["eg"], m["eb"] = specular_color
+    m["roughness"] = specular_roughness
+    m["ior"] = specular_ior
+    m["transmission_weight"] = transmission_weight
+    m["transmission_color"] = transmission_color
+    m["transmission_depth"] = transmission_depth
+    m["thin_walled"] = thin_walled
+    m["thickness"] = thickness
+    m["coat_weight"] = coat_weight
+    m["coat_color"] = coat_color
+    m["coat_roughness"] = coat_roughness
+    m["coat_ior"] = coat_ior
+    m["coat_thickness"] = coat_thickness
+    m["sheen_weight"] = sheen_weight
+    m["sheen_color"] = sheen_color
+    m["sheen_roughness"] = sheen_roughness
+    m["transparency"] = 1.0 - opacity
+    m["emission"] = emission
+    return m
+
+
+def spheres_standard(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4, max_depth=8, palette=24):
+    """The sun/sky scene of spheres_lobes with kind-5 StandardMaterials: partial metalness, partial
+    transmission (a rough dielectric beside the coated diffuse base), coloured solid glass (the
+    interior medium absorbs), coats over metal and over glass, thin-walled partial transmission
+    under a coat (shadow rays pass through), opacity and sheen over mixes, base and specular weights
+    below 1, and a coat over a diffuse-only base; plus coloured kind-2 glass."""
+    rng = Pcg32(seed)
+    mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
+    for i in range(palette):
+        v = i % 8
+        rgb = (rng.f(0.1, 0.9), rng.f(0.1, 0.9), rng.f(0.1, 0.9))
+        rough = math.sqrt(rng.f(0.02, 0.4))
+        tint = (rng.f(0.3, 1.0), rng.f(0.3, 1.0), rng.f(0.3, 1.0))
+        if v == 0:
+            m = standard_material(rgb, metalness=rng.f(0.2, 0.8), specular_roughness=rough)
+        elif v == 1:
+            m = standard_material(rgb, transmission_weight=rng.f(0.2, 0.8), specular_roughness=rough,
+                                  transmission_color=tint, transmission_depth=rng.f(0.5, 2.0))
+        elif v == 2:
+            m = standard_material(rgb, metalness=1.0, specular_roughness=rough, coat_weight=rng.f(0.4, 1.0),
+                                  coat_color=tint, coat_roughness=math.sqrt(rng.f(0.01, 0.2)))
+        elif v == 3:
+            m = standard_material(rgb, transmission_weight=1.0, specular_roughness=rough, transmission_color=tint,
+                                  transmission_depth=rng.f(0.3, 1.5), coat_weight=rng.f(0.3, 1.0),
+                                  coat_roughness=math.sqrt(rng.f(0.01, 0.2)))
+        elif v == 4:
+            m = standard_material(rgb, metalness=rng.f(0.2, 0.5), transmission_weight=rng.f(0.2, 0.6),
+                                  diffuse_roughness=rng.f(0.1, 0.9), specular_roughness=rough, coat_weight=rng.f(0.3, 0.9),
+                                  sheen_weight=rng.f(0.2, 0.9), sheen_color=tint, opacity=rng.f(0.5, 0.95))
+        elif v == 5:
+            m = standard_material(rgb, transmission_weight=rng.f(0.3, 0.9), thin_walled=1, thickness=rng.f(0.2, 1.5),
+                                  transmission_color=tint, specular_roughness=rough, coat_weight=rng.f(0.3, 0.9),
+                                  specular_ior=1.0 if i == 5 else 1.5)
+        elif v == 6:
+            m = standard_material(rgb, base_weight=rng.f(0.3, 0.9), specular_weight=rng.f(0.2, 0.9),
+                                  metalness=rng.f(0.2, 0.8), specular_roughness=rough)
+        else:
+            m = standard_material(rgb, specular_weight=0.0, coat_weight=rng.f(0.3, 1.0),
+                                  metalness=rng.f(0.2, 0.6) if (i // 8) % 2 else 0.0, specular_roughness=rough)
+        mats.append(m)
+    for i in range(palette // 6):  # coloured kind-2 glass
+        g = _material(KIND_DIELECTRIC, roughness=math.sqrt(rng.f(0.02, 0.3)), ior=1.5)
+        g["transmission_color"] = (rng.f(0.3, 1.0), rng.f(0.3, 1.0), rng.f(0.3, 1.0))
+        g["transmission_depth"] = rng.f(0.5, 2.0)
+        mats.append(g)
+    palette = len(mats) - 2
+    pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, lambda i: 2 + rng.u32() % palette)
+    ss = load_sunsky()
+    d = np.array([-0.6, 0.7, -0.4])
+    d /= np.linalg.norm(d)
+    lights = np.zeros(3, LIGHT_DTYPE)
+    lights[0] = (LIGHT_DISTANT, 300.0, 280.0, 250.0, d[0], d[1], d[2], 0.5 * math.radians(2.0))
+    sr, sd = ss["sun_radiance"], ss["sun_dir"]
+    lights[1] = (LIGHT_DISTANT, sr[0], sr[1], sr[2], sd[0], sd[1], sd[2], ss["sun_radius"])
+    lights[2] = (LIGHT_IMAGE, 1.0, 1.0, 1.0, 0, 0, 0, 0)
+    return dict(name="spheres_standard", positions=pos, indices=idx, tri_material=tri_mat,
+                materials=np.array(mats, MATERIAL_DTYPE), lights=lights, env=ss["sky"],
+                camera=_camera(), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)

@@ -102,7 +102,7 @@ def lib():
         L.orc_adjoint.restype = ctypes.c_double
         L.orc_adjoint.argtypes = [vp, vp, i, i, i, u, i, i, f, vp, i, vp]
         L.orc_bsdf_grad.argtypes = [i, vp, vp, vp, vp]
-        L.orc_bsdf_mat.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_bsdf_mat.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_set_pixel_sample.argtypes = [vp, i, f, f, i, f, f, i]
         L.orc_pixel_sample.argtypes = [vp, i, vp, vp]
         _lib = L
@@ -194,10 +194,11 @@ def bsdf(luts_flat, params, geo, s):
     return so, st, ev
 
 
-def bsdf_mat(luts_flat, mats, geo, s):
+def bsdf_mat(luts_flat, mats, geo, s, inside=None):
     """The renderer's whole BSDF for material records (MATERIAL_DTYPE) on supplied frames:
     (sample n x 7 [weight wi pdf], type n, eval n x 4 [value pdf], aux n x 9 [albedo, normal,
-    transparency of a transparent material])."""
+    transparency of a transparent material]).  inside (n, optional): the path travels in the
+    material's interior medium, so a media interface is seen from its back."""
     mats = np.ascontiguousarray(mats, MATERIAL_DTYPE)
     geo, s = c32(geo), c32(s)
     n = mats.shape[0]
@@ -205,8 +206,10 @@ def bsdf_mat(luts_flat, mats, geo, s):
     st = np.zeros(n, np.int32)
     ev = np.zeros((n, 4), np.float32)
     aux = np.zeros((n, 9), np.float32)
+    ins = None if inside is None else i32(inside)
     lib().orc_bsdf_mat(n, c32(luts_flat).ctypes.data, mats.ctypes.data, geo.ctypes.data, s.ctypes.data,
-                       so.ctypes.data, st.ctypes.data, ev.ctypes.data, aux.ctypes.data)
+                       so.ctypes.data, st.ctypes.data, ev.ctypes.data, aux.ctypes.data,
+                       None if ins is None else ins.ctypes.data)
     return so, st, ev, aux
 
 

@@ -11,6 +11,7 @@
 #pragma once
 
 #include <cmath>
+#include <memory>
 #include <cstdint>
 #include <cfloat>
 #include <algorithm>
@@ -575,6 +576,7 @@ enum : int {
 // Any kind: transparency = 1 - opacity mixes in a TransparentBtdf; diffuseRoughness (kinds 0
 // and 3) selects the Oren-Nayar base.
 enum MatKind : int { kDiffuse = 0, kConductor = 1, kDielectric = 2, kCoated = 3, kThinDielectric = 4, kStandard = 5 };
+constexpr int kTransparentLobe = 6;  // the transparent BTDF as a base of kind 5 (never a material kind)
 
 struct Material  // 148-byte record, the prt_material layout of the HIP path
 {
@@ -602,7 +604,7 @@ static_assert(sizeof(Material) == 148, "the prt_material layout");
 inline bool isTransparent(const Material& m)
 {
   if (m.transparency != 0.f || m.kind == kThinDielectric) return true;
-  return m.kind == kStandard && m.transmissionWeight > 0.f && m.thinWalled;
+  return m.kind == kStandard && m.transmissionWeight != 0.f && m.thinWalled;
 }
 
 // MaterialType::MediaInterface (material/standard.cpp:159-175): a transmissive surface that is not
@@ -610,7 +612,7 @@ inline bool isTransparent(const Material& m)
 inline bool isMediaInterface(const Material& m)
 {
   if (m.kind == kDielectric) return true;
-  return m.kind == kStandard && m.transmissionWeight > 0.f && !m.thinWalled;
+  return m.kind == kStandard && m.transmissionWeight != 0.f && !m.thinWalled;
 }
 // The IOR of that interface, as the medium records it (>= 1).
 inline float interfaceIor(const Material& m) { return m.ior < 1.f ? 1.0f / m.ior : m.ior; }
@@ -652,7 +654,10 @@ inline Basis makeShadingFrame(const Ctx& ctx, V3 wo)
   return fr;
 }
 
-// Per-hit BSDF instance (MultiBsdf{child, weight 1}).
+struct StdLobes;
+
+// Per-hit BSDF instance (MultiBsdf{child, weight 1}, or for kind 5 the weighted mix of
+// StandardMaterial's bases in `mix`, under a coat when `coat` is set).
 struct Bsdf
 {
   int kind;
@@ -681,7 +686,197 @@ struct Bsdf
   float roughness;
   GGX ggx{0.001f};
   float importance;
+  // kind 5: the bases (mix) and whether a coat (coatColor, thickness, scale, ceta, Ravg, roughness,
+  // ggx above, baseImportance = the mix's importance) lies over them
+  std::shared_ptr<const StdLobes> mixLobes;
+  bool coat = false;
 };
+
+// kind 5: StandardMaterial's bases as one MultiBsdfSimd (multi_bsdf.h:151-187): up to three
+// single-lobe children (the opaque base, the translucent base, the metal base) with their scales
+// and importances (child importance x scale).
+struct StdLobes
+{
+  int n = 0;
+  Bsdf child[3];
+  float scale[3] = {0.f, 0.f, 0.f}, imp[3] = {0.f, 0.f, 0.f};
+  float impSum = 0.f;
+  int flags = 0;
+  void add(const Bsdf& c, float s)
+  {
+    child[n] = c;
+    scale[n] = s;
+    imp[n] = c.importance * s;
+    impSum += imp[n];
+    flags |= c.flags;
+    ++n;
+  }
+};
+
+// The single lobes (each the constructor of its reference class for the given parameters).
+inline void initDiffuse(Bsdf& b, V3 color, float diffRough)
+{
+  b.kind = kDiffuse;
+  b.color = color;
+  b.diffRough = diffRough;
+  b.flags = BsdfType::DiffuseReflection;
+  b.importance = reduceMax(b.color);  // bsdf/diffuse.h:72-75
+}
+inline void initConductor(const Luts& L, Bsdf& b, V3 rr, V3 gg, float roughness)
+{
+  // rough_conductor.h:121-130 + optics.h:243-261
+  b.kind = kConductor;
+  b.roughness = roughness;
+  b.ggx = GGX(roughnessToAlpha(roughness));
+  b.rr = rr;
+  b.gg = gg;
+  artisticToIor(b.rr.x, b.gg.x, b.eta.x, b.k.x);
+  artisticToIor(b.rr.y, b.gg.y, b.eta.y, b.k.y);
+  artisticToIor(b.rr.z, b.gg.z, b.eta.z, b.k.z);
+  b.Eavg = ggxAlbedoAvg(L, roughness);
+  V3 Favg(artisticAvg(b.rr.x, b.gg.x), artisticAvg(b.rr.y, b.gg.y), artisticAvg(b.rr.z, b.gg.z));
+  V3 one(1.f);
+  V3 den = one - Favg * (one - V3(b.Eavg));
+  b.fmsScale = V3(sqr(Favg.x) * b.Eavg / den.x, sqr(Favg.y) * b.Eavg / den.y, sqr(Favg.z) * b.Eavg / den.z);
+  b.flags = BsdfType::GlossyReflection;
+  b.importance = 1.f;
+}
+inline void initDielectric(const Luts& L, Bsdf& b, float eta, float roughness)
+{
+  // rough_dielectric.h:279-291
+  b.kind = kDielectric;
+  b.roughness = roughness;
+  b.ggx = GGX(roughnessToAlpha(roughness));
+  b.deta = eta;
+  b.EavgEta = dielAlbedoAvg(L, eta, roughness);
+  b.EavgRcpEta = dielAlbedoAvg(L, 1.0f / eta, roughness);
+  float FavgEta = fresnelDielectricAvg(eta);
+  float FavgRcpEta = fresnelDielectricAvg(1.0f / eta);
+  float a = (1.f - FavgEta) / (1.f - b.EavgRcpEta);
+  float bb = (1.f - FavgRcpEta) / (1.f - b.EavgEta);
+  float x = bb / (a + bb);
+  b.fmsRatio = 1.f - x * (1.f - FavgEta);
+  b.flags = BsdfType::GlossyReflection | BsdfType::GlossyTransmission;
+  b.importance = 1.f;
+}
+// RoughCoatingBsdf over a diffuse base; etaArg as the constructor receives it (rough_coating.h:
+// 287-295: the member eta is folded to <= 1, Ravg uses the argument as given)
+inline void initCoated(const Luts& L, Bsdf& b, V3 color, float diffRough, V3 coatColor, float thickness, float scale,
+                       float etaArg, float roughness)
+{
+  b.kind = kCoated;
+  b.roughness = roughness;
+  b.ggx = GGX(roughnessToAlpha(roughness));
+  b.color = color;
+  b.diffRough = diffRough;
+  b.coatColor = coatColor;
+  b.thickness = thickness;
+  b.scale = scale;
+  b.Ravg = dielReflAlbedoAvg(L, etaArg, roughness) * b.scale;
+  b.ceta = etaArg <= 1.f ? etaArg : 1.0f / etaArg;
+  b.baseImportance = reduceMax(b.color);  // bsdf/diffuse.h:70-73
+  b.flags = BsdfType::DiffuseReflection | BsdfType::GlossyReflection;
+  b.importance = std::max(1.f, b.baseImportance);  // rough_coating.h:562-565
+}
+inline void initThin(const Luts& L, Bsdf& b, float eta, V3 color, float thickness, float roughness)
+{
+  // thin_rough_dielectric.h:180-192
+  b.kind = kThinDielectric;
+  b.roughness = roughness;
+  b.ggx = GGX(roughnessToAlpha(roughness));
+  b.teta = eta;
+  b.color = color;
+  b.tthick = thickness;
+  b.tEavg = ggxAlbedoAvg(L, roughness);
+  float Favg = fresnelDielectricAvg(b.teta);
+  b.tfms = Favg * (1.f - b.tEavg) / (1.f - Favg * b.tEavg);
+  b.flags = BsdfType::GlossyReflection | BsdfType::UnbentTransmission;
+  b.importance = 1.f;
+}
+inline void initTransparent(Bsdf& b)  // bsdf/transparent.h:26-71
+{
+  b.kind = kTransparentLobe;
+  b.flags = BsdfType::Transparency;
+  b.importance = 1.f;
+}
+
+inline V3 clamp01(float r, float g, float b) { return V3(clampf(r, 0.f, 1.f), clampf(g, 0.f, 1.f), clampf(b, 0.f, 1.f)); }
+inline float foldIor(float ior) { return ior < 1.f ? 1.0f / ior : ior; }
+
+// StandardMaterial's lobes for kind 5 (material/standard.cpp:409-611, ambient IOR 1): the opaque,
+// translucent and metal bases in one weighted mix, then the coat over the mix.
+inline void initStandard(const Luts& L, Bsdf& b, const Material& m, bool fromOutside)
+{
+  const float eps = 1e-5f;  // material/standard.h:79
+  auto lobes = std::make_shared<StdLobes>();
+  const float base = clampf(m.baseWeight, 0.f, 1.f);
+  const V3 baseColor = clamp01(m.r, m.g, m.b);
+  const float metalness = clampf(m.metalness, 0.f, 1.f), specular = clampf(m.specularWeight, 0.f, 1.f);
+  const float specRough = clampf(m.roughness, 0.f, 1.f), specIor = foldIor(std::max(m.ior, 0.f));
+  const float diffRough = clampf(m.diffuseRoughness, 0.f, 1.f);
+  const float dielectric = 1.f - metalness;
+  if (dielectric > eps)
+  {
+    const float transmission = clampf(m.transmissionWeight, 0.f, 1.f);
+    const float eta = clampf(fromOutside ? 1.0f / specIor : specIor, 1.f / 3.f, 3.f);
+    const float opaque = dielectric * (1.f - transmission);
+    if (opaque > eps)  // (:428-505): diffuse, under a specular coating where the IOR differs
+    {
+      Bsdf c;
+      if (std::fabs(specIor - 1.f) > eps && specular > eps)
+        initCoated(L, c, baseColor * base, diffRough, V3(1.f), 1.f, specular, eta, specRough);
+      else
+        initDiffuse(c, baseColor * base, diffRough);
+      lobes->add(c, opaque);
+    }
+    const float translucent = dielectric * transmission * specular;
+    if (translucent > eps)  // (:507-530)
+    {
+      Bsdf c;
+      if (std::fabs(specIor - 1.f) > eps)
+      {
+        if (m.thinWalled)
+          initThin(L, c, eta, clamp01(m.transmissionColor[0], m.transmissionColor[1], m.transmissionColor[2]),
+                   std::max(m.thickness, 0.f) / std::max(m.transmissionDepth, 0.f), specRough);
+        else
+          initDielectric(L, c, eta, specRough);
+      }
+      else
+        initTransparent(c);
+      lobes->add(c, translucent);
+    }
+  }
+  const float metal = metalness * specular;
+  if (metal > eps)  // (:533-546)
+  {
+    Bsdf c;
+    initConductor(L, c, baseColor * base, clamp01(m.er, m.eg, m.eb), specRough);
+    lobes->add(c, metal);
+  }
+  for (int i = 0; i < lobes->n; ++i) lobes->child[i].fr = b.fr;
+  b.kind = kStandard;
+  b.flags = lobes->flags;
+  b.importance = lobes->impSum;
+  b.baseImportance = lobes->impSum;
+  b.coat = false;
+  const float coat = clampf(m.coatWeight, 0.f, 1.f), coatIor = foldIor(std::max(m.coatIor, 0.f));
+  if (coat > eps && std::fabs(coatIor - 1.f) > eps)  // (:551-566)
+  {
+    const float etaArg = clampf(fromOutside ? 1.0f / coatIor : coatIor, 1.f / 3.f, 3.f);
+    const float rough = clampf(m.coatRoughness, 0.f, 1.f);
+    b.coat = true;
+    b.roughness = rough;
+    b.ggx = GGX(roughnessToAlpha(rough));
+    b.coatColor = clamp01(m.coatColor[0], m.coatColor[1], m.coatColor[2]);
+    b.thickness = std::max(m.coatThickness, 0.f);
+    b.scale = coat;
+    b.Ravg = dielReflAlbedoAvg(L, etaArg, rough) * b.scale;
+    b.ceta = etaArg <= 1.f ? etaArg : 1.0f / etaArg;
+    b.flags |= BsdfType::GlossyReflection;
+    b.importance = std::max(1.f, lobes->impSum);  // rough_coating.h:562-565
+  }
+  b.mixLobes = lobes;
+}
 
 inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, int mediumId, int mediumIntId)
 {
@@ -703,80 +898,24 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
   b.sheenColor = V3(clampf(m.sheenColor[0], 0.f, 1.f), clampf(m.sheenColor[1], 0.f, 1.f), clampf(m.sheenColor[2], 0.f, 1.f));
   b.sheenRough = clampf(m.sheenRoughness, 0.f, 1.f);
   b.sheenDist = SheenDist{roughnessToAlpha(b.sheenRough)};
+  // the media interface's orientation: backfacing = (mediumId == mediumIntId), other materials
+  // are always seen from outside (material/material.h:54-61); thin-walled surfaces too
+  // (material/standard.cpp:429)
+  const bool backfacing = isMediaInterface(m) && (mediumId == mediumIntId);
   if (m.kind == kDiffuse)
-  {
-    b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
-    b.flags = BsdfType::DiffuseReflection;
-    b.importance = reduceMax(b.color);  // bsdf/diffuse.h:72-75
-  }
+    initDiffuse(b, clamp01(m.r, m.g, m.b), b.diffRough);
   else if (m.kind == kConductor)
-  {
-    // rough_conductor.h:121-130 + optics.h:243-261
-    b.rr = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
-    b.gg = V3(clampf(m.er, 0.f, 1.f), clampf(m.eg, 0.f, 1.f), clampf(m.eb, 0.f, 1.f));
-    artisticToIor(b.rr.x, b.gg.x, b.eta.x, b.k.x);
-    artisticToIor(b.rr.y, b.gg.y, b.eta.y, b.k.y);
-    artisticToIor(b.rr.z, b.gg.z, b.eta.z, b.k.z);
-    b.Eavg = ggxAlbedoAvg(L, roughness);
-    V3 Favg(artisticAvg(b.rr.x, b.gg.x), artisticAvg(b.rr.y, b.gg.y), artisticAvg(b.rr.z, b.gg.z));
-    V3 one(1.f);
-    V3 den = one - Favg * (one - V3(b.Eavg));
-    b.fmsScale = V3(sqr(Favg.x) * b.Eavg / den.x, sqr(Favg.y) * b.Eavg / den.y, sqr(Favg.z) * b.Eavg / den.z);
-    b.flags = BsdfType::GlossyReflection;
-    b.importance = 1.f;
-  }
-  else if (m.kind == kDielectric)
-  {
-    // material/standard.cpp:436-437 (fromOutside = !backfacing; the media interface sets
-    // backfacing = (mediumId == mediumIntId), material/material.h:66-76)
-    bool backfacing = (mediumId == mediumIntId);
-    float ior = m.ior < 1.f ? 1.0f / m.ior : m.ior;
-    float eta = !backfacing ? 1.0f / ior : ior;
-    eta = clampf(eta, 1.f / 3.f, 3.f);
-    b.deta = eta;
-    // rough_dielectric.h:279-291
-    b.EavgEta = dielAlbedoAvg(L, eta, roughness);
-    b.EavgRcpEta = dielAlbedoAvg(L, 1.0f / eta, roughness);
-    float FavgEta = fresnelDielectricAvg(eta);
-    float FavgRcpEta = fresnelDielectricAvg(1.0f / eta);
-    float a = (1.f - FavgEta) / (1.f - b.EavgRcpEta);
-    float bb = (1.f - FavgRcpEta) / (1.f - b.EavgEta);
-    float x = bb / (a + bb);
-    b.fmsRatio = 1.f - x * (1.f - FavgEta);
-    b.flags = BsdfType::GlossyReflection | BsdfType::GlossyTransmission;
-    b.importance = 1.f;
-  }
-  else if (m.kind == kCoated)
-  {
-    // coated diffuse (material/standard.cpp:512-518, 580-597): RoughCoatingBsdf over a diffuse
-    // base; opaque materials are never media interfaces, so fromOutside holds and
-    // eta = ambientIor / coatIor (material/material.h:54-61)
-    b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
-    b.coatColor = V3(clampf(m.er, 0.f, 1.f), clampf(m.eg, 0.f, 1.f), clampf(m.eb, 0.f, 1.f));
-    b.thickness = std::max(m.thickness, 0.f);
-    b.scale = clampf(m.weight, 0.f, 1.f);
-    float ior = m.ior < 1.f ? 1.0f / m.ior : m.ior;
-    float eta = clampf(1.0f / ior, 1.f / 3.f, 3.f);
-    // rough_coating.h:287-295: the member eta is folded to <= 1, Ravg uses the argument as given
-    b.Ravg = dielReflAlbedoAvg(L, eta, roughness) * b.scale;
-    b.ceta = eta <= 1.f ? eta : 1.0f / eta;
-    b.baseImportance = reduceMax(b.color);  // bsdf/diffuse.h:70-73
-    b.flags = BsdfType::DiffuseReflection | BsdfType::GlossyReflection;
-    b.importance = std::max(1.f, b.baseImportance);  // rough_coating.h:562-565
-  }
+    initConductor(L, b, clamp01(m.r, m.g, m.b), clamp01(m.er, m.eg, m.eb), roughness);
+  else if (m.kind == kDielectric)  // material/standard.cpp:436-437 (fromOutside = !backfacing)
+    initDielectric(L, b, clampf(!backfacing ? 1.0f / foldIor(m.ior) : foldIor(m.ior), 1.f / 3.f, 3.f), roughness);
+  else if (m.kind == kCoated)  // material/standard.cpp:512-518, 580-597: never a media interface
+    initCoated(L, b, clamp01(m.r, m.g, m.b), b.diffRough, clamp01(m.er, m.eg, m.eb), std::max(m.thickness, 0.f),
+               clampf(m.weight, 0.f, 1.f), clampf(1.0f / foldIor(m.ior), 1.f / 3.f, 3.f), roughness);
   else if (m.kind == kThinDielectric)
-  {
-    // thin_rough_dielectric.h:180-192; thin-walled, always seen from outside (standard.cpp:429)
-    float ior = m.ior < 1.f ? 1.0f / m.ior : m.ior;
-    b.teta = clampf(1.0f / ior, 1.f / 3.f, 3.f);
-    b.color = V3(clampf(m.r, 0.f, 1.f), clampf(m.g, 0.f, 1.f), clampf(m.b, 0.f, 1.f));
-    b.tthick = std::max(m.thickness, 0.f);
-    b.tEavg = ggxAlbedoAvg(L, roughness);
-    float Favg = fresnelDielectricAvg(b.teta);
-    b.tfms = Favg * (1.f - b.tEavg) / (1.f - Favg * b.tEavg);
-    b.flags = BsdfType::GlossyReflection | BsdfType::UnbentTransmission;
-    b.importance = 1.f;
-  }
+    initThin(L, b, clampf(1.0f / foldIor(m.ior), 1.f / 3.f, 3.f), clamp01(m.r, m.g, m.b), std::max(m.thickness, 0.f),
+             roughness);
+  else
+    initStandard(L, b, m, m.thinWalled || !backfacing);
   if (b.sheen) b.flags |= BsdfType::DiffuseReflection;  // sheen.h:210-211
   if (b.mix == 1) b.flags |= BsdfType::Transparency;
   if (b.mix == 2) b.flags = BsdfType::Transparency;
@@ -962,8 +1101,9 @@ inline float refractCos(float cosI, float eta)  // optics.h:50-56
   float k = 1.f - eta * eta * (1.f - cosI * cosI);
   return std::sqrt(std::max(k, 0.f));
 }
-// absorption through the coat and energy conservation for a diffuse base (:331-350)
-inline V3 coatBase(const Luts& L, const Bsdf& b, V3 baseValue, float cosO, float cosI, float Ro)
+// absorption through the coat and energy conservation (:331-350): Kelemen's term for a base with
+// diffuse lobes only, min(1 - Ro, 1 - Ri) for any other base [Kulla and Conty 2017]
+inline V3 coatBase(const Luts& L, const Bsdf& b, V3 baseValue, float cosO, float cosI, float Ro, bool diffuseOnly = true)
 {
   float length = 1.0f / refractCos(cosO, b.ceta) + 1.0f / refractCos(std::fabs(cosI), b.ceta);
   if (cosI <= 0.f) length *= 0.5f;
@@ -972,7 +1112,7 @@ inline V3 coatBase(const Luts& L, const Bsdf& b, V3 baseValue, float cosO, float
                baseValue.z * std::pow(b.coatColor.z, t));
   baseValue = baseValue + b.scale * (absorbed - baseValue);
   float Ri = dielReflAlbedo(L, std::fabs(cosI), b.ceta, b.roughness) * b.scale;
-  float T = (1.f - Ro) * (1.f - Ri) * (1.0f / (1.f - b.Ravg));
+  float T = diffuseOnly ? (1.f - Ro) * (1.f - Ri) * (1.0f / (1.f - b.Ravg)) : std::min(1.f - Ro, 1.f - Ri);
   return baseValue * T;
 }
 inline float coatPickProb(const Bsdf& b, float Ro)
@@ -993,7 +1133,10 @@ inline float coatReflection(const Bsdf& b, V3 wo, V3 wi, V3 wh, float cosO, floa
   coatPdf = whPdf * (1.0f / (4.f * std::fabs(cosOH)));
   return F * D * G * (1.0f / (4.f * cosO));
 }
-inline V3 coatEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+// RoughCoatingBsdfSimd::eval over a base (:297-378); Base: eval(wo, wi, pdf), sample(wo, wi, pdf,
+// type, sx, sy, sc), diffuseOnly()
+template <class Base>
+inline V3 coatEvalOver(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf, const Base& base)
 {
   pdf = 0.f;
   float cosO = dot(wo, b.fr.N);
@@ -1001,9 +1144,9 @@ inline V3 coatEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, f
   float cosI = dot(wi, b.fr.N);
   if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
   float basePdf;
-  V3 baseValue = diffuseEval(b, wo, wi, basePdf);
+  V3 baseValue = base.eval(wo, wi, basePdf);
   float Ro = dielReflAlbedo(L, cosO, b.ceta, b.roughness) * b.scale;
-  baseValue = coatBase(L, b, baseValue, cosO, cosI, Ro);
+  baseValue = coatBase(L, b, baseValue, cosO, cosI, Ro, base.diffuseOnly());
   float coatPick = coatPickProb(b, Ro), basePick = 1.f - coatPick;
   if (cosI > 0.f)
   {
@@ -1015,7 +1158,10 @@ inline V3 coatEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, f
   pdf = basePdf * basePick;
   return baseValue;
 }
-inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+// RoughCoatingBsdfSimd::sample over a base (:380-537)
+template <class Base>
+inline V3 coatSampleOver(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy,
+                         float sc, const Base& base)
 {
   float cosO = dot(wo, b.fr.N);
   if (!(cosO > 0.f)) return V3(0.f);
@@ -1034,25 +1180,29 @@ inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
     wi = reflect(wo, wh, dot(wo, wh));
     if (!(dot(wi, b.fr.N) > 0.f)) return V3(0.f);
     type = BsdfType::GlossyReflection;
-    baseValue = diffuseEval(b, wo, wi, basePdf);
+    baseValue = base.eval(wo, wi, basePdf);
   }
   else
   {
-    // the diffuse base uses only the 2D part of the remapped sample (:424-427)
     V3 baseWi;
-    float basePdf2;
-    int baseType;
-    V3 baseWeight = diffuseSample(b, ctx, wo, baseWi, basePdf2, baseType, sx, sy);
+    float basePdf2 = 0.f;
+    int baseType = 0;
+    V3 baseWeight = base.sample(wo, baseWi, basePdf2, baseType, sx, sy, (sc - coatPick) * (1.0f / basePick));
     if (!(reduceMax(baseWeight) > 0.f)) return V3(0.f);
     wi = baseWi;
     basePdf = basePdf2;
     type = baseType;
-    baseValue = baseWeight * basePdf;
+    baseValue = baseWeight * ((type & BsdfType::Delta) ? 1.f : basePdf);
     wh = normalize(wo + wi);
   }
   float cosI = dot(wi, b.fr.N);
   if (!(cosI * dot(wi, ctx.Ng) > 0.f)) return V3(0.f);
-  baseValue = coatBase(L, b, baseValue, cosO, cosI, Ro);
+  baseValue = coatBase(L, b, baseValue, cosO, cosI, Ro, base.diffuseOnly());
+  if (type & BsdfType::Delta)  // a delta base sample: no coat reflection to add (:484-490)
+  {
+    pdf = basePdf;
+    return baseValue * (1.0f / basePick);
+  }
   if (cosI <= 0.f)
   {
     pdf = basePick * basePdf;
@@ -1062,6 +1212,26 @@ inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi
   float coatValue = coatReflection(b, wo, wi, wh, cosO, coatPdf);
   pdf = coatPick * coatPdf + basePick * basePdf;
   return (V3(coatValue) + baseValue) * (1.0f / pdf);
+}
+// the diffuse base of the coated kind (it reads only the 2D part of the remapped sample, :424-427)
+struct DiffuseBase
+{
+  const Bsdf& b;
+  const Ctx& ctx;
+  V3 eval(V3 wo, V3 wi, float& pdf) const { return diffuseEval(b, wo, wi, pdf); }
+  V3 sample(V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float) const
+  {
+    return diffuseSample(b, ctx, wo, wi, pdf, type, sx, sy);
+  }
+  bool diffuseOnly() const { return true; }
+};
+inline V3 coatEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  return coatEvalOver(L, b, ctx, wo, wi, pdf, DiffuseBase{b, ctx});
+}
+inline V3 coatSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  return coatSampleOver(L, b, ctx, wo, wi, pdf, type, sx, sy, sc, DiffuseBase{b, ctx});
 }
 
 // --- thin rough dielectric (bsdf/thin_rough_dielectric.h:170-345)
@@ -1136,36 +1306,34 @@ inline V3 thinTransparency(const Luts& L, const Bsdf& b, V3 wo)  // :307-321
   return thinTransmittance(b, cosO) * (1.f - R);
 }
 
-// The lobe: MultiBsdf with one child of weight 1 (bsdf/multi_bsdf.h:166-210): eval pdf is
-// scaled by importance / importanceSum; sample returns the child's weight times the child scale
-// (1); a zero importance sum disables sampling.
-inline V3 lobeEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+// One lobe of a single kind, or the transparent BTDF (bsdf/transparent.h:32-71: evaluates to
+// zero, samples -wo with pdf 1).
+inline V3 leafEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
 {
-  V3 v(0.f);
   pdf = 0.f;
-  if (b.importance > 0.f)
-  {
-    float cp = 0.f;
-    if (b.kind == kDiffuse) v = diffuseEval(b, wo, wi, cp);
-    else if (b.kind == kConductor) v = conductorEval(L, b, wo, wi, cp);
-    else if (b.kind == kCoated) v = coatEval(L, b, ctx, wo, wi, cp);
-    else if (b.kind == kThinDielectric) v = thinEval(L, b, ctx, wo, wi, cp);
-    else v = dielEval(L, b, ctx, wo, wi, cp);
-    pdf = cp * b.importance;
-  }
-  pdf *= 1.0f / b.importance;
-  return v;
+  if (b.kind == kDiffuse) return diffuseEval(b, wo, wi, pdf);
+  if (b.kind == kConductor) return conductorEval(L, b, wo, wi, pdf);
+  if (b.kind == kCoated) return coatEval(L, b, ctx, wo, wi, pdf);
+  if (b.kind == kThinDielectric) return thinEval(L, b, ctx, wo, wi, pdf);
+  if (b.kind == kTransparentLobe) return V3(0.f);
+  return dielEval(L, b, ctx, wo, wi, pdf);
 }
-inline V3 lobeSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+inline V3 leafSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
 {
-  if (!(b.importance > 0.f)) return V3(0.f);
   if (b.kind == kDiffuse) return diffuseSample(b, ctx, wo, wi, pdf, type, sx, sy);
   if (b.kind == kConductor) return conductorSample(L, b, ctx, wo, wi, pdf, type, sx, sy);
   if (b.kind == kCoated) return coatSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
   if (b.kind == kThinDielectric) return thinSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
+  if (b.kind == kTransparentLobe)
+  {
+    wi = -wo;
+    type = BsdfType::Transparency;
+    pdf = 1.f;
+    return V3(1.f);
+  }
   return dielSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
 }
-inline V3 lobeAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
+inline V3 leafAlbedo(const Bsdf& b)
 {
   if (b.kind == kDiffuse) return b.color;
   if (b.kind == kConductor) return b.rr;
@@ -1177,7 +1345,174 @@ inline V3 lobeAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child
   }
   if (b.kind == kThinDielectric)  // thin_rough_dielectric.h:324-327
     return V3(std::pow(b.color.x, b.tthick), std::pow(b.color.y, b.tthick), std::pow(b.color.z, b.tthick));
+  if (b.kind == kTransparentLobe) return V3(-1.f);  // transparent.h:52-55
   return V3(1.f);
+}
+inline V3 leafTransparency(const Luts& L, const Bsdf& b, V3 wo)  // what a shadow ray keeps through the lobe
+{
+  if (b.kind == kThinDielectric) return thinTransparency(L, b, wo);
+  return V3(b.kind == kTransparentLobe ? 1.f : 0.f);
+}
+
+// kind 5's weighted mix of bases (multi_bsdf.h:166-287): eval sums the children's values times
+// their scales and their pdfs times their importances; sample picks a child by importance with
+// the remapped sample, then (unless the pick is a delta) adds the other children's values and pdfs.
+inline V3 mixEval(const Luts& L, const StdLobes& s, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  V3 value(0.f);
+  pdf = 0.f;
+  for (int i = 0; i < s.n; ++i)
+  {
+    if (!(s.imp[i] > 0.f)) continue;
+    float cp;
+    V3 cv = leafEval(L, s.child[i], ctx, wo, wi, cp);
+    value += cv * s.scale[i];
+    pdf += cp * s.imp[i];
+  }
+  pdf *= 1.0f / s.impSum;
+  return value;
+}
+inline V3 mixSample(const Luts& L, const StdLobes& s, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  if (!(s.impSum > 0.f)) return V3(0.f);
+  if (s.n == 1) return leafSample(L, s.child[0], ctx, wo, wi, pdf, type, sx, sy, sc) * s.scale[0];
+  const float x = sc * s.impSum;
+  int choice = 0;
+  float prefix = s.imp[0], imp = s.imp[0], scale = s.scale[0];
+  for (int i = 1; i < s.n; ++i)
+  {
+    if (!(x > prefix)) break;
+    choice = i;
+    prefix += s.imp[i];
+    imp = s.imp[i];
+    scale = s.scale[i];
+  }
+  const float sc1 = (x + imp - prefix) * (1.0f / imp);
+  V3 cwi(0.f);
+  float cpdf = 0.f;
+  int ctype = 0;
+  V3 cw = leafSample(L, s.child[choice], ctx, wo, cwi, cpdf, ctype, sx, sy, sc1);
+  V3 w(0.f);
+  wi = V3(0.f);
+  pdf = 0.f;
+  type = 0;
+  if (reduceMax(cw) > 0.f) { w = cw; wi = cwi; pdf = cpdf; type = ctype; }
+  w = w * scale;
+  if (!(reduceMax(w) > 0.f && pdf > 0.f)) return V3(0.f);
+  if (type & BsdfType::Delta) return w * (s.impSum * (1.0f / imp));
+  V3 value = w * pdf;
+  pdf = pdf * imp;
+  for (int i = 0; i < s.n; ++i)
+  {
+    if (i == choice || !(s.imp[i] > 0.f)) continue;
+    float cp;
+    V3 cv = leafEval(L, s.child[i], ctx, wo, wi, cp);
+    value += cv * s.scale[i];
+    pdf += cp * s.imp[i];
+  }
+  pdf *= 1.0f / s.impSum;
+  return value * (1.0f / pdf);
+}
+// getAlbedo (:314-337): children with a negative albedo (the transparent BTDF) are left out and
+// the rest rescaled to the total scale
+inline V3 mixAlbedo(const StdLobes& s)
+{
+  V3 value(0.f);
+  float scaleSum = 0.f, total = 0.f;
+  for (int i = 0; i < s.n; ++i)
+  {
+    if (!(s.imp[i] > 0.f)) continue;
+    V3 cv = leafAlbedo(s.child[i]);
+    total += s.scale[i];
+    if (!(cv.x >= 0.f)) continue;
+    value += cv * s.scale[i];
+    scaleSum += s.scale[i];
+  }
+  return value * (total * rcpSafe(scaleSum));
+}
+// getNormal (:408-428): the first child with a non-zero importance and a frame
+inline bool mixHasNormal(const StdLobes& s)
+{
+  for (int i = 0; i < s.n; ++i)
+    if (s.imp[i] > 0.f && s.child[i].kind != kTransparentLobe) return true;
+  return false;
+}
+inline V3 mixTransparency(const Luts& L, const StdLobes& s, V3 wo)  // :289-305
+{
+  V3 value(0.f);
+  for (int i = 0; i < s.n; ++i)
+    if (s.imp[i] > 0.f) value += leafTransparency(L, s.child[i], wo) * s.scale[i];
+  return value;
+}
+struct MixBase
+{
+  const Luts& L;
+  const StdLobes& s;
+  const Ctx& ctx;
+  V3 eval(V3 wo, V3 wi, float& pdf) const { return mixEval(L, s, ctx, wo, wi, pdf); }
+  V3 sample(V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc) const
+  {
+    return mixSample(L, s, ctx, wo, wi, pdf, type, sx, sy, sc);
+  }
+  bool diffuseOnly() const { return (s.flags & ~BsdfType::Diffuse) == 0; }  // rough_coating.h:349
+};
+
+// The material's lobe: a MultiBsdf with one child of weight 1 for kinds 0-4 (eval scales the
+// child's pdf by importance / importanceSum, sample returns the child's weight times its scale 1,
+// a zero importance disables sampling), kind 5's mix of bases, under its coat when it has one.
+inline V3 lobeEval(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3 wi, float& pdf)
+{
+  if (b.kind == kStandard)
+  {
+    const MixBase base{L, *b.mixLobes, ctx};
+    return b.coat ? coatEvalOver(L, b, ctx, wo, wi, pdf, base) : base.eval(wo, wi, pdf);
+  }
+  V3 v(0.f);
+  pdf = 0.f;
+  if (b.importance > 0.f)
+  {
+    float cp = 0.f;
+    v = leafEval(L, b, ctx, wo, wi, cp);
+    pdf = cp * b.importance;
+  }
+  pdf *= 1.0f / b.importance;
+  return v;
+}
+inline V3 lobeSample(const Luts& L, const Bsdf& b, const Ctx& ctx, V3 wo, V3& wi, float& pdf, int& type, float sx, float sy, float sc)
+{
+  if (b.kind == kStandard)
+  {
+    const MixBase base{L, *b.mixLobes, ctx};
+    return b.coat ? coatSampleOver(L, b, ctx, wo, wi, pdf, type, sx, sy, sc, base) : base.sample(wo, wi, pdf, type, sx, sy, sc);
+  }
+  if (!(b.importance > 0.f)) return V3(0.f);
+  return leafSample(L, b, ctx, wo, wi, pdf, type, sx, sy, sc);
+}
+inline V3 lobeAlbedo(const Bsdf& b)  // multi_bsdf.h:314-337 with one child; kind 5: the mix, coated
+{
+  if (b.kind != kStandard) return leafAlbedo(b);
+  V3 a = mixAlbedo(*b.mixLobes);
+  if (!b.coat) return a;
+  V3 absorbed(a.x * std::pow(b.coatColor.x, b.thickness), a.y * std::pow(b.coatColor.y, b.thickness),
+               a.z * std::pow(b.coatColor.z, b.thickness));
+  return a + b.scale * (absorbed - a);  // rough_coating.h:567-571
+}
+// whether the lobe reports a normal (the first child with an importance and a frame)
+inline bool lobeHasNormal(const Bsdf& b)
+{
+  return b.kind == kStandard ? mixHasNormal(*b.mixLobes) : b.importance > 0.f;
+}
+inline V3 lobeTransparency(const Luts& L, const Bsdf& b, V3 wo)
+{
+  if (b.kind != kStandard) return b.importance > 0.f ? leafTransparency(L, b, wo) : V3(0.f);
+  V3 t = mixTransparency(L, *b.mixLobes, wo);
+  if (!b.coat) return t;
+  float cosO = dot(wo, b.fr.N);  // rough_coating.h:539-560
+  if (!(cosO > 0.f)) return V3(0.f);
+  const float e = b.thickness * (1.0f / refractCos(cosO, b.ceta));
+  V3 absorbed(t.x * std::pow(b.coatColor.x, e), t.y * std::pow(b.coatColor.y, e), t.z * std::pow(b.coatColor.z, e));
+  t = t + b.scale * (absorbed - t);
+  return t * (1.f - dielReflAlbedo(L, cosO, b.ceta, b.roughness) * b.scale);
 }
 
 // --- sheen layer over the lobe (bsdf/sheen.h:199-451)
@@ -1348,13 +1683,13 @@ inline V3 bsdfAlbedo(const Bsdf& b)
 // getNormal (multi_bsdf.h:408-428): the first child with a non-zero importance
 inline V3 bsdfNormal(const Bsdf& b)
 {
-  return (b.mix != 2 && b.importance > 0.f && (b.mix == 0 || surfImportance(b) * b.opacity > 0.f)) ? b.fr.N : V3(0.f);
+  return (b.mix != 2 && lobeHasNormal(b) && (b.mix == 0 || surfImportance(b) * b.opacity > 0.f)) ? b.fr.N : V3(0.f);
 }
 // getTransparency of a transparent material (material/standard.cpp:633-661)
 inline V3 bsdfTransparency(const Luts& L, const Bsdf& b, V3 wo)
 {
   if (b.mix == 2) return V3(1.f);
-  V3 t = (b.kind == kThinDielectric && b.importance > 0.f) ? thinTransparency(L, b, wo) : V3(0.f);
+  V3 t = lobeTransparency(L, b, wo);
   if (b.sheen)  // sheen.h:393-401
   {
     float cosO = dot(wo, b.fr.N);

@@ -72,7 +72,8 @@ static void assignMedia(Scene& s)
     if (!isMediaInterface(s.mats[m])) continue;
     const float ior = interfaceIor(s.mats[m]);
     const V3 col = interiorColor(s.mats[m]);
-    int id = -1;
+    // a clear medium at the ambient IOR is the ambient medium itself
+    int id = (ior == 1.f && col.x == 1.f && col.y == 1.f && col.z == 1.f) ? 0 : -1;
     for (size_t k = 0; k < iors.size(); ++k)
       if (iors[k] == ior && s.mediumColor[k + 1].x == col.x && s.mediumColor[k + 1].y == col.y &&
           s.mediumColor[k + 1].z == col.z)
@@ -1068,7 +1069,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
         nextO = ctx.p; nextD = wi; nextNear = ctx.eps;
         if ((type & BsdfType::Continuation) == 0) { lastPdf = pdf; lastType = type; }
         // material.h:84-95: crossing a media interface swaps interior/exterior
-        if (m.kind == kDielectric && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
+        if (isMediaInterface(m) && dot(wo, ctx.Ng) * dot(wi, ctx.Ng) < 0.f)
           mediumId = (mediumId == mediumIntId) ? 0 : mediumIntId;
       }
       if (adj)
@@ -1638,7 +1639,7 @@ void orc_bsdf(int n, const float* luts, const float* params, const float* geo, c
 // outSample n x 7 [weight wi pdf], outType n, outEval n x 4 [value pdf], outAux n x 9
 // [albedo, normal, transparency].
 void orc_bsdf_mat(int n, const float* luts, const void* mats, const float* geo, const float* s, float* outSample, int* outType,
-                  float* outEval, float* outAux)
+                  float* outEval, float* outAux, const int* inside)
 {
   const Luts& L = *(const Luts*)luts;
   const Material* ms = (const Material*)mats;
@@ -1649,8 +1650,15 @@ void orc_bsdf_mat(int n, const float* luts, const void* mats, const float* geo, 
     ctx.f.U = V3(g[0], g[1], g[2]); ctx.f.V = V3(g[3], g[4], g[5]); ctx.f.N = V3(g[6], g[7], g[8]);
     ctx.Ng = V3(g[9], g[10], g[11]);
     V3 wo(g[12], g[13], g[14]), wi(g[15], g[16], g[17]);
-    Bsdf b = makeBsdf(L, ms[i], ctx, wo, 0, -1);
+    // inside[i]: the path is in the material's interior medium (a media interface is backfacing)
+    Bsdf b = makeBsdf(L, ms[i], ctx, wo, (inside && inside[i]) ? 1 : 0, 1);
     b.fr = ctx.f;
+    if (b.mixLobes)
+    {
+      auto lobes = std::make_shared<StdLobes>(*b.mixLobes);
+      for (int k = 0; k < lobes->n; ++k) lobes->child[k].fr = ctx.f;
+      b.mixLobes = lobes;
+    }
     V3 swi(0.f);
     float spdf = 0.f;
     int stype = 0;

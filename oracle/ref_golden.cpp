@@ -20,6 +20,9 @@
 //   lobes     bsdf/rough_diffuse.h:55-106, bsdf/thin_rough_dielectric.h:170-345,
 //             bsdf/transparent.h:26-71, bsdf/multi_bsdf.h:151-459, composed as
 //             material/standard.cpp:409-627 composes them (opacity mix, coat over rough diffuse)
+//   standard  material/standard.cpp:409-627 restated for uniform parameters over the classes
+//             above: weighted opaque / translucent / metal bases, a coat over any base, sheen,
+//             opacity (bsdf/rough_coating.h with a generic base, bsdf/multi_bsdf.h)
 //   postisect geometry/triangle_mesh.cpp:152-257, core/ray_simd.h:63-68,
 //             core/shading_context.h:144-191 (makeFrame)
 
@@ -542,6 +545,255 @@ static void goldenPostIntersect()
 // Camera matrices of PinholeCamera (camera/pinhole_camera.h:41-68; ThinLensCamera inherits them):
 // worldToViewD3D, viewToClipD3D and worldToRaster, written row-major (m[r][c] = column c, row r).
 // in: [width, height, fov, nearClip, pos xyz, U xyz, V xyz, N xyz]
+// StandardMaterial's lobes for uniform parameters (material/standard.cpp:409-627, Default mode
+// with an ambient IOR of 1): the opaque base (diffuse or rough diffuse, under a specular coating
+// of weight specularWeight when the IOR differs from the ambient one), the translucent base (rough
+// dielectric, thin rough dielectric when thin-walled, the transparent BTDF at the ambient IOR) and
+// the metal base (rough conductor, artistic Fresnel) in one MultiBsdfSimd weighted
+// (1 - metalness)(1 - transmission), (1 - metalness) transmission specular and metalness specular;
+// a rough coating of weight coatWeight over that mix; the sheen layer; the opacity mix.  The
+// orientation follows fromOutside = thinWalled || !backfacing.  wantOpaque = false builds the
+// lobes getTransparency uses (:633-661).  Row parameters (std_params, 33 columns): baseWeight,
+// baseColor rgb, metalness, diffuseRoughness, specularWeight, specularColor rgb,
+// specularRoughness, specularIOR, transmissionWeight, transmissionColor rgb, transmissionDepth,
+// thinWalled, thickness, coatWeight, coatColor rgb, coatRoughness, coatIOR, coatThickness,
+// sheenWeight, sheenColor rgb, sheenRoughness, opacity, backfacing.
+struct BsdfPool
+{
+  std::vector<BsdfSimd*> v;
+  template <class T, class... A>
+  T* make(A&&... a)
+  {
+    T* p = new T(std::forward<A>(a)...);
+    v.push_back(p);
+    return p;
+  }
+  ~BsdfPool()
+  {
+    for (BsdfSimd* b : v) delete b;
+  }
+};
+
+static const BsdfSimd* standardLobes(vbool m, const Basis3vf* frame, const float* p, bool wantOpaque, BsdfPool& pool)
+{
+  const float eps = 1e-5f;  // material/standard.h:79
+  const float ambientIor = 1.f;
+  auto cl = [](float x) { return std::min(std::max(x, 0.f), 1.f); };
+  auto rcpIor = [](float x) { x = std::max(x, 0.f); return x < 1.f ? 1.f / x : x; };
+  const float opacity = cl(p[31]), transparency = 1.f - opacity;
+  const BsdfSimd* bsdf = nullptr;
+  if (opacity > eps)
+  {
+    const float base = cl(p[0]);
+    const Vec3f baseColor(cl(p[1]), cl(p[2]), cl(p[3]));
+    const float metalness = cl(p[4]), diffRough = cl(p[5]), specular = cl(p[6]);
+    const Vec3f specColor(cl(p[7]), cl(p[8]), cl(p[9]));
+    const float specRough = cl(p[10]), specIor = rcpIor(p[11]);
+    const Vec3f transColor(cl(p[13]), cl(p[14]), cl(p[15]));
+    const float transDepth = std::max(p[16], 0.f);
+    const bool thin = p[17] != 0.f;
+    const bool fromOutside = thin ? true : !(p[32] != 0.f);
+    MultiBsdfSimd* baseBsdf = pool.make<MultiBsdfSimd>();
+    const float dielectric = 1.f - metalness;
+    if (dielectric > eps)
+    {
+      const float transmission = cl(p[12]);
+      float eta = fromOutside ? ambientIor / specIor : specIor / ambientIor;
+      eta = std::min(std::max(eta, 1.f / 3.f), 3.f);
+      if (wantOpaque)
+      {
+        const float opaque = dielectric * (1.f - transmission);
+        if (opaque > eps)
+        {
+          BsdfSimd* opaqueBsdf;
+          const Vec3vf diffuseColor(baseColor * base);
+          if (diffRough != 0.f) opaqueBsdf = pool.make<RoughDiffuseBrdfSimd>(m, frame, diffuseColor, vfloat(diffRough));
+          else opaqueBsdf = pool.make<DiffuseBrdfSimd>(m, frame, diffuseColor);
+          if (std::fabs(specIor - ambientIor) > eps && specular > eps)
+            opaqueBsdf = pool.make<RoughCoatingBsdfSimd>(m, frame, opaqueBsdf, vfloat(eta), Vec3vf(1.f), vfloat(1.f),
+                                                         vfloat(specRough), vfloat(0.f), vfloat(specular));
+          baseBsdf->add(m, opaqueBsdf, vfloat(opaque));
+        }
+      }
+      const float translucent = dielectric * transmission * specular;
+      if (translucent > eps)
+      {
+        BsdfSimd* tb;
+        if (std::fabs(specIor - ambientIor) > eps)
+        {
+          if (thin)
+            tb = pool.make<ThinRoughDielectricBsdfSimd>(m, frame, vfloat(eta), vfloat(specRough), vfloat(0.f), Vec3vf(transColor),
+                                                        vfloat(std::max(p[18], 0.f) / transDepth));
+          else tb = pool.make<RoughDielectricBsdfSimd>(m, frame, vfloat(eta), vfloat(specRough), vfloat(0.f));
+        }
+        else tb = pool.make<TransparentBtdfSimd>();
+        baseBsdf->add(m, tb, vfloat(translucent));
+      }
+    }
+    if (wantOpaque)
+    {
+      const float metal = metalness * specular;
+      if (metal > eps)
+      {
+        FresnelConductorArtistic<vfloat> fresnel(Vec3vf(baseColor * base), Vec3vf(specColor));
+        BsdfSimd* metalBsdf = pool.make<RoughConductorBrdfSimd<FresnelConductorArtistic<vfloat>>>(m, frame, fresnel, vfloat(specRough), vfloat(0.f));
+        baseBsdf->add(m, metalBsdf, vfloat(metal));
+      }
+    }
+    bsdf = baseBsdf;
+    const float coat = cl(p[19]), coatIor = rcpIor(p[24]);
+    if (coat > eps && std::fabs(coatIor - ambientIor) > eps)
+    {
+      float coatEta = fromOutside ? ambientIor / coatIor : coatIor / ambientIor;
+      coatEta = std::min(std::max(coatEta, 1.f / 3.f), 3.f);
+      const Vec3f coatColor(cl(p[20]), cl(p[21]), cl(p[22]));
+      bsdf = pool.make<RoughCoatingBsdfSimd>(m, frame, bsdf, vfloat(coatEta), Vec3vf(coatColor), vfloat(std::max(p[25], 0.f)),
+                                             vfloat(cl(p[23])), vfloat(0.f), vfloat(coat));
+    }
+    const float sheen = cl(p[26]);
+    if (sheen > eps)
+      bsdf = pool.make<SheenBsdfSimd>(m, frame, bsdf, Vec3vf(Vec3f(cl(p[27]), cl(p[28]), cl(p[29]))), vfloat(cl(p[30])), vfloat(sheen));
+  }
+  if (transparency > eps)
+  {
+    BsdfSimd* tr = pool.make<TransparentBtdfSimd>();
+    if (bsdf)
+    {
+      MultiBsdfSimd* blend = pool.make<MultiBsdfSimd>();
+      blend->add(m, bsdf, vfloat(opacity));
+      blend->add(m, tr, vfloat(transparency));
+      bsdf = blend;
+    }
+    else bsdf = tr;
+  }
+  return bsdf;
+}
+
+// Variants by batch: 0 partial metalness; 1 partial transmission (rough dielectric beside the
+// coated diffuse base), seen from outside or inside; 2 a coat over metal; 3 a coat over glass,
+// from outside or inside; 4 every lobe at once with sheen, opacity and a rough diffuse base;
+// 5 thin-walled partial transmission under a coat (at the ambient IOR in every third batch: the
+// transparent BTDF); 6 base and specular weights below 1 with partial metalness; 7 a coat over a
+// diffuse-only base (specular weight 0, the Kelemen energy term), with and without metal.
+static void goldenStandard()
+{
+  InputRng rng(8484);
+  const int nTotal = 1024;
+  std::vector<float> params, geo, samp, outSample, outEval, outAlb, outNormal, outTransp;
+  std::vector<int32_t> outType;
+  int n = 0;
+  for (int i = 0; i < nTotal; i += simdSize)
+  {
+    const int batch = i / simdSize, variant = batch % 8;
+    float p[33];
+    auto u = [&](float a, float b) { return a + (b - a) * rng.f(); };
+    p[0] = 1.f;
+    p[1] = u(0.05f, 0.95f); p[2] = u(0.05f, 0.95f); p[3] = u(0.05f, 0.95f);
+    p[4] = 0.f;
+    p[5] = 0.f;
+    p[6] = 1.f;
+    p[7] = u(0.3f, 1.f); p[8] = u(0.3f, 1.f); p[9] = u(0.3f, 1.f);
+    p[10] = std::sqrt(0.01f + 0.5f * rng.f());
+    const float iors[3] = {1.5f, 1.33f, 2.2f};
+    p[11] = iors[batch % 3];
+    p[12] = 0.f;
+    p[13] = u(0.2f, 1.f); p[14] = u(0.2f, 1.f); p[15] = u(0.2f, 1.f);
+    p[16] = u(0.5f, 2.f);
+    p[17] = 0.f;
+    p[18] = 1.f;
+    p[19] = 0.f;
+    p[20] = u(0.3f, 1.f); p[21] = u(0.3f, 1.f); p[22] = u(0.3f, 1.f);
+    p[23] = std::sqrt(0.01f + 0.3f * rng.f());
+    p[24] = (batch % 2) ? 1.6f : 1.45f;
+    p[25] = 0.5f;
+    p[26] = 0.f;
+    p[27] = u(0.2f, 1.f); p[28] = u(0.2f, 1.f); p[29] = u(0.2f, 1.f);
+    p[30] = u(0.1f, 0.9f);
+    p[31] = 1.f;
+    p[32] = 0.f;
+    const bool inside = (batch / 8) % 2 == 1;
+    switch (variant)
+    {
+      case 0: p[4] = u(0.2f, 0.8f); break;
+      case 1: p[12] = u(0.2f, 0.8f); p[32] = inside ? 1.f : 0.f; break;
+      case 2: p[4] = 1.f; p[19] = u(0.3f, 1.f); break;
+      case 3: p[12] = 1.f; p[19] = u(0.3f, 1.f); p[32] = inside ? 1.f : 0.f; break;
+      case 4:
+        p[4] = u(0.2f, 0.5f); p[12] = u(0.2f, 0.6f); p[19] = u(0.3f, 0.9f); p[26] = u(0.2f, 0.9f); p[31] = u(0.5f, 0.95f);
+        p[5] = u(0.1f, 0.9f);
+        break;
+      case 5:
+        p[12] = u(0.3f, 0.9f); p[17] = 1.f; p[18] = u(0.2f, 1.5f); p[19] = u(0.3f, 0.9f);
+        if ((batch / 8) % 3 == 0) p[11] = 1.f;
+        break;
+      case 6: p[0] = u(0.3f, 0.9f); p[6] = u(0.2f, 0.9f); p[4] = u(0.2f, 0.8f); break;
+      default: p[6] = 0.f; p[19] = u(0.3f, 1.f); p[4] = ((batch / 8) % 2) ? u(0.2f, 0.6f) : 0.f; break;
+    }
+
+    Basis3vf frame;
+    ShadingContextSimd* ctx = new ShadingContextSimd();
+    Vec3vf wo, wi, s3;
+    for (int k = 0; k < simdSize; ++k)
+    {
+      Vec3f N = rng.dir();
+      Vec3f U, V;
+      makeFrame(U, V, N);
+      Vec3f Ng = N;
+      if (rng.f() < 0.3f) { Ng = normalize(N + 0.2f * rng.dir()); if (dot(Ng, N) < 0.2f) Ng = N; }
+      Vec3f o = rng.dir();
+      if (rng.f() < 0.9f && dot(o, N) < 0.f) o = -o;
+      Vec3f w = rng.dir();
+      if (rng.f() < 0.6f && dot(w, N) < 0.f) w = -w;
+      frame.U.x[k] = U.x; frame.U.y[k] = U.y; frame.U.z[k] = U.z;
+      frame.V.x[k] = V.x; frame.V.y[k] = V.y; frame.V.z[k] = V.z;
+      frame.N.x[k] = N.x; frame.N.y[k] = N.y; frame.N.z[k] = N.z;
+      ctx->Ng.x[k] = Ng.x; ctx->Ng.y[k] = Ng.y; ctx->Ng.z[k] = Ng.z;
+      wo.x[k] = o.x; wo.y[k] = o.y; wo.z[k] = o.z;
+      wi.x[k] = w.x; wi.y[k] = w.y; wi.z[k] = w.z;
+      s3.x[k] = rng.f(); s3.y[k] = rng.f(); s3.z[k] = rng.f();
+      params.insert(params.end(), p, p + 33);
+      geo.insert(geo.end(), {U.x, U.y, U.z, V.x, V.y, V.z, N.x, N.y, N.z, Ng.x, Ng.y, Ng.z, o.x, o.y, o.z, w.x, w.y, w.z});
+      samp.insert(samp.end(), {s3.x[k], s3.y[k], s3.z[k]});
+    }
+    vbool m = one;
+    BsdfPool pool, poolT;
+    const BsdfSimd* bsdf = standardLobes(m, &frame, p, true, pool);
+    Vec3vf swi = zero;
+    vfloat spdf = zero;
+    vint stype = zero;
+    Vec3vf sw = bsdf->sample(m, *ctx, wo, swi, spdf, stype, BsdfSampleSimd(s3));
+    vfloat epdf;
+    Vec3vf ev = bsdf->eval(m, *ctx, wo, wi, epdf);
+    Vec3vf alb = bsdf->getAlbedo(m);
+    Vec3vf nrm = bsdf->getNormal(m);
+    // StandardMaterial::getTransparency: a transmissive material evaluates the lobes built without
+    // the opaque and metal bases, any other returns 1 - opacity
+    Vec3vf tr;
+    if (p[12] != 0.f) tr = standardLobes(m, &frame, p, false, poolT)->getTransparency(m, *ctx, wo);
+    else tr = Vec3vf(vfloat(1.f - std::min(std::max(p[31], 0.f), 1.f)));
+    for (int k = 0; k < simdSize; ++k)
+    {
+      outSample.insert(outSample.end(), {sw.x[k], sw.y[k], sw.z[k], swi.x[k], swi.y[k], swi.z[k], spdf[k]});
+      outType.push_back(stype[k]);
+      outEval.insert(outEval.end(), {ev.x[k], ev.y[k], ev.z[k], epdf[k]});
+      outAlb.insert(outAlb.end(), {alb.x[k], alb.y[k], alb.z[k]});
+      outNormal.insert(outNormal.end(), {nrm.x[k], nrm.y[k], nrm.z[k]});
+      outTransp.insert(outTransp.end(), {tr.x[k], tr.y[k], tr.z[k]});
+    }
+    delete ctx;
+    n += simdSize;
+  }
+  dump("std_params", "f4", params, {n, 33});
+  dump("std_geo", "f4", geo, {n, 18});
+  dump("std_s", "f4", samp, {n, 3});
+  dump("std_sample", "f4", outSample, {n, 7});
+  dump("std_sample_type", "i4", outType, {n});
+  dump("std_eval", "f4", outEval, {n, 4});
+  dump("std_albedo", "f4", outAlb, {n, 3});
+  dump("std_normal", "f4", outNormal, {n, 3});
+  dump("std_transparency", "f4", outTransp, {n, 3});
+}
+
 static void goldenCameraMatrices()
 {
   InputRng rng(4321);
@@ -852,6 +1104,7 @@ int main(int argc, char** argv)
   goldenPostIntersect();
   goldenLobes();
   goldenSheen();
+  goldenStandard();
   goldenCameraMatrices();
   fprintf(gManifest, "\n}\n");
   fclose(gManifest);

@@ -39,6 +39,28 @@ def _wavefront(dev, cam, W, H, seed, spp):
     return out, dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1), dev.blitFrame(P.BUF_DEPTH)
 
 
+def test_batch_equals_wavefront_renderer_standard_materials():
+    # kind-5 materials: the batch kernel built with them (and, in the same launch, a frame of a
+    # scene without them) equals the wavefront renderer's level-2 shading kernel
+    scS = G.spheres_standard(seed=5)
+    scA = G.spheres_sunsky(seed=1, n_spheres=32, subdiv=2, palette=16)
+    dS, dA = _scene_dev(scS), _scene_dev(scA)
+    cam = D.look_at((0, 2, -2), (1, 1, 15), 50)
+    frames = [(dS, 64, 48, 3, 4), (dA, 32, 32, 1, 4)]
+    batch = P.BatchHip(0)
+    for d, W, H, seed, spp in frames:
+        batch.addFrame(d, cam, W, H, seed, spp, snapshots=(1, 4), aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1 | P.AOV_DEPTH)
+    batch.render()
+    for i, (d, W, H, seed, spp) in enumerate(frames):
+        ref, alb, nrm, dep = _wavefront(d, cam, W, H, seed, spp)
+        np.testing.assert_allclose(batch.get(i), ref[spp], rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(batch.get(i, P.BATCH_ALBEDO1), alb, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(batch.get(i, P.BATCH_NORMAL1), nrm, rtol=1e-5, atol=1e-6)
+    batch.close()
+    dS.close()
+    dA.close()
+
+
 def test_batch_equals_wavefront_renderer():
     scA = G.spheres_sunsky(seed=1, n_spheres=64, subdiv=2, palette=32, coated=0.5)
     scB = G.spheres_ambient(seed=2, n_spheres=48, subdiv=2, emissive=4, ambient=0.3)

@@ -200,6 +200,18 @@ def test_render_parity_remaining_lobes(dev):
     _parity(dev, scene, 96, 64, 4, 0.998)
 
 
+def test_render_parity_standard_materials(dev):
+    # kind-5 StandardMaterials (material/standard.cpp:409-627): the opaque, translucent and metal
+    # bases mixed by weight (partial metalness, partial transmission, base / specular weights below
+    # 1), coats over metal, glass and a diffuse-only base, coloured solid glass whose interior
+    # medium absorbs (kind 5 and kind 2), thin-walled partial transmission under a coat (the
+    # transparent-shadow filter), opacity and sheen over the mix
+    scene = G.spheres_standard()
+    kinds = scene["materials"]["kind"]
+    assert (kinds == G.KIND_STANDARD).sum() >= 20 and (kinds == G.KIND_DIELECTRIC).sum() >= 3
+    _parity(dev, scene, 96, 64, 4, 0.998)
+
+
 @pytest.mark.parametrize("ambient", [0.5, None])
 def test_render_parity_area_lights(dev, ambient):
     # emissive spheres (one TriangleLight over their triangles) with and without an ambient light

@@ -247,3 +247,58 @@ def test_post_intersect_and_frame(golden, suffix):
     np.testing.assert_allclose(ctx[:, 3], ref[:, 3], rtol=1e-6)
     np.testing.assert_allclose(ctx[:, 4:], ref[:, 4:], atol=1e-5)
     np.testing.assert_allclose(fr, golden["pi_frame" + suffix], atol=1e-5)
+
+
+STD_FIELDS = ("base_weight", "r", "g", "b", "metalness", "diffuse_roughness", "specular_weight", "er", "eg", "eb",
+              "roughness", "ior", "transmission_weight", "transmission_color", "transmission_depth", "thin_walled",
+              "thickness", "coat_weight", "coat_color", "coat_roughness", "coat_ior", "coat_thickness", "sheen_weight",
+              "sheen_color", "sheen_roughness")
+
+
+def std_materials(golden):
+    """Kind-5 material records of the golden StandardMaterial rows, and their backfacing flags."""
+    p = golden["std_params"]
+    m = np.zeros(p.shape[0], O.MATERIAL_DTYPE)
+    m["kind"] = 5
+    c = 0
+    for f in STD_FIELDS:
+        width = 3 if f in ("transmission_color", "coat_color", "sheen_color") else 1
+        m[f] = p[:, c:c + width] if width == 3 else (p[:, c].astype(np.int32) if f == "thin_walled" else p[:, c])
+        c += width
+    m["transparency"] = 1.0 - p[:, 31]
+    return m, p[:, 32].astype(np.int32)
+
+
+@pytest.mark.parametrize("variant", range(8))
+def test_standard_material_sample_eval(golden, variant):
+    # material/standard.cpp:409-627 for uniform parameters: the opaque, translucent and metal bases
+    # weighted in one MultiBsdfSimd (multi_bsdf.h:151-459), the coat over any base
+    # (rough_coating.h:270-593 with a generic base), sheen and opacity; variants: 0 partial
+    # metalness, 1 partial transmission (outside / inside), 2 coat over metal, 3 coat over glass
+    # (outside / inside), 4 every lobe with sheen, opacity and a rough diffuse base, 5 thin-walled
+    # partial transmission under a coat (the transparent BTDF at the ambient IOR), 6 base and
+    # specular weights below 1, 7 a coat over a diffuse-only base (Kelemen's energy term)
+    flat, _ = O.compute_luts()
+    mats, inside = std_materials(golden)
+    so, st, ev, aux = O.bsdf_mat(flat, mats, golden["std_geo"], golden["std_s"], inside)
+    sel = (np.arange(len(mats)) // 8) % 8 == variant
+    ref_s, ref_t, ref_e = golden["std_sample"][sel], golden["std_sample_type"][sel], golden["std_eval"][sel]
+    so, st, ev, aux = so[sel], st[sel], ev[sel], aux[sel]
+    valid_ref = np.abs(ref_s[:, :3]).max(1) > 0
+    valid = np.abs(so[:, :3]).max(1) > 0
+    assert valid.mean() > 0.5
+    assert np.array_equal(valid, valid_ref)
+    assert np.array_equal(st[valid], ref_t[valid])
+    np.testing.assert_allclose(so[valid, :3], ref_s[valid, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(so[valid, 3:6], ref_s[valid, 3:6], atol=2e-5)
+    np.testing.assert_allclose(so[valid, 6], ref_s[valid, 6], rtol=1e-3, atol=1e-7)
+    assert (np.abs(ev[:, :3]).max(1) > 0).mean() > 0.3
+    np.testing.assert_allclose(ev[:, :3], ref_e[:, :3], rtol=5e-4, atol=1e-6)
+    np.testing.assert_allclose(ev[:, 3], ref_e[:, 3], rtol=1e-3, atol=1e-6)
+    np.testing.assert_allclose(aux[:, 0:3], golden["std_albedo"][sel], rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(aux[:, 3:6], golden["std_normal"][sel], atol=1e-6)
+    transparent = (mats["transparency"][sel] != 0) | ((mats["transmission_weight"][sel] > 0) & (mats["thin_walled"][sel] != 0))
+    if transparent.any():  # what a shadow ray keeps through the surface
+        np.testing.assert_allclose(aux[transparent, 6:9], golden["std_transparency"][sel][transparent], rtol=5e-4, atol=1e-6)
+    types = set(np.unique(st[valid]).tolist())
+    assert len(types) > 1 or variant == 2  # more than one lobe gets sampled

@@ -201,9 +201,11 @@ def test_gltf_numbering_and_records(tmp_path, data_uri):
     assert metal["kind"] == 1 and np.allclose([metal["r"], metal["g"], metal["b"]], [0.95, 0.64, 0.54])
     assert glass["kind"] == 2 and np.isclose(glass["ior"], 1.33) and np.isclose(glass["roughness"], 0.1)
     assert plastic["kind"] == 3 and np.isclose(plastic["roughness"], 0.35) and np.isclose(plastic["ior"], 1.5)
-    # KHR_materials_clearcoat over a specular layer: mapped to the specular layer, counted
-    assert varnish["kind"] == 3
-    assert sc["info"]["approximated_materials"] >= 1
+    # KHR_materials_clearcoat over a specular layer: the coat over the coated diffuse base (kind 5,
+    # material/standard.cpp:551-566), StandardMaterial's coat thickness 0.5
+    assert varnish["kind"] == 5 and varnish["coat_weight"] > 0 and varnish["specular_weight"] > 0
+    assert np.isclose(varnish["coat_ior"], 1.5) and np.isclose(varnish["coat_thickness"], 0.5)
+    assert sc["info"]["approximated_materials"] == 0
     assert sc["info"]["instance_count"] == 5
     assert sc["positions"].shape[0] == 42 * 4 + 4 * 2
 
