@@ -404,6 +404,25 @@ int prt_denoiser_train_step(prt_denoiser* dn, const float* input, const float* t
                             float lr, double* loss, double* ms);
 int prt_denoiser_gradient(prt_denoiser* dn, const float* input, const float* target, int32_t n, int32_t h, int32_t w,
                           double* loss, float* grad, int64_t grad_count);
+/* Denoising a frame where rendering left it, in HBM, with no host copy in between: the
+ * reference's render window blits the accumulation buffer into host images and calls
+ * Denoiser::execute on them (main/render/render_window.cpp:485-530 init, :1034-1064
+ * updateFrame then execute; image/denoiser.h:10-41);
+ * here the network reads the frame's planar sums and weights directly.  execute_frame takes the
+ * context's frame colour (sum / weight, as prt_blit forms it) and, selected by inputs, its
+ * first-hit albedo and normal (PRT_AOV_ALBEDO1 / PRT_AOV_NORMAL1; a normal needs an albedo);
+ * execute_batch takes a rendered batch frame's colour (buffer PRT_BATCH_COLOR) or a snapshot
+ * (PRT_BATCH_SNAPSHOT + k) with that frame's first-hit AOVs.  The call waits for the kernels
+ * enqueued on the producer's stream, runs the network on the denoiser's stream, and returns
+ * when the result is in the denoiser's device buffer (seconds: the call's wall time;
+ * prt_denoiser_kernel_ms: its kernels).  get_output copies that W x H x 3 result to the host.
+ * The output equals prt_denoiser_execute on the blitted images. */
+#define PRT_DENOISE_ALBEDO 1
+#define PRT_DENOISE_NORMAL 2
+int prt_denoiser_execute_frame(prt_denoiser* dn, prt_context* ctx, int32_t inputs, float hdr_scale, double* seconds);
+int prt_denoiser_execute_batch(prt_denoiser* dn, prt_batch* batch, int32_t frame, int32_t buffer, int32_t inputs,
+                               float hdr_scale, double* seconds);
+int prt_denoiser_get_output(prt_denoiser* dn, float* dst, int32_t width, int32_t height);
 
 /* --- scene files ------------------------------------------------------------------------
  * The loaders of Scene::Scene(path, props) (render/scene.cpp:64-372): .rkscene (JSON + .rkbin,

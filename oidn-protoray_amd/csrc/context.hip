@@ -2230,3 +2230,47 @@ int prt_batch_get(prt_batch* b, int32_t frame, int32_t buffer, float* dst)
 }
 
 }  // extern "C"
+
+namespace prt {
+
+int contextFrameImages(prt_context* c, FrameImages* out)
+{
+  CTX_CHECK(c);
+  if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
+  FrameImages f;
+  f.device = c->device;
+  f.W = c->W;
+  f.H = c->H;
+  f.color = c->color.p;
+  f.colorW = c->weight.p;
+  f.albedo = c->albedo1.p;
+  f.albedoW = c->albedo1W.p;
+  f.normal = c->normal1.p;
+  f.normalW = c->normal1W.p;
+  f.stream = c->stream;
+  *out = f;
+  return PRT_OK;
+}
+
+int batchFrameImages(prt_batch* b, int frame, int buffer, FrameImages* out)
+{
+  if (!b || !out) return fail(PRT_ERR_INVALID, "null argument");
+  if (frame < 0 || frame >= int(b->frames.size())) return fail(PRT_ERR_INVALID, "invalid frame");
+  if (!b->prepared) return fail(PRT_ERR_STATE, "batch not rendered");
+  const BatchFrameDev& d = b->desc[frame];
+  FrameImages f;
+  f.device = b->device;
+  f.W = d.W;
+  f.H = d.H;
+  if (buffer == PRT_BATCH_COLOR) f.color = d.color;
+  else if (buffer >= PRT_BATCH_SNAPSHOT && buffer < PRT_BATCH_SNAPSHOT + d.nSnap)
+    f.color = d.snap + size_t(buffer - PRT_BATCH_SNAPSHOT) * 3 * d.npix;
+  else return fail(PRT_ERR_INVALID, "buffer must be the frame's colour or one of its snapshots");
+  f.albedo = d.albedo1;
+  f.normal = d.normal1;
+  f.stream = b->stream;
+  *out = f;
+  return PRT_OK;
+}
+
+}  // namespace prt

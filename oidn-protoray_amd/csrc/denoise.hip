@@ -52,6 +52,24 @@ __host__ __device__ inline float fromBf16(uint16_t h)
 
 constexpr int kBlock = 256;
 
+// Denoiser input images on the device: colour, albedo and normal as 3-channel FP32 images of
+// W x H pixels, element (channel k, pixel i) at k * kStride + i * iStride (a frame's planar sums:
+// kStride = W H, iStride = 1; interleaved means: 1 and 3), each optionally divided by a per-pixel
+// sample weight (sum * (weight == 0 ? FLT_MAX : 1 / weight), as a blit forms the mean,
+// image/accum_buffer.h:607-649).  A missing auxiliary image is a zero channel.
+struct FrameIn
+{
+  const float *color, *colorW, *albedo, *albedoW, *normal, *normalW;
+  int W, H, kStride, iStride;
+};
+__device__ inline float frameMean(const float* s, const float* w, const FrameIn& f, size_t i, int k)
+{
+  const float v = s[size_t(k) * f.kStride + i * f.iStride];
+  if (!w) return v;
+  const float wt = w[i];
+  return v * (wt == 0.f ? 3.402823466e38f : 1.0f / wt);
+}
+
 struct ConvArgs
 {
   const uint16_t* x;  // input NHWC bf16, channel stride xC (= cinPad)
@@ -64,6 +82,14 @@ struct ConvArgs
   int yC, yOff;
   float* yf;  // optional FP32 output [N*H*W][coutReal]
   const uint16_t* mask;  // optional: outputs where mask <= 0 are zeroed (mask has y's layout)
+  const uint16_t* lo;    // optional: input channels [0, loC) from this half-resolution tensor
+  int loC;               // (loC channels, nearest-upsampled); channels [loC, cinPad) from x
+  uint16_t* pool;        // optional: 2x2 max pooling of the BF16 output (coutPad channels)
+  // output layer of an inference: expm1(value) * invScale written to out (outW x outH x 3, the
+  // padded image cropped)
+  float* out;
+  int outW, outH;
+  float invScale;
 };
 
 // 8 BF16 values <-> one 16-byte word
@@ -94,135 +120,178 @@ __device__ inline void storeBf8(uint16_t* p, const Bf8& r)
   *reinterpret_cast<uint4*>(p) = v;
 }
 
-// Tiled convolution.  A block (8 waves) computes a 16 x 32 pixel tile of one image for 16*NB
-// output channels.  Per 16-channel slice of the input, the haloed input tile (18 x 34 pixels)
-// and the weights of that slice (9 taps x 16*NB outputs) are staged in LDS (40-byte slots:
-// 16 channels + 4 padding, so the 16 lanes of an MFMA operand hit distinct banks); each wave
-// computes two image rows (4 MFMA pixel blocks of 16) for all output blocks.  The MFMA takes the
-// weights as the row operand and the pixels as the column operand, so each lane ends up with 4
-// consecutive output channels of one pixel and stores them as one 8-byte word.
-constexpr int kCtY = 16, kCtX = 32, kCtIY = kCtY + 2, kCtIX = kCtX + 2;
-constexpr int kCtThreads = 512;
-constexpr int kSlot = 20;
+// Tiled convolution (forward and data gradient) on v_mfma_f32_16x16x32_bf16.  A block of ROWS/2
+// waves computes ROWS x 32 pixels of one image for 16*NB output channels; each wave owns two
+// image rows (4 pixel blocks of 16) for all output blocks.  K runs over 16-channel input slices:
+// per slice the haloed (ROWS+2) x 34 input tile and the slice's weights (9 taps x 16 NB outputs)
+// sit in LDS, double-buffered, so one barrier per slice separates staging from the MFMAs, and
+// the next slice's global loads are in flight while the current one is multiplied.  One MFMA
+// covers two taps: lanes 0-31 carry tap t (channels 0-7 and 8-15), lanes 32-63 tap t+1; the
+// ninth tap pairs with zeros.  Both LDS arrays are stored half-major (the 8-channel half of a
+// pixel or weight row is the outer index), so the 16 lanes of an operand read 16 consecutive
+// 16-byte words.  The MFMA takes the weights as the row operand and the pixels as the column
+// operand, so each lane ends with 4 consecutive output channels of one pixel and stores them as
+// one 8-byte word.
+// Optional fusions: the input's first loC channels come from a half-resolution tensor `lo`,
+// upsampled by nearest neighbour (the decoder's upsample + concatenation, cat = [up(lo), x]);
+// and with `pool`, the 2x2 max pooling of the (BF16) output is written too (a wave's two rows
+// hold whole pooling windows; H and W are even).
+constexpr int kCvX = 32, kCvIX = kCvX + 2;
 
-template <int NB>
-__global__ void __launch_bounds__(kCtThreads) kConvTile(ConvArgs a)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int NB, int ROWS>
+constexpr int convStageWords()  // 16-byte words of one LDS stage (input tile + weight slice)
 {
-  __shared__ __align__(16) uint16_t xs[kCtIY * kCtIX * kSlot];
-  __shared__ __align__(16) uint16_t ws[9 * 16 * NB * kSlot];
+  return (ROWS + 2) * kCvIX * 2 + 9 * 16 * NB * 2;
+}
+
+template <int NB, int ROWS>
+__global__ void __launch_bounds__(ROWS * 32) kConv(ConvArgs a)
+{
+  constexpr int kThreads = ROWS * 32;
+  constexpr int kInPix = (ROWS + 2) * kCvIX;
+  constexpr int kInChunks = kInPix * 2;      // 16-byte words of the input tile
+  constexpr int kWChunks = 9 * 16 * NB * 2;  // 16-byte words of the weight slice
+  constexpr int kXItems = (kInChunks + kThreads - 1) / kThreads;
+  constexpr int kWItems = (kWChunks + kThreads - 1) / kThreads;
+  // one stage per buffer: the input tile [half * kInPix + pixel], then the weight slice
+  // [((tap * NB + b) * 2 + half) * 16 + row]; a layer of one 16-channel slice gets one buffer
+  extern __shared__ uint4 smem[];
+  constexpr int kStage = convStageWords<NB, ROWS>();
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tilesX = (a.W + kCtX - 1) / kCtX, tilesY = (a.H + kCtY - 1) / kCtY;
+  const int tilesX = (a.W + kCvX - 1) / kCvX, tilesY = (a.H + ROWS - 1) / ROWS;
   int t = blockIdx.x;
   const int tx = t % tilesX; t /= tilesX;
   const int ty = t % tilesY;
   const int n = t / tilesY;
   const int co0 = blockIdx.y * 16 * NB;
   const int nb = min(NB, (a.coutPad - co0) / 16);
-  const int x0 = tx * kCtX, y0 = ty * kCtY;
-  const int li = lane & 15, lk = (lane >> 4) * 4;
+  const int x0 = tx * kCvX, y0 = ty * ROWS;
+  const int li = lane & 15, g = lane >> 4, tapSel = g >> 1, half = g & 1, lk = 4 * g;
   f32x4 acc[4][NB];
 #pragma unroll
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[p][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // Staging: each thread owns fixed slots of the input tile and of the weight slice; the next
-  // slice's global loads are issued before the current slice's MFMAs, so their latency is hidden.
-  constexpr int kXItems = (kCtIY * kCtIX * 2 + kCtThreads - 1) / kCtThreads;
-  constexpr int kWItems = (9 * 16 * NB * 2 + kCtThreads - 1) / kCtThreads;
+  const int Hl = a.H >> 1, Wl = a.W >> 1;
   uint4 xr[kXItems], wr[kWItems];
   auto fetch = [&](int c0) {
 #pragma unroll
     for (int k = 0; k < kXItems; ++k)
     {
-      const int i = tid + k * kCtThreads, half = i & 1, pix = i >> 1;
-      const int yy = y0 + pix / kCtIX - 1, xx = x0 + pix % kCtIX - 1;
+      const int i = tid + k * kThreads, hf = i & 1, pix = i >> 1;
+      const int yy = y0 + pix / kCvIX - 1, xx = x0 + pix % kCvIX - 1;
       xr[k] = make_uint4(0u, 0u, 0u, 0u);
-      if (i < kCtIY * kCtIX * 2 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-        xr[k] = *reinterpret_cast<const uint4*>(a.x + ((size_t(n) * a.H + yy) * a.W + xx) * a.xC + c0 + 8 * half);
-    }
-#pragma unroll
-    for (int k = 0; k < kWItems; ++k)
-    {
-      const int i = tid + k * kCtThreads, half = i & 1, r = i >> 1;
-      const int co = co0 + r % (16 * NB), tap = r / (16 * NB);
-      wr[k] = make_uint4(0u, 0u, 0u, 0u);
-      if (i < 9 * 16 * NB * 2 && co < a.coutPad)
-        wr[k] = *reinterpret_cast<const uint4*>(a.w + (size_t(tap) * a.coutPad + co) * a.cinPad + c0 + 8 * half);
-    }
-  };
-  auto stage = [&]() {
-#pragma unroll
-    for (int k = 0; k < kXItems; ++k)
-    {
-      const int i = tid + k * kCtThreads;
-      if (i < kCtIY * kCtIX * 2)
+      if (i < kInChunks && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
       {
-        uint2* d = reinterpret_cast<uint2*>(xs + (i >> 1) * kSlot + 8 * (i & 1));
-        d[0] = make_uint2(xr[k].x, xr[k].y);
-        d[1] = make_uint2(xr[k].z, xr[k].w);
+        const uint16_t* src = c0 < a.loC ? a.lo + ((size_t(n) * Hl + (yy >> 1)) * Wl + (xx >> 1)) * a.loC + c0 + 8 * hf
+                                         : a.x + ((size_t(n) * a.H + yy) * a.W + xx) * a.xC + (c0 - a.loC) + 8 * hf;
+        xr[k] = *reinterpret_cast<const uint4*>(src);
       }
     }
 #pragma unroll
     for (int k = 0; k < kWItems; ++k)
     {
-      const int i = tid + k * kCtThreads;
-      if (i < 9 * 16 * NB * 2)
+      const int i = tid + k * kThreads, hf = i & 1, r = i >> 1;
+      const int co = co0 + r % (16 * NB), tap = r / (16 * NB);
+      wr[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (i < kWChunks && co < a.coutPad)
+        wr[k] = *reinterpret_cast<const uint4*>(a.w + (size_t(tap) * a.coutPad + co) * a.cinPad + c0 + 8 * hf);
+    }
+  };
+  auto stage = [&](int buf) {
+    uint4* const xsb = smem + buf * kStage;
+    uint4* const wsb = xsb + kInChunks;
+#pragma unroll
+    for (int k = 0; k < kXItems; ++k)
+    {
+      const int i = tid + k * kThreads;
+      if (i < kInChunks) xsb[(i & 1) * kInPix + (i >> 1)] = xr[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kWItems; ++k)
+    {
+      const int i = tid + k * kThreads, r = i >> 1;
+      if (i < kWChunks)
       {
-        uint2* d = reinterpret_cast<uint2*>(ws + (i >> 1) * kSlot + 8 * (i & 1));
-        d[0] = make_uint2(wr[k].x, wr[k].y);
-        d[1] = make_uint2(wr[k].z, wr[k].w);
+        const int rr = r % (16 * NB), tap = r / (16 * NB);
+        wsb[((tap * NB + (rr >> 4)) * 2 + (i & 1)) * 16 + (rr & 15)] = wr[k];
       }
     }
   };
   fetch(0);
-  for (int c0 = 0; c0 < a.cinPad; c0 += 16)
+  stage(0);
+  __syncthreads();
+  const int slices = a.cinPad / 16;
+  for (int sl = 0; sl < slices; ++sl)
   {
-    __syncthreads();  // the previous slice's readers are done
-    stage();
-    __syncthreads();
-    if (c0 + 16 < a.cinPad) fetch(c0 + 16);
-#pragma unroll 3
-    for (int tap = 0; tap < 9; ++tap)
+    const int buf = sl & 1;
+    const uint4* const xsb = smem + buf * kStage;
+    const uint4* const wsb = xsb + kInChunks;
+    if (sl + 1 < slices) fetch((sl + 1) * 16);
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
     {
+      const int tapRaw = 2 * q + tapSel;
+      const bool valid = tapRaw < 9;
+      const int tap = valid ? tapRaw : 8;
       const int oy = tap / 3, ox = tap % 3;
-      bf16x4 B[4];
+      bf16x8 B[4];
 #pragma unroll
       for (int p = 0; p < 4; ++p)
       {
         const int ly = 2 * wave + (p >> 1) + oy, lx = 16 * (p & 1) + li + ox;
-        B[p] = *reinterpret_cast<const bf16x4*>(xs + (ly * kCtIX + lx) * kSlot + lk);
+        uint4 v = xsb[half * kInPix + ly * kCvIX + lx];
+        if (!valid) v = make_uint4(0u, 0u, 0u, 0u);
+        B[p] = __builtin_bit_cast(bf16x8, v);
       }
 #pragma unroll
       for (int b = 0; b < NB; ++b)
         if (b < nb)
         {
-          const bf16x4 A = *reinterpret_cast<const bf16x4*>(ws + ((tap * NB + b) * 16 + li) * kSlot + lk);
+          uint4 v = wsb[((tap * NB + b) * 2 + half) * 16 + li];
+          if (!valid) v = make_uint4(0u, 0u, 0u, 0u);
+          const bf16x8 A = __builtin_bit_cast(bf16x8, v);
 #pragma unroll
-          for (int p = 0; p < 4; ++p) acc[p][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B[p], acc[p][b], 0, 0, 0);
+          for (int p = 0; p < 4; ++p) acc[p][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[p], acc[p][b], 0, 0, 0);
         }
     }
+    if (sl + 1 < slices) stage(buf ^ 1);
+    __syncthreads();
   }
-  // D of (p, b): lane holds pixel li of pixel block p, output channels co0 + 16 b + lk + j
+  // D of (p, b): lane holds pixel li of pixel block p, output channels co0 + 16 b + lk + j.
+  // Epilogue: bias, ReLU, zero padding channels, the mask; the BF16 values are kept for pooling.
+  float bias[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+      const int co = co0 + 16 * b + lk + j;
+      bias[b][j] = a.bias && co < a.coutReal ? a.bias[co] : 0.f;
+    }
+  uint16_t hv[4][NB][4];
 #pragma unroll
   for (int p = 0; p < 4; ++p)
   {
     const int y = y0 + 2 * wave + (p >> 1), x = x0 + 16 * (p & 1) + li;
-    if (y >= a.H || x >= a.W) continue;
+    const bool inside = y < a.H && x < a.W;
     const size_t pix = (size_t(n) * a.H + y) * a.W + x;
 #pragma unroll
     for (int b = 0; b < NB; ++b)
     {
-      if (b >= nb) continue;
       const int co = co0 + 16 * b + lk;
       float v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
       {
-        float u = acc[p][b][j];
-        if (a.bias && co + j < a.coutReal) u += a.bias[co + j];
+        float u = acc[p][b][j] + bias[b][j];
         if (a.relu) u = fmaxf(u, 0.f);
         v[j] = co + j < a.coutReal ? u : 0.f;
+        hv[p][b][j] = toBf16(v[j]);
       }
+      if (!inside || b >= nb) continue;
       const size_t o = pix * a.yC + a.yOff + co;
       if (a.mask)
       {
@@ -230,15 +299,54 @@ __global__ void __launch_bounds__(kCtThreads) kConvTile(ConvArgs a)
         const uint32_t mw[2] = {m.x, m.y};
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          if (!(fromBf16(uint16_t(mw[j >> 1] >> (16 * (j & 1)))) > 0.f)) v[j] = 0.f;
+          if (!(fromBf16(uint16_t(mw[j >> 1] >> (16 * (j & 1)))) > 0.f)) { v[j] = 0.f; hv[p][b][j] = 0; }
       }
       if (a.y)
-        *reinterpret_cast<uint2*>(a.y + o) = make_uint2(uint32_t(toBf16(v[0])) | uint32_t(toBf16(v[1])) << 16,
-                                                         uint32_t(toBf16(v[2])) | uint32_t(toBf16(v[3])) << 16);
+        *reinterpret_cast<uint2*>(a.y + o) = make_uint2(uint32_t(hv[p][b][0]) | uint32_t(hv[p][b][1]) << 16,
+                                                         uint32_t(hv[p][b][2]) | uint32_t(hv[p][b][3]) << 16);
       if (a.yf)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (co + j < a.coutReal) a.yf[pix * a.coutReal + co + j] = v[j];
+      if (a.out && y < a.outH && x < a.outW)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (co + j < a.coutReal)
+            a.out[((size_t(n) * a.outH + y) * a.outW + x) * a.coutReal + co + j] = expm1f(v[j]) * a.invScale;
+    }
+  }
+  if (a.pool)
+  {
+    // windows (2 wave, 2 i) .. (2 wave + 1, 2 i + 1): rows from pixel blocks p and p + 2, columns
+    // from lanes li and li ^ 1; the maximum is taken in kPool's order on the stored BF16 values
+    const int y = y0 + 2 * wave;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+    {
+      const int x = x0 + 16 * p + li;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+      {
+        if (b >= nb) continue;
+        uint16_t m[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+          const uint16_t v00 = hv[p][b][j], v10 = hv[p + 2][b][j];
+          const uint16_t v01 = uint16_t(__shfl_xor(int(v00), 1)), v11 = uint16_t(__shfl_xor(int(v10), 1));
+          uint16_t r = v00;
+          if (fromBf16(v01) > fromBf16(r)) r = v01;
+          if (fromBf16(v10) > fromBf16(r)) r = v10;
+          if (fromBf16(v11) > fromBf16(r)) r = v11;
+          m[j] = r;
+        }
+        if (!(li & 1) && y < a.H && x < a.W)
+        {
+          const size_t pp = (size_t(n) * Hl + (y >> 1)) * Wl + (x >> 1);
+          *reinterpret_cast<uint2*>(a.pool + pp * a.coutPad + co0 + 16 * b + lk) =
+              make_uint2(uint32_t(m[0]) | uint32_t(m[1]) << 16, uint32_t(m[2]) | uint32_t(m[3]) << 16);
+        }
+      }
     }
   }
 }
@@ -460,28 +568,6 @@ __global__ void __launch_bounds__(kBlock) kWgradReduce(const float* __restrict__
   }
 }
 
-// 2x2 max pooling (NHWC, H and W even); a thread takes 8 channels of one output pixel.
-__global__ void __launch_bounds__(kBlock) kPool(const uint16_t* __restrict__ x, int N, int H, int W, int C, uint16_t* __restrict__ y)
-{
-  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  const int Ho = H / 2, Wo = W / 2, octs = C / 8;
-  if (i >= size_t(N) * Ho * Wo * octs) return;
-  const int c = int(i % octs) * 8;
-  size_t p = i / octs;
-  const int xo = int(p % Wo); p /= Wo;
-  const int yo = int(p % Ho);
-  const int n = int(p / Ho);
-  Bf8 m = loadBf8(x + ((size_t(n) * H + 2 * yo) * W + 2 * xo) * C + c);
-  for (int k = 1; k < 4; ++k)
-  {
-    const Bf8 v = loadBf8(x + ((size_t(n) * H + 2 * yo + k / 2) * W + 2 * xo + k % 2) * C + c);
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (fromBf16(v.h[j]) > fromBf16(m.h[j])) m.h[j] = v.h[j];
-  }
-  storeBf8(y + i * 8, m);
-}
-
 // Encoder level backward: de = (pool gradient routed to the first maximum of each 2x2 window
 // + the skip slice of the decoder's concat gradient), masked by ReLU(e) > 0.  A thread takes
 // 8 channels of one 2x2 window.
@@ -579,19 +665,26 @@ __global__ void __launch_bounds__(kBlock) kUpBwd(const uint16_t* __restrict__ dc
   storeBf8(dlow + i * 8, o);
 }
 
-// Network input: log(1 + colour), albedo, normal -> 16 BF16 channels (9 used).
-__global__ void __launch_bounds__(kBlock) kInput(const float* __restrict__ in9, size_t npix, float hdrScale, uint16_t* __restrict__ x)
+// Network input: log(1 + colour * hdrScale), albedo, normal -> 16 BF16 channels (9 used) of
+// the Wp x Hp input, pixels beyond the images' W x H repeating their last row / column.
+__global__ void __launch_bounds__(kBlock) kInput(FrameIn f, int Wp, int Hp, float hdrScale, uint16_t* __restrict__ x)
 {
   const size_t p = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (p >= npix) return;
+  if (p >= size_t(Wp) * Hp) return;
+  const int py = int(p / Wp), px = int(p % Wp);
+  const size_t i = size_t(min(py, f.H - 1)) * f.W + min(px, f.W - 1);
   Bf8 lo{}, hi{};
+  float v[9];
 #pragma unroll
-  for (int c = 0; c < 9; ++c)
+  for (int c = 0; c < 3; ++c)
   {
-    const float v = c < 3 ? log1pf(fmaxf(in9[p * 9 + c] * hdrScale, 0.f)) : in9[p * 9 + c];
-    if (c < 8) lo.h[c] = toBf16(v);
-    else hi.h[0] = toBf16(v);
+    v[c] = log1pf(fmaxf(frameMean(f.color, f.colorW, f, i, c) * hdrScale, 0.f));
+    v[3 + c] = f.albedo ? frameMean(f.albedo, f.albedoW, f, i, c) : 0.f;
+    v[6 + c] = f.normal ? frameMean(f.normal, f.normalW, f, i, c) : 0.f;
   }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) lo.h[c] = toBf16(v[c]);
+  hi.h[0] = toBf16(v[8]);
   storeBf8(x + p * 16, lo);
   storeBf8(x + p * 16 + 8, hi);
 }
@@ -625,12 +718,6 @@ __global__ void __launch_bounds__(kBlock) kLoss(const float* __restrict__ pred, 
     for (int w = 0; w < kBlock / 64; ++w) t += part[w];
     if (t != 0.f) atomicAdd(loss, t);
   }
-}
-
-__global__ void __launch_bounds__(kBlock) kOutput(const float* __restrict__ pred, size_t npix, float invScale, float* __restrict__ out)
-{
-  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i < npix * 3) out[i] = expm1f(pred[i]) * invScale;
 }
 
 // Adam on the FP32 master parameters; then the layers' BF16 weights (forward layout and the
@@ -717,6 +804,11 @@ struct prt_denoiser
   uint16_t *x = nullptr, *e[5] = {}, *p[4] = {}, *cat[4] = {}, *d[4] = {};
   uint16_t *dout = nullptr, *dd[4] = {}, *dcat[4] = {}, *de[5] = {}, *dp[4] = {};
   float *pred = nullptr, *in9 = nullptr, *tgt = nullptr, *loss = nullptr, *outF = nullptr;
+  // result of the last execute on a rendered frame: W x H x 3 on the device
+  float* outDev = nullptr;
+  size_t outDevCap = 0;
+  int outW = 0, outH = 0;
+  hipEvent_t evIn = nullptr;  // the frame's producer stream -> the denoiser stream
   // Denoiser::init parameters
   prt_denoiser_params params{};
   bool inited = false;
@@ -804,22 +896,61 @@ int ensureActs(prt_denoiser* dn, int N, int H, int W)
   return PRT_OK;
 }
 
-void launchConv(prt_denoiser* dn, const ConvArgs& a)
+template <int NB, int ROWS>
+void launchConvT(prt_denoiser* dn, const ConvArgs& a, int groups)
 {
-  const unsigned tiles = unsigned(a.N * ((a.H + kCtY - 1) / kCtY) * ((a.W + kCtX - 1) / kCtX));
-  if (a.coutPad <= 32)
-    kConvTile<2><<<dim3(tiles, (a.coutPad + 31) / 32), kCtThreads, 0, dn->stream>>>(a);
-  else
-    kConvTile<4><<<dim3(tiles, (a.coutPad + 63) / 64), kCtThreads, 0, dn->stream>>>(a);
+  const unsigned tiles = unsigned(a.N * ((a.H + ROWS - 1) / ROWS) * ((a.W + kCvX - 1) / kCvX));
+  const size_t lds = size_t(a.cinPad > 16 ? 2 : 1) * convStageWords<NB, ROWS>() * 16;
+  static const bool ldsAttr = hipFuncSetAttribute(reinterpret_cast<const void*>(&kConv<NB, ROWS>),
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  2 * convStageWords<NB, ROWS>() * 16) == hipSuccess;
+  (void)ldsAttr;  // a refusal surfaces as the launch's error
+  kConv<NB, ROWS><<<dim3(tiles, unsigned(groups)), ROWS * 32, lds, dn->stream>>>(a);
 }
 
-void conv(prt_denoiser* dn, int layer, const uint16_t* x, int N, int H, int W, uint16_t* y, int yC, int yOff, bool relu, float* yf)
+template <int NB>
+void launchConvNB(prt_denoiser* dn, const ConvArgs& a, int rows, int groups)
+{
+  switch (rows)
+  {
+    case 16: launchConvT<NB, 16>(dn, a, groups); break;
+    case 8: launchConvT<NB, 8>(dn, a, groups); break;
+    case 4: launchConvT<NB, 4>(dn, a, groups); break;
+    default: launchConvT<NB, 2>(dn, a, groups); break;
+  }
+}
+
+// Output channels per block (16 NB, NB <= 4, split evenly) and tile height: 16 rows of 32
+// pixels, halved (down to 2) while the launch would leave CUs without a block; every block
+// stages the whole weight slice, so smaller tiles only pay where the level is too small to
+// fill the GPU.
+void launchConv(prt_denoiser* dn, const ConvArgs& a)
+{
+  const int blocks16 = a.coutPad / 16;
+  const int groups = (blocks16 + 3) / 4;
+  const int NB = (blocks16 + groups - 1) / groups;
+  int rows = 16;
+  while (rows > 2 && size_t(a.N) * ((a.H + rows - 1) / rows) * ((a.W + kCvX - 1) / kCvX) * groups < 256) rows /= 2;
+  switch (NB)
+  {
+    case 1: launchConvNB<1>(dn, a, rows, groups); break;
+    case 2: launchConvNB<2>(dn, a, rows, groups); break;
+    case 3: launchConvNB<3>(dn, a, rows, groups); break;
+    default: launchConvNB<4>(dn, a, rows, groups); break;
+  }
+}
+
+// Forward convolution of layer `layer`.  With lo, the input is [up(lo) (loC channels), x]; with
+// pool, the 2x2 max pooling of y is written too.
+void conv(prt_denoiser* dn, int layer, const uint16_t* x, int N, int H, int W, uint16_t* y, int yC, int yOff, bool relu, float* yf,
+          const uint16_t* lo = nullptr, int loC = 0, uint16_t* pool = nullptr)
 {
   const DenoiseLayer& l = dn->L[layer];
   ConvArgs a{};
-  a.x = x; a.xC = l.cinPad; a.w = dn->wf[layer]; a.bias = dn->w + l.wOff + size_t(l.cout) * l.cin * 9;
+  a.x = x; a.xC = l.cinPad - loC; a.w = dn->wf[layer]; a.bias = dn->w + l.wOff + size_t(l.cout) * l.cin * 9;
   a.N = N; a.H = H; a.W = W; a.cinPad = l.cinPad; a.coutPad = l.coutPad; a.coutReal = l.cout;
   a.relu = relu ? 1 : 0; a.y = y; a.yC = yC; a.yOff = yOff; a.yf = yf; a.mask = nullptr;
+  a.lo = lo; a.loC = loC; a.pool = pool;
   launchConv(dn, a);
 }
 
@@ -849,20 +980,31 @@ void wgrad(prt_denoiser* dn, int layer, const uint16_t* dy, const uint16_t* x, i
                                                                      dn->dwp[layer], dn->g + l.wOff + size_t(l.cout) * l.cin * 9);
 }
 
-void forward(prt_denoiser* dn, int N, int H, int W)
+// Source and destination of an inference: the input is formed from `fin` (padded to the
+// network's size by edge replication), the output layer writes the cropped image to `out`.
+struct NetIO
 {
+  FrameIn fin;
+  float hdrScale;
+  float* out;
+  int outW, outH;
+};
+
+// Forward pass.  The encoder convolutions write their pooled outputs too; the decoder
+// convolutions read the upsampled lower level and the skip directly.  Training (io null) reads
+// the BF16 input dn->x its caller formed and writes the FP32 prediction dn->pred, and also
+// materialises each concatenation, which the weight gradient and the ReLU mask read.
+void forward(prt_denoiser* dn, int N, int H, int W, const NetIO* io)
+{
+  const bool keepCat = io == nullptr;
   const int encC[5] = {32, 48, 64, 96, 128};
   const uint16_t* in = dn->x;
   for (int l = 0; l < 5; ++l)
   {
-    const int s = l;
-    conv(dn, l, in, N, H >> s, W >> s, dn->e[l], encC[l], 0, true, nullptr);
-    if (l < 4)
-    {
-      const size_t n = size_t(N) * (H >> (s + 1)) * (W >> (s + 1)) * (encC[l] / 8);
-      kPool<<<blocksFor(n), kBlock, 0, dn->stream>>>(dn->e[l], N, H >> s, W >> s, encC[l], dn->p[l]);
-      in = dn->p[l];
-    }
+    if (l == 0 && io)
+      kInput<<<blocksFor(size_t(N) * H * W), kBlock, 0, dn->stream>>>(io->fin, W, N * H, io->hdrScale, dn->x);
+    conv(dn, l, in, N, H >> l, W >> l, dn->e[l], encC[l], 0, true, nullptr, nullptr, 0, l < 4 ? dn->p[l] : nullptr);
+    if (l < 4) in = dn->p[l];
   }
   const int decC[4] = {32, 48, 64, 96};
   const uint16_t* low = dn->e[4];
@@ -870,13 +1012,26 @@ void forward(prt_denoiser* dn, int N, int H, int W)
   for (int l = 3; l >= 0; --l)
   {
     const int h = H >> l, w = W >> l;
-    const size_t n = size_t(N) * h * w * ((lowC + encC[l]) / 8);
-    kUpcat<<<blocksFor(n), kBlock, 0, dn->stream>>>(low, lowC, dn->e[l], encC[l], N, h, w, dn->cat[l]);
-    conv(dn, 8 - l, dn->cat[l], N, h, w, dn->d[l], decC[l], 0, true, nullptr);  // dec3 = layer 5 ... dec0 = layer 8
+    if (keepCat)
+    {
+      const size_t n = size_t(N) * h * w * ((lowC + encC[l]) / 8);
+      kUpcat<<<blocksFor(n), kBlock, 0, dn->stream>>>(low, lowC, dn->e[l], encC[l], N, h, w, dn->cat[l]);
+    }
+    conv(dn, 8 - l, dn->e[l], N, h, w, dn->d[l], decC[l], 0, true, nullptr, low, lowC);  // dec3 = layer 5 ... dec0 = layer 8
     low = dn->d[l];
     lowC = decC[l];
   }
-  conv(dn, 9, dn->d[0], N, H, W, nullptr, 16, 0, false, dn->pred);
+  if (io)
+  {
+    const DenoiseLayer& L9 = dn->L[9];
+    ConvArgs a{};
+    a.x = dn->d[0]; a.xC = L9.cinPad; a.w = dn->wf[9]; a.bias = dn->w + L9.wOff + size_t(L9.cout) * L9.cin * 9;
+    a.N = N; a.H = H; a.W = W; a.cinPad = L9.cinPad; a.coutPad = L9.coutPad; a.coutReal = L9.cout;
+    a.out = io->out; a.outW = io->outW; a.outH = io->outH; a.invScale = 1.0f / io->hdrScale;
+    launchConv(dn, a);
+  }
+  else
+    conv(dn, 9, dn->d[0], N, H, W, nullptr, 16, 0, false, dn->pred);
 }
 
 void backward(prt_denoiser* dn, int N, int H, int W)
@@ -934,8 +1089,10 @@ int lossAndGrad(prt_denoiser* dn, int N, int H, int W, double* lossOut)
   const size_t npix = size_t(N) * H * W;
   DN_TRY(hipMemsetAsync(dn->loss, 0, sizeof(float), dn->stream));
   DN_TRY(hipMemsetAsync(dn->g, 0, sizeof(float) * dn->nParams, dn->stream));
-  kInput<<<blocksFor(npix), kBlock, 0, dn->stream>>>(dn->in9, npix, 1.f, dn->x);
-  forward(dn, N, H, W);
+  // the batch as one image of W x (N H) pixels, 9 interleaved channels (colour, albedo, normal)
+  const FrameIn f{dn->in9, nullptr, dn->in9 + 3, nullptr, dn->in9 + 6, nullptr, W, N * H, 1, 9};
+  kInput<<<blocksFor(npix), kBlock, 0, dn->stream>>>(f, W, N * H, 1.f, dn->x);
+  forward(dn, N, H, W, nullptr);
   kLoss<<<blocksFor(npix), kBlock, 0, dn->stream>>>(dn->pred, dn->tgt, npix, 1.0f / float(npix * 3), dn->dout, dn->loss);
   backward(dn, N, H, W);
   for (int i = 0; i < 10; ++i)
@@ -992,6 +1149,7 @@ int prt_denoiser_create(int32_t device_ordinal, prt_denoiser** out)
   hipError_t e = hipStreamCreateWithFlags(&dn->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&dn->ev[0]);
   if (e == hipSuccess) e = hipEventCreate(&dn->ev[1]);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&dn->evIn, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&dn->w, sizeof(float) * off);
   if (e == hipSuccess) e = hipMalloc(&dn->m, sizeof(float) * off);
   if (e == hipSuccess) e = hipMalloc(&dn->v, sizeof(float) * off);
@@ -1044,6 +1202,8 @@ int prt_denoiser_destroy(prt_denoiser* dn)
     if (dn->wt[i]) hipFree(dn->wt[i]);
     if (dn->dwp[i]) hipFree(dn->dwp[i]);
   }
+  if (dn->outDev) hipFree(dn->outDev);
+  if (dn->evIn) (void)hipEventDestroy(dn->evIn);
   for (auto& e : dn->ev)
     if (e) (void)hipEventDestroy(e);
   if (dn->stream) (void)hipStreamDestroy(dn->stream);
@@ -1144,38 +1304,116 @@ int prt_denoiser_execute(prt_denoiser* dn, double* seconds)
   const auto t0 = std::chrono::high_resolution_clock::now();
   const prt_denoiser_params& p = dn->params;
   const int W = p.width, H = p.height;
-  const int Wp = (W + 15) / 16 * 16, Hp = (H + 15) / 16 * 16;  // padded by edge replication
-  std::vector<float> in9(size_t(Wp) * Hp * 9);
-  for (int y = 0; y < Hp; ++y)
-    for (int x = 0; x < Wp; ++x)
-    {
-      const size_t s = size_t(std::min(y, H - 1)) * W + std::min(x, W - 1), d = size_t(y) * Wp + x;
-      for (int c = 0; c < 3; ++c)
-      {
-        in9[d * 9 + c] = p.color[s * 3 + c];
-        in9[d * 9 + 3 + c] = p.albedo ? p.albedo[s * 3 + c] : 0.f;
-        in9[d * 9 + 6 + c] = p.normal ? p.normal[s * 3 + c] : 0.f;
-      }
-    }
+  const int Wp = (W + 15) / 16 * 16, Hp = (H + 15) / 16 * 16;  // the network pads by edge replication
   int rc = ensureActs(dn, 1, Hp, Wp);
   if (rc != PRT_OK) return rc;
-  const size_t npix = size_t(Hp) * Wp;
+  const size_t n = size_t(W) * H;
   const float scale = std::isnan(p.hdrScale) || p.hdrScale <= 0.f ? 1.f : p.hdrScale;
-  DN_TRY(hipMemcpyAsync(dn->in9, in9.data(), sizeof(float) * npix * 9, hipMemcpyHostToDevice, dn->stream));
+  // the images as given (interleaved H x W x 3) into dn->in9: colour, then albedo, then normal
+  DN_TRY(hipMemcpyAsync(dn->in9, p.color, sizeof(float) * n * 3, hipMemcpyHostToDevice, dn->stream));
+  if (p.albedo) DN_TRY(hipMemcpyAsync(dn->in9 + 3 * n, p.albedo, sizeof(float) * n * 3, hipMemcpyHostToDevice, dn->stream));
+  if (p.normal) DN_TRY(hipMemcpyAsync(dn->in9 + 6 * n, p.normal, sizeof(float) * n * 3, hipMemcpyHostToDevice, dn->stream));
+  NetIO io{};
+  io.fin = FrameIn{dn->in9, nullptr, p.albedo ? dn->in9 + 3 * n : nullptr, nullptr, p.normal ? dn->in9 + 6 * n : nullptr, nullptr,
+                   W, H, 1, 3};
+  io.hdrScale = scale;
+  io.out = dn->outF;
+  io.outW = W;
+  io.outH = H;
   DN_TRY(hipEventRecord(dn->ev[0], dn->stream));
-  kInput<<<blocksFor(npix), kBlock, 0, dn->stream>>>(dn->in9, npix, scale, dn->x);
-  forward(dn, 1, Hp, Wp);
-  kOutput<<<blocksFor(npix * 3), kBlock, 0, dn->stream>>>(dn->pred, npix, 1.0f / scale, dn->outF);
+  forward(dn, 1, Hp, Wp, &io);
   DN_TRY(hipEventRecord(dn->ev[1], dn->stream));
-  std::vector<float> out(npix * 3);
-  DN_TRY(hipMemcpyAsync(out.data(), dn->outF, sizeof(float) * npix * 3, hipMemcpyDeviceToHost, dn->stream));
+  DN_TRY(hipMemcpyAsync(p.output, dn->outF, sizeof(float) * n * 3, hipMemcpyDeviceToHost, dn->stream));
   DN_TRY(hipStreamSynchronize(dn->stream));
   DN_TRY(hipGetLastError());
-  for (int y = 0; y < H; ++y)
-    for (int x = 0; x < W; ++x)
-      for (int c = 0; c < 3; ++c) p.output[(size_t(y) * W + x) * 3 + c] = out[(size_t(y) * Wp + x) * 3 + c];
   const auto t1 = std::chrono::high_resolution_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return PRT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Denoises planar images already on the device into dn->outDev (W x H x 3).
+int executeImages(prt_denoiser* dn, const FrameImages& f, int32_t inputs, float hdrScale, double* seconds)
+{
+  if (f.device != dn->device) return dfail(PRT_ERR_INVALID, "the frame is on another device than the denoiser");
+  if (!f.color || f.W < 1 || f.H < 1) return dfail(PRT_ERR_STATE, "the frame has no colour image");
+  if (inputs & ~(PRT_DENOISE_ALBEDO | PRT_DENOISE_NORMAL)) return dfail(PRT_ERR_INVALID, "unknown input flags");
+  if ((inputs & PRT_DENOISE_NORMAL) && !(inputs & PRT_DENOISE_ALBEDO))
+    return dfail(PRT_ERR_INVALID, "a normal image needs an albedo image");
+  if ((inputs & PRT_DENOISE_ALBEDO) && !f.albedo)
+    return dfail(PRT_ERR_INVALID, "the frame has no first-hit albedo (enable PRT_AOV_ALBEDO1)");
+  if ((inputs & PRT_DENOISE_NORMAL) && !f.normal)
+    return dfail(PRT_ERR_INVALID, "the frame has no first-hit normal (enable PRT_AOV_NORMAL1)");
+  DN_TRY(hipSetDevice(dn->device));
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  const int W = f.W, H = f.H;
+  const int Wp = (W + 15) / 16 * 16, Hp = (H + 15) / 16 * 16;
+  int rc = ensureActs(dn, 1, Hp, Wp);
+  if (rc != PRT_OK) return rc;
+  const size_t nOut = size_t(W) * H * 3;
+  if (dn->outDevCap < nOut)
+  {
+    if (dn->outDev) DN_TRY(hipFree(dn->outDev));
+    dn->outDev = nullptr;
+    dn->outDevCap = 0;
+    DN_TRY(hipMalloc(&dn->outDev, sizeof(float) * nOut));
+    dn->outDevCap = nOut;
+  }
+  NetIO io{};
+  io.fin = FrameIn{f.color, f.colorW, (inputs & PRT_DENOISE_ALBEDO) ? f.albedo : nullptr,
+                   (inputs & PRT_DENOISE_ALBEDO) ? f.albedoW : nullptr, (inputs & PRT_DENOISE_NORMAL) ? f.normal : nullptr,
+                   (inputs & PRT_DENOISE_NORMAL) ? f.normalW : nullptr, W, H, W * H, 1};
+  io.hdrScale = std::isnan(hdrScale) || hdrScale <= 0.f ? 1.f : hdrScale;
+  io.out = dn->outDev;
+  io.outW = W;
+  io.outH = H;
+  DN_TRY(hipEventRecord(dn->evIn, f.stream));
+  DN_TRY(hipStreamWaitEvent(dn->stream, dn->evIn, 0));
+  DN_TRY(hipEventRecord(dn->ev[0], dn->stream));
+  forward(dn, 1, Hp, Wp, &io);
+  DN_TRY(hipEventRecord(dn->ev[1], dn->stream));
+  DN_TRY(hipStreamSynchronize(dn->stream));
+  DN_TRY(hipGetLastError());
+  dn->outW = W;
+  dn->outH = H;
+  const auto t1 = std::chrono::high_resolution_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return PRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prt_denoiser_execute_frame(prt_denoiser* dn, prt_context* ctx, int32_t inputs, float hdr_scale, double* seconds)
+{
+  if (!dn || !ctx) return dfail(PRT_ERR_INVALID, "null argument");
+  FrameImages f;
+  int rc = contextFrameImages(ctx, &f);
+  if (rc != PRT_OK) return dfail(rc, prt_last_error());
+  return executeImages(dn, f, inputs, hdr_scale, seconds);
+}
+
+int prt_denoiser_execute_batch(prt_denoiser* dn, prt_batch* batch, int32_t frame, int32_t buffer, int32_t inputs, float hdr_scale,
+                               double* seconds)
+{
+  if (!dn || !batch) return dfail(PRT_ERR_INVALID, "null argument");
+  FrameImages f;
+  int rc = batchFrameImages(batch, frame, buffer, &f);
+  if (rc != PRT_OK) return dfail(rc, prt_last_error());
+  return executeImages(dn, f, inputs, hdr_scale, seconds);
+}
+
+int prt_denoiser_get_output(prt_denoiser* dn, float* dst, int32_t width, int32_t height)
+{
+  if (!dn || !dst) return dfail(PRT_ERR_INVALID, "null argument");
+  if (!dn->outW) return dfail(PRT_ERR_STATE, "no frame has been denoised");
+  if (width != dn->outW || height != dn->outH) return dfail(PRT_ERR_INVALID, "size differs from the denoised frame's");
+  DN_TRY(hipSetDevice(dn->device));
+  DN_TRY(hipMemcpy(dst, dn->outDev, sizeof(float) * size_t(width) * height * 3, hipMemcpyDeviceToHost));
   return PRT_OK;
 }
 

@@ -9,6 +9,8 @@
 #include "trace.h"
 
 struct prt_pixel_hit;  // include/prt_hip.h
+struct prt_context;
+struct prt_batch;
 
 namespace prt {
 
@@ -146,6 +148,23 @@ void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o
 constexpr int kTraceBlock = 256;
 constexpr int kStatCount = 12;
 constexpr int kShadeBlock = 256;
+
+// Planar FP32 images of a frame that rendering left on the device, as the denoiser reads them:
+// [3][W*H] sums with [W*H] sample weights (the frame buffer of a context) or [3][W*H] means
+// (weights null: a batch frame's outputs).  An auxiliary image is null when the frame has none.
+struct FrameImages
+{
+  int device = 0, W = 0, H = 0;
+  const float *color = nullptr, *colorW = nullptr;
+  const float *albedo = nullptr, *albedoW = nullptr;
+  const float *normal = nullptr, *normalW = nullptr;
+  hipStream_t stream = nullptr;  // the stream the producing kernels were enqueued on
+};
+// A context's frame: colour and the first-hit albedo / normal AOVs (context.hip).
+int contextFrameImages(prt_context* c, FrameImages* out);
+// A batch frame: colour (buffer PRT_BATCH_COLOR) or a snapshot, with the frame's first-hit
+// albedo / normal means (context.hip).
+int batchFrameImages(prt_batch* b, int frame, int buffer, FrameImages* out);
 
 // Slot <-> pixel mapping: slot = (tile * 64) + (ly * 8 + lx), tiles row-major over W/8 columns.
 PRT_DEV void slotToPixel(int slot, int W, int& x, int& y)

@@ -105,7 +105,17 @@ def relmse(x, ref):
 
 
 def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4, 8, 16), device=0, log=print):
-    """relMSE against a ref_spp render of the 1080p sun/sky workload for plain and denoised images."""
+    """relMSE against a ref_spp render of the 1080p sun/sky workload for plain and denoised images.
+
+    The reference is rendered in two halves of ref_spp / 2 samples (A: sample indices 0 .. n/2-1,
+    then the frame continues to n samples); B = 2 * full - A is the mean of the second half.  Each
+    half is an independent estimate, so mean((A - B)^2 / (ref^2 + 1e-2)) / 4 estimates the
+    reference's own noise term, which every relMSE against it contains in expectation:
+    E[relMSE(x, ref)] = relMSE(x, truth) + that term.  relmse_corrected subtracts it.
+
+    Denoised points run the denoiser on the frame where the renderer left it (prt_denoiser_
+    execute_frame / execute_batch): "ms" is render + denoise wall time, "ms_device" render +
+    denoiser kernels."""
     sc = G.spheres_sunsky()
     dev = P.DeviceHip(device)
     P.load_scene(dev, sc)
@@ -115,10 +125,16 @@ def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4
     dev.initSampler(7)
     dev.clearFrame()
     t0 = time.perf_counter()
-    for _ in range(ref_spp // 64):
+    half = ref_spp // 2
+    for _ in range(half // 64):
+        dev.render(64)
+    ref_a = dev.blitFrame()
+    for _ in range((ref_spp - half) // 64):
         dev.render(64)
     ref = dev.blitFrame()
-    log(f"reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    ref_b = 2.0 * ref - ref_a
+    ref_noise = float(np.mean((ref_a - ref_b) ** 2 / (ref ** 2 + 1e-2)) / 4.0)
+    log(f"reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s; its noise term {ref_noise:.5f}")
     res = {"plain": [], "denoised": []}
     dev.initSampler(0)
     dev.clearFrame()
@@ -126,7 +142,6 @@ def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4
     want = sorted(set(plain_spp) | set(denoised_spp))
     for spp in want:
         dev.synchronize()
-        t0 = time.perf_counter()
         dev.render(spp - done)
         dev.synchronize()
         done = spp
@@ -134,23 +149,19 @@ def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4
         if spp in plain_spp:
             res["plain"].append({"spp": spp, "relmse": relmse(img, ref)})
         if spp in denoised_spp:
-            alb, nrm = dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1)
-            out = np.zeros_like(img)
-            dn.init(img, alb, nrm, out)
-            secs = dn.execute()
+            secs = dn.executeFrame(dev)
+            out = dn.output()
             res["denoised"].append({"spp": spp, "relmse": relmse(out, ref), "denoise_s": secs, "denoise_kernel_ms": dn.kernelMs(),
                                     "noisy_relmse": relmse(img, ref)})
     # the batched form: the same 4-sample frame (seed 0) from the persistent batch renderer in one
-    # launch (it equals the wavefront renderer's image), then the denoiser
+    # launch (it equals the wavefront renderer's image), then the denoiser on the batch's output
     batch = P.BatchHip(device)
     batch.addFrame(dev, sc["camera"], W, H, 0, 4, aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1)
     batch.render()  # first launch builds the graph
     br = batch.render()
-    bimg, balb, bnrm = batch.get(0), batch.get(0, P.BATCH_ALBEDO1), batch.get(0, P.BATCH_NORMAL1)
+    bsecs = dn.executeBatch(batch, 0)
+    bout = dn.output()
     batch.close()
-    bout = np.zeros_like(bimg)
-    dn.init(bimg, balb, bnrm, bout)
-    bsecs = dn.execute()
     res["denoised_batched"] = {"spp": 4, "render_ms": br["ms"], "denoise_s": bsecs, "denoise_kernel_ms": dn.kernelMs(),
                                "relmse": relmse(bout, ref), "ms": br["ms"] + bsecs * 1e3,
                                "ms_device": br["ms"] + dn.kernelMs()}
@@ -166,9 +177,15 @@ def evaluate(dn, ref_spp=4096, plain_spp=(4, 16, 64, 256, 1024), denoised_spp=(4
     for r in res["denoised"]:
         r["ms"] = r["spp"] * ms_per_spp + r["denoise_s"] * 1e3
         r["ms_device"] = r["spp"] * ms_per_spp + r["denoise_kernel_ms"]
+    for r in res["plain"] + res["denoised"] + [res["denoised_batched"]]:
+        r["relmse_corrected"] = r["relmse"] - ref_noise
     res["ms_per_spp"] = ms_per_spp
     res["ref_spp"] = ref_spp
+    res["reference_noise_relmse"] = ref_noise
     res["time_to_relmse_1e-3_ms"] = {"plain": time_to(res["plain"], 1e-3), "denoised": time_to(res["denoised"], 1e-3)}
+    corrected = lambda pts: [{"ms": r["ms"], "relmse": r["relmse_corrected"]} for r in pts if r["relmse_corrected"] > 0]
+    res["time_to_relmse_1e-3_ms_corrected"] = {"plain": time_to(corrected(res["plain"]), 1e-3),
+                                               "denoised": time_to(corrected(res["denoised"]), 1e-3)}
     log(json.dumps(res))
     return res
 

@@ -22,6 +22,7 @@ AOV_ALBEDO, AOV_ALBEDO1, AOV_NORMAL, AOV_NORMAL1, AOV_DEPTH, AOV_VAR, AOV_HALF =
 FILTER_BOX, FILTER_TRIANGLE, FILTER_GAUSSIAN, FILTER_BLACKMAN_HARRIS = 0, 1, 2, 3
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
 BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_DEPTH, BATCH_SNAPSHOT = 0, 1, 2, 3, 16
+DENOISE_ALBEDO, DENOISE_NORMAL = 1, 2
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
@@ -110,6 +111,9 @@ SIGNATURES = {
     "prt_denoiser_set_weights": [_vp, _vp, ctypes.c_int64],
     "prt_denoiser_train_step": [_vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp],
     "prt_denoiser_gradient": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, ctypes.c_int64],
+    "prt_denoiser_execute_frame": [_vp, _vp, _i, _f, _vp],
+    "prt_denoiser_execute_batch": [_vp, _vp, _i, _i, _i, _f, _vp],
+    "prt_denoiser_get_output": [_vp, _vp, _i, _i],
     "prt_batch_create": [_i, _vp],
     "prt_batch_destroy": [_vp],
     "prt_batch_add_frame": [_vp, _vp, _vp, _i, _i, _u, _i, _vp, _i, _u, _vp],
@@ -570,6 +574,30 @@ class DenoiserHip:
         t = ctypes.c_double(0.0)
         _check(_lib.prt_denoiser_kernel_ms(self.h, ctypes.byref(t)))
         return t.value
+
+    def executeFrame(self, dev, albedo=True, normal=True, hdrScale=float("nan")):
+        """Denoises dev's frame on the device (colour + first-hit albedo / normal AOVs); the
+        result stays on the device until output().  Returns the call's wall time in seconds."""
+        t = ctypes.c_double(0.0)
+        inputs = (DENOISE_ALBEDO if albedo else 0) | (DENOISE_NORMAL if normal else 0)
+        _check(_lib.prt_denoiser_execute_frame(self.h, dev.h, inputs, hdrScale, ctypes.byref(t)))
+        self._outShape = dev.size[::-1]
+        return t.value
+
+    def executeBatch(self, batch, frame, buffer=0, albedo=True, normal=True, hdrScale=float("nan")):
+        """Denoises a rendered batch frame (its colour or a snapshot) on the device."""
+        t = ctypes.c_double(0.0)
+        inputs = (DENOISE_ALBEDO if albedo else 0) | (DENOISE_NORMAL if normal else 0)
+        _check(_lib.prt_denoiser_execute_batch(self.h, batch.h, frame, buffer, inputs, hdrScale, ctypes.byref(t)))
+        self._outShape = batch.frames[frame][1::-1]
+        return t.value
+
+    def output(self):
+        """The last executeFrame / executeBatch result as an H x W x 3 array."""
+        H, W = self._outShape
+        out = np.zeros((H, W, 3), np.float32)
+        _check(_lib.prt_denoiser_get_output(self.h, _ptr(out), W, H))
+        return out
 
     def denoise(self, color, albedo, normal):
         out = np.zeros(np.shape(color), np.float32)

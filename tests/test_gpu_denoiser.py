@@ -106,3 +106,81 @@ def test_trained_network_beats_the_noisy_render(dn, inputs):
     e_noisy, e_out = G.relmse(noisy, ref), G.relmse(out, ref)
     print(f"relMSE vs 256 spp: 4 spp {e_noisy:.4f}, denoised 4 spp {e_out:.4f}")
     assert e_out < 0.25 * e_noisy
+
+
+@pytest.mark.parametrize("inputs", [(True, True), (True, False), (False, False)])
+def test_execute_on_the_rendered_frame_equals_host_execute(dn, inputs):
+    # prt_denoiser_execute_frame reads the frame's sums and weights in HBM; its output must equal
+    # prt_denoiser_execute on the blitted images bit for bit (same means, same edge padding: the
+    # height 120 is not a multiple of 16)
+    import scenegen as G
+    alb_on, nrm_on = inputs
+    rng = np.random.default_rng(5)
+    dn.setWeights(R.he_init() + rng.standard_normal(R.param_count()).astype(np.float32) * 0.01)
+    sc = G.spheres_sunsky(n_spheres=40, subdiv=2, width=224, height=120)
+    dev = P.DeviceHip(0)
+    try:
+        P.load_scene(dev, sc)
+        dev.initFrame(224, 120, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+        dev.initSampler(0)
+        dev.clearFrame()
+        dev.render(3)
+        img, alb, nrm = dev.blitFrame(), dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1)
+        host = dn.denoise(img, alb if alb_on else None, nrm if nrm_on else None)
+        secs = dn.executeFrame(dev, albedo=alb_on, normal=nrm_on)
+        out = dn.output()
+    finally:
+        dev.close()
+    assert secs > 0 and dn.kernelMs() > 0
+    assert out.shape == host.shape
+    assert np.array_equal(out, host)
+
+
+def test_execute_on_a_batch_frame_equals_host_execute(dn):
+    import scenegen as G
+    rng = np.random.default_rng(6)
+    dn.setWeights(R.he_init() + rng.standard_normal(R.param_count()).astype(np.float32) * 0.01)
+    sc = G.spheres_sunsky(n_spheres=40, subdiv=2, width=160, height=96)
+    dev = P.DeviceHip(0)
+    batch = P.BatchHip(0)
+    try:
+        P.load_scene(dev, sc)
+        batch.addFrame(dev, sc["camera"], 160, 96, 0, 8, snapshots=(4,), aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+        batch.render()
+        for buf in (P.BATCH_COLOR, P.BATCH_SNAPSHOT):
+            img, alb, nrm = batch.get(0, buf), batch.get(0, P.BATCH_ALBEDO1), batch.get(0, P.BATCH_NORMAL1)
+            host = dn.denoise(img, alb, nrm)
+            dn.executeBatch(batch, 0, buf)
+            assert np.array_equal(dn.output(), host)
+        with pytest.raises(ValueError):
+            dn.executeBatch(batch, 0, P.BATCH_ALBEDO1)
+    finally:
+        batch.close()
+        dev.close()
+
+
+def test_inference_at_1080p(dn):
+    # the learned-denoiser workload's inference size: one 1920 x 1080 frame (padded to 1088 rows),
+    # denoised where the renderer left it; equals the host path, and its kernel time is reported
+    import scenegen as G
+    dn.setWeights(np.load(__import__("denoise_train").weights_path("full")))
+    sc = G.spheres_sunsky(n_spheres=64, subdiv=2, width=1920, height=1080)
+    dev = P.DeviceHip(0)
+    try:
+        P.load_scene(dev, sc)
+        dev.initFrame(1920, 1080, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
+        dev.initSampler(0)
+        dev.clearFrame()
+        dev.render(4)
+        img, alb, nrm = dev.blitFrame(), dev.blitFrame(P.BUF_ALBEDO1), dev.blitFrame(P.BUF_NORMAL1)
+        host = dn.denoise(img, alb, nrm)
+        ms = []
+        for _ in range(3):
+            dn.executeFrame(dev)
+            ms.append(dn.kernelMs())
+        out = dn.output()
+    finally:
+        dev.close()
+    print(f"1080p inference: {min(ms):.3f} ms of kernels")
+    assert np.array_equal(out, host)
+    assert np.all(np.isfinite(out))
