@@ -200,6 +200,37 @@ int prt_get_info(prt_context* ctx, char* buf, int len);
 int prt_scene_set_geometry(prt_context* ctx, int32_t nverts, const float* positions, const float* normals,
                            int32_t ntris, const int32_t* indices, const int32_t* tri_material);
 int prt_scene_set_materials(prt_context* ctx, int32_t n, const prt_material* materials);
+/* Image textures on material parameters (TriangleMesh texcoords, geometry/triangle_mesh.cpp:
+ * 83-90, 131; ImageTexture, image/image_texture.h:148-300; MaterialParam3f,
+ * material/material_param.h:14-60).  set_uvs: texture coordinate set 0, nverts x 2 floats for the
+ * vertices of the current geometry (interpolated barycentrically at a hit; NULL or no call: the
+ * hit's barycentric (u, v) are the coordinates, as for a mesh without texcoords; new geometry
+ * drops them).  set_textures: the scene's textures, each width x height texels given as image
+ * rows top to bottom (the reference stores them bottom to top, image_texture.h:352-353, so
+ * (u, v) = (0, 0) is the lower-left corner of the image), decoded when uploaded: 8-bit RGBA
+ * through the reference's 2.2-gamma table (sRGB, image/pixel.cpp:21) or /255 (linear), or RGB
+ * FP32; the lookup is the reference's bilinear filter with half-texel offset and repeat or clamp
+ * addressing per axis, at uv * uv_scale + uv_offset (ImageTextureParams).  material_set_texture:
+ * parameter PRT_PARAM_BASE_COLOR of a diffuse or coated material (kinds 0, 3) becomes
+ * clamp(texture(uv) x (r, g, b)), the value a textured baseColor parameter gives
+ * (material/standard.cpp:424); texture -1 unbinds.  prt_scene_set_materials drops the bindings.
+ * The adjoint rejects scenes with textured materials. */
+#define PRT_TEX_RGBA8_SRGB 0
+#define PRT_TEX_RGBA8_LINEAR 1
+#define PRT_TEX_RGB32F 2
+#define PRT_TEX_REPEAT 0
+#define PRT_TEX_CLAMP 1
+#define PRT_PARAM_BASE_COLOR 0
+typedef struct prt_texture {
+  int32_t width, height;
+  int32_t format;                /* PRT_TEX_* */
+  int32_t address_u, address_v;  /* PRT_TEX_REPEAT or PRT_TEX_CLAMP */
+  float uv_scale[2], uv_offset[2];
+  const void* texels;            /* width x height: 4 bytes (R, G, B, A) or 3 floats each */
+} prt_texture;
+int prt_scene_set_uvs(prt_context* ctx, const float* uv, int32_t nverts);
+int prt_scene_set_textures(prt_context* ctx, int32_t n, const prt_texture* textures);
+int prt_material_set_texture(prt_context* ctx, int32_t material, int32_t parameter, int32_t texture);
 /* Device::initLights (render/scene.cpp:365-450): n (0..8) environment lights in reference
  * order, optional lat-long environment image env_rgb (env_h x env_w x 3, texture row order:
  * row 0 = nadir, image/image_texture.h:396), lut_max as the image light's lutSize prop.  Call

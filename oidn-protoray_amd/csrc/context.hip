@@ -339,7 +339,16 @@ struct prt_context
   DevBuf<float> pos, nrm;
   DevBuf<int> idx, triMat;
   std::vector<prt_material> materials;
+  std::vector<prt_material_constants> consts;  // host copy of mats, before texture bindings
   DevBuf<prt_material_constants> mats;
+  // image textures: per-vertex UVs, texture records, texels (decoded, rows bottom to top) and
+  // the baseColor texture of each material (-1 none; reset by prt_scene_set_materials)
+  DevBuf<float> uv;
+  std::vector<prt_texture_dev> texHost;
+  DevBuf<prt_texture_dev> tex;
+  DevBuf<float4> texels;
+  std::vector<int> baseTex;
+  bool textured = false;  // some material has a texture bound
   DevBuf<float4> media;  // interior media: (r, g, b, ior) per medium ID, ID 0 clear
   std::vector<float> luts;
   DevBuf<float> lut;
@@ -540,6 +549,7 @@ int prt_scene_set_geometry(prt_context* c, int32_t nverts, const float* position
     HIP_TRY(hipMemcpy(c->nrm.p, normals, sizeof(float) * 3 * nverts, hipMemcpyHostToDevice));
   }
   else c->nrm.release();
+  c->uv.release();  // UVs belong to the vertices: set them again after new geometry
   // scene bounds over the referenced vertices (render/scene.cpp:81, :367)
   float lo[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f}, hi[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
   for (int64_t i = 0; i < int64_t(ntris) * 3; ++i)
@@ -679,6 +689,22 @@ static void lobeConstants(const std::vector<float>& luts, const LobeParams& p, p
 }
 
 static float foldIor(float ior) { return ior < 1.f ? 1.0f / ior : ior; }
+
+// The material records with their texture bindings applied, to the device.
+static int uploadMaterialConstants(prt_context* c)
+{
+  std::vector<prt_material_constants> k = c->consts;
+  c->textured = false;
+  for (size_t i = 0; i < c->baseTex.size(); ++i)
+  {
+    k[i].texBase = c->baseTex[i] + 1;
+    for (int j = 0; j < 3; ++j) k[i].texScale[j] = (&c->materials[i].r)[j];
+    c->textured = c->textured || c->baseTex[i] >= 0;
+  }
+  HIP_TRY(c->mats.alloc(k.size()));
+  HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * k.size(), hipMemcpyHostToDevice));
+  return PRT_OK;
+}
 
 int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
 {
@@ -894,12 +920,118 @@ int prt_scene_set_materials(prt_context* c, int32_t n, const prt_material* m)
   for (size_t t = 1; t < media.size(); ++t)  // a coloured medium absorbs (the base kernel skips that)
     if (media[t].x != 1.f || media[t].y != 1.f || media[t].z != 1.f) c->extMaterials = 1;
   k.insert(k.end(), extra.begin(), extra.end());
-  HIP_TRY(c->mats.alloc(k.size()));
-  HIP_TRY(hipMemcpy(c->mats.p, k.data(), sizeof(prt_material_constants) * k.size(), hipMemcpyHostToDevice));
+  c->consts = k;
+  c->baseTex.assign(size_t(n), -1);
+  int rc = uploadMaterialConstants(c);
+  if (rc != PRT_OK) return rc;
   HIP_TRY(c->media.alloc(media.size()));
   HIP_TRY(hipMemcpy(c->media.p, media.data(), sizeof(float4) * media.size(), hipMemcpyHostToDevice));
   c->lightsReady = false;  // the area light depends on the materials: set the lights again
   return PRT_OK;
+}
+
+int prt_scene_set_uvs(prt_context* c, const float* uv, int32_t nverts)
+{
+  CTX_CHECK(c);
+  ++c->generation;
+  if (!uv)
+  {
+    c->uv.release();
+    return PRT_OK;
+  }
+  if (nverts != c->nverts) return fail(PRT_ERR_INVALID, "UVs need one pair per vertex of the geometry");
+  for (int64_t i = 0; i < int64_t(nverts) * 2; ++i)
+    if (!std::isfinite(uv[i])) return fail(PRT_ERR_INVALID, "UVs must be finite");
+  HIP_TRY(c->uv.alloc(size_t(nverts) * 2));
+  if (nverts) HIP_TRY(hipMemcpy(c->uv.p, uv, sizeof(float) * 2 * size_t(nverts), hipMemcpyHostToDevice));
+  return PRT_OK;
+}
+
+int prt_scene_set_textures(prt_context* c, int32_t n, const prt_texture* t)
+{
+  CTX_CHECK(c);
+  ++c->generation;
+  if (n < 0 || (n > 0 && !t)) return fail(PRT_ERR_INVALID, "invalid texture array");
+  for (int b : c->baseTex)
+    if (b >= n) return fail(PRT_ERR_INVALID, "a material is bound to a texture beyond the new set");
+  // 8-bit sRGB decode: the reference's 2.2-gamma table (image/pixel.cpp:21-86, generated there
+  // as pow(i / 255, 2.2) in float)
+  float srgb[256];
+  for (int i = 0; i < 256; ++i) srgb[i] = std::pow(float(i) / 255.0f, 2.2f);
+  std::vector<prt_texture_dev> recs(static_cast<size_t>(n));
+  int64_t total = 0;
+  for (int i = 0; i < n; ++i)
+  {
+    const prt_texture& x = t[i];
+    if (x.width < 1 || x.height < 1 || !x.texels) return fail(PRT_ERR_INVALID, "texture needs a size and texels");
+    if (x.format < PRT_TEX_RGBA8_SRGB || x.format > PRT_TEX_RGB32F) return fail(PRT_ERR_INVALID, "unknown texture format");
+    if ((x.address_u != PRT_TEX_REPEAT && x.address_u != PRT_TEX_CLAMP) ||
+        (x.address_v != PRT_TEX_REPEAT && x.address_v != PRT_TEX_CLAMP))
+      return fail(PRT_ERR_INVALID, "unknown texture address mode");
+    prt_texture_dev& r = recs[size_t(i)];
+    r.offset = int32_t(total);
+    r.w = x.width;
+    r.h = x.height;
+    r.addrU = x.address_u == PRT_TEX_CLAMP ? PRT_TEX_ADDR_CLAMP : PRT_TEX_ADDR_REPEAT;
+    r.addrV = x.address_v == PRT_TEX_CLAMP ? PRT_TEX_ADDR_CLAMP : PRT_TEX_ADDR_REPEAT;
+    r.scaleU = x.uv_scale[0];
+    r.scaleV = x.uv_scale[1];
+    r.offU = x.uv_offset[0];
+    r.offV = x.uv_offset[1];
+    r.halfU = 0.5f / float(x.width);  // halfTexel = 0.5 / size (image_texture.h:163)
+    r.halfV = 0.5f / float(x.height);
+    r.pad = 0;
+    total += int64_t(x.width) * x.height;
+    if (total >= (int64_t(1) << 31)) return fail(PRT_ERR_INVALID, "textures exceed 2^31 texels");
+  }
+  std::vector<float4> texels(size_t(std::max<int64_t>(total, 1)));
+  for (int i = 0; i < n; ++i)
+  {
+    const prt_texture& x = t[i];
+    const prt_texture_dev& r = recs[size_t(i)];
+    for (int y = 0; y < x.height; ++y)
+      for (int xx = 0; xx < x.width; ++xx)
+      {
+        // texel row y counts from the bottom: image row height - 1 - y (image_texture.h:352-353)
+        const size_t src = size_t(x.height - 1 - y) * x.width + xx;
+        float4 v;
+        if (x.format == PRT_TEX_RGB32F)
+        {
+          const float* f = static_cast<const float*>(x.texels) + 3 * src;
+          v = make_float4(f[0], f[1], f[2], 1.f);
+        }
+        else
+        {
+          const uint8_t* b = static_cast<const uint8_t*>(x.texels) + 4 * src;
+          if (x.format == PRT_TEX_RGBA8_SRGB)  // decodeBgra8 (image/pixel.h:80-91)
+            v = make_float4(srgb[b[0]], srgb[b[1]], srgb[b[2]], float(b[3]) * (1.f / 255.f));
+          else  // decodeBgra8Linear (:137-145)
+            v = make_float4(float(b[0]) * (1.f / 255.f), float(b[1]) * (1.f / 255.f), float(b[2]) * (1.f / 255.f),
+                            float(b[3]) * (1.f / 255.f));
+        }
+        texels[size_t(r.offset) + size_t(y) * x.width + xx] = v;
+      }
+  }
+  c->texHost = recs;
+  HIP_TRY(c->tex.alloc(std::max<size_t>(recs.size(), 1)));
+  if (n) HIP_TRY(hipMemcpy(c->tex.p, recs.data(), sizeof(prt_texture_dev) * recs.size(), hipMemcpyHostToDevice));
+  HIP_TRY(c->texels.alloc(texels.size()));
+  HIP_TRY(hipMemcpy(c->texels.p, texels.data(), sizeof(float4) * texels.size(), hipMemcpyHostToDevice));
+  return PRT_OK;
+}
+
+int prt_material_set_texture(prt_context* c, int32_t material, int32_t parameter, int32_t texture)
+{
+  CTX_CHECK(c);
+  ++c->generation;
+  if (material < 0 || material >= int32_t(c->materials.size())) return fail(PRT_ERR_INVALID, "invalid material ID");
+  if (parameter != PRT_PARAM_BASE_COLOR) return fail(PRT_ERR_INVALID, "only the base colour can be textured");
+  if (texture < -1 || texture >= int32_t(c->texHost.size())) return fail(PRT_ERR_INVALID, "invalid texture index");
+  const int kind = c->materials[size_t(material)].kind;
+  if (texture >= 0 && kind != PRT_KIND_DIFFUSE && kind != PRT_KIND_COATED)
+    return fail(PRT_ERR_INVALID, "base colour textures apply to diffuse and coated materials");
+  c->baseTex[size_t(material)] = texture;
+  return uploadMaterialConstants(c);
 }
 
 // light/triangle_light.h:36-64 + render/scene.cpp:83-116: one TriangleLight over the emissive
@@ -1305,6 +1437,7 @@ static RenderArgs makeArgs(prt_context* c)
   RenderArgs a;
   std::memset(&a, 0, sizeof(a));
   a.pos = c->pos.p; a.nrm = c->nrm.p; a.idx = c->idx.p; a.triMat = c->triMat.p; a.mats = c->mats.p; a.media = c->media.p; a.lut = c->lut.p;
+  a.uv = c->uv.p; a.tex = c->tex.p; a.texels = c->texels.p;
   a.anyTransparent = c->anyTransparent;
   a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
   a.nLights = int(c->lights.size());
@@ -1415,7 +1548,8 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
     {
       launchIntersect(ad, depth, c->launch, c->countSteps, c->stream);
       HIP_TRY(hipEventRecord(ev[2 + 3 * depth], c->stream));
-      launchShade(ad, depth, c->sampleIndex, classB, c->launch, c->stdMaterials ? 2 : (c->extMaterials ? 1 : 0), c->stream);
+      launchShade(ad, depth, c->sampleIndex, classB, c->launch, c->stdMaterials ? 2 : (c->extMaterials || c->textured ? 1 : 0),
+                  c->stream);
       HIP_TRY(hipEventRecord(ev[3 + 3 * depth], c->stream));
       launchOccluded(ad, depth, classB, c->launch, c->anyTransparent ? (c->stdMaterials ? 2 : 1) : 0, c->countSteps, c->stream);
       HIP_TRY(hipEventRecord(ev[4 + 3 * depth], c->stream));
@@ -1504,6 +1638,7 @@ int prt_adjoint(prt_context* c, const float* target, int32_t spp, double* loss, 
   if (spp < 1) return fail(PRT_ERR_INVALID, "spp must be >= 1");
   if (c->params.maxDepth > PRT_ADJ_MAX_VERTICES)
     return fail(PRT_ERR_INVALID, "the adjoint supports maxDepth <= " + std::to_string(PRT_ADJ_MAX_VERTICES));
+  if (c->textured) return fail(PRT_ERR_INVALID, "the adjoint does not differentiate textured materials");
   const int nMat = int(c->materials.size());
   const int nParams = 4 * nMat;
   if (grad_capacity < nParams) return fail(PRT_ERR_INVALID, "grad needs 4 floats per material");

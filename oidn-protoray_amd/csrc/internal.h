@@ -42,6 +42,11 @@ struct RenderArgs
   const prt_material_constants* mats;
   const float4* media;  // interior media (r, g, b, ior) by medium ID; ID 0 is clear
   const float* lut;
+  // image textures: per-vertex UVs (set 0, 2 floats per vertex, null: the hit's barycentric
+  // u, v stand in, geometry/triangle_mesh.cpp:131), texture records and their texels
+  const float* uv;
+  const prt_texture_dev* tex;
+  const float4* texels;
   BvhView bvh;
   int anyTransparent;  // some material lets shadow rays through (render/scene.cpp:452-462)
   prt_light_dev lights[PRT_MAX_LIGHTS];  // environment lights

@@ -96,6 +96,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
     const f3 wo = -d;
     const bool inside = m.mediaInterface && (mediumId == m.mediumInt);  // material/material.h:54-61
     Bsdf b = makeBsdf(m, ctx, wo, inside, true, kStd);
+    if (m.texBase) applyBaseTexture(a, m, prim, u, v, b);
     Sampler smp;
     smp.set(a.seed, sampleIndex, pixelId);
     if (mediumId > 0) T = T * mediumTransmittance(a.media[mediumId], th);  // volume transmission (:652-657)

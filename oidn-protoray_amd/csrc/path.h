@@ -105,6 +105,62 @@ PRT_DEV f3 mediumTransmittance(float4 medium, float t)
   return f3(powf(medium.x, t), powf(medium.y, t), powf(medium.z, t));
 }
 
+// clamp(x, lo, hi) as math/math.h:83-86 writes it: min(lo < x ? x : lo, hi)
+PRT_DEV float clampRef(float x, float lo, float hi) { return fminf(lo < x ? x : lo, hi); }
+
+// Bilinear lookup (image/image_texture.h:186-227): half-texel offset, repeat or clamp per axis,
+// the four decoded texels interpolated as lerp(a, b, s) = a + s (b - a) (math/math.h:88-92),
+// first along x.
+PRT_DEV f3 textureLookup(const prt_texture_dev& t, const float4* texels, float u, float v)
+{
+  const float x = u - t.halfU, y = v - t.halfV;
+  const float fw = float(t.w), fh = float(t.h);
+  const float xr = t.addrU == PRT_TEX_ADDR_REPEAT ? (x - floorf(x)) * fw : clampRef(x * fw, 0.f, float(t.w - 1));
+  const float yr = t.addrV == PRT_TEX_ADDR_REPEAT ? (y - floorf(y)) * fh : clampRef(y * fh, 0.f, float(t.h - 1));
+  const float sx = xr - floorf(xr), sy = yr - floorf(yr);
+  const int x0 = min(max(int(xr), 0), t.w - 1), y0 = min(max(int(yr), 0), t.h - 1);
+  const int x1 = t.addrU == PRT_TEX_ADDR_REPEAT ? (x0 + 1 == t.w ? 0 : x0 + 1) : min(x0 + 1, t.w - 1);
+  const int y1 = t.addrV == PRT_TEX_ADDR_REPEAT ? (y0 + 1 == t.h ? 0 : y0 + 1) : min(y0 + 1, t.h - 1);
+  const float4* T = texels + t.offset;
+  const float4 c00 = T[y0 * t.w + x0], c11 = T[y1 * t.w + x1], c01 = T[y1 * t.w + x0], c10 = T[y0 * t.w + x1];
+  const f3 a00(c00.x, c00.y, c00.z), a11(c11.x, c11.y, c11.z), a01(c01.x, c01.y, c01.z), a10(c10.x, c10.y, c10.z);
+  const f3 c0 = a00 + sx * (a10 - a00), c1 = a01 + sx * (a11 - a01);
+  return c0 + sy * (c1 - c0);
+}
+
+// A textured baseColor (kinds 0 and 3): baseColor = clamp(texture(uv) * scale)
+// (material/material_param.h:37-41, material/standard.cpp:424) replaces the record's colour and
+// what depends on it: the diffuse colour and importance (bsdf/diffuse.h:70-75) and, under a
+// coating, the base importance, the coating's importance and its albedo (bsdf/rough_coating.h:
+// 562-571).  UV set 0 interpolates barycentrically (geometry/triangle_mesh.cpp:191-200).
+PRT_DEV void applyBaseTexture(const RenderArgs& a, const prt_material_constants& m, int prim, float u, float v, Bsdf& b)
+{
+  const prt_texture_dev& t = a.tex[m.texBase - 1];
+  float tu = u, tv = v;
+  if (a.uv)
+  {
+    const int i0 = a.idx[3 * prim], i1 = a.idx[3 * prim + 1], i2 = a.idx[3 * prim + 2];
+    // b0 uv0 + b1 uv1 + b2 uv2 in the reference build's evaluation: two fused multiply-adds
+    const float b0 = 1.f - u - v;
+    tu = fmaf(v, a.uv[2 * i2], fmaf(u, a.uv[2 * i1], b0 * a.uv[2 * i0]));
+    tv = fmaf(v, a.uv[2 * i2 + 1], fmaf(u, a.uv[2 * i1 + 1], b0 * a.uv[2 * i0 + 1]));
+  }
+  const f3 c = textureLookup(t, a.texels, tu * t.scaleU + t.offU, tv * t.scaleV + t.offV);
+  const f3 rgb(clampRef(c.x * m.texScale[0], 0.f, 1.f), clampRef(c.y * m.texScale[1], 0.f, 1.f),
+               clampRef(c.z * m.texScale[2], 0.f, 1.f));
+  b.c0 = rgb;
+  const float imp = fmaxf(fmaxf(rgb.x, rgb.y), rgb.z);
+  if (b.kind == PRT_KIND_COATED)
+  {
+    b.c2.z = imp;
+    b.importance = fmaxf(1.f, imp);
+    b.rr = f3(rgb.x + b.c2.y * (rgb.x * powf(b.c1.x, b.c2.x) - rgb.x), rgb.y + b.c2.y * (rgb.y * powf(b.c1.y, b.c2.x) - rgb.y),
+              rgb.z + b.c2.y * (rgb.z * powf(b.c1.z, b.c2.x) - rgb.z));
+  }
+  else
+    b.importance = imp;
+}
+
 PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, float u, float v)
 {
   Ctx c;

@@ -94,8 +94,27 @@ struct prt_material_constants
   float coatEta[2], coatRavg[2];
   float coatColor[3];
   float coatThickness, coatScale, coatRoughness, coatAlpha;
+  // textured baseColor of kinds 0 and 3 (material/material_param.h:37-41, material/standard.cpp:
+  // 424): 1 + texture index (0: none) and the parameter's scale, the material's (r, g, b)
+  int32_t texBase;
+  float texScale[3];
 };
-static_assert(sizeof(prt_material_constants) == 312, "material constants: 312 bytes");
+static_assert(sizeof(prt_material_constants) == 328, "material constants: 328 bytes");
+
+// Image texture as the kernels read it (image/image_texture.h:148-227): texels decoded to linear
+// float4 at upload, rows bottom to top (the reference stores images upside down,
+// image_texture.h:352-353), from texel `offset` of the scene's texel array.  48 bytes.
+#define PRT_TEX_ADDR_REPEAT 0
+#define PRT_TEX_ADDR_CLAMP 1
+struct prt_texture_dev
+{
+  int32_t offset;
+  int32_t w, h;
+  int32_t addrU, addrV;
+  float scaleU, scaleV, offU, offV;  // uv * scale + offset (image_texture.h:266-269)
+  float halfU, halfV;                // half-texel size 0.5 / size (:163)
+  int32_t pad;
+};
 
 // Environment light record as the kernels read it.  64 bytes.
 struct prt_light_dev

@@ -277,6 +277,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
         int mediumId = fl >> PRT_F_MEDIUM_SHIFT;
         bool inside = m.mediaInterface && (mediumId == m.mediumInt);  // material/material.h:54-61
         Bsdf b = makeBsdf(m, ctx, wo, inside, kLevel >= 1, kLevel >= 2);
+        if (kLevel >= 1 && m.texBase) applyBaseTexture(a, m, prim, h.y, h.z, b);
         Sampler smp;
         smp.set(a.seed, sampleIndex, pixelId);  // resetSample (:641-642)
         if (kLevel >= 1 && mediumId > 0) T = T * mediumTransmittance(a.media[mediumId], h.x);  // volume transmission (:652-657)

@@ -23,6 +23,14 @@ FILTER_BOX, FILTER_TRIANGLE, FILTER_GAUSSIAN, FILTER_BLACKMAN_HARRIS = 0, 1, 2, 
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
 BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_DEPTH, BATCH_SNAPSHOT = 0, 1, 2, 3, 16
 DENOISE_ALBEDO, DENOISE_NORMAL = 1, 2
+TEX_RGBA8_SRGB, TEX_RGBA8_LINEAR, TEX_RGB32F = 0, 1, 2
+TEX_REPEAT, TEX_CLAMP = 0, 1
+
+
+class Texture(ctypes.Structure):  # prt_texture (image/image_texture.h:148-300)
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("format", ctypes.c_int32),
+                ("address_u", ctypes.c_int32), ("address_v", ctypes.c_int32),
+                ("uv_scale", ctypes.c_float * 2), ("uv_offset", ctypes.c_float * 2), ("texels", ctypes.c_void_p)]
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("r", "<f4"), ("g", "<f4"), ("b", "<f4"),
                            ("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("roughness", "<f4"),
                            ("ior", "<f4"), ("thickness", "<f4"), ("weight", "<f4"),
@@ -112,6 +120,9 @@ SIGNATURES = {
     "prt_denoiser_train_step": [_vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp],
     "prt_denoiser_gradient": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, ctypes.c_int64],
     "prt_denoiser_execute_frame": [_vp, _vp, _i, _f, _vp],
+    "prt_scene_set_uvs": [_vp, _vp, _i],
+    "prt_scene_set_textures": [_vp, _i, _vp],
+    "prt_material_set_texture": [_vp, _i, _i, _i],
     "prt_denoiser_execute_batch": [_vp, _vp, _i, _i, _i, _f, _vp],
     "prt_denoiser_get_output": [_vp, _vp, _i, _i],
     "prt_batch_create": [_i, _vp],
@@ -225,6 +236,32 @@ class DeviceHip:
     def initCamera(self, camera):
         c = np.ascontiguousarray(camera, CAMERA_DTYPE)
         _check(_lib.prt_set_camera(self.h, _ptr(c)))
+
+    def initUVs(self, uv):
+        """Texture coordinate set 0 (nverts x 2), or None to drop it."""
+        if uv is None:
+            _check(_lib.prt_scene_set_uvs(self.h, None, 0))
+            return
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        _check(_lib.prt_scene_set_uvs(self.h, _ptr(uv), uv.shape[0]))
+
+    def initTextures(self, textures):
+        """Textures as dicts: texels (H x W x 4 uint8 for formats 0 / 1, H x W x 3 float32 for 2;
+        image rows top to bottom), format, address (u, v), uv_scale, uv_offset."""
+        recs = (Texture * max(len(textures), 1))()
+        keep = []
+        for k, t in enumerate(textures):
+            tx = np.ascontiguousarray(t["texels"], np.float32 if t["format"] == TEX_RGB32F else np.uint8)
+            keep.append(tx)
+            au, av = t.get("address", (TEX_REPEAT, TEX_REPEAT))
+            su, sv = t.get("uv_scale", (1.0, 1.0))
+            ou, ov = t.get("uv_offset", (0.0, 0.0))
+            recs[k] = Texture(tx.shape[1], tx.shape[0], int(t["format"]), int(au), int(av), (su, sv), (ou, ov), tx.ctypes.data)
+        _check(_lib.prt_scene_set_textures(self.h, len(textures), recs))
+
+    def setMaterialTexture(self, material, texture, parameter=0):
+        """Binds texture index `texture` (-1: none) to the material's base colour."""
+        _check(_lib.prt_material_set_texture(self.h, material, parameter, texture))
 
     def initFrame(self, width, height, aov_mask=0):
         _check(_lib.prt_frame_init(self.h, width, height, aov_mask))
@@ -469,6 +506,12 @@ def load_scene_file(path):
 def load_scene(dev, scene):
     """Uploads a scene dict from scenes/scenegen.py and configures renderer, camera and frame."""
     dev.initScene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"], scene.get("normals"))
+    if scene.get("uvs") is not None:
+        dev.initUVs(scene["uvs"])
+    if scene.get("textures"):
+        dev.initTextures(scene["textures"])
+    for mat, tex in scene.get("material_textures", ()):
+        dev.setMaterialTexture(mat, tex)
     dev.initLights(scene["lights"], scene.get("env"))
     dev.initRenderer(maxDepth=scene["max_depth"], rrDepth=scene.get("rr_depth", 8))
     dev.initSampler(scene.get("seed", 0))

@@ -298,6 +298,65 @@ def spheres_sunsky(seed=0, n_spheres=4096, subdiv=4, width=1920, height=1080, sp
                 max_depth=max_depth, seed=0)
 
 
+def spheres_textured(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4, max_depth=8, palette=24):
+    """The sun/sky spheres with image textures on base colours: per-vertex UVs (the ground's
+    planar x/z map, a latitude/longitude map on each sphere), three textures (an 8x8 sRGB RGBA8
+    checker with repeat addressing and a 4x tiling on the ground, a 16x4 RGB FP32 gradient with
+    clamp addressing, a 5x3 linear RGBA8 texture repeating in u and clamped in v with an offset),
+    bound to the ground, to diffuse spheres (some with Oren-Nayar bases) and to coated spheres;
+    conductors and dielectrics stay uniform."""
+    rng = Pcg32(seed)
+    mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.9, 0.9, 0.9))]
+    bind = [(1, 0)]
+    for i in range(palette):
+        v = i % 4
+        rgb = (rng.f(0.5, 1.2), rng.f(0.5, 1.2), rng.f(0.5, 1.2))  # scales of the texture (clamped product)
+        if v == 0:
+            mats.append(_material(KIND_DIFFUSE, rgb, diffuse_roughness=0.6 if i % 8 == 0 else 0.0))
+            bind.append((len(mats) - 1, 1 + (i // 4) % 2))
+        elif v == 1:
+            mats.append(_material(KIND_COATED, rgb, (1, 1, 1), math.sqrt(rng.f(0.02, 0.4)), rng.f(1.3, 1.8), thickness=1.0,
+                                  weight=1.0))
+            bind.append((len(mats) - 1, (i // 4) % 3))
+        elif v == 2:
+            r, g = gold_artistic(rng)
+            mats.append(_material(KIND_CONDUCTOR, r, g, math.sqrt(rng.f(0.05, 0.5))))
+        else:
+            mats.append(_material(KIND_DIELECTRIC, roughness=math.sqrt(rng.f(0.01, 0.3)), ior=1.5))
+    pos, idx, tri_mat = _assemble(rng, n_spheres, subdiv, lambda i: 2 + rng.u32() % palette)
+    # UVs: ground x/z over [0, 1]^2; spheres by latitude / longitude of the unit icosphere vertex
+    Vs, _ = icosphere(subdiv)
+    ground = pos[:4]
+    uv_g = np.stack([(ground[:, 0] + 20.0) / 40.0, (ground[:, 2] + 2.5) / 40.0], axis=1)
+    uv_s = np.stack([0.5 + np.arctan2(Vs[:, 2], Vs[:, 0]) / (2 * math.pi), 0.5 + np.arcsin(np.clip(Vs[:, 1], -1, 1)) / math.pi],
+                    axis=1)
+    uvs = np.concatenate([uv_g, np.tile(uv_s, (n_spheres, 1))]).astype(np.float32)
+    trng = Pcg32(seed + 101)
+    checker = np.zeros((8, 8, 4), np.uint8)
+    ca = [int(trng.f(40, 255)) for _ in range(3)]
+    cb = [int(trng.f(0, 120)) for _ in range(3)]
+    for y in range(8):
+        for x in range(8):
+            checker[y, x, :3] = ca if (x + y) % 2 == 0 else cb
+            checker[y, x, 3] = 255
+    gradient = np.zeros((4, 16, 3), np.float32)
+    for y in range(4):
+        for x in range(16):
+            gradient[y, x] = (x / 15.0, 0.2 + 0.2 * y, 1.0 - x / 15.0)
+    small = np.array([[[int(trng.f(0, 255)) for _ in range(3)] + [255] for _ in range(5)] for _ in range(3)], np.uint8)
+    textures = [
+        dict(texels=checker, format=0, address=(0, 0), uv_scale=(4.0, 4.0), uv_offset=(0.0, 0.0)),
+        dict(texels=gradient, format=2, address=(1, 1), uv_scale=(1.0, 1.0), uv_offset=(0.0, 0.0)),
+        dict(texels=small, format=1, address=(0, 1), uv_scale=(2.0, 1.5), uv_offset=(0.25, -0.2)),
+    ]
+    sc = spheres_sunsky(n_spheres=0, width=width, height=height, spp=spp, max_depth=max_depth, palette=1)
+    return dict(name="spheres_textured", positions=pos, indices=idx, tri_material=tri_mat,
+                materials=np.array(mats, MATERIAL_DTYPE), lights=sc["lights"], env=sc["env"], uvs=uvs,
+                textures=textures, material_textures=bind,
+                camera=_camera(lens_radius=0.05, focal=15.0), width=width, height=height, spp=spp,
+                max_depth=max_depth, seed=0)
+
+
 def spheres_lobes(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4, max_depth=8, palette=24):
     """The sun/sky scene of spheres_sunsky with the remaining StandardMaterial lobes: rough
     diffuse (Oren-Nayar) bases, coats over them, thin rough dielectrics (thin-walled bubbles with

@@ -78,6 +78,12 @@ def lib():
         L.orc_scene_create.restype = vp
         L.orc_scene_create.argtypes = [i, vp, vp, i, vp, vp, i, vp]
         L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_set_uvs.argtypes = [vp, vp, i]
+        L.orc_scene_add_texture.restype = i
+        L.orc_scene_add_texture.argtypes = [vp, i, i, i, i, i, vp, vp]
+        L.orc_material_set_texture.argtypes = [vp, i, i]
+        L.orc_texture_lookup.argtypes = [vp, i, i, vp, vp]
+        L.orc_hit_uv.argtypes = [vp, i, vp, vp, vp]
         L.orc_scene_radius.restype = f
         L.orc_scene_radius.argtypes = [vp]
         L.orc_scene_set_lights.restype = i
@@ -259,6 +265,39 @@ class Scene:
                                         None if env is None else env.ctypes.data, lut_max)
         if rc != 0:
             raise ValueError("invalid light list")
+
+    def set_textures(self, scene):
+        """UV set 0, textures and baseColor bindings of a scene dict (keys "uvs", "textures",
+        "material_textures"; see scenes/scenegen.py), as the renderer gets them."""
+        uv = scene.get("uvs")
+        if uv is not None:
+            self._uv = c32(uv)
+            lib().orc_scene_set_uvs(self.h, self._uv.ctypes.data, self._uv.shape[0])
+        for t in scene.get("textures", ()):
+            self.add_texture(t)
+        for mat, tex in scene.get("material_textures", ()):
+            lib().orc_material_set_texture(self.h, int(mat), int(tex))
+
+    def add_texture(self, t):
+        texels = np.ascontiguousarray(t["texels"], np.float32 if t["format"] == 2 else np.uint8)
+        H, W = texels.shape[:2]
+        au, av = t.get("address", (0, 0))
+        prm = c32(list(t.get("uv_scale", (1.0, 1.0))) + list(t.get("uv_offset", (0.0, 0.0))))
+        return lib().orc_scene_add_texture(self.h, W, H, int(t["format"]), int(au), int(av), prm.ctypes.data, texels.ctypes.data)
+
+    def hit_uv(self, prim, uv):
+        """UV set 0 at hits (prim, barycentric u, v)."""
+        prim, uv = i32(prim), c32(uv).reshape(-1, 2)
+        out = np.zeros_like(uv)
+        lib().orc_hit_uv(self.h, prim.shape[0], prim.ctypes.data, uv.ctypes.data, out.ctypes.data)
+        return out
+
+    def texture_lookup(self, tex, uv):
+        """The bilinear lookup of texture `tex` at n (u, v) pairs (n x 3)."""
+        uv = c32(uv).reshape(-1, 2)
+        out = np.zeros((uv.shape[0], 3), np.float32)
+        lib().orc_texture_lookup(self.h, tex, uv.shape[0], uv.ctypes.data, out.ctypes.data)
+        return out
 
     def set_pixel_sample(self, jitter=None, filter_type=0, width=0.0, sigma=0.0, bias=1):
         """Pixel sample of later renders: a fixed jitter (jx, jy), or the pixel filter (0 box,

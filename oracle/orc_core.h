@@ -878,7 +878,45 @@ inline void initStandard(const Luts& L, Bsdf& b, const Material& m, bool fromOut
   b.mixLobes = lobes;
 }
 
-inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, int mediumId, int mediumIntId)
+// baseColor: a textured baseColor parameter's value at the hit, clamp(texture(uv) * (r, g, b))
+// (material/material_param.h:37-41, material/standard.cpp:424), for kinds 0 and 3; null: the
+// record's uniform (r, g, b).
+// ---------------------------------------------------------------------------------------
+// Image textures (image/image_texture.h:148-227): texels decoded to linear RGB, stored bottom
+// row first (image_texture.h:352-353); bilinear lookup with a half-texel offset and repeat or
+// clamp addressing per axis.
+// ---------------------------------------------------------------------------------------
+struct Texture
+{
+  int w = 0, h = 0;
+  bool repeatU = true, repeatV = true;
+  float scale[2] = {1.f, 1.f}, offset[2] = {0.f, 0.f};
+  std::vector<V3> texels;  // [y * w + x], y from the bottom
+};
+
+// clamp(x, lo, hi) = min(select(lo < x, x, lo), hi) (math/math.h:83-86)
+inline float clampSel(float x, float lo, float hi) { return std::min(lo < x ? x : lo, hi); }
+
+inline V3 textureGet(const Texture& t, float u, float v)
+{
+  // uv = ctx.uv * uvScale + uvOffset (image_texture.h:266-269), then get(uv) (:186-227)
+  const float su = u * t.scale[0] + t.offset[0], sv = v * t.scale[1] + t.offset[1];
+  const float x = su - 0.5f / float(t.w), y = sv - 0.5f / float(t.h);
+  const float xr = t.repeatU ? (x - std::floor(x)) * float(t.w) : clampSel(x * float(t.w), 0.f, float(t.w - 1));
+  const float yr = t.repeatV ? (y - std::floor(y)) * float(t.h) : clampSel(y * float(t.h), 0.f, float(t.h - 1));
+  const float sx = xr - std::floor(xr), sy = yr - std::floor(yr);
+  const int x0 = std::min(std::max(int(xr), 0), t.w - 1), y0 = std::min(std::max(int(yr), 0), t.h - 1);
+  const int x1 = t.repeatU ? (x0 + 1 == t.w ? 0 : x0 + 1) : std::min(x0 + 1, t.w - 1);
+  const int y1 = t.repeatV ? (y0 + 1 == t.h ? 0 : y0 + 1) : std::min(y0 + 1, t.h - 1);
+  auto at = [&](int xx, int yy) { return t.texels[size_t(yy) * t.w + xx]; };
+  auto lerp = [](V3 a, V3 b, float s) { return a + s * (b - a); };  // math/math.h:88-92
+  const V3 c0 = lerp(at(x0, y0), at(x1, y0), sx);
+  const V3 c1 = lerp(at(x0, y1), at(x1, y1), sx);
+  return lerp(c0, c1, sy);
+}
+
+inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, int mediumId, int mediumIntId,
+                     const V3* baseColor = nullptr)
 {
   Bsdf b;
   b.kind = m.kind;
@@ -902,14 +940,15 @@ inline Bsdf makeBsdf(const Luts& L, const Material& m, const Ctx& ctx, V3 wo, in
   // are always seen from outside (material/material.h:54-61); thin-walled surfaces too
   // (material/standard.cpp:429)
   const bool backfacing = isMediaInterface(m) && (mediumId == mediumIntId);
+  const V3 base = baseColor ? *baseColor : clamp01(m.r, m.g, m.b);
   if (m.kind == kDiffuse)
-    initDiffuse(b, clamp01(m.r, m.g, m.b), b.diffRough);
+    initDiffuse(b, base, b.diffRough);
   else if (m.kind == kConductor)
     initConductor(L, b, clamp01(m.r, m.g, m.b), clamp01(m.er, m.eg, m.eb), roughness);
   else if (m.kind == kDielectric)  // material/standard.cpp:436-437 (fromOutside = !backfacing)
     initDielectric(L, b, clampf(!backfacing ? 1.0f / foldIor(m.ior) : foldIor(m.ior), 1.f / 3.f, 3.f), roughness);
   else if (m.kind == kCoated)  // material/standard.cpp:512-518, 580-597: never a media interface
-    initCoated(L, b, clamp01(m.r, m.g, m.b), b.diffRough, clamp01(m.er, m.eg, m.eb), std::max(m.thickness, 0.f),
+    initCoated(L, b, base, b.diffRough, clamp01(m.er, m.eg, m.eb), std::max(m.thickness, 0.f),
                clampf(m.weight, 0.f, 1.f), clampf(1.0f / foldIor(m.ior), 1.f / 3.f, 3.f), roughness);
   else if (m.kind == kThinDielectric)
     initThin(L, b, clampf(1.0f / foldIor(m.ior), 1.f / 3.f, 3.f), clamp01(m.r, m.g, m.b), std::max(m.thickness, 0.f),

@@ -49,7 +49,38 @@ struct Scene
   float jitterXY[2] = {0.f, 0.f};
   std::vector<float> filterCdf;
   float filterWidth = 0.f;
+  // image textures: UV set 0 (2 floats per vertex, empty: the hit's barycentrics), textures,
+  // and the baseColor texture of each material (-1 none)
+  std::vector<float> uv;
+  std::vector<Texture> textures;
+  std::vector<int> baseTex;
 };
+
+// A textured baseColor at a hit: clamp(texture(uv) * (r, g, b)) (material/material_param.h:
+// 37-41, material/standard.cpp:424); UV set 0 interpolated barycentrically
+// (geometry/triangle_mesh.cpp:50-53, :83-90), or the hit's (u, v) without texcoords (:131).
+static void hitUV(const Scene& s, int prim, float u, float v, float& tu, float& tv)
+{
+  tu = u;
+  tv = v;
+  if (!s.uv.empty())
+  {
+    const int i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    // b0 uv0 + b1 uv1 + b2 uv2 as the reference's build evaluates it: contracted into two fused
+    // multiply-adds (GCC's default -ffp-contract=fast on the AVX2 + FMA build)
+    const float b1 = u, b2 = v, b0 = 1.f - b1 - b2;
+    tu = std::fma(b2, s.uv[2 * i2], std::fma(b1, s.uv[2 * i1], b0 * s.uv[2 * i0]));
+    tv = std::fma(b2, s.uv[2 * i2 + 1], std::fma(b1, s.uv[2 * i1 + 1], b0 * s.uv[2 * i0 + 1]));
+  }
+}
+
+static V3 texturedBase(const Scene& s, const Material& m, int prim, float u, float v, int tex)
+{
+  float tu, tv;
+  hitUV(s, prim, u, v, tu, tv);
+  const V3 c = textureGet(s.textures[size_t(tex)], tu, tv);
+  return V3(clampSel(c.x * m.r, 0.f, 1.f), clampSel(c.y * m.g, 0.f, 1.f), clampSel(c.z * m.b, 0.f, 1.f));
+}
 
 // The colour of a material's interior medium: pow(clamp(transmissionColor), transmissionDepth)
 // (material/standard.cpp:48-49, :169-173).
@@ -984,7 +1015,10 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
       Ctx ctx = postIntersect(s, org, dir, r.tfar, r.prim, r.u, r.v);
       V3 wo = -dir;
       const int mediumIntId = s.matMedium[matId];
-      Bsdf b = makeBsdf(s.luts, m, ctx, wo, mediumId, mediumIntId);
+      V3 texBase;
+      const bool textured = !s.baseTex.empty() && s.baseTex[size_t(matId)] >= 0;
+      if (textured) texBase = texturedBase(s, m, r.prim, r.u, r.v, s.baseTex[size_t(matId)]);
+      Bsdf b = makeBsdf(s.luts, m, ctx, wo, mediumId, mediumIntId, textured ? &texBase : nullptr);
       // sampler.resetSample with the same index and pixel (:641-642) leaves the state unchanged
       bool mAov1 = !aov1Done, mAov2 = !aov2Done;
       bool mRR = !(mAov1 || mAov2);
@@ -1156,6 +1190,83 @@ void* orc_scene_create(int nverts, const float* pos, const float* nrm, int ntris
 }
 
 void orc_scene_destroy(void* p) { delete (Scene*)p; }
+
+// UV set 0 for the scene's vertices (null: none).
+void orc_scene_set_uvs(void* p, const float* uv, int nverts)
+{
+  Scene& s = *(Scene*)p;
+  if (uv) s.uv.assign(uv, uv + 2 * size_t(nverts));
+  else s.uv.clear();
+}
+
+// Appends a texture: w x h texels as image rows top to bottom, format 0: RGBA8 sRGB (the 2.2
+// gamma table of image/pixel.cpp:21, pow(i / 255, 2.2) in float), 1: RGBA8 linear (/ 255,
+// image/pixel.h:137-145), 2: RGB FP32; address 0 repeat, 1 clamp; uvParams: scale u, v, offset
+// u, v.  Returns its index.
+int orc_scene_add_texture(void* p, int w, int h, int format, int addrU, int addrV, const float* uvParams, const void* texels)
+{
+  Scene& s = *(Scene*)p;
+  Texture t;
+  t.w = w;
+  t.h = h;
+  t.repeatU = addrU == 0;
+  t.repeatV = addrV == 0;
+  t.scale[0] = uvParams[0];
+  t.scale[1] = uvParams[1];
+  t.offset[0] = uvParams[2];
+  t.offset[1] = uvParams[3];
+  t.texels.resize(size_t(w) * h);
+  float gamma[256];
+  for (int i = 0; i < 256; ++i) gamma[i] = std::pow((float)i / 255.0f, 2.2f);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x)
+    {
+      const size_t src = size_t(h - 1 - y) * w + x;  // bottom row first (image_texture.h:352-353)
+      V3 c;
+      if (format == 2)
+      {
+        const float* f = (const float*)texels + 3 * src;
+        c = V3(f[0], f[1], f[2]);
+      }
+      else
+      {
+        const unsigned char* b = (const unsigned char*)texels + 4 * src;
+        c = format == 0 ? V3(gamma[b[0]], gamma[b[1]], gamma[b[2]])
+                        : V3((float)b[0] * (1.f / 255.f), (float)b[1] * (1.f / 255.f), (float)b[2] * (1.f / 255.f));
+      }
+      t.texels[size_t(y) * w + x] = c;
+    }
+  s.textures.push_back(t);
+  return int(s.textures.size()) - 1;
+}
+
+void orc_material_set_texture(void* p, int material, int texture)
+{
+  Scene& s = *(Scene*)p;
+  if (s.baseTex.size() != s.mats.size()) s.baseTex.assign(s.mats.size(), -1);
+  s.baseTex[size_t(material)] = texture;
+}
+
+// UV set 0 at n hits (prim, u, v) into out (n x 2).
+void orc_hit_uv(void* p, int n, const int* prim, const float* uv, float* out)
+{
+  const Scene& s = *(const Scene*)p;
+  for (int i = 0; i < n; ++i) hitUV(s, prim[i], uv[2 * i], uv[2 * i + 1], out[2 * i], out[2 * i + 1]);
+}
+
+// Bilinear lookup of texture `tex` at n (u, v) pairs into out (n x 3): the lookup alone, for
+// the texture tests.
+void orc_texture_lookup(void* p, int tex, int n, const float* uv, float* out)
+{
+  const Scene& s = *(const Scene*)p;
+  for (int i = 0; i < n; ++i)
+  {
+    const V3 c = textureGet(s.textures[size_t(tex)], uv[2 * i], uv[2 * i + 1]);
+    out[3 * i] = c.x;
+    out[3 * i + 1] = c.y;
+    out[3 * i + 2] = c.z;
+  }
+}
 
 float orc_scene_radius(void* p) { return ((Scene*)p)->sceneRadius; }
 

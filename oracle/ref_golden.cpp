@@ -25,6 +25,10 @@
 //             opacity (bsdf/rough_coating.h with a generic base, bsdf/multi_bsdf.h)
 //   postisect geometry/triangle_mesh.cpp:152-257, core/ray_simd.h:63-68,
 //             core/shading_context.h:144-191 (makeFrame)
+//   texcoord  geometry/triangle_mesh.cpp:191-200, :234-237 (UV set 0 at a hit, with and
+//             without texcoords)
+//   pixel     image/pixel.h:80-91, :137-145 (8-bit texel decode: the sRGB table of
+//             image/pixel.cpp:21 and the linear /255)
 
 #include <cstdio>
 #include <cstdint>
@@ -49,6 +53,7 @@
 #include "bsdf/transparent.h"
 #include "bsdf/sheen.h"
 #include "geometry/triangle_mesh.h"
+#include "image/pixel.h"
 
 using namespace prt;
 
@@ -1086,6 +1091,78 @@ static void goldenSheen()
   dump("sheen_transparency", "f4", outTransp, {n, 3});
 }
 
+// UV set 0 at random hits of a mesh with texcoords (and the same hits on one without).
+static void goldenTexcoords()
+{
+  InputRng rng(4242);
+  const int nTri = 32, nHit = 256;
+  std::vector<float> uvIn, hitIn, uvOut, uvOutNone;
+  std::vector<int32_t> primIn;
+  TriangleMesh mesh, bare;
+  mesh.alloc(nTri, nTri * 3, false, 1, false);
+  bare.alloc(nTri, nTri * 3, false, 0, false);
+  for (int t = 0; t < nTri; ++t)
+  {
+    for (int j = 0; j < 3; ++j)
+    {
+      Vec3f p(10.f * rng.f(), 10.f * rng.f(), 10.f * rng.f());
+      mesh.setPosition(t * 3 + j, p);
+      bare.setPosition(t * 3 + j, p);
+      Vec2f uv(4.f * rng.f() - 2.f, 4.f * rng.f() - 2.f);
+      mesh.setTexcoord(0, t * 3 + j, uv);
+      uvIn.insert(uvIn.end(), {uv.x, uv.y});
+    }
+    mesh.indices[t] = Vec3i(t * 3, t * 3 + 1, t * 3 + 2);
+    bare.indices[t] = Vec3i(t * 3, t * 3 + 1, t * 3 + 2);
+  }
+  for (int i = 0; i < nHit; i += simdSize)
+  {
+    RaySimd ray;
+    HitSimd hit;
+    for (int k = 0; k < simdSize; ++k)
+    {
+      int prim = int(rng.u() % nTri);
+      float u = rng.f(), v = rng.f();
+      if (u + v > 1.f) { u = 1.f - u; v = 1.f - v; }
+      ray.org.x[k] = 0.f; ray.org.y[k] = 0.f; ray.org.z[k] = -1.f;
+      ray.dir.x[k] = 0.f; ray.dir.y[k] = 0.f; ray.dir.z[k] = 1.f;
+      ray.near[k] = 0.f; ray.far[k] = 5.f;
+      hit.primId[k] = prim; hit.geomId[k] = 0; hit.instId[k] = -1; hit.u[k] = u; hit.v[k] = v;
+      hitIn.insert(hitIn.end(), {u, v});
+      primIn.push_back(prim);
+    }
+    vbool m = one;
+    ShadingContextSimd* ctx = new ShadingContextSimd();
+    mesh.postIntersect(m, ray, hit, *ctx, 0);
+    for (int k = 0; k < simdSize; ++k) uvOut.insert(uvOut.end(), {ctx->uv[0].x[k], ctx->uv[0].y[k]});
+    delete ctx;
+    ctx = new ShadingContextSimd();
+    bare.postIntersect(m, ray, hit, *ctx, 0);
+    for (int k = 0; k < simdSize; ++k) uvOutNone.insert(uvOutNone.end(), {ctx->uv[0].x[k], ctx->uv[0].y[k]});
+    delete ctx;
+  }
+  dump("texcoord_vertex_uv", "<f4", uvIn, {nTri * 3, 2});
+  dump("texcoord_hit_uv", "<f4", hitIn, {nHit, 2});
+  dump("texcoord_prim", "<i4", primIn, {nHit});
+  dump("texcoord_out", "<f4", uvOut, {nHit, 2});
+  dump("texcoord_out_none", "<f4", uvOutNone, {nHit, 2});
+}
+
+// 8-bit texel decode of every byte value: sRGB (the 2.2-gamma table) and linear.
+static void goldenPixelDecode()
+{
+  std::vector<float> srgb, lin;
+  for (int i = 0; i < 256; ++i)
+  {
+    const int c = (i << 16) | (i << 8) | i | (i << 24);
+    Vec4f a = decodeBgra8(c), b = decodeBgra8Linear(c);
+    srgb.insert(srgb.end(), {a.x, a.y, a.z, a.w});
+    lin.insert(lin.end(), {b.x, b.y, b.z, b.w});
+  }
+  dump("pixel_srgb8", "<f4", srgb, {256, 4});
+  dump("pixel_linear8", "<f4", lin, {256, 4});
+}
+
 int main(int argc, char** argv)
 {
   if (argc < 2) { fprintf(stderr, "usage: %s outdir\n", argv[0]); return 2; }
@@ -1106,6 +1183,8 @@ int main(int argc, char** argv)
   goldenSheen();
   goldenStandard();
   goldenCameraMatrices();
+  goldenTexcoords();
+  goldenPixelDecode();
   fprintf(gManifest, "\n}\n");
   fclose(gManifest);
   return 0;

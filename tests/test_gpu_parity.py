@@ -126,6 +126,7 @@ def _parity(dev, scene, W, H, spp, min_match):
     dev.initFrame(W, H, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
     osc = O.Scene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"], scene.get("normals"))
     osc.set_lights(scene["lights"], scene.get("env"))
+    osc.set_textures(scene)
     acc = None
     for s in range(spp):
         dev.render(1)
@@ -209,6 +210,15 @@ def test_render_parity_standard_materials(dev):
     scene = G.spheres_standard()
     kinds = scene["materials"]["kind"]
     assert (kinds == G.KIND_STANDARD).sum() >= 20 and (kinds == G.KIND_DIELECTRIC).sum() >= 3
+    _parity(dev, scene, 96, 64, 4, 0.998)
+
+
+def test_render_parity_textured_materials(dev):
+    # image textures on base colours (diffuse with Lambert and Oren-Nayar bases, coated diffuse):
+    # per-vertex UVs, an sRGB RGBA8 checker tiled with repeat addressing, an FP32 gradient with
+    # clamp addressing, a linear RGBA8 texture with mixed addressing, UV scales and offsets
+    scene = G.spheres_textured()
+    assert len(scene["material_textures"]) >= 10
     _parity(dev, scene, 96, 64, 4, 0.998)
 
 
