@@ -238,6 +238,28 @@ int prt_material_set_texture(prt_context* ctx, int32_t material, int32_t paramet
  * and is built here; with no lights at all a default ambient light (Ke = 1) is added. */
 int prt_scene_set_lights(prt_context* ctx, int32_t n, const prt_light* lights, int32_t env_w, int32_t env_h,
                          const float* env_rgb, int32_t lut_max);
+/* Two-level scenes (Instance, geometry/instance.h:12-60; Embree instances,
+ * accel/embree_intersector.cpp:100-129).  The geometry's triangles fall into n_meshes meshes,
+ * mesh m being triangles [mesh_begin[m], mesh_begin[m + 1]) (mesh_begin[0] = 0, mesh_begin[n_meshes]
+ * = the triangle count); each of the n instances places one mesh in the world by the affine map
+ * x_world = L x + t given as the rows of [L | t].  Positions, normals and UVs of the geometry are
+ * in the meshes' object space; only instanced meshes are in the world.  prt_bvh_build then builds
+ * one object-space tree per mesh and a top-level tree over the instances; rays enter an instance
+ * through the inverse map (t keeps its meaning), and the hit frame leaves it by the linear part
+ * (tangents) and its inverse transpose (normals), renormalised.  Hits report the scene triangle
+ * (mesh_begin[m] + local index); prt_query_rays_inst also reports the instance (ties in t go to
+ * the smaller instance, then the smaller triangle).  n = 0 returns to a one-level scene; new
+ * geometry drops the instances.  Two-level scenes render with prt_render and answer
+ * prt_query_rays / prt_query_rays_device / prt_query_occlusion of opaque scenes; emissive
+ * geometry must stay one-level (the reference flattens emissive objects, render/scene.cpp:
+ * 343-363), and the batch renderer, the adjoint, the pixel and hit queries, the transparent-
+ * shadow occlusion query and the allocation-free query take one-level scenes. */
+typedef struct prt_instance {
+  int32_t mesh;
+  float xfm[12];  /* rows of [L | t] */
+} prt_instance;
+int prt_scene_set_instances(prt_context* ctx, int32_t n_meshes, const int32_t* mesh_begin, int32_t n,
+                            const prt_instance* instances);
 /* BVH build on the device (replaces the Embree build, accel/embree_intersector.cpp:11-90).
  * build_ms receives the build time (Embree's buildMs stat). */
 int prt_bvh_build(prt_context* ctx, double* build_ms);
@@ -306,6 +328,10 @@ int prt_frame_error_exposure(prt_context* ctx, float exposure, float* error);
 /* --- queries (Renderer::queryRay / queryOcclusionRay, render/renderer.h:47-128) ------- */
 /* rays: n x [ox oy oz dx dy dz near far]; closest hit per ray: prim (-1 = miss), t, u, v. */
 int prt_query_rays(prt_context* ctx, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v);
+/* The same with the hit's instance (-1 for a miss or a one-level scene): the instID of an
+ * Embree hit in a two-level scene. */
+int prt_query_rays_inst(prt_context* ctx, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v,
+                        int32_t* inst);
 /* any hit in [near, far]: occluded[i] = 1 / 0.  In a scene with transparent materials the ray
  * passes them as a shadow ray does (renderer_stream.h:64-97 with the transparent-shadow
  * filter): occluded once the transmittance falls below 1e-5. */

@@ -6,6 +6,7 @@
 // launches the wavefront kernels of render.hip on the context's stream.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -367,6 +368,17 @@ struct prt_context
   bool lightsReady = false;
   BvhData bvh;
   bool bvhReady = false;
+  // two-level scene (geometry/instance.h): meshes are triangle ranges [meshBegin[m],
+  // meshBegin[m + 1]) of the geometry, each with an object-space tree; the instances place them
+  std::vector<int> meshBegin;
+  std::vector<prt_instance> instances;
+  std::vector<BvhData> blas;
+  DevBuf<prt_bvh4_node> tlas;
+  int tlasRoot = 0;
+  DevBuf<prt_instance_dev> instDev;
+  DevBuf<MeshView> meshDev;
+  DevBuf<int> hitInst;
+  bool instanced() const { return !instances.empty(); }
   // renderer
   prt_renderer_params params{100, 8, 1e-4f, 3.402823466e38f};
   bool rendererReady = false;
@@ -468,6 +480,7 @@ int prt_context_destroy(prt_context* c)
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   freeBvh(c->bvh);
+  for (auto& b : c->blas) freeBvh(b);
   if (c->argsHost) hipHostFree(c->argsHost);
   for (auto& ev : c->ev) if (ev) hipEventDestroy(ev);
   for (auto& set : c->evPass)
@@ -550,6 +563,8 @@ int prt_scene_set_geometry(prt_context* c, int32_t nverts, const float* position
   }
   else c->nrm.release();
   c->uv.release();  // UVs belong to the vertices: set them again after new geometry
+  c->meshBegin.clear();  // and so do the meshes and their instances
+  c->instances.clear();
   // scene bounds over the referenced vertices (render/scene.cpp:81, :367)
   float lo[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f}, hi[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
   for (int64_t i = 0; i < int64_t(ntris) * 3; ++i)
@@ -1239,16 +1254,267 @@ int prt_scene_set_lights(prt_context* c, int32_t n, const prt_light* lights, int
   return PRT_OK;
 }
 
+// Top-level tree of a two-level scene, built on the host over the instances' world boxes: a
+// 4-wide tree of nodes in the kernels' layout whose leaves each hold one instance (leaf code
+// instance << 3), split top-down at the median of the longest centroid extent into four.
+struct TlasBox { float lo[3], hi[3]; int inst; };
+static int buildTlasNode(std::vector<prt_bvh4_node>& nodes, std::vector<TlasBox>& boxes, int begin, int end, int depth,
+                         int& maxDepth)
+{
+  maxDepth = std::max(maxDepth, depth + 1);
+  const int me = int(nodes.size());
+  nodes.push_back(prt_bvh4_node{});
+  // up to four groups: halve the range twice along the longest centroid axis
+  int cuts[5] = {begin, begin, begin, begin, end};
+  auto split = [&](int b, int e) {
+    if (e - b < 2) return b + (e - b) / 2;
+    float cl[3] = {3.4e38f, 3.4e38f, 3.4e38f}, ch[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (int i = b; i < e; ++i)
+      for (int k = 0; k < 3; ++k)
+      {
+        const float cc = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+        cl[k] = std::min(cl[k], cc);
+        ch[k] = std::max(ch[k], cc);
+      }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+      if (ch[k] - cl[k] > ch[ax] - cl[ax]) ax = k;
+    const int mid = b + (e - b) / 2;
+    std::nth_element(boxes.begin() + b, boxes.begin() + mid, boxes.begin() + e, [&](const TlasBox& x, const TlasBox& y) {
+      return x.lo[ax] + x.hi[ax] < y.lo[ax] + y.hi[ax];
+    });
+    return mid;
+  };
+  if (end - begin <= 4)
+    for (int q = 0; q < 4; ++q) cuts[q] = std::min(begin + q, end);
+  else
+  {
+    cuts[2] = split(begin, end);
+    cuts[1] = split(begin, cuts[2]);
+    cuts[3] = split(cuts[2], end);
+  }
+  cuts[4] = end;
+  int child[4];
+  float lo[4][3], hi[4][3];
+  for (int q = 0; q < 4; ++q)
+  {
+    const int b = cuts[q], e = q == 3 ? end : cuts[q + 1];
+    for (int k = 0; k < 3; ++k) { lo[q][k] = 3.4028235e38f; hi[q][k] = -3.4028235e38f; }
+    if (e <= b) { child[q] = -1; continue; }
+    for (int i = b; i < e; ++i)
+      for (int k = 0; k < 3; ++k)
+      {
+        lo[q][k] = std::min(lo[q][k], boxes[i].lo[k]);
+        hi[q][k] = std::max(hi[q][k], boxes[i].hi[k]);
+      }
+    child[q] = e - b == 1 ? int(0x80000000u | (uint32_t(boxes[b].inst) << 3)) : buildTlasNode(nodes, boxes, b, e, depth + 1, maxDepth);
+  }
+  prt_bvh4_node& n = nodes[size_t(me)];
+  for (int q = 0; q < 4; ++q)
+  {
+    const bool empty = child[q] == -1;
+    n.lox[q] = empty ? __builtin_inff() : lo[q][0];
+    n.hix[q] = empty ? -__builtin_inff() : hi[q][0];
+    n.loy[q] = empty ? __builtin_inff() : lo[q][1];
+    n.hiy[q] = empty ? -__builtin_inff() : hi[q][1];
+    n.loz[q] = empty ? __builtin_inff() : lo[q][2];
+    n.hiz[q] = empty ? -__builtin_inff() : hi[q][2];
+    n.child[q] = child[q];
+    n.pad[q] = 0;
+  }
+  return me;
+}
+
+// Mesh trees, the top-level tree and the instance records of a two-level scene.
+static int buildTwoLevel(prt_context* c)
+{
+  const int nMesh = int(c->meshBegin.size()) - 1;
+  for (auto& b : c->blas) freeBvh(b);
+  c->blas.assign(size_t(nMesh), BvhData{});
+  double ms = 0.0;
+  int blasDepth = 0;
+  std::string err;
+  for (int m = 0; m < nMesh; ++m)
+  {
+    const int b = c->meshBegin[size_t(m)], n = c->meshBegin[size_t(m) + 1] - b;
+    hipError_t e = buildBvh(c->pos.p, c->idx.p + 3 * size_t(b), n, c->blas[size_t(m)], c->stream, err);
+    if (e != hipSuccess) return fail(PRT_ERR_HIP, err);
+    ms += c->blas[size_t(m)].buildMs;
+    blasDepth = std::max(blasDepth, c->blas[size_t(m)].maxDepth);
+  }
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  // object-space boxes of the meshes from the uploaded vertices
+  std::vector<float> pos(size_t(c->nverts) * 3);
+  std::vector<int> idx(size_t(c->ntris) * 3);
+  if (c->nverts) HIP_TRY(hipMemcpy(pos.data(), c->pos.p, sizeof(float) * pos.size(), hipMemcpyDeviceToHost));
+  if (c->ntris) HIP_TRY(hipMemcpy(idx.data(), c->idx.p, sizeof(int) * idx.size(), hipMemcpyDeviceToHost));
+  std::vector<std::array<double, 6>> mbox(static_cast<size_t>(nMesh));
+  for (int m = 0; m < nMesh; ++m)
+  {
+    auto& bx = mbox[size_t(m)];
+    bx = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int t = c->meshBegin[size_t(m)]; t < c->meshBegin[size_t(m) + 1]; ++t)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+        {
+          const double v = pos[3 * size_t(idx[3 * size_t(t) + j]) + k];
+          bx[k] = std::min(bx[k], v);
+          bx[3 + k] = std::max(bx[3 + k], v);
+        }
+  }
+  // world boxes: the eight corners through x' = L x + t (double), padded outward by a relative
+  // 2^-20, as the builder pads triangle boxes, so rounding never culls an instance
+  std::vector<TlasBox> boxes;
+  double sceneLo[3] = {1e300, 1e300, 1e300}, sceneHi[3] = {-1e300, -1e300, -1e300};
+  std::vector<prt_instance_dev> recs(c->instances.size());
+  for (size_t i = 0; i < c->instances.size(); ++i)
+  {
+    const prt_instance& I = c->instances[i];
+    const auto& bx = mbox[size_t(I.mesh)];
+    prt_instance_dev& r = recs[i];
+    std::memset(&r, 0, sizeof(r));
+    r.mesh = I.mesh;
+    const float* x = I.xfm;  // rows of [L | t]
+    double L[9] = {x[0], x[1], x[2], x[4], x[5], x[6], x[8], x[9], x[10]}, T[3] = {x[3], x[7], x[11]};
+    const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) + L[2] * (L[3] * L[7] - L[4] * L[6]);
+    double A[9] = {(L[4] * L[8] - L[5] * L[7]) / det, (L[2] * L[7] - L[1] * L[8]) / det, (L[1] * L[5] - L[2] * L[4]) / det,
+                   (L[5] * L[6] - L[3] * L[8]) / det, (L[0] * L[8] - L[2] * L[6]) / det, (L[2] * L[3] - L[0] * L[5]) / det,
+                   (L[3] * L[7] - L[4] * L[6]) / det, (L[1] * L[6] - L[0] * L[7]) / det, (L[0] * L[4] - L[1] * L[3]) / det};
+    for (int row = 0; row < 3; ++row)
+    {
+      for (int k = 0; k < 3; ++k)
+      {
+        r.inv[4 * row + k] = float(A[3 * row + k]);
+        r.lin[3 * row + k] = float(L[3 * row + k]);
+        r.nrm[3 * row + k] = float(A[3 * k + row]);  // (L^-1)^T
+      }
+      r.inv[4 * row + 3] = float(-(A[3 * row] * T[0] + A[3 * row + 1] * T[1] + A[3 * row + 2] * T[2]));
+    }
+    if (c->meshBegin[size_t(I.mesh) + 1] == c->meshBegin[size_t(I.mesh)]) continue;  // an empty mesh is never hit
+    TlasBox wb;
+    wb.inst = int(i);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int corner = 0; corner < 8; ++corner)
+    {
+      const double p[3] = {bx[(corner & 1) ? 3 : 0], bx[(corner & 2) ? 4 : 1], bx[(corner & 4) ? 5 : 2]};
+      for (int k = 0; k < 3; ++k)
+      {
+        const double w = L[3 * k] * p[0] + L[3 * k + 1] * p[1] + L[3 * k + 2] * p[2] + T[k];
+        lo[k] = std::min(lo[k], w);
+        hi[k] = std::max(hi[k], w);
+      }
+    }
+    for (int k = 0; k < 3; ++k)
+    {
+      sceneLo[k] = std::min(sceneLo[k], lo[k]);
+      sceneHi[k] = std::max(sceneHi[k], hi[k]);
+      const double pad = std::max(std::fabs(lo[k]), std::fabs(hi[k])) * 9.5367431640625e-7 + 1e-30;
+      wb.lo[k] = std::nextafter(float(lo[k] - pad), -3.4028235e38f);
+      wb.hi[k] = std::nextafter(float(hi[k] + pad), 3.4028235e38f);
+    }
+    boxes.push_back(wb);
+  }
+  std::vector<prt_bvh4_node> nodes;
+  int tlasDepth = 0;
+  if (boxes.size() == 1) c->tlasRoot = int(0x80000000u | (uint32_t(boxes[0].inst) << 3));
+  else if (boxes.empty())
+  {
+    // nothing to hit: a root whose four slots are unused
+    prt_bvh4_node n{};
+    for (int q = 0; q < 4; ++q)
+    {
+      n.lox[q] = n.loy[q] = n.loz[q] = __builtin_inff();
+      n.hix[q] = n.hiy[q] = n.hiz[q] = -__builtin_inff();
+      n.child[q] = -1;
+    }
+    nodes.push_back(n);
+    c->tlasRoot = 0;
+    tlasDepth = 1;
+  }
+  else c->tlasRoot = buildTlasNode(nodes, boxes, 0, int(boxes.size()), 0, tlasDepth);
+  // a wide node pushes at most three children, plus one marker per level change
+  if (3 * (tlasDepth + blasDepth) + 1 >= PRT_MAX_STACK)
+    return fail(PRT_ERR_INVALID, "two-level tree too deep for the traversal stack");
+  HIP_TRY(c->tlas.alloc(std::max<size_t>(nodes.size(), 1)));
+  if (!nodes.empty()) HIP_TRY(hipMemcpy(c->tlas.p, nodes.data(), sizeof(prt_bvh4_node) * nodes.size(), hipMemcpyHostToDevice));
+  HIP_TRY(c->instDev.alloc(recs.size()));
+  HIP_TRY(hipMemcpy(c->instDev.p, recs.data(), sizeof(prt_instance_dev) * recs.size(), hipMemcpyHostToDevice));
+  std::vector<MeshView> mv(static_cast<size_t>(nMesh));
+  for (int m = 0; m < nMesh; ++m)
+    mv[size_t(m)] = MeshView{c->blas[size_t(m)].nodes, c->blas[size_t(m)].tris, c->blas[size_t(m)].root, c->meshBegin[size_t(m)]};
+  HIP_TRY(c->meshDev.alloc(mv.size()));
+  HIP_TRY(hipMemcpy(c->meshDev.p, mv.data(), sizeof(MeshView) * mv.size(), hipMemcpyHostToDevice));
+  // scene bounds and radius (render/scene.cpp:367): the instances' world boxes, the mesh boxes'
+  // corners through their transforms (Instance::getBounds, geometry/instance.h:52-55), in double
+  // rounded to float; the light powers depend on the radius, so set the lights after this build
+  for (int k = 0; k < 3; ++k) { c->boundsLo[k] = 1.f; c->boundsHi[k] = -1.f; }
+  c->sceneRadius = 0.f;
+  if (!boxes.empty())
+  {
+    for (int k = 0; k < 3; ++k)
+    {
+      c->boundsLo[k] = float(sceneLo[k]);
+      c->boundsHi[k] = float(sceneHi[k]);
+    }
+    H3 ext = hmk(c->boundsHi[0] - c->boundsLo[0], c->boundsHi[1] - c->boundsLo[1], c->boundsHi[2] - c->boundsLo[2]);
+    c->sceneRadius = std::sqrt(hdot(ext, ext)) * 0.5f;
+  }
+  c->lightsReady = false;
+  c->bvh.maxDepth = tlasDepth + blasDepth;
+  c->bvh.buildMs = ms + std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+  return PRT_OK;
+}
+
 int prt_bvh_build(prt_context* c, double* build_ms)
 {
   CTX_CHECK(c);
   ++c->generation;
   if (!c->pos.p && c->ntris > 0) return fail(PRT_ERR_STATE, "no geometry");
-  std::string err;
-  hipError_t e = buildBvh(c->pos.p, c->idx.p, c->ntris, c->bvh, c->stream, err);
-  if (e != hipSuccess) return fail(PRT_ERR_HIP, err);
+  c->bvhReady = false;
+  if (c->instanced())
+  {
+    int rc = buildTwoLevel(c);
+    if (rc != PRT_OK) return rc;
+  }
+  else
+  {
+    std::string err;
+    hipError_t e = buildBvh(c->pos.p, c->idx.p, c->ntris, c->bvh, c->stream, err);
+    if (e != hipSuccess) return fail(PRT_ERR_HIP, err);
+  }
   c->bvhReady = true;
   if (build_ms) *build_ms = c->bvh.buildMs;
+  return PRT_OK;
+}
+
+int prt_scene_set_instances(prt_context* c, int32_t n_meshes, const int32_t* mesh_begin, int32_t n, const prt_instance* inst)
+{
+  CTX_CHECK(c);
+  ++c->generation;
+  c->bvhReady = false;
+  if (n == 0)
+  {
+    c->meshBegin.clear();
+    c->instances.clear();
+    return PRT_OK;
+  }
+  if (n < 0 || n_meshes < 1 || !mesh_begin || !inst) return fail(PRT_ERR_INVALID, "invalid instance arrays");
+  if (n >= (1 << 28)) return fail(PRT_ERR_INVALID, "too many instances (limit 2^28)");
+  if (mesh_begin[0] != 0 || mesh_begin[n_meshes] != c->ntris) return fail(PRT_ERR_INVALID, "meshes must cover the triangles in order");
+  for (int m = 0; m < n_meshes; ++m)
+    if (mesh_begin[m + 1] < mesh_begin[m]) return fail(PRT_ERR_INVALID, "mesh ranges must not decrease");
+  for (int i = 0; i < n; ++i)
+  {
+    if (inst[i].mesh < 0 || inst[i].mesh >= n_meshes) return fail(PRT_ERR_INVALID, "instance of an unknown mesh");
+    for (int k = 0; k < 12; ++k)
+      if (!std::isfinite(inst[i].xfm[k])) return fail(PRT_ERR_INVALID, "instance transforms must be finite");
+    const float* x = inst[i].xfm;
+    const double det = double(x[0]) * (double(x[5]) * x[10] - double(x[6]) * x[9]) - double(x[1]) * (double(x[4]) * x[10] - double(x[6]) * x[8]) +
+                       double(x[2]) * (double(x[4]) * x[9] - double(x[5]) * x[8]);
+    if (!(std::fabs(det) > 1e-30)) return fail(PRT_ERR_INVALID, "instance transforms must be invertible");
+  }
+  c->meshBegin.assign(mesh_begin, mesh_begin + n_meshes + 1);
+  c->instances.assign(inst, inst + n);
   return PRT_OK;
 }
 
@@ -1440,6 +1706,11 @@ static RenderArgs makeArgs(prt_context* c)
   a.uv = c->uv.p; a.tex = c->tex.p; a.texels = c->texels.p;
   a.anyTransparent = c->anyTransparent;
   a.bvh.nodes = c->bvh.nodes; a.bvh.tris = c->bvh.tris; a.bvh.root = c->bvh.root;
+  if (c->instanced())
+  {
+    a.inst.nodes = c->tlas.p; a.inst.root = c->tlasRoot; a.inst.inst = c->instDev.p; a.inst.meshes = c->meshDev.p;
+    a.hitInst = c->hitInst.p;
+  }
   a.nLights = int(c->lights.size());
   a.lightOffset = c->areaCount > 0 ? 1 : 0;
   for (int i = 0; i < a.nLights; ++i) a.lights[i] = c->lights[i];
@@ -1489,6 +1760,12 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
   if (!c->camReady) return fail(PRT_ERR_STATE, "camera not set");
   if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
   if (n_spp < 0) return fail(PRT_ERR_INVALID, "negative sample count");
+  if (c->instanced())
+  {
+    // the reference flattens emissive objects (render/scene.cpp:343-363): area lights are one level
+    if (c->areaCount > 0) return fail(PRT_ERR_INVALID, "emissive geometry in a two-level scene: flatten the emissive objects");
+    if (c->hitInst.n < size_t(c->npix)) HIP_TRY(c->hitInst.alloc(size_t(c->npix)));
+  }
   RenderArgs a = makeArgs(c);
   *c->argsHost = a;
   HIP_TRY(hipMemcpyAsync(c->argsDev.p, c->argsHost, sizeof(RenderArgs), hipMemcpyHostToDevice, c->stream));
@@ -1546,12 +1823,13 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
     int depth = 0;
     for (; depth <= a.maxDepth; ++depth)
     {
-      launchIntersect(ad, depth, c->launch, c->countSteps, c->stream);
+      launchIntersect(ad, depth, c->launch, c->countSteps, c->instanced(), c->stream);
       HIP_TRY(hipEventRecord(ev[2 + 3 * depth], c->stream));
-      launchShade(ad, depth, c->sampleIndex, classB, c->launch, c->stdMaterials ? 2 : (c->extMaterials || c->textured ? 1 : 0),
-                  c->stream);
+      launchShade(ad, depth, c->sampleIndex, classB, c->launch,
+                  c->stdMaterials ? 2 : (c->extMaterials || c->textured || c->instanced() ? 1 : 0), c->stream);
       HIP_TRY(hipEventRecord(ev[3 + 3 * depth], c->stream));
-      launchOccluded(ad, depth, classB, c->launch, c->anyTransparent ? (c->stdMaterials ? 2 : 1) : 0, c->countSteps, c->stream);
+      launchOccluded(ad, depth, classB, c->launch, c->anyTransparent ? (c->stdMaterials ? 2 : 1) : 0, c->countSteps,
+                     c->instanced(), c->stream);
       HIP_TRY(hipEventRecord(ev[4 + 3 * depth], c->stream));
       if (depth + 1 >= kFreeDepths && depth < a.maxDepth)
       {
@@ -1639,6 +1917,7 @@ int prt_adjoint(prt_context* c, const float* target, int32_t spp, double* loss, 
   if (c->params.maxDepth > PRT_ADJ_MAX_VERTICES)
     return fail(PRT_ERR_INVALID, "the adjoint supports maxDepth <= " + std::to_string(PRT_ADJ_MAX_VERTICES));
   if (c->textured) return fail(PRT_ERR_INVALID, "the adjoint does not differentiate textured materials");
+  if (c->instanced()) return fail(PRT_ERR_INVALID, "the adjoint takes one-level scenes");
   const int nMat = int(c->materials.size());
   const int nParams = 4 * nMat;
   if (grad_capacity < nParams) return fail(PRT_ERR_INVALID, "grad needs 4 floats per material");
@@ -1833,6 +2112,8 @@ int prt_get_filter_width(prt_context* c, float* width)
   return PRT_OK;
 }
 
+static InstView instView(prt_context* c) { return InstView{c->tlas.p, c->tlasRoot, c->instDev.p, c->meshDev.p}; }
+
 int prt_query_rays_device(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v)
 {
   CTX_CHECK(c);
@@ -1840,8 +2121,12 @@ int prt_query_rays_device(prt_context* c, int32_t n, const float* rays, int32_t*
   if (n < 0 || (n > 0 && (!rays || !prim))) return fail(PRT_ERR_INVALID, "invalid query arrays");
   if (n == 0) return PRT_OK;
   BvhView b{c->bvh.nodes, c->bvh.tris, c->bvh.root};
-  launchQuery(b, n, rays, prim, t, u, v, 0, c->spill.p, c->spillStride, PRT_MAX_STACK - PRT_LDS_STACK, c->launch.gridTrace,
-              c->stream);
+  if (c->instanced())
+    launchQueryInst(instView(c), n, rays, prim, t, u, v, nullptr, 0, c->spill.p, c->spillStride, PRT_MAX_STACK - PRT_LDS_STACK,
+                    c->launch.gridTrace, c->stream);
+  else
+    launchQuery(b, n, rays, prim, t, u, v, 0, c->spill.p, c->spillStride, PRT_MAX_STACK - PRT_LDS_STACK, c->launch.gridTrace,
+                c->stream);
   HIP_TRY(hipGetLastError());
   return PRT_OK;
 }
@@ -1855,6 +2140,7 @@ int prt_query_rays_device_noalloc(prt_context* c, int32_t n, const float* rays, 
   if (3 * c->bvh.maxDepth >= PRT_LDS_STACK + kScratchOverflow)
     return fail(PRT_ERR_INVALID, "BVH of " + std::to_string(c->bvh.maxDepth) + " levels exceeds the static traversal scratch (at most " +
                                      std::to_string((PRT_LDS_STACK + kScratchOverflow - 1) / 3) + " levels)");
+  if (c->instanced()) return fail(PRT_ERR_INVALID, "the allocation-free query takes one-level scenes");
   int* sp = scratchOf(c->device);
   if (!sp) return fail(PRT_ERR_STATE, "static scratch not set up on this device");
   if (n == 0) return PRT_OK;
@@ -1864,7 +2150,8 @@ int prt_query_rays_device_noalloc(prt_context* c, int32_t n, const float* rays, 
   return PRT_OK;
 }
 
-static int queryHost(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v, int anyHit)
+static int queryHost(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v, int anyHit,
+                     int32_t* inst = nullptr)
 {
   CTX_CHECK(c);
   if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
@@ -1879,11 +2166,25 @@ static int queryHost(prt_context* c, int32_t n, const float* rays, int32_t* prim
   HIP_TRY(dv.alloc(n));
   HIP_TRY(hipMemcpy(dr.p, rays, sizeof(float) * 8 * n, hipMemcpyHostToDevice));
   BvhView b{c->bvh.nodes, c->bvh.tris, c->bvh.root};
-  launchQuery(b, n, dr.p, dp.p, dt.p, du.p, dv.p, anyHit, c->spill.p, c->spillStride, PRT_MAX_STACK - PRT_LDS_STACK,
-              c->launch.gridTrace, c->stream);
+  DevBuf<int> di;
+  if (c->instanced())
+  {
+    HIP_TRY(di.alloc(n));
+    launchQueryInst(instView(c), n, dr.p, dp.p, dt.p, du.p, dv.p, di.p, anyHit, c->spill.p, c->spillStride,
+                    PRT_MAX_STACK - PRT_LDS_STACK, c->launch.gridTrace, c->stream);
+  }
+  else
+    launchQuery(b, n, dr.p, dp.p, dt.p, du.p, dv.p, anyHit, c->spill.p, c->spillStride, PRT_MAX_STACK - PRT_LDS_STACK,
+                c->launch.gridTrace, c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(prim, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (inst)
+  {
+    if (di.p) HIP_TRY(hipMemcpy(inst, di.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    else
+      for (int32_t i = 0; i < n; ++i) inst[i] = -1;
+  }
   if (t) HIP_TRY(hipMemcpy(t, dt.p, sizeof(float) * n, hipMemcpyDeviceToHost));
   if (u) HIP_TRY(hipMemcpy(u, du.p, sizeof(float) * n, hipMemcpyDeviceToHost));
   if (v) HIP_TRY(hipMemcpy(v, dv.p, sizeof(float) * n, hipMemcpyDeviceToHost));
@@ -1895,9 +2196,16 @@ int prt_query_rays(prt_context* c, int32_t n, const float* rays, int32_t* prim, 
   return queryHost(c, n, rays, prim, t, u, v, 0);
 }
 
+int prt_query_rays_inst(prt_context* c, int32_t n, const float* rays, int32_t* prim, float* t, float* u, float* v, int32_t* inst)
+{
+  return queryHost(c, n, rays, prim, t, u, v, 0, inst);
+}
+
 int prt_query_occlusion(prt_context* c, int32_t n, const float* rays, int32_t* occluded)
 {
   CTX_CHECK(c);
+  if (c->anyTransparent && c->instanced())
+    return fail(PRT_ERR_INVALID, "transparent-shadow queries take one-level scenes");
   if (c->anyTransparent && c->bvhReady && n > 0 && rays && occluded)
   {
     // the transparent-shadow filter needs the materials: the renderer's argument block
@@ -1960,6 +2268,7 @@ int prt_query_pixel(prt_context* c, float px, float py, int32_t max_hits, prt_pi
   if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
   if (c->W <= 0 || c->H <= 0) return fail(PRT_ERR_STATE, "no frame: the pixel query needs the image size");
   if (max_hits < 1 || max_hits > 32 || !hits || !count) return fail(PRT_ERR_INVALID, "invalid pixel query arguments");
+  if (c->instanced()) return fail(PRT_ERR_INVALID, "the pixel query takes one-level scenes");
   return queryHits(c, px, py, nullptr, max_hits, hits, count);
 }
 
@@ -1969,6 +2278,7 @@ int prt_query_ray_hits(prt_context* c, const float* ray, int32_t max_hits, prt_p
   if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
   if (!c->mats.p) return fail(PRT_ERR_STATE, "materials not set");
   if (!ray || max_hits < 1 || max_hits > 32 || !hits || !count) return fail(PRT_ERR_INVALID, "invalid ray query arguments");
+  if (c->instanced()) return fail(PRT_ERR_INVALID, "the hit query takes one-level scenes");
   return queryHits(c, 0.f, 0.f, ray, max_hits, hits, count);
 }
 
@@ -2240,6 +2550,7 @@ int prt_batch_add_frame(prt_batch* b, prt_context* scene, const prt_camera* cam,
   if (scene->device != b->device) return fail(PRT_ERR_INVALID, "scene context on another device");
   const int rcScene = sceneReady(scene);
   if (rcScene != PRT_OK) return rcScene;
+  if (scene->instanced()) return fail(PRT_ERR_INVALID, "the batch renderer takes one-level scenes");
   if (width <= 0 || height <= 0 || width % 8 != 0 || height % 8 != 0 || width > 4096 || height > 4096)
     return fail(PRT_ERR_INVALID, "batch frame size must be a multiple of 8 and at most 4096");
   if (!(cam->fov > 0.f && cam->fov < 3.14159265f)) return fail(PRT_ERR_INVALID, "invalid fov");

@@ -48,6 +48,9 @@ struct RenderArgs
   const prt_texture_dev* tex;
   const float4* texels;
   BvhView bvh;
+  // two-level scenes: the top-level tree, instances and meshes (inst.nodes null: one level)
+  InstView inst;
+  int* hitInst;  // per path slot: the instance of the closest hit (two-level scenes only)
   int anyTransparent;  // some material lets shadow rays through (render/scene.cpp:452-462)
   prt_light_dev lights[PRT_MAX_LIGHTS];  // environment lights
   float lightCdf[PRT_MAX_LIGHTS + 2];    // power distribution over [area light,] environment lights
@@ -132,16 +135,20 @@ struct LaunchCfg
 // The kernels read RenderArgs from device memory (one copy per render call); passing the
 // ~1 KB record by value would make every lane copy it to scratch.
 void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStream_t s);
-void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool count, hipStream_t s);
+void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool count, bool inst, hipStream_t s);
 // level: 0 the original lobes, 1 the extended lobes and coloured media, 2 kind-5 materials too;
 // filter: 0 opaque scene, 1 transparent materials, 2 transparent materials and kind-5 materials
 void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, int level, hipStream_t s);
-void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, int filter, bool count, hipStream_t s);
+void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, int filter, bool count, bool inst,
+                    hipStream_t s);
 void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s);
 void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, const float* ray8,
                       hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
                  int* spill, int spillStride, int spillCap, int grid, hipStream_t s);
+// The same on a two-level scene; inst (may be null) receives the hit's instance.
+void launchQueryInst(const InstView& iv, int n, const float* rays, int* prim, float* t, float* u, float* v, int* inst,
+                     int anyHit, int* spill, int spillStride, int spillCap, int grid, hipStream_t s);
 // Path-replay adjoint (adjoint.hip): partial is [blocks][nParams] (zeroed by the caller before the
 // first sample pass), grad [nParams]; delta is the per-pixel dLoss/dI (planar rgb).
 int adjointBlocks(int npix, int spillStride);

@@ -161,7 +161,11 @@ PRT_DEV void applyBaseTexture(const RenderArgs& a, const prt_material_constants&
     b.importance = imp;
 }
 
-PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, float u, float v)
+// inst >= 0: the hit is on that instance of its mesh: the mesh's post-intersection runs in
+// object space, then the frame's tangents go back to world space by the linear part and the
+// normals by its inverse transpose, each renormalised (geometry/instance.h:17-26); the hit
+// point is the world ray's.
+PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, float u, float v, int inst = -1)
 {
   Ctx c;
   c.p = o + t * d;
@@ -183,6 +187,14 @@ PRT_DEV Ctx postIntersect(const RenderArgs& a, f3 o, f3 d, float t, int prim, fl
     }
   }
   makeFrame(c.f.U, c.f.V, N);
+  if (inst >= 0)
+  {
+    const prt_instance_dev& I = a.inst.inst[inst];
+    c.f.U = normalize(xfmRows(I.lin, c.f.U));
+    c.f.V = normalize(xfmRows(I.lin, c.f.V));
+    N = normalize(xfmRows(I.nrm, N));
+    Ng = normalize(xfmRows(I.nrm, Ng));
+  }
   if (dot(Ng, d) > 0.f) { Ng = -Ng; N = -N; }
   c.Ng = Ng;
   c.f.N = N;
@@ -200,12 +212,12 @@ struct ShadowFilterT
   const RenderArgs* a;
   f3 o, d, T;
   uint32_t crossed;  // transparent surfaces passed (part of the path's identity in debug hashes)
-  PRT_DEV bool operator()(int prim, float t, float u, float v)
+  PRT_DEV bool operator()(int prim, float t, float u, float v, int inst = -1)
   {
     const prt_material_constants& m = a->mats[a->triMat[prim]];
     if (!m.transparent) { T = f3(0.f); return true; }
     ++crossed;
-    Ctx c = postIntersect(*a, o, d, t, prim, u, v);
+    Ctx c = postIntersect(*a, o, d, t, prim, u, v, inst);
     Bsdf b = makeBsdf(m, c, -d, false, true, kStd);
     T = T * bsdfTransparency(a->lut, b, -d);
     return reduceMax(T) < 1e-5f;

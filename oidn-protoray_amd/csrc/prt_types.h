@@ -116,6 +116,19 @@ struct prt_texture_dev
   int32_t pad;
 };
 
+// An instance of a mesh (geometry/instance.h:12-60) as the kernels read it: the world -> object
+// affine map that takes rays into the mesh's space (Embree's instance traversal), the object ->
+// world linear part that takes the frame's tangents back (xfmVector) and its inverse transpose
+// that takes the normals back (xfmNormal), all as rows.  128 bytes.
+struct prt_instance_dev
+{
+  float inv[12];  // x_obj = inv[0..2] . x + inv[3], ... (rows of [A | b])
+  float lin[9];   // rows of L, x_world = L x_obj + t
+  float nrm[9];   // rows of L^-T
+  int32_t mesh;
+  int32_t pad[1];
+};
+
 // Environment light record as the kernels read it.  64 bytes.
 struct prt_light_dev
 {

@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256) kGenerate(const RenderArgs* __restrict__ 
 // Closest hits of the queued paths of depth `depth`.  kCount: also count the traversal loop
 // iterations per lane and per wave (the SIMD utilisation), in a separate build that profiling
 // passes use, so the timed kernel keeps its registers and occupancy.
-template <bool kCount>
+template <bool kCount, bool kInst>
 __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __restrict__ ap, int depth)
 {
   const RenderArgs& a = *ap;
@@ -136,7 +136,14 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __re
     int slot = depth == 0 ? i : q[i];
     float4 ro = a.rayO[slot], rd = a.rayD[slot];
     float t, u = 0.f, v = 0.f;
-    int prim = traceRay<false>(a.bvh, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, st, nodes, tris);
+    int prim;
+    if (kInst)
+    {
+      int inst = -1;
+      prim = traceRayInst<false>(a.inst, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, inst, st, nodes, tris);
+      a.hitInst[slot] = inst;
+    }
+    else prim = traceRay<false>(a.bvh, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, st, nodes, tris);
     a.hit[slot] = make_float4(t, u, v, __int_as_float(prim));
     ++rays;
   };
@@ -215,6 +222,9 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
       int fl = a.flags[slot];
       int lastType = fl & PRT_F_LASTTYPE_MASK;
       uint32_t hsh = a.hash ? hashMix(a.hash[slot], uint32_t(prim + 1)) : 0u;
+      // two-level scenes: the hit's instance (part of the path's identity too)
+      const int inst = (kLevel >= 1 && a.hitInst && prim >= 0) ? a.hitInst[slot] : -1;
+      if (a.hash && inst >= 0) hsh = hashMix(hsh, 0x40000000u | uint32_t(inst));
       if (depth == 0 && a.fb.dbgFirst) a.fb.dbgFirst[pix] = prim;
       int matId = prim >= 0 ? a.triMat[prim] : 0;
       fl &= ~(PRT_F_DIRECT | PRT_F_TERMINATE);
@@ -272,7 +282,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
       else
       {
         const prt_material_constants& m = a.mats[matId];
-        Ctx ctx = postIntersect(a, o, d, h.x, prim, h.y, h.z);
+        Ctx ctx = postIntersect(a, o, d, h.x, prim, h.y, h.z, inst);
         f3 wo = -d;
         int mediumId = fl >> PRT_F_MEDIUM_SHIFT;
         bool inside = m.mediaInterface && (mediumId == m.mediumInt);  // material/material.h:54-61
@@ -387,8 +397,8 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
 }
 
 // One shadow ray of the queue (path_renderer_stream.h:951-1065).  kFilter: 0 opaque scene, 1
-// transparent materials, 2 transparent materials and kind-5 materials.
-template <int kFilter>
+// transparent materials, 2 transparent materials and kind-5 materials; kInst: a two-level scene.
+template <int kFilter, bool kInst>
 PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& nodes, uint32_t& tris, uint32_t& rays, bool classB)
 {
   {
@@ -399,14 +409,18 @@ PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& n
     int prim;
     f3 Tsh(1.f);
     uint32_t crossed = 0;
+    int inst;
     if (kFilter)
     {
       ShadowFilterT<(kFilter >= 2)> flt{&a, o, d, f3(1.f), 0u};
-      prim = traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris, flt);
+      prim = kInst ? traceRayInst<true>(a.inst, o, d, so.w, sd.w, t, u, v, inst, st, nodes, tris, flt)
+                   : traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris, flt);
       Tsh = flt.T;
       crossed = flt.crossed;
     }
-    else prim = traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris);
+    else
+      prim = kInst ? traceRayInst<true>(a.inst, o, d, so.w, sd.w, t, u, v, inst, st, nodes, tris)
+                   : traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris);
     ++rays;
     float4 lr = a.Lr[slot];
     f3 L(lr.x, lr.y, lr.z);
@@ -430,7 +444,7 @@ PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& n
 // rays gather the transmittance of the transparent surfaces they cross (path_renderer_stream.h:
 // 951-1002 with the TransparentShadowFilter of render/transparent_filter.h); 2: some of them are
 // kind-5 materials.
-template <int kFilter, bool kCount>
+template <int kFilter, bool kCount, bool kInst>
 __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth, bool classB)
 {
   const RenderArgs& a = *ap;
@@ -441,12 +455,12 @@ __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __res
   uint32_t nodes = 0, tris = 0, rays = 0, waveSteps = 0;
   const int stride = gridDim.x * blockDim.x;
   if (!kCount)
-    for (int i = gtid; i < count; i += stride) occludedRay<kFilter>(a, i, st, nodes, tris, rays, classB);
+    for (int i = gtid; i < count; i += stride) occludedRay<kFilter, kInst>(a, i, st, nodes, tris, rays, classB);
   else
     for (int base = gtid - laneId(); base < count; base += stride)
     {
       const uint32_t before = st.steps;
-      if (base + laneId() < count) occludedRay<kFilter>(a, base + laneId(), st, nodes, tris, rays, classB);
+      if (base + laneId() < count) occludedRay<kFilter, kInst>(a, base + laneId(), st, nodes, tris, rays, classB);
       waveSteps += waveMax(st.steps - before);
     }
   unsigned long long r = waveSum(rays), nv = waveSum(nodes), tt = waveSum(tris);
@@ -486,6 +500,32 @@ __global__ void __launch_bounds__(kTraceBlock) kQuery(BvhView bvh, int n, const 
     if (tOut) tOut[i] = p >= 0 ? t : r[7];
     if (uOut) uOut[i] = u;
     if (vOut) vOut[i] = v;
+  }
+}
+
+// kQuery on a two-level scene; inst (may be null) receives the hit's instance.
+__global__ void __launch_bounds__(kTraceBlock) kQueryInst(InstView iv, int n, const float* __restrict__ rays, int* __restrict__ prim,
+                                                          float* __restrict__ tOut, float* __restrict__ uOut,
+                                                          float* __restrict__ vOut, int* __restrict__ instOut, int anyHit,
+                                                          int* spill, int spillStride, int spillCap)
+{
+  __shared__ TraceLds<kTraceBlock> lds;
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  TraceStack st = makeStack(lds, spill, spillStride, gtid, spillCap);
+  uint32_t nodes = 0, tris = 0;
+  for (int i = gtid; i < n; i += gridDim.x * blockDim.x)
+  {
+    const float* r = rays + 8 * i;
+    f3 o(r[0], r[1], r[2]), d(r[3], r[4], r[5]);
+    float t, u = 0.f, v = 0.f;
+    int inst = -1;
+    int p = anyHit ? traceRayInst<true>(iv, o, d, r[6], r[7], t, u, v, inst, st, nodes, tris)
+                   : traceRayInst<false>(iv, o, d, r[6], r[7], t, u, v, inst, st, nodes, tris);
+    prim[i] = p;
+    if (tOut) tOut[i] = p >= 0 ? t : r[7];
+    if (uOut) uOut[i] = u;
+    if (vOut) vOut[i] = v;
+    if (instOut) instOut[i] = p >= 0 ? inst : -1;
   }
 }
 
@@ -594,10 +634,15 @@ void launchGenerate(const RenderArgs* a, int npix, uint32_t sampleIndex, hipStre
 {
   kGenerate<<<(npix + 255) / 256, 256, 0, s>>>(a, sampleIndex);
 }
-void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool count, hipStream_t s)
+void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool count, bool inst, hipStream_t s)
 {
-  if (count) kIntersect<true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
-  else kIntersect<false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  if (inst)
+  {
+    if (count) kIntersect<true, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+    else kIntersect<false, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  }
+  else if (count) kIntersect<true, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
+  else kIntersect<false, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth);
 }
 void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, int level, hipStream_t s)
 {
@@ -605,22 +650,35 @@ void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool clas
   else if (level == 1) kShade<1><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
   else kShade<0><<<c.gridShade, kShadeBlock, 0, s>>>(a, depth, sampleIndex, classB);
 }
-void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, int filter, bool count, hipStream_t s)
+void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, int filter, bool count, bool inst,
+                    hipStream_t s)
 {
+  if (inst)  // two-level scenes: no counting build
+  {
+    if (filter >= 2) kOccluded<2, false, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    else if (filter == 1) kOccluded<1, false, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    else kOccluded<0, false, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    return;
+  }
   if (count)
   {
-    if (filter >= 2) kOccluded<2, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
-    else if (filter == 1) kOccluded<1, true><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
-    else kOccluded<0, true><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
+    if (filter >= 2) kOccluded<2, true, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    else if (filter == 1) kOccluded<1, true, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+    else kOccluded<0, true, false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
   }
-  else if (filter >= 2) kOccluded<2, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
-  else if (filter == 1) kOccluded<1, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
-  else kOccluded<0, false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
+  else if (filter >= 2) kOccluded<2, false, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+  else if (filter == 1) kOccluded<1, false, false><<<c.gridTrace, kTraceBlock, 0, s>>>(a, depth, classB);
+  else kOccluded<0, false, false><<<c.gridOccluded, kTraceBlock, 0, s>>>(a, depth, classB);
 }
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit, int* spill,
                  int spillStride, int spillCap, int grid, hipStream_t s)
 {
   kQuery<<<grid, kTraceBlock, 0, s>>>(bvh, n, rays, prim, t, u, v, anyHit, spill, spillStride, spillCap);
+}
+void launchQueryInst(const InstView& iv, int n, const float* rays, int* prim, float* t, float* u, float* v, int* inst,
+                     int anyHit, int* spill, int spillStride, int spillCap, int grid, hipStream_t s)
+{
+  kQueryInst<<<grid, kTraceBlock, 0, s>>>(iv, n, rays, prim, t, u, v, inst, anyHit, spill, spillStride, spillCap);
 }
 void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s)
 {

@@ -129,8 +129,45 @@ PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int 
 // Any-hit filter that accepts every hit (a plain occlusion test).
 struct AcceptHit
 {
-  PRT_DEV bool operator()(int, float, float, float) const { return true; }
+  PRT_DEV bool operator()(int, float, float, float, int = -1) const { return true; }
 };
+
+// A two-level scene (geometry/instance.h, accel/embree_intersector.cpp:100-129): a top-level
+// tree over the instances' world boxes whose leaves each hold one instance, and per mesh its
+// object-space tree.  A mesh is a contiguous range of the scene's triangles: its trees report
+// triangle triBegin + local index.
+struct MeshView
+{
+  const prt_bvh4_node* nodes;
+  const float4* tris;
+  int root;
+  int triBegin;
+};
+struct InstView
+{
+  const prt_bvh4_node* nodes;  // top level
+  int root;
+  const prt_instance_dev* inst;
+  const MeshView* meshes;
+};
+
+// The world -> object map of an instance, written out in one order (no contraction in the
+// path kernels) so the CPU oracle transforms rays bit for bit the same.
+PRT_DEV f3 xfmPointInv(const prt_instance_dev& I, f3 p)
+{
+  return f3(I.inv[0] * p.x + I.inv[1] * p.y + I.inv[2] * p.z + I.inv[3],
+            I.inv[4] * p.x + I.inv[5] * p.y + I.inv[6] * p.z + I.inv[7],
+            I.inv[8] * p.x + I.inv[9] * p.y + I.inv[10] * p.z + I.inv[11]);
+}
+PRT_DEV f3 xfmVectorInv(const prt_instance_dev& I, f3 v)
+{
+  return f3(I.inv[0] * v.x + I.inv[1] * v.y + I.inv[2] * v.z, I.inv[4] * v.x + I.inv[5] * v.y + I.inv[6] * v.z,
+            I.inv[8] * v.x + I.inv[9] * v.y + I.inv[10] * v.z);
+}
+PRT_DEV f3 xfmRows(const float* m, f3 v)
+{
+  return f3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
 
 // Closest hit (kAnyHit = false) or any hit.  Ties in t go to the smaller primitive ID, so the
 // result does not depend on the traversal order.  Returns the primitive ID or -1; nodeVisits
@@ -201,6 +238,107 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
     do
     {
       if (st.sp == 0) { tHit = tfar; return best; }
+      node = st.pop(tEntry);
+    } while (!kAnyHit && tEntry > tfar);
+  }
+}
+
+// Stack marker of the two-level traversal: leave the current instance (restore the world ray).
+// Inner-node indices stay below it; it is pushed with entry distance -inf, so the closest-hit
+// skip never drops it.
+constexpr int kLeaveInstance = 0x7ffffffe;
+
+// Closest or any hit in a two-level scene.  At a top-level leaf the ray moves into the
+// instance's object space (the parameter t keeps its meaning, the direction is not
+// renormalised, as in Embree's instance traversal) and the mesh's tree is traversed with the
+// same stack above a marker; popping the marker restores the world ray.  Ties in t go to the
+// smaller (instance, triangle) pair.  Returns the triangle (scene index) or -1, and its instance.
+template <bool kAnyHit, class Filter = AcceptHit>
+PRT_DEV int traceRayInst(const InstView& iv, f3 wo, f3 wd, float tnear, float tfar, float& tHit, float& uHit, float& vHit,
+                         int& instHit, TraceStack& st, uint32_t& nodeVisits, uint32_t& triTests, Filter&& filter = Filter())
+{
+  const f3 wid(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+  f3 o = wo, d = wd, id = wid;
+  int best = -1, bestInst = -1;
+  st.sp = 0;
+  int node = iv.root;
+  const prt_bvh4_node* nodes = iv.nodes;
+  const float4* tris = nullptr;
+  int inst = -1, triBegin = 0;
+  const float inf = __int_as_float(0x7f800000);
+  for (;;)
+  {
+    ++st.steps;
+    if (node == kLeaveInstance)
+    {
+      o = wo; d = wd; id = wid;
+      nodes = iv.nodes;
+      inst = -1;
+    }
+    else if (node >= 0)
+    {
+      float t[4];
+      int c[4];
+      ++nodeVisits;
+      boxQuad(nodes, node, o, id, tnear, tfar, t, c);
+      cswap(t[0], c[0], t[1], c[1]);
+      cswap(t[2], c[2], t[3], c[3]);
+      cswap(t[0], c[0], t[2], c[2]);
+      cswap(t[1], c[1], t[3], c[3]);
+      cswap(t[1], c[1], t[2], c[2]);
+      if (t[0] != inf)
+      {
+        if (t[3] != inf) st.push(c[3], t[3]);
+        if (t[2] != inf) st.push(c[2], t[2]);
+        if (t[1] != inf) st.push(c[1], t[1]);
+        node = c[0];
+        continue;
+      }
+    }
+    else if (inst < 0)  // a top-level leaf: enter its instance
+    {
+      inst = (node & 0x7fffffff) >> 3;
+      const prt_instance_dev& I = iv.inst[inst];
+      const MeshView& M = iv.meshes[I.mesh];
+      st.push(kLeaveInstance, -inf);
+      o = xfmPointInv(I, wo);
+      d = xfmVectorInv(I, wd);
+      id = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+      nodes = M.nodes;
+      tris = M.tris;
+      triBegin = M.triBegin;
+      node = M.root;
+      continue;
+    }
+    else
+    {
+      int code = node & 0x7fffffff;
+      int first = code >> 3, count = (code & 7) + 1;
+      for (int k = 0; k < count; ++k)
+      {
+        const float4* tp = tris + 3 * (first + k);
+        float4 a = tp[0], b = tp[1], cc = tp[2];
+        float th, u, v;
+        ++triTests;
+        if (intersectTriangle(a, b, cc, o, d, tnear, tfar, th, u, v))
+        {
+          int prim = triBegin + __float_as_int(a.w);
+          if (kAnyHit)
+          {
+            if (filter(prim, th, u, v, inst)) { tHit = th; instHit = inst; return prim; }
+            continue;
+          }
+          if (th < tfar || best < 0 || inst < bestInst || (inst == bestInst && prim < best))
+          {
+            tfar = th; best = prim; bestInst = inst; uHit = u; vHit = v;
+          }
+        }
+      }
+    }
+    float tEntry;
+    do
+    {
+      if (st.sp == 0) { tHit = tfar; instHit = bestInst; return best; }
       node = st.pop(tEntry);
     } while (!kAnyHit && tEntry > tfar);
   }

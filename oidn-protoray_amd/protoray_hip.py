@@ -23,6 +23,7 @@ FILTER_BOX, FILTER_TRIANGLE, FILTER_GAUSSIAN, FILTER_BLACKMAN_HARRIS = 0, 1, 2, 
 BUF_COLOR, BUF_ALBEDO, BUF_ALBEDO1, BUF_NORMAL, BUF_NORMAL1, BUF_DEPTH = range(6)
 BATCH_COLOR, BATCH_ALBEDO1, BATCH_NORMAL1, BATCH_DEPTH, BATCH_SNAPSHOT = 0, 1, 2, 3, 16
 DENOISE_ALBEDO, DENOISE_NORMAL = 1, 2
+INSTANCE_DTYPE = np.dtype([("mesh", "<i4"), ("xfm", "<f4", 12)])  # prt_instance
 TEX_RGBA8_SRGB, TEX_RGBA8_LINEAR, TEX_RGB32F = 0, 1, 2
 TEX_REPEAT, TEX_CLAMP = 0, 1
 
@@ -121,6 +122,8 @@ SIGNATURES = {
     "prt_denoiser_gradient": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, ctypes.c_int64],
     "prt_denoiser_execute_frame": [_vp, _vp, _i, _f, _vp],
     "prt_scene_set_uvs": [_vp, _vp, _i],
+    "prt_scene_set_instances": [_vp, _i, _vp, _i, _vp],
+    "prt_query_rays_inst": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "prt_scene_set_textures": [_vp, _i, _vp],
     "prt_material_set_texture": [_vp, _i, _i, _i],
     "prt_denoiser_execute_batch": [_vp, _vp, _i, _i, _i, _f, _vp],
@@ -236,6 +239,33 @@ class DeviceHip:
     def initCamera(self, camera):
         c = np.ascontiguousarray(camera, CAMERA_DTYPE)
         _check(_lib.prt_set_camera(self.h, _ptr(c)))
+
+    def initInstances(self, mesh_begin, instances):
+        """Two-level scene: meshes as triangle ranges (mesh_begin, n_meshes + 1 entries) placed by
+        instances [(mesh, 3 x 4 [L | t]), ...]; rebuilds the trees (set the lights afterwards)."""
+        if not instances:
+            _check(_lib.prt_scene_set_instances(self.h, 0, None, 0, None))
+        else:
+            mb = np.ascontiguousarray(mesh_begin, np.int32)
+            rec = np.zeros(len(instances), INSTANCE_DTYPE)
+            for k, (mesh, xfm) in enumerate(instances):
+                rec[k]["mesh"] = mesh
+                rec[k]["xfm"] = np.asarray(xfm, np.float32).reshape(12)
+            self._inst = (mb, rec)
+            _check(_lib.prt_scene_set_instances(self.h, mb.shape[0] - 1, _ptr(mb), rec.shape[0], _ptr(rec)))
+        ms = ctypes.c_double(0)
+        _check(_lib.prt_bvh_build(self.h, ctypes.byref(ms)))
+        self.build_ms = ms.value
+        return self.build_ms
+
+    def queryRaysInst(self, rays):
+        """Closest hits with their instances: (prim, t, u, v, inst)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        prim, inst = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        t, u, v = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        _check(_lib.prt_query_rays_inst(self.h, n, _ptr(rays), _ptr(prim), _ptr(t), _ptr(u), _ptr(v), _ptr(inst)))
+        return prim, t, u, v, inst
 
     def initUVs(self, uv):
         """Texture coordinate set 0 (nverts x 2), or None to drop it."""
@@ -512,6 +542,8 @@ def load_scene(dev, scene):
         dev.initTextures(scene["textures"])
     for mat, tex in scene.get("material_textures", ()):
         dev.setMaterialTexture(mat, tex)
+    if scene.get("instances"):
+        dev.initInstances(scene["mesh_begin"], scene["instances"])
     dev.initLights(scene["lights"], scene.get("env"))
     dev.initRenderer(maxDepth=scene["max_depth"], rrDepth=scene.get("rr_depth", 8))
     dev.initSampler(scene.get("seed", 0))

@@ -357,6 +357,66 @@ def spheres_textured(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4,
                 max_depth=max_depth, seed=0)
 
 
+def _rotation(rng):
+    """A random rotation (unit quaternion from three uniforms, Shoemake 1992) as a 3 x 3 matrix."""
+    u1, u2, u3 = rng.f(), rng.f(), rng.f()
+    a, b = math.sqrt(1 - u1), math.sqrt(u1)
+    x, y, z, w = a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2), b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def spheres_instanced(seed=0, n_instances=40, subdiv=2, width=96, height=64, spp=4, max_depth=8):
+    """A two-level scene: meshes in object space (the ground quad and six unit icospheres with
+    smooth vertex normals, each with its own material: diffuse, Oren-Nayar diffuse with a base
+    colour texture, coated, rough conductor, rough dielectric (a media interface), thin rough
+    dielectric (transparent to shadow rays)), placed by instances: the ground once (identity),
+    the spheres n_instances times with random rotations, non-uniform scales (0.3 to 1.2 per axis)
+    and translations over the sun/sky spheres' volume.  Lit by the sun/sky pair and a distant
+    light."""
+    rng = Pcg32(seed)
+    Vs, Fs = icosphere(subdiv)
+    mats = [_material(KIND_DIFFUSE, (0.8, 0.8, 0.8)), _material(KIND_DIFFUSE, (0.5, 0.5, 0.5))]
+    r, g = gold_artistic(rng)
+    mats += [_material(KIND_DIFFUSE, (0.7, 0.3, 0.2)),
+             _material(KIND_DIFFUSE, (1.0, 1.0, 1.0), diffuse_roughness=0.5),
+             _material(KIND_COATED, (0.2, 0.5, 0.7), (1, 1, 1), 0.3, 1.5, thickness=1.0, weight=1.0),
+             _material(KIND_CONDUCTOR, r, g, 0.35),
+             _material(KIND_DIELECTRIC, roughness=0.2, ior=1.5),
+             _material(KIND_THIN_DIELECTRIC, (0.8, 0.9, 0.7), roughness=0.25, ior=1.4, thickness=0.5)]
+    gv = np.array([[-20, -1, -2.5], [20, -1, -2.5], [20, -1, 37.5], [-20, -1, 37.5]], np.float64)
+    pos = [gv] + [Vs] * 6
+    nrm = [np.tile([0.0, 1.0, 0.0], (4, 1))] + [Vs] * 6
+    idx, tm, begin = [np.array([[0, 2, 1], [0, 3, 2]])], [np.ones(2, np.int64)], [0, 2]
+    off = 4
+    for k in range(6):
+        idx.append(Fs + off)
+        tm.append(np.full(Fs.shape[0], 2 + k))
+        off += Vs.shape[0]
+        begin.append(begin[-1] + Fs.shape[0])
+    uv_s = np.stack([0.5 + np.arctan2(Vs[:, 2], Vs[:, 0]) / (2 * math.pi), 0.5 + np.arcsin(np.clip(Vs[:, 1], -1, 1)) / math.pi], axis=1)
+    uvs = np.concatenate([(gv[:, [0, 2]] + [20.0, 2.5]) / 40.0] + [uv_s] * 6).astype(np.float32)
+    instances = [(0, np.hstack([np.eye(3), np.zeros((3, 1))]))]
+    for _ in range(n_instances):
+        mesh = 1 + rng.u32() % 6
+        R = _rotation(rng)
+        S = np.diag([rng.f(0.3, 1.2), rng.f(0.3, 1.2), rng.f(0.3, 1.2)])
+        t = np.array([rng.f(-8, 8), rng.f(0, 6), rng.f(6, 28)])
+        instances.append((mesh, np.hstack([R @ S, t[:, None]])))
+    checker = np.zeros((4, 4, 4), np.uint8)
+    for y in range(4):
+        for x in range(4):
+            checker[y, x] = (230, 200, 60, 255) if (x + y) % 2 == 0 else (40, 90, 200, 255)
+    sc = spheres_sunsky(n_spheres=0, width=width, height=height, spp=spp, max_depth=max_depth, palette=1)
+    return dict(name="spheres_instanced", positions=np.concatenate(pos).astype(np.float32),
+                normals=np.concatenate(nrm).astype(np.float32), indices=np.concatenate(idx).astype(np.int32),
+                tri_material=np.concatenate(tm).astype(np.int32), materials=np.array(mats, MATERIAL_DTYPE),
+                uvs=uvs, textures=[dict(texels=checker, format=0, address=(0, 0), uv_scale=(3.0, 2.0))],
+                material_textures=[(3, 0)], mesh_begin=begin, instances=instances, lights=sc["lights"], env=sc["env"],
+                camera=_camera(lens_radius=0.05, focal=15.0), width=width, height=height, spp=spp, max_depth=max_depth, seed=0)
+
+
 def spheres_lobes(seed=0, n_spheres=48, subdiv=2, width=96, height=64, spp=4, max_depth=8, palette=24):
     """The sun/sky scene of spheres_sunsky with the remaining StandardMaterial lobes: rough
     diffuse (Oren-Nayar) bases, coats over them, thin rough dielectrics (thin-walled bubbles with

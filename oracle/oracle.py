@@ -84,6 +84,8 @@ def lib():
         L.orc_material_set_texture.argtypes = [vp, i, i]
         L.orc_texture_lookup.argtypes = [vp, i, i, vp, vp]
         L.orc_hit_uv.argtypes = [vp, i, vp, vp, vp]
+        L.orc_scene_set_instances.argtypes = [vp, i, vp, i, vp]
+        L.orc_intersect_inst.argtypes = [vp, i, vp, i, vp, vp, vp, vp, vp]
         L.orc_scene_radius.restype = f
         L.orc_scene_radius.argtypes = [vp]
         L.orc_scene_set_lights.restype = i
@@ -277,6 +279,29 @@ class Scene:
             self.add_texture(t)
         for mat, tex in scene.get("material_textures", ()):
             lib().orc_material_set_texture(self.h, int(mat), int(tex))
+        if scene.get("instances"):
+            self.set_instances(scene["mesh_begin"], scene["instances"])
+
+    def set_instances(self, mesh_begin, instances):
+        """Two-level scene: meshes as triangle ranges, instances [(mesh, 3 x 4 [L | t]), ...]
+        (empty: one level).  Set the lights afterwards (their powers use the scene radius)."""
+        mb = i32(mesh_begin)
+        rec = np.zeros(len(instances), np.dtype([("mesh", "<i4"), ("xfm", "<f4", 12)]))
+        for k, (mesh, xfm) in enumerate(instances):
+            rec[k]["mesh"] = mesh
+            rec[k]["xfm"] = np.asarray(xfm, np.float32).reshape(12)
+        self._inst = (mb, rec)
+        lib().orc_scene_set_instances(self.h, max(mb.shape[0] - 1, 0), mb.ctypes.data, rec.shape[0], rec.ctypes.data)
+
+    def intersect_inst(self, rays, mode=0):
+        """Closest hits with their instances: (prim, t, u, v, inst); mode 1 = brute force."""
+        rays = c32(rays)
+        n = rays.shape[0]
+        prim, inst = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        t, u, v = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        lib().orc_intersect_inst(self.h, n, rays.ctypes.data, mode, prim.ctypes.data, t.ctypes.data, u.ctypes.data,
+                                 v.ctypes.data, inst.ctypes.data)
+        return prim, t, u, v, inst
 
     def add_texture(self, t):
         texels = np.ascontiguousarray(t["texels"], np.float32 if t["format"] == 2 else np.uint8)

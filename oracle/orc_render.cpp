@@ -54,6 +54,13 @@ struct Scene
   std::vector<float> uv;
   std::vector<Texture> textures;
   std::vector<int> baseTex;
+  // two-level scene (geometry/instance.h): meshes are triangle ranges [meshBegin[m],
+  // meshBegin[m + 1]) with object-space trees; instances place them (empty: one level)
+  struct Instance { int mesh; float inv[12], lin[9], nrm[9]; };
+  struct MeshTree { std::vector<Node> nodes; std::vector<Tri> tris; };
+  std::vector<int> meshBegin;
+  std::vector<Instance> inst;
+  std::vector<MeshTree> meshTrees;
 };
 
 // A textured baseColor at a hit: clamp(texture(uv) * (r, g, b)) (material/material_param.h:
@@ -269,15 +276,17 @@ static void buildRecursive(std::vector<Node>& nodes, std::vector<BuildRef>& refs
   buildRecursive(nodes, refs, left + 1, mid, end, depth + 1);
 }
 
-static void buildBvh(Scene& s)
+// A tree over the scene triangles [begin, end) (their Tri records carry the scene index).
+static void buildTree(const Scene& s, int begin, int end, std::vector<Node>& nodes, std::vector<Tri>& tris)
 {
-  int nt = (int)s.triMat.size();
+  int nt = end - begin;
   std::vector<BuildRef> refs(nt);
-  for (int t = 0; t < nt; ++t)
+  for (int k = 0; k < nt; ++k)
   {
+    const int t = begin + k;
     V3 p0 = s.pos[s.idx[3 * t]], p1 = s.pos[s.idx[3 * t + 1]], p2 = s.pos[s.idx[3 * t + 2]];
     V3 lo = vmin(p0, vmin(p1, p2)), hi = vmax(p0, vmax(p1, p2));
-    BuildRef& r = refs[t];
+    BuildRef& r = refs[k];
     for (int a = 0; a < 3; ++a)
     {
       // pad by a relative 2^-20 so a box never culls a hit the triangle test reports
@@ -287,21 +296,44 @@ static void buildBvh(Scene& s)
     }
     r.tri = t;
   }
-  s.nodes.clear();
-  s.nodes.reserve(size_t(nt) * 2 / 2 + 16);
-  s.nodes.push_back(Node());
-  if (nt > 0) buildRecursive(s.nodes, refs, 0, 0, nt, 0);
-  else { s.nodes[0].count = 0; s.nodes[0].first = 0; for (int a = 0; a < 3; ++a) { s.nodes[0].bmin[a] = 1; s.nodes[0].bmax[a] = -1; } }
-  s.tris.resize(nt);
+  nodes.clear();
+  nodes.reserve(size_t(nt) * 2 / 2 + 16);
+  nodes.push_back(Node());
+  if (nt > 0) buildRecursive(nodes, refs, 0, 0, nt, 0);
+  else
+  {
+    // an empty tree: a leaf with no triangles and an empty box (never entered)
+    nodes[0].count = 0; nodes[0].first = 0;
+    for (int a = 0; a < 3; ++a) { nodes[0].bmin[a] = 1; nodes[0].bmax[a] = -1; }
+  }
+  tris.resize(nt);
   for (int i = 0; i < nt; ++i)
   {
     int t = refs[i].tri;
     V3 p0 = s.pos[s.idx[3 * t]], p1 = s.pos[s.idx[3 * t + 1]], p2 = s.pos[s.idx[3 * t + 2]];
-    s.tris[i].v0 = p0; s.tris[i].e1 = p1 - p0; s.tris[i].e2 = p2 - p0; s.tris[i].prim = t;
+    tris[i].v0 = p0; tris[i].e1 = p1 - p0; tris[i].e2 = p2 - p0; tris[i].prim = t;
   }
 }
 
-struct RayHit { V3 o, d; float tnear, tfar; int prim; float u, v; };
+static void buildBvh(Scene& s) { buildTree(s, 0, (int)s.triMat.size(), s.nodes, s.tris); }
+
+struct RayHit { V3 o, d; float tnear, tfar; int prim; float u, v; int inst = -1; };
+
+// The world -> object map of an instance (rows of [A | b]), in the kernels' expression order.
+static V3 xfmPointInv(const Scene::Instance& I, V3 p)
+{
+  return V3(I.inv[0] * p.x + I.inv[1] * p.y + I.inv[2] * p.z + I.inv[3], I.inv[4] * p.x + I.inv[5] * p.y + I.inv[6] * p.z + I.inv[7],
+            I.inv[8] * p.x + I.inv[9] * p.y + I.inv[10] * p.z + I.inv[11]);
+}
+static V3 xfmVectorInv(const Scene::Instance& I, V3 v)
+{
+  return V3(I.inv[0] * v.x + I.inv[1] * v.y + I.inv[2] * v.z, I.inv[4] * v.x + I.inv[5] * v.y + I.inv[6] * v.z,
+            I.inv[8] * v.x + I.inv[9] * v.y + I.inv[10] * v.z);
+}
+static V3 xfmRows(const float* m, V3 v)
+{
+  return V3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
 
 static inline bool boxHit(const Node& n, V3 o, V3 inv, float tnear, float tfar, float& tEntry)
 {
@@ -321,7 +353,42 @@ static inline bool boxHit(const Node& n, V3 o, V3 inv, float tnear, float tfar, 
   return true;
 }
 
+static void traverseTree(const std::vector<Node>& nodes, const std::vector<Tri>& tris, RayHit& r, bool anyHit,
+                         long long* nodeVisits = nullptr);
+
+// Closest or any hit.  In a two-level scene every instance is tried in turn: the ray moves into
+// its object space (t keeps its meaning) and the mesh's tree answers; ties in t go to the
+// smaller (instance, triangle) pair (the order the kernels' tie rule gives).
 static void traverse(const Scene& s, RayHit& r, bool anyHit, long long* nodeVisits = nullptr)
+{
+  r.inst = -1;
+  if (s.inst.empty())
+  {
+    traverseTree(s.nodes, s.tris, r, anyHit, nodeVisits);
+    return;
+  }
+  int best = -1, bestInst = -1;
+  float bu = 0.f, bv = 0.f, tfar = r.tfar;
+  for (size_t i = 0; i < s.inst.size(); ++i)
+  {
+    const Scene::Instance& I = s.inst[i];
+    RayHit q;
+    q.o = xfmPointInv(I, r.o);
+    q.d = xfmVectorInv(I, r.d);
+    q.tnear = r.tnear;
+    q.tfar = tfar;
+    q.u = q.v = 0.f;
+    traverseTree(s.meshTrees[size_t(I.mesh)].nodes, s.meshTrees[size_t(I.mesh)].tris, q, anyHit, nodeVisits);
+    if (q.prim < 0) continue;
+    if (anyHit) { r.prim = q.prim; r.tfar = q.tfar; r.inst = int(i); return; }
+    // instances in increasing order: an equal t keeps the earlier (smaller) instance
+    if (best < 0 || q.tfar < tfar) { tfar = q.tfar; best = q.prim; bestInst = int(i); bu = q.u; bv = q.v; }
+  }
+  r.prim = best;
+  if (best >= 0) { r.tfar = tfar; r.u = bu; r.v = bv; r.inst = bestInst; }
+}
+
+static void traverseTree(const std::vector<Node>& nodes, const std::vector<Tri>& tris, RayHit& r, bool anyHit, long long* nodeVisits)
 {
   V3 inv(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
   int stack[128];
@@ -332,7 +399,7 @@ static void traverse(const Scene& s, RayHit& r, bool anyHit, long long* nodeVisi
   long long visits = 0;
   while (sp)
   {
-    const Node& n = s.nodes[stack[--sp]];
+    const Node& n = nodes[stack[--sp]];
     float te;
     ++visits;
     if (!boxHit(n, r.o, inv, r.tnear, tfar, te)) continue;
@@ -340,7 +407,7 @@ static void traverse(const Scene& s, RayHit& r, bool anyHit, long long* nodeVisi
     {
       for (int i = 0; i < n.count; ++i)
       {
-        const Tri& tr = s.tris[n.first + i];
+        const Tri& tr = tris[n.first + i];
         float t, u, v;
         if (intersectTri(tr, r.o, r.d, r.tnear, tfar, t, u, v))
         {
@@ -355,8 +422,8 @@ static void traverse(const Scene& s, RayHit& r, bool anyHit, long long* nodeVisi
     else
     {
       // push the far child first so the near child is popped first
-      const Node& a = s.nodes[n.first];
-      const Node& b = s.nodes[n.first + 1];
+      const Node& a = nodes[n.first];
+      const Node& b = nodes[n.first + 1];
       float ta, tb;
       bool ha = boxHit(a, r.o, inv, r.tnear, tfar, ta), hb = boxHit(b, r.o, inv, r.tnear, tfar, tb);
       if (ha && hb)
@@ -375,14 +442,26 @@ static void traverse(const Scene& s, RayHit& r, bool anyHit, long long* nodeVisi
 static void bruteForce(const Scene& s, RayHit& r, bool anyHit)
 {
   r.prim = -1;
+  r.inst = -1;
   float tfar = r.tfar;
-  for (size_t i = 0; i < s.tris.size(); ++i)
+  const size_t nInst = s.inst.empty() ? 1 : s.inst.size();
+  for (size_t k = 0; k < nInst; ++k)
   {
-    float t, u, v;
-    if (intersectTri(s.tris[i], r.o, r.d, r.tnear, tfar, t, u, v))
+    // one level: every triangle; two levels: each instance's mesh in its object space
+    const std::vector<Tri>& tris = s.inst.empty() ? s.tris : s.meshTrees[size_t(s.inst[k].mesh)].tris;
+    const V3 o = s.inst.empty() ? r.o : xfmPointInv(s.inst[k], r.o), d = s.inst.empty() ? r.d : xfmVectorInv(s.inst[k], r.d);
+    const int inst = s.inst.empty() ? -1 : int(k);
+    for (size_t i = 0; i < tris.size(); ++i)
     {
-      if (anyHit) { r.prim = s.tris[i].prim; r.tfar = t; return; }
-      if (r.prim < 0 || t < tfar || (t == tfar && s.tris[i].prim < r.prim)) { tfar = t; r.prim = s.tris[i].prim; r.u = u; r.v = v; }
+      float t, u, v;
+      if (intersectTri(tris[i], o, d, r.tnear, tfar, t, u, v))
+      {
+        if (anyHit) { r.prim = tris[i].prim; r.tfar = t; r.inst = inst; return; }
+        if (r.prim < 0 || t < tfar || (t == tfar && (inst < r.inst || (inst == r.inst && tris[i].prim < r.prim))))
+        {
+          tfar = t; r.prim = tris[i].prim; r.u = u; r.v = v; r.inst = inst;
+        }
+      }
     }
   }
   if (r.prim >= 0) r.tfar = tfar;
@@ -391,7 +470,10 @@ static void bruteForce(const Scene& s, RayHit& r, bool anyHit)
 // ---------------------------------------------------------------------------------------
 // Hit geometry: geometry/triangle_mesh.cpp:152-213 + render/scene.h:61-72
 // ---------------------------------------------------------------------------------------
-static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u, float v)
+// inst >= 0: the mesh's post-intersection in object space, then the tangents to world space by
+// the instance's linear part and the normals by its inverse transpose, renormalised
+// (geometry/instance.h:17-26), before the orientation flip.
+static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u, float v, int inst = -1)
 {
   Ctx c;
   c.p = o + t * d;  // core/ray_simd.h:58-68
@@ -415,7 +497,15 @@ static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u,
   c.Ng = Ng;
   c.f.N = N;
   makeFrame(c.f.U, c.f.V, N);  // no texcoords: triangle_mesh.cpp:235-242
-  c.backfacing = dot(Ng, d) > 0.f;
+  if (inst >= 0)
+  {
+    const Scene::Instance& I = s.inst[size_t(inst)];
+    c.f.U = normalize(xfmRows(I.lin, c.f.U));
+    c.f.V = normalize(xfmRows(I.lin, c.f.V));
+    c.f.N = normalize(xfmRows(I.nrm, c.f.N));
+    c.Ng = normalize(xfmRows(I.nrm, c.Ng));
+  }
+  c.backfacing = dot(c.Ng, d) > 0.f;
   if (c.backfacing) { c.Ng = -c.Ng; c.f.N = -c.f.N; }
   return c;
 }
@@ -423,31 +513,31 @@ static Ctx postIntersect(const Scene& s, V3 o, V3 d, float t, int prim, float u,
 // Any-hit traversal with the transparent-shadow filter (render/transparent_filter.h:42-93):
 // each hit on a transparent material multiplies T by the material's transparency along -d and
 // the ray goes on until T falls below 1e-5; a hit on any other material occludes (T = 0).
-static void traverseShadow(const Scene& s, RayHit& r, V3& T, uint32_t& crossed)
+// One tree of the filtered any-hit traversal: triangles tested with the ray (o, d) of the tree's
+// space, hits shaded with the world ray of r on instance inst.  Returns true when occluded.
+static bool traverseShadowTree(const Scene& s, const std::vector<Node>& nodes, const std::vector<Tri>& tris, V3 o, V3 d,
+                               RayHit& r, V3& T, uint32_t& crossed, int inst)
 {
-  T = V3(1.f);
-  crossed = 0;
-  V3 inv(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+  V3 inv(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   int stack[128];
   int sp = 0;
   stack[sp++] = 0;
-  r.prim = -1;
   while (sp)
   {
-    const Node& n = s.nodes[stack[--sp]];
+    const Node& n = nodes[stack[--sp]];
     float te;
-    if (!boxHit(n, r.o, inv, r.tnear, r.tfar, te)) continue;
+    if (!boxHit(n, o, inv, r.tnear, r.tfar, te)) continue;
     if (n.count > 0)
     {
       for (int i = 0; i < n.count; ++i)
       {
-        const Tri& tr = s.tris[n.first + i];
+        const Tri& tr = tris[n.first + i];
         float t, u, v;
-        if (!intersectTri(tr, r.o, r.d, r.tnear, r.tfar, t, u, v)) continue;
+        if (!intersectTri(tr, o, d, r.tnear, r.tfar, t, u, v)) continue;
         const Material& m = s.mats[s.triMat[tr.prim]];
         if (isTransparent(m))
         {
-          Ctx c = postIntersect(s, r.o, r.d, t, tr.prim, u, v);
+          Ctx c = postIntersect(s, r.o, r.d, t, tr.prim, u, v, inst);
           Bsdf b = makeBsdf(s.luts, m, c, -r.d, 0, -1);
           T *= bsdfTransparency(s.luts, b, -r.d);
           ++crossed;
@@ -456,7 +546,8 @@ static void traverseShadow(const Scene& s, RayHit& r, V3& T, uint32_t& crossed)
         else T = V3(0.f);
         r.prim = tr.prim;
         r.tfar = t;
-        return;
+        r.inst = inst;
+        return true;
       }
     }
     else
@@ -464,6 +555,26 @@ static void traverseShadow(const Scene& s, RayHit& r, V3& T, uint32_t& crossed)
       stack[sp++] = n.first + 1;
       stack[sp++] = n.first;
     }
+  }
+  return false;
+}
+
+static void traverseShadow(const Scene& s, RayHit& r, V3& T, uint32_t& crossed)
+{
+  T = V3(1.f);
+  crossed = 0;
+  r.prim = -1;
+  r.inst = -1;
+  if (s.inst.empty())
+  {
+    traverseShadowTree(s, s.nodes, s.tris, r.o, r.d, r, T, crossed, -1);
+    return;
+  }
+  for (size_t i = 0; i < s.inst.size(); ++i)
+  {
+    const Scene::Instance& I = s.inst[i];
+    const Scene::MeshTree& M = s.meshTrees[size_t(I.mesh)];
+    if (traverseShadowTree(s, M.nodes, M.tris, xfmPointInv(I, r.o), xfmVectorInv(I, r.d), r, T, crossed, int(i))) return;
   }
 }
 
@@ -936,6 +1047,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
     st.rays++;
     if (depth == 0) firstPrim = r.prim;
     hash = hashStep(hash, uint32_t(r.prim + 1));
+    if (r.inst >= 0) hash = hashStep(hash, 0x40000000u | uint32_t(r.inst));  // the instance hit (two-level scenes)
     int matId = r.prim >= 0 ? s.triMat[r.prim] : 0;
     if (matId == 0)
     {
@@ -1012,7 +1124,7 @@ static void tracePath(const Scene& s, const Camera& cam, const RenderParams& P, 
     if (depth < P.maxDepth)
     {
       const Material& m = s.mats[matId];
-      Ctx ctx = postIntersect(s, org, dir, r.tfar, r.prim, r.u, r.v);
+      Ctx ctx = postIntersect(s, org, dir, r.tfar, r.prim, r.u, r.v, r.inst);
       V3 wo = -dir;
       const int mediumIntId = s.matMedium[matId];
       V3 texBase;
@@ -1190,6 +1302,106 @@ void* orc_scene_create(int nverts, const float* pos, const float* nrm, int ntris
 }
 
 void orc_scene_destroy(void* p) { delete (Scene*)p; }
+
+// Two-level scene: meshes [meshBegin[m], meshBegin[m + 1]) of the triangles and n instances,
+// each an int mesh followed by the 12 floats of the rows of [L | t] (x_world = L x + t); n = 0:
+// one level.  The inverse, the linear part and its inverse transpose are formed in double and
+// rounded to float, as the product's host does.
+void orc_scene_set_instances(void* p, int nMeshes, const int* meshBegin, int n, const void* inst)
+{
+  Scene& s = *(Scene*)p;
+  s.inst.clear();
+  s.meshTrees.clear();
+  s.meshBegin.clear();
+  if (n <= 0) return;
+  s.meshBegin.assign(meshBegin, meshBegin + nMeshes + 1);
+  s.meshTrees.resize(size_t(nMeshes));
+  for (int m = 0; m < nMeshes; ++m) buildTree(s, meshBegin[m], meshBegin[m + 1], s.meshTrees[size_t(m)].nodes, s.meshTrees[size_t(m)].tris);
+  const char* q = (const char*)inst;
+  for (int i = 0; i < n; ++i, q += 52)
+  {
+    Scene::Instance I;
+    std::memcpy(&I.mesh, q, 4);
+    float x[12];
+    std::memcpy(x, q + 4, 48);
+    const double L[9] = {x[0], x[1], x[2], x[4], x[5], x[6], x[8], x[9], x[10]}, Tt[3] = {x[3], x[7], x[11]};
+    // inverse by cofactors / determinant
+    const double c00 = L[4] * L[8] - L[5] * L[7], c01 = L[5] * L[6] - L[3] * L[8], c02 = L[3] * L[7] - L[4] * L[6];
+    const double det = L[0] * c00 + L[1] * c01 + L[2] * c02;
+    double A[9];
+    A[0] = c00 / det; A[1] = (L[2] * L[7] - L[1] * L[8]) / det; A[2] = (L[1] * L[5] - L[2] * L[4]) / det;
+    A[3] = c01 / det; A[4] = (L[0] * L[8] - L[2] * L[6]) / det; A[5] = (L[2] * L[3] - L[0] * L[5]) / det;
+    A[6] = c02 / det; A[7] = (L[1] * L[6] - L[0] * L[7]) / det; A[8] = (L[0] * L[4] - L[1] * L[3]) / det;
+    for (int r = 0; r < 3; ++r)
+    {
+      for (int k = 0; k < 3; ++k)
+      {
+        I.inv[4 * r + k] = float(A[3 * r + k]);
+        I.lin[3 * r + k] = float(L[3 * r + k]);
+        I.nrm[3 * r + k] = float(A[3 * k + r]);
+      }
+      I.inv[4 * r + 3] = float(-(A[3 * r] * Tt[0] + A[3 * r + 1] * Tt[1] + A[3 * r + 2] * Tt[2]));
+    }
+    s.inst.push_back(I);
+  }
+  // scene bounds and radius (render/scene.cpp:367): each instanced mesh's vertex box, its
+  // corners through x' = L x + t in double, rounded to float
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  bool any = false;
+  for (int i = 0; i < n; ++i)
+  {
+    const int m = s.inst[size_t(i)].mesh;
+    if (meshBegin[m + 1] == meshBegin[m]) continue;
+    double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
+    for (int t = meshBegin[m]; t < meshBegin[m + 1]; ++t)
+      for (int j = 0; j < 3; ++j)
+      {
+        const V3 p = s.pos[s.idx[3 * t + j]];
+        const double pv[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; ++k) { blo[k] = std::min(blo[k], pv[k]); bhi[k] = std::max(bhi[k], pv[k]); }
+      }
+    float x[12];
+    std::memcpy(x, (const char*)inst + 52 * size_t(i) + 4, 48);
+    for (int corner = 0; corner < 8; ++corner)
+    {
+      const double pc[3] = {(corner & 1) ? bhi[0] : blo[0], (corner & 2) ? bhi[1] : blo[1], (corner & 4) ? bhi[2] : blo[2]};
+      for (int k = 0; k < 3; ++k)
+      {
+        const double w = double(x[4 * k]) * pc[0] + double(x[4 * k + 1]) * pc[1] + double(x[4 * k + 2]) * pc[2] + double(x[4 * k + 3]);
+        lo[k] = std::min(lo[k], w);
+        hi[k] = std::max(hi[k], w);
+      }
+    }
+    any = true;
+  }
+  if (any)
+  {
+    s.bmin = V3(float(lo[0]), float(lo[1]), float(lo[2]));
+    s.bmax = V3(float(hi[0]), float(hi[1]), float(hi[2]));
+    s.sceneRadius = length(s.bmax - s.bmin) * 0.5f;
+  }
+}
+
+// Closest hit with the instance of each hit (-1: miss or one level); mode as orc_intersect.
+void orc_intersect_inst(void* p, int n, const float* rays, int mode, int* prim, float* t, float* u, float* v, int* inst)
+{
+  Scene& s = *(Scene*)p;
+  for (int i = 0; i < n; ++i)
+  {
+    const float* q = rays + 8 * i;
+    RayHit r;
+    r.o = V3(q[0], q[1], q[2]); r.d = V3(q[3], q[4], q[5]); r.tnear = q[6]; r.tfar = q[7];
+    r.u = r.v = 0.f;
+    const bool any = mode >= 2;
+    if (mode == 0 || mode == 2) traverse(s, r, any);
+    else bruteForce(s, r, any);
+    prim[i] = r.prim;
+    t[i] = r.prim >= 0 ? r.tfar : q[7];
+    u[i] = r.u;
+    v[i] = r.v;
+    inst[i] = r.prim >= 0 ? r.inst : -1;
+  }
+}
 
 // UV set 0 for the scene's vertices (null: none).
 void orc_scene_set_uvs(void* p, const float* uv, int nverts)

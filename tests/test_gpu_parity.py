@@ -125,8 +125,8 @@ def _parity(dev, scene, W, H, spp, min_match):
     P.load_scene(dev, scene)
     dev.initFrame(W, H, P.AOV_ALBEDO1 | P.AOV_NORMAL1)
     osc = O.Scene(scene["positions"], scene["indices"], scene["tri_material"], scene["materials"], scene.get("normals"))
+    osc.set_textures(scene)  # UVs, textures and instances (before the lights: the radius)
     osc.set_lights(scene["lights"], scene.get("env"))
-    osc.set_textures(scene)
     acc = None
     for s in range(spp):
         dev.render(1)
@@ -220,6 +220,16 @@ def test_render_parity_textured_materials(dev):
     scene = G.spheres_textured()
     assert len(scene["material_textures"]) >= 10
     _parity(dev, scene, 96, 64, 4, 0.998)
+
+
+def test_render_parity_instanced_scene(dev):
+    # a two-level scene: six meshes in object space (smooth normals, a textured Oren-Nayar base,
+    # coated, conductor, dielectric media interface, thin dielectric seen by shadow rays through
+    # the transparent-shadow filter) placed by 41 instances with rotations, non-uniform scales and
+    # translations
+    scene = G.spheres_instanced()
+    _parity(dev, scene, 96, 64, 4, 0.998)
+    dev.initInstances(None, [])  # back to one level for the other tests on this context
 
 
 @pytest.mark.parametrize("ambient", [0.5, None])
