@@ -251,8 +251,9 @@ int prt_scene_set_lights(prt_context* ctx, int32_t n, const prt_light* lights, i
  * the smaller instance, then the smaller triangle).  n = 0 returns to a one-level scene; new
  * geometry drops the instances.  Two-level scenes render with prt_render and answer
  * prt_query_rays / prt_query_rays_device / prt_query_occlusion of opaque scenes; emissive
- * geometry must stay one-level (the reference flattens emissive objects, render/scene.cpp:
- * 343-363), and the batch renderer, the adjoint, the pixel and hit queries, the transparent-
+ * triangles must lie in a mesh placed exactly once by the identity (the reference flattens
+ * emissive objects, render/scene.cpp:343-363; prt_scene_set_lights checks it), and the batch
+ * renderer, the adjoint, the pixel and hit queries, the transparent-
  * shadow occlusion query and the allocation-free query take one-level scenes. */
 typedef struct prt_instance {
   int32_t mesh;
@@ -506,6 +507,15 @@ typedef struct prt_scene_file_info {
   int32_t instance_count;          /* instances flattened */
 } prt_scene_file_info;
 int prt_scene_file_load(const char* path, prt_scene_file** out, prt_scene_file_info* info);
+/* The same file kept in two levels (render/scene.cpp:325-363 without the `flatten` prop): each
+ * instanced object becomes a mesh in object space placed by one prt_instance per transform;
+ * objects without instances and emissive objects (which the reference always flattens) form one
+ * world-space mesh placed by the identity.  get_instances returns the meshes' triangle ranges
+ * (n_meshes + 1 entries) and the instances, for prt_scene_set_instances; null arrays return
+ * the counts only.  info.instance_count counts the instance records. */
+int prt_scene_file_load_instanced(const char* path, prt_scene_file** out, prt_scene_file_info* info);
+int prt_scene_file_get_instances(prt_scene_file* f, int32_t* n_meshes, int32_t* mesh_begin, int32_t* n_instances,
+                                 prt_instance* instances);
 int prt_scene_file_get(prt_scene_file* f, float* positions, float* normals, int32_t* indices, int32_t* tri_material,
                        prt_material* materials);
 int prt_scene_file_material_name(prt_scene_file* f, int32_t id, char* buf, int32_t len);

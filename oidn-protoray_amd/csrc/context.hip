@@ -1070,11 +1070,33 @@ static int buildAreaLight(prt_context* c)
   std::vector<int> map(c->ntris, -1);
   std::vector<float4> tris;
   std::vector<float> w;
+  // a two-level scene: emissive triangles must lie in world space, in meshes placed exactly once
+  // by the identity (the reference flattens emissive objects, render/scene.cpp:343-363)
+  std::vector<char> worldMesh;
+  if (c->instanced())
+  {
+    const int nMesh = int(c->meshBegin.size()) - 1;
+    std::vector<int> uses(size_t(nMesh), 0);
+    std::vector<char> ident(size_t(nMesh), 1);
+    for (const prt_instance& I : c->instances)
+    {
+      ++uses[size_t(I.mesh)];
+      const float id[12] = {1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+      for (int k = 0; k < 12; ++k)
+        if (I.xfm[k] != id[k]) ident[size_t(I.mesh)] = 0;
+    }
+    worldMesh.resize(size_t(c->ntris));
+    for (int m = 0; m < nMesh; ++m)
+      for (int t = c->meshBegin[size_t(m)]; t < c->meshBegin[size_t(m) + 1]; ++t)
+        worldMesh[size_t(t)] = uses[size_t(m)] == 1 && ident[size_t(m)];
+  }
   for (int t = 0; t < c->ntris; ++t)
   {
     if (triMat[t] >= int(c->materials.size())) return fail(PRT_ERR_INVALID, "triangle material ID beyond the material table");
     const prt_material& m = c->materials[triMat[t]];
     if (!(m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f)) continue;
+    if (!worldMesh.empty() && !worldMesh[size_t(t)])
+      return fail(PRT_ERR_INVALID, "emissive triangles of a two-level scene must be in a mesh placed once by the identity");
     const float* p0 = &pos[3 * size_t(idx[3 * t])];
     const float* p1 = &pos[3 * size_t(idx[3 * t + 1])];
     const float* p2 = &pos[3 * size_t(idx[3 * t + 2])];
@@ -1762,8 +1784,6 @@ int prt_render(prt_context* c, int32_t n_spp, prt_stats* stats)
   if (n_spp < 0) return fail(PRT_ERR_INVALID, "negative sample count");
   if (c->instanced())
   {
-    // the reference flattens emissive objects (render/scene.cpp:343-363): area lights are one level
-    if (c->areaCount > 0) return fail(PRT_ERR_INVALID, "emissive geometry in a two-level scene: flatten the emissive objects");
     if (c->hitInst.n < size_t(c->npix)) HIP_TRY(c->hitInst.alloc(size_t(c->npix)));
   }
   RenderArgs a = makeArgs(c);

@@ -1139,6 +1139,9 @@ struct prt_scene_file
   std::vector<prt_material> mats;
   std::vector<std::string> names;
   prt_scene_file_info info{};
+  // two-level load: meshes as triangle ranges and the instances that place them (empty: flat)
+  std::vector<int32_t> meshBegin;
+  std::vector<prt_instance> inst;
 };
 
 namespace {
@@ -1281,6 +1284,67 @@ void flatten(const Desc& d, prt_scene_file& out)
   out.info.sun_direction[2] = d.sunDir.z;
 }
 
+// Two levels (render/scene.cpp:325-363 without `flatten`): each instanced object becomes a mesh
+// in its object space, placed by one instance per transform; objects without instances, and
+// emissive objects (which the reference always flattens, :343-363), go to one world-space mesh
+// placed once by the identity.
+void twoLevel(const Desc& d, prt_scene_file& out)
+{
+  std::vector<int> fileToId;
+  int defaultId = 0;
+  numberMaterials(d.mats, out, fileToId, defaultId);
+  bool withNormals = false;
+  for (const Geom& g : d.geoms)
+    if (!g.nrm.empty()) withNormals = true;
+  auto matOf = [&](const Geom& g) {
+    return g.material >= 0 && size_t(g.material) < fileToId.size() ? fileToId[size_t(g.material)] : defaultId;
+  };
+  auto emissive = [&](const Obj& o) {
+    for (int gi : o.geoms)
+    {
+      if (gi < 0 || size_t(gi) >= d.geoms.size()) fail("object references a missing geometry");
+      const prt_material& m = out.mats[size_t(matOf(d.geoms[size_t(gi)]))];
+      if (m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f) return true;
+    }
+    return false;
+  };
+  auto twoLevelObj = [&](const Obj& o) { return !o.instances.empty() && !emissive(o); };
+  out.meshBegin.assign(1, 0);
+  // the world-space mesh
+  for (const Obj& o : d.objs)
+  {
+    if (twoLevelObj(o)) continue;
+    const int copies = o.instances.empty() ? 1 : int(o.instances.size());
+    for (int c = 0; c < copies; ++c)
+      for (int gi : o.geoms)
+        appendGeometry(out, d.geoms[size_t(gi)], o.instances.empty() ? nullptr : &o.instances[size_t(c)],
+                       matOf(d.geoms[size_t(gi)]), withNormals);
+  }
+  const prt_instance identity{0, {1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f}};
+  if (!out.idx.empty())
+  {
+    out.meshBegin.push_back(int32_t(out.idx.size() / 3));
+    out.inst.push_back(identity);
+  }
+  // one mesh per instanced object, in object space
+  for (const Obj& o : d.objs)
+  {
+    if (!twoLevelObj(o)) continue;
+    for (int gi : o.geoms) appendGeometry(out, d.geoms[size_t(gi)], nullptr, matOf(d.geoms[size_t(gi)]), withNormals);
+    const int mesh = int(out.meshBegin.size()) - 1;
+    out.meshBegin.push_back(int32_t(out.idx.size() / 3));
+    for (const Xfm& x : o.instances)  // columns v[0..2] of L, v[3] = t -> rows of [L | t]
+      out.inst.push_back(prt_instance{mesh, {x.v[0].x, x.v[1].x, x.v[2].x, x.v[3].x, x.v[0].y, x.v[1].y, x.v[2].y, x.v[3].y,
+                                             x.v[0].z, x.v[1].z, x.v[2].z, x.v[3].z}});
+  }
+  if (!withNormals) out.nrm.clear();
+  out.info.instance_count = int32_t(out.inst.size());
+  out.info.has_sun_sky = d.hasSun ? 1 : 0;
+  out.info.sun_direction[0] = d.sunDir.x;
+  out.info.sun_direction[1] = d.sunDir.y;
+  out.info.sun_direction[2] = d.sunDir.z;
+}
+
 void fromMesh(const MeshFile& m, prt_scene_file& out)
 {
   // .mesh material IDs are stored already numbered (render/scene.cpp:136-172): no
@@ -1315,7 +1379,7 @@ void fromMesh(const MeshFile& m, prt_scene_file& out)
 
 extern "C" {
 
-int prt_scene_file_load(const char* path, prt_scene_file** out, prt_scene_file_info* info)
+static int loadFile(const char* path, prt_scene_file** out, prt_scene_file_info* info, bool instanced)
 {
   if (!path || !out) return prt::setLastError(PRT_ERR_INVALID, "null argument");
   *out = nullptr;
@@ -1323,8 +1387,16 @@ int prt_scene_file_load(const char* path, prt_scene_file** out, prt_scene_file_i
   try
   {
     const std::string p(path), ext = extOf(p);
-    if (ext == "rkscene") flatten(loadRkscene(p), *f);
-    else if (ext == "gltf") flatten(loadGltf(p), *f);
+    if (ext == "rkscene")
+    {
+      if (instanced) twoLevel(loadRkscene(p), *f);
+      else flatten(loadRkscene(p), *f);
+    }
+    else if (ext == "gltf")
+    {
+      if (instanced) twoLevel(loadGltf(p), *f);
+      else flatten(loadGltf(p), *f);
+    }
     else if (ext == "mesh") fromMesh(loadMesh(p), *f);
     else return prt::setLastError(PRT_ERR_INVALID, "unknown scene format: " + p);
   }
@@ -1344,6 +1416,27 @@ int prt_scene_file_load(const char* path, prt_scene_file** out, prt_scene_file_i
   f->info.has_normals = f->nrm.empty() ? 0 : 1;
   if (info) *info = f->info;
   *out = f.release();
+  return PRT_OK;
+}
+
+int prt_scene_file_load(const char* path, prt_scene_file** out, prt_scene_file_info* info)
+{
+  return loadFile(path, out, info, false);
+}
+
+int prt_scene_file_load_instanced(const char* path, prt_scene_file** out, prt_scene_file_info* info)
+{
+  return loadFile(path, out, info, true);
+}
+
+int prt_scene_file_get_instances(prt_scene_file* f, int32_t* n_meshes, int32_t* mesh_begin, int32_t* n_instances,
+                                 prt_instance* instances)
+{
+  if (!f) return prt::setLastError(PRT_ERR_INVALID, "null scene file");
+  if (n_meshes) *n_meshes = f->meshBegin.empty() ? 0 : int32_t(f->meshBegin.size()) - 1;
+  if (n_instances) *n_instances = int32_t(f->inst.size());
+  if (mesh_begin && !f->meshBegin.empty()) std::memcpy(mesh_begin, f->meshBegin.data(), f->meshBegin.size() * sizeof(int32_t));
+  if (instances && !f->inst.empty()) std::memcpy(instances, f->inst.data(), f->inst.size() * sizeof(prt_instance));
   return PRT_OK;
 }
 

@@ -124,6 +124,8 @@ SIGNATURES = {
     "prt_scene_set_uvs": [_vp, _vp, _i],
     "prt_scene_set_instances": [_vp, _i, _vp, _i, _vp],
     "prt_query_rays_inst": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "prt_scene_file_load_instanced": [ctypes.c_char_p, _vp, _vp],
+    "prt_scene_file_get_instances": [_vp, _vp, _vp, _vp, _vp],
     "prt_scene_set_textures": [_vp, _i, _vp],
     "prt_material_set_texture": [_vp, _i, _i, _i],
     "prt_denoiser_execute_batch": [_vp, _vp, _i, _i, _i, _f, _vp],
@@ -503,15 +505,18 @@ class SceneFileInfo(ctypes.Structure):  # prt_scene_file_info
                 ("approximated_materials", ctypes.c_int32), ("instance_count", ctypes.c_int32)]
 
 
-def load_scene_file(path):
+def load_scene_file(path, instanced=False):
     """Scene::Scene(path) for .rkscene / .gltf / .mesh files (render/scene.cpp:64-372): the
     flattened world-space mesh, per-triangle material IDs and material records as a scene dict
     (positions, normals or None, indices, tri_material, materials, material_names, info).
-    Host-side parsing only; lights, camera and renderer settings are the caller's."""
+    instanced: keep instanced objects in two levels (meshes in object space, "mesh_begin" and
+    "instances" for DeviceHip.initInstances).  Host-side parsing only; lights, camera and
+    renderer settings are the caller's."""
     L = load_library()
     h = ctypes.c_void_p()
     info = SceneFileInfo()
-    _check(L.prt_scene_file_load(os.fsencode(str(path)), ctypes.byref(h), ctypes.byref(info)))
+    loader = L.prt_scene_file_load_instanced if instanced else L.prt_scene_file_load
+    _check(loader(os.fsencode(str(path)), ctypes.byref(h), ctypes.byref(info)))
     try:
         nv, nt, nm = info.vertex_count, info.triangle_count, info.material_count
         pos = np.zeros((nv, 3), np.float32)
@@ -525,10 +530,18 @@ def load_scene_file(path):
         for i in range(nm):
             _check(L.prt_scene_file_material_name(h, i, buf, 1024))
             names.append(buf.value.decode(errors="replace"))
+        two = {}
+        if instanced:
+            nmesh, ninst = ctypes.c_int32(0), ctypes.c_int32(0)
+            _check(L.prt_scene_file_get_instances(h, ctypes.byref(nmesh), None, ctypes.byref(ninst), None))
+            mb = np.zeros(nmesh.value + 1, np.int32)
+            rec = np.zeros(ninst.value, INSTANCE_DTYPE)
+            _check(L.prt_scene_file_get_instances(h, ctypes.byref(nmesh), _ptr(mb), ctypes.byref(ninst), _ptr(rec)))
+            two = {"mesh_begin": mb, "instances": [(int(r["mesh"]), r["xfm"].reshape(3, 4)) for r in rec]}
     finally:
         L.prt_scene_file_destroy(h)
     return {"positions": pos, "normals": nrm, "indices": idx, "tri_material": tm, "materials": mats,
-            "material_names": names,
+            "material_names": names, **two,
             "info": {"approximated_materials": info.approximated_materials, "instance_count": info.instance_count,
                      "has_sun_sky": bool(info.has_sun_sky), "sun_direction": tuple(info.sun_direction)}}
 

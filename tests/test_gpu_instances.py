@@ -93,9 +93,7 @@ def test_two_level_limits():
         mats[2]["emission"] = (1.0, 1.0, 1.0)
         dev.initScene(sc["positions"], sc["indices"], sc["tri_material"], mats, sc["normals"])
         dev.initInstances(sc["mesh_begin"], sc["instances"])
-        dev.initLights(sc["lights"], sc["env"])
-        dev.initFrame(64, 48)
-        with pytest.raises(ValueError):
-            dev.render(1)
+        with pytest.raises(ValueError):  # emissive triangles in a mesh placed several times
+            dev.initLights(sc["lights"], sc["env"])
     finally:
         dev.close()

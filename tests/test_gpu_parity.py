@@ -239,7 +239,8 @@ def test_render_parity_area_lights(dev, ambient):
     _parity(dev, scene, 64, 64, 4, 0.998)
 
 
-def test_render_parity_scene_file(dev, tmp_path):
+@pytest.mark.parametrize("instanced", [False, True])
+def test_render_parity_scene_file(dev, tmp_path, instanced):
     # a glTF file (smooth-normal spheres instanced by a node tree, a ground quad, an emissive quad,
     # Standard / metal / glass / emissive materials) through prt_scene_file_load, rendered with
     # interpolated vertex normals, against the oracle on the same arrays
@@ -275,15 +276,22 @@ def test_render_parity_scene_file(dev, tmp_path):
     w.node(mesh=lamp, root=True, translation=[0, 6, 10], rotation=S.quaternion((1, 0, 0), math.pi), scale=[2, 1, 2])
     path = tmp_path / "parity.gltf"
     w.save(str(path))
-    sc = P.load_scene_file(path)
-    assert sc["normals"] is not None and sc["info"]["instance_count"] == 14
+    sc = P.load_scene_file(path, instanced=instanced)
+    if instanced:  # the spheres' three meshes and the floor in two levels, the emissive lamp flattened
+        assert len(sc["instances"]) == 14 and len(sc["mesh_begin"]) == 6
+    else:
+        assert sc["info"]["instance_count"] == 14
+    assert sc["normals"] is not None
     lights = np.zeros(2, G.LIGHT_DTYPE)
     d = np.array([-0.4, 0.8, -0.45])
     d /= np.linalg.norm(d)
     lights[0] = (G.LIGHT_AMBIENT, 0.3, 0.35, 0.4, 0, 0, 0, 0)
     lights[1] = (G.LIGHT_DISTANT, 20.0, 19.0, 17.0, d[0], d[1], d[2], math.radians(2.0))
-    sc.update(name="gltf_file", lights=lights, env=None, camera=G._camera(), max_depth=5, seed=0)
+    sc.update(name="gltf_file_two_level" if instanced else "gltf_file", lights=lights, env=None, camera=G._camera(),
+              max_depth=5, seed=0)
     _parity(dev, sc, 64, 64, 3, 0.998)
+    if instanced:
+        dev.initInstances(None, [])
 
 
 def test_render_stats_and_error(dev):

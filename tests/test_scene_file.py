@@ -347,3 +347,58 @@ def test_remaining_lobe_parameters_map_to_records(tmp_path):
     assert np.allclose(velvet["sheen_color"], [0.9, 0.8, 0.7]) and np.isclose(velvet["sheen_roughness"], 0.45)
     for r in (thin, cutout, clay):
         assert r["sheen_weight"] == 0
+
+
+def _world_triangles(sc):
+    """World-space triangles (n x 3 x 3) and their material IDs of a scene dict: a flattened
+    load as is, a two-level load with every instance's mesh moved by its [L | t]."""
+    pos, idx, tm = sc["positions"].astype(np.float64), sc["indices"], sc["tri_material"]
+    if "instances" not in sc:
+        return pos[idx], tm
+    tris, mats = [], []
+    mb = sc["mesh_begin"]
+    for mesh, xfm in sc["instances"]:
+        b, e = mb[mesh], mb[mesh + 1]
+        p = pos[idx[b:e]]
+        tris.append(p @ np.asarray(xfm, np.float64)[:, :3].T + np.asarray(xfm, np.float64)[:, 3])
+        mats.append(tm[b:e])
+    return np.concatenate(tris), np.concatenate(mats)
+
+
+@pytest.mark.parametrize("fmt", ["gltf", "rkscene"])
+def test_two_level_load_places_the_same_triangles(tmp_path, fmt):
+    # the two-level load (meshes in object space + instances) places exactly the triangles the
+    # flattened load produces, with the same materials; emissive objects stay in the world mesh
+    sp, si = S.icosphere(1)
+    quad = np.array([[-1, 0, -1], [1, 0, -1], [-1, 0, 1], [1, 0, 1]], np.float32)
+    qi = np.array([[0, 2, 1], [1, 2, 3]])
+    rng = np.random.default_rng(11)
+    path = tmp_path / ("two." + fmt)
+    if fmt == "gltf":
+        w = S.GltfWriter()
+        w.doc["materials"] = [{"name": "red", "pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.1, 0.1, 1]}},
+                              {"name": "lamp", "emissiveFactor": [1.0, 1.0, 1.0]}]
+        pa, ia = w.accessor(sp, 5126, "VEC3"), w.accessor(si.ravel(), 5125, "SCALAR")
+        qa, qia = w.accessor(quad, 5126, "VEC3"), w.accessor(qi.ravel(), 5125, "SCALAR")
+        sphere = w.mesh([{"attributes": {"POSITION": pa}, "indices": ia, "material": 0}])
+        lamp = w.mesh([{"attributes": {"POSITION": qa}, "indices": qia, "material": 1}])
+        kids = [w.node(mesh=sphere, translation=[float(x) for x in rng.uniform(-4, 4, 3)],
+                       scale=[float(x) for x in rng.uniform(0.4, 1.2, 3)]) for _ in range(4)]
+        w.node(children=kids, root=True, rotation=S.quaternion((0, 1, 0), 0.3))
+        w.node(mesh=lamp, root=True, translation=[0, 5, 0], scale=[2, 1, 2])
+        w.save(str(path))
+    else:
+        path = _sample_rkscene(tmp_path)  # a plain object holding the lamp, objects with 2 and 1 instances
+    flat = P.load_scene_file(path)
+    two = P.load_scene_file(path, instanced=True)
+    tf, mf = _world_triangles(flat)
+    tt, mt = _world_triangles(two)
+    assert tf.shape == tt.shape
+    key = lambda t, m: np.lexsort((m, *np.round(t.mean(axis=1), 4).T[::-1]))
+    of, ot = key(tf, mf), key(tt, mt)
+    np.testing.assert_allclose(tt[ot], tf[of], rtol=1e-5, atol=1e-5)
+    assert np.array_equal(mt[ot], mf[of])
+    # the emissive object is flattened into the world mesh: placed once, by the identity
+    lamp_ids = [i for i, m in enumerate(two["materials"]) if m["emission"].max() > 0]
+    world = [k for k, (mesh, x) in enumerate(two["instances"]) if np.any(np.isin(two["tri_material"][two["mesh_begin"][mesh]:two["mesh_begin"][mesh + 1]], lamp_ids))]
+    assert len(world) == 1 and np.array_equal(two["instances"][world[0]][1], np.hstack([np.eye(3), np.zeros((3, 1))]))
