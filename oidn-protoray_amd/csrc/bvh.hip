@@ -125,6 +125,62 @@ __global__ void __launch_bounds__(kBlock) kLeafInit(const float* __restrict__ po
   ccnt[k] = 1;
 }
 
+// Half the surface area of a box given by its corners (the same expression wherever a cost is
+// computed and later recomputed, so the collapse's comparisons reproduce the clustering's).
+__device__ __forceinline__ float boxHalfArea(float lx, float ly, float lz, float hx, float hy, float hz)
+{
+  float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Cost model of the wide tree (surface-area heuristic): entering a wide node costs `node` per
+// unit of its area, a leaf `tri` per triangle per unit of its area; leaves hold at most
+// `leafMax` triangles.  C(x, j) is the least cost of the subtree of binary node x cut into at
+// most j subtrees (each a leaf or a wide node); the collapse reads C to place the cuts.
+struct CollapseCost
+{
+  float node, tri;
+  int leafMax;
+};
+static_assert(PRT_LEAF_MAX >= 1 && PRT_LEAF_MAX <= PRT_LEAF_CAP, "leaf size beyond the leaf reference's count field");
+inline CollapseCost collapseCost() { return CollapseCost{1.0f, 0.5f, PRT_LEAF_MAX}; }
+
+// Sorted triangle k's leaf cost, the same for every j (a triangle is never cut).
+__device__ __forceinline__ float4 subtreeCost(int c, const float4* cost, const float* leafBox, float tri)
+{
+  if (c >= 0) return cost[c];
+  const float* b = leafBox + 6 * size_t(~c);
+  const float a = tri * boxHalfArea(b[0], b[1], b[2], b[3], b[4], b[5]);
+  return make_float4(a, a, a, a);
+}
+
+__device__ __forceinline__ float costAt(float4 c, int j) { return j == 1 ? c.x : j == 2 ? c.y : j == 3 ? c.z : c.w; }
+
+// Least cost of the two children cut into at most j >= 2 subtrees together, and the left
+// child's share k of the cuts (the smallest k on ties).
+__device__ __forceinline__ float splitCost(float4 l, float4 r, int j, int& kBest)
+{
+  kBest = 1;
+  float best = costAt(l, 1) + costAt(r, j - 1);
+  for (int k = 2; k < j; ++k)
+  {
+    const float v = costAt(l, k) + costAt(r, j - k);
+    if (v < best) { best = v; kBest = k; }
+  }
+  return best;
+}
+
+// Cost of a new binary node over children l, r with `cnt` triangles and half area `area`.
+__device__ __forceinline__ float4 nodeCost(float4 l, float4 r, int cnt, float area, const CollapseCost& cc)
+{
+  int k;
+  const float d2 = splitCost(l, r, 2, k), d3 = splitCost(l, r, 3, k), d4 = splitCost(l, r, 4, k);
+  const float inner = cc.node * area + d4;
+  const float leaf = cnt <= cc.leafMax ? cc.tri * float(cnt) * area : __int_as_float(0x7f800000);
+  const float c1 = fminf(leaf, inner);
+  return make_float4(c1, fminf(c1, d2), fminf(c1, d3), fminf(c1, d4));
+}
+
 __device__ __forceinline__ float mergedArea(float4 alo, float4 ahi, float4 blo, float4 bhi)
 {
   float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
@@ -195,7 +251,8 @@ __global__ void __launch_bounds__(kBlock) kCompact(const int* __restrict__ clus,
                                                    const unsigned long long* __restrict__ scan, int m, int nodeBase,
                                                    int* __restrict__ clus2, float4* __restrict__ clo2, float4* __restrict__ chi2,
                                                    int* __restrict__ ccnt2, int* __restrict__ childL, int* __restrict__ childR,
-                                                   int* __restrict__ nodeCnt, float* __restrict__ innerBox)
+                                                   int* __restrict__ nodeCnt, float* __restrict__ innerBox,
+                                                   const float* __restrict__ leafBox, float4* __restrict__ cost, CollapseCost cc)
 {
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= m) return;
@@ -216,6 +273,9 @@ __global__ void __launch_bounds__(kBlock) kCompact(const int* __restrict__ clus,
     nodeCnt[id] = cnt;
     float* ib = innerBox + 6 * size_t(id);
     ib[0] = lo.x; ib[1] = lo.y; ib[2] = lo.z; ib[3] = hi.x; ib[4] = hi.y; ib[5] = hi.z;
+    // the children were made in earlier iterations, so their costs are final
+    cost[id] = nodeCost(subtreeCost(clus[i], cost, leafBox, cc.tri), subtreeCost(clus[j], cost, leafBox, cc.tri), cnt,
+                        boxHalfArea(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z), cc);
     clus2[p] = id;
     clo2[p] = lo;
     chi2[p] = hi;
@@ -284,15 +344,19 @@ __device__ void writeNode(prt_bvh4_node* dst, const Box* bb, const int* ref, int
 
 // Top-down collapse of the binary tree into 4-wide nodes, one level of the wide tree per
 // launch.  A frontier entry is (binary node, wide node, first triangle slot of its subtree).
-// Each wide node starts from the two children of its binary node and repeatedly opens the
-// child with the largest surface area until it has four children or none can be opened;
-// children covering <= PRT_LEAF_MAX triangles become leaves, and their triangles are written at
-// consecutive slots (subtree slots follow the children's order, so slots never overlap).
+// The wide node of binary node b takes the cut of b's subtree into at most four subtrees that
+// the cost C chose (C(b, 4) below b's own cost): a subtree x given j cuts stays whole when
+// C(x, j) = C(x, 1), else its cuts are split between its children as C's minimum splits them.
+// A whole subtree is a leaf when it is one triangle or its leaf cost is C(x, 1), else a wide
+// node; inner children get consecutive wide-node indices, and leaf triangles are written at
+// consecutive slots in child order (subtree slots follow the children's order, so they never
+// overlap).
 __global__ void __launch_bounds__(kBlock) kCollapse(const int4* __restrict__ frontier, int count, int4* __restrict__ next,
                                                     int* __restrict__ nextCount, int* __restrict__ nodeCounter,
                                                     const int* __restrict__ childL, const int* __restrict__ childR,
                                                     const int* __restrict__ nodeCnt, const float* __restrict__ leafBox,
-                                                    const float* __restrict__ innerBox, prt_bvh4_node* __restrict__ nodes,
+                                                    const float* __restrict__ innerBox, const float4* __restrict__ cost,
+                                                    CollapseCost cc, prt_bvh4_node* __restrict__ nodes,
                                                     const float* __restrict__ pos, const int* __restrict__ idx,
                                                     const int* __restrict__ vals, float4* __restrict__ tris)
 {
@@ -300,29 +364,41 @@ __global__ void __launch_bounds__(kBlock) kCollapse(const int4* __restrict__ fro
   if (f >= count) return;
   int4 e = frontier[f];
   int b = e.x, k = e.y, slot = e.z;
+  // cut b's subtree: (subtree, cuts) pairs on a stack, left child on top, so the subtrees come
+  // out in left-to-right order; the cuts on the stack and in codes[] add up to at most four
   int codes[4];
-  int n = 2;
-  codes[0] = childL[b];
-  codes[1] = childR[b];
-  auto openable = [&](int c) { return c >= 0 && nodeCnt[c] > PRT_LEAF_MAX; };
-  while (n < 4)
+  int n = 0;
+  int2 stk[4];
+  int sp = 0;
   {
-    int best = -1;
-    float bestA = -1.f;
-    for (int q = 0; q < n; ++q)
-      if (openable(codes[q]))
-      {
-        float a = halfArea(innerBox + 6 * codes[q]);
-        if (a > bestA) { bestA = a; best = q; }
-      }
-    if (best < 0) break;
-    int c = codes[best];
-    codes[best] = childL[c];
-    codes[n++] = childR[c];
+    int kl;
+    splitCost(subtreeCost(childL[b], cost, leafBox, cc.tri), subtreeCost(childR[b], cost, leafBox, cc.tri), 4, kl);
+    stk[sp++] = make_int2(childR[b], 4 - kl);
+    stk[sp++] = make_int2(childL[b], kl);
   }
+  while (sp > 0)
+  {
+    const int2 x = stk[--sp];
+    if (x.x < 0 || x.y == 1) { codes[n++] = x.x; continue; }
+    const float4 cx = cost[x.x];
+    if (costAt(cx, x.y) == cx.x) { codes[n++] = x.x; continue; }
+    int kl;
+    splitCost(subtreeCost(childL[x.x], cost, leafBox, cc.tri), subtreeCost(childR[x.x], cost, leafBox, cc.tri), x.y, kl);
+    stk[sp++] = make_int2(childR[x.x], x.y - kl);
+    stk[sp++] = make_int2(childL[x.x], kl);
+  }
+  // a whole inner subtree is a leaf when its leaf cost is its cost
+  auto isLeaf = [&](int c) {
+    if (c < 0) return true;
+    const int cnt = nodeCnt[c];
+    if (cnt > cc.leafMax) return false;
+    const float* ib = innerBox + 6 * size_t(c);
+    return cc.tri * float(cnt) * boxHalfArea(ib[0], ib[1], ib[2], ib[3], ib[4], ib[5]) == cost[c].x;
+  };
   // child boxes (padded) and references; inner children get consecutive wide-node indices
   Box bb[4];
   int ref[4];
+  bool leaf[4];
   int nInner = 0;
   for (int q = 0; q < n; ++q)
   {
@@ -330,7 +406,8 @@ __global__ void __launch_bounds__(kBlock) kCollapse(const int4* __restrict__ fro
     const float* src = c < 0 ? leafBox + 6 * size_t(~c) : innerBox + 6 * size_t(c);
     for (int a = 0; a < 3; ++a) { bb[q].lo[a] = src[a]; bb[q].hi[a] = src[3 + a]; }
     padBox(bb[q]);
-    if (openable(c)) ++nInner;
+    leaf[q] = isLeaf(c);
+    if (!leaf[q]) ++nInner;
   }
   const int base = nInner ? atomicAdd(nodeCounter, nInner) : 0;
   const int fbase = nInner ? atomicAdd(nextCount, nInner) : 0;
@@ -339,18 +416,18 @@ __global__ void __launch_bounds__(kBlock) kCollapse(const int4* __restrict__ fro
   {
     int c = codes[q];
     const int cnt = c < 0 ? 1 : nodeCnt[c];
-    if (!openable(c))
+    if (leaf[q])
     {
       ref[q] = leafCode(slot, cnt);
-      // gather the subtree's triangles (at most PRT_LEAF_MAX leaves, 2 PRT_LEAF_MAX - 1 nodes)
-      int stk[2 * PRT_LEAF_MAX];
-      int sp = 0, o = slot;
-      stk[sp++] = c;
-      while (sp > 0)
+      // gather the subtree's triangles (at most PRT_LEAF_CAP leaves, 2 PRT_LEAF_CAP - 1 nodes)
+      int tstk[2 * PRT_LEAF_CAP];
+      int tsp = 0, o = slot;
+      tstk[tsp++] = c;
+      while (tsp > 0)
       {
-        int x = stk[--sp];
+        int x = tstk[--tsp];
         if (x < 0) writeTri(pos, idx, vals, ~x, tris, o++);
-        else { stk[sp++] = childR[x]; stk[sp++] = childL[x]; }
+        else { tstk[tsp++] = childR[x]; tstk[tsp++] = childL[x]; }
       }
     }
     else
@@ -404,6 +481,8 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
   unsigned long long *flags = nullptr, *scan = nullptr;
   int4 *front0 = nullptr, *front1 = nullptr;
   float *leafBox = nullptr, *innerBox = nullptr;
+  float4* cost = nullptr;
+  const CollapseCost cc = collapseCost();
   void *temp = nullptr, *scanTemp = nullptr;
   size_t tempBytes = 0, scanBytes = 0;
   const int inner = n > 1 ? n - 1 : 1;
@@ -431,6 +510,7 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
   BVH_CHECK(hipMalloc(&counts, sizeof(int) * 4));
   BVH_CHECK(hipMalloc(&leafBox, sizeof(float) * 6 * size_t(n)));
   BVH_CHECK(hipMalloc(&innerBox, sizeof(float) * 6 * size_t(inner)));
+  BVH_CHECK(hipMalloc(&cost, sizeof(float4) * size_t(inner)));
   BVH_CHECK(hipMalloc(&out.nodes, sizeof(prt_bvh4_node) * inner));
   BVH_CHECK(hipMalloc(&out.tris, sizeof(float4) * 3 * size_t(n)));
 
@@ -458,7 +538,7 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
     kTotals<<<1, 1, 0, stream>>>(flags, scan, m, counts);
     kCompact<<<blocks(m), kBlock, 0, stream>>>(clus[cur], clo[cur], chi[cur], ccnt[cur], nn, flags, scan, m, nodeBase,
                                                clus[cur ^ 1], clo[cur ^ 1], chi[cur ^ 1], ccnt[cur ^ 1], childL, childR, nodeCnt,
-                                               innerBox);
+                                               innerBox, leafBox, cost, cc);
     BVH_CHECK(hipGetLastError());
     int h[2] = {0, 0};
     BVH_CHECK(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, stream));
@@ -492,7 +572,7 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
     {
       BVH_CHECK(hipMemsetAsync(counts + 1, 0, sizeof(int), stream));
       kCollapse<<<blocks(count), kBlock, 0, stream>>>(front0, count, front1, counts + 1, counts, childL, childR, nodeCnt, leafBox,
-                                                      innerBox, out.nodes, dPos, dIdx, vals2, out.tris);
+                                                      innerBox, cost, cc, out.nodes, dPos, dIdx, vals2, out.tris);
       BVH_CHECK(hipGetLastError());
       int h[2] = {0, 0};
       BVH_CHECK(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, stream));
@@ -510,7 +590,7 @@ hipError_t buildBvh(const float* dPos, const int* dIdx, int n, BvhData& out, hip
   hipFree(cent); hipFree(cbox); hipFree(keys); hipFree(keys2); hipFree(vals); hipFree(vals2);
   hipFree(childL); hipFree(childR); hipFree(nodeCnt); hipFree(nn); hipFree(flags); hipFree(scan);
   for (int q = 0; q < 2; ++q) { hipFree(clus[q]); hipFree(ccnt[q]); hipFree(clo[q]); hipFree(chi[q]); }
-  hipFree(front0); hipFree(front1); hipFree(counts); hipFree(leafBox); hipFree(innerBox); hipFree(temp); hipFree(scanTemp);
+  hipFree(front0); hipFree(front1); hipFree(counts); hipFree(leafBox); hipFree(innerBox); hipFree(cost); hipFree(temp); hipFree(scanTemp);
   auto t1 = std::chrono::high_resolution_clock::now();
   out.buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (3 * out.maxDepth >= PRT_MAX_STACK)  // a wide node pushes at most three children

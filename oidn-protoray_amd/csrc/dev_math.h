@@ -13,6 +13,43 @@
 
 namespace prt {
 
+// Loads and stores through a pointer known to address device (global) memory.  The kernels
+// receive their buffers inside a struct in device memory, so the compiler cannot infer the
+// address space of those pointers and would use flat instructions, which also count against
+// the LDS counter (every wait on a node load then waits on the traversal stack as well) and
+// take 64-bit VGPR addresses.
+template <class T>
+PRT_DEV T ldg(const T* p)
+{
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+PRT_DEV void stg(T* p, const T& v)
+{
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+// float4 / int4 go through the native vector types (copying HIP's vector structs would pass a
+// generic reference to their copy constructor).
+typedef float prtV4f __attribute__((ext_vector_type(4)));
+typedef int prtV4i __attribute__((ext_vector_type(4)));
+template <>
+PRT_DEV float4 ldg<float4>(const float4* p)
+{
+  const prtV4f v = *(const __attribute__((address_space(1))) prtV4f*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+template <>
+PRT_DEV int4 ldg<int4>(const int4* p)
+{
+  const prtV4i v = *(const __attribute__((address_space(1))) prtV4i*)p;
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+template <>
+PRT_DEV void stg<float4>(float4* p, const float4& v)
+{
+  *(__attribute__((address_space(1))) prtV4f*)p = prtV4f{v.x, v.y, v.z, v.w};
+}
+
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kPosMax = 3.402823466e38f;
 

@@ -155,10 +155,12 @@ struct prt_bvh4_node
   int32_t pad[4];
 };
 
-// Triangles per BVH leaf at most (2 measured best against 1, 3, 4 and 6).
+// Triangles per BVH leaf at most: the builder's cost model may make leaves up to PRT_LEAF_MAX;
+// a leaf reference encodes up to PRT_LEAF_CAP.
 #ifndef PRT_LEAF_MAX
 #define PRT_LEAF_MAX 2
 #endif
+#define PRT_LEAF_CAP 8
 // Traversal stack entries per lane kept in LDS (8 B each: node + entry distance) and resident
 // blocks of 256 lanes per CU for the persistent traversal kernels.  10 x 8 B x 256 = 20 KB per
 // block, 7 blocks per CU (140 KB of the 160 KB LDS, 7 waves per SIMD): measured best on the

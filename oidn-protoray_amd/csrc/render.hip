@@ -133,18 +133,18 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __re
   const int* q = a.queue[depth & 1];
   uint32_t nodes = 0, tris = 0, rays = 0, waveSteps = 0;
   auto one = [&](int i) {
-    int slot = depth == 0 ? i : q[i];
-    float4 ro = a.rayO[slot], rd = a.rayD[slot];
+    int slot = depth == 0 ? i : ldg(q + i);
+    float4 ro = ldg(a.rayO + slot), rd = ldg(a.rayD + slot);
     float t, u = 0.f, v = 0.f;
     int prim;
     if (kInst)
     {
       int inst = -1;
       prim = traceRayInst<false>(a.inst, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, inst, st, nodes, tris);
-      a.hitInst[slot] = inst;
+      stg(a.hitInst + slot, inst);
     }
     else prim = traceRay<false>(a.bvh, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, st, nodes, tris);
-    a.hit[slot] = make_float4(t, u, v, __int_as_float(prim));
+    stg(a.hit + slot, make_float4(t, u, v, __int_as_float(prim)));
     ++rays;
   };
   const int stride = gridDim.x * blockDim.x;
@@ -402,8 +402,8 @@ template <int kFilter, bool kInst>
 PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& nodes, uint32_t& tris, uint32_t& rays, bool classB)
 {
   {
-    int slot = a.shadowQ[i];
-    float4 so = a.shO[slot], sd = a.shD[slot];
+    int slot = ldg(a.shadowQ + i);
+    float4 so = ldg(a.shO + slot), sd = ldg(a.shD + slot);
     float t, u, v;
     const f3 o(so.x, so.y, so.z), d(sd.x, sd.y, sd.z);
     int prim;
@@ -422,13 +422,13 @@ PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& n
       prim = kInst ? traceRayInst<true>(a.inst, o, d, so.w, sd.w, t, u, v, inst, st, nodes, tris)
                    : traceRay<true>(a.bvh, o, d, so.w, sd.w, t, u, v, st, nodes, tris);
     ++rays;
-    float4 lr = a.Lr[slot];
+    float4 lr = ldg(a.Lr + slot);
     f3 L(lr.x, lr.y, lr.z);
     if (prim < 0)  // unoccluded: add the direct light times the shadow transmittance (:991-1002)
     {
-      float4 dl = a.dirL[slot];
+      float4 dl = ldg(a.dirL + slot);
       L = L + (kFilter ? f3(dl.x, dl.y, dl.z) * Tsh : f3(dl.x, dl.y, dl.z));
-      a.Lr[slot] = make_float4(L.x, L.y, L.z, 0.f);
+      stg(a.Lr + slot, make_float4(L.x, L.y, L.z, 0.f));
     }
     uint32_t hsh = 0u;
     if (a.hash)  // the visibility event is part of the path's identity

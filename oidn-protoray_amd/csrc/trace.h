@@ -42,8 +42,8 @@ PRT_DEV bool intersectTriangle(float4 a, float4 b, float4 c, f3 o, f3 d, float t
 PRT_DEV void boxQuad(const prt_bvh4_node* nodes, int n, f3 o, f3 id, float tnear, float tfar, float t[4], int c[4])
 {
   const float4* p = reinterpret_cast<const float4*>(nodes + n);
-  float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-  int4 ch = reinterpret_cast<const int4*>(p)[6];
+  float4 lx = ldg(p), hx = ldg(p + 1), ly = ldg(p + 2), hy = ldg(p + 3), lz = ldg(p + 4), hz = ldg(p + 5);
+  int4 ch = ldg(reinterpret_cast<const int4*>(p) + 6);
   const float g = 1.0000004f;
   const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
   const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
@@ -87,8 +87,8 @@ struct TraceStack
     if (sp < PRT_LDS_STACK) { lds[sp * 64] = v; ldsT[sp * 64] = t; }
     else if (sp < PRT_LDS_STACK + cap)
     {
-      spill[(sp - PRT_LDS_STACK) * stride] = v;
-      spill[(sp - PRT_LDS_STACK + cap) * stride] = __float_as_int(t);
+      stg(spill + (sp - PRT_LDS_STACK) * stride, v);
+      stg(spill + (sp - PRT_LDS_STACK + cap) * stride, __float_as_int(t));
     }
     else return;
     ++sp;
@@ -97,8 +97,8 @@ struct TraceStack
   {
     --sp;
     if (sp < PRT_LDS_STACK) { t = ldsT[sp * 64]; return lds[sp * 64]; }
-    t = __int_as_float(spill[(sp - PRT_LDS_STACK + cap) * stride]);
-    return spill[(sp - PRT_LDS_STACK) * stride];
+    t = __int_as_float(ldg(spill + (sp - PRT_LDS_STACK + cap) * stride));
+    return ldg(spill + (sp - PRT_LDS_STACK) * stride);
   }
 };
 
@@ -169,6 +169,15 @@ PRT_DEV f3 xfmRows(const float* m, f3 v)
   return f3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z);
 }
 
+// One traversal step per loop iteration: a lane at an inner node visits it, a lane at a leaf
+// tests its triangles, and a lane that has nothing to descend into pops, all in the same
+// iteration.  Written with `continue` after a descent, the loop has two back edges, and the
+// compiler turns it into a descent loop nested in the leaf loop: lanes that reach a leaf then
+// idle until every lane of the wave has reached one (measured: closest-hit traversal 12 %
+// slower, shadow rays 15 %).  The empty asm hides the flag's value from jump threading, which
+// would otherwise restore the second back edge.
+PRT_DEV void keepOneLoop(int& popNext) { __asm__ volatile("" : "+v"(popNext)); }
+
 // Closest hit (kAnyHit = false) or any hit.  Ties in t go to the smaller primitive ID, so the
 // result does not depend on the traversal order.  Returns the primitive ID or -1; nodeVisits
 // counts wide-node fetches.  An any-hit traversal offers each hit in [tnear, tfar] to
@@ -186,6 +195,7 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
   for (;;)
   {
     ++st.steps;
+    int popNext = 1;
     if (node >= 0)
     {
       float t[4];
@@ -204,7 +214,7 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
         if (t[2] != inf) st.push(c[2], t[2]);
         if (t[1] != inf) st.push(c[1], t[1]);
         node = c[0];
-        continue;
+        popNext = 0;
       }
     }
     else
@@ -214,7 +224,7 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
       for (int k = 0; k < count; ++k)
       {
         const float4* tp = bvh.tris + 3 * (first + k);
-        float4 a = tp[0], b = tp[1], cc = tp[2];
+        float4 a = ldg(tp), b = ldg(tp + 1), cc = ldg(tp + 2);
         float th, u, v;
         ++triTests;
         if (intersectTriangle(a, b, cc, o, d, tnear, tfar, th, u, v))
@@ -234,12 +244,16 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
     }
     // next node: skip entries that start beyond the closest hit found so far (an entry at
     // exactly tfar may still hold a tie with a smaller primitive ID)
-    float tEntry;
-    do
+    keepOneLoop(popNext);
+    if (popNext)
     {
-      if (st.sp == 0) { tHit = tfar; return best; }
-      node = st.pop(tEntry);
-    } while (!kAnyHit && tEntry > tfar);
+      float tEntry;
+      do
+      {
+        if (st.sp == 0) { tHit = tfar; return best; }
+        node = st.pop(tEntry);
+      } while (!kAnyHit && tEntry > tfar);
+    }
   }
 }
 
@@ -269,6 +283,7 @@ PRT_DEV int traceRayInst(const InstView& iv, f3 wo, f3 wd, float tnear, float tf
   for (;;)
   {
     ++st.steps;
+    int popNext = 1;
     if (node == kLeaveInstance)
     {
       o = wo; d = wd; id = wid;
@@ -292,7 +307,7 @@ PRT_DEV int traceRayInst(const InstView& iv, f3 wo, f3 wd, float tnear, float tf
         if (t[2] != inf) st.push(c[2], t[2]);
         if (t[1] != inf) st.push(c[1], t[1]);
         node = c[0];
-        continue;
+        popNext = 0;
       }
     }
     else if (inst < 0)  // a top-level leaf: enter its instance
@@ -308,7 +323,7 @@ PRT_DEV int traceRayInst(const InstView& iv, f3 wo, f3 wd, float tnear, float tf
       tris = M.tris;
       triBegin = M.triBegin;
       node = M.root;
-      continue;
+      popNext = 0;
     }
     else
     {
@@ -317,7 +332,7 @@ PRT_DEV int traceRayInst(const InstView& iv, f3 wo, f3 wd, float tnear, float tf
       for (int k = 0; k < count; ++k)
       {
         const float4* tp = tris + 3 * (first + k);
-        float4 a = tp[0], b = tp[1], cc = tp[2];
+        float4 a = ldg(tp), b = ldg(tp + 1), cc = ldg(tp + 2);
         float th, u, v;
         ++triTests;
         if (intersectTriangle(a, b, cc, o, d, tnear, tfar, th, u, v))
@@ -335,12 +350,16 @@ PRT_DEV int traceRayInst(const InstView& iv, f3 wo, f3 wd, float tnear, float tf
         }
       }
     }
-    float tEntry;
-    do
+    keepOneLoop(popNext);
+    if (popNext)
     {
-      if (st.sp == 0) { tHit = tfar; instHit = bestInst; return best; }
-      node = st.pop(tEntry);
-    } while (!kAnyHit && tEntry > tfar);
+      float tEntry;
+      do
+      {
+        if (st.sp == 0) { tHit = tfar; instHit = bestInst; return best; }
+        node = st.pop(tEntry);
+      } while (!kAnyHit && tEntry > tfar);
+    }
   }
 }
 
