@@ -162,11 +162,16 @@ struct prt_bvh4_node
 #endif
 #define PRT_LEAF_CAP 8
 // Traversal stack entries per lane kept in LDS (8 B each: node + entry distance) and resident
-// blocks of 256 lanes per CU for the persistent traversal kernels.  10 x 8 B x 256 = 20 KB per
-// block, 7 blocks per CU (140 KB of the 160 KB LDS, 7 waves per SIMD): measured best on the
-// 1080p workload against (16, 4), (12, 5), (12, 6), (9, 8) and (8, 8).
+// blocks of 256 lanes per CU for the persistent traversal kernels.  11 x 8 B x 256 = 22 KB per
+// block, 7 blocks per CU (154 KB of the 160 KB LDS, 7 waves per SIMD): measured best on the
+// 1080p workload against (10, 7), (16, 4), (12, 5), (12, 6), (9, 8) and (8, 8).
 #ifndef PRT_LDS_STACK
-#define PRT_LDS_STACK 10
+#define PRT_LDS_STACK 11
+#endif
+// The shadow-ray kernel's stacks hold nodes only (any-hit traversal never reads the entry
+// distances): 24 x 4 B x 256 = 24 KB per block at its 6 blocks per CU.
+#ifndef PRT_LDS_STACK_ANYHIT
+#define PRT_LDS_STACK_ANYHIT 24
 #endif
 #define PRT_MAX_STACK 128
 // Path vertices the adjoint records per lane (the adjoint requires maxDepth <= this).

@@ -143,7 +143,8 @@ __global__ void __launch_bounds__(kTraceBlock) kIntersect(const RenderArgs* __re
       prim = traceRayInst<false>(a.inst, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, inst, st, nodes, tris);
       stg(a.hitInst + slot, inst);
     }
-    else prim = traceRay<false>(a.bvh, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, st, nodes, tris);
+    else
+      prim = traceRay<false>(a.bvh, f3(ro.x, ro.y, ro.z), f3(rd.x, rd.y, rd.z), ro.w, kPosMax, t, u, v, st, nodes, tris);
     stg(a.hit + slot, make_float4(t, u, v, __int_as_float(prim)));
     ++rays;
   };
@@ -399,7 +400,7 @@ kShade(const RenderArgs* __restrict__ ap, int depth, uint32_t sampleIndex, bool 
 // One shadow ray of the queue (path_renderer_stream.h:951-1065).  kFilter: 0 opaque scene, 1
 // transparent materials, 2 transparent materials and kind-5 materials; kInst: a two-level scene.
 template <int kFilter, bool kInst>
-PRT_DEV void occludedRay(const RenderArgs& a, int i, TraceStack& st, uint32_t& nodes, uint32_t& tris, uint32_t& rays, bool classB)
+PRT_DEV void occludedRay(const RenderArgs& a, int i, ShadowStack& st, uint32_t& nodes, uint32_t& tris, uint32_t& rays, bool classB)
 {
   {
     int slot = ldg(a.shadowQ + i);
@@ -448,9 +449,9 @@ template <int kFilter, bool kCount, bool kInst>
 __global__ void __launch_bounds__(kTraceBlock) kOccluded(const RenderArgs* __restrict__ ap, int depth, bool classB)
 {
   const RenderArgs& a = *ap;
-  __shared__ TraceLds<kTraceBlock> lds;
+  __shared__ ShadowLds<kTraceBlock> lds;
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  TraceStack st = makeStack(lds, a.spill, a.spillStride, gtid);
+  ShadowStack st = makeShadowStack(lds, a.spill, a.spillStride, gtid);
   const int count = a.counters[2 * depth + 1];
   uint32_t nodes = 0, tris = 0, rays = 0, waveSteps = 0;
   const int stride = gridDim.x * blockDim.x;

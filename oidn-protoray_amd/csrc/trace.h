@@ -39,6 +39,9 @@ PRT_DEV bool intersectTriangle(float4 a, float4 b, float4 c, f3 o, f3 d, float t
 // Slab tests of the four children of a wide node.  The far distance is widened by
 // 1 + 2*gamma(3) and the boxes are padded at build time, so rounding never culls a hit the
 // triangle test reports.  Returns the entry distance per child (+inf where missed or empty).
+// kTwoPlus: the node has at least two children (every node of the device builder's trees; a
+// host-built top-level node over one instance has one), so slots 0 and 1 are never empty.
+template <bool kTwoPlus = false>
 PRT_DEV void boxQuad(const prt_bvh4_node* nodes, int n, f3 o, f3 id, float tnear, float tfar, float t[4], int c[4])
 {
   const float4* p = reinterpret_cast<const float4*>(nodes + n);
@@ -57,7 +60,7 @@ PRT_DEV void boxQuad(const prt_bvh4_node* nodes, int n, f3 o, f3 id, float tnear
     float z0 = (lzs[q] - o.z) * id.z, z1 = (hzs[q] - o.z) * id.z;
     float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tnear));
     float tf = fminf(fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * g, tfar);
-    t[q] = (tn <= tf && c[q] != -1) ? tn : __int_as_float(0x7f800000);
+    t[q] = (tn <= tf && ((kTwoPlus && q < 2) || c[q] != -1)) ? tn : __int_as_float(0x7f800000);
   }
 }
 
@@ -66,14 +69,18 @@ PRT_DEV void cswap(float& ta, int& ca, float& tb, int& cb)
   if (tb < ta) { float t = ta; ta = tb; tb = t; int c = ca; ca = cb; cb = c; }
 }
 
-// Per-lane traversal stack of (node, entry distance): the first PRT_LDS_STACK entries in LDS
-// (two lane-interleaved columns, so a wave's accesses hit 64 distinct banks), deeper entries in
-// the lane's overflow column in HBM (nodes in rows [0, cap), the distances in the rows after
-// them; cap = PRT_MAX_STACK - PRT_LDS_STACK except for the static-scratch query).
-struct TraceStack
+// Per-lane traversal stack of (node, entry distance): the first kEntries entries in LDS (two
+// lane-interleaved columns, so a wave's accesses hit 64 distinct banks), deeper entries in the
+// lane's overflow column in HBM (nodes in rows [0, cap), the distances in the rows after them;
+// cap = PRT_MAX_STACK - kEntries except for the static-scratch query).  kDist = false: a stack
+// for any-hit traversal, which never reads the entry distances, so only nodes are kept and the
+// same LDS holds more entries.
+template <int kEntries, bool kDist>
+struct TraceStackT
 {
+  static constexpr bool kHasDist = kDist;
   int* lds;       // this lane's node column: lds[k * 64]
-  float* ldsT;    // this lane's distance column: ldsT[k * 64]
+  float* ldsT;    // this lane's distance column: ldsT[k * 64] (kDist)
   int* spill;     // this lane's overflow column: spill[k * stride]
   int stride;
   int cap;        // overflow entries per lane: nodes in rows [0, cap), distances in [cap, 2 cap)
@@ -84,11 +91,15 @@ struct TraceStack
   // the overflow area.
   PRT_DEV void push(int v, float t)
   {
-    if (sp < PRT_LDS_STACK) { lds[sp * 64] = v; ldsT[sp * 64] = t; }
-    else if (sp < PRT_LDS_STACK + cap)
+    if (sp < kEntries)
     {
-      stg(spill + (sp - PRT_LDS_STACK) * stride, v);
-      stg(spill + (sp - PRT_LDS_STACK + cap) * stride, __float_as_int(t));
+      lds[sp * 64] = v;
+      if (kDist) ldsT[sp * 64] = t;
+    }
+    else if (sp < kEntries + cap)
+    {
+      stg(spill + (sp - kEntries) * stride, v);
+      if (kDist) stg(spill + (sp - kEntries + cap) * stride, __float_as_int(t));
     }
     else return;
     ++sp;
@@ -96,11 +107,17 @@ struct TraceStack
   PRT_DEV int pop(float& t)
   {
     --sp;
-    if (sp < PRT_LDS_STACK) { t = ldsT[sp * 64]; return lds[sp * 64]; }
-    t = __int_as_float(ldg(spill + (sp - PRT_LDS_STACK + cap) * stride));
-    return ldg(spill + (sp - PRT_LDS_STACK) * stride);
+    if (sp < kEntries)
+    {
+      t = kDist ? ldsT[sp * 64] : 0.f;
+      return lds[sp * 64];
+    }
+    t = kDist ? __int_as_float(ldg(spill + (sp - kEntries + cap) * stride)) : 0.f;
+    return ldg(spill + (sp - kEntries) * stride);
   }
 };
+using TraceStack = TraceStackT<PRT_LDS_STACK, true>;
+using ShadowStack = TraceStackT<PRT_LDS_STACK_ANYHIT, false>;
 
 // LDS a block of kThreads lanes needs for its stacks (nodes then distances).
 template <int kThreads>
@@ -108,6 +125,11 @@ struct TraceLds
 {
   int node[PRT_LDS_STACK * kThreads];
   float dist[PRT_LDS_STACK * kThreads];
+};
+template <int kThreads>
+struct ShadowLds
+{
+  int node[PRT_LDS_STACK_ANYHIT * kThreads];
 };
 // This lane's stack over the block's LDS and the overflow area (gtid-th column of spill).
 template <int kThreads>
@@ -121,6 +143,20 @@ PRT_DEV TraceStack makeStack(TraceLds<kThreads>& l, int* spill, int stride, int 
   st.spill = spill + gtid;
   st.stride = stride;
   st.cap = cap;
+  st.sp = 0;
+  st.steps = 0;
+  return st;
+}
+template <int kThreads>
+PRT_DEV ShadowStack makeShadowStack(ShadowLds<kThreads>& l, int* spill, int stride, int gtid)
+{
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  ShadowStack st;
+  st.lds = l.node + wave * (PRT_LDS_STACK_ANYHIT * 64) + lane;
+  st.ldsT = nullptr;
+  st.spill = spill + gtid;
+  st.stride = stride;
+  st.cap = PRT_MAX_STACK - PRT_LDS_STACK_ANYHIT;
   st.sp = 0;
   st.steps = 0;
   return st;
@@ -183,10 +219,11 @@ PRT_DEV void keepOneLoop(int& popNext) { __asm__ volatile("" : "+v"(popNext)); }
 // counts wide-node fetches.  An any-hit traversal offers each hit in [tnear, tfar] to
 // filter(prim, t, u, v), which accepts it (the ray is occluded) or lets the ray go on, as
 // Embree's filter callbacks do (accel/embree_intersector.cpp:154-215).
-template <bool kAnyHit, class Filter = AcceptHit>
+template <bool kAnyHit, class Filter = AcceptHit, class Stack = TraceStack>
 PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, float& tHit, float& uHit, float& vHit,
-                     TraceStack& st, uint32_t& nodeVisits, uint32_t& triTests, Filter&& filter = Filter())
+                     Stack& st, uint32_t& nodeVisits, uint32_t& triTests, Filter&& filter = Filter())
 {
+  static_assert(kAnyHit || Stack::kHasDist, "closest-hit traversal needs the entry distances");
   f3 id(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   int best = -1;
   st.sp = 0;
@@ -201,7 +238,7 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
       float t[4];
       int c[4];
       ++nodeVisits;
-      boxQuad(bvh.nodes, node, o, id, tnear, tfar, t, c);
+      boxQuad<true>(bvh.nodes, node, o, id, tnear, tfar, t, c);
       // closest first: sorting network on (t, child), misses sort to the end as +inf
       cswap(t[0], c[0], t[1], c[1]);
       cswap(t[2], c[2], t[3], c[3]);
@@ -267,10 +304,11 @@ constexpr int kLeaveInstance = 0x7ffffffe;
 // renormalised, as in Embree's instance traversal) and the mesh's tree is traversed with the
 // same stack above a marker; popping the marker restores the world ray.  Ties in t go to the
 // smaller (instance, triangle) pair.  Returns the triangle (scene index) or -1, and its instance.
-template <bool kAnyHit, class Filter = AcceptHit>
+template <bool kAnyHit, class Filter = AcceptHit, class Stack = TraceStack>
 PRT_DEV int traceRayInst(const InstView& iv, f3 wo, f3 wd, float tnear, float tfar, float& tHit, float& uHit, float& vHit,
-                         int& instHit, TraceStack& st, uint32_t& nodeVisits, uint32_t& triTests, Filter&& filter = Filter())
+                         int& instHit, Stack& st, uint32_t& nodeVisits, uint32_t& triTests, Filter&& filter = Filter())
 {
+  static_assert(kAnyHit || Stack::kHasDist, "closest-hit traversal needs the entry distances");
   const f3 wid(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
   f3 o = wo, d = wd, id = wid;
   int best = -1, bestInst = -1;
