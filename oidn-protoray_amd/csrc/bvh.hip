@@ -97,11 +97,13 @@ __global__ void __launch_bounds__(kBlock) kMorton(const float4* __restrict__ cen
 
 struct Box { float lo[3], hi[3]; };
 
-// Neighbour window of the clustering.  Measured on the 1080p workload (4096 icospheres):
-// radius 2 / 3 / 4 / 6 / 8 / 16 / 32 -> 110.3 / 112.8 / 112.3 / 111.2 / 109.5 / 107.0 / 97.9
-// Msamples/s (22.4 / 21.3 / 21.5 / 21.9 / 22.7 / 24.4 / 26.9 node fetches per ray).
+// Neighbour window of the clustering.  Measured on the 1080p workload (4096 icospheres) with
+// the greedy collapse: radius 2 / 3 / 4 / 6 / 8 / 16 / 32 -> 110.3 / 112.8 / 112.3 / 111.2 /
+// 109.5 / 107.0 / 97.9 Msamples/s (22.4 / 21.3 / 21.5 / 21.9 / 22.7 / 24.4 / 26.9 node fetches
+// per ray); with the cost-driven collapse, radius 3 / 4 / 6 -> closest-hit traversal 6.31 /
+// 6.36 / 6.39 ms per pass (21.1 / 20.7 / 21.0 node fetches per ray).
 #ifndef PRT_PLOC_RADIUS
-#define PRT_PLOC_RADIUS 4
+#define PRT_PLOC_RADIUS 3
 #endif
 constexpr int kRadius = PRT_PLOC_RADIUS;
 
