@@ -385,7 +385,9 @@ int prt_query_rays_device_noalloc(prt_context* ctx, int32_t n, const float* rays
  * sheen layer attenuates the derivatives of the lobe under it; opacity scales value and
  * derivatives alike.  The gradient is that of the
  * primal estimator with its samples fixed (detached sampling); paths are replayed from the
- * seed, not stored.  Requires maxDepth <= 8.  *ms: device time of the replay kernels. */
+ * seed, not stored; a path deeper than 8 vertices is replayed once more per further 8 depths
+ * (the records of its vertices are reduced 8 depths at a time).  *ms: device time of the replay
+ * kernels. */
 int prt_adjoint(prt_context* ctx, const float* target, int32_t spp, double* loss, float* grad, int32_t grad_capacity,
                 double* ms);
 /* Replays sample sample_index of every pixel as the adjoint does and returns its radiance

@@ -38,22 +38,35 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
   TraceStack st = makeStack(lds, a.spill, a.spillStride, blockIdx.x * kAdjBlock + threadIdx.x);
   float* out = partial + size_t(blockIdx.x) * nParams;
   uint32_t nodes = 0, tris = 0;
+  // path vertices are recorded PRT_ADJ_MAX_VERTICES depths at a time: a deeper path is replayed
+  // once more per further group of depths (shading happens at depths 0 .. maxDepth - 1)
+  const int groups = max(1, (a.maxDepth + PRT_ADJ_MAX_VERTICES - 1) / PRT_ADJ_MAX_VERTICES);
   for (int base = blockIdx.x * kAdjBlock; base < a.npix; base += gridDim.x * kAdjBlock)
   {
   const int slot = base + threadIdx.x;
-  for (int v = 0; v < PRT_ADJ_MAX_VERTICES; ++v) recMat[v][threadIdx.x] = -1;
+  int x = 0, y = 0, pix = 0;
+  f3 dl(0.f), Ltotal(0.f);
   if (slot < a.npix)
   {
-    int x, y;
     slotToPixel(slot, a.W, x, y);
-    const int pix = y * a.W + x;
-    const f3 dl(delta[pix] * invSpp, delta[a.npix + pix] * invSpp, delta[2 * a.npix + pix] * invSpp);
-    // pass 1 recovers the sample's radiance; pass 2 replays the same path with the gradient hook
-    const f3 Ltotal = tracePathMega<true>(a, st, x, y, sampleIndex, nullptr, NoVertexHook(), nodes, tris);
-    if (nParams > 0)
+    pix = y * a.W + x;
+    dl = f3(delta[pix] * invSpp, delta[a.npix + pix] * invSpp, delta[2 * a.npix + pix] * invSpp);
+    // the first replay recovers the sample's radiance; the next ones replay the same path with
+    // the gradient hook
+    Ltotal = tracePathMega<true>(a, st, x, y, sampleIndex, nullptr, NoVertexHook(), nodes, tris);
+    if (replayL)
+    {
+      replayL[pix] = Ltotal.x; replayL[a.npix + pix] = Ltotal.y; replayL[2 * a.npix + pix] = Ltotal.z;
+    }
+  }
+  for (int group = 0; group < groups && nParams > 0; ++group)
+  {
+    const int v0 = group * PRT_ADJ_MAX_VERTICES;
+    for (int v = 0; v < PRT_ADJ_MAX_VERTICES; ++v) recMat[v][threadIdx.x] = -1;
+    if (slot < a.npix)
     {
       auto hook = [&](const PathVertex& pv) {
-        if (pv.depth >= PRT_ADJ_MAX_VERTICES) return;
+        if (pv.depth < v0 || pv.depth >= v0 + PRT_ADJ_MAX_VERTICES) return;
         float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
         auto add = [&](f3 dir, f3 radiance) {
           BsdfGrad g = bsdfEvalGrad(a.lut, *pv.b, *pv.m, *pv.ctx, pv.wo, dir);
@@ -66,32 +79,28 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
         // everything gathered after it, through a smooth lobe (a delta or continuation lobe's
         // weight holds no differentiated parameter)
         if (pv.ext && !(pv.type & BsdfType::Delta)) add(pv.wi, Ltotal - pv.L);
-        recMat[pv.depth][threadIdx.x] = pv.matId;
-        recG[pv.depth][threadIdx.x] = gv;
+        recMat[pv.depth - v0][threadIdx.x] = pv.matId;
+        recG[pv.depth - v0][threadIdx.x] = gv;
       };
       tracePathMega<true>(a, st, x, y, sampleIndex, nullptr, hook, nodes, tris);
     }
-    if (replayL)
+    __syncthreads();
+    // block reduction in a fixed order: parameter p = 4 * material + (r, g, b, alpha)
+    for (int p = threadIdx.x; p < nParams; p += kAdjBlock)
     {
-      replayL[pix] = Ltotal.x; replayL[a.npix + pix] = Ltotal.y; replayL[2 * a.npix + pix] = Ltotal.z;
+      const int mat = p >> 2, comp = p & 3;
+      float sum = 0.f;
+      for (int v = 0; v < PRT_ADJ_MAX_VERTICES; ++v)
+        for (int t = 0; t < kAdjBlock; ++t)
+          if (recMat[v][t] == mat)
+          {
+            float4 g = recG[v][t];
+            sum += comp == 0 ? g.x : comp == 1 ? g.y : comp == 2 ? g.z : g.w;
+          }
+      out[p] += sum;
     }
+    __syncthreads();
   }
-  __syncthreads();
-  // block reduction in a fixed order: parameter p = 4 * material + (r, g, b, alpha)
-  for (int p = threadIdx.x; p < nParams; p += kAdjBlock)
-  {
-    const int mat = p >> 2, comp = p & 3;
-    float sum = 0.f;
-    for (int v = 0; v < PRT_ADJ_MAX_VERTICES; ++v)
-      for (int t = 0; t < kAdjBlock; ++t)
-        if (recMat[v][t] == mat)
-        {
-          float4 g = recG[v][t];
-          sum += comp == 0 ? g.x : comp == 1 ? g.y : comp == 2 ? g.z : g.w;
-        }
-    out[p] += sum;
-  }
-  __syncthreads();
   }
 }
 

@@ -1934,8 +1934,6 @@ int prt_adjoint(prt_context* c, const float* target, int32_t spp, double* loss, 
   CTX_CHECK(c);
   if (!target || !grad) return fail(PRT_ERR_INVALID, "target and grad are required");
   if (spp < 1) return fail(PRT_ERR_INVALID, "spp must be >= 1");
-  if (c->params.maxDepth > PRT_ADJ_MAX_VERTICES)
-    return fail(PRT_ERR_INVALID, "the adjoint supports maxDepth <= " + std::to_string(PRT_ADJ_MAX_VERTICES));
   if (c->textured) return fail(PRT_ERR_INVALID, "the adjoint does not differentiate textured materials");
   if (c->instanced()) return fail(PRT_ERR_INVALID, "the adjoint takes one-level scenes");
   const int nMat = int(c->materials.size());
@@ -1987,8 +1985,6 @@ int prt_debug_adjoint_replay(prt_context* c, uint32_t sample_index, float* radia
   CTX_CHECK(c);
   if (!radiance) return fail(PRT_ERR_INVALID, "radiance is null");
   if (!c->npix) return fail(PRT_ERR_STATE, "frame not initialized");
-  if (c->params.maxDepth > PRT_ADJ_MAX_VERTICES)
-    return fail(PRT_ERR_INVALID, "the adjoint supports maxDepth <= " + std::to_string(PRT_ADJ_MAX_VERTICES));
   // the kernel arguments of the last prt_render call are on the device
   if (c->count == 0) return fail(PRT_ERR_STATE, "render at least one pass first");
   const size_t n = size_t(c->npix);

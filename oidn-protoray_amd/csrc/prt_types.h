@@ -174,7 +174,8 @@ struct prt_bvh4_node
 #define PRT_LDS_STACK_ANYHIT 24
 #endif
 #define PRT_MAX_STACK 128
-// Path vertices the adjoint records per lane (the adjoint requires maxDepth <= this).
+// Path vertices the adjoint records per lane and replay (deeper paths are replayed once more per
+// further group of this many depths).
 #define PRT_ADJ_MAX_VERTICES 8
 #ifndef PRT_TRACE_BLOCKS_PER_CU
 #define PRT_TRACE_BLOCKS_PER_CU 7

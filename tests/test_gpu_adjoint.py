@@ -87,6 +87,66 @@ def test_gradient_matches_fp64_oracle(dev):
     assert np.all(err <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
 
 
+def open_room(albedo=(0.9, 0.85, 0.8)):
+    """A box [-2, 2] x [-2, 2] x [1, 9] open towards the pinhole camera at the origin (its face at
+    z = 1 missing), coated walls (material 1) and floor, ceiling and back (material 2) of high
+    albedo, lit by a unit ambient light seen only through the opening: most paths bounce many
+    times before they leave."""
+    x0, x1, y0, y1, z0, z1 = -2.0, 2.0, -2.0, 2.0, 1.0, 9.0
+    quads = [[(x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1)],   # floor
+             [(x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0)],   # ceiling
+             [(x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)],   # back
+             [(x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0)],   # left
+             [(x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1)]]   # right
+    pos, idx, tm = [], [], []
+    for qi, q in enumerate(quads):
+        b = len(pos)
+        pos += q
+        idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+        tm += [2 if qi < 3 else 1] * 2
+    mats = np.zeros(3, G.MATERIAL_DTYPE)
+    for i, rough in ((1, 0.45), (2, 0.3)):
+        mats[i]["kind"] = G.KIND_COATED
+        mats[i]["r"], mats[i]["g"], mats[i]["b"] = albedo
+        mats[i]["er"], mats[i]["eg"], mats[i]["eb"] = (1.0, 1.0, 1.0)
+        mats[i]["roughness"], mats[i]["ior"], mats[i]["thickness"], mats[i]["weight"] = rough, 1.5, 1.0, 1.0
+    lights = np.zeros(1, G.LIGHT_DTYPE)
+    lights[0] = (G.LIGHT_AMBIENT, 1.0, 1.0, 1.0, 0, 0, 0, 0)
+    return dict(positions=np.array(pos, np.float32), indices=np.array(idx, np.int32),
+                tri_material=np.array(tm, np.int32), materials=mats, lights=lights, env=None,
+                camera=G._camera(fov_deg=60.0))
+
+
+def test_gradient_beyond_eight_vertices_matches_fp64_oracle(dev):
+    # the kernel records path vertices eight depths at a time, so at depth 12 the vertices at
+    # depths 8..11 come from a second hook replay and a second block reduction; in the open room
+    # most paths are still bouncing at depth 8
+    W, H, spp = 64, 64, 4
+    sc = open_room()
+    target_mats = G.perturbed_materials(sc["materials"])
+    osc = O.Scene(sc["positions"], sc["indices"], sc["tri_material"], sc["materials"])
+    osc.set_lights(sc["lights"])
+    res = {}
+    for depth in (8, 12):
+        _load(dev, sc, target_mats, max_depth=depth)
+        target = _render(dev, W, H, spp)
+        _load(dev, sc, max_depth=depth)
+        loss, grad, _ = dev.adjoint(target.reshape(H, W, 3), spp, sc["materials"].shape[0])
+        oloss, ograd = osc.adjoint(sc["camera"], W, H, target, spp, max_depth=depth)
+        print("depth", depth, "loss", loss, oloss)
+        assert abs(loss - oloss) <= 1e-4 * oloss
+        scale = np.abs(ograd).max(axis=0)
+        assert np.all(np.abs(grad - ograd) <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
+        res[depth] = (grad, ograd)
+    # the deeper vertices change the gradient, and the change is the oracle's change
+    dg, dog = res[12][0] - res[8][0], res[12][1] - res[8][1]
+    print("depth 12 - depth 8, oracle: max |change| per (r, g, b, alpha):", np.abs(dog).max(axis=0))
+    print("GPU change - oracle change:", np.abs(dg - dog).max(axis=0))
+    print("scale:", np.abs(res[12][1]).max(axis=0))
+    assert np.all(np.abs(dog).max(axis=0) > 0.02 * np.abs(res[12][1]).max(axis=0))
+    assert np.all(np.abs(dg - dog) <= 0.05 * np.abs(dog).max(axis=0))
+
+
 def test_gradient_matches_fp64_oracle_conductor_dielectric(dev):
     # coated, rough conductor and rough dielectric materials in turn: reflectivity and alpha of
     # the conductors, alpha of the dielectrics (refraction into the spheres and out again)
