@@ -17,8 +17,9 @@ namespace {
 
 constexpr int kBatchBlock = kBatchBlockSize;
 
-// kStd: some frame's scene holds kind-5 materials.
-template <bool kStd>
+// kStd: some frame's scene holds kind-5 materials; kInst: some frame's scene is two-level (each
+// frame then takes the traversal of its own scene).
+template <bool kStd, bool kInst>
 __global__ void __launch_bounds__(kBatchBlock) kBatch(const BatchFrameDev* __restrict__ frames, int nFrames, int nTiles,
                                                       int* __restrict__ counter, int tilesPerFetch, int* __restrict__ spill,
                                                       int spillStride, unsigned long long* __restrict__ samples)
@@ -56,7 +57,9 @@ __global__ void __launch_bounds__(kBatchBlock) kBatch(const BatchFrameDev* __res
       const bool wantFirst = f.albedo1 || f.depth;
       for (int s = 0; s < f.spp; ++s)
       {
-        f3 L = tracePathMega<kStd>(a, st, x, y, uint32_t(s), wantFirst ? &fh : nullptr, NoVertexHook(), nodes, tris);
+        f3 L = kInst && a.inst.nodes
+                   ? tracePathMega<kStd, true>(a, st, x, y, uint32_t(s), wantFirst ? &fh : nullptr, NoVertexHook(), nodes, tris)
+                   : tracePathMega<kStd, false>(a, st, x, y, uint32_t(s), wantFirst ? &fh : nullptr, NoVertexHook(), nodes, tris);
         sum = sum + clampL(L, a.maxRadiance);
         if (f.albedo1) { alb = alb + fh.albedo; nrm = nrm + fh.normal; }
         if (f.depth) dep += fh.depth;
@@ -94,11 +97,17 @@ __global__ void kBatchReset(int* counter, unsigned long long* samples)
 }  // namespace
 
 void launchBatch(const BatchFrameDev* frames, int nFrames, int nTiles, int* counter, int tilesPerFetch, int grid, int* spill,
-                 int spillStride, unsigned long long* samples, bool std, hipStream_t s)
+                 int spillStride, unsigned long long* samples, bool std, bool inst, hipStream_t s)
 {
   kBatchReset<<<1, 1, 0, s>>>(counter, samples);
-  if (std) kBatch<true><<<grid, kBatchBlock, 0, s>>>(frames, nFrames, nTiles, counter, tilesPerFetch, spill, spillStride, samples);
-  else kBatch<false><<<grid, kBatchBlock, 0, s>>>(frames, nFrames, nTiles, counter, tilesPerFetch, spill, spillStride, samples);
+  if (std && inst)
+    kBatch<true, true><<<grid, kBatchBlock, 0, s>>>(frames, nFrames, nTiles, counter, tilesPerFetch, spill, spillStride, samples);
+  else if (std)
+    kBatch<true, false><<<grid, kBatchBlock, 0, s>>>(frames, nFrames, nTiles, counter, tilesPerFetch, spill, spillStride, samples);
+  else if (inst)
+    kBatch<false, true><<<grid, kBatchBlock, 0, s>>>(frames, nFrames, nTiles, counter, tilesPerFetch, spill, spillStride, samples);
+  else
+    kBatch<false, false><<<grid, kBatchBlock, 0, s>>>(frames, nFrames, nTiles, counter, tilesPerFetch, spill, spillStride, samples);
 }
 
 }  // namespace prt

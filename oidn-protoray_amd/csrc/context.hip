@@ -2516,10 +2516,14 @@ bool batchStale(const prt_batch* b)
 
 void batchLaunch(prt_batch* b)
 {
-  bool std = false;
-  for (const auto& f : b->frames) std = std || f.scene->stdMaterials != 0;
+  bool std = false, inst = false;
+  for (const auto& f : b->frames)
+  {
+    std = std || f.scene->stdMaterials != 0;
+    inst = inst || f.scene->instanced();
+  }
   launchBatch(b->fdev.p, int(b->frames.size()), b->nTiles, b->counter.p, b->tilesPerFetch, b->blocksPerCu * b->cus, b->spill.p,
-              b->spillStride, b->samples.p, std, b->stream);
+              b->spillStride, b->samples.p, std, inst, b->stream);
 }
 }  // namespace
 
@@ -2566,7 +2570,6 @@ int prt_batch_add_frame(prt_batch* b, prt_context* scene, const prt_camera* cam,
   if (scene->device != b->device) return fail(PRT_ERR_INVALID, "scene context on another device");
   const int rcScene = sceneReady(scene);
   if (rcScene != PRT_OK) return rcScene;
-  if (scene->instanced()) return fail(PRT_ERR_INVALID, "the batch renderer takes one-level scenes");
   if (width <= 0 || height <= 0 || width % 8 != 0 || height % 8 != 0 || width > 4096 || height > 4096)
     return fail(PRT_ERR_INVALID, "batch frame size must be a multiple of 8 and at most 4096");
   if (!(cam->fov > 0.f && cam->fov < 3.14159265f)) return fail(PRT_ERR_INVALID, "invalid fov");

@@ -123,7 +123,7 @@ struct BatchFrameDev
 };
 constexpr int kBatchBlockSize = 256;
 void launchBatch(const BatchFrameDev* frames, int nFrames, int nTiles, int* counter, int tilesPerFetch, int grid, int* spill,
-                 int spillStride, unsigned long long* samples, bool std, hipStream_t s);
+                 int spillStride, unsigned long long* samples, bool std, bool inst, hipStream_t s);
 
 // Kernel launchers (render.hip)
 struct LaunchCfg

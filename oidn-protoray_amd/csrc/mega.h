@@ -43,8 +43,10 @@ struct NoVertexHook
 
 // Traces sample sampleIndex of pixel (x, y); returns the path's radiance (unclamped).  fh, when
 // not null, receives the first-hit AOVs; hook(vertex) runs at every shaded vertex.  kStd: the
-// scene may hold kind-5 materials (false keeps their code out of the kernel).
-template <bool kStd, class Hook>
+// scene may hold kind-5 materials (false keeps their code out of the kernel); kInst: a two-level
+// scene (its rays go through the top-level tree and the instances' mesh trees, and the hit's
+// frame through the instance's transform).
+template <bool kStd, bool kInst = false, class Hook>
 PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint32_t sampleIndex, FirstHit* fh, Hook&& hook,
                          uint32_t& nodes, uint32_t& tris)
 {
@@ -65,7 +67,9 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
   {
     const int sampleDim = 4 + 7 * depth;
     float th, u = 0.f, v = 0.f;
-    const int prim = traceRay<false>(a.bvh, o, d, tnear, kPosMax, th, u, v, st, nodes, tris);
+    int inst = -1;
+    const int prim = kInst ? traceRayInst<false>(a.inst, o, d, tnear, kPosMax, th, u, v, inst, st, nodes, tris)
+                           : traceRay<false>(a.bvh, o, d, tnear, kPosMax, th, u, v, st, nodes, tris);
     const int matId = prim >= 0 ? a.triMat[prim] : 0;
     if (matId == 0)  // miss (path_renderer_stream.h:285-415)
     {
@@ -92,7 +96,7 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
       break;
     }
     if (depth >= a.maxDepth) break;
-    Ctx ctx = postIntersect(a, o, d, th, prim, u, v);
+    Ctx ctx = postIntersect(a, o, d, th, prim, u, v, inst);
     const f3 wo = -d;
     const bool inside = m.mediaInterface && (mediumId == m.mediumInt);  // material/material.h:54-61
     Bsdf b = makeBsdf(m, ctx, wo, inside, true, kStd);
@@ -131,10 +135,17 @@ PRT_DEV f3 tracePathMega(const RenderArgs& a, TraceStack& st, int x, int y, uint
           if (a.anyTransparent)
           {
             ShadowFilterT<kStd> flt{&a, ctx.p, wiL, f3(1.f), 0u};
-            nee = traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris, flt) < 0;
+            int si;
+            nee = (kInst ? traceRayInst<true>(a.inst, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, si, st, nodes, tris, flt)
+                         : traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris, flt)) < 0;
             dirL = dirL * flt.T;
           }
-          else nee = traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris) < 0;
+          else
+          {
+            int si;
+            nee = (kInst ? traceRayInst<true>(a.inst, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, si, st, nodes, tris)
+                         : traceRay<true>(a.bvh, ctx.p, wiL, ctx.eps, lightDist - ctx.eps, ts, us, vs, st, nodes, tris)) < 0;
+          }
         }
       }
     }

@@ -61,6 +61,31 @@ def test_batch_equals_wavefront_renderer_standard_materials():
     dA.close()
 
 
+def test_batch_equals_wavefront_renderer_two_level_scene():
+    # a two-level scene (41 instances of 7 meshes, a textured and a transparent material) and a
+    # one-level scene in the same launch: the kernel built for two-level scenes takes each frame's
+    # traversal from its own scene
+    scI = G.spheres_instanced(seed=2)
+    scA = G.spheres_sunsky(seed=1, n_spheres=32, subdiv=2, palette=16)
+    dI, dA = _scene_dev(scI), _scene_dev(scA)
+    frames = [(dI, scI["camera"], 64, 48, 3, 4), (dA, D.look_at((0, 2, -2), (1, 1, 15), 50), 32, 32, 1, 4)]
+    batch = P.BatchHip(0)
+    for d, cam, W, H, seed, spp in frames:
+        batch.addFrame(d, cam, W, H, seed, spp, snapshots=(1, 4), aov_mask=P.AOV_ALBEDO1 | P.AOV_NORMAL1 | P.AOV_DEPTH)
+    batch.render()
+    for i, (d, cam, W, H, seed, spp) in enumerate(frames):
+        ref, alb, nrm, dep = _wavefront(d, cam, W, H, seed, spp)
+        assert ref[spp].max() > 0
+        np.testing.assert_allclose(batch.get(i), ref[spp], rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(batch.get(i, P.BATCH_SNAPSHOT), ref[1], rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(batch.get(i, P.BATCH_ALBEDO1), alb, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(batch.get(i, P.BATCH_NORMAL1), nrm, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(batch.get(i, P.BATCH_DEPTH), dep, rtol=1e-5, atol=1e-5)
+    batch.close()
+    dI.close()
+    dA.close()
+
+
 def test_batch_equals_wavefront_renderer():
     scA = G.spheres_sunsky(seed=1, n_spheres=64, subdiv=2, palette=32, coated=0.5)
     scB = G.spheres_ambient(seed=2, n_spheres=48, subdiv=2, emissive=4, ambient=0.3)

@@ -79,10 +79,6 @@ def test_two_level_limits():
             dev.initInstances([0, nt], [(1, eye)])
         dev.initInstances(sc["mesh_begin"], sc["instances"])
         dev.initLights(sc["lights"], sc["env"])
-        batch = P.BatchHip(0)
-        with pytest.raises(ValueError):  # the batch renderer takes one-level scenes
-            batch.addFrame(dev, sc["camera"], 64, 64, 0, 1)
-        batch.close()
         dev.initFrame(64, 48)
         with pytest.raises(ValueError):
             dev.adjoint(np.zeros((48, 64, 3), np.float32), 1, sc["materials"].shape[0])
