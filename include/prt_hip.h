@@ -250,11 +250,11 @@ int prt_scene_set_lights(prt_context* ctx, int32_t n, const prt_light* lights, i
  * (mesh_begin[m] + local index); prt_query_rays_inst also reports the instance (ties in t go to
  * the smaller instance, then the smaller triangle).  n = 0 returns to a one-level scene; new
  * geometry drops the instances.  Two-level scenes render with prt_render and the batch
- * renderer and answer prt_query_rays / prt_query_rays_device / prt_query_occlusion of opaque
- * scenes; emissive triangles must lie in a mesh placed exactly once by the identity (the
- * reference flattens emissive objects, render/scene.cpp:343-363; prt_scene_set_lights checks
- * it), and the adjoint, the pixel and hit queries, the transparent-shadow occlusion query and
- * the allocation-free query take one-level scenes. */
+ * renderer, differentiate with prt_adjoint and answer prt_query_rays / prt_query_rays_device /
+ * prt_query_occlusion of opaque scenes; emissive triangles must lie in a mesh placed exactly
+ * once by the identity (the reference flattens emissive objects, render/scene.cpp:343-363;
+ * prt_scene_set_lights checks it), and the pixel and hit queries, the transparent-shadow
+ * occlusion query and the allocation-free query take one-level scenes. */
 typedef struct prt_instance {
   int32_t mesh;
   float xfm[12];  /* rows of [L | t] */

@@ -26,6 +26,8 @@ namespace {
 
 constexpr int kAdjBlock = 256;
 
+// kInst: a two-level scene (the replays trace through its top-level and mesh trees).
+template <bool kInst>
 __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restrict__ ap, const float* __restrict__ delta,
                                                       float invSpp, int nParams, float* __restrict__ partial,
                                                       uint32_t sampleIndex, float* __restrict__ replayL)
@@ -53,7 +55,7 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
     dl = f3(delta[pix] * invSpp, delta[a.npix + pix] * invSpp, delta[2 * a.npix + pix] * invSpp);
     // the first replay recovers the sample's radiance; the next ones replay the same path with
     // the gradient hook
-    Ltotal = tracePathMega<true>(a, st, x, y, sampleIndex, nullptr, NoVertexHook(), nodes, tris);
+    Ltotal = tracePathMega<true, kInst>(a, st, x, y, sampleIndex, nullptr, NoVertexHook(), nodes, tris);
     if (replayL)
     {
       replayL[pix] = Ltotal.x; replayL[a.npix + pix] = Ltotal.y; replayL[2 * a.npix + pix] = Ltotal.z;
@@ -82,7 +84,7 @@ __global__ void __launch_bounds__(kAdjBlock) kAdjoint(const RenderArgs* __restri
         recMat[pv.depth - v0][threadIdx.x] = pv.matId;
         recG[pv.depth - v0][threadIdx.x] = gv;
       };
-      tracePathMega<true>(a, st, x, y, sampleIndex, nullptr, hook, nodes, tris);
+      tracePathMega<true, kInst>(a, st, x, y, sampleIndex, nullptr, hook, nodes, tris);
     }
     __syncthreads();
     // block reduction in a fixed order: parameter p = 4 * material + (r, g, b, alpha)
@@ -120,9 +122,10 @@ __global__ void kAdjointReduce(const float* __restrict__ partial, int nBlocks, i
 int adjointBlocks(int npix, int spillStride) { return std::max(1, std::min((npix + kAdjBlock - 1) / kAdjBlock, spillStride / kAdjBlock)); }
 
 void launchAdjoint(const RenderArgs* a, int blocks, const float* delta, float invSpp, int nParams, float* partial,
-                   uint32_t sampleIndex, float* replayL, hipStream_t s)
+                   uint32_t sampleIndex, float* replayL, bool inst, hipStream_t s)
 {
-  kAdjoint<<<blocks, kAdjBlock, 0, s>>>(a, delta, invSpp, nParams, partial, sampleIndex, replayL);
+  if (inst) kAdjoint<true><<<blocks, kAdjBlock, 0, s>>>(a, delta, invSpp, nParams, partial, sampleIndex, replayL);
+  else kAdjoint<false><<<blocks, kAdjBlock, 0, s>>>(a, delta, invSpp, nParams, partial, sampleIndex, replayL);
 }
 
 void launchAdjointReduce(const float* partial, int nBlocks, int nParams, float* grad, hipStream_t s)

@@ -1935,7 +1935,6 @@ int prt_adjoint(prt_context* c, const float* target, int32_t spp, double* loss, 
   if (!target || !grad) return fail(PRT_ERR_INVALID, "target and grad are required");
   if (spp < 1) return fail(PRT_ERR_INVALID, "spp must be >= 1");
   if (c->textured) return fail(PRT_ERR_INVALID, "the adjoint does not differentiate textured materials");
-  if (c->instanced()) return fail(PRT_ERR_INVALID, "the adjoint takes one-level scenes");
   const int nMat = int(c->materials.size());
   const int nParams = 4 * nMat;
   if (grad_capacity < nParams) return fail(PRT_ERR_INVALID, "grad needs 4 floats per material");
@@ -1968,7 +1967,8 @@ int prt_adjoint(prt_context* c, const float* target, int32_t spp, double* loss, 
   HIP_TRY(hipMemsetAsync(partial.p, 0, sizeof(float) * partial.n, c->stream));
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   for (int s = 0; s < spp; ++s)
-    launchAdjoint(c->argsDev.p, blocks, dDelta.p, 1.0f / float(spp), nParams, partial.p, uint32_t(s), nullptr, c->stream);
+    launchAdjoint(c->argsDev.p, blocks, dDelta.p, 1.0f / float(spp), nParams, partial.p, uint32_t(s), nullptr, c->instanced(),
+                  c->stream);
   launchAdjointReduce(partial.p, blocks, nParams, dGrad.p, c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
@@ -1995,7 +1995,7 @@ int prt_debug_adjoint_replay(prt_context* c, uint32_t sample_index, float* radia
   HIP_TRY(dL.alloc(3 * n));
   HIP_TRY(hipMemsetAsync(dDelta.p, 0, sizeof(float) * 3 * n, c->stream));
   HIP_TRY(hipMemsetAsync(partial.p, 0, sizeof(float) * partial.n, c->stream));
-  launchAdjoint(c->argsDev.p, blocks, dDelta.p, 1.f, 0, partial.p, sample_index, dL.p, c->stream);
+  launchAdjoint(c->argsDev.p, blocks, dDelta.p, 1.f, 0, partial.p, sample_index, dL.p, c->instanced(), c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(radiance, dL.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));

@@ -153,7 +153,7 @@ void launchQueryInst(const InstView& iv, int n, const float* rays, int* prim, fl
 // first sample pass), grad [nParams]; delta is the per-pixel dLoss/dI (planar rgb).
 int adjointBlocks(int npix, int spillStride);
 void launchAdjoint(const RenderArgs* a, int blocks, const float* delta, float invSpp, int nParams, float* partial,
-                   uint32_t sampleIndex, float* replayL, hipStream_t s);
+                   uint32_t sampleIndex, float* replayL, bool inst, hipStream_t s);
 void launchAdjointReduce(const float* partial, int nBlocks, int nParams, float* grad, hipStream_t s);
 void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s);
 

@@ -147,6 +147,32 @@ def test_gradient_beyond_eight_vertices_matches_fp64_oracle(dev):
     assert np.all(np.abs(dg - dog) <= 0.05 * np.abs(dog).max(axis=0))
 
 
+def test_gradient_two_level_scene_matches_fp64_oracle(dev):
+    # 40 instances of six sphere meshes (diffuse, Oren-Nayar, coated, conductor, dielectric, thin
+    # dielectric) over a ground quad: the replays trace through the top-level tree and the mesh
+    # trees and shade in the instances' frames; base-colour textures dropped (not differentiated)
+    W, H, spp = 64, 64, 4
+    sc = {k: v for k, v in G.spheres_instanced(seed=3).items() if k not in ("uvs", "textures", "material_textures")}
+    target_mats = G.perturbed_materials(sc["materials"])
+    _load_any = lambda mats: (P.load_scene(dev, dict(sc, materials=mats)), dev.initRenderer(maxDepth=4),
+                              dev.initSampler(0), dev.initCamera(sc["camera"]))
+    _load_any(target_mats)
+    target = _render(dev, W, H, spp)
+    _load_any(sc["materials"])
+    loss, grad, _ = dev.adjoint(target.reshape(H, W, 3), spp, sc["materials"].shape[0])
+    osc = O.Scene(sc["positions"], sc["indices"], sc["tri_material"], sc["materials"], sc.get("normals"))
+    osc.set_textures(sc)
+    osc.set_lights(sc["lights"], sc.get("env"))
+    oloss, ograd = osc.adjoint(sc["camera"], W, H, target, spp, max_depth=4)
+    print("loss", loss, oloss)
+    assert abs(loss - oloss) <= 1e-4 * oloss
+    scale = np.abs(ograd).max(axis=0)
+    err = np.abs(grad - ograd)
+    print("max abs err / max |g| per (r, g, b, alpha):", err.max(axis=0) / scale)
+    assert np.abs(ograd).max() > 0
+    assert np.all(err <= 2e-3 * np.abs(ograd) + 1e-3 * scale)
+
+
 def test_gradient_matches_fp64_oracle_conductor_dielectric(dev):
     # coated, rough conductor and rough dielectric materials in turn: reflectivity and alpha of
     # the conductors, alpha of the dielectrics (refraction into the spheres and out again)
