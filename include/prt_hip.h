@@ -251,10 +251,10 @@ int prt_scene_set_lights(prt_context* ctx, int32_t n, const prt_light* lights, i
  * the smaller instance, then the smaller triangle).  n = 0 returns to a one-level scene; new
  * geometry drops the instances.  Two-level scenes render with prt_render and the batch
  * renderer, differentiate with prt_adjoint and answer prt_query_rays / prt_query_rays_device /
- * prt_query_occlusion of opaque scenes; emissive triangles must lie in a mesh placed exactly
- * once by the identity (the reference flattens emissive objects, render/scene.cpp:343-363;
- * prt_scene_set_lights checks it), and the pixel and hit queries, the transparent-shadow
- * occlusion query and the allocation-free query take one-level scenes. */
+ * prt_query_occlusion (transparent shadows included) and the pixel and hit queries; emissive
+ * triangles must lie in a mesh placed exactly once by the identity (the reference flattens
+ * emissive objects, render/scene.cpp:343-363; prt_scene_set_lights checks it), and the
+ * allocation-free query takes one-level scenes. */
 typedef struct prt_instance {
   int32_t mesh;
   float xfm[12];  /* rows of [L | t] */

@@ -2220,8 +2220,6 @@ int prt_query_rays_inst(prt_context* c, int32_t n, const float* rays, int32_t* p
 int prt_query_occlusion(prt_context* c, int32_t n, const float* rays, int32_t* occluded)
 {
   CTX_CHECK(c);
-  if (c->anyTransparent && c->instanced())
-    return fail(PRT_ERR_INVALID, "transparent-shadow queries take one-level scenes");
   if (c->anyTransparent && c->bvhReady && n > 0 && rays && occluded)
   {
     // the transparent-shadow filter needs the materials: the renderer's argument block
@@ -2234,7 +2232,7 @@ int prt_query_occlusion(prt_context* c, int32_t n, const float* rays, int32_t* o
     HIP_TRY(docc.alloc(n));
     HIP_TRY(hipMemcpy(da.p, &a, sizeof(a), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dr.p, rays, sizeof(float) * 8 * n, hipMemcpyHostToDevice));
-    launchQueryOcclusionFilter(da.p, n, dr.p, docc.p, c->launch.gridTrace, c->stream);
+    launchQueryOcclusionFilter(da.p, n, dr.p, docc.p, c->launch.gridTrace, c->instanced(), c->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(occluded, docc.p, sizeof(int) * n, hipMemcpyDeviceToHost));
@@ -2270,7 +2268,7 @@ static int queryHits(prt_context* c, float px, float py, const float* ray8, int3
     HIP_TRY(dr.alloc(8));
     HIP_TRY(hipMemcpy(dr.p, ray8, sizeof(float) * 8, hipMemcpyHostToDevice));
   }
-  launchQueryPixel(da.p, px, py, max_hits, dh.p, dn.p, dr.p, c->stream);
+  launchQueryPixel(da.p, px, py, max_hits, dh.p, dn.p, dr.p, c->instanced(), c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(count, dn.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -2284,7 +2282,6 @@ int prt_query_pixel(prt_context* c, float px, float py, int32_t max_hits, prt_pi
   if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
   if (c->W <= 0 || c->H <= 0) return fail(PRT_ERR_STATE, "no frame: the pixel query needs the image size");
   if (max_hits < 1 || max_hits > 32 || !hits || !count) return fail(PRT_ERR_INVALID, "invalid pixel query arguments");
-  if (c->instanced()) return fail(PRT_ERR_INVALID, "the pixel query takes one-level scenes");
   return queryHits(c, px, py, nullptr, max_hits, hits, count);
 }
 
@@ -2294,7 +2291,6 @@ int prt_query_ray_hits(prt_context* c, const float* ray, int32_t max_hits, prt_p
   if (!c->bvhReady) return fail(PRT_ERR_STATE, "BVH not built");
   if (!c->mats.p) return fail(PRT_ERR_STATE, "materials not set");
   if (!ray || max_hits < 1 || max_hits > 32 || !hits || !count) return fail(PRT_ERR_INVALID, "invalid ray query arguments");
-  if (c->instanced()) return fail(PRT_ERR_INVALID, "the hit query takes one-level scenes");
   return queryHits(c, 0.f, 0.f, ray, max_hits, hits, count);
 }
 

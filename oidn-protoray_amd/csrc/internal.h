@@ -141,9 +141,9 @@ void launchIntersect(const RenderArgs* a, int depth, const LaunchCfg& c, bool co
 void launchShade(const RenderArgs* a, int depth, uint32_t sampleIndex, bool classB, const LaunchCfg& c, int level, hipStream_t s);
 void launchOccluded(const RenderArgs* a, int depth, bool classB, const LaunchCfg& c, int filter, bool count, bool inst,
                     hipStream_t s);
-void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s);
+void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, bool inst, hipStream_t s);
 void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, const float* ray8,
-                      hipStream_t s);
+                      bool inst, hipStream_t s);
 void launchQuery(const BvhView& bvh, int n, const float* rays, int* prim, float* t, float* u, float* v, int anyHit,
                  int* spill, int spillStride, int spillCap, int grid, hipStream_t s);
 // The same on a two-level scene; inst (may be null) receives the hit's instance.

@@ -532,6 +532,7 @@ __global__ void __launch_bounds__(kTraceBlock) kQueryInst(InstView iv, int n, co
 
 // Shadow-style occlusion queries in a scene with transparent materials: the transparent-shadow
 // filter decides (render/renderer_stream.h:64-97).
+template <bool kInst>
 __global__ void __launch_bounds__(kTraceBlock) kQueryOcclusionFilter(const RenderArgs* __restrict__ ap, int n,
                                                                      const float* __restrict__ rays, int* __restrict__ occ)
 {
@@ -546,13 +547,16 @@ __global__ void __launch_bounds__(kTraceBlock) kQueryOcclusionFilter(const Rende
     f3 o(r[0], r[1], r[2]), d(r[3], r[4], r[5]);
     float t, u, v;
     ShadowFilter flt{&a, o, d, f3(1.f), 0u};
-    occ[i] = traceRay<true>(a.bvh, o, d, r[6], r[7], t, u, v, st, nodes, tris, flt) >= 0 ? 1 : 0;
+    int inst;
+    occ[i] = (kInst ? traceRayInst<true>(a.inst, o, d, r[6], r[7], t, u, v, inst, st, nodes, tris, flt)
+                    : traceRay<true>(a.bvh, o, d, r[6], r[7], t, u, v, st, nodes, tris, flt)) >= 0 ? 1 : 0;
   }
 }
 
 // Renderer::queryPixel (render/renderer.h:82-115): one lane walks the hits of the pixel's ray;
 // with ray8 (origin, direction, near, far) the hits of that ray instead (RendererStream::queryRay,
 // render/renderer_stream.h:16-62), their depth being the distance along the ray.
+template <bool kInst>
 __global__ void __launch_bounds__(64) kQueryPixel(const RenderArgs* __restrict__ ap, float px, float py, int maxHits,
                                                   prt_pixel_hit* __restrict__ out, int* __restrict__ count,
                                                   const float* __restrict__ ray8)
@@ -590,9 +594,11 @@ __global__ void __launch_bounds__(64) kQueryPixel(const RenderArgs* __restrict__
   for (; n < maxHits; ++n)
   {
     float t, u = 0.f, v = 0.f;
-    const int prim = traceRay<false>(a.bvh, o, d, tnear, tfar, t, u, v, st, nodes, tris);
+    int inst = -1;
+    const int prim = kInst ? traceRayInst<false>(a.inst, o, d, tnear, tfar, t, u, v, inst, st, nodes, tris)
+                           : traceRay<false>(a.bvh, o, d, tnear, tfar, t, u, v, st, nodes, tris);
     if (prim < 0) break;
-    const Ctx c = postIntersect(a, o, d, t, prim, u, v);
+    const Ctx c = postIntersect(a, o, d, t, prim, u, v, inst);
     const int matId = a.triMat[prim];
     const prt_material_constants& m = a.mats[matId];
     f3 T(0.f);
@@ -681,14 +687,16 @@ void launchQueryInst(const InstView& iv, int n, const float* rays, int* prim, fl
 {
   kQueryInst<<<grid, kTraceBlock, 0, s>>>(iv, n, rays, prim, t, u, v, inst, anyHit, spill, spillStride, spillCap);
 }
-void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, hipStream_t s)
+void launchQueryOcclusionFilter(const RenderArgs* a, int n, const float* rays, int* occ, int grid, bool inst, hipStream_t s)
 {
-  kQueryOcclusionFilter<<<grid, kTraceBlock, 0, s>>>(a, n, rays, occ);
+  if (inst) kQueryOcclusionFilter<true><<<grid, kTraceBlock, 0, s>>>(a, n, rays, occ);
+  else kQueryOcclusionFilter<false><<<grid, kTraceBlock, 0, s>>>(a, n, rays, occ);
 }
 void launchQueryPixel(const RenderArgs* a, float px, float py, int maxHits, prt_pixel_hit* out, int* count, const float* ray8,
-                      hipStream_t s)
+                      bool inst, hipStream_t s)
 {
-  kQueryPixel<<<1, 64, 0, s>>>(a, px, py, maxHits, out, count, ray8);
+  if (inst) kQueryPixel<true><<<1, 64, 0, s>>>(a, px, py, maxHits, out, count, ray8);
+  else kQueryPixel<false><<<1, 64, 0, s>>>(a, px, py, maxHits, out, count, ray8);
 }
 void launchDebugSampler(int n, const uint32_t* in, int ndim, float* o1, float* o2, float* o3, hipStream_t s)
 {

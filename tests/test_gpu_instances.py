@@ -82,8 +82,6 @@ def test_two_level_limits():
         dev.initFrame(64, 48)
         with pytest.raises(ValueError):
             dev.adjoint(np.zeros((48, 64, 3), np.float32), 1, sc["materials"].shape[0])
-        with pytest.raises(ValueError):
-            dev.queryPixel(10, 10)
         # emissive geometry stays one-level
         mats = sc["materials"].copy()
         mats[2]["emission"] = (1.0, 1.0, 1.0)
@@ -93,3 +91,47 @@ def test_two_level_limits():
             dev.initLights(sc["lights"], sc["env"])
     finally:
         dev.close()
+
+
+def test_queries_on_identity_instance_equal_the_one_level_scene():
+    # a scene with transparent materials, uploaded flat and as one mesh placed once by the
+    # identity: the transparent-shadow occlusion query and the pixel query (hit list with
+    # post-intersection frames and transparencies) agree; the identity still renormalises the
+    # frame, so vectors agree to rounding
+    sc = G.spheres_lobes()
+    eye = np.hstack([np.eye(3), np.zeros((3, 1))])
+    two = dict(sc, mesh_begin=[0, sc["indices"].shape[0]], instances=[(0, eye)])
+    d1, d2 = P.DeviceHip(0), P.DeviceHip(0)
+    try:
+        P.load_scene(d1, sc)
+        P.load_scene(d2, two)
+        W, H = 96, 64
+        d1.initFrame(W, H)
+        d2.initFrame(W, H)
+        rng = np.random.default_rng(5)
+        n = 4096
+        o = np.tile(np.asarray(sc["camera"]["pos"], np.float32), (n, 1))
+        dirs = rng.normal(size=(n, 3))
+        dirs[:, 2] = np.abs(dirs[:, 2]) + 0.5
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        rays = np.concatenate([o, dirs, np.full((n, 1), 1e-3), np.full((n, 1), 3.4e38)], 1).astype(np.float32)
+        occ1, occ2 = d1.queryOcclusion(rays), d2.queryOcclusion(rays)
+        assert occ1.sum() > n // 4 and np.array_equal(occ1, occ2)
+        compared = multi = 0
+        for y in range(0, H, 4):
+            for x in range(0, W, 4):
+                h1, h2 = d1.queryPixel(x, y), d2.queryPixel(x, y)
+                assert len(h1) == len(h2)
+                if len(h1) == 0:
+                    continue
+                compared += 1
+                multi += len(h1) > 1
+                for f in ("prim", "material", "material_type", "dist", "depth", "uv"):
+                    assert np.array_equal(h1[f], h2[f]), f
+                np.testing.assert_allclose(h2["p"], h1["p"], rtol=1e-6, atol=1e-6)
+                for f in ("Ng", "N", "U", "V", "T"):
+                    np.testing.assert_allclose(h2[f], h1[f], rtol=0, atol=2e-6)
+        assert compared > 100 and multi > 0
+    finally:
+        d1.close()
+        d2.close()
