@@ -229,6 +229,66 @@ PRT_DEV int traceRay(const BvhView& bvh, f3 o, f3 d, float tnear, float tfar, fl
   st.sp = 0;
   int node = bvh.root;
   const float inf = __int_as_float(0x7f800000);
+  if constexpr (kAnyHit)
+  {
+    // any hit: a leaf reached by a descent or a pop is tested in the same iteration (the ray may
+    // end there), then the next entry is popped; measured on the 1080p workload's shadow rays:
+    // 4.44 against 4.60 ms per pass with the closest-hit order below, which in turn is faster for
+    // closest hit (6.36 against 7.02)
+    for (;;)
+    {
+      ++st.steps;
+      int popNext = 0;
+      if (node >= 0)
+      {
+        float t[4];
+        int c[4];
+        ++nodeVisits;
+        boxQuad<true>(bvh.nodes, node, o, id, tnear, tfar, t, c);
+        cswap(t[0], c[0], t[1], c[1]);
+        cswap(t[2], c[2], t[3], c[3]);
+        cswap(t[0], c[0], t[2], c[2]);
+        cswap(t[1], c[1], t[3], c[3]);
+        cswap(t[1], c[1], t[2], c[2]);
+        popNext = 1;
+        if (t[0] != inf)
+        {
+          if (t[3] != inf) st.push(c[3], t[3]);
+          if (t[2] != inf) st.push(c[2], t[2]);
+          if (t[1] != inf) st.push(c[1], t[1]);
+          node = c[0];
+          popNext = 0;
+        }
+      }
+      keepOneLoop(popNext);
+      if (popNext)
+      {
+        if (st.sp == 0) { tHit = tfar; return -1; }
+        float tEntry;
+        node = st.pop(tEntry);
+      }
+      if (node < 0)
+      {
+        const int code = node & 0x7fffffff;
+        const int first = code >> 3, count = (code & 7) + 1;
+        for (int k = 0; k < count; ++k)
+        {
+          const float4* tp = bvh.tris + 3 * (first + k);
+          float4 a = ldg(tp), b = ldg(tp + 1), cc = ldg(tp + 2);
+          float th, u, v;
+          ++triTests;
+          if (intersectTriangle(a, b, cc, o, d, tnear, tfar, th, u, v))
+          {
+            const int prim = __float_as_int(a.w);
+            if (filter(prim, th, u, v)) { tHit = th; return prim; }
+          }
+        }
+        if (st.sp == 0) { tHit = tfar; return -1; }
+        float tEntry;
+        node = st.pop(tEntry);
+      }
+    }
+  }
   for (;;)
   {
     ++st.steps;
